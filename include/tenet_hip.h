@@ -1,0 +1,209 @@
+/*
+ * tenet_hip.h -- C-ABI of libtenet_hip.so, the MI355X (gfx950) engine for
+ * Tenet.jl's contraction / QR-SVD truncation / transfer-matrix hot path.
+ *
+ * Conventions
+ *  - Every entry point returns an int status (tenet_status); on failure the
+ *    thread-local message is available from tenet_last_error().
+ *  - Device buffers are plain pointers obtained from tenet_malloc (or any
+ *    device allocation of the same device).  Complex data are interleaved
+ *    (re, im) pairs: TENET_C128 = Julia ComplexF64, TENET_C64 = ComplexF32.
+ *  - Matrices and tensors are column-major (Julia layout) unless a stride
+ *    descriptor says otherwise.
+ *  - A NULL tenet_stream means the context's own stream.
+ *  - No entry point falls back to the CPU: without a usable gfx950 device
+ *    tenet_create fails with TENET_ERR_NODEVICE.
+ *
+ * The reference interfaces replaced are Muscle.jl's Tensor kernels as called
+ * from Tenet.jl v0.10.3 (Muscle is not vendored; the call sites are cited).
+ */
+#ifndef TENET_HIP_H
+#define TENET_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TENET_MAX_RANK 8
+
+typedef enum {
+  TENET_OK = 0,
+  TENET_ERR_HIP = 1,         /* a HIP runtime call failed */
+  TENET_ERR_SHAPE = 2,       /* labels / extents / strides inconsistent */
+  TENET_ERR_OOM = 3,         /* device allocation failed or budget exceeded */
+  TENET_ERR_NOCONV = 4,      /* Jacobi SVD hit its sweep limit */
+  TENET_ERR_UNSUPPORTED = 5, /* valid request this build does not implement */
+  TENET_ERR_ARG = 6,         /* bad argument (null pointer, bad enum, ...) */
+  TENET_ERR_NODEVICE = 7     /* no gfx950 device visible */
+} tenet_status;
+
+typedef enum { TENET_C128 = 0, TENET_C64 = 1 } tenet_dtype;
+
+/* Arithmetic of a contraction.  NATIVE = the dtype's own precision
+ * (FP64 for C128, FP32 for C64).  SPLIT_* = operands split into 8-bit
+ * slices (bf16 / fp16 MFMA), exact FP32 accumulation in short K blocks,
+ * FP64 summation across blocks and slices; the result is written in the
+ * tensor dtype. */
+typedef enum {
+  TENET_PREC_NATIVE = 0,
+  TENET_PREC_FP32 = 1,
+  TENET_PREC_SPLIT_BF16 = 2,
+  TENET_PREC_SPLIT_FP16 = 3
+} tenet_prec;
+
+typedef enum { TENET_ABSORB_NONE = 0, TENET_ABSORB_EQUAL = 1, TENET_ABSORB_LEFT = 2, TENET_ABSORB_RIGHT = 3 } tenet_absorb;
+
+typedef struct tenet_ctx_s* tenet_ctx;
+typedef struct tenet_plan_s* tenet_plan;
+typedef struct tenet_graph_s* tenet_graph;
+typedef void* tenet_stream; /* hipStream_t */
+
+/* Labelled dense tensor: element (i_0..i_{r-1}) lives at sum_d i_d*stride[d]
+ * (in elements).  Labels are arbitrary int64 tags (Tenet's Index tags). */
+typedef struct {
+  int32_t rank;
+  int32_t reserved;
+  int64_t extent[TENET_MAX_RANK];
+  int64_t stride[TENET_MAX_RANK];
+  int64_t label[TENET_MAX_RANK];
+} tenet_tdesc;
+
+/* ---- library / context ------------------------------------------------- */
+const char* tenet_version(void);
+const char* tenet_last_error(void);
+int tenet_device_count(int* count);
+int tenet_create(int device, tenet_ctx* ctx);
+int tenet_destroy(tenet_ctx ctx);
+int tenet_stream_create(tenet_ctx ctx, tenet_stream* stream);
+int tenet_stream_destroy(tenet_ctx ctx, tenet_stream stream);
+int tenet_stream_sync(tenet_ctx ctx, tenet_stream stream);
+int tenet_device_sync(tenet_ctx ctx);
+
+/* ---- memory (stream-ordered caching pool owned by the context) ---------- */
+int tenet_malloc(tenet_ctx ctx, size_t bytes, void** dptr);
+int tenet_free(tenet_ctx ctx, void* dptr);
+int tenet_memcpy_h2d(tenet_ctx ctx, void* dst, const void* src, size_t bytes, tenet_stream stream);
+int tenet_memcpy_d2h(tenet_ctx ctx, void* dst, const void* src, size_t bytes, tenet_stream stream);
+int tenet_memcpy_d2d(tenet_ctx ctx, void* dst, const void* src, size_t bytes, tenet_stream stream);
+int tenet_memset_zero(tenet_ctx ctx, void* dst, size_t bytes, tenet_stream stream); /* fill kernel */
+int tenet_mem_info(tenet_ctx ctx, size_t* free_bytes, size_t* total_bytes, size_t* pool_bytes);
+
+/* ---- pairwise contraction (K1) ------------------------------------------
+ * Replaces Muscle.binary_einsum(a, b; dims) at Tenet call sites
+ * src/Operations/canonize.jl:44,47,61,100; evolve.jl:29,57,76,86,108,118;
+ * overlap.jl:26,28,42,46; simple_update.jl:34; Algorithms/DMRG.jl:17,298,300.
+ * Label classes: contracted = (a ∩ b) \ c; batch (Hadamard) = a ∩ b ∩ c;
+ * free = labels of one operand, all of which must appear in c.
+ * dims=Index[] of the reference is a c that keeps every shared label.
+ * C = alpha * op(A) op(B) + beta * C, op = identity or conjugate. */
+int tenet_einsum_plan(tenet_ctx ctx, int dtype, int prec, const tenet_tdesc* a, const tenet_tdesc* b,
+                      const tenet_tdesc* c, tenet_plan* plan);
+int tenet_einsum_exec(tenet_plan plan, tenet_stream stream, const void* A, const void* B, void* C, int conj_a,
+                      int conj_b, const double alpha[2], const double beta[2]);
+int tenet_plan_destroy(tenet_plan plan);
+/* plan (cached by descriptor) + exec in one call */
+int tenet_einsum(tenet_ctx ctx, tenet_stream stream, int dtype, int prec, const tenet_tdesc* a, const void* A,
+                 const tenet_tdesc* b, const void* B, const tenet_tdesc* c, void* C, int conj_a, int conj_b);
+
+/* ---- thin QR (K2) ---------------------------------------------------------
+ * Replaces Muscle.tensor_qr_thin at src/Operations/canonize.jl:58.
+ * A is m x n (column-major, lda >= m); k = min(m, n).  Q: m x k (ldq),
+ * R: k x n upper triangular (ldr).  Householder reflections in FP64. */
+int tenet_qr_thin(tenet_ctx ctx, tenet_stream stream, int dtype, int64_t m, int64_t n, const void* A, int64_t lda,
+                  void* Q, int64_t ldq, void* R, int64_t ldr);
+/* Operand options for the same factorisation, used by right-to-left canonize
+ * moves (canonize.jl:142-145) so no transposed copy is made:
+ * IN_ADJ: factorise A^H, where the given A is n x m (lda >= n);
+ * Q_ADJ: write Q^H (k x m, ldq >= k); R_ADJ: write R^H (n x k, ldr >= n). */
+typedef enum { TENET_QR_IN_ADJ = 1, TENET_QR_Q_ADJ = 2, TENET_QR_R_ADJ = 4 } tenet_qr_flags;
+int tenet_qr_thin_ex(tenet_ctx ctx, tenet_stream stream, int dtype, int flags, int64_t m, int64_t n, const void* A,
+                     int64_t lda, void* Q, int64_t ldq, void* R, int64_t ldr);
+
+/* ---- thin SVD + truncation (K3) -------------------------------------------
+ * Replaces Muscle.tensor_svd_thin at canonize.jl:41,97; evolve.jl:62,92;
+ * DMRG.jl:338,437 plus the truncation rule of compress.jl:46-59.
+ * A = U diag(S) Vh with S descending; kmax = min(m, n).  U: m x kmax (ldu),
+ * S: kmax doubles, Vh: kmax x n (ldvh).  Kept count k = min(maxdim, #S >
+ * threshold) (maxdim <= 0: no cap; threshold <= 0: none); if k_dev /
+ * discarded_dev are non-NULL, k (int64) and sum_{j>=k} S_j^2 (double) are
+ * written to device memory.  One-sided block Jacobi, FP64 rotations. */
+int tenet_svd_thin(tenet_ctx ctx, tenet_stream stream, int dtype, int64_t m, int64_t n, const void* A, int64_t lda,
+                   void* U, int64_t ldu, double* S, void* Vh, int64_t ldvh, int64_t maxdim, double threshold,
+                   int64_t* k_dev, double* discarded_dev);
+
+/* ---- fused two-site gate update (K1 + K3) -----------------------------------
+ * Replaces Muscle.simple_update(A, ia, B, ib, bond, G, oa, ob; maxdim,
+ * absorb) at src/Operations/simple_update.jl:53-64.
+ * A: (chil, d1, chi), B: (chi, d2, chir), G: (d1, d2, d1, d2) holding
+ * G[oa, ob, ia, ib], all column-major.  theta = A.B over chi; theta.G over
+ * (ia, ib); SVD with rows (chil, oa), cols (ob, chir); truncation as in
+ * tenet_svd_thin; singular values absorbed per `absorb`.
+ * Outputs: Aout (chil, d1, k) and Bout (k, d2, chir), compact column-major,
+ * each with capacity for kmax = min(chil*d1, d2*chir); S_out (kmax doubles,
+ * may be NULL); k and the discarded weight go to k_dev / disc_dev (device,
+ * may be NULL) and, if k_host is non-NULL, k is also returned to the host
+ * (this synchronises the stream). */
+int tenet_simple_update(tenet_ctx ctx, tenet_stream stream, int dtype, int prec, int64_t chil, int64_t d1,
+                        int64_t chi, int64_t d2, int64_t chir, const void* A, const void* B, const void* G, void* Aout,
+                        void* Bout, double* S_out, int64_t maxdim, double threshold, int absorb, int64_t* k_dev,
+                        double* disc_dev, int64_t* k_host);
+
+/* ---- transfer-matrix sweeps (K4) -------------------------------------------
+ * overlap(a::MPS, b::MPS) = <b|a> (src/Operations/overlap.jl:36-50) with a
+ * running FP64 log-scale: result = out[0:2] * exp(out[2]).  Sites are
+ * (chi_l, d, chi_r) column-major; chi arrays hold n+1 bond sizes with
+ * chi[0] = chi[n] = 1.  a_sites / b_sites are HOST arrays of device
+ * pointers.  out is a device pointer to 3 doubles. */
+int tenet_overlap(tenet_ctx ctx, tenet_stream stream, int dtype, int prec, int64_t n, int64_t d,
+                  const int64_t* chi_a, const void* const* a_sites, const int64_t* chi_b,
+                  const void* const* b_sites, double* out);
+
+/* Batched amplitudes <x|psi> (overlap.jl:22-32: no conjugation of x) for
+ * nreq requests, one workgroup each.  Device arrays: site_ptrs[req] points to
+ * a device array of n_req device pointers; chis[req] to n_req+1 int64 bond
+ * sizes; bits[req] to n_req uint8 basis indices.  out: device, 3 doubles per
+ * request (re, im, log-scale). */
+int tenet_amplitudes(tenet_ctx ctx, tenet_stream stream, int dtype, int64_t nreq, const int64_t* n_dev,
+                     const uint64_t* site_ptrs_dev, const uint64_t* chis_dev, const uint64_t* bits_dev, int64_t d,
+                     double* out);
+
+/* Pauli-string expectation <psi|P|psi> on sites [start, start+w) given the
+ * left environment env_l (chi_start x chi_start, device; NULL = identity,
+ * for a left-canonical prefix) and a right-canonical suffix (identity right
+ * environment).  paulis: w chars from "IXYZ" (ext/TenetPythonCallExt/
+ * pytket.jl:23-31 conventions).  out: device, 3 doubles (re, im, log). */
+int tenet_expect_pauli(tenet_ctx ctx, tenet_stream stream, int dtype, int prec, int64_t start, int64_t w,
+                       int64_t d, const int64_t* chi, const void* const* sites, const void* env_l,
+                       const char* paulis, double* out);
+
+/* ---- grouped (persistent) launches ---------------------------------------
+ * Many independent contractions in one launch.  descs holds 3*count
+ * descriptors (a, b, c per problem); ptrs holds 3*count device pointers. */
+int tenet_grouped_einsum(tenet_ctx ctx, tenet_stream stream, int dtype, int prec, int64_t count,
+                         const tenet_tdesc* descs, void* const* ptrs);
+/* Many independent thin SVDs (column-major, lda = m) with per-problem
+ * truncation.  All pointer arrays are HOST arrays of device pointers;
+ * ms, ns, maxdims are host arrays; k_dev / disc_dev device arrays (count). */
+int tenet_grouped_svd(tenet_ctx ctx, tenet_stream stream, int dtype, int64_t count, const int64_t* ms,
+                      const int64_t* ns, const void* const* As, void* const* Us, double* const* Ss,
+                      void* const* Vhs, const int64_t* maxdims, double threshold, int64_t* k_dev, double* disc_dev);
+
+/* ---- HIP graph capture of a sequence of calls on one stream --------------- */
+int tenet_graph_begin(tenet_ctx ctx, tenet_stream stream);
+int tenet_graph_end(tenet_ctx ctx, tenet_stream stream, tenet_graph* graph);
+int tenet_graph_launch(tenet_graph graph, tenet_stream stream);
+int tenet_graph_destroy(tenet_graph graph);
+
+/* ---- diagnostics -------------------------------------------------------- */
+/* Average duration (ms) of the last `tenet_timer_*` bracket, measured with
+ * HIP events on the given stream. */
+int tenet_timer_start(tenet_ctx ctx, tenet_stream stream);
+int tenet_timer_stop(tenet_ctx ctx, tenet_stream stream, float* ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TENET_HIP_H */

@@ -1,0 +1,127 @@
+// Internal declarations shared by the host runtime (tenet_hip.cpp) and the
+// HIP kernel files.  Nothing here crosses the C-ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+namespace tenet {
+
+// ---- error plumbing ------------------------------------------------------
+void set_error(const std::string& msg);
+#define TENET_HIP_CHECK(expr)                                                                  \
+  do {                                                                                         \
+    hipError_t _e = (expr);                                                                    \
+    if (_e != hipSuccess) {                                                                    \
+      ::tenet::set_error(std::string(#expr) + " failed: " + hipGetErrorString(_e));            \
+      return TENET_ERR_HIP;                                                                    \
+    }                                                                                          \
+  } while (0)
+
+// ---- K1: gather-GEMM over offset tables ------------------------------------
+// C[rowC[m]+colC[n]+bC[z]] = alpha * sum_k A[rowA[m]+kA[k]+bA[z]] * B[kB[k]+colB[n]+bB[z]] + beta*C
+struct GemmTables {
+  const int64_t* rowA;
+  const int64_t* kA;
+  const int64_t* bA;
+  const int64_t* kB;
+  const int64_t* colB;
+  const int64_t* bB;
+  const int64_t* rowC;
+  const int64_t* colC;
+  const int64_t* bC;
+};
+
+struct GemmLaunch {
+  const void* A;
+  const void* B;
+  void* C;
+  GemmTables t;
+  int64_t M, N, K, batch;
+  int conjA, conjB;
+  int a_mfast;  // rowA has unit stride: load A tiles with consecutive lanes along m
+  int b_nfast;  // colB has unit stride: load B tiles with consecutive lanes along n
+  double alpha_re, alpha_im, beta_re, beta_im;
+};
+
+// prec: 0 native (fp64 for c128), 1 fp32, 2 split-bf16, 3 split-fp16
+int launch_gemm_c128(const GemmLaunch& g, int prec, hipStream_t s, void* ws, size_t ws_bytes);
+int launch_gemm_c64(const GemmLaunch& g, int prec, hipStream_t s, void* ws, size_t ws_bytes);
+size_t gemm_workspace_bytes(const GemmLaunch& g, int dtype, int prec);
+
+// Grouped GEMM: an array of problems in device memory, one launch.
+struct GroupedProblem {
+  const void* A;
+  const void* B;
+  void* C;
+  GemmTables t;
+  int64_t M, N, K, batch;
+  int32_t a_mfast, b_nfast;
+  int64_t tiles_m, tiles_n;
+  int64_t tile_begin;  // prefix sum of tiles over problems
+};
+int launch_grouped_gemm(const GroupedProblem* probs_dev, int64_t nprob, int64_t total_tiles, int dtype,
+                        hipStream_t s);
+
+// ---- K2: Householder QR ----------------------------------------------------
+// In-place QR of column-major W (m x n, ld = m) for `batch` matrices laid out
+// back to back (stride m*n); tau (k per matrix).  Then Q (m x k) / R (k x n).
+int launch_qr_factor(double* W, double* tau, int64_t m, int64_t n, int64_t batch, hipStream_t s);
+// flags: TENET_QR_Q_ADJ writes Q^H (k x m, ldq); TENET_QR_R_ADJ writes R^H (n x k, ldr)
+int launch_qr_extract(const double* W, const double* tau, int64_t m, int64_t n, int64_t batch, void* Q,
+                      int64_t ldq, void* R, int64_t ldr, int dtype, int flags, hipStream_t s);
+
+// ---- K3: one-sided block Jacobi SVD -----------------------------------------
+struct SvdProblem {
+  double* X;      // nr x ncp work matrix (complex, column-major)
+  double* V;      // ncp x ncp accumulated rotations
+  double* Gpart;  // npairs * q partial Gram blocks (2b x 2b complex)
+  double* sigma;  // ncp
+  int32_t* perm;  // ncp
+  int32_t* flag;  // [0] = rotated in this sweep, [1] = converged
+  int32_t nr, ncols, ncp, b, nblk, q;
+  int32_t m, n, transposed;
+  const void* A;  // input (m x n, lda)
+  int64_t lda;
+  void* U;
+  int64_t ldu;
+  void* Vh;
+  int64_t ldvh;
+  double* S;
+  int64_t maxdim;
+  double threshold;
+  int64_t* k_dev;
+  double* disc_dev;
+  double tol;
+};
+// Problems are passed either as a device array (grouped) or inline (one).
+struct SvdBatch {
+  const SvdProblem* arr;
+  SvdProblem one;
+};
+int launch_svd_init(const SvdBatch& b, int64_t nprob, int64_t max_elems, int dtype, hipStream_t s);
+int launch_svd_round(const SvdBatch& b, int64_t nprob, int max_items, int round, int phase, hipStream_t s);
+int launch_svd_sweep_end(const SvdBatch& b, int64_t nprob, hipStream_t s);
+int launch_svd_finalize(const SvdBatch& b, int64_t nprob, int max_ncp, int64_t max_elems, int dtype,
+                        hipStream_t s);
+
+// ---- misc elementwise / K4 ----------------------------------------------------
+int launch_fill_zero(void* dst, size_t bytes, hipStream_t s);
+// Gate update epilogue: Aout = U[:, :k] * f(S), Bout = compact(f(S) * Vh[:k, :]).
+int launch_absorb(const void* U, int64_t m, const void* Vh, int64_t ldvh, int64_t n, const double* S,
+                  const int64_t* k_dev, int absorb, void* Aout, void* Bout, int64_t kmax, int dtype,
+                  hipStream_t s);
+// scale by 1/max|x| and accumulate log(max) into acc[2]; optionally mantissa out
+int launch_rescale_log(void* x, int64_t count, double* acc, int dtype, hipStream_t s);
+int launch_amplitudes(int64_t nreq, const int64_t* n_dev, const uint64_t* site_ptrs, const uint64_t* chis,
+                      const uint64_t* bits, int64_t d, double* out, int dtype, int max_chi, hipStream_t s);
+int launch_write_scalar_from(const void* env, double* out, int dtype, hipStream_t s);
+int launch_trace_diag(const void* env, int64_t chi, double* out, int dtype, hipStream_t s);
+int launch_convert(const void* src, int src_dtype, void* dst, int dst_dtype, int64_t count, hipStream_t s);
+int launch_pack_c128(const void* src, int dtype, int64_t m, int64_t n, int64_t lda, int adj, void* dst,
+                     hipStream_t s);
+int launch_apply_pauli(const void* A, int64_t chil, int64_t chir, char which, void* out, int dtype, hipStream_t s);
+int launch_set_identity(void* out, int64_t chi, int dtype, hipStream_t s);
+
+}  // namespace tenet

@@ -1,0 +1,408 @@
+// Small kernels around the hot path: fills, the gate-update epilogue that
+// absorbs singular values (Muscle.simple_update's AbsorbEqually,
+// simple_update.jl:63), the running log-scale of transfer-matrix sweeps, and
+// K4's batched amplitude sweep <x|psi> (overlap.jl:22-32).
+#include "../../include/tenet_hip.h"
+#include "internal.h"
+
+namespace tenet {
+
+__global__ void fill_zero_kernel(uint32_t* p, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0u;
+}
+__global__ void fill_zero_bytes_kernel(uint8_t* p, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0;
+}
+
+int launch_fill_zero(void* dst, size_t bytes, hipStream_t s) {
+  if (bytes == 0) return TENET_OK;
+  if (((uintptr_t)dst % 4) == 0 && bytes % 4 == 0) {
+    const size_t n = bytes / 4;
+    const unsigned g = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(fill_zero_kernel, dim3(g), dim3(256), 0, s, (uint32_t*)dst, n);
+  } else {
+    const unsigned g = (unsigned)std::min<size_t>((bytes + 255) / 256, 4096);
+    hipLaunchKernelGGL(fill_zero_bytes_kernel, dim3(g), dim3(256), 0, s, (uint8_t*)dst, bytes);
+  }
+  TENET_HIP_CHECK(hipGetLastError());
+  return TENET_OK;
+}
+
+// Aout (m x k) = U[:, :k] * fa(S);  Bout (k x n, ld k) = fb(S) * Vh[:k, :] (ld kmax)
+template <typename S2>
+__global__ void absorb_kernel(const S2* __restrict__ U, int64_t m, const S2* __restrict__ Vh, int64_t ldvh, int64_t n,
+                              const double* __restrict__ S, const int64_t* __restrict__ k_dev, int mode,
+                              S2* __restrict__ Aout, S2* __restrict__ Bout) {
+  const int64_t k = *k_dev;
+  const size_t na = (size_t)m * k, nb = (size_t)k * n;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < na + nb; e += (size_t)gridDim.x * blockDim.x) {
+    if (e < na) {
+      const int64_t j = (int64_t)(e / m);
+      const double s = S[j];
+      const double f = mode == TENET_ABSORB_EQUAL ? sqrt(s) : (mode == TENET_ABSORB_LEFT ? s : 1.0);
+      S2 v = U[e];
+      v.x = (double)v.x * f;
+      v.y = (double)v.y * f;
+      Aout[e] = v;
+    } else {
+      const size_t f2 = e - na;
+      const int64_t j = (int64_t)(f2 % k), c = (int64_t)(f2 / k);
+      const double s = S[j];
+      const double f = mode == TENET_ABSORB_EQUAL ? sqrt(s) : (mode == TENET_ABSORB_RIGHT ? s : 1.0);
+      S2 v = Vh[(size_t)c * ldvh + j];
+      v.x = (double)v.x * f;
+      v.y = (double)v.y * f;
+      Bout[f2] = v;
+    }
+  }
+}
+
+int launch_absorb(const void* U, int64_t m, const void* Vh, int64_t ldvh, int64_t n, const double* S,
+                  const int64_t* k_dev, int absorb, void* Aout, void* Bout, int64_t kmax, int dtype, hipStream_t s) {
+  const size_t total = (size_t)(m + n) * kmax;
+  const unsigned g = (unsigned)std::max<size_t>(1, std::min<size_t>((total + 255) / 256, 2048));
+  if (dtype == TENET_C128)
+    hipLaunchKernelGGL(absorb_kernel<double2>, dim3(g), dim3(256), 0, s, (const double2*)U, m, (const double2*)Vh,
+                       ldvh, n, S, k_dev, absorb, (double2*)Aout, (double2*)Bout);
+  else
+    hipLaunchKernelGGL(absorb_kernel<float2>, dim3(g), dim3(256), 0, s, (const float2*)U, m, (const float2*)Vh, ldvh,
+                       n, S, k_dev, absorb, (float2*)Aout, (float2*)Bout);
+  TENET_HIP_CHECK(hipGetLastError());
+  return TENET_OK;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
+__device__ __forceinline__ double wave_add(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// x /= max|x| (max over re and im parts); acc[2] += log(max)
+template <typename S2>
+__global__ __launch_bounds__(1024) void rescale_log_kernel(S2* __restrict__ x, int64_t count, double* acc) {
+  __shared__ double red[16];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  double mx = 0.0;
+  for (int64_t i = tid; i < count; i += blockDim.x) {
+    const S2 v = x[i];
+    mx = fmax(mx, fmax(fabs((double)v.x), fabs((double)v.y)));
+  }
+  mx = wave_max(mx);
+  if (lane == 0) red[wave] = mx;
+  __syncthreads();
+  if (tid == 0) {
+    double m = 0.0;
+    for (int w = 0; w < (int)(blockDim.x / 64); ++w) m = fmax(m, red[w]);
+    red[0] = m;
+  }
+  __syncthreads();
+  const double m = red[0];
+  if (m > 0.0) {
+    const double inv = 1.0 / m;
+    for (int64_t i = tid; i < count; i += blockDim.x) {
+      S2 v = x[i];
+      v.x = (double)v.x * inv;
+      v.y = (double)v.y * inv;
+      x[i] = v;
+    }
+    if (tid == 0) acc[2] += log(m);
+  }
+}
+
+int launch_rescale_log(void* x, int64_t count, double* acc, int dtype, hipStream_t s) {
+  if (dtype == TENET_C128)
+    hipLaunchKernelGGL(rescale_log_kernel<double2>, dim3(1), dim3(1024), 0, s, (double2*)x, count, acc);
+  else
+    hipLaunchKernelGGL(rescale_log_kernel<float2>, dim3(1), dim3(1024), 0, s, (float2*)x, count, acc);
+  TENET_HIP_CHECK(hipGetLastError());
+  return TENET_OK;
+}
+
+template <typename S2>
+__global__ void write_scalar_kernel(const S2* env, double* out) {
+  out[0] = (double)env[0].x;
+  out[1] = (double)env[0].y;
+}
+int launch_write_scalar_from(const void* env, double* out, int dtype, hipStream_t s) {
+  if (dtype == TENET_C128)
+    hipLaunchKernelGGL(write_scalar_kernel<double2>, dim3(1), dim3(1), 0, s, (const double2*)env, out);
+  else
+    hipLaunchKernelGGL(write_scalar_kernel<float2>, dim3(1), dim3(1), 0, s, (const float2*)env, out);
+  TENET_HIP_CHECK(hipGetLastError());
+  return TENET_OK;
+}
+
+// out[0:2] = trace(env) for a chi x chi column-major env
+template <typename S2>
+__global__ __launch_bounds__(256) void trace_kernel(const S2* env, int64_t chi, double* out) {
+  __shared__ double rr[4], ri[4];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  double sr = 0.0, si = 0.0;
+  for (int64_t i = tid; i < chi; i += 256) {
+    const S2 v = env[i * chi + i];
+    sr += (double)v.x;
+    si += (double)v.y;
+  }
+  sr = wave_add(sr);
+  si = wave_add(si);
+  if (lane == 0) {
+    rr[wave] = sr;
+    ri[wave] = si;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    out[0] = rr[0] + rr[1] + rr[2] + rr[3];
+    out[1] = ri[0] + ri[1] + ri[2] + ri[3];
+  }
+}
+int launch_trace_diag(const void* env, int64_t chi, double* out, int dtype, hipStream_t s) {
+  if (dtype == TENET_C128)
+    hipLaunchKernelGGL(trace_kernel<double2>, dim3(1), dim3(256), 0, s, (const double2*)env, chi, out);
+  else
+    hipLaunchKernelGGL(trace_kernel<float2>, dim3(1), dim3(256), 0, s, (const float2*)env, chi, out);
+  TENET_HIP_CHECK(hipGetLastError());
+  return TENET_OK;
+}
+
+template <typename SI, typename SO>
+__global__ void convert_kernel(const SI* __restrict__ src, SO* __restrict__ dst, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const SI v = src[i];
+    SO o;
+    o.x = v.x;
+    o.y = v.y;
+    dst[i] = o;
+  }
+}
+int launch_convert(const void* src, int src_dtype, void* dst, int dst_dtype, int64_t count, hipStream_t s) {
+  const unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>((count + 255) / 256, 4096));
+  if (src_dtype == TENET_C128 && dst_dtype == TENET_C64)
+    hipLaunchKernelGGL((convert_kernel<double2, float2>), dim3(g), dim3(256), 0, s, (const double2*)src, (float2*)dst,
+                       count);
+  else if (src_dtype == TENET_C64 && dst_dtype == TENET_C128)
+    hipLaunchKernelGGL((convert_kernel<float2, double2>), dim3(g), dim3(256), 0, s, (const float2*)src, (double2*)dst,
+                       count);
+  else if (src_dtype == TENET_C128)
+    hipLaunchKernelGGL((convert_kernel<double2, double2>), dim3(g), dim3(256), 0, s, (const double2*)src,
+                       (double2*)dst, count);
+  else
+    hipLaunchKernelGGL((convert_kernel<float2, float2>), dim3(g), dim3(256), 0, s, (const float2*)src, (float2*)dst,
+                       count);
+  TENET_HIP_CHECK(hipGetLastError());
+  return TENET_OK;
+}
+
+// ---- K4: batched amplitude sweep ------------------------------------------------
+// One workgroup per request.  The running row vector v (chi <= max_chi) lives
+// in LDS as FP64; per site v'[r] = sum_l v[l] A[l, x, r] with A stored
+// (chi_l, d, chi_r) column-major, so the slice A[:, x, :] is a chi_l x chi_r
+// matrix of leading dimension chi_l*d whose columns are contiguous.  Each wave
+// takes columns r; its lanes stride over l (coalesced) and reduce with
+// cross-lane shuffles.  v is rescaled by max|v| after each site with the log
+// of the scale accumulated in FP64 (unnormalised states overflow otherwise).
+constexpr int AMP_THREADS = 256;
+constexpr int AMP_MAX_CHI = 2048;
+
+template <typename S2>
+__global__ __launch_bounds__(AMP_THREADS) void amplitude_kernel(const int64_t* __restrict__ n_dev,
+                                                                 const uint64_t* __restrict__ site_ptrs,
+                                                                 const uint64_t* __restrict__ chis,
+                                                                 const uint64_t* __restrict__ bits, int64_t d,
+                                                                 double* __restrict__ out) {
+  __shared__ double vr[2][AMP_MAX_CHI], vi[2][AMP_MAX_CHI];
+  __shared__ double red[AMP_THREADS / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t req = blockIdx.x;
+  const int64_t n = n_dev[req];
+  const uint64_t* const sites = reinterpret_cast<const uint64_t*>(site_ptrs[req]);
+  const int64_t* const chi = reinterpret_cast<const int64_t*>(chis[req]);
+  const uint8_t* const x = reinterpret_cast<const uint8_t*>(bits[req]);
+  if (tid == 0) {
+    vr[0][0] = 1.0;
+    vi[0][0] = 0.0;
+  }
+  double logscale = 0.0;
+  int cur = 0;
+  __syncthreads();
+  for (int64_t i = 0; i < n; ++i) {
+    const int64_t cl = chi[i], cr = chi[i + 1];
+    if (cl < 1 || cr < 1 || cl > AMP_MAX_CHI || cr > AMP_MAX_CHI || (int64_t)x[i] >= d) {
+      // uniform across the workgroup: report NaN instead of reading out of bounds
+      if (tid == 0) {
+        out[3 * req + 0] = __builtin_nan("");
+        out[3 * req + 1] = __builtin_nan("");
+        out[3 * req + 2] = 0.0;
+      }
+      return;
+    }
+    const S2* A = reinterpret_cast<const S2*>(sites[i]) + (int64_t)x[i] * cl;
+    const int64_t ld = cl * d;
+    for (int64_t r = wave; r < cr; r += AMP_THREADS / 64) {
+      const S2* col = A + r * ld;
+      double sr = 0.0, si = 0.0;
+      for (int64_t l = lane; l < cl; l += 64) {
+        const S2 a = col[l];
+        const double ar = (double)a.x, ai = (double)a.y;
+        const double pr = vr[cur][l], pi = vi[cur][l];
+        sr += pr * ar - pi * ai;
+        si += pr * ai + pi * ar;
+      }
+      sr = wave_add(sr);
+      si = wave_add(si);
+      if (lane == 0) {
+        vr[cur ^ 1][r] = sr;
+        vi[cur ^ 1][r] = si;
+      }
+    }
+    __syncthreads();
+    cur ^= 1;
+    double mx = 0.0;
+    for (int64_t r = tid; r < cr; r += AMP_THREADS) mx = fmax(mx, fmax(fabs(vr[cur][r]), fabs(vi[cur][r])));
+    mx = wave_max(mx);
+    if (lane == 0) red[wave] = mx;
+    __syncthreads();
+    double m = 0.0;
+    for (int w = 0; w < AMP_THREADS / 64; ++w) m = fmax(m, red[w]);
+    if (m > 0.0) {
+      const double inv = 1.0 / m;
+      for (int64_t r = tid; r < cr; r += AMP_THREADS) {
+        vr[cur][r] *= inv;
+        vi[cur][r] *= inv;
+      }
+      logscale += log(m);
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    out[3 * req + 0] = vr[cur][0];
+    out[3 * req + 1] = vi[cur][0];
+    out[3 * req + 2] = logscale;
+  }
+}
+
+int launch_amplitudes(int64_t nreq, const int64_t* n_dev, const uint64_t* site_ptrs, const uint64_t* chis,
+                      const uint64_t* bits, int64_t d, double* out, int dtype, int max_chi, hipStream_t s) {
+  if (nreq <= 0) return TENET_OK;
+  if (max_chi > AMP_MAX_CHI) {
+    set_error("amplitudes: bond dimension above 2048");
+    return TENET_ERR_UNSUPPORTED;
+  }
+  if (dtype == TENET_C128)
+    hipLaunchKernelGGL(amplitude_kernel<double2>, dim3((unsigned)nreq), dim3(AMP_THREADS), 0, s, n_dev, site_ptrs,
+                       chis, bits, d, out);
+  else
+    hipLaunchKernelGGL(amplitude_kernel<float2>, dim3((unsigned)nreq), dim3(AMP_THREADS), 0, s, n_dev, site_ptrs,
+                       chis, bits, d, out);
+  TENET_HIP_CHECK(hipGetLastError());
+  return TENET_OK;
+}
+
+// dst (m x n, ld m, complex FP64) <- src (m x n, ld lda, dtype), or, with
+// adj, <- src^H where src is n x m with leading dimension lda.
+template <typename S2>
+__global__ void pack_c128_kernel(const S2* __restrict__ src, int64_t m, int64_t n, int64_t lda, int adj,
+                                 double2* __restrict__ dst) {
+  const size_t total = (size_t)m * n;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    const int64_t r = (int64_t)(e % m), c = (int64_t)(e / m);
+    double2 o;
+    if (!adj) {
+      const S2 v = src[(size_t)c * lda + r];
+      o.x = v.x;
+      o.y = v.y;
+    } else {
+      const S2 v = src[(size_t)r * lda + c];
+      o.x = v.x;
+      o.y = -(double)v.y;
+    }
+    dst[e] = o;
+  }
+}
+int launch_pack_c128(const void* src, int dtype, int64_t m, int64_t n, int64_t lda, int adj, void* dst,
+                     hipStream_t s) {
+  const size_t total = (size_t)m * n;
+  const unsigned g = (unsigned)std::max<size_t>(1, std::min<size_t>((total + 255) / 256, 4096));
+  if (dtype == TENET_C128)
+    hipLaunchKernelGGL(pack_c128_kernel<double2>, dim3(g), dim3(256), 0, s, (const double2*)src, m, n, lda, adj,
+                       (double2*)dst);
+  else
+    hipLaunchKernelGGL(pack_c128_kernel<float2>, dim3(g), dim3(256), 0, s, (const float2*)src, m, n, lda, adj,
+                       (double2*)dst);
+  TENET_HIP_CHECK(hipGetLastError());
+  return TENET_OK;
+}
+
+// out (chil, 2, chir) = P applied to the physical index of A (chil, 2, chir)
+// Pauli matrices as in ext/TenetPythonCallExt/pytket.jl:23-31.
+template <typename S2>
+__global__ void pauli_kernel(const S2* __restrict__ A, int64_t chil, int64_t chir, int which, S2* __restrict__ out) {
+  const size_t total = (size_t)chil * 2 * chir;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    const int64_t l = (int64_t)(e % chil);
+    const int64_t o = (int64_t)((e / chil) % 2);
+    const int64_t r = (int64_t)(e / (2 * chil));
+    const S2 a0 = A[l + chil * (0 + 2 * r)];
+    const S2 a1 = A[l + chil * (1 + 2 * r)];
+    S2 v;
+    if (which == 'X') {
+      v = o == 0 ? a1 : a0;
+    } else if (which == 'Y') {  // [[0, -i], [i, 0]]
+      if (o == 0) {
+        v.x = a1.y;
+        v.y = -a1.x;
+      } else {
+        v.x = -a0.y;
+        v.y = a0.x;
+      }
+    } else if (which == 'Z') {
+      v = o == 0 ? a0 : a1;
+      if (o == 1) {
+        v.x = -v.x;
+        v.y = -v.y;
+      }
+    } else {
+      v = o == 0 ? a0 : a1;
+    }
+    out[e] = v;
+  }
+}
+int launch_apply_pauli(const void* A, int64_t chil, int64_t chir, char which, void* out, int dtype, hipStream_t s) {
+  const size_t total = (size_t)chil * 2 * chir;
+  const unsigned g = (unsigned)std::max<size_t>(1, std::min<size_t>((total + 255) / 256, 4096));
+  if (dtype == TENET_C128)
+    hipLaunchKernelGGL(pauli_kernel<double2>, dim3(g), dim3(256), 0, s, (const double2*)A, chil, chir, (int)which,
+                       (double2*)out);
+  else
+    hipLaunchKernelGGL(pauli_kernel<float2>, dim3(g), dim3(256), 0, s, (const float2*)A, chil, chir, (int)which,
+                       (float2*)out);
+  TENET_HIP_CHECK(hipGetLastError());
+  return TENET_OK;
+}
+
+template <typename S2>
+__global__ void identity_kernel(S2* __restrict__ out, int64_t chi) {
+  const size_t total = (size_t)chi * chi;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    S2 v;
+    v.x = ((int64_t)(e % chi) == (int64_t)(e / chi)) ? 1 : 0;
+    v.y = 0;
+    out[e] = v;
+  }
+}
+int launch_set_identity(void* out, int64_t chi, int dtype, hipStream_t s) {
+  const size_t total = (size_t)chi * chi;
+  const unsigned g = (unsigned)std::max<size_t>(1, std::min<size_t>((total + 255) / 256, 4096));
+  if (dtype == TENET_C128)
+    hipLaunchKernelGGL(identity_kernel<double2>, dim3(g), dim3(256), 0, s, (double2*)out, chi);
+  else
+    hipLaunchKernelGGL(identity_kernel<float2>, dim3(g), dim3(256), 0, s, (float2*)out, chi);
+  TENET_HIP_CHECK(hipGetLastError());
+  return TENET_OK;
+}
+
+}  // namespace tenet

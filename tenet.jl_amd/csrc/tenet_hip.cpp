@@ -1,0 +1,1244 @@
+// Host runtime of libtenet_hip.so: contexts, the stream-ordered memory pool,
+// the contraction planner (labels -> offset tables), and the orchestration
+// of QR / SVD / gate update / transfer sweeps on top of the HIP kernels.
+// Every exported symbol is declared in include/tenet_hip.h.
+#include "../../include/tenet_hip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <list>
+#include <map>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "internal.h"
+
+namespace tenet {
+static thread_local std::string g_err;
+void set_error(const std::string& m) { g_err = m; }
+}  // namespace tenet
+
+using namespace tenet;
+
+#define CHECK_ARG(cond, msg)      \
+  do {                            \
+    if (!(cond)) {                \
+      set_error(msg);             \
+      return TENET_ERR_ARG;       \
+    }                             \
+  } while (0)
+#define TRY(expr)              \
+  do {                         \
+    int _st = (expr);          \
+    if (_st != TENET_OK) return _st; \
+  } while (0)
+
+struct Workspace {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+};
+
+struct tenet_plan_s {
+  tenet_ctx ctx = nullptr;
+  int dtype = 0, prec = 0;
+  int64_t M = 0, N = 0, K = 0, batch = 0;
+  int a_mfast = 0, b_nfast = 0;
+  int64_t* tables = nullptr;
+  GemmTables t{};
+  size_t ws_bytes = 0;
+  bool cached = false;
+  // spans (elements) each operand must cover, for pool bounds checks
+  int64_t spanA = 0, spanB = 0, spanC = 0;
+};
+
+struct tenet_graph_s {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int device = 0;
+};
+
+struct tenet_ctx_s {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::mutex mu;
+  std::map<size_t, std::vector<void*>> free_blocks;
+  std::map<uintptr_t, size_t> live;  // base -> class size
+  size_t pool_bytes = 0;
+  std::unordered_map<hipStream_t, Workspace> ws;
+  std::unordered_map<hipStream_t, Workspace> ws_gemm;
+  std::unordered_map<std::string, tenet_plan> plans;
+  std::list<std::string> plan_lru;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int32_t* host_flag = nullptr;  // pinned
+  int64_t* host_k = nullptr;     // pinned
+};
+
+static const size_t ELEM[2] = {16, 8};
+
+static inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+// ---------------------------------------------------------------------------
+// device guard / workspace
+// ---------------------------------------------------------------------------
+static int use_device(tenet_ctx ctx) {
+  if (!ctx) {
+    set_error("null context");
+    return TENET_ERR_ARG;
+  }
+  TENET_HIP_CHECK(hipSetDevice(ctx->device));
+  return TENET_OK;
+}
+
+static hipStream_t pick(tenet_ctx ctx, tenet_stream s) { return s ? (hipStream_t)s : ctx->stream; }
+
+static bool capturing(hipStream_t s) {
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &st) != hipSuccess) return false;
+  return st == hipStreamCaptureStatusActive;
+}
+
+// Stream-ordered scratch: calls on one stream reuse the same region in order.
+// `which` 0: the API call's own scratch; 1: contraction scratch (split packs),
+// so an orchestrating call and the contractions it issues never overlap.
+static int ws_get(tenet_ctx ctx, hipStream_t s, size_t bytes, void** out, int which = 0) {
+  std::lock_guard<std::mutex> g(ctx->mu);
+  Workspace& w = which ? ctx->ws_gemm[s] : ctx->ws[s];
+  if (w.bytes < bytes) {
+    if (capturing(s)) {
+      set_error("workspace would grow during graph capture: run the sequence once before capturing");
+      return TENET_ERR_OOM;
+    }
+    const size_t nb = align_up(std::max(bytes, w.bytes + w.bytes / 2), 1 << 20);
+    TENET_HIP_CHECK(hipStreamSynchronize(s));
+    if (w.ptr) TENET_HIP_CHECK(hipFree(w.ptr));
+    w.ptr = nullptr;
+    w.bytes = 0;
+    if (hipMalloc(&w.ptr, nb) != hipSuccess) {
+      (void)hipGetLastError();
+      w.ptr = nullptr;
+      set_error("workspace allocation failed");
+      return TENET_ERR_OOM;
+    }
+    w.bytes = nb;
+  }
+  *out = w.ptr;
+  return TENET_OK;
+}
+
+struct Bump {
+  char* base;
+  size_t off = 0;
+  explicit Bump(void* p) : base((char*)p) {}
+  void* take(size_t b) {
+    void* p = base + off;
+    off = align_up(off + b, 256);
+    return p;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// pool bounds check: if ptr lies in a pool block, [ptr, ptr+bytes) must fit
+// ---------------------------------------------------------------------------
+static int check_span(tenet_ctx ctx, const void* ptr, size_t bytes, const char* what) {
+  if (!ptr) {
+    set_error(std::string("null device pointer: ") + what);
+    return TENET_ERR_ARG;
+  }
+  std::lock_guard<std::mutex> g(ctx->mu);
+  const uintptr_t p = (uintptr_t)ptr;
+  auto it = ctx->live.upper_bound(p);
+  if (it == ctx->live.begin()) return TENET_OK;
+  --it;
+  if (p >= it->first && p < it->first + it->second) {
+    if (p + bytes > it->first + it->second) {
+      set_error(std::string("operand exceeds its allocation: ") + what);
+      return TENET_ERR_SHAPE;
+    }
+  }
+  return TENET_OK;
+}
+
+// ---------------------------------------------------------------------------
+// library / context
+// ---------------------------------------------------------------------------
+extern "C" const char* tenet_version(void) { return "tenet_hip 0.1.0 (gfx950)"; }
+extern "C" const char* tenet_last_error(void) { return g_err.c_str(); }
+
+extern "C" int tenet_device_count(int* count) {
+  CHECK_ARG(count, "null count");
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) {
+    (void)hipGetLastError();
+    n = 0;
+  }
+  *count = n;
+  return TENET_OK;
+}
+
+extern "C" int tenet_create(int device, tenet_ctx* out) {
+  CHECK_ARG(out, "null output");
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+    (void)hipGetLastError();
+    set_error("no HIP device visible");
+    return TENET_ERR_NODEVICE;
+  }
+  if (device < 0 || device >= n) {
+    set_error("device index out of range");
+    return TENET_ERR_ARG;
+  }
+  hipDeviceProp_t prop;
+  TENET_HIP_CHECK(hipGetDeviceProperties(&prop, device));
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+    set_error(std::string("device is not gfx950: ") + prop.gcnArchName);
+    return TENET_ERR_NODEVICE;
+  }
+  TENET_HIP_CHECK(hipSetDevice(device));
+  auto* c = new tenet_ctx_s();
+  c->device = device;
+  TENET_HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  TENET_HIP_CHECK(hipEventCreate(&c->ev0));
+  TENET_HIP_CHECK(hipEventCreate(&c->ev1));
+  TENET_HIP_CHECK(hipHostMalloc((void**)&c->host_flag, 4096, hipHostMallocDefault));
+  TENET_HIP_CHECK(hipHostMalloc((void**)&c->host_k, 64, hipHostMallocDefault));
+  *out = c;
+  return TENET_OK;
+}
+
+extern "C" int tenet_destroy(tenet_ctx ctx) {
+  if (!ctx) return TENET_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipDeviceSynchronize();
+  for (auto& kv : ctx->plans) {
+    if (kv.second->tables) (void)hipFree(kv.second->tables);
+    delete kv.second;
+  }
+  for (auto& kv : ctx->free_blocks)
+    for (void* p : kv.second) (void)hipFree(p);
+  for (auto& kv : ctx->live) (void)hipFree((void*)kv.first);
+  for (auto& kv : ctx->ws)
+    if (kv.second.ptr) (void)hipFree(kv.second.ptr);
+  for (auto& kv : ctx->ws_gemm)
+    if (kv.second.ptr) (void)hipFree(kv.second.ptr);
+  (void)hipHostFree(ctx->host_flag);
+  (void)hipHostFree(ctx->host_k);
+  (void)hipEventDestroy(ctx->ev0);
+  (void)hipEventDestroy(ctx->ev1);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return TENET_OK;
+}
+
+extern "C" int tenet_stream_create(tenet_ctx ctx, tenet_stream* s) {
+  TRY(use_device(ctx));
+  CHECK_ARG(s, "null stream out");
+  hipStream_t st;
+  TENET_HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  *s = (tenet_stream)st;
+  return TENET_OK;
+}
+extern "C" int tenet_stream_destroy(tenet_ctx ctx, tenet_stream s) {
+  TRY(use_device(ctx));
+  if (!s) return TENET_OK;
+  TENET_HIP_CHECK(hipStreamSynchronize((hipStream_t)s));
+  {
+    std::lock_guard<std::mutex> g(ctx->mu);
+    for (auto* mp : {&ctx->ws, &ctx->ws_gemm}) {
+      auto it = mp->find((hipStream_t)s);
+      if (it != mp->end()) {
+        if (it->second.ptr) (void)hipFree(it->second.ptr);
+        mp->erase(it);
+      }
+    }
+  }
+  TENET_HIP_CHECK(hipStreamDestroy((hipStream_t)s));
+  return TENET_OK;
+}
+extern "C" int tenet_stream_sync(tenet_ctx ctx, tenet_stream s) {
+  TRY(use_device(ctx));
+  TENET_HIP_CHECK(hipStreamSynchronize(pick(ctx, s)));
+  return TENET_OK;
+}
+extern "C" int tenet_device_sync(tenet_ctx ctx) {
+  TRY(use_device(ctx));
+  TENET_HIP_CHECK(hipDeviceSynchronize());
+  return TENET_OK;
+}
+
+// ---------------------------------------------------------------------------
+// memory pool
+// ---------------------------------------------------------------------------
+static size_t size_class(size_t b) {
+  if (b <= 256) return 256;
+  if (b <= (1u << 20)) {
+    size_t c = 256;
+    while (c < b) c <<= 1;
+    return c;
+  }
+  return align_up(b, 1 << 20);
+}
+
+extern "C" int tenet_malloc(tenet_ctx ctx, size_t bytes, void** dptr) {
+  TRY(use_device(ctx));
+  CHECK_ARG(dptr, "null output");
+  const size_t cls = size_class(std::max<size_t>(bytes, 1));
+  std::lock_guard<std::mutex> g(ctx->mu);
+  auto it = ctx->free_blocks.find(cls);
+  if (it != ctx->free_blocks.end() && !it->second.empty()) {
+    void* p = it->second.back();
+    it->second.pop_back();
+    ctx->live[(uintptr_t)p] = cls;
+    *dptr = p;
+    return TENET_OK;
+  }
+  void* p = nullptr;
+  if (hipMalloc(&p, cls) != hipSuccess) {
+    (void)hipGetLastError();
+    // release cached blocks and retry once
+    (void)hipDeviceSynchronize();
+    for (auto& kv : ctx->free_blocks)
+      for (void* q : kv.second) {
+        (void)hipFree(q);
+        ctx->pool_bytes -= kv.first;
+      }
+    ctx->free_blocks.clear();
+    if (hipMalloc(&p, cls) != hipSuccess) {
+      (void)hipGetLastError();
+      set_error("device allocation failed");
+      return TENET_ERR_OOM;
+    }
+  }
+  ctx->pool_bytes += cls;
+  ctx->live[(uintptr_t)p] = cls;
+  *dptr = p;
+  return TENET_OK;
+}
+
+extern "C" int tenet_free(tenet_ctx ctx, void* dptr) {
+  if (!ctx) return TENET_ERR_ARG;
+  if (!dptr) return TENET_OK;
+  std::lock_guard<std::mutex> g(ctx->mu);
+  auto it = ctx->live.find((uintptr_t)dptr);
+  if (it == ctx->live.end()) {
+    set_error("tenet_free: pointer not from tenet_malloc");
+    return TENET_ERR_ARG;
+  }
+  ctx->free_blocks[it->second].push_back(dptr);
+  ctx->live.erase(it);
+  return TENET_OK;
+}
+
+extern "C" int tenet_memcpy_h2d(tenet_ctx ctx, void* dst, const void* src, size_t bytes, tenet_stream s) {
+  TRY(use_device(ctx));
+  if (bytes == 0) return TENET_OK;
+  TRY(check_span(ctx, dst, bytes, "h2d dst"));
+  hipStream_t st = pick(ctx, s);
+  TENET_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st));
+  TENET_HIP_CHECK(hipStreamSynchronize(st));  // pageable source: complete before returning
+  return TENET_OK;
+}
+extern "C" int tenet_memcpy_d2h(tenet_ctx ctx, void* dst, const void* src, size_t bytes, tenet_stream s) {
+  TRY(use_device(ctx));
+  if (bytes == 0) return TENET_OK;
+  TRY(check_span(ctx, src, bytes, "d2h src"));
+  hipStream_t st = pick(ctx, s);
+  TENET_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st));
+  TENET_HIP_CHECK(hipStreamSynchronize(st));
+  return TENET_OK;
+}
+extern "C" int tenet_memcpy_d2d(tenet_ctx ctx, void* dst, const void* src, size_t bytes, tenet_stream s) {
+  TRY(use_device(ctx));
+  if (bytes == 0) return TENET_OK;
+  TRY(check_span(ctx, dst, bytes, "d2d dst"));
+  TRY(check_span(ctx, src, bytes, "d2d src"));
+  TENET_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, pick(ctx, s)));
+  return TENET_OK;
+}
+extern "C" int tenet_memset_zero(tenet_ctx ctx, void* dst, size_t bytes, tenet_stream s) {
+  TRY(use_device(ctx));
+  TRY(check_span(ctx, dst, bytes, "memset dst"));
+  return launch_fill_zero(dst, bytes, pick(ctx, s));
+}
+extern "C" int tenet_mem_info(tenet_ctx ctx, size_t* free_b, size_t* total_b, size_t* pool_b) {
+  TRY(use_device(ctx));
+  size_t f = 0, t = 0;
+  TENET_HIP_CHECK(hipMemGetInfo(&f, &t));
+  if (free_b) *free_b = f;
+  if (total_b) *total_b = t;
+  if (pool_b) *pool_b = ctx->pool_bytes;
+  return TENET_OK;
+}
+
+// ---------------------------------------------------------------------------
+// contraction planner
+// ---------------------------------------------------------------------------
+struct LabelInfo {
+  int64_t label, extent, sa, sb, sc;
+  bool ina, inb, inc;
+};
+
+static std::string plan_key(int dtype, int prec, const tenet_tdesc* a, const tenet_tdesc* b, const tenet_tdesc* c) {
+  std::string k;
+  k.reserve(1024);
+  auto put = [&](int64_t v) { k.append(reinterpret_cast<const char*>(&v), sizeof v); };
+  put(dtype);
+  put(prec);
+  for (const tenet_tdesc* d : {a, b, c}) {
+    put(d->rank);
+    for (int i = 0; i < d->rank; ++i) {
+      put(d->extent[i]);
+      put(d->stride[i]);
+      put(d->label[i]);
+    }
+  }
+  return k;
+}
+
+static void radix_table(const std::vector<LabelInfo*>& ls, int which, std::vector<int64_t>& out) {
+  int64_t total = 1;
+  for (auto* l : ls) total *= l->extent;
+  out.assign((size_t)total, 0);
+  int64_t span = 1;
+  for (auto* l : ls) {
+    const int64_t st = which == 0 ? l->sa : (which == 1 ? l->sb : l->sc);
+    for (int64_t i = 0; i < total; ++i) out[(size_t)i] += ((i / span) % l->extent) * st;
+    span *= l->extent;
+  }
+}
+
+static int build_plan(tenet_ctx ctx, int dtype, int prec, const tenet_tdesc* a, const tenet_tdesc* b,
+                      const tenet_tdesc* c, tenet_plan p) {
+  for (const tenet_tdesc* d : {a, b, c}) {
+    if (d->rank < 0 || d->rank > TENET_MAX_RANK) {
+      set_error("rank out of range");
+      return TENET_ERR_SHAPE;
+    }
+    for (int i = 0; i < d->rank; ++i) {
+      if (d->extent[i] < 0) {
+        set_error("negative extent");
+        return TENET_ERR_SHAPE;
+      }
+      for (int j = 0; j < i; ++j)
+        if (d->label[i] == d->label[j]) {
+          set_error("repeated label within one tensor");
+          return TENET_ERR_SHAPE;
+        }
+    }
+  }
+  std::vector<LabelInfo> L;
+  auto find = [&](int64_t lab) -> LabelInfo* {
+    for (auto& x : L)
+      if (x.label == lab) return &x;
+    return nullptr;
+  };
+  for (int i = 0; i < a->rank; ++i) L.push_back({a->label[i], a->extent[i], a->stride[i], 0, 0, true, false, false});
+  for (int i = 0; i < b->rank; ++i) {
+    LabelInfo* x = find(b->label[i]);
+    if (x) {
+      if (x->extent != b->extent[i]) {
+        set_error("extent mismatch on a shared label");
+        return TENET_ERR_SHAPE;
+      }
+      x->inb = true;
+      x->sb = b->stride[i];
+    } else {
+      L.push_back({b->label[i], b->extent[i], 0, b->stride[i], 0, false, true, false});
+    }
+  }
+  for (int i = 0; i < c->rank; ++i) {
+    LabelInfo* x = find(c->label[i]);
+    if (!x) {
+      set_error("output label absent from both operands");
+      return TENET_ERR_SHAPE;
+    }
+    if (x->extent != c->extent[i]) {
+      set_error("extent mismatch on an output label");
+      return TENET_ERR_SHAPE;
+    }
+    x->inc = true;
+    x->sc = c->stride[i];
+  }
+  std::vector<LabelInfo*> fa, fb, kk, bt;
+  for (auto& x : L) {
+    if (x.ina && x.inb) {
+      (x.inc ? bt : kk).push_back(&x);
+    } else if (x.ina) {
+      if (!x.inc) {
+        set_error("label of A alone must appear in the output");
+        return TENET_ERR_SHAPE;
+      }
+      fa.push_back(&x);
+    } else {
+      if (!x.inc) {
+        set_error("label of B alone must appear in the output");
+        return TENET_ERR_SHAPE;
+      }
+      fb.push_back(&x);
+    }
+  }
+  auto by = [](int w) {
+    return [w](const LabelInfo* x, const LabelInfo* y) {
+      const int64_t sx = std::llabs(w == 0 ? x->sa : (w == 1 ? x->sb : x->sc));
+      const int64_t sy = std::llabs(w == 0 ? y->sa : (w == 1 ? y->sb : y->sc));
+      return sx < sy;
+    };
+  };
+  std::stable_sort(fa.begin(), fa.end(), by(0));
+  std::stable_sort(fb.begin(), fb.end(), by(1));
+  std::stable_sort(kk.begin(), kk.end(), by(0));
+  std::stable_sort(bt.begin(), bt.end(), by(2));
+
+  std::vector<int64_t> rowA, kA, bA, kB, colB, bB, rowC, colC, bC;
+  radix_table(fa, 0, rowA);
+  radix_table(fa, 2, rowC);
+  radix_table(kk, 0, kA);
+  radix_table(kk, 1, kB);
+  radix_table(fb, 1, colB);
+  radix_table(fb, 2, colC);
+  radix_table(bt, 0, bA);
+  radix_table(bt, 1, bB);
+  radix_table(bt, 2, bC);
+  p->ctx = ctx;
+  p->dtype = dtype;
+  p->prec = prec;
+  p->M = (int64_t)rowA.size();
+  p->N = (int64_t)colB.size();
+  p->K = (int64_t)kA.size();
+  p->batch = (int64_t)bA.size();
+  p->a_mfast = (!fa.empty() && fa[0]->sa == 1) ? 1 : 0;
+  p->b_nfast = (!fb.empty() && fb[0]->sb == 1) ? 1 : 0;
+  auto span = [](const tenet_tdesc* d) {
+    int64_t s = 0;
+    for (int i = 0; i < d->rank; ++i) {
+      if (d->extent[i] == 0) return (int64_t)0;
+      s += (d->extent[i] - 1) * std::llabs(d->stride[i]);
+    }
+    return s + 1;
+  };
+  p->spanA = span(a);
+  p->spanB = span(b);
+  p->spanC = span(c);
+  const size_t total = rowA.size() + kA.size() + bA.size() + kB.size() + colB.size() + bB.size() + rowC.size() +
+                       colC.size() + bC.size();
+  std::vector<int64_t> host;
+  host.reserve(total);
+  std::vector<size_t> off;
+  for (auto* v : {&rowA, &kA, &bA, &kB, &colB, &bB, &rowC, &colC, &bC}) {
+    off.push_back(host.size());
+    host.insert(host.end(), v->begin(), v->end());
+  }
+  if (host.empty()) host.push_back(0);
+  TENET_HIP_CHECK(hipMalloc((void**)&p->tables, host.size() * sizeof(int64_t)));
+  TENET_HIP_CHECK(hipMemcpy(p->tables, host.data(), host.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+  int64_t* T = p->tables;
+  p->t.rowA = T + off[0];
+  p->t.kA = T + off[1];
+  p->t.bA = T + off[2];
+  p->t.kB = T + off[3];
+  p->t.colB = T + off[4];
+  p->t.bB = T + off[5];
+  p->t.rowC = T + off[6];
+  p->t.colC = T + off[7];
+  p->t.bC = T + off[8];
+  GemmLaunch g{};
+  g.M = p->M;
+  g.N = p->N;
+  g.K = p->K;
+  g.batch = p->batch;
+  p->ws_bytes = gemm_workspace_bytes(g, dtype, prec);
+  return TENET_OK;
+}
+
+static int valid_dtype_prec(int dtype, int prec) {
+  if (dtype != TENET_C128 && dtype != TENET_C64) {
+    set_error("unknown dtype");
+    return TENET_ERR_ARG;
+  }
+  if (prec < TENET_PREC_NATIVE || prec > TENET_PREC_SPLIT_FP16) {
+    set_error("unknown precision");
+    return TENET_ERR_ARG;
+  }
+  return TENET_OK;
+}
+
+extern "C" int tenet_einsum_plan(tenet_ctx ctx, int dtype, int prec, const tenet_tdesc* a, const tenet_tdesc* b,
+                                 const tenet_tdesc* c, tenet_plan* plan) {
+  TRY(use_device(ctx));
+  CHECK_ARG(a && b && c && plan, "null argument");
+  TRY(valid_dtype_prec(dtype, prec));
+  auto* p = new tenet_plan_s();
+  int st = build_plan(ctx, dtype, prec, a, b, c, p);
+  if (st != TENET_OK) {
+    if (p->tables) (void)hipFree(p->tables);
+    delete p;
+    return st;
+  }
+  *plan = p;
+  return TENET_OK;
+}
+
+static int exec_plan(tenet_plan p, hipStream_t s, const void* A, const void* B, void* C, int conj_a, int conj_b,
+                     const double* alpha, const double* beta) {
+  tenet_ctx ctx = p->ctx;
+  const size_t es = ELEM[p->dtype];
+  if (p->M * p->N * p->batch == 0) return TENET_OK;
+  TRY(check_span(ctx, C, (size_t)p->spanC * es, "C"));
+  if (p->K > 0) {
+    TRY(check_span(ctx, A, (size_t)p->spanA * es, "A"));
+    TRY(check_span(ctx, B, (size_t)p->spanB * es, "B"));
+  }
+  GemmLaunch g{};
+  g.A = A;
+  g.B = B;
+  g.C = C;
+  g.t = p->t;
+  g.M = p->M;
+  g.N = p->N;
+  g.K = p->K;
+  g.batch = p->batch;
+  g.conjA = conj_a;
+  g.conjB = conj_b;
+  g.a_mfast = p->a_mfast;
+  g.b_nfast = p->b_nfast;
+  g.alpha_re = alpha ? alpha[0] : 1.0;
+  g.alpha_im = alpha ? alpha[1] : 0.0;
+  g.beta_re = beta ? beta[0] : 0.0;
+  g.beta_im = beta ? beta[1] : 0.0;
+  void* ws = nullptr;
+  if (p->ws_bytes) TRY(ws_get(ctx, s, p->ws_bytes, &ws, 1));
+  if (p->dtype == TENET_C128) return launch_gemm_c128(g, p->prec, s, ws, p->ws_bytes);
+  return launch_gemm_c64(g, p->prec, s, ws, p->ws_bytes);
+}
+
+extern "C" int tenet_einsum_exec(tenet_plan plan, tenet_stream stream, const void* A, const void* B, void* C,
+                                 int conj_a, int conj_b, const double alpha[2], const double beta[2]) {
+  CHECK_ARG(plan, "null plan");
+  TRY(use_device(plan->ctx));
+  return exec_plan(plan, pick(plan->ctx, stream), A, B, C, conj_a, conj_b, alpha, beta);
+}
+
+extern "C" int tenet_plan_destroy(tenet_plan plan) {
+  if (!plan) return TENET_OK;
+  if (plan->cached) return TENET_OK;  // owned by the context cache
+  (void)hipSetDevice(plan->ctx->device);
+  if (plan->tables) (void)hipFree(plan->tables);
+  delete plan;
+  return TENET_OK;
+}
+
+static int cached_plan(tenet_ctx ctx, hipStream_t s, int dtype, int prec, const tenet_tdesc* a, const tenet_tdesc* b,
+                       const tenet_tdesc* c, tenet_plan* out) {
+  const std::string key = plan_key(dtype, prec, a, b, c);
+  {
+    std::lock_guard<std::mutex> g(ctx->mu);
+    auto it = ctx->plans.find(key);
+    if (it != ctx->plans.end()) {
+      *out = it->second;
+      return TENET_OK;
+    }
+  }
+  if (capturing(s)) {
+    set_error("contraction plan not cached: run the sequence once before graph capture");
+    return TENET_ERR_UNSUPPORTED;
+  }
+  auto* p = new tenet_plan_s();
+  int st = build_plan(ctx, dtype, prec, a, b, c, p);
+  if (st != TENET_OK) {
+    if (p->tables) (void)hipFree(p->tables);
+    delete p;
+    return st;
+  }
+  p->cached = true;
+  std::lock_guard<std::mutex> g(ctx->mu);
+  ctx->plans[key] = p;
+  ctx->plan_lru.push_back(key);
+  if (ctx->plan_lru.size() > 8192) {
+    // evict the oldest entry; plans are only freed after the device drains
+    (void)hipDeviceSynchronize();
+    const std::string old = ctx->plan_lru.front();
+    ctx->plan_lru.pop_front();
+    auto it = ctx->plans.find(old);
+    if (it != ctx->plans.end()) {
+      if (it->second->tables) (void)hipFree(it->second->tables);
+      delete it->second;
+      ctx->plans.erase(it);
+    }
+  }
+  *out = p;
+  return TENET_OK;
+}
+
+extern "C" int tenet_einsum(tenet_ctx ctx, tenet_stream stream, int dtype, int prec, const tenet_tdesc* a,
+                            const void* A, const tenet_tdesc* b, const void* B, const tenet_tdesc* c, void* C,
+                            int conj_a, int conj_b) {
+  TRY(use_device(ctx));
+  CHECK_ARG(a && b && c, "null descriptor");
+  TRY(valid_dtype_prec(dtype, prec));
+  hipStream_t s = pick(ctx, stream);
+  tenet_plan p = nullptr;
+  TRY(cached_plan(ctx, s, dtype, prec, a, b, c, &p));
+  return exec_plan(p, s, A, B, C, conj_a, conj_b, nullptr, nullptr);
+}
+
+// column-major contiguous descriptor helper
+static tenet_tdesc mkdesc(std::initializer_list<int64_t> ext, std::initializer_list<int64_t> lab) {
+  tenet_tdesc d{};
+  d.rank = (int32_t)ext.size();
+  int i = 0;
+  int64_t st = 1;
+  auto li = lab.begin();
+  for (int64_t e : ext) {
+    d.extent[i] = e;
+    d.stride[i] = st;
+    d.label[i] = *li++;
+    st *= e;
+    ++i;
+  }
+  return d;
+}
+
+static int einsum_internal(tenet_ctx ctx, hipStream_t s, int dtype, int prec, const tenet_tdesc& a, const void* A,
+                           const tenet_tdesc& b, const void* B, const tenet_tdesc& c, void* C, int conj_a,
+                           int conj_b) {
+  tenet_plan p = nullptr;
+  TRY(cached_plan(ctx, s, dtype, prec, &a, &b, &c, &p));
+  return exec_plan(p, s, A, B, C, conj_a, conj_b, nullptr, nullptr);
+}
+
+// ---------------------------------------------------------------------------
+// QR
+// ---------------------------------------------------------------------------
+static size_t qr_ws_bytes(int64_t m, int64_t n) {
+  const int64_t k = std::min(m, n);
+  return align_up((size_t)m * n * 16 + (size_t)m * k * 16, 256) + align_up((size_t)k * 16, 256) + 256;
+}
+
+static int qr_run(tenet_ctx ctx, hipStream_t s, int dtype, int flags, int64_t m, int64_t n, const void* A,
+                  int64_t lda, void* Q, int64_t ldq, void* R, int64_t ldr, void* wsp) {
+  (void)ctx;
+  Bump b(wsp);
+  const int64_t k = std::min(m, n);
+  double* W = (double*)b.take((size_t)m * n * 16 + (size_t)m * k * 16);
+  double* tau = (double*)b.take((size_t)k * 16);
+  TRY(launch_pack_c128(A, dtype, m, n, lda, (flags & TENET_QR_IN_ADJ) ? 1 : 0, W, s));
+  TRY(launch_qr_factor(W, tau, m, n, 1, s));
+  TRY(launch_qr_extract(W, tau, m, n, 1, Q, ldq, R, ldr, dtype, flags & (TENET_QR_Q_ADJ | TENET_QR_R_ADJ), s));
+  return TENET_OK;
+}
+
+extern "C" int tenet_qr_thin_ex(tenet_ctx ctx, tenet_stream stream, int dtype, int flags, int64_t m, int64_t n,
+                                const void* A, int64_t lda, void* Q, int64_t ldq, void* R, int64_t ldr) {
+  TRY(use_device(ctx));
+  TRY(valid_dtype_prec(dtype, 0));
+  CHECK_ARG(m > 0 && n > 0, "empty matrix");
+  CHECK_ARG((flags & ~7) == 0, "unknown QR flags");
+  const int64_t k = std::min(m, n);
+  const bool in_adj = flags & TENET_QR_IN_ADJ, q_adj = flags & TENET_QR_Q_ADJ, r_adj = flags & TENET_QR_R_ADJ;
+  CHECK_ARG(lda >= (in_adj ? n : m), "lda too small");
+  CHECK_ARG(ldq >= (q_adj ? k : m), "ldq too small");
+  CHECK_ARG(ldr >= (r_adj ? n : k), "ldr too small");
+  const size_t es = ELEM[dtype];
+  TRY(check_span(ctx, A, (in_adj ? ((size_t)(m - 1) * lda + n) : ((size_t)(n - 1) * lda + m)) * es, "A"));
+  TRY(check_span(ctx, Q, (q_adj ? ((size_t)(m - 1) * ldq + k) : ((size_t)(k - 1) * ldq + m)) * es, "Q"));
+  TRY(check_span(ctx, R, (r_adj ? ((size_t)(k - 1) * ldr + n) : ((size_t)(n - 1) * ldr + k)) * es, "R"));
+  hipStream_t s = pick(ctx, stream);
+  void* w = nullptr;
+  TRY(ws_get(ctx, s, qr_ws_bytes(m, n), &w));
+  return qr_run(ctx, s, dtype, flags, m, n, A, lda, Q, ldq, R, ldr, w);
+}
+
+extern "C" int tenet_qr_thin(tenet_ctx ctx, tenet_stream stream, int dtype, int64_t m, int64_t n, const void* A,
+                             int64_t lda, void* Q, int64_t ldq, void* R, int64_t ldr) {
+  return tenet_qr_thin_ex(ctx, stream, dtype, 0, m, n, A, lda, Q, ldq, R, ldr);
+}
+
+// ---------------------------------------------------------------------------
+// SVD (one-sided block Jacobi)
+// ---------------------------------------------------------------------------
+struct SvdShape {
+  int64_t nr, ncols, b, nblk, ncp, q, npairs;
+  bool transposed;
+};
+
+static SvdShape svd_shape(int64_t m, int64_t n, bool allow_split) {
+  SvdShape sh{};
+  sh.transposed = m < n;
+  sh.nr = std::max(m, n);
+  sh.ncols = std::min(m, n);
+  if (sh.ncols <= 32) {
+    sh.b = std::max<int64_t>(1, (sh.ncols + 1) / 2);
+    sh.nblk = 2;
+  } else {
+    sh.b = 16;
+    sh.nblk = (sh.ncols + 15) / 16;
+    if (sh.nblk % 2) sh.nblk += 1;
+  }
+  sh.ncp = sh.nblk * sh.b;
+  sh.npairs = sh.nblk / 2;
+  sh.q = 1;
+  if (allow_split) sh.q = std::min<int64_t>(16, std::max<int64_t>(1, sh.nr / 128));
+  return sh;
+}
+
+static size_t svd_ws_bytes(const SvdShape& sh) {
+  const int64_t n2 = 2 * sh.b;
+  size_t t = 0;
+  t += align_up((size_t)sh.nr * sh.ncp * 16, 256);
+  t += align_up((size_t)sh.ncp * sh.ncp * 16, 256);
+  t += align_up((size_t)sh.npairs * sh.q * n2 * n2 * 16, 256);
+  t += align_up((size_t)sh.ncp * 8, 256);
+  t += align_up((size_t)sh.ncp * 4, 256);
+  t += 256;  // flags
+  t += 256;  // k / disc fallback
+  return t + 256;
+}
+
+static double svd_tol(int64_t nr) { return 1e-14 * std::max(1.0, std::sqrt((double)nr / 16.0)); }
+
+static int svd_run(tenet_ctx ctx, hipStream_t s, int dtype, int64_t m, int64_t n, const void* A, int64_t lda, void* U,
+                   int64_t ldu, double* S, void* Vh, int64_t ldvh, int64_t maxdim, double thr, int64_t* k_dev,
+                   double* disc_dev, void* wsp, bool allow_split) {
+  const SvdShape sh = svd_shape(m, n, allow_split);
+  Bump bm(wsp);
+  SvdProblem P{};
+  P.X = (double*)bm.take((size_t)sh.nr * sh.ncp * 16);
+  P.V = (double*)bm.take((size_t)sh.ncp * sh.ncp * 16);
+  P.Gpart = (double*)bm.take((size_t)sh.npairs * sh.q * (2 * sh.b) * (2 * sh.b) * 16);
+  P.sigma = (double*)bm.take((size_t)sh.ncp * 8);
+  P.perm = (int32_t*)bm.take((size_t)sh.ncp * 4);
+  P.flag = (int32_t*)bm.take(256);
+  int64_t* kfall = (int64_t*)bm.take(256);
+  P.nr = (int32_t)sh.nr;
+  P.ncols = (int32_t)sh.ncols;
+  P.ncp = (int32_t)sh.ncp;
+  P.b = (int32_t)sh.b;
+  P.nblk = (int32_t)sh.nblk;
+  P.q = (int32_t)sh.q;
+  P.m = (int32_t)m;
+  P.n = (int32_t)n;
+  P.transposed = sh.transposed ? 1 : 0;
+  P.A = A;
+  P.lda = lda;
+  P.U = U;
+  P.ldu = ldu;
+  P.Vh = Vh;
+  P.ldvh = ldvh;
+  P.S = S;
+  P.maxdim = maxdim;
+  P.threshold = thr;
+  P.k_dev = k_dev ? k_dev : kfall;
+  P.disc_dev = disc_dev ? disc_dev : (double*)(kfall + 1);
+  P.tol = svd_tol(sh.nr);
+  SvdBatch B{};
+  B.arr = nullptr;
+  B.one = P;
+  TRY(launch_svd_init(B, 1, sh.nr * sh.ncp + sh.ncp * sh.ncp, dtype, s));
+  const bool sync_ok = !capturing(s);
+  const int max_sweeps = sync_ok ? 40 : 14;
+  bool converged = false;
+  for (int sw = 0; sw < max_sweeps && !converged; ++sw) {
+    for (int r = 0; r < sh.nblk - 1; ++r) {
+      if (sh.q == 1) {
+        TRY(launch_svd_round(B, 1, (int)sh.npairs, r, 2, s));
+      } else {
+        TRY(launch_svd_round(B, 1, (int)(sh.npairs * sh.q), r, 0, s));
+        TRY(launch_svd_round(B, 1, (int)(sh.npairs * sh.q), r, 1, s));
+      }
+    }
+    TRY(launch_svd_sweep_end(B, 1, s));
+    if (sync_ok && sw >= 1) {
+      TENET_HIP_CHECK(hipMemcpyAsync(ctx->host_flag, P.flag, 16, hipMemcpyDeviceToHost, s));
+      TENET_HIP_CHECK(hipStreamSynchronize(s));
+      converged = ctx->host_flag[1] != 0;
+    }
+  }
+  TRY(launch_svd_finalize(B, 1, (int)sh.ncp, (int64_t)(m + n) * sh.ncols, dtype, s));
+  if (sync_ok && !converged) {
+    set_error("Jacobi SVD did not converge within the sweep limit");
+    return TENET_ERR_NOCONV;
+  }
+  return TENET_OK;
+}
+
+extern "C" int tenet_svd_thin(tenet_ctx ctx, tenet_stream stream, int dtype, int64_t m, int64_t n, const void* A,
+                              int64_t lda, void* U, int64_t ldu, double* S, void* Vh, int64_t ldvh, int64_t maxdim,
+                              double threshold, int64_t* k_dev, double* discarded_dev) {
+  TRY(use_device(ctx));
+  TRY(valid_dtype_prec(dtype, 0));
+  CHECK_ARG(m > 0 && n > 0, "empty matrix");
+  const int64_t k = std::min(m, n);
+  CHECK_ARG(lda >= m && ldu >= m && ldvh >= k, "leading dimension too small");
+  CHECK_ARG(k <= 4096, "min(m, n) above 4096 is not supported");
+  const size_t es = ELEM[dtype];
+  TRY(check_span(ctx, A, ((size_t)(n - 1) * lda + m) * es, "A"));
+  TRY(check_span(ctx, U, ((size_t)(k - 1) * ldu + m) * es, "U"));
+  TRY(check_span(ctx, Vh, ((size_t)(n - 1) * ldvh + k) * es, "Vh"));
+  if (S) TRY(check_span(ctx, S, (size_t)k * 8, "S"));
+  hipStream_t s = pick(ctx, stream);
+  const SvdShape sh = svd_shape(m, n, true);
+  void* w = nullptr;
+  TRY(ws_get(ctx, s, svd_ws_bytes(sh), &w));
+  return svd_run(ctx, s, dtype, m, n, A, lda, U, ldu, S, Vh, ldvh, maxdim, threshold, k_dev, discarded_dev, w, true);
+}
+
+// ---------------------------------------------------------------------------
+// fused gate update
+// ---------------------------------------------------------------------------
+extern "C" int tenet_simple_update(tenet_ctx ctx, tenet_stream stream, int dtype, int prec, int64_t chil, int64_t d1,
+                                   int64_t chi, int64_t d2, int64_t chir, const void* A, const void* B, const void* G,
+                                   void* Aout, void* Bout, double* S_out, int64_t maxdim, double threshold, int absorb,
+                                   int64_t* k_dev, double* disc_dev, int64_t* k_host) {
+  TRY(use_device(ctx));
+  TRY(valid_dtype_prec(dtype, prec));
+  CHECK_ARG(chil > 0 && d1 > 0 && chi > 0 && d2 > 0 && chir > 0, "non-positive extent");
+  CHECK_ARG(absorb >= TENET_ABSORB_NONE && absorb <= TENET_ABSORB_RIGHT, "bad absorb mode");
+  hipStream_t s = pick(ctx, stream);
+  const size_t es = ELEM[dtype];
+  const int64_t m = chil * d1, n = d2 * chir, kmax = std::min(m, n);
+  CHECK_ARG(kmax <= 4096, "bond matrix too large");
+  TRY(check_span(ctx, A, (size_t)chil * d1 * chi * es, "A"));
+  TRY(check_span(ctx, B, (size_t)chi * d2 * chir * es, "B"));
+  TRY(check_span(ctx, G, (size_t)d1 * d2 * d1 * d2 * es, "G"));
+  TRY(check_span(ctx, Aout, (size_t)m * kmax * es, "Aout"));
+  TRY(check_span(ctx, Bout, (size_t)kmax * n * es, "Bout"));
+  if (S_out) TRY(check_span(ctx, S_out, (size_t)kmax * 8, "S_out"));
+  const SvdShape sh = svd_shape(m, n, true);
+  const size_t theta_b = align_up((size_t)m * n * es, 256);
+  const size_t need = 2 * theta_b + align_up((size_t)m * kmax * es, 256) + align_up((size_t)kmax * n * es, 256) +
+                      align_up((size_t)kmax * 8, 256) + 512 + svd_ws_bytes(sh);
+  void* w = nullptr;
+  TRY(ws_get(ctx, s, need, &w));
+  Bump bm(w);
+  void* theta = bm.take(theta_b);
+  void* theta2 = bm.take(theta_b);
+  void* U = bm.take((size_t)m * kmax * es);
+  void* Vh = bm.take((size_t)kmax * n * es);
+  double* S = S_out ? S_out : (double*)bm.take((size_t)kmax * 8);
+  int64_t* kd = (int64_t*)bm.take(256);
+  int64_t* kuse = k_dev ? k_dev : kd;
+  double* duse = disc_dev ? disc_dev : (double*)(kd + 1);
+  void* svd_ws = bm.take(svd_ws_bytes(sh));
+  // theta(l, ia, ib, r) = A(l, ia, c) B(c, ib, r)
+  const tenet_tdesc dA = mkdesc({chil, d1, chi}, {1, 2, 3});
+  const tenet_tdesc dB = mkdesc({chi, d2, chir}, {3, 4, 5});
+  const tenet_tdesc dT = mkdesc({chil, d1, d2, chir}, {1, 2, 4, 5});
+  TRY(einsum_internal(ctx, s, dtype, prec, dA, A, dB, B, dT, theta, 0, 0));
+  // theta2(l, oa, ob, r) = theta(l, ia, ib, r) G(oa, ob, ia, ib)
+  const tenet_tdesc dG = mkdesc({d1, d2, d1, d2}, {6, 7, 2, 4});
+  const tenet_tdesc dT2 = mkdesc({chil, d1, d2, chir}, {1, 6, 7, 5});
+  const int gprec = (prec == TENET_PREC_SPLIT_BF16 || prec == TENET_PREC_SPLIT_FP16) ? TENET_PREC_NATIVE : prec;
+  TRY(einsum_internal(ctx, s, dtype, gprec, dT, theta, dG, G, dT2, theta2, 0, 0));
+  int st = svd_run(ctx, s, dtype, m, n, theta2, m, U, m, S, Vh, kmax, maxdim, threshold, kuse, duse, svd_ws, true);
+  if (st != TENET_OK && st != TENET_ERR_NOCONV) return st;
+  TRY(launch_absorb(U, m, Vh, kmax, n, S, kuse, absorb, Aout, Bout, kmax, dtype, s));
+  if (k_host) {
+    TENET_HIP_CHECK(hipMemcpyAsync(ctx->host_k, kuse, 8, hipMemcpyDeviceToHost, s));
+    TENET_HIP_CHECK(hipStreamSynchronize(s));
+    *k_host = ctx->host_k[0];
+  }
+  return st;
+}
+
+// ---------------------------------------------------------------------------
+// transfer-matrix sweeps
+// ---------------------------------------------------------------------------
+extern "C" int tenet_overlap(tenet_ctx ctx, tenet_stream stream, int dtype, int prec, int64_t n, int64_t d,
+                             const int64_t* chi_a, const void* const* a_sites, const int64_t* chi_b,
+                             const void* const* b_sites, double* out) {
+  TRY(use_device(ctx));
+  TRY(valid_dtype_prec(dtype, prec));
+  CHECK_ARG(n >= 1 && d >= 1 && chi_a && chi_b && a_sites && b_sites && out, "bad argument");
+  CHECK_ARG(chi_a[0] == 1 && chi_a[n] == 1 && chi_b[0] == 1 && chi_b[n] == 1, "open boundary bonds must be 1");
+  hipStream_t s = pick(ctx, stream);
+  const size_t es = ELEM[dtype];
+  int64_t max_env = 1, max_t = 1;
+  for (int64_t i = 0; i <= n; ++i) max_env = std::max(max_env, chi_a[i] * chi_b[i]);
+  for (int64_t i = 0; i < n; ++i) max_t = std::max(max_t, chi_b[i] * d * chi_a[i + 1]);
+  for (int64_t i = 0; i < n; ++i) {
+    TRY(check_span(ctx, a_sites[i], (size_t)chi_a[i] * d * chi_a[i + 1] * es, "a site"));
+    TRY(check_span(ctx, b_sites[i], (size_t)chi_b[i] * d * chi_b[i + 1] * es, "b site"));
+  }
+  void* w = nullptr;
+  TRY(ws_get(ctx, s, 2 * align_up((size_t)max_env * es, 256) + align_up((size_t)max_t * es, 256) + 256, &w));
+  Bump bm(w);
+  void* env[2] = {bm.take((size_t)max_env * es), bm.take((size_t)max_env * es)};
+  void* T = bm.take((size_t)max_t * es);
+  TRY(launch_fill_zero(out, 3 * sizeof(double), s));
+  int cur = 0;
+  // site 0: env(ra, rb) = sum_s A(1, s, ra) conj(B(1, s, rb))
+  {
+    const tenet_tdesc dA = mkdesc({1, d, chi_a[1]}, {100, 3, 4});
+    const tenet_tdesc dB = mkdesc({1, d, chi_b[1]}, {100, 3, 5});
+    const tenet_tdesc dE = mkdesc({1, chi_a[1], chi_b[1]}, {100, 4, 5});
+    TRY(einsum_internal(ctx, s, dtype, prec, dA, a_sites[0], dB, b_sites[0], dE, env[cur], 0, 1));
+    TRY(launch_rescale_log(env[cur], chi_a[1] * chi_b[1], out, dtype, s));
+  }
+  for (int64_t i = 1; i < n; ++i) {
+    const int64_t la = chi_a[i], lb = chi_b[i], ra = chi_a[i + 1], rb = chi_b[i + 1];
+    // T(lb, s, ra) = env(la, lb) A(la, s, ra)
+    const tenet_tdesc dE = mkdesc({la, lb}, {1, 2});
+    const tenet_tdesc dA = mkdesc({la, d, ra}, {1, 3, 4});
+    const tenet_tdesc dT = mkdesc({lb, d, ra}, {2, 3, 4});
+    TRY(einsum_internal(ctx, s, dtype, prec, dE, env[cur], dA, a_sites[i], dT, T, 0, 0));
+    // env'(ra, rb) = T(lb, s, ra) conj(B(lb, s, rb))
+    const tenet_tdesc dB = mkdesc({lb, d, rb}, {2, 3, 5});
+    const tenet_tdesc dE2 = mkdesc({ra, rb}, {4, 5});
+    TRY(einsum_internal(ctx, s, dtype, prec, dT, T, dB, b_sites[i], dE2, env[cur ^ 1], 0, 1));
+    cur ^= 1;
+    TRY(launch_rescale_log(env[cur], ra * rb, out, dtype, s));
+  }
+  return launch_write_scalar_from(env[cur], out, dtype, s);
+}
+
+extern "C" int tenet_amplitudes(tenet_ctx ctx, tenet_stream stream, int dtype, int64_t nreq, const int64_t* n_dev,
+                                const uint64_t* site_ptrs_dev, const uint64_t* chis_dev, const uint64_t* bits_dev,
+                                int64_t d, double* out) {
+  TRY(use_device(ctx));
+  TRY(valid_dtype_prec(dtype, 0));
+  CHECK_ARG(nreq >= 0 && d >= 1, "bad argument");
+  if (nreq == 0) return TENET_OK;
+  CHECK_ARG(n_dev && site_ptrs_dev && chis_dev && bits_dev && out, "null pointer");
+  return launch_amplitudes(nreq, n_dev, site_ptrs_dev, chis_dev, bits_dev, d, out, dtype, 2048, pick(ctx, stream));
+}
+
+extern "C" int tenet_expect_pauli(tenet_ctx ctx, tenet_stream stream, int dtype, int prec, int64_t start, int64_t w,
+                                  int64_t d, const int64_t* chi, const void* const* sites, const void* env_l,
+                                  const char* paulis, double* out) {
+  TRY(use_device(ctx));
+  TRY(valid_dtype_prec(dtype, prec));
+  CHECK_ARG(d == 2, "Pauli strings need d = 2");
+  CHECK_ARG(w >= 1 && start >= 0 && chi && sites && paulis && out, "bad argument");
+  for (int64_t j = 0; j < w; ++j) {
+    const char c = paulis[j];
+    CHECK_ARG(c == 'I' || c == 'X' || c == 'Y' || c == 'Z', "Pauli letters must be I, X, Y or Z");
+  }
+  hipStream_t s = pick(ctx, stream);
+  const size_t es = ELEM[dtype];
+  int64_t max_env = 1, max_site = 1;
+  for (int64_t j = 0; j <= w; ++j) max_env = std::max(max_env, chi[start + j] * chi[start + j]);
+  for (int64_t j = 0; j < w; ++j) max_site = std::max(max_site, chi[start + j] * d * chi[start + j + 1]);
+  void* wsp = nullptr;
+  TRY(ws_get(ctx, s, 2 * align_up((size_t)max_env * es, 256) + 2 * align_up((size_t)max_site * es, 256) + 256, &wsp));
+  Bump bm(wsp);
+  void* env[2] = {bm.take((size_t)max_env * es), bm.take((size_t)max_env * es)};
+  void* PA = bm.take((size_t)max_site * es);
+  void* T = bm.take((size_t)max_site * es);
+  TRY(launch_fill_zero(out, 3 * sizeof(double), s));
+  int cur = 0;
+  const int64_t c0 = chi[start];
+  if (env_l) {
+    TENET_HIP_CHECK(hipMemcpyAsync(env[cur], env_l, (size_t)c0 * c0 * es, hipMemcpyDeviceToDevice, s));
+  } else {
+    TRY(launch_set_identity(env[cur], c0, dtype, s));
+  }
+  for (int64_t j = 0; j < w; ++j) {
+    const int64_t i = start + j, l = chi[i], r = chi[i + 1];
+    TRY(launch_apply_pauli(sites[i], l, r, paulis[j], PA, dtype, s));
+    // T(a, o, r') = env(a, a') PA(a', o, r')
+    const tenet_tdesc dE = mkdesc({l, l}, {1, 2});
+    const tenet_tdesc dP = mkdesc({l, d, r}, {2, 3, 4});
+    const tenet_tdesc dT = mkdesc({l, d, r}, {1, 3, 4});
+    TRY(einsum_internal(ctx, s, dtype, prec, dE, env[cur], dP, PA, dT, T, 0, 0));
+    // env'(r, r') = conj(A(a, o, r)) T(a, o, r')
+    const tenet_tdesc dA = mkdesc({l, d, r}, {1, 3, 5});
+    const tenet_tdesc dE2 = mkdesc({r, r}, {5, 4});
+    TRY(einsum_internal(ctx, s, dtype, prec, dA, sites[i], dT, T, dE2, env[cur ^ 1], 1, 0));
+    cur ^= 1;
+    TRY(launch_rescale_log(env[cur], r * r, out, dtype, s));
+  }
+  return launch_trace_diag(env[cur], chi[start + w], out, dtype, s);
+}
+
+// ---------------------------------------------------------------------------
+// grouped launches
+// ---------------------------------------------------------------------------
+extern "C" int tenet_grouped_einsum(tenet_ctx ctx, tenet_stream stream, int dtype, int prec, int64_t count,
+                                    const tenet_tdesc* descs, void* const* ptrs) {
+  TRY(use_device(ctx));
+  TRY(valid_dtype_prec(dtype, prec));
+  CHECK_ARG(count >= 0 && (count == 0 || (descs && ptrs)), "bad argument");
+  if (count == 0) return TENET_OK;
+  CHECK_ARG(prec == TENET_PREC_NATIVE || (prec == TENET_PREC_FP32 && dtype == TENET_C64),
+            "grouped contractions run in the dtype's native precision");
+  hipStream_t s = pick(ctx, stream);
+  std::vector<GroupedProblem> probs((size_t)count);
+  int64_t tiles = 0;
+  for (int64_t i = 0; i < count; ++i) {
+    tenet_plan p = nullptr;
+    TRY(cached_plan(ctx, s, dtype, prec, &descs[3 * i], &descs[3 * i + 1], &descs[3 * i + 2], &p));
+    GroupedProblem& g = probs[(size_t)i];
+    g.A = ptrs[3 * i];
+    g.B = ptrs[3 * i + 1];
+    g.C = ptrs[3 * i + 2];
+    g.t = p->t;
+    g.M = p->M;
+    g.N = p->N;
+    g.K = p->K;
+    g.batch = p->batch;
+    g.a_mfast = p->a_mfast;
+    g.b_nfast = p->b_nfast;
+    g.tiles_m = (p->M + 63) / 64;
+    g.tiles_n = (p->N + 63) / 64;
+    g.tile_begin = tiles;
+    tiles += g.tiles_m * g.tiles_n * p->batch;
+  }
+  void* w = nullptr;
+  TRY(ws_get(ctx, s, probs.size() * sizeof(GroupedProblem) + 256, &w));
+  TENET_HIP_CHECK(hipMemcpyAsync(w, probs.data(), probs.size() * sizeof(GroupedProblem), hipMemcpyHostToDevice, s));
+  TRY(launch_grouped_gemm((const GroupedProblem*)w, count, tiles, dtype, s));
+  // the host staging vector must outlive the copy
+  TENET_HIP_CHECK(hipStreamSynchronize(s));
+  return TENET_OK;
+}
+
+extern "C" int tenet_grouped_svd(tenet_ctx ctx, tenet_stream stream, int dtype, int64_t count, const int64_t* ms,
+                                 const int64_t* ns, const void* const* As, void* const* Us, double* const* Ss,
+                                 void* const* Vhs, const int64_t* maxdims, double threshold, int64_t* k_dev,
+                                 double* disc_dev) {
+  TRY(use_device(ctx));
+  TRY(valid_dtype_prec(dtype, 0));
+  CHECK_ARG(count >= 0, "bad count");
+  if (count == 0) return TENET_OK;
+  CHECK_ARG(ms && ns && As && Us && Vhs, "null argument");
+  hipStream_t s = pick(ctx, stream);
+  std::vector<SvdShape> shapes((size_t)count);
+  size_t total = align_up((size_t)count * sizeof(SvdProblem), 256) + align_up((size_t)count * 16, 256);
+  int64_t max_rounds = 0, max_pairs = 0, max_ncp = 0, max_elems = 0, max_out = 0;
+  for (int64_t i = 0; i < count; ++i) {
+    CHECK_ARG(ms[i] > 0 && ns[i] > 0 && std::min(ms[i], ns[i]) <= 4096, "bad matrix shape");
+    shapes[(size_t)i] = svd_shape(ms[i], ns[i], false);
+    const SvdShape& sh = shapes[(size_t)i];
+    total += svd_ws_bytes(sh);
+    max_rounds = std::max(max_rounds, sh.nblk - 1);
+    max_pairs = std::max(max_pairs, sh.npairs);
+    max_ncp = std::max(max_ncp, sh.ncp);
+    max_elems = std::max(max_elems, sh.nr * sh.ncp + sh.ncp * sh.ncp);
+    max_out = std::max(max_out, (ms[i] + ns[i]) * sh.ncols);
+  }
+  void* w = nullptr;
+  TRY(ws_get(ctx, s, total, &w));
+  Bump bm(w);
+  SvdProblem* dprobs = (SvdProblem*)bm.take((size_t)count * sizeof(SvdProblem));
+  int32_t* flags_base = nullptr;
+  std::vector<SvdProblem> probs((size_t)count);
+  for (int64_t i = 0; i < count; ++i) {
+    const SvdShape& sh = shapes[(size_t)i];
+    SvdProblem& P = probs[(size_t)i];
+    P.X = (double*)bm.take((size_t)sh.nr * sh.ncp * 16);
+    P.V = (double*)bm.take((size_t)sh.ncp * sh.ncp * 16);
+    P.Gpart = (double*)bm.take((size_t)sh.npairs * sh.q * (2 * sh.b) * (2 * sh.b) * 16);
+    P.sigma = (double*)bm.take((size_t)sh.ncp * 8);
+    P.perm = (int32_t*)bm.take((size_t)sh.ncp * 4);
+    P.nr = (int32_t)sh.nr;
+    P.ncols = (int32_t)sh.ncols;
+    P.ncp = (int32_t)sh.ncp;
+    P.b = (int32_t)sh.b;
+    P.nblk = (int32_t)sh.nblk;
+    P.q = 1;
+    P.m = (int32_t)ms[i];
+    P.n = (int32_t)ns[i];
+    P.transposed = sh.transposed ? 1 : 0;
+    P.A = As[i];
+    P.lda = ms[i];
+    P.U = Us[i];
+    P.ldu = ms[i];
+    P.Vh = Vhs[i];
+    P.ldvh = sh.ncols;
+    P.S = Ss ? Ss[i] : nullptr;
+    P.maxdim = maxdims ? maxdims[i] : 0;
+    P.threshold = threshold;
+    P.k_dev = k_dev ? k_dev + i : nullptr;
+    P.disc_dev = disc_dev ? disc_dev + i : nullptr;
+    P.tol = svd_tol(sh.nr);
+  }
+  flags_base = (int32_t*)bm.take((size_t)count * 16);
+  for (int64_t i = 0; i < count; ++i) probs[(size_t)i].flag = flags_base + 4 * i;
+  TENET_HIP_CHECK(hipMemcpyAsync(dprobs, probs.data(), probs.size() * sizeof(SvdProblem), hipMemcpyHostToDevice, s));
+  SvdBatch B{};
+  B.arr = dprobs;
+  TRY(launch_svd_init(B, count, max_elems, dtype, s));
+  std::vector<int32_t> hflags((size_t)count * 4);
+  bool all = false;
+  for (int sw = 0; sw < 40 && !all; ++sw) {
+    for (int r = 0; r < max_rounds; ++r) TRY(launch_svd_round(B, count, (int)max_pairs, r, 2, s));
+    TRY(launch_svd_sweep_end(B, count, s));
+    if (sw >= 1) {
+      TENET_HIP_CHECK(hipMemcpyAsync(hflags.data(), flags_base, (size_t)count * 16, hipMemcpyDeviceToHost, s));
+      TENET_HIP_CHECK(hipStreamSynchronize(s));
+      all = true;
+      for (int64_t i = 0; i < count; ++i) all = all && hflags[(size_t)(4 * i + 1)] != 0;
+    }
+  }
+  TRY(launch_svd_finalize(B, count, (int)max_ncp, max_out, dtype, s));
+  TENET_HIP_CHECK(hipStreamSynchronize(s));
+  if (!all) {
+    set_error("grouped Jacobi SVD: some problems did not converge");
+    return TENET_ERR_NOCONV;
+  }
+  return TENET_OK;
+}
+
+// ---------------------------------------------------------------------------
+// graphs and timers
+// ---------------------------------------------------------------------------
+extern "C" int tenet_graph_begin(tenet_ctx ctx, tenet_stream stream) {
+  TRY(use_device(ctx));
+  CHECK_ARG(stream, "capture needs an explicit stream");
+  TENET_HIP_CHECK(hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeRelaxed));
+  return TENET_OK;
+}
+extern "C" int tenet_graph_end(tenet_ctx ctx, tenet_stream stream, tenet_graph* graph) {
+  TRY(use_device(ctx));
+  CHECK_ARG(stream && graph, "bad argument");
+  auto* g = new tenet_graph_s();
+  g->device = ctx->device;
+  hipError_t e = hipStreamEndCapture((hipStream_t)stream, &g->graph);
+  if (e != hipSuccess) {
+    delete g;
+    set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    return TENET_ERR_HIP;
+  }
+  e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(g->graph);
+    delete g;
+    set_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    return TENET_ERR_HIP;
+  }
+  *graph = g;
+  return TENET_OK;
+}
+extern "C" int tenet_graph_launch(tenet_graph graph, tenet_stream stream) {
+  CHECK_ARG(graph && stream, "bad argument");
+  TENET_HIP_CHECK(hipSetDevice(graph->device));
+  TENET_HIP_CHECK(hipGraphLaunch(graph->exec, (hipStream_t)stream));
+  return TENET_OK;
+}
+extern "C" int tenet_graph_destroy(tenet_graph graph) {
+  if (!graph) return TENET_OK;
+  (void)hipSetDevice(graph->device);
+  (void)hipGraphExecDestroy(graph->exec);
+  (void)hipGraphDestroy(graph->graph);
+  delete graph;
+  return TENET_OK;
+}
+
+extern "C" int tenet_timer_start(tenet_ctx ctx, tenet_stream stream) {
+  TRY(use_device(ctx));
+  TENET_HIP_CHECK(hipEventRecord(ctx->ev0, pick(ctx, stream)));
+  return TENET_OK;
+}
+extern "C" int tenet_timer_stop(tenet_ctx ctx, tenet_stream stream, float* ms) {
+  TRY(use_device(ctx));
+  CHECK_ARG(ms, "null output");
+  TENET_HIP_CHECK(hipEventRecord(ctx->ev1, pick(ctx, stream)));
+  TENET_HIP_CHECK(hipEventSynchronize(ctx->ev1));
+  TENET_HIP_CHECK(hipEventElapsedTime(ms, ctx->ev0, ctx->ev1));
+  return TENET_OK;
+}

@@ -1,0 +1,157 @@
+"""Device context and device-resident column-major arrays (the Julia layout
+the C-ABI expects).  Memory comes from the context's caching pool."""
+from __future__ import annotations
+
+import ctypes as C
+import weakref
+
+import numpy as np
+
+from . import _lib as L
+
+DTYPES = {np.dtype(np.complex128): L.C128, np.dtype(np.complex64): L.C64}
+NP_OF = {L.C128: np.complex128, L.C64: np.complex64}
+
+
+class Context:
+    """One device (tenet_create); owns the pool, plan cache and scratch."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        L.check(L.lib.tenet_create(int(device), C.byref(h)), "tenet_create")
+        self.h = h
+        self.device = device
+        self._closed = False
+
+    def close(self):
+        if not self._closed:
+            L.lib.tenet_destroy(self.h)
+            self._closed = True
+
+    def sync(self, stream=None):
+        L.check(L.lib.tenet_stream_sync(self.h, stream), "sync")
+
+    def mem_info(self):
+        f, t, p = C.c_size_t(), C.c_size_t(), C.c_size_t()
+        L.check(L.lib.tenet_mem_info(self.h, C.byref(f), C.byref(t), C.byref(p)))
+        return f.value, t.value, p.value
+
+    def malloc(self, nbytes: int) -> int:
+        p = C.c_void_p()
+        L.check(L.lib.tenet_malloc(self.h, C.c_size_t(max(int(nbytes), 1)), C.byref(p)), "tenet_malloc")
+        return p.value
+
+    def free(self, ptr: int):
+        if ptr and not self._closed:
+            L.check(L.lib.tenet_free(self.h, C.c_void_p(ptr)), "tenet_free")
+
+    def stream(self):
+        s = C.c_void_p()
+        L.check(L.lib.tenet_stream_create(self.h, C.byref(s)), "stream_create")
+        return s
+
+    def timer_start(self, stream=None):
+        L.check(L.lib.tenet_timer_start(self.h, stream))
+
+    def timer_stop(self, stream=None) -> float:
+        ms = C.c_float()
+        L.check(L.lib.tenet_timer_stop(self.h, stream, C.byref(ms)))
+        return ms.value
+
+
+def _free(ctx_ref, ptr):
+    ctx = ctx_ref()
+    if ctx is not None and not ctx._closed:
+        L.lib.tenet_free(ctx.h, C.c_void_p(ptr))
+
+
+class DeviceArray:
+    """A column-major complex array in device memory."""
+
+    def __init__(self, ctx: Context, shape, dtype=np.complex128, ptr=None, owner=True):
+        self.ctx = ctx
+        self.shape = tuple(int(s) for s in shape)
+        self.dtype = np.dtype(dtype)
+        if self.dtype not in DTYPES:
+            raise TypeError("only complex128 / complex64 device arrays")
+        self.code = DTYPES[self.dtype]
+        self.size = int(np.prod(self.shape, dtype=np.int64)) if self.shape else 1
+        self.nbytes = self.size * self.dtype.itemsize
+        self.ptr = ptr if ptr is not None else ctx.malloc(self.nbytes)
+        self._fin = weakref.finalize(self, _free, weakref.ref(ctx), self.ptr) if owner else None
+
+    @property
+    def strides_el(self):
+        st, acc = [], 1
+        for s in self.shape:
+            st.append(acc)
+            acc *= s
+        return st
+
+    @staticmethod
+    def from_host(ctx: Context, arr, dtype=None):
+        a = np.asarray(arr)
+        dt = np.dtype(dtype) if dtype is not None else (a.dtype if a.dtype in DTYPES else np.dtype(np.complex128))
+        a = np.asfortranarray(a.astype(dt, copy=False))
+        d = DeviceArray(ctx, a.shape, dt)
+        if d.nbytes:
+            L.check(L.lib.tenet_memcpy_h2d(ctx.h, C.c_void_p(d.ptr), a.ctypes.data_as(C.c_void_p),
+                                           C.c_size_t(d.nbytes), None), "h2d")
+        return d
+
+    def to_host(self):
+        out = np.empty(self.shape, dtype=self.dtype, order="F")
+        if self.nbytes:
+            L.check(L.lib.tenet_memcpy_d2h(self.ctx.h, out.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr),
+                                           C.c_size_t(self.nbytes), None), "d2h")
+        return out
+
+    def zero_(self):
+        L.check(L.lib.tenet_memset_zero(self.ctx.h, C.c_void_p(self.ptr), C.c_size_t(self.nbytes), None))
+        return self
+
+    def copy(self):
+        d = DeviceArray(self.ctx, self.shape, self.dtype)
+        L.check(L.lib.tenet_memcpy_d2d(self.ctx.h, C.c_void_p(d.ptr), C.c_void_p(self.ptr),
+                                       C.c_size_t(self.nbytes), None), "d2d")
+        return d
+
+    def reshape(self, shape):
+        """Column-major reshape sharing the buffer (keeps this array alive)."""
+        if int(np.prod(shape, dtype=np.int64)) != self.size:
+            raise ValueError("reshape size mismatch")
+        v = DeviceArray(self.ctx, shape, self.dtype, ptr=self.ptr, owner=False)
+        v._base = self
+        return v
+
+    def view(self, shape):
+        """Column-major view of the first prod(shape) elements (keeps this array alive)."""
+        if int(np.prod(shape, dtype=np.int64)) > self.size:
+            raise ValueError("view larger than the buffer")
+        v = DeviceArray(self.ctx, shape, self.dtype, ptr=self.ptr, owner=False)
+        v._base = self
+        return v
+
+    def free(self):
+        if self._fin is not None and self._fin.alive:
+            self._fin()
+
+    def __repr__(self):
+        return f"DeviceArray(shape={self.shape}, dtype={self.dtype})"
+
+
+def tdesc(shape, labels, strides=None):
+    d = L.TDesc()
+    d.rank = len(shape)
+    if d.rank > L.MAX_RANK:
+        raise ValueError("rank above 8")
+    if strides is None:
+        strides, acc = [], 1
+        for s in shape:
+            strides.append(acc)
+            acc *= int(s)
+    for i, (e, st, lab) in enumerate(zip(shape, strides, labels)):
+        d.extent[i] = int(e)
+        d.stride[i] = int(st)
+        d.label[i] = int(lab)
+    return d

@@ -1,0 +1,445 @@
+"""Device-resident MPS with Tenet.jl's canonical-form semantics.
+
+Each site is a column-major (chi_l, d, chi_r) DeviceArray (chi_l = 1 on the
+first site, chi_r = 1 on the last; Tenet stores the edge sites as 2-way
+arrays, MPS.jl:63-69, and those size-1 legs are never treated as bonds).
+Sites are 0-based here; Tenet's site"i" is site i-1.
+
+Python names carry a trailing underscore where Tenet's carry a '!':
+  canonize_        canonize!            src/Operations/canonize.jl:121-255
+  canonize_bond_   canonize!(Bond)      canonize.jl:151-179, 221-246
+  compress_        compress!/truncate!  src/Operations/compress.jl:13-72
+  simple_update_   simple_update!(MPS)  src/Operations/simple_update.jl:82-92
+  evolve_          evolve!(tn, ::Tensor) src/Operations/evolve.jl:140-143
+  overlap          overlap(a, b) = <b|a>  src/Operations/overlap.jl:36-50
+  norm / normalize_  norm.jl:17-37 / normalize.jl:15-37
+  amplitudes       overlap(::ProductState, ::MPS), overlap.jl:22-32
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _lib as L
+from .device import DTYPES, Context, DeviceArray, tdesc
+
+
+def _gate_tensor(U4, d):
+    """4x4 gate with site a as the most significant digit, U4[d*oa+ob, d*ia+ib]
+    (test/unit/simple_update.jl:19-33) -> column-major G[oa, ob, ia, ib]."""
+    return np.asfortranarray(np.asarray(U4).reshape(d, d, d, d))
+
+
+class MPS:
+    def __init__(self, ctx: Context, sites, form=None, dtype=np.complex128):
+        self.ctx = ctx
+        self.sites = list(sites)
+        self.dtype = np.dtype(dtype)
+        self.code = DTYPES[self.dtype]
+        self.form = form if form is not None else ("mixed", 0, len(self.sites) - 1)  # MPS.jl:18
+        self.lam = None  # host Lambda of a BondCanonical MPS
+        self.prec = L.PREC_NATIVE
+
+    # ---- construction ---------------------------------------------------
+    @staticmethod
+    def from_host(ctx: Context, arrays_lor, dtype=np.complex128, form=None):
+        """From (chi_l, d, chi_r) host arrays."""
+        sites = [DeviceArray.from_host(ctx, np.asarray(a).reshape(np.shape(a)), dtype) for a in arrays_lor]
+        return MPS(ctx, sites, form, dtype)
+
+    @staticmethod
+    def from_arrays(ctx: Context, arrays, order=("l", "r", "o"), dtype=np.complex128):
+        """Tenet's ``MPS(arrays; order)`` (MPS.jl:50-94): edge arrays are 2-way."""
+        order = list(order)
+        n = len(arrays)
+        out = []
+        for i, arr in enumerate(arrays):
+            loc = [x for x in order if not ((i == 0 and x == "l") or (i == n - 1 and x == "r"))]
+            a = np.asarray(arr)
+            if i == 0:
+                a = a[None]
+                loc = ["l"] + loc
+            if i == n - 1:
+                a = a[..., None]
+                loc = loc + ["r"]
+            out.append(np.transpose(a, [loc.index("l"), loc.index("o"), loc.index("r")]))
+        return MPS.from_host(ctx, out, dtype)
+
+    @staticmethod
+    def product_state(ctx: Context, vectors, dtype=np.complex128):
+        """``convert(MPS, ProductState(...))`` (MPS.jl:167-201)."""
+        return MPS.from_host(ctx, [np.asarray(v).reshape(1, -1, 1) for v in vectors], dtype)
+
+    def to_host(self):
+        return [s.to_host() for s in self.sites]
+
+    def copy(self):
+        o = MPS(self.ctx, [s.copy() for s in self.sites], self.form, self.dtype)
+        o.lam = None if self.lam is None else self.lam.copy()
+        o.prec = self.prec
+        return o
+
+    @property
+    def n(self):
+        return len(self.sites)
+
+    def bondsizes(self):
+        return [s.shape[2] for s in self.sites[:-1]]
+
+    def chis(self):
+        return [1] + self.bondsizes() + [1]
+
+    # ---- primitive moves --------------------------------------------------
+    def _einsum(self, a, la, b, lb, c_shape, lc, conj_a=0, conj_b=0):
+        c = DeviceArray(self.ctx, c_shape, self.dtype)
+        da, db, dc = tdesc(a.shape, la), tdesc(b.shape, lb), tdesc(c_shape, lc)
+        L.check(L.lib.tenet_einsum(self.ctx.h, None, self.code, self.prec, C.byref(da), C.c_void_p(a.ptr),
+                                   C.byref(db), C.c_void_p(b.ptr), C.byref(dc), C.c_void_p(c.ptr), conj_a, conj_b),
+                "einsum")
+        return c
+
+    def _qr_right(self, i):
+        """generic_canonize_site!(site i, bond i-(i+1); :qr), canonize.jl:23-76."""
+        A, B = self.sites[i], self.sites[i + 1]
+        chil, d, chi = A.shape
+        if chi == 1:  # canonize.jl:38
+            return
+        m = chil * d
+        k = min(m, chi)
+        Q = DeviceArray(self.ctx, (chil, d, k), self.dtype)
+        R = DeviceArray(self.ctx, (k, chi), self.dtype)
+        L.check(L.lib.tenet_qr_thin(self.ctx.h, None, self.code, m, chi, C.c_void_p(A.ptr), m, C.c_void_p(Q.ptr), m,
+                                    C.c_void_p(R.ptr), k), "qr")
+        _, d2, chir = B.shape
+        self.sites[i + 1] = self._einsum(R, [1, 2], B, [2, 3, 4], (k, d2, chir), [1, 3, 4])
+        self.sites[i] = Q
+
+    def _qr_left(self, i):
+        """generic_canonize_site!(site i, bond (i-1)-i; :qr): QR of A^H, so
+        A = R^H Q^H with Q^H a right isometry."""
+        A, B = self.sites[i], self.sites[i - 1]
+        chi, d, chir = A.shape
+        if chi == 1:
+            return
+        m = d * chir  # rows of A^H
+        k = min(m, chi)
+        Qh = DeviceArray(self.ctx, (k, d, chir), self.dtype)
+        Rh = DeviceArray(self.ctx, (chi, k), self.dtype)
+        L.check(L.lib.tenet_qr_thin_ex(self.ctx.h, None, self.code, L.QR_IN_ADJ | L.QR_Q_ADJ | L.QR_R_ADJ, m, chi,
+                                       C.c_void_p(A.ptr), chi, C.c_void_p(Qh.ptr), k, C.c_void_p(Rh.ptr), chi),
+                "qr (adjoint)")
+        chil2, d2, _ = B.shape
+        self.sites[i - 1] = self._einsum(B, [1, 2, 3], Rh, [3, 4], (chil2, d2, k), [1, 2, 4])
+        self.sites[i] = Qh
+
+    def _svd(self, M: DeviceArray, m, n, maxdim=0, threshold=0.0):
+        kmax = min(m, n)
+        U = DeviceArray(self.ctx, (m, kmax), self.dtype)
+        Vh = DeviceArray(self.ctx, (kmax, n), self.dtype)
+        sdev = self.ctx.malloc(kmax * 8 + 16)
+        try:
+            st = L.lib.tenet_svd_thin(self.ctx.h, None, self.code, m, n, C.c_void_p(M.ptr), m, C.c_void_p(U.ptr), m,
+                                      C.c_void_p(sdev), C.c_void_p(Vh.ptr), kmax, int(maxdim), float(threshold),
+                                      C.c_void_p(sdev + kmax * 8), None)
+            L.check(st, "svd")
+            s = np.zeros(kmax)
+            kb = np.zeros(1, dtype=np.int64)
+            L.check(L.lib.tenet_memcpy_d2h(self.ctx.h, s.ctypes.data_as(C.c_void_p), C.c_void_p(sdev), kmax * 8, None))
+            L.check(L.lib.tenet_memcpy_d2h(self.ctx.h, kb.ctypes.data_as(C.c_void_p), C.c_void_p(sdev + kmax * 8), 8,
+                                           None))
+        finally:
+            self.ctx.free(sdev)
+        return U, s, Vh, int(kb[0])
+
+    def _bond_right(self, i):
+        """generic_bond_canonize_site! at site i across bond i-(i+1) (canonize.jl:78-116)."""
+        A, B = self.sites[i], self.sites[i + 1]
+        chil, d, chi = A.shape
+        if chi == 1:
+            self.lam = np.ones(1)
+            return
+        U, s, Vh, _ = self._svd(A, chil * d, chi)
+        k = len(s)
+        _, d2, chir = B.shape
+        self.sites[i + 1] = self._einsum(Vh, [1, 2], B, [2, 3, 4], (k, d2, chir), [1, 3, 4])
+        self.sites[i] = U.view((chil, d, k))
+        self.lam = s
+
+    def _bond_left(self, i):
+        """Bond-canonize at site i across bond (i-1)-i: SVD of A (chi x d chi_r)
+        = U s Vh; A <- Vh, A_{i-1} <- A_{i-1} U."""
+        A, B = self.sites[i], self.sites[i - 1]
+        chi, d, chir = A.shape
+        if chi == 1:
+            self.lam = np.ones(1)
+            return
+        U, s, Vh, _ = self._svd(A, chi, d * chir)
+        k = len(s)
+        chil2, d2, _ = B.shape
+        self.sites[i - 1] = self._einsum(B, [1, 2, 3], U, [3, 4], (chil2, d2, k), [1, 2, 4])
+        self.sites[i] = Vh.view((k, d, chir))
+        self.lam = s
+
+    def _scale_bond(self, site, vec, left):
+        """hadamard(tensor, s): scale the left (or right) bond of ``site``."""
+        t = DeviceArray.from_host(self.ctx, np.asarray(vec, dtype=self.dtype).reshape(-1), self.dtype)
+        A = self.sites[site]
+        if left:
+            self.sites[site] = self._einsum(t, [1], A, [1, 2, 3], A.shape, [1, 2, 3])
+        else:
+            self.sites[site] = self._einsum(A, [1, 2, 3], t, [3], A.shape, [1, 2, 3])
+
+    # ---- canonical forms ----------------------------------------------------
+    def canonize_(self, lo, hi=None):
+        """Mixed -> Mixed (canonize.jl:128-149); from NonCanonical first set
+        MixedCanonical(all sites) (:123-126); from Bond absorb Lambda (:181-219)."""
+        hi = lo if hi is None else hi
+        if self.form[0] == "none":
+            self.form = ("mixed", 0, self.n - 1)
+        if self.form[0] == "bond":
+            j = self.form[1]
+            if j + 1 <= lo:
+                self._scale_bond(j + 1, self.lam, left=True)
+                self.form = ("mixed", j + 1, j + 1)
+            elif hi <= j:
+                self._scale_bond(j, self.lam, left=False)
+                self.form = ("mixed", j, j)
+            else:
+                raise ValueError("cannot canonize from this BondCanonical form to the requested MixedCanonical")
+            self.lam = None
+        _, slo, shi = self.form
+        if (slo, shi) == (lo, hi):
+            return self
+        for i in range(slo, lo):
+            self._qr_right(i)
+        for i in range(shi, hi, -1):
+            self._qr_left(i)
+        self.form = ("mixed", lo, hi)
+        return self
+
+    def canonize_bond_(self, i):
+        """-> BondCanonical(bond i-(i+1)) (canonize.jl:151-179, 221-246)."""
+        if self.form[0] == "bond" and self.form[1] == i:
+            return self
+        if self.form[0] == "bond":
+            j = self.form[1]
+            if j < i:
+                self.canonize_(i)
+            else:
+                self.canonize_(i + 1)
+        if self.form[0] == "none":
+            self.form = ("mixed", 0, self.n - 1)
+        _, lo, hi = self.form
+        if hi <= i:
+            self.canonize_(i)
+            self._bond_right(i)
+        elif i + 1 <= lo:
+            self.canonize_(i + 1)
+            self._bond_left(i + 1)
+        elif lo <= i and i + 1 <= hi:
+            self.canonize_(i)
+            self._bond_right(i)
+        else:
+            raise ValueError("cannot canonize")
+        self.form = ("bond", i)
+        return self
+
+    def compress_bond_(self, i, maxdim=None, threshold=None):
+        """generic_mps_compress!(tn, bond) (compress.jl:29-69)."""
+        if maxdim is None and threshold is None:
+            raise ValueError("Either `maxdim` or `threshold` must be specified")
+        if maxdim is not None and maxdim <= 0:
+            raise ValueError("maxdim must be a positive integer")
+        if threshold is not None and threshold <= 0:
+            raise ValueError("Threshold must be positive")
+        self.canonize_bond_(i)
+        s = self.lam
+        k = len(s) if maxdim is None else min(int(maxdim), len(s))
+        if threshold is not None:
+            k = int(np.count_nonzero(np.abs(s[:k]) > threshold))
+        A, B = self.sites[i], self.sites[i + 1]
+        chil, d, chi = A.shape
+        if k < chi:
+            self.sites[i] = A.view((chil, d, k)).copy()
+            _, d2, chir = B.shape
+            src = tdesc((k, d2, chir), [1, 2, 3], [1, chi, chi * d2])
+            self.sites[i + 1] = self._gather(B, src, (k, d2, chir))
+        self.lam = s[:k].copy()
+        return self
+
+    def _gather(self, src_arr, src_desc, shape):
+        from .tensor import _one
+        one = _one(self.ctx, self.dtype)
+        out = DeviceArray(self.ctx, shape, self.dtype)
+        db = tdesc((), [])
+        dc = tdesc(shape, [1, 2, 3])
+        L.check(L.lib.tenet_einsum(self.ctx.h, None, self.code, L.PREC_NATIVE, C.byref(src_desc),
+                                   C.c_void_p(src_arr.ptr), C.byref(db), C.c_void_p(one.ptr), C.byref(dc),
+                                   C.c_void_p(out.ptr), 0, 0), "gather")
+        return out
+
+    def compress_(self, maxdim=None, threshold=None):
+        """All bonds sequentially (compress.jl:21-27)."""
+        for i in range(self.n - 1):
+            self.compress_bond_(i, maxdim, threshold)
+        return self
+
+    truncate_ = compress_
+
+    # ---- gates ----------------------------------------------------------------
+    def apply_one_site_(self, U2, a):
+        """1-site shortcut of generic_simple_update! (simple_update.jl:28-37)."""
+        g = DeviceArray.from_host(self.ctx, np.asarray(U2), self.dtype)
+        A = self.sites[a]
+        self.sites[a] = self._einsum(g, [4, 2], A, [1, 2, 3], A.shape, [1, 4, 3])
+        return self
+
+    def simple_update_(self, gate, a, maxdim=None, threshold=None, prec=None, absorb=L.ABSORB_EQUAL):
+        """simple_update!(::MPS, gate on sites (a, a+1); maxdim) (simple_update.jl:82-92):
+        move the orthogonality centre onto (a, a+1), then the fused gate update
+        (Muscle.simple_update, AbsorbEqually).  ``gate`` is a 4x4 host matrix or
+        a device (d, d, d, d) G[oa, ob, ia, ib] array.  Returns the kept bond dim."""
+        self.canonize_(a, a + 1)
+        A, B = self.sites[a], self.sites[a + 1]
+        chil, d1, chi = A.shape
+        _, d2, chir = B.shape
+        if isinstance(gate, DeviceArray):
+            G = gate
+        else:
+            G = DeviceArray.from_host(self.ctx, _gate_tensor(gate, d1), self.dtype)
+        m, n = chil * d1, d2 * chir
+        kmax = min(m, n)
+        Aout = DeviceArray(self.ctx, (m * kmax,), self.dtype)
+        Bout = DeviceArray(self.ctx, (kmax * n,), self.dtype)
+        k = C.c_int64(0)
+        st = L.lib.tenet_simple_update(self.ctx.h, None, self.code, self.prec if prec is None else prec, chil, d1, chi,
+                                       d2, chir, C.c_void_p(A.ptr), C.c_void_p(B.ptr), C.c_void_p(G.ptr),
+                                       C.c_void_p(Aout.ptr), C.c_void_p(Bout.ptr), None, int(maxdim or 0),
+                                       float(threshold or 0.0), int(absorb), None, None, C.byref(k))
+        L.check(st, "simple_update")
+        k = int(k.value)
+        self.sites[a] = Aout.view((chil, d1, k))
+        self.sites[a + 1] = Bout.view((k, d2, chir))
+        self.form = ("mixed", a, a + 1)
+        return k
+
+    def evolve_(self, gate, a):
+        """evolve!(tn, op::Tensor) forwards no keyword arguments (evolve.jl:140-143)."""
+        return self.simple_update_(gate, a)
+
+    # ---- transfer-matrix sweeps ----------------------------------------------------
+    def _site_ptrs(self):
+        return (C.c_void_p * self.n)(*[s.ptr for s in self.sites])
+
+    def overlap(self, other: "MPS", prec=L.PREC_NATIVE, log=False):
+        """overlap(a, b) = <b|a> (overlap.jl:36-50).  With log=True returns
+        (mantissa, log_scale)."""
+        if self.n != other.n:
+            raise ValueError("site count mismatch")
+        d = self.sites[0].shape[1]
+        ca = (C.c_int64 * (self.n + 1))(*self.chis())
+        cb = (C.c_int64 * (other.n + 1))(*other.chis())
+        out = self.ctx.malloc(64)
+        try:
+            L.check(L.lib.tenet_overlap(self.ctx.h, None, self.code, prec, self.n, d, ca, self._site_ptrs(), cb,
+                                        other._site_ptrs(), C.c_void_p(out)), "overlap")
+            h = np.zeros(3)
+            L.check(L.lib.tenet_memcpy_d2h(self.ctx.h, h.ctypes.data_as(C.c_void_p), C.c_void_p(out), 24, None))
+        finally:
+            self.ctx.free(out)
+        val = complex(h[0], h[1])
+        if log:
+            return val, float(h[2])
+        return val * math.exp(h[2])
+
+    def norm(self):
+        """norm(::MPS) (norm.jl:17-37): canonize to the minimum centre, then
+        the Frobenius norm of that tensor (or of Lambda when BondCanonical)."""
+        if self.form[0] == "bond":
+            return float(np.linalg.norm(self.lam))
+        if self.form[0] == "none":
+            self.form = ("mixed", 0, self.n - 1)
+        lo = self.form[1]
+        self.canonize_(lo, lo)
+        A = self.sites[lo]
+        v = self._einsum(A, [1, 2, 3], A, [1, 2, 3], (), [], conj_b=1)
+        return math.sqrt(abs(v.to_host().reshape(()).item()))
+
+    def normalize_(self):
+        """normalize!(::MPS) (normalize.jl:15-37)."""
+        if self.form[0] == "bond":
+            self.lam = self.lam / np.linalg.norm(self.lam)
+            return self
+        nrm = self.norm()
+        lo = self.form[1]
+        A = self.sites[lo]
+        t = DeviceArray.from_host(self.ctx, np.full((), 1.0 / nrm, dtype=self.dtype), self.dtype)
+        self.sites[lo] = self._einsum(A, [1, 2, 3], t, [], A.shape, [1, 2, 3])
+        return self
+
+    def amplitudes(self, bitstrings):
+        """<x|psi> for each 0/1 bitstring (overlap.jl:22-32), one workgroup
+        per request (K4).  Returns complex128 values."""
+        bits = np.asarray(bitstrings, dtype=np.uint8).reshape(-1, self.n)
+        nreq = bits.shape[0]
+        chis = np.asarray(self.chis(), dtype=np.int64)
+        ptrs = np.asarray([s.ptr for s in self.sites], dtype=np.uint64)
+        ctx = self.ctx
+        buf = ctx.malloc(ptrs.nbytes + chis.nbytes + bits.nbytes + nreq * (8 * 4 + 24) + 1024)
+        try:
+            p_ptrs = buf
+            p_chis = p_ptrs + ptrs.nbytes
+            p_bits = p_chis + chis.nbytes
+            p_tab = (p_bits + bits.nbytes + 63) // 64 * 64
+            p_out = p_tab + nreq * 32
+            tab = np.zeros((nreq, 4), dtype=np.uint64)
+            tab[:, 0] = self.n
+            tab[:, 1] = p_ptrs
+            tab[:, 2] = p_chis
+            tab[:, 3] = p_bits + np.arange(nreq, dtype=np.uint64) * self.n
+            # columns are gathered into the four device arrays the kernel reads
+            n_arr = np.ascontiguousarray(tab[:, 0].astype(np.int64))
+            sp = np.ascontiguousarray(tab[:, 1])
+            ch = np.ascontiguousarray(tab[:, 2])
+            bt = np.ascontiguousarray(tab[:, 3])
+            blob = b"".join(x.tobytes() for x in (n_arr, sp, ch, bt))
+            hostblob = np.frombuffer(blob, dtype=np.uint8)
+            for src, dst in ((ptrs, p_ptrs), (chis, p_chis), (bits, p_bits)):
+                L.check(L.lib.tenet_memcpy_h2d(ctx.h, C.c_void_p(dst), src.ctypes.data_as(C.c_void_p), src.nbytes,
+                                               None))
+            L.check(L.lib.tenet_memcpy_h2d(ctx.h, C.c_void_p(p_tab), hostblob.ctypes.data_as(C.c_void_p),
+                                           hostblob.nbytes, None))
+            L.check(L.lib.tenet_amplitudes(ctx.h, None, self.code, nreq, C.c_void_p(p_tab),
+                                           C.c_void_p(p_tab + 8 * nreq), C.c_void_p(p_tab + 16 * nreq),
+                                           C.c_void_p(p_tab + 24 * nreq), self.sites[0].shape[1], C.c_void_p(p_out)),
+                    "amplitudes")
+            h = np.zeros((nreq, 3))
+            L.check(L.lib.tenet_memcpy_d2h(ctx.h, h.ctypes.data_as(C.c_void_p), C.c_void_p(p_out), h.nbytes, None))
+        finally:
+            ctx.free(buf)
+        return (h[:, 0] + 1j * h[:, 1]) * np.exp(h[:, 2])
+
+    def expect_pauli(self, start, paulis, prec=L.PREC_NATIVE):
+        """<psi|P|psi> for a Pauli string on sites start..start+w-1, using the
+        canonical form: sites left of the centre are left isometries and
+        sites right of it right isometries, so the sweep only spans the
+        centre and the string."""
+        if self.form[0] != "mixed":
+            raise ValueError("expect_pauli needs a MixedCanonical MPS")
+        lo, hi = self.form[1], self.form[2]
+        w = len(paulis)
+        s0 = min(lo, start)
+        e0 = max(hi, start + w - 1)
+        ps = "".join(paulis[i - start] if start <= i < start + w else "I" for i in range(s0, e0 + 1))
+        chis = (C.c_int64 * (self.n + 1))(*self.chis())
+        out = self.ctx.malloc(64)
+        try:
+            L.check(L.lib.tenet_expect_pauli(self.ctx.h, None, self.code, prec, s0, len(ps), 2, chis,
+                                             self._site_ptrs(), None, ps.encode(), C.c_void_p(out)), "expect_pauli")
+            h = np.zeros(3)
+            L.check(L.lib.tenet_memcpy_d2h(self.ctx.h, h.ctypes.data_as(C.c_void_p), C.c_void_p(out), 24, None))
+        finally:
+            self.ctx.free(out)
+        return complex(h[0], h[1]) * math.exp(h[2])

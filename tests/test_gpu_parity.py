@@ -1,0 +1,347 @@
+"""GPU parity of the HIP path against the CPU oracle (oracle/tenet_oracle.py)
+on seeded inputs, through the C-ABI.
+
+Tolerances: FP64 results within 1e-12 relative (norm-wise) of the oracle;
+integer-valued FP64 contractions bit-exact; FP32 (c64) within 2e-5 relative.
+Gauge-dependent factors (Q, R, U, V) are compared through gauge-invariant
+quantities only (products, projectors, isometry), as the reference's tests do.
+"""
+import numpy as np
+import pytest
+
+from oracle import tenet_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+# ---------------------------------------------------------------- K1: einsum
+def _random_case(rng, max_rank=5, max_ext=6):
+    labels = list("abcdefghij")
+    while True:
+        ra, rb = rng.integers(1, max_rank + 1, size=2)
+        la = list(rng.choice(labels, ra, replace=False))
+        lb = list(rng.choice(labels, rb, replace=False))
+        shared = [l for l in la if l in lb]
+        if not shared:
+            continue
+        size = {l: int(rng.integers(1, max_ext + 1)) for l in labels}
+        nd = int(rng.integers(0, len(shared) + 1))
+        dims = list(rng.choice(shared, nd, replace=False)) if nd else []
+        return la, lb, size, dims
+
+
+@pytest.mark.parametrize("seed", range(40))
+def test_einsum_random_labels_c128(ctx, seed):
+    import tenet_hip as T
+
+    rng = np.random.default_rng(1000 + seed)
+    la, lb, size, dims = _random_case(rng)
+    a = O.complex_gaussian(rng, [size[l] for l in la])
+    b = O.complex_gaussian(rng, [size[l] for l in lb])
+    ref = O.binary_einsum(O.OTensor(a, la), O.OTensor(b, lb), dims)
+    ta, tb = T.Tensor.from_host(ctx, a, la), T.Tensor.from_host(ctx, b, lb)
+    got = T.binary_einsum(ta, tb, dims)
+    assert got.labels == ref.labels
+    assert _rel(got.to_host(), ref.data) < 1e-12
+
+
+@pytest.mark.parametrize("seed", range(10))
+def test_einsum_integer_inputs_bit_exact(ctx, seed):
+    import tenet_hip as T
+
+    rng = np.random.default_rng(2000 + seed)
+    la, lb, size, dims = _random_case(rng)
+    a = rng.integers(-8, 9, [size[l] for l in la]) + 1j * rng.integers(-8, 9, [size[l] for l in la])
+    b = rng.integers(-8, 9, [size[l] for l in lb]) + 1j * rng.integers(-8, 9, [size[l] for l in lb])
+    ref = O.binary_einsum(O.OTensor(a, la), O.OTensor(b, lb), dims)
+    got = T.binary_einsum(T.Tensor.from_host(ctx, a, la), T.Tensor.from_host(ctx, b, lb), dims)
+    assert np.array_equal(got.to_host(), ref.data)
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_einsum_random_labels_c64(ctx, seed):
+    import tenet_hip as T
+
+    rng = np.random.default_rng(3000 + seed)
+    la, lb, size, dims = _random_case(rng)
+    a = O.complex_gaussian(rng, [size[l] for l in la], np.complex64)
+    b = O.complex_gaussian(rng, [size[l] for l in lb], np.complex64)
+    ref = O.binary_einsum(O.OTensor(a.astype(np.complex128), la), O.OTensor(b.astype(np.complex128), lb), dims)
+    got = T.binary_einsum(T.Tensor.from_host(ctx, a, la), T.Tensor.from_host(ctx, b, lb), dims)
+    assert _rel(got.to_host(), ref.data) < 2e-5
+
+
+@pytest.mark.parametrize("shape", [(64, 96, 80), (257, 130, 70), (512, 512, 256), (1, 300, 7), (300, 1, 5)])
+def test_einsum_matrix_shapes(ctx, shape):
+    import tenet_hip as T
+
+    m, n, k = shape
+    rng = np.random.default_rng(m + n + k)
+    a = O.complex_gaussian(rng, (m, k))
+    b = O.complex_gaussian(rng, (k, n))
+    got = T.binary_einsum(T.Tensor.from_host(ctx, a, ["i", "k"]), T.Tensor.from_host(ctx, b, ["k", "j"]))
+    assert _rel(got.to_host(), a @ b) < 1e-13
+
+
+def test_einsum_hadamard_dims_empty(ctx):
+    """dims=[] absorbs a vector into a tensor (canonize.jl:44)."""
+    import tenet_hip as T
+
+    rng = np.random.default_rng(5)
+    s = rng.random(7)
+    v = O.complex_gaussian(rng, (7, 3, 5))
+    got = T.binary_einsum(T.Tensor.from_host(ctx, s.astype(complex), ["k"]), T.Tensor.from_host(ctx, v, ["k", "o", "r"]),
+                          dims=[])
+    assert _rel(got.to_host(), s[:, None, None] * v) < 1e-15
+
+
+def test_einsum_shape_errors(ctx):
+    import tenet_hip as T
+
+    a = T.Tensor.from_host(ctx, np.ones((2, 3)), ["i", "k"])
+    b = T.Tensor.from_host(ctx, np.ones((4, 5)), ["k", "j"])
+    with pytest.raises(T.TenetError) as e:
+        T.binary_einsum(a, b)
+    assert e.value.status == 2  # ERR_SHAPE
+
+
+# ---------------------------------------------------------------- K2: QR
+@pytest.mark.parametrize("m,n", [(6, 4), (4, 6), (64, 32), (2, 32), (512, 256), (1, 5), (7, 1), (130, 130)])
+def test_qr_invariants_and_r_diagonal(ctx, m, n):
+    import tenet_hip as T
+
+    rng = np.random.default_rng(m * 1000 + n)
+    A = O.complex_gaussian(rng, (m, n))
+    Q, R = T.tensor_qr_thin(T.Tensor.from_host(ctx, A, ["i", "j"]), ["i"], ["j"], "v")
+    q, r = Q.to_host(), R.to_host()
+    k = min(m, n)
+    assert q.shape == (m, k) and r.shape == (k, n)
+    assert np.linalg.norm(q.conj().T @ q - np.eye(k)) < 1e-13 * max(1, k)
+    assert _rel(q @ r, A) < 1e-13
+    assert np.allclose(np.tril(r, -1), 0)
+    _, r_ref = np.linalg.qr(A)
+    assert np.allclose(np.abs(np.diag(r)), np.abs(np.diag(r_ref)), rtol=1e-11, atol=1e-13)
+
+
+def test_qr_rank_deficient_integer(ctx):
+    """reshape(1:64, 16, 4) has rank 2: Q must still be an isometry."""
+    import tenet_hip as T
+
+    A = np.arange(1, 65, dtype=float).reshape(16, 4, order="F").astype(complex)
+    Q, R = T.tensor_qr_thin(T.Tensor.from_host(ctx, A, ["i", "j"]), ["i"], ["j"], "v")
+    q, r = Q.to_host(), R.to_host()
+    assert np.linalg.norm(q.conj().T @ q - np.eye(4)) < 1e-13
+    assert _rel(q @ r, A) < 1e-14
+
+
+# ---------------------------------------------------------------- K3: SVD
+@pytest.mark.parametrize("m,n", [(6, 4), (4, 6), (33, 20), (64, 64), (100, 300), (512, 512), (1, 1), (1, 9), (9, 1),
+                                 (40, 70)])
+def test_svd_values_and_factors(ctx, m, n):
+    import tenet_hip as T
+
+    rng = np.random.default_rng(m * 7919 + n)
+    A = O.complex_gaussian(rng, (m, n))
+    U, s, V = T.tensor_svd_thin(T.Tensor.from_host(ctx, A, ["i", "j"]), ["i"], "s")
+    u, v = U.to_host(), V.to_host()
+    s_ref = np.linalg.svd(A, compute_uv=False)
+    k = min(m, n)
+    assert np.allclose(s, s_ref, rtol=1e-12, atol=1e-13 * s_ref[0])
+    assert np.all(np.diff(s) <= 0)
+    assert np.linalg.norm(u.conj().T @ u - np.eye(k)) < 1e-12 * max(1, k)
+    assert np.linalg.norm(v @ v.conj().T - np.eye(k)) < 1e-12 * max(1, k)
+    assert _rel((u * s) @ v, A) < 1e-12
+
+
+def test_svd_rank_deficient_and_truncation(ctx):
+    """Rank-1 first tensor of mps.jl:465-472: one nonzero singular value."""
+    import tenet_hip as T
+
+    A = np.ones((4, 16), dtype=complex)
+    U, s, V = T.tensor_svd_thin(T.Tensor.from_host(ctx, A, ["i", "j"]), ["i"], "s", threshold=1e-10)
+    assert len(s) == 1 and abs(s[0] - 8.0) < 1e-12
+    rec = (U.to_host().reshape(4, 1) * s) @ V.to_host().reshape(1, 16)
+    assert _rel(rec, A) < 1e-14
+    rng = np.random.default_rng(4)
+    B = O.complex_gaussian(rng, (50, 40))
+    U, s, V = T.tensor_svd_thin(T.Tensor.from_host(ctx, B, ["i", "j"]), ["i"], "s", maxdim=7)
+    assert len(s) == 7 and U.shape == (50, 7) and V.shape == (7, 40)
+    assert np.allclose(s, np.linalg.svd(B, compute_uv=False)[:7], rtol=1e-12)
+
+
+def test_svd_graded_spectrum(ctx):
+    """Singular values spanning 1 .. 1e-12: absolute error ~ eps * s_max."""
+    import tenet_hip as T
+
+    rng = np.random.default_rng(9)
+    n = 96
+    qa, _ = np.linalg.qr(O.complex_gaussian(rng, (n, n)))
+    qb, _ = np.linalg.qr(O.complex_gaussian(rng, (n, n)))
+    sv = np.logspace(0, -12, n)
+    A = (qa * sv) @ qb.conj().T
+    _, s, _ = T.tensor_svd_thin(T.Tensor.from_host(ctx, A, ["i", "j"]), ["i"], "s")
+    assert np.max(np.abs(s - sv)) < 1e-13
+
+
+# ---------------------------------------------------------------- canonize / compress
+def _int_arrays():
+    def cm(n, shape):
+        return np.arange(1, n + 1, dtype=float).reshape(shape, order="F")
+    return [cm(16, (4, 4)), cm(64, (4, 4, 4)), cm(64, (4, 4, 4)), cm(64, (4, 4, 4)), cm(16, (4, 4))]
+
+
+def _dense(sites):
+    v = sites[0].reshape(sites[0].shape[1], sites[0].shape[2])
+    for a in sites[1:]:
+        v = np.tensordot(v, a, axes=([v.ndim - 1], [0])).reshape(-1, a.shape[2])
+    return v.reshape(-1)
+
+
+@pytest.mark.parametrize("lo,hi", [(2, 2), (1, 2), (0, 0), (4, 4), (0, 4)])
+def test_canonize_integer_mps(ctx, lo, hi):
+    """test/unit/mps.jl:62-443: isometry pattern + contract(psi_c) ~ contract(psi)."""
+    import tenet_hip as T
+
+    psi = T.MPS.from_arrays(ctx, _int_arrays())
+    psi.form = ("none",)
+    ref = O.OMPS.from_tenet_arrays(_int_arrays()).to_dense()
+    psi.canonize_(lo, hi)
+    host = psi.to_host()
+    om = O.OMPS(host, ("mixed", lo, hi))
+    assert O.checkform(om, atol=1e-12)
+    assert _rel(_dense(host), ref) < 1e-13
+
+
+def test_canonize_sweep_and_bond_transitions(ctx):
+    import tenet_hip as T
+
+    psi = T.MPS.from_arrays(ctx, _int_arrays())
+    ref = O.OMPS.from_tenet_arrays(_int_arrays()).to_dense()
+    for step in [("m", 4), ("m", 2), ("b", 1), ("b", 3), ("b", 0), ("m", 3)]:
+        if step[0] == "m":
+            psi.canonize_(step[1])
+        else:
+            psi.canonize_bond_(step[1])
+        om = O.OMPS(psi.to_host(), psi.form)
+        assert O.checkform(om, atol=1e-12), step
+    assert psi.form == ("mixed", 3, 3)
+    assert _rel(_dense(psi.to_host()), ref) < 1e-13
+
+
+def test_compress_reference_cases(ctx):
+    """mps.jl:465-522."""
+    import tenet_hip as T
+
+    arrays = _int_arrays()
+    arrays[0] = np.ones((4, 4))
+    ref = O.OMPS.from_tenet_arrays(arrays).to_dense()
+    for kw in ({"maxdim": 1}, {"threshold": 0.1}):
+        psi = T.MPS.from_arrays(ctx, arrays)
+        psi.compress_bond_(0, **kw)
+        assert psi.bondsizes()[0] == 1
+        host = psi.to_host()
+        host[0] = host[0] * psi.lam[None, None, :]
+        assert _rel(_dense(host), ref) < 1.49e-8
+    psi = T.MPS.from_arrays(ctx, arrays)
+    psi.compress_(maxdim=1)
+    assert psi.bondsizes() == [1, 1, 1, 1]
+    # threshold = 1.1 x second Schmidt value of bond 1-2 (mps.jl:511)
+    thr = O.schmidt_values(O.OMPS.from_tenet_arrays(arrays), 0)[1] * 1.1
+    psi = T.MPS.from_arrays(ctx, arrays)
+    psi.compress_(threshold=thr)
+    assert psi.bondsizes()[0] == 1
+    host = psi.to_host()
+    host[-2] = host[-2] * psi.lam[None, None, :]
+    assert _rel(_dense(host), ref) < 1.49e-8
+
+
+def test_truncate_bondsizes(ctx):
+    """mps.jl:524-531: [2,4,8,16,8,4,2] then truncate!(maxdim=10) -> max 10."""
+    import tenet_hip as T
+
+    rng = np.random.default_rng(1)
+    om = O.random_gaussian_mps(rng, 8, 2, 128, cap_edges=True)
+    psi = T.MPS.from_host(ctx, om.A)
+    assert psi.bondsizes() == [2, 4, 8, 16, 8, 4, 2]
+    psi.truncate_(maxdim=10)
+    assert max(psi.bondsizes()) == 10
+    O.compress(om, maxdim=10)
+    assert psi.bondsizes() == om.bondsizes()
+
+
+# ---------------------------------------------------------------- gates / TEBD
+def test_apply_x_and_cx(ctx):
+    """test/unit/simple_update.jl:4-43."""
+    import tenet_hip as T
+
+    X = np.array([[0, 1], [1, 0]], dtype=complex)
+    p = T.MPS.product_state(ctx, [[1, 0], [0, 1]])
+    p.apply_one_site_(X, 0)
+    assert np.array_equal(p.to_host()[0].reshape(-1), [0, 1])
+    CX = np.array([[1, 0, 0, 0], [0, 1, 0, 0], [0, 0, 0, 1], [0, 0, 1, 0]], dtype=complex)
+    p = T.MPS.product_state(ctx, [[0, 1], [0, 1]])
+    p.simple_update_(CX, 0)
+    a, b = p.to_host()
+    assert np.allclose(np.abs(a[0, :, 0]), [0, 1])
+    assert np.allclose(np.abs(b[0, :, 0]), [1, 0])
+    if a.shape[2] > 1:
+        assert np.allclose(np.abs(a[0, :, 1]), 0)
+
+
+@pytest.mark.parametrize("n,depth,maxdim", [(8, 6, None), (10, 8, 8), (12, 10, 16)])
+def test_tebd_vs_oracle(ctx, n, depth, maxdim):
+    import tenet_hip as T
+    from tenet_hip.tebd import tebd_sweep, upload_gates
+
+    sched, gates = O.tebd_gates(77 + n, n, depth)
+    ref = O.tebd_run(n, sched, gates, maxdim, 1e-10)
+    psi = tebd_sweep(ctx, n, sched, upload_gates(ctx, gates), maxdim, 1e-10)
+    assert psi.bondsizes() == ref.bondsizes()
+    assert _rel(_dense(psi.to_host()), ref.to_dense()) < 1e-10
+
+
+# ---------------------------------------------------------------- sweeps
+def test_overlap_norm_amplitudes_expectation(ctx):
+    import tenet_hip as T
+
+    rng = np.random.default_rng(11)
+    om = O.random_gaussian_mps(rng, 9, 2, 6)
+    vec = om.to_dense()
+    psi = T.MPS.from_host(ctx, om.A)
+    ov = psi.overlap(psi)
+    assert abs(ov - np.vdot(vec, vec)) < 1e-12 * abs(np.vdot(vec, vec))
+    bits = rng.integers(0, 2, (5, 9))
+    amps = psi.amplitudes(bits)
+    for b, a in zip(bits, amps):
+        idx = int("".join(map(str, b)), 2)
+        assert abs(a - vec[idx]) < 1e-12 * np.abs(vec).max()
+    nrm = psi.copy().norm()
+    assert abs(nrm ** 2 - np.vdot(vec, vec).real) < 1e-12 * nrm ** 2
+    # expectation on a canonical copy
+    p2 = psi.copy()
+    p2.canonize_(0)
+    o2 = O.OMPS([a.copy() for a in om.A])
+    O.canonize_mixed(o2, 0, 0)
+    for start, ps in [(0, "X"), (3, "ZZ"), (5, "XYZ"), (2, "YXZI")]:
+        got = p2.expect_pauli(start, ps)
+        ref = O.expectation_pauli(o2, start, ps)
+        assert abs(got - ref) < 1e-10 * abs(np.vdot(vec, vec))
+
+
+def test_overlap_large_dynamic_range(ctx):
+    """Unnormalised Gaussian MPS (n=20, chi=32): <psi|psi> ~ 1e36 needs the
+    running log-scale."""
+    import tenet_hip as T
+
+    rng = np.random.default_rng(20261202)
+    om = O.random_gaussian_mps(rng, 20, 2, 32)
+    psi = T.MPS.from_host(ctx, om.A)
+    m, lg = psi.overlap(psi, log=True)
+    mr, lgr = O.overlap_mps_logscaled(om, om)
+    assert abs(np.log(abs(m)) + lg - (np.log(abs(mr)) + lgr)) < 1e-10
