@@ -198,6 +198,21 @@ int tenet_graph_launch(tenet_graph graph, tenet_stream stream);
 int tenet_graph_destroy(tenet_graph graph);
 
 /* ---- diagnostics -------------------------------------------------------- */
+/* Per-kernel timing: while enabled for a kernel family, every launch of that
+ * family is bracketed by HIP events on its stream, and the algorithmic work
+ * of the launch is accumulated (flops: complex MAC = 8; bytes: operands
+ * read + written once).  tenet_stats_read synchronises, returns the totals
+ * since the last read and clears them. */
+typedef enum {
+  TENET_KSTAT_OFF = 0,
+  TENET_KSTAT_SVD_ROUND = 1, /* one-sided Jacobi round (Gram, rotations, apply) */
+  TENET_KSTAT_GEMM = 2,      /* contraction kernels (all precisions) */
+  TENET_KSTAT_QR_PANEL = 3,  /* Householder panel factorisation */
+  TENET_KSTAT_QR_UPDATE = 4, /* blocked Householder trailing / Q updates */
+  TENET_KSTAT_AMPLITUDE = 5  /* batched amplitude sweep */
+} tenet_kstat;
+int tenet_stats_enable(tenet_ctx ctx, int kernel_family);
+int tenet_stats_read(tenet_ctx ctx, int64_t* launches, double* total_ms, double* flops, double* bytes);
 /* Average duration (ms) of the last `tenet_timer_*` bracket, measured with
  * HIP events on the given stream. */
 int tenet_timer_start(tenet_ctx ctx, tenet_stream stream);

@@ -72,6 +72,12 @@ int launch_qr_factor(double* W, double* tau, int64_t m, int64_t n, int64_t batch
 int launch_qr_extract(const double* W, const double* tau, int64_t m, int64_t n, int64_t batch, void* Q,
                       int64_t ldq, void* R, int64_t ldr, int dtype, int flags, hipStream_t s);
 
+bool qr_blocked_fits(int64_t m);
+size_t qr_blocked_ws_bytes(int64_t m, int64_t n);
+int launch_qr_blocked(double* W, int64_t m, int64_t n, void* Qout, int64_t ldq, void* Rout, int64_t ldr, int dtype,
+                      int flags, hipStream_t s);
+int qr_setup_attributes();
+
 // ---- K3: one-sided block Jacobi SVD -----------------------------------------
 struct SvdProblem {
   double* X;      // nr x ncp work matrix (complex, column-major)
@@ -79,7 +85,8 @@ struct SvdProblem {
   double* Gpart;  // npairs * q partial Gram blocks (2b x 2b complex)
   double* sigma;  // ncp
   int32_t* perm;  // ncp
-  int32_t* flag;  // [0] = rotated in this sweep, [1] = converged
+  int32_t* flag;  // [0] = rotated in this sweep, [1] = converged, [2] = sweeps
+  double* fro2;   // ||A||_F^2 (absolute rotation floor eps^2 * ||A||_F^2)
   int32_t nr, ncols, ncp, b, nblk, q;
   int32_t m, n, transposed;
   const void* A;  // input (m x n, lda)
@@ -101,7 +108,8 @@ struct SvdBatch {
   SvdProblem one;
 };
 int launch_svd_init(const SvdBatch& b, int64_t nprob, int64_t max_elems, int dtype, hipStream_t s);
-int launch_svd_round(const SvdBatch& b, int64_t nprob, int max_items, int round, int phase, hipStream_t s);
+int launch_svd_round(const SvdBatch& b, int64_t nprob, int max_items, int round, int phase, int mode,
+                     hipStream_t s);
 int launch_svd_sweep_end(const SvdBatch& b, int64_t nprob, hipStream_t s);
 int launch_svd_finalize(const SvdBatch& b, int64_t nprob, int max_ncp, int64_t max_elems, int dtype,
                         hipStream_t s);

@@ -13,7 +13,8 @@
 // 32x32 quadrant made of 2x2 16x16 MFMA blocks, K step 16.
 //   FP64: v_mfma_f64_16x16x4_f64      FP32: v_mfma_f32_16x16x4_f32
 // MFMA 16x16x4 operand layout (lane l): A[i = l%16][k = l/16],
-// B[k = l/16][j = l%16], D[i = 4*(l/16) + r][j = l%16], r = 0..3.
+// B[k = l/16][j = l%16]; result register r of lane l holds D[i][j = l%16]
+// with i = 4*(l/16) + r for FP32 and i = 4*r + l/16 for FP64.
 #include "../../include/tenet_hip.h"
 #include "internal.h"
 
@@ -194,7 +195,10 @@ __device__ __forceinline__ void gemm_tile(const TileView& p, int64_t tm, int64_t
     for (int i = 0; i < 2; ++i) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int64_t row = m0 + wm * 32 + i * 16 + (lane >> 4) * 4 + r;
+        // D layout: FP32 16x16x4 keeps rows 4*(l/16)+r in lane l; FP64
+        // 16x16x4 interleaves them as rows 4*r + l/16.
+        const int rr = (sizeof(R) == 8) ? (r * 4 + (lane >> 4)) : ((lane >> 4) * 4 + r);
+        const int64_t row = m0 + wm * 32 + i * 16 + rr;
         if (row >= M) continue;
         const int64_t off = p.t.rowC[row] + ccol;
         const double cr = (double)accR[i][j][r], ci = (double)accI[i][j][r];

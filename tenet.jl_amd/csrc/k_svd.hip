@@ -60,25 +60,48 @@ __device__ __forceinline__ void stage_rows(SvSmem& sm, const double* M, int64_t 
   }
 }
 
-// acc (this thread's Gram entries) += staged^H staged
-constexpr int GE_PER_THREAD = (B2MAX * B2MAX + SV_THREADS - 1) / SV_THREADS;
+// acc += staged^H staged for this thread's 2x2 block of the Gram matrix:
+// thread t < (n2/2)^2 owns rows {2i, 2i+1} x cols {2j, 2j+1}, i = t % (n2/2),
+// j = t / (n2/2); acc[0..3] = (2i,2j), (2i+1,2j), (2i,2j+1), (2i+1,2j+1).
+constexpr int GE_PER_THREAD = 4;
 
 __device__ __forceinline__ void gram_accum(SvSmem& sm, int n2, int rows, double* ar, double* ai) {
-#pragma unroll
-  for (int t = 0; t < GE_PER_THREAD; ++t) {
-    const int e = threadIdx.x + t * SV_THREADS;
-    if (e >= n2 * n2) continue;
-    const int l1 = e % n2, l2 = e / n2;
-    double sr = 0.0, si = 0.0;
-    for (int r = 0; r < rows; ++r) {
-      const double x1r = sm.xr[r][l1], x1i = sm.xi[r][l1];
-      const double x2r = sm.xr[r][l2], x2i = sm.xi[r][l2];
-      sr += x1r * x2r + x1i * x2i;  // conj(x1) * x2
-      si += x1r * x2i - x1i * x2r;
-    }
-    ar[t] += sr;
-    ai[t] += si;
+  const int h = n2 / 2;
+  const int t = threadIdx.x;
+  if (t >= h * h) return;
+  const int i0 = 2 * (t % h), j0 = 2 * (t / h);
+  double s0r = 0, s0i = 0, s1r = 0, s1i = 0, s2r = 0, s2i = 0, s3r = 0, s3i = 0;
+  for (int r = 0; r < rows; ++r) {
+    const double ar0 = sm.xr[r][i0], ai0 = sm.xi[r][i0];
+    const double ar1 = sm.xr[r][i0 + 1], ai1 = sm.xi[r][i0 + 1];
+    const double br0 = sm.xr[r][j0], bi0 = sm.xi[r][j0];
+    const double br1 = sm.xr[r][j0 + 1], bi1 = sm.xi[r][j0 + 1];
+    // conj(a) * b
+    s0r += ar0 * br0 + ai0 * bi0;
+    s0i += ar0 * bi0 - ai0 * br0;
+    s1r += ar1 * br0 + ai1 * bi0;
+    s1i += ar1 * bi0 - ai1 * br0;
+    s2r += ar0 * br1 + ai0 * bi1;
+    s2i += ar0 * bi1 - ai0 * br1;
+    s3r += ar1 * br1 + ai1 * bi1;
+    s3i += ar1 * bi1 - ai1 * br1;
   }
+  ar[0] += s0r;
+  ai[0] += s0i;
+  ar[1] += s1r;
+  ai[1] += s1i;
+  ar[2] += s2r;
+  ai[2] += s2i;
+  ar[3] += s3r;
+  ai[3] += s3i;
+}
+
+// Gram entry index (row, col) of this thread's slot t (0..3)
+__device__ __forceinline__ void gram_slot(int n2, int slot, int& row, int& col) {
+  const int h = n2 / 2;
+  const int t = threadIdx.x;
+  row = 2 * (t % h) + (slot & 1);
+  col = 2 * (t / h) + (slot >> 1);
 }
 
 __device__ void gram_over_rows(SvSmem& sm, const double* X, int64_t ld, int r0, int r1, int I, int J, int b,
@@ -97,44 +120,76 @@ __device__ void gram_over_rows(SvSmem& sm, const double* X, int64_t ld, int r0, 
   }
 }
 
-// Diagonalise the Hermitian sm.g (n2 x n2) in place, W accumulates the
-// rotations.  Returns (via sm.any_outer) whether the first sweep found any
-// off-diagonal entry above tol (the pair was not yet orthogonal).
-__device__ void jacobi_inner(SvSmem& sm, int n2, double tol, int max_sweeps) {
+// Diagonalise the Hermitian sm.g (n2 x n2, n2 <= 32) in place by cyclic
+// Jacobi in the round-robin order; W accumulates the rotations.  Per step the
+// n2/2 disjoint rotations J_k are formed, then G <- J^H G J is applied one
+// 2x2 block at a time, B(k,l) <- J_k^H B(k,l) J_l for k <= l (the mirror
+// block is its conjugate transpose), so G passes through LDS once per step
+// with two barriers.  sm.any_outer reports whether the first sweep rotated.
+// cross_only: the b = n2/2 rotations of step s pair local column k (block I)
+// with column b + (k + s) % b (block J), b steps per sweep, so every I-J pair
+// is visited once and pairs inside a block are left to the full rounds.
+__device__ __forceinline__ void inner_pair(int k, int step, int n2, bool cross_only, int& p, int& q) {
+  if (cross_only) {
+    const int b = n2 / 2;
+    p = k;
+    q = b + (k + step) % b;
+  } else {
+    p = circle_pos(k, step, n2);
+    q = circle_pos(n2 - 1 - k, step, n2);
+    if (p > q) {
+      const int t = p;
+      p = q;
+      q = t;
+    }
+  }
+}
+
+__device__ void jacobi_inner(SvSmem& sm, int n2, double tol, double abs_tol, int max_sweeps, bool cross_only) {
   const int tid = threadIdx.x;
+  const int npair = n2 / 2;
+  const int nblk2 = npair * (npair + 1) / 2;
   for (int e = tid; e < n2 * n2; e += SV_THREADS) {
     const int i = e % n2, j = e / n2;
     sm.wr[i][j] = (i == j) ? 1.0 : 0.0;
     sm.wi[i][j] = 0.0;
   }
   if (tid == 0) sm.any_outer = 0;
-  __syncthreads();
-  const int npair = n2 / 2;
+  // block (k, l), k <= l, owned by thread tid < nblk2
+  int bk = 0, bl = 0;
+  if (tid < nblk2) {
+    int rem = tid;
+    while (rem >= npair - bk) {
+      rem -= npair - bk;
+      ++bk;
+    }
+    bl = bk + rem;
+  }
   for (int sweep = 0; sweep < max_sweeps; ++sweep) {
     if (tid == 0) sm.any_rot = 0;
-    __syncthreads();
-    for (int step = 0; step < n2 - 1; ++step) {
+    const int nsteps = cross_only ? npair : n2 - 1;
+    for (int step = 0; step < nsteps; ++step) {
+      __syncthreads();
       if (tid < npair) {
-        int p = circle_pos(tid, step, n2), q = circle_pos(n2 - 1 - tid, step, n2);
-        if (p > q) {
-          const int t = p;
-          p = q;
-          q = t;
-        }
+        int p, q;
+        inner_pair(tid, step, n2, cross_only, p, q);
         const double a = sm.gr[p][p], d = sm.gr[q][q];
         const double gr = sm.gr[p][q], gi = sm.gi[p][q];
-        const double g = sqrt(gr * gr + gi * gi);
+        const double g2 = gr * gr + gi * gi;
         int act = 0;
         double c = 1.0, sr = 0.0, si = 0.0;
-        if (g > tol * sqrt(fmax(a, 0.0) * fmax(d, 0.0)) && g > 1e-300) {
+        // |g| > tol sqrt(a d) and |g| > abs_tol, compared squared
+        if (g2 > tol * tol * fmax(a, 0.0) * fmax(d, 0.0) && g2 > abs_tol * abs_tol) {
           act = 1;
-          const double tau = (d - a) / (2.0 * g);
+          const double rg = rsqrt(g2);  // 1/|g|
+          const double tau = 0.5 * (d - a) * rg;
           const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-          c = 1.0 / sqrt(1.0 + t * t);
-          const double s = t * c;
-          // s * e^{i phi}, e^{i phi} = g_pq / |g_pq|
-          sr = s * gr / g;
-          si = s * gi / g;
+          c = rsqrt(1.0 + t * t);
+          const double sn = t * c * rg;
+          sr = sn * gr;
+          si = sn * gi;
+          sm.any_rot = 1;
+          if (sweep == 0) sm.any_outer = 1;
         }
         sm.rp[tid] = p;
         sm.rq[tid] = q;
@@ -142,104 +197,130 @@ __device__ void jacobi_inner(SvSmem& sm, int n2, double tol, int max_sweeps) {
         sm.rsr[tid] = sr;
         sm.rsi[tid] = si;
         sm.ract[tid] = act;
-        if (act) {
-          sm.any_rot = 1;
-          if (sweep == 0) sm.any_outer = 1;
+      }
+      __syncthreads();
+      if (tid < nblk2 && (sm.ract[bk] || sm.ract[bl])) {
+        const int p0 = sm.rp[bk], q0 = sm.rq[bk], p1 = sm.rp[bl], q1 = sm.rq[bl];
+        const double ck = sm.rc[bk], ekr = sm.rsr[bk], eki = sm.rsi[bk];
+        const double cl = sm.rc[bl], elr = sm.rsr[bl], eli = sm.rsi[bl];
+        // B = G[{p0,q0}][{p1,q1}]
+        const double b00r = sm.gr[p0][p1], b00i = sm.gi[p0][p1];
+        const double b01r = sm.gr[p0][q1], b01i = sm.gi[p0][q1];
+        const double b10r = sm.gr[q0][p1], b10i = sm.gi[q0][p1];
+        const double b11r = sm.gr[q0][q1], b11i = sm.gi[q0][q1];
+        // T = J_k^H B: T0 = c B0 - e B1 ; T1 = conj(e) B0 + c B1
+        const double t00r = ck * b00r - (ekr * b10r - eki * b10i), t00i = ck * b00i - (ekr * b10i + eki * b10r);
+        const double t01r = ck * b01r - (ekr * b11r - eki * b11i), t01i = ck * b01i - (ekr * b11i + eki * b11r);
+        const double t10r = (ekr * b00r + eki * b00i) + ck * b10r, t10i = (ekr * b00i - eki * b00r) + ck * b10i;
+        const double t11r = (ekr * b01r + eki * b01i) + ck * b11r, t11i = (ekr * b01i - eki * b01r) + ck * b11i;
+        // B' = T J_l: col0 = c T0 - conj(e) T1 ; col1 = e T0 + c T1 (columns of T)
+        double n00r = cl * t00r - (elr * t01r + eli * t01i), n00i = cl * t00i - (elr * t01i - eli * t01r);
+        double n10r = cl * t10r - (elr * t11r + eli * t11i), n10i = cl * t10i - (elr * t11i - eli * t11r);
+        double n01r = (elr * t00r - eli * t00i) + cl * t01r, n01i = (elr * t00i + eli * t00r) + cl * t01i;
+        double n11r = (elr * t10r - eli * t10i) + cl * t11r, n11i = (elr * t10i + eli * t10r) + cl * t11i;
+        if (bk == bl) {  // diagonal block: exactly diagonal and real after the rotation
+          n00i = 0.0;
+          n11i = 0.0;
+          if (sm.ract[bk]) {
+            n01r = n01i = n10r = n10i = 0.0;
+          }
+        }
+        sm.gr[p0][p1] = n00r;
+        sm.gi[p0][p1] = n00i;
+        sm.gr[p0][q1] = n01r;
+        sm.gi[p0][q1] = n01i;
+        sm.gr[q0][p1] = n10r;
+        sm.gi[q0][p1] = n10i;
+        sm.gr[q0][q1] = n11r;
+        sm.gi[q0][q1] = n11i;
+        if (bk != bl) {
+          sm.gr[p1][p0] = n00r;
+          sm.gi[p1][p0] = -n00i;
+          sm.gr[q1][p0] = n01r;
+          sm.gi[q1][p0] = -n01i;
+          sm.gr[p1][q0] = n10r;
+          sm.gi[p1][q0] = -n10i;
+          sm.gr[q1][q0] = n11r;
+          sm.gi[q1][q0] = -n11i;
         }
       }
-      __syncthreads();
-      // J = [[c, s e^{i phi}], [-s e^{-i phi}, c]] on (p, q).
-      // rows:  G_p <- c G_p - s e^{i phi} G_q ;  G_q <- s e^{-i phi} G_p + c G_q
+      // W <- W J
       for (int e = tid; e < npair * n2; e += SV_THREADS) {
-        const int k = e / n2, col = e % n2;
+        const int k = e / n2, row = e % n2;
         if (!sm.ract[k]) continue;
         const int p = sm.rp[k], q = sm.rq[k];
-        const double c = sm.rc[k], sr = sm.rsr[k], si = sm.rsi[k];
-        const double pr = sm.gr[p][col], pi = sm.gi[p][col];
-        const double qr = sm.gr[q][col], qi = sm.gi[q][col];
-        sm.gr[p][col] = c * pr - (sr * qr - si * qi);
-        sm.gi[p][col] = c * pi - (sr * qi + si * qr);
-        sm.gr[q][col] = (sr * pr + si * pi) + c * qr;
-        sm.gi[q][col] = (sr * pi - si * pr) + c * qi;
+        const double c = sm.rc[k], er = sm.rsr[k], ei = sm.rsi[k];
+        const double pr = sm.wr[row][p], pi = sm.wi[row][p];
+        const double qr = sm.wr[row][q], qi = sm.wi[row][q];
+        sm.wr[row][p] = c * pr - (er * qr + ei * qi);
+        sm.wi[row][p] = c * pi - (er * qi - ei * qr);
+        sm.wr[row][q] = (er * pr - ei * pi) + c * qr;
+        sm.wi[row][q] = (er * pi + ei * pr) + c * qi;
       }
-      __syncthreads();
-      // columns of G and W:  M_p <- c M_p - s e^{-i phi} M_q ; M_q <- s e^{i phi} M_p + c M_q
-      for (int e = tid; e < 2 * npair * n2; e += SV_THREADS) {
-        const int which = e / (npair * n2);
-        const int ee = e % (npair * n2);
-        const int k = ee / n2, row = ee % n2;
-        if (!sm.ract[k]) continue;
-        const int p = sm.rp[k], q = sm.rq[k];
-        const double c = sm.rc[k], sr = sm.rsr[k], si = sm.rsi[k];
-        double(*Mr)[B2MAX + 1] = which ? sm.wr : sm.gr;
-        double(*Mi)[B2MAX + 1] = which ? sm.wi : sm.gi;
-        const double pr = Mr[row][p], pi = Mi[row][p];
-        const double qr = Mr[row][q], qi = Mi[row][q];
-        Mr[row][p] = c * pr - (sr * qr + si * qi);
-        Mi[row][p] = c * pi - (sr * qi - si * qr);
-        Mr[row][q] = (sr * pr - si * pi) + c * qr;
-        Mi[row][q] = (sr * pi + si * pr) + c * qi;
-      }
-      __syncthreads();
-      if (tid < npair && sm.ract[tid]) {
-        const int p = sm.rp[tid], q = sm.rq[tid];
-        sm.gr[p][q] = 0.0;
-        sm.gi[p][q] = 0.0;
-        sm.gr[q][p] = 0.0;
-        sm.gi[q][p] = 0.0;
-        sm.gi[p][p] = 0.0;
-        sm.gi[q][q] = 0.0;
-      }
-      __syncthreads();
     }
+    __syncthreads();
     const int rotated = sm.any_rot;
     __syncthreads();  // everyone has read any_rot before thread 0 resets it
     if (!rotated) break;
   }
 }
 
-// rows [r0, r1) of the pair's columns of M <- M W
+// rows [r0, r1) of the pair's columns of M <- M W.  Thread t: row t % 64 of
+// the staged chunk and 8 consecutive output columns (group t / 64).
 __device__ void apply_rows(SvSmem& sm, double* M, int64_t ld, int r0, int r1, int I, int J, int b) {
   const int n2 = 2 * b;
+  const int r = threadIdx.x % RS, cg = 8 * (threadIdx.x / RS);
   for (int rb = r0; rb < r1; rb += RS) {
     const int rows = min(RS, r1 - rb);
     __syncthreads();
     stage_rows(sm, M, ld, rb, rows, I, J, b);
     __syncthreads();
-    for (int e = threadIdx.x; e < RS * n2; e += SV_THREADS) {
-      const int r = e % RS, l = e / RS;
-      if (r >= rows) continue;
-      double sr = 0.0, si = 0.0;
+    if (r < rows && cg < n2) {
+      double sr[8] = {0, 0, 0, 0, 0, 0, 0, 0}, si[8] = {0, 0, 0, 0, 0, 0, 0, 0};
       for (int t = 0; t < n2; ++t) {
         const double xr = sm.xr[r][t], xi = sm.xi[r][t];
-        const double wr = sm.wr[t][l], wi = sm.wi[t][l];
-        sr += xr * wr - xi * wi;
-        si += xr * wi + xi * wr;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const double wr = sm.wr[t][cg + j], wi = sm.wi[t][cg + j];
+          sr[j] += xr * wr - xi * wi;
+          si[j] += xr * wi + xi * wr;
+        }
       }
-      double2 v;
-      v.x = sr;
-      v.y = si;
-      reinterpret_cast<double2*>(M)[(size_t)pair_col(l, I, J, b) * ld + rb + r] = v;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        if (cg + j < n2) {
+          double2 v;
+          v.x = sr[j];
+          v.y = si[j];
+          reinterpret_cast<double2*>(M)[(size_t)pair_col(cg + j, I, J, b) * ld + rb + r] = v;
+        }
+      }
     }
   }
   __syncthreads();
 }
 
-__global__ __launch_bounds__(SV_THREADS) void svd_round_kernel(SvdBatch Bt, int round, int phase) {
+// mode 0: full inner sweep on the fixed pairs (2p, 2p+1) (one such round per
+// sweep); mode 1: cross-block inner steps on the circle pairs of `round`.
+__global__ __launch_bounds__(SV_THREADS) void svd_round_kernel(SvdBatch Bt, int round, int phase, int mode) {
   __shared__ SvSmem sm;
   const SvdProblem& P = Bt.arr ? Bt.arr[blockIdx.y] : Bt.one;
   if (P.flag[1]) return;  // converged
   const int nb = P.nblk;
-  if (round >= nb - 1) return;
+  if (mode == 1 && round >= nb - 1) return;
   const int q = P.q;
   const int item = blockIdx.x;
   const int p = item / q, c = item % q;
   if (p >= nb / 2) return;
-  int I = circle_pos(p, round, nb), J = circle_pos(nb - 1 - p, round, nb);
-  if (I > J) {
-    const int t = I;
-    I = J;
-    J = t;
+  int I = 2 * p, J = 2 * p + 1;
+  if (mode == 1) {
+    I = circle_pos(p, round, nb);
+    J = circle_pos(nb - 1 - p, round, nb);
+    if (I > J) {
+      const int t = I;
+      I = J;
+      J = t;
+    }
   }
   const int b = P.b, n2 = 2 * b;
   const int rcx = (P.nr + q - 1) / q;
@@ -249,12 +330,15 @@ __global__ __launch_bounds__(SV_THREADS) void svd_round_kernel(SvdBatch Bt, int 
   if (phase == 0) {
     gram_over_rows(sm, P.X, P.nr, r0, r1, I, J, b, ar, ai);
     double* out = P.Gpart + ((size_t)p * q + c) * (size_t)n2 * n2 * 2;
+    if ((int)threadIdx.x < (n2 / 2) * (n2 / 2)) {
 #pragma unroll
-    for (int t = 0; t < GE_PER_THREAD; ++t) {
-      const int e = threadIdx.x + t * SV_THREADS;
-      if (e >= n2 * n2) continue;
-      out[2 * e] = ar[t];
-      out[2 * e + 1] = ai[t];
+      for (int t = 0; t < GE_PER_THREAD; ++t) {
+        int row, col;
+        gram_slot(n2, t, row, col);
+        const int e = col * n2 + row;
+        out[2 * e] = ar[t];
+        out[2 * e + 1] = ai[t];
+      }
     }
     return;
   }
@@ -267,26 +351,34 @@ __global__ __launch_bounds__(SV_THREADS) void svd_round_kernel(SvdBatch Bt, int 
     }
     for (int cc = 0; cc < q; ++cc) {
       const double* in = P.Gpart + ((size_t)p * q + cc) * (size_t)n2 * n2 * 2;
+      if ((int)threadIdx.x < (n2 / 2) * (n2 / 2)) {
 #pragma unroll
-      for (int t = 0; t < GE_PER_THREAD; ++t) {
-        const int e = threadIdx.x + t * SV_THREADS;
-        if (e >= n2 * n2) continue;
-        ar[t] += in[2 * e];
-        ai[t] += in[2 * e + 1];
+        for (int t = 0; t < GE_PER_THREAD; ++t) {
+          int row, col;
+          gram_slot(n2, t, row, col);
+          const int e = col * n2 + row;
+          ar[t] += in[2 * e];
+          ai[t] += in[2 * e + 1];
+        }
       }
     }
   }
   __syncthreads();
+  if ((int)threadIdx.x < (n2 / 2) * (n2 / 2)) {
 #pragma unroll
-  for (int t = 0; t < GE_PER_THREAD; ++t) {
-    const int e = threadIdx.x + t * SV_THREADS;
-    if (e >= n2 * n2) continue;
-    const int l1 = e % n2, l2 = e / n2;
-    sm.gr[l1][l2] = ar[t];
-    sm.gi[l1][l2] = ai[t];
+    for (int t = 0; t < GE_PER_THREAD; ++t) {
+      int row, col;
+      gram_slot(n2, t, row, col);
+      sm.gr[row][col] = ar[t];
+      sm.gi[row][col] = ai[t];
+    }
   }
   __syncthreads();
-  jacobi_inner(sm, n2, P.tol, 6);
+  // One cyclic sweep of the 2b x 2b problem per pair visit (the outer sweeps
+  // converge regardless; a full inner diagonalisation costs ~5x more).  Rotations are skipped only below eps^2 ||A||_F^2: a larger
+  // absolute floor (e.g. eps ||A||_F^2) stops the late small-angle rotations
+  // that give one-sided Jacobi its accuracy on small singular values.
+  jacobi_inner(sm, n2, P.tol, fmax(4.930380657631324e-32 * (*P.fro2), 1e-300), 1, mode == 1);
   if (!sm.any_outer) return;  // pair already orthogonal: W = I (uniform branch)
   if (threadIdx.x == 0 && c == 0) P.flag[0] = 1;
   const int rx0 = (phase == 2) ? 0 : r0, rx1 = (phase == 2) ? P.nr : r1;
@@ -343,6 +435,24 @@ __global__ void svd_init_kernel(SvdBatch Bt) {
     P.flag[1] = 0;
     P.flag[2] = 0;
   }
+}
+
+// ||X||_F^2, one workgroup per problem, fixed summation order
+__global__ __launch_bounds__(SV_THREADS) void svd_fro_kernel(SvdBatch Bt) {
+  __shared__ double red[SV_THREADS / 64];
+  const SvdProblem& P = Bt.arr ? Bt.arr[blockIdx.x] : Bt.one;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const size_t total = (size_t)P.nr * P.ncp;
+  const double2* X = reinterpret_cast<const double2*>(P.X);
+  double s = 0.0;
+  for (size_t e = tid; e < total; e += SV_THREADS) {
+    const double2 v = X[e];
+    s += v.x * v.x + v.y * v.y;
+  }
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if (lane == 0) red[wave] = s;
+  __syncthreads();
+  if (tid == 0) *P.fro2 = red[0] + red[1] + red[2] + red[3];
 }
 
 // norms, descending sort, S, kept count and discarded weight
@@ -479,12 +589,15 @@ int launch_svd_init(const SvdBatch& b, int64_t nprob, int64_t max_elems, int dty
   else
     hipLaunchKernelGGL(svd_init_kernel<float2>, dim3(gx, (unsigned)nprob), dim3(256), 0, s, b);
   TENET_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(svd_fro_kernel, dim3((unsigned)nprob), dim3(SV_THREADS), 0, s, b);
+  TENET_HIP_CHECK(hipGetLastError());
   return TENET_OK;
 }
 
-int launch_svd_round(const SvdBatch& b, int64_t nprob, int max_items, int round, int phase, hipStream_t s) {
+int launch_svd_round(const SvdBatch& b, int64_t nprob, int max_items, int round, int phase, int mode,
+                     hipStream_t s) {
   hipLaunchKernelGGL(svd_round_kernel, dim3((unsigned)max_items, (unsigned)nprob), dim3(SV_THREADS), 0, s, b, round,
-                     phase);
+                     phase, mode);
   TENET_HIP_CHECK(hipGetLastError());
   return TENET_OK;
 }

@@ -76,6 +76,56 @@ struct tenet_ctx_s {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int32_t* host_flag = nullptr;  // pinned
   int64_t* host_k = nullptr;     // pinned
+  // per-kernel timing (tenet_stats_*)
+  int stat_family = 0;
+  std::vector<hipEvent_t> stat_events;  // pairs
+  size_t stat_used = 0;
+  int64_t stat_launches = 0;
+  double stat_ms = 0.0, stat_flops = 0.0, stat_bytes = 0.0;
+};
+
+// Drain recorded event pairs into the running totals (synchronises).
+static int stats_drain(tenet_ctx ctx) {
+  for (size_t i = 0; i + 1 < ctx->stat_used; i += 2) {
+    TENET_HIP_CHECK(hipEventSynchronize(ctx->stat_events[i + 1]));
+    float ms = 0.f;
+    TENET_HIP_CHECK(hipEventElapsedTime(&ms, ctx->stat_events[i], ctx->stat_events[i + 1]));
+    ctx->stat_ms += ms;
+  }
+  ctx->stat_used = 0;
+  return TENET_OK;
+}
+
+// RAII bracket around one launch of a tracked family.
+struct StatScope {
+  tenet_ctx ctx;
+  hipStream_t s;
+  bool on = false;
+  StatScope(tenet_ctx c, hipStream_t st, int family, double flops, double bytes) : ctx(c), s(st) {
+    if (!ctx || ctx->stat_family != family || family == 0) return;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cs) == hipSuccess && cs == hipStreamCaptureStatusActive) return;
+    if (ctx->stat_used + 2 > ctx->stat_events.size()) {
+      if (ctx->stat_events.size() < 8192) {
+        for (int i = 0; i < 512; ++i) {
+          hipEvent_t e;
+          if (hipEventCreate(&e) != hipSuccess) return;
+          ctx->stat_events.push_back(e);
+        }
+      } else if (stats_drain(ctx) != TENET_OK) {
+        return;
+      }
+    }
+    if (hipEventRecord(ctx->stat_events[ctx->stat_used], s) != hipSuccess) return;
+    on = true;
+    ctx->stat_launches += 1;
+    ctx->stat_flops += flops;
+    ctx->stat_bytes += bytes;
+  }
+  ~StatScope() {
+    if (!on) return;
+    if (hipEventRecord(ctx->stat_events[ctx->stat_used + 1], s) == hipSuccess) ctx->stat_used += 2;
+  }
 };
 
 static const size_t ELEM[2] = {16, 8};
@@ -230,6 +280,7 @@ extern "C" int tenet_destroy(tenet_ctx ctx) {
   (void)hipHostFree(ctx->host_k);
   (void)hipEventDestroy(ctx->ev0);
   (void)hipEventDestroy(ctx->ev1);
+  for (hipEvent_t e : ctx->stat_events) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return TENET_OK;
@@ -612,6 +663,8 @@ static int exec_plan(tenet_plan p, hipStream_t s, const void* A, const void* B, 
   g.beta_im = beta ? beta[1] : 0.0;
   void* ws = nullptr;
   if (p->ws_bytes) TRY(ws_get(ctx, s, p->ws_bytes, &ws, 1));
+  StatScope st(ctx, s, TENET_KSTAT_GEMM, 8.0 * p->M * p->N * p->K * p->batch,
+               (double)es * ((double)p->M * p->K + (double)p->K * p->N + (double)p->M * p->N) * p->batch);
   if (p->dtype == TENET_C128) return launch_gemm_c128(g, p->prec, s, ws, p->ws_bytes);
   return launch_gemm_c64(g, p->prec, s, ws, p->ws_bytes);
 }
@@ -716,7 +769,8 @@ static int einsum_internal(tenet_ctx ctx, hipStream_t s, int dtype, int prec, co
 // ---------------------------------------------------------------------------
 static size_t qr_ws_bytes(int64_t m, int64_t n) {
   const int64_t k = std::min(m, n);
-  return align_up((size_t)m * n * 16 + (size_t)m * k * 16, 256) + align_up((size_t)k * 16, 256) + 256;
+  return std::max(align_up((size_t)m * n * 16 + (size_t)m * k * 16, 256) + align_up((size_t)k * 16, 256) + 256,
+                  qr_blocked_ws_bytes(m, n) + 256);
 }
 
 static int qr_run(tenet_ctx ctx, hipStream_t s, int dtype, int flags, int64_t m, int64_t n, const void* A,
@@ -727,6 +781,10 @@ static int qr_run(tenet_ctx ctx, hipStream_t s, int dtype, int flags, int64_t m,
   double* W = (double*)b.take((size_t)m * n * 16 + (size_t)m * k * 16);
   double* tau = (double*)b.take((size_t)k * 16);
   TRY(launch_pack_c128(A, dtype, m, n, lda, (flags & TENET_QR_IN_ADJ) ? 1 : 0, W, s));
+  if (qr_blocked_fits(m)) {
+    TRY(qr_setup_attributes());
+    return launch_qr_blocked(W, m, n, Q, ldq, R, ldr, dtype, flags & (TENET_QR_Q_ADJ | TENET_QR_R_ADJ), s);
+  }
   TRY(launch_qr_factor(W, tau, m, n, 1, s));
   TRY(launch_qr_extract(W, tau, m, n, 1, Q, ldq, R, ldr, dtype, flags & (TENET_QR_Q_ADJ | TENET_QR_R_ADJ), s));
   return TENET_OK;
@@ -782,7 +840,7 @@ static SvdShape svd_shape(int64_t m, int64_t n, bool allow_split) {
   sh.ncp = sh.nblk * sh.b;
   sh.npairs = sh.nblk / 2;
   sh.q = 1;
-  if (allow_split) sh.q = std::min<int64_t>(16, std::max<int64_t>(1, sh.nr / 128));
+  if (allow_split) sh.q = std::min<int64_t>(16, std::max<int64_t>(1, (sh.nr + 63) / 64));
   return sh;
 }
 
@@ -813,6 +871,7 @@ static int svd_run(tenet_ctx ctx, hipStream_t s, int dtype, int64_t m, int64_t n
   P.sigma = (double*)bm.take((size_t)sh.ncp * 8);
   P.perm = (int32_t*)bm.take((size_t)sh.ncp * 4);
   P.flag = (int32_t*)bm.take(256);
+  P.fro2 = (double*)(P.flag + 8);
   int64_t* kfall = (int64_t*)bm.take(256);
   P.nr = (int32_t)sh.nr;
   P.ncols = (int32_t)sh.ncols;
@@ -842,15 +901,30 @@ static int svd_run(tenet_ctx ctx, hipStream_t s, int dtype, int64_t m, int64_t n
   const bool sync_ok = !capturing(s);
   const int max_sweeps = sync_ok ? 40 : 14;
   bool converged = false;
-  for (int sw = 0; sw < max_sweeps && !converged; ++sw) {
-    for (int r = 0; r < sh.nblk - 1; ++r) {
-      if (sh.q == 1) {
-        TRY(launch_svd_round(B, 1, (int)sh.npairs, r, 2, s));
-      } else {
-        TRY(launch_svd_round(B, 1, (int)(sh.npairs * sh.q), r, 0, s));
-        TRY(launch_svd_round(B, 1, (int)(sh.npairs * sh.q), r, 1, s));
-      }
+  // one sweep = a full-inner round on the fixed pairs (2i, 2i+1), then the
+  // circle rounds with cross-block inner steps (every column pair once)
+  // algorithmic work of one round: Gram of every pair over nr rows, and the
+  // rotation applied to the pair's nr rows of X and ncp rows of V
+  const double n2 = 2.0 * sh.b;
+  const double gram_fl = 8.0 * sh.npairs * n2 * n2 * sh.nr;
+  const double apply_fl = 8.0 * sh.npairs * n2 * n2 * (sh.nr + sh.ncp);
+  const double xv_bytes = 16.0 * sh.npairs * n2 * (sh.nr + sh.ncp);
+  auto round = [&](int r, int mode) -> int {
+    if (sh.q == 1) {
+      StatScope st(ctx, s, TENET_KSTAT_SVD_ROUND, gram_fl + apply_fl, 16.0 * sh.npairs * n2 * sh.nr + 2.0 * xv_bytes);
+      return launch_svd_round(B, 1, (int)sh.npairs, r, 2, mode, s);
     }
+    {
+      StatScope st(ctx, s, TENET_KSTAT_SVD_ROUND, gram_fl, 16.0 * sh.npairs * n2 * sh.nr);
+      TRY(launch_svd_round(B, 1, (int)(sh.npairs * sh.q), r, 0, mode, s));
+    }
+    StatScope st(ctx, s, TENET_KSTAT_SVD_ROUND, apply_fl, 2.0 * xv_bytes);
+    return launch_svd_round(B, 1, (int)(sh.npairs * sh.q), r, 1, mode, s);
+  };
+  for (int sw = 0; sw < max_sweeps && !converged; ++sw) {
+    TRY(round(0, 0));
+    if (sh.nblk > 2)
+      for (int r = 0; r < sh.nblk - 1; ++r) TRY(round(r, 1));
     TRY(launch_svd_sweep_end(B, 1, s));
     if (sync_ok && sw >= 1) {
       TENET_HIP_CHECK(hipMemcpyAsync(ctx->host_flag, P.flag, 16, hipMemcpyDeviceToHost, s));
@@ -1108,7 +1182,8 @@ extern "C" int tenet_grouped_svd(tenet_ctx ctx, tenet_stream stream, int dtype, 
   CHECK_ARG(ms && ns && As && Us && Vhs, "null argument");
   hipStream_t s = pick(ctx, stream);
   std::vector<SvdShape> shapes((size_t)count);
-  size_t total = align_up((size_t)count * sizeof(SvdProblem), 256) + align_up((size_t)count * 16, 256);
+  size_t total = align_up((size_t)count * sizeof(SvdProblem), 256) + align_up((size_t)count * 16, 256) +
+                 align_up((size_t)count * 8, 256);
   int64_t max_rounds = 0, max_pairs = 0, max_ncp = 0, max_elems = 0, max_out = 0;
   for (int64_t i = 0; i < count; ++i) {
     CHECK_ARG(ms[i] > 0 && ns[i] > 0 && std::min(ms[i], ns[i]) <= 4096, "bad matrix shape");
@@ -1158,7 +1233,11 @@ extern "C" int tenet_grouped_svd(tenet_ctx ctx, tenet_stream stream, int dtype, 
     P.tol = svd_tol(sh.nr);
   }
   flags_base = (int32_t*)bm.take((size_t)count * 16);
-  for (int64_t i = 0; i < count; ++i) probs[(size_t)i].flag = flags_base + 4 * i;
+  double* fro_base = (double*)bm.take((size_t)count * 8);
+  for (int64_t i = 0; i < count; ++i) {
+    probs[(size_t)i].flag = flags_base + 4 * i;
+    probs[(size_t)i].fro2 = fro_base + i;
+  }
   TENET_HIP_CHECK(hipMemcpyAsync(dprobs, probs.data(), probs.size() * sizeof(SvdProblem), hipMemcpyHostToDevice, s));
   SvdBatch B{};
   B.arr = dprobs;
@@ -1166,7 +1245,8 @@ extern "C" int tenet_grouped_svd(tenet_ctx ctx, tenet_stream stream, int dtype, 
   std::vector<int32_t> hflags((size_t)count * 4);
   bool all = false;
   for (int sw = 0; sw < 40 && !all; ++sw) {
-    for (int r = 0; r < max_rounds; ++r) TRY(launch_svd_round(B, count, (int)max_pairs, r, 2, s));
+    TRY(launch_svd_round(B, count, (int)max_pairs, 0, 2, 0, s));
+    for (int r = 0; r < max_rounds; ++r) TRY(launch_svd_round(B, count, (int)max_pairs, r, 2, 1, s));
     TRY(launch_svd_sweep_end(B, count, s));
     if (sw >= 1) {
       TENET_HIP_CHECK(hipMemcpyAsync(hflags.data(), flags_base, (size_t)count * 16, hipMemcpyDeviceToHost, s));
@@ -1226,6 +1306,28 @@ extern "C" int tenet_graph_destroy(tenet_graph graph) {
   (void)hipGraphExecDestroy(graph->exec);
   (void)hipGraphDestroy(graph->graph);
   delete graph;
+  return TENET_OK;
+}
+
+extern "C" int tenet_stats_enable(tenet_ctx ctx, int family) {
+  TRY(use_device(ctx));
+  CHECK_ARG(family >= TENET_KSTAT_OFF && family <= TENET_KSTAT_AMPLITUDE, "unknown kernel family");
+  TRY(stats_drain(ctx));
+  ctx->stat_family = family;
+  ctx->stat_launches = 0;
+  ctx->stat_ms = ctx->stat_flops = ctx->stat_bytes = 0.0;
+  return TENET_OK;
+}
+
+extern "C" int tenet_stats_read(tenet_ctx ctx, int64_t* launches, double* total_ms, double* flops, double* bytes) {
+  TRY(use_device(ctx));
+  TRY(stats_drain(ctx));
+  if (launches) *launches = ctx->stat_launches;
+  if (total_ms) *total_ms = ctx->stat_ms;
+  if (flops) *flops = ctx->stat_flops;
+  if (bytes) *bytes = ctx->stat_bytes;
+  ctx->stat_launches = 0;
+  ctx->stat_ms = ctx->stat_flops = ctx->stat_bytes = 0.0;
   return TENET_OK;
 }
 

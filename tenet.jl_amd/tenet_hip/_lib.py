@@ -29,6 +29,7 @@ C128, C64 = 0, 1
 PREC_NATIVE, PREC_FP32, PREC_SPLIT_BF16, PREC_SPLIT_FP16 = 0, 1, 2, 3
 ABSORB_NONE, ABSORB_EQUAL, ABSORB_LEFT, ABSORB_RIGHT = 0, 1, 2, 3
 QR_IN_ADJ, QR_Q_ADJ, QR_R_ADJ = 1, 2, 4
+KSTAT_OFF, KSTAT_SVD_ROUND, KSTAT_GEMM, KSTAT_QR_PANEL, KSTAT_QR_UPDATE, KSTAT_AMPLITUDE = 0, 1, 2, 3, 4, 5
 MAX_RANK = 8
 
 
@@ -101,6 +102,8 @@ SIGNATURES = {
     "tenet_graph_end": [_p, _p, C.POINTER(_p)],
     "tenet_graph_launch": [_p, _p],
     "tenet_graph_destroy": [_p],
+    "tenet_stats_enable": [_p, _i],
+    "tenet_stats_read": [_p, _pi64, _pd, _pd, _pd],
     "tenet_timer_start": [_p, _p],
     "tenet_timer_stop": [_p, _p, C.POINTER(C.c_float)],
 }

@@ -50,6 +50,14 @@ class Context:
         L.check(L.lib.tenet_stream_create(self.h, C.byref(s)), "stream_create")
         return s
 
+    def stats_enable(self, family):
+        L.check(L.lib.tenet_stats_enable(self.h, int(family)), "stats_enable")
+
+    def stats_read(self):
+        n, ms, fl, by = C.c_int64(), C.c_double(), C.c_double(), C.c_double()
+        L.check(L.lib.tenet_stats_read(self.h, C.byref(n), C.byref(ms), C.byref(fl), C.byref(by)), "stats_read")
+        return {"launches": n.value, "ms": ms.value, "flops": fl.value, "bytes": by.value}
+
     def timer_start(self, stream=None):
         L.check(L.lib.tenet_timer_start(self.h, stream))
 

@@ -26,6 +26,16 @@ from . import _lib as L
 from .device import DTYPES, Context, DeviceArray, tdesc
 
 
+def qr_flops(m, n):
+    p, q = max(m, n), min(m, n)
+    return 4.0 * (4.0 * p * q * q - 4.0 * q ** 3 / 3.0)
+
+
+def svd_flops(m, n):
+    p, q = max(m, n), min(m, n)
+    return 4.0 * (4.0 * p * p * q + 8.0 * p * q * q + 9.0 * q ** 3)
+
+
 def _gate_tensor(U4, d):
     """4x4 gate with site a as the most significant digit, U4[d*oa+ob, d*ia+ib]
     (test/unit/simple_update.jl:19-33) -> column-major G[oa, ob, ia, ib]."""
@@ -41,6 +51,9 @@ class MPS:
         self.form = form if form is not None else ("mixed", 0, len(self.sites) - 1)  # MPS.jl:18
         self.lam = None  # host Lambda of a BondCanonical MPS
         self.prec = L.PREC_NATIVE
+        # nominal work (real flops): complex MAC = 8; thin QR (p >= q) =
+        # 4(4pq^2 - 4q^3/3); thin SVD (p >= q) = 4(4p^2q + 8pq^2 + 9q^3)
+        self.flops = {"qr": 0.0, "svd": 0.0, "contract": 0.0}
 
     # ---- construction ---------------------------------------------------
     @staticmethod
@@ -113,6 +126,8 @@ class MPS:
         L.check(L.lib.tenet_qr_thin(self.ctx.h, None, self.code, m, chi, C.c_void_p(A.ptr), m, C.c_void_p(Q.ptr), m,
                                     C.c_void_p(R.ptr), k), "qr")
         _, d2, chir = B.shape
+        self.flops["qr"] += qr_flops(m, chi)
+        self.flops["contract"] += 8.0 * k * chi * d2 * chir
         self.sites[i + 1] = self._einsum(R, [1, 2], B, [2, 3, 4], (k, d2, chir), [1, 3, 4])
         self.sites[i] = Q
 
@@ -131,6 +146,8 @@ class MPS:
                                        C.c_void_p(A.ptr), chi, C.c_void_p(Qh.ptr), k, C.c_void_p(Rh.ptr), chi),
                 "qr (adjoint)")
         chil2, d2, _ = B.shape
+        self.flops["qr"] += qr_flops(m, chi)
+        self.flops["contract"] += 8.0 * chil2 * d2 * chi * k
         self.sites[i - 1] = self._einsum(B, [1, 2, 3], Rh, [3, 4], (chil2, d2, k), [1, 2, 4])
         self.sites[i] = Qh
 
@@ -246,6 +263,13 @@ class MPS:
         self.form = ("bond", i)
         return self
 
+    def schmidt_values(self, i):
+        """Tenet.schmidt_values (src/Operations/entropy.jl:25-31): Lambda of
+        BondCanonical(bond i-(i+1)) on a copy."""
+        p = self.copy()
+        p.canonize_bond_(i)
+        return p.lam.copy()
+
     def compress_bond_(self, i, maxdim=None, threshold=None):
         """generic_mps_compress!(tn, bond) (compress.jl:29-69)."""
         if maxdim is None and threshold is None:
@@ -320,6 +344,8 @@ class MPS:
                                        float(threshold or 0.0), int(absorb), None, None, C.byref(k))
         L.check(st, "simple_update")
         k = int(k.value)
+        self.flops["contract"] += 8.0 * m * chi * n + 8.0 * chil * chir * (d1 * d2) ** 2
+        self.flops["svd"] += svd_flops(m, n)
         self.sites[a] = Aout.view((chil, d1, k))
         self.sites[a + 1] = Bout.view((k, d2, chir))
         self.form = ("mixed", a, a + 1)

@@ -22,6 +22,20 @@ def brickwork_schedule(n, depth):
     return out
 
 
+def haar_gates(seed, count, dim=4):
+    """Haar-random unitaries by QR of complex Ginibre matrices with the phase
+    fix of Mezzadri (Notices AMS 54, 2007); entries (N(0,1) + i N(0,1))/sqrt(2)
+    from numpy's PCG64 stream ``seed`` (the workload's generator)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        z = ((rng.standard_normal((dim, dim)) + 1j * rng.standard_normal((dim, dim))) / np.sqrt(2)).astype(np.complex128)
+        q, r = np.linalg.qr(z)
+        ph = np.diag(r) / np.abs(np.diag(r))
+        out.append(q * ph[None, :])
+    return np.stack(out)
+
+
 def upload_gates(ctx: Context, gates, d=2, dtype=np.complex128):
     """Stack 4x4 gates (site a most significant) as column-major
     G[oa, ob, ia, ib] tensors in one device buffer; returns per-gate views."""
