@@ -118,6 +118,227 @@ def _svd_fl(m, n):
     return 4.0 * (4.0 * p * p * q + 8.0 * p * q * q + 9.0 * q ** 3)
 
 
+# ---------------------------------------------------------------- DMRG H_eff
+HEFF_CHI, HEFF_W, HEFF_SEED = 2048, 5, 20261202 + 3
+HEFF_FORMS = {"fp64": 0, "fp32": 1, "split_bf16": 2, "split_fp16": 3}
+HEFF_SAMPLE_ROWS = 256
+
+
+def heff_flops(chi, w):
+    lt = 8.0 * (chi * w) * (4 * chi) * chi
+    op = 8.0 * (chi * chi) * (w * 4) * (w * 4)
+    rt = 8.0 * (chi * 4) * chi * (w * chi)
+    return lt + op + rt
+
+
+def bench_heff(args):
+    """Two-site effective-Hamiltonian apply (DMRG.jl:10-18, 297-300):
+    L, R (2048, 5, 2048) and theta (2048, 2, 2, 2048) complex128 Gaussians
+    generated on the device (theta normalised), XXZ MPO (J = 1, Delta = 0.5).
+    A step is one apply.  Accuracy: relative l2 error of the first 256
+    output rows against the FP64 oracle on the same rows."""
+    import numpy as np
+    import tenet_hip as T
+    from tenet_hip import _lib as L
+    from tenet_hip.dmrg import heff_apply, xxz_mpo_tensor
+
+    chi, w = HEFF_CHI, HEFF_W
+    ctx = T.Context(0)
+    Ld = T.DeviceArray(ctx, (chi, w, chi)).gaussian_(HEFF_SEED)
+    Rd = T.DeviceArray(ctx, (chi, w, chi)).gaussian_(HEFF_SEED + 1)
+    th = T.DeviceArray(ctx, (chi, 2, 2, chi)).gaussian_(HEFF_SEED + 2)
+    nrm = np.sqrt(th.nrm2sq())
+    th.scale_(1.0 / nrm)
+    Wh = xxz_mpo_tensor(1.0, 0.5)
+    tL = T.Tensor(Ld, ["l", "w", "lk"])
+    tR = T.Tensor(Rd, ["r", "wr", "rk"])
+    tth = T.Tensor(th, ["lk", "i1", "i2", "rk"])
+    tW1 = T.Tensor.from_host(ctx, Wh, ["w", "m", "o1", "i1"])
+    tW2 = T.Tensor.from_host(ctx, Wh, ["m", "wr", "o2", "i2"])
+    flops = heff_flops(chi, w)
+
+    ref_rows = None
+    cpu = None
+    if not args.no_cpu:
+        from oracle import synth
+        from oracle import tenet_oracle as O
+
+        S = HEFF_SAMPLE_ROWS
+        Lh = synth.gaussian(HEFF_SEED, chi * w * chi).reshape(chi, w, chi, order="F")[:S].copy()
+        Rh = synth.gaussian(HEFF_SEED + 1, chi * w * chi).reshape(chi, w, chi, order="F")
+        thh = synth.gaussian(HEFF_SEED + 2, chi * 4 * chi).reshape(chi, 2, 2, chi, order="F") / nrm
+        O.heff_apply(Lh[:8], Wh, Wh, Rh, thh)  # warm BLAS
+        t0 = time.perf_counter()
+        ref_rows = O.heff_apply(Lh, Wh, Wh, Rh, thh)
+        dt = time.perf_counter() - t0
+        threads = os.environ.get("OPENBLAS_NUM_THREADS") or str(os.cpu_count())
+        cpu = {"value": round(heff_flops(chi, w) * S / chi / dt / 1e9, 3), "unit": "GFLOP/s", "cores": int(threads),
+               "kind": "port",
+               "sample": f"oracle H_eff apply on the first {S} of {chi} output rows ({dt:.1f} s, numpy/OpenBLAS, "
+                         f"{threads} threads)"}
+        del Rh, thh
+
+    results = {}
+    for form in [f for f in args.forms.split(",") if f]:
+        prec = HEFF_FORMS[form]
+        for _ in range(args.warmup):
+            y = heff_apply(tL, tW1, tW2, tR, tth, prec=prec)
+            ctx.sync()
+            del y
+        ctx.stats_sample_every(1)
+        ctx.stats_enable(L.KSTAT_GEMM)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            y = heff_apply(tL, tW1, tW2, tR, tth, prec=prec)
+            ctx.sync()
+        wall = (time.perf_counter() - t0) / args.steps
+        kst = ctx.stats_read()
+        ctx.stats_enable(L.KSTAT_OFF)
+        ctx.stats_sample_every(17)
+        rel = None
+        if ref_rows is not None:
+            yh = y.to_host()[:HEFF_SAMPLE_ROWS]
+            rel = float(np.linalg.norm(yh - ref_rows) / np.linalg.norm(ref_rows))
+        del y
+        results[form] = {"ms": wall * 1e3, "relerr": rel, "kstats": kst}
+        log(f"heff {form}: {wall * 1e3:.2f} ms/apply, {flops / wall / 1e12:.2f} TFLOP/s, relerr {rel}")
+
+    main_form = min(results, key=lambda f: results[f]["ms"])
+    r = results[main_form]
+    k = r["kstats"]
+    roof = None
+    if k["launches"] > 0 and k["ms"] > 0:
+        ach = k["flops"] / (k["ms"] * 1e-3) / 1e12
+        peak = PEAK_FP64_TFLOPS if main_form == "fp64" else (157.3 if main_form == "fp32" else 2500.0)
+        roof = {"bound": "mfma", "achieved": round(ach, 4), "peak": peak, "unit": "TFLOP/s",
+                "frac": round(ach / peak, 5), "traffic": None, "kernel": "contraction kernels of the apply",
+                "launches": k["launches"], "avg_launch_us": round(k["ms"] * 1e3 / k["launches"], 2)}
+    line = {
+        "metric": "DMRG two-site H_eff apply throughput (nominal complex flops)",
+        "value": round(flops / (r["ms"] * 1e-3) / 1e9, 3), "unit": "GFLOP/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(r["ms"], 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "c128",
+        "data": "synthetic (device-generated seeded Gaussians)",
+        "config": {"workload": "DMRG H_eff apply L,R (2048,5,2048) theta (2048,2,2,2048) XXZ J=1 Delta=0.5",
+                   "form": main_form, "gflop_nominal": round(flops / 1e9, 1),
+                   "forms": {f: {"ms_per_apply": round(v["ms"], 3), "relerr_vs_fp64_oracle": v["relerr"],
+                                 "tflops": round(flops / (v["ms"] * 1e-3) / 1e12, 3)} for f, v in results.items()}},
+        "roofline": roof, "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
+# ---------------------------------------------------------------- plain small
+PLAIN_N, PLAIN_CHI, PLAIN_SEED = 20, 32, 20261202 + 0
+
+
+def plain_arrays():
+    """n = 20, d = 2, uniform chi = 32 (edges included: site 1 is (1, 2, 32)
+    as a 3-way array), standard complex Gaussian entries from numpy PCG64."""
+    import numpy as np
+
+    rng = np.random.default_rng(PLAIN_SEED)
+    out = []
+    for i in range(PLAIN_N):
+        chil = 1 if i == 0 else PLAIN_CHI
+        chir = 1 if i == PLAIN_N - 1 else PLAIN_CHI
+        shp = (chil, 2, chir)
+        out.append((rng.standard_normal(shp) + 1j * rng.standard_normal(shp)) / np.sqrt(2.0))
+    return out
+
+
+def bench_plain(args):
+    """canonize!(psi, site 1) from MixedCanonical(all sites) (19 QR moves,
+    canonize.jl:128-149), then <psi|psi> by the transfer-matrix sweep
+    (overlap.jl:36-50) and norm(psi) (norm.jl:17-37).  A step is that
+    sequence on a fresh copy of the state (copies prepared untimed);
+    latency-bound, reported per step."""
+    import numpy as np
+    import tenet_hip as T
+    from tenet_hip import _lib as L
+    from tenet_hip.mps import MPS, qr_flops
+
+    arrs = plain_arrays()
+    ctx = T.Context(0)
+    n, chi = PLAIN_N, PLAIN_CHI
+    # nominal work: 19 QR moves (QR of (chi_l d) x chi, R . next site), overlap sweep
+    fl_qr = 0.0
+    for i in range(n - 1, 0, -1):
+        chil, d, chir = arrs[i].shape
+        fl_qr += qr_flops(d * chir, chil) + 8.0 * arrs[i - 1].shape[0] * 2 * chil * min(d * chir, chil)
+    fl_ov = 0.0
+    for a in arrs:
+        chil, d, chir = a.shape
+        fl_ov += 8.0 * chil * chil * d * chir + 8.0 * chil * d * chir * chir
+    states = [MPS.from_host(ctx, arrs) for _ in range(args.warmup + args.steps)]
+    ctx.sync()
+    vals = []
+
+    def one(psi):
+        psi.canonize_(0)
+        v = psi.overlap(psi)
+        nr = psi.norm()
+        return v, nr
+
+    for w in range(args.warmup):
+        one(states[w])
+    ctx.sync()
+    ctx.stats_sample_every(1)
+    ctx.stats_enable(L.KSTAT_QR)
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        vals.append(one(states[args.warmup + s]))
+    wall = (time.perf_counter() - t0) / args.steps
+    kst = ctx.stats_read()
+    ctx.stats_enable(L.KSTAT_OFF)
+    ctx.stats_sample_every(17)
+    ov, nr = vals[-1]
+
+    rel = None
+    cpu = None
+    if not args.no_cpu:
+        from oracle import tenet_oracle as O
+
+        ref = O.overlap_mps(O.OMPS(arrs), O.OMPS(arrs))
+        rel = abs(ov - ref) / abs(ref)
+        reps = 0
+        t0 = time.perf_counter()
+        while True:
+            p = O.OMPS([a.copy() for a in arrs])
+            O.canonize_mixed(p, 0, 0)
+            O.overlap_mps(p, p)
+            O.norm(p)
+            reps += 1
+            if time.perf_counter() - t0 > min(args.cpu_seconds, 10.0):
+                break
+        dt = (time.perf_counter() - t0) / reps
+        threads = os.environ.get("OPENBLAS_NUM_THREADS") or str(os.cpu_count())
+        cpu = {"value": round(dt * 1e6, 1), "unit": "us/step", "cores": int(threads), "kind": "port",
+               "sample": f"oracle canonize!+overlap+norm on the same state, {reps} repetitions (numpy/OpenBLAS, "
+                         f"{threads} threads)"}
+    roof = None
+    if kst["launches"]:
+        ach = kst["flops"] / (kst["ms"] * 1e-3) / 1e12
+        roof = {"bound": "mfma", "achieved": round(ach, 5), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(ach / PEAK_FP64_TFLOPS, 6), "traffic": None,
+                "kernel": "thin QR (one 64x32 / 2x32 Householder QR per move)", "launches": kst["launches"],
+                "avg_launch_us": round(kst["ms"] * 1e3 / kst["launches"], 2),
+                "note": "latency-bound: 19 dependent QR moves of 64x32"}
+    line = {
+        "metric": "plain canonize!+norm latency (n=20, chi=32)", "value": round(wall * 1e6, 1), "unit": "us/step",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall * 1e3, 4),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "c128",
+        "data": "synthetic (seeded Gaussian MPS)",
+        "config": {"workload": "open MPS n=20 d=2 chi=32 Gaussian: canonize!(site 1) + overlap + norm",
+                   "gflop_nominal": round((fl_qr + fl_ov) / 1e9, 4), "overlap": [ov.real, ov.imag], "norm": nr,
+                   "overlap_relerr_vs_fp64_oracle": rel},
+        "roofline": roof, "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -126,11 +347,18 @@ def main():
     ap.add_argument("--forms", default="fp64,split_bf16")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", default="tebd", choices=["tebd", "heff", "plain"])
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: this engine has no multi-GPU path (the TEBD sweep is "
               "sequential in MixedCanonical form); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "heff":
+        if args.forms == "fp64,split_bf16":
+            args.forms = "fp64,fp32,split_bf16,split_fp16"
+        return bench_heff(args)
+    if args.workload == "plain":
+        return bench_plain(args)
 
     import numpy as np
     import tenet_hip as T

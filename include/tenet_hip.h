@@ -191,6 +191,20 @@ int tenet_grouped_svd(tenet_ctx ctx, tenet_stream stream, int dtype, int64_t cou
                       const int64_t* ns, const void* const* As, void* const* Us, double* const* Ss,
                       void* const* Vhs, const int64_t* maxdims, double threshold, int64_t* k_dev, double* disc_dev);
 
+/* ---- elementwise helpers ---------------------------------------------------
+ * Synthetic inputs on the device: element i of dst becomes a standard
+ * complex Gaussian (N(0,1) + i N(0,1))/sqrt(2) drawn from a counter-based
+ * stream (splitmix64 finaliser + Box-Muller) keyed by (seed, offset + i), so
+ * any slice can be regenerated on the host.  Workload generator only (no
+ * reference counterpart). */
+int tenet_fill_gaussian(tenet_ctx ctx, tenet_stream stream, int dtype, int64_t count, void* dst, uint64_t seed,
+                        uint64_t offset);
+/* x *= alpha (normalize!(tensor) at src/Operations/normalize.jl:9,26 scales
+ * the centre tensor this way) */
+int tenet_scale(tenet_ctx ctx, tenet_stream stream, int dtype, int64_t count, void* x, const double alpha[2]);
+/* out (device double) = sum |x_i|^2 in FP64 (norm(tensor), norm.jl:28) */
+int tenet_nrm2sq(tenet_ctx ctx, tenet_stream stream, int dtype, int64_t count, const void* x, double* out);
+
 /* ---- HIP graph capture of a sequence of calls on one stream --------------- */
 int tenet_graph_begin(tenet_ctx ctx, tenet_stream stream);
 int tenet_graph_end(tenet_ctx ctx, tenet_stream stream, tenet_graph* graph);
@@ -198,21 +212,26 @@ int tenet_graph_launch(tenet_graph graph, tenet_stream stream);
 int tenet_graph_destroy(tenet_graph graph);
 
 /* ---- diagnostics -------------------------------------------------------- */
-/* Per-kernel timing: while enabled for a kernel family, every launch of that
- * family is bracketed by HIP events on its stream, and the algorithmic work
- * of the launch is accumulated (flops: complex MAC = 8; bytes: operands
- * read + written once).  tenet_stats_read synchronises, returns the totals
- * since the last read and clears them. */
+/* Per-kernel timing: while enabled for a kernel family, every 17th launch of
+ * that family (see tenet_stats_sample_every) is bracketed by HIP events on
+ * its stream and the algorithmic
+ * work of every launch is accumulated (flops: complex MAC = 8; bytes:
+ * operands read + written once).  tenet_stats_read synchronises and returns
+ * the launch count, the timed total scaled to all launches, and the work
+ * totals since the last read, then clears them. */
 typedef enum {
   TENET_KSTAT_OFF = 0,
   TENET_KSTAT_SVD_ROUND = 1, /* one-sided Jacobi round (Gram, rotations, apply) */
   TENET_KSTAT_GEMM = 2,      /* contraction kernels (all precisions) */
-  TENET_KSTAT_QR_PANEL = 3,  /* Householder panel factorisation */
-  TENET_KSTAT_QR_UPDATE = 4, /* blocked Householder trailing / Q updates */
-  TENET_KSTAT_AMPLITUDE = 5  /* batched amplitude sweep */
+  TENET_KSTAT_QR = 3,        /* one thin QR: pack, panels, trailing and Q updates */
+  TENET_KSTAT_AMPLITUDE = 4  /* batched amplitude sweep (time only: the work
+                                depends on device-side tables) */
 } tenet_kstat;
 int tenet_stats_enable(tenet_ctx ctx, int kernel_family);
 int tenet_stats_read(tenet_ctx ctx, int64_t* launches, double* total_ms, double* flops, double* bytes);
+/* Event-bracket every `every`-th launch of the tracked family (default 17;
+ * 1 times every launch, for short runs of unlike launches). */
+int tenet_stats_sample_every(tenet_ctx ctx, int every);
 /* Average duration (ms) of the last `tenet_timer_*` bracket, measured with
  * HIP events on the given stream. */
 int tenet_timer_start(tenet_ctx ctx, tenet_stream stream);

@@ -703,7 +703,7 @@ def heff_apply(L, W1, W2, R, theta):
     t = np.tensordot(L, theta, axes=([2], [0]))  # (a, w, i1, i2, c)
     op = np.einsum("ab xi, bc yj->a c x y i j".replace(" ", ""), W1, W2, optimize=False)  # (wl, wr, o1, o2, i1, i2)
     t = np.einsum("awijc,wexyij->axyec", t, op, optimize=False)  # (a, o1, o2, wr, c)
-    y = np.einsum("axyec,dec->axyd", t, R, optimize=False)
+    y = np.tensordot(t, R, axes=([3, 4], [1, 2]))  # (a, o1, o2, d)
     return y
 
 

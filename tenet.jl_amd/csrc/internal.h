@@ -131,5 +131,8 @@ int launch_pack_c128(const void* src, int dtype, int64_t m, int64_t n, int64_t l
                      hipStream_t s);
 int launch_apply_pauli(const void* A, int64_t chil, int64_t chir, char which, void* out, int dtype, hipStream_t s);
 int launch_set_identity(void* out, int64_t chi, int dtype, hipStream_t s);
+int launch_gaussian(void* out, int dtype, int64_t count, uint64_t seed, uint64_t offset, hipStream_t s);
+int launch_scale(void* x, int dtype, int64_t count, double ar, double ai, hipStream_t s);
+int launch_nrm2(const void* x, int dtype, int64_t count, double* part, double* out, hipStream_t s);
 
 }  // namespace tenet

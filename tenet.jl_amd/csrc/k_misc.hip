@@ -405,4 +405,97 @@ int launch_set_identity(void* out, int64_t chi, int dtype, hipStream_t s) {
   return TENET_OK;
 }
 
+// Counter-based standard complex Gaussians for synthetic workloads:
+// element i gets u1 = U(h(2i)), u2 = U(h(2i+1)), h(k) = splitmix64 finaliser of
+// (k * 0x9E3779B97F4A7C15 + seed * 0xD1B54A32D192ED03), U(x) = (x >> 11) 2^-53;
+// z = sqrt(-2 ln(1 - u1)) (cos 2 pi u2 + i sin 2 pi u2) / sqrt(2)  (Box-Muller).
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+template <typename S2>
+__global__ void gauss_kernel(S2* __restrict__ out, int64_t count, uint64_t seed, uint64_t offset) {
+  const uint64_t sm = seed * 0xD1B54A32D192ED03ull;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t k = 2ull * ((uint64_t)i + offset);
+    const double u1 = (double)(mix64(k * 0x9E3779B97F4A7C15ull + sm) >> 11) * 0x1p-53;
+    const double u2 = (double)(mix64((k + 1) * 0x9E3779B97F4A7C15ull + sm) >> 11) * 0x1p-53;
+    const double r = sqrt(-2.0 * log(1.0 - u1)) * 0.70710678118654752440;
+    double sn, cs;
+    sincospi(2.0 * u2, &sn, &cs);
+    S2 v;
+    v.x = r * cs;
+    v.y = r * sn;
+    out[i] = v;
+  }
+}
+int launch_gaussian(void* out, int dtype, int64_t count, uint64_t seed, uint64_t offset, hipStream_t s) {
+  if (count <= 0) return TENET_OK;
+  const unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>((count + 255) / 256, 16384));
+  if (dtype == TENET_C128)
+    hipLaunchKernelGGL(gauss_kernel<double2>, dim3(g), dim3(256), 0, s, (double2*)out, count, seed, offset);
+  else
+    hipLaunchKernelGGL(gauss_kernel<float2>, dim3(g), dim3(256), 0, s, (float2*)out, count, seed, offset);
+  TENET_HIP_CHECK(hipGetLastError());
+  return TENET_OK;
+}
+
+// x *= alpha (complex) for count elements
+template <typename S2>
+__global__ void scale_kernel(S2* __restrict__ x, int64_t count, double ar, double ai) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x) {
+    const S2 v = x[i];
+    S2 o;
+    o.x = ar * (double)v.x - ai * (double)v.y;
+    o.y = ar * (double)v.y + ai * (double)v.x;
+    x[i] = o;
+  }
+}
+int launch_scale(void* x, int dtype, int64_t count, double ar, double ai, hipStream_t s) {
+  if (count <= 0) return TENET_OK;
+  const unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>((count + 255) / 256, 16384));
+  if (dtype == TENET_C128)
+    hipLaunchKernelGGL(scale_kernel<double2>, dim3(g), dim3(256), 0, s, (double2*)x, count, ar, ai);
+  else
+    hipLaunchKernelGGL(scale_kernel<float2>, dim3(g), dim3(256), 0, s, (float2*)x, count, ar, ai);
+  TENET_HIP_CHECK(hipGetLastError());
+  return TENET_OK;
+}
+
+// sum |x|^2 in FP64: per-block partials then one block; fixed order
+template <typename S2>
+__global__ __launch_bounds__(256) void nrm2_partial_kernel(const S2* __restrict__ x, int64_t count, double* part) {
+  __shared__ double red[4];
+  double s = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x) {
+    const S2 v = x[i];
+    s += (double)v.x * (double)v.x + (double)v.y * (double)v.y;
+  }
+  s = wave_add(s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+__global__ __launch_bounds__(256) void nrm2_final_kernel(const double* part, int n, double* out) {
+  __shared__ double red[4];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) s += part[i];
+  s = wave_add(s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) out[0] = red[0] + red[1] + red[2] + red[3];
+}
+int launch_nrm2(const void* x, int dtype, int64_t count, double* part, double* out, hipStream_t s) {
+  constexpr int NB = 1024;
+  if (dtype == TENET_C128)
+    hipLaunchKernelGGL(nrm2_partial_kernel<double2>, dim3(NB), dim3(256), 0, s, (const double2*)x, count, part);
+  else
+    hipLaunchKernelGGL(nrm2_partial_kernel<float2>, dim3(NB), dim3(256), 0, s, (const float2*)x, count, part);
+  TENET_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(nrm2_final_kernel, dim3(1), dim3(256), 0, s, part, NB, out);
+  TENET_HIP_CHECK(hipGetLastError());
+  return TENET_OK;
+}
+
 }  // namespace tenet

@@ -5,17 +5,20 @@
 //
 // Every row of A (over k) and every column of B gets a power-of-two scale
 // 2^e with max(|Re|, |Im|) < 2^e.  The scaled value x = v 2^-e in (-1, 1) is
-// cut into four 8-bit integer slices,
-//     x 2^8 = s1 + 2^-8 s2 + 2^-16 s3 + 2^-24 s4 + O(2^-25),  |s_i| <= 256,
+// cut into S 8-bit integer slices (S = 6 for complex128 results, 3 for
+// complex64),
+//     x 2^8 = s1 + 2^-8 s2 + ... + 2^-8(S-1) sS + O(2^-(8S+1)),  |s_i| <= 256,
 // each exact in bf16 (8 significant bits) and fp16.  A product of two slices
 // is an integer below 2^16 in magnitude, so a K block of 128 complex terms
 // (256 real products per accumulator) sums exactly in FP32 (< 2^24): the
-// dominant level s1*t1 is error free.  Levels i + j = 3, 4, 5 are
-// accumulated separately and all K blocks are summed in FP64:
-//     C = 2^(e_m + f_n - 16) sum_{l=2..5} 2^(-8 (l-2)) sum_{i+j=l} S_i T_j.
-// Dropped terms (i + j > 5 and the slice remainders) are ~2^-33 relative to
-// the row / column maxima, well inside a 1e-8 target; 10 slice products of 4
-// real MFMAs each replace one complex FP64 product.
+// two leading levels (i + j = 2, 3) are error free, later levels carry
+// weights <= 2^-16.  Each level is accumulated separately and all K blocks
+// are summed in FP64:
+//     C = 2^(e_m + f_n - 16) sum_{l} 2^(-8 (l-2)) sum_{i+j=l} S_i T_j,
+// keeping the S(S+1)/2 products with i + j <= S + 1.  The dropped terms and
+// slice remainders are ~2^-(8S+1) relative to the row / column maxima:
+// 2^-49 (FP64 grade, needed where a TEBD sweep amplifies contraction error
+// by ~1e5) for S = 6, 2^-25 (FP32 grade) for S = 3.
 #include "../../include/tenet_hip.h"
 #include "internal.h"
 
@@ -26,7 +29,7 @@ typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 typedef unsigned int u4 __attribute__((ext_vector_type(4)));
 
-constexpr int NSL = 4;    // slices
+constexpr int NSL_MAX = 6;  // slices for complex128 results
 constexpr int SKB = 128;  // exact-accumulation K block
 constexpr int SKP = 32;   // MFMA K
 
@@ -40,8 +43,8 @@ static SplitLayout split_layout(const GemmLaunch& g) {
   L.Mp = (g.M + 63) / 64 * 64;
   L.Np = (g.N + 31) / 32 * 32;
   L.Kp = (g.K + SKB - 1) / SKB * SKB;
-  L.a_bytes = (size_t)g.batch * NSL * 2 * L.Mp * L.Kp * 2;
-  L.b_bytes = (size_t)g.batch * NSL * 2 * L.Np * L.Kp * 2;
+  L.a_bytes = (size_t)g.batch * NSL_MAX * 2 * L.Mp * L.Kp * 2;
+  L.b_bytes = (size_t)g.batch * NSL_MAX * 2 * L.Np * L.Kp * 2;
   L.ea_bytes = (size_t)g.batch * L.Mp * 4;
   L.eb_bytes = (size_t)g.batch * L.Np * 4;
   return L;
@@ -71,7 +74,7 @@ __device__ __forceinline__ unsigned short to_bits<_Float16>(double v) {
 
 // One wave per packed row (A: row m over k; B: column n over k).  out layout:
 // [z][slice][part][row][Kp] of 16-bit codes; exps[z][row] = e.
-template <typename S2, typename T>
+template <typename S2, typename T, int NSL>
 __global__ __launch_bounds__(256) void split_pack_kernel(const S2* __restrict__ src, const int64_t* __restrict__ tfix,
                                                          const int64_t* __restrict__ tk,
                                                          const int64_t* __restrict__ tb, int64_t R, int64_t K,
@@ -147,8 +150,9 @@ struct SplitArgs {
 // 64 x 32 output tile per workgroup; wave w owns rows 16w..16w+15 and two
 // 16-column blocks.  Operand fragments come straight from the packed planes
 // (16-byte loads: 8 consecutive k of one row / column per lane).
-template <typename S2, typename T>
+template <typename S2, typename T, int NSL>
 __global__ __launch_bounds__(256) void split_gemm_kernel(SplitArgs a) {
+  constexpr int NLEV = NSL;  // levels i + j = 0 .. NSL-1 (0-based slices)
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int64_t per = a.tiles_m * a.tiles_n;
   const int64_t tile = blockIdx.x;
@@ -169,9 +173,9 @@ __global__ __launch_bounds__(256) void split_gemm_kernel(SplitArgs a) {
     }
   const u4 neg = {0x80008000u, 0x80008000u, 0x80008000u, 0x80008000u};
   for (int64_t kb = 0; kb < a.Kp; kb += SKB) {
-    f4s accr[4][2], acci[4][2];
+    f4s accr[NLEV][2], acci[NLEV][2];
 #pragma unroll
-    for (int l = 0; l < 4; ++l)
+    for (int l = 0; l < NLEV; ++l)
 #pragma unroll
       for (int cb = 0; cb < 2; ++cb) {
         accr[l][cb] = f4s{0.f, 0.f, 0.f, 0.f};
@@ -213,10 +217,15 @@ __global__ __launch_bounds__(256) void split_gemm_kernel(SplitArgs a) {
     for (int cb = 0; cb < 2; ++cb)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        totr[cb][r] += (double)accr[0][cb][r] + 0x1p-8 * (double)accr[1][cb][r] + 0x1p-16 * (double)accr[2][cb][r] +
-                       0x1p-24 * (double)accr[3][cb][r];
-        toti[cb][r] += (double)acci[0][cb][r] + 0x1p-8 * (double)acci[1][cb][r] + 0x1p-16 * (double)acci[2][cb][r] +
-                       0x1p-24 * (double)acci[3][cb][r];
+        double sr = 0.0, si = 0.0, w = 1.0;
+#pragma unroll
+        for (int l = 0; l < NLEV; ++l) {
+          sr += w * (double)accr[l][cb][r];
+          si += w * (double)acci[l][cb][r];
+          w *= 0x1p-8;
+        }
+        totr[cb][r] += sr;
+        toti[cb][r] += si;
       }
   }
   // epilogue: D layout rows 4*(lane/16) + r, column lane % 16
@@ -250,7 +259,7 @@ __global__ __launch_bounds__(256) void split_gemm_kernel(SplitArgs a) {
   }
 }
 
-template <typename S2, typename T>
+template <typename S2, typename T, int NSL>
 static int run_split(const GemmLaunch& g, hipStream_t s, void* ws) {
   const SplitLayout L = split_layout(g);
   char* w = static_cast<char*>(ws);
@@ -263,14 +272,14 @@ static int run_split(const GemmLaunch& g, hipStream_t s, void* ws) {
   int* eb = reinterpret_cast<int*>(w);
   {
     const int64_t rows = L.Mp * g.batch;
-    hipLaunchKernelGGL((split_pack_kernel<S2, T>), dim3((unsigned)((rows + 3) / 4)), dim3(256), 0, s,
+    hipLaunchKernelGGL((split_pack_kernel<S2, T, NSL>), dim3((unsigned)((rows + 3) / 4)), dim3(256), 0, s,
                        static_cast<const S2*>(g.A), g.t.rowA, g.t.kA, g.t.bA, g.M, g.K, L.Mp, L.Kp, g.batch, g.conjA,
                        Ap, ea);
     TENET_HIP_CHECK(hipGetLastError());
   }
   {
     const int64_t rows = L.Np * g.batch;
-    hipLaunchKernelGGL((split_pack_kernel<S2, T>), dim3((unsigned)((rows + 3) / 4)), dim3(256), 0, s,
+    hipLaunchKernelGGL((split_pack_kernel<S2, T, NSL>), dim3((unsigned)((rows + 3) / 4)), dim3(256), 0, s,
                        static_cast<const S2*>(g.B), g.t.colB, g.t.kB, g.t.bB, g.N, g.K, L.Np, L.Kp, g.batch, g.conjB,
                        Bp, eb);
     TENET_HIP_CHECK(hipGetLastError());
@@ -301,7 +310,7 @@ static int run_split(const GemmLaunch& g, hipStream_t s, void* ws) {
     set_error("split contraction too large for one launch");
     return TENET_ERR_UNSUPPORTED;
   }
-  hipLaunchKernelGGL((split_gemm_kernel<S2, T>), dim3((unsigned)total), dim3(256), 0, s, a);
+  hipLaunchKernelGGL((split_gemm_kernel<S2, T, NSL>), dim3((unsigned)total), dim3(256), 0, s, a);
   TENET_HIP_CHECK(hipGetLastError());
   return TENET_OK;
 }
@@ -316,8 +325,9 @@ int launch_split_gemm(const GemmLaunch& g, int dtype, int prec, hipStream_t s, v
     return TENET_ERR_UNSUPPORTED;
   }
   if (dtype == TENET_C128)
-    return prec == TENET_PREC_SPLIT_BF16 ? run_split<double2, __bf16>(g, s, ws) : run_split<double2, _Float16>(g, s, ws);
-  return prec == TENET_PREC_SPLIT_BF16 ? run_split<float2, __bf16>(g, s, ws) : run_split<float2, _Float16>(g, s, ws);
+    return prec == TENET_PREC_SPLIT_BF16 ? run_split<double2, __bf16, 6>(g, s, ws)
+                                         : run_split<double2, _Float16, 6>(g, s, ws);
+  return prec == TENET_PREC_SPLIT_BF16 ? run_split<float2, __bf16, 3>(g, s, ws) : run_split<float2, _Float16, 3>(g, s, ws);
 }
 
 }  // namespace tenet

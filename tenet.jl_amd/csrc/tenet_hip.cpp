@@ -80,9 +80,12 @@ struct tenet_ctx_s {
   int stat_family = 0;
   std::vector<hipEvent_t> stat_events;  // pairs
   size_t stat_used = 0;
-  int64_t stat_launches = 0;
+  int64_t stat_launches = 0, stat_sampled = 0, stat_every = 17;
   double stat_ms = 0.0, stat_flops = 0.0, stat_bytes = 0.0;
 };
+// Every stat_every-th launch of the tracked family is bracketed by events
+// (default 17: an odd stride, so the two alternating phases of a Jacobi round
+// are both sampled); the timed total is scaled by launches / sampled.
 
 // Drain recorded event pairs into the running totals (synchronises).
 static int stats_drain(tenet_ctx ctx) {
@@ -105,6 +108,10 @@ struct StatScope {
     if (!ctx || ctx->stat_family != family || family == 0) return;
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     if (hipStreamIsCapturing(s, &cs) == hipSuccess && cs == hipStreamCaptureStatusActive) return;
+    ctx->stat_launches += 1;
+    ctx->stat_flops += flops;
+    ctx->stat_bytes += bytes;
+    if ((ctx->stat_launches - 1) % ctx->stat_every != 0) return;
     if (ctx->stat_used + 2 > ctx->stat_events.size()) {
       if (ctx->stat_events.size() < 8192) {
         for (int i = 0; i < 512; ++i) {
@@ -118,9 +125,7 @@ struct StatScope {
     }
     if (hipEventRecord(ctx->stat_events[ctx->stat_used], s) != hipSuccess) return;
     on = true;
-    ctx->stat_launches += 1;
-    ctx->stat_flops += flops;
-    ctx->stat_bytes += bytes;
+    ctx->stat_sampled += 1;
   }
   ~StatScope() {
     if (!on) return;
@@ -773,11 +778,24 @@ static size_t qr_ws_bytes(int64_t m, int64_t n) {
                   qr_blocked_ws_bytes(m, n) + 256);
 }
 
+static int qr_run_inner(hipStream_t s, int dtype, int flags, int64_t m, int64_t n, const void* A, int64_t lda,
+                        void* Q, int64_t ldq, void* R, int64_t ldr, Bump& b, int64_t k);
+
 static int qr_run(tenet_ctx ctx, hipStream_t s, int dtype, int flags, int64_t m, int64_t n, const void* A,
                   int64_t lda, void* Q, int64_t ldq, void* R, int64_t ldr, void* wsp) {
-  (void)ctx;
   Bump b(wsp);
   const int64_t k = std::min(m, n);
+  {
+    const double p = (double)std::max(m, n), q = (double)k;
+    StatScope st(ctx, s, TENET_KSTAT_QR, 4.0 * (4.0 * p * q * q - 4.0 * q * q * q / 3.0),
+                 (double)ELEM[dtype] * ((double)m * n + (double)m * k + (double)k * n));
+    TRY(qr_run_inner(s, dtype, flags, m, n, A, lda, Q, ldq, R, ldr, b, k));
+  }
+  return TENET_OK;
+}
+
+static int qr_run_inner(hipStream_t s, int dtype, int flags, int64_t m, int64_t n, const void* A, int64_t lda,
+                        void* Q, int64_t ldq, void* R, int64_t ldr, Bump& b, int64_t k) {
   double* W = (double*)b.take((size_t)m * n * 16 + (size_t)m * k * 16);
   double* tau = (double*)b.take((size_t)k * 16);
   TRY(launch_pack_c128(A, dtype, m, n, lda, (flags & TENET_QR_IN_ADJ) ? 1 : 0, W, s));
@@ -1078,7 +1096,9 @@ extern "C" int tenet_amplitudes(tenet_ctx ctx, tenet_stream stream, int dtype, i
   CHECK_ARG(nreq >= 0 && d >= 1, "bad argument");
   if (nreq == 0) return TENET_OK;
   CHECK_ARG(n_dev && site_ptrs_dev && chis_dev && bits_dev && out, "null pointer");
-  return launch_amplitudes(nreq, n_dev, site_ptrs_dev, chis_dev, bits_dev, d, out, dtype, 2048, pick(ctx, stream));
+  hipStream_t s = pick(ctx, stream);
+  StatScope st(ctx, s, TENET_KSTAT_AMPLITUDE, 0.0, 0.0);
+  return launch_amplitudes(nreq, n_dev, site_ptrs_dev, chis_dev, bits_dev, d, out, dtype, 2048, s);
 }
 
 extern "C" int tenet_expect_pauli(tenet_ctx ctx, tenet_stream stream, int dtype, int prec, int64_t start, int64_t w,
@@ -1265,6 +1285,42 @@ extern "C" int tenet_grouped_svd(tenet_ctx ctx, tenet_stream stream, int dtype, 
 }
 
 // ---------------------------------------------------------------------------
+// elementwise helpers
+// ---------------------------------------------------------------------------
+extern "C" int tenet_fill_gaussian(tenet_ctx ctx, tenet_stream stream, int dtype, int64_t count, void* dst,
+                                   uint64_t seed, uint64_t offset) {
+  TRY(use_device(ctx));
+  TRY(valid_dtype_prec(dtype, 0));
+  CHECK_ARG(count >= 0, "negative count");
+  if (count == 0) return TENET_OK;
+  TRY(check_span(ctx, dst, (size_t)count * ELEM[dtype], "dst"));
+  return launch_gaussian(dst, dtype, count, seed, offset, pick(ctx, stream));
+}
+
+extern "C" int tenet_scale(tenet_ctx ctx, tenet_stream stream, int dtype, int64_t count, void* x,
+                           const double alpha[2]) {
+  TRY(use_device(ctx));
+  TRY(valid_dtype_prec(dtype, 0));
+  CHECK_ARG(count >= 0 && alpha, "bad argument");
+  if (count == 0) return TENET_OK;
+  TRY(check_span(ctx, x, (size_t)count * ELEM[dtype], "x"));
+  return launch_scale(x, dtype, count, alpha[0], alpha[1], pick(ctx, stream));
+}
+
+extern "C" int tenet_nrm2sq(tenet_ctx ctx, tenet_stream stream, int dtype, int64_t count, const void* x,
+                            double* out) {
+  TRY(use_device(ctx));
+  TRY(valid_dtype_prec(dtype, 0));
+  CHECK_ARG(count >= 0 && out, "bad argument");
+  hipStream_t s = pick(ctx, stream);
+  if (count == 0) return launch_fill_zero(out, sizeof(double), s);
+  TRY(check_span(ctx, x, (size_t)count * ELEM[dtype], "x"));
+  void* w = nullptr;
+  TRY(ws_get(ctx, s, 1024 * sizeof(double) + 256, &w));
+  return launch_nrm2(x, dtype, count, (double*)w, out, s);
+}
+
+// ---------------------------------------------------------------------------
 // graphs and timers
 // ---------------------------------------------------------------------------
 extern "C" int tenet_graph_begin(tenet_ctx ctx, tenet_stream stream) {
@@ -1314,8 +1370,16 @@ extern "C" int tenet_stats_enable(tenet_ctx ctx, int family) {
   CHECK_ARG(family >= TENET_KSTAT_OFF && family <= TENET_KSTAT_AMPLITUDE, "unknown kernel family");
   TRY(stats_drain(ctx));
   ctx->stat_family = family;
-  ctx->stat_launches = 0;
+  ctx->stat_launches = ctx->stat_sampled = 0;
   ctx->stat_ms = ctx->stat_flops = ctx->stat_bytes = 0.0;
+  return TENET_OK;
+}
+
+extern "C" int tenet_stats_sample_every(tenet_ctx ctx, int every) {
+  TRY(use_device(ctx));
+  CHECK_ARG(every >= 1, "sampling stride must be >= 1");
+  TRY(stats_drain(ctx));
+  ctx->stat_every = every;
   return TENET_OK;
 }
 
@@ -1323,10 +1387,11 @@ extern "C" int tenet_stats_read(tenet_ctx ctx, int64_t* launches, double* total_
   TRY(use_device(ctx));
   TRY(stats_drain(ctx));
   if (launches) *launches = ctx->stat_launches;
-  if (total_ms) *total_ms = ctx->stat_ms;
+  if (total_ms)
+    *total_ms = ctx->stat_sampled ? ctx->stat_ms * (double)ctx->stat_launches / (double)ctx->stat_sampled : 0.0;
   if (flops) *flops = ctx->stat_flops;
   if (bytes) *bytes = ctx->stat_bytes;
-  ctx->stat_launches = 0;
+  ctx->stat_launches = ctx->stat_sampled = 0;
   ctx->stat_ms = ctx->stat_flops = ctx->stat_bytes = 0.0;
   return TENET_OK;
 }

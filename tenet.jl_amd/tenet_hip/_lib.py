@@ -29,7 +29,7 @@ C128, C64 = 0, 1
 PREC_NATIVE, PREC_FP32, PREC_SPLIT_BF16, PREC_SPLIT_FP16 = 0, 1, 2, 3
 ABSORB_NONE, ABSORB_EQUAL, ABSORB_LEFT, ABSORB_RIGHT = 0, 1, 2, 3
 QR_IN_ADJ, QR_Q_ADJ, QR_R_ADJ = 1, 2, 4
-KSTAT_OFF, KSTAT_SVD_ROUND, KSTAT_GEMM, KSTAT_QR_PANEL, KSTAT_QR_UPDATE, KSTAT_AMPLITUDE = 0, 1, 2, 3, 4, 5
+KSTAT_OFF, KSTAT_SVD_ROUND, KSTAT_GEMM, KSTAT_QR, KSTAT_AMPLITUDE = 0, 1, 2, 3, 4
 MAX_RANK = 8
 
 
@@ -98,12 +98,16 @@ SIGNATURES = {
     "tenet_grouped_einsum": [_p, _p, _i, _i, _i64, C.POINTER(TDesc), C.POINTER(_p)],
     "tenet_grouped_svd": [_p, _p, _i, _i64, _pi64, _pi64, C.POINTER(_p), C.POINTER(_p), C.POINTER(_p),
                           C.POINTER(_p), _pi64, _d, _p, _p],
+    "tenet_fill_gaussian": [_p, _p, _i, _i64, _p, C.c_uint64, C.c_uint64],
+    "tenet_scale": [_p, _p, _i, _i64, _p, _pd],
+    "tenet_nrm2sq": [_p, _p, _i, _i64, _p, _p],
     "tenet_graph_begin": [_p, _p],
     "tenet_graph_end": [_p, _p, C.POINTER(_p)],
     "tenet_graph_launch": [_p, _p],
     "tenet_graph_destroy": [_p],
     "tenet_stats_enable": [_p, _i],
     "tenet_stats_read": [_p, _pi64, _pd, _pd, _pd],
+    "tenet_stats_sample_every": [_p, _i],
     "tenet_timer_start": [_p, _p],
     "tenet_timer_stop": [_p, _p, C.POINTER(C.c_float)],
 }

@@ -45,6 +45,13 @@ class Context:
         if ptr and not self._closed:
             L.check(L.lib.tenet_free(self.h, C.c_void_p(ptr)), "tenet_free")
 
+    def scalar_buffer(self) -> int:
+        """A small device buffer (64 doubles) owned by the context, for
+        scalar results read back right away."""
+        if getattr(self, "_scalar", None) is None:
+            self._scalar = self.malloc(512)
+        return self._scalar
+
     def stream(self):
         s = C.c_void_p()
         L.check(L.lib.tenet_stream_create(self.h, C.byref(s)), "stream_create")
@@ -52,6 +59,9 @@ class Context:
 
     def stats_enable(self, family):
         L.check(L.lib.tenet_stats_enable(self.h, int(family)), "stats_enable")
+
+    def stats_sample_every(self, every: int):
+        L.check(L.lib.tenet_stats_sample_every(self.h, int(every)), "stats_sample_every")
 
     def stats_read(self):
         n, ms, fl, by = C.c_int64(), C.c_double(), C.c_double(), C.c_double()
@@ -117,6 +127,29 @@ class DeviceArray:
     def zero_(self):
         L.check(L.lib.tenet_memset_zero(self.ctx.h, C.c_void_p(self.ptr), C.c_size_t(self.nbytes), None))
         return self
+
+    def gaussian_(self, seed: int, offset: int = 0, stream=None):
+        """Fill with the library's seeded standard complex Gaussians
+        (element i keyed by (seed, offset + i)); nothing crosses PCIe."""
+        L.check(L.lib.tenet_fill_gaussian(self.ctx.h, stream, self.code, self.size, C.c_void_p(self.ptr),
+                                          C.c_uint64(seed), C.c_uint64(offset)), "fill_gaussian")
+        return self
+
+    def scale_(self, alpha, stream=None):
+        al = (C.c_double * 2)(float(np.real(alpha)), float(np.imag(alpha)))
+        L.check(L.lib.tenet_scale(self.ctx.h, stream, self.code, self.size, C.c_void_p(self.ptr), al), "scale")
+        return self
+
+    def nrm2sq(self, stream=None) -> float:
+        """sum |x|^2 accumulated in FP64 on the device."""
+        out = self.ctx.scalar_buffer()
+        L.check(L.lib.tenet_nrm2sq(self.ctx.h, stream, self.code, self.size, C.c_void_p(self.ptr),
+                                   C.c_void_p(out)), "nrm2sq")
+        v = np.zeros(1)
+        L.check(L.lib.tenet_memcpy_d2h(self.ctx.h, v.ctypes.data_as(C.c_void_p), C.c_void_p(out), C.c_size_t(8),
+                                       stream), "d2h")
+        self.ctx.sync(stream)
+        return float(v[0])
 
     def copy(self):
         d = DeviceArray(self.ctx, self.shape, self.dtype)

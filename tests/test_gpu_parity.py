@@ -396,3 +396,82 @@ def test_einsum_split_integer_exact(ctx):
     b = rng.integers(-100, 101, (300, 50)) + 1j * rng.integers(-100, 101, (300, 50))
     got = T.binary_einsum(T.Tensor.from_host(ctx, a, ["i", "k"]), T.Tensor.from_host(ctx, b, ["k", "j"]), prec=2)
     assert np.array_equal(got.to_host(), a @ b)
+
+
+# ---------------------------------------------------------------- DMRG H_eff
+@pytest.mark.parametrize("prec,tol", [(0, 1e-13), (1, 1e-5), (2, 1e-12), (3, 1e-12)])
+def test_heff_apply_vs_oracle(ctx, prec, tol):
+    """EffectiveHamiltonian apply (DMRG.jl:16-18) on an XXZ MPO with random
+    environments, every precision against the FP64 oracle."""
+    import tenet_hip as T
+    from tenet_hip.dmrg import heff_apply, xxz_mpo_tensor
+
+    rng = np.random.default_rng(77)
+    chi = 24
+    W = xxz_mpo_tensor(1.0, 0.5)
+    assert np.allclose(W, O.xxz_mpo_W(1.0, 0.5))
+    Lh = O.complex_gaussian(rng, (chi, 5, chi))
+    Rh = O.complex_gaussian(rng, (chi, 5, chi))
+    th = O.complex_gaussian(rng, (chi, 2, 2, chi))
+    th /= np.linalg.norm(th)
+    ref = O.heff_apply(Lh, W, W, Rh, th)
+    y = heff_apply(T.Tensor.from_host(ctx, Lh, ["l", "w", "lk"]), T.Tensor.from_host(ctx, W, ["w", "m", "o1", "i1"]),
+                   T.Tensor.from_host(ctx, W, ["m", "wr", "o2", "i2"]), T.Tensor.from_host(ctx, Rh, ["r", "wr", "rk"]),
+                   T.Tensor.from_host(ctx, th, ["lk", "i1", "i2", "rk"]), prec=prec)
+    assert y.labels == ["l", "o1", "o2", "r"]
+    assert _rel(y.to_host(), ref) < tol
+
+
+def test_heff_ising_spectrum(ctx):
+    """The Ising ground energy pinned by test/unit/dmrg.jl:38 enters through
+    the MPO; on a 4-site chain with unitary outer sites H_eff has exactly the
+    spectrum of H (eigvalsh of the 16 x 16 H_eff built column by column)."""
+    import tenet_hip as T
+    from tenet_hip.dmrg import heff_apply
+
+    W = O.ising_mpo_W(1.0, 1.0)
+    lv = np.zeros(3); lv[2] = 1
+    rv = np.zeros(3); rv[0] = 1
+    rng = np.random.default_rng(2)
+    A1, A4 = O.haar_unitary(rng, 2), O.haar_unitary(rng, 2)
+    lW = np.einsum("l,lroi->roi", lv, W)
+    rW = np.einsum("lroi,r->loi", W, rv)
+    Lenv = np.einsum("oa,woi,ib->awb", A1.conj(), lW, A1)
+    Renv = np.einsum("do,woi,ci->dwc", A4.conj(), rW, A4)
+    M = np.zeros((16, 16), complex)
+    tl = [T.Tensor.from_host(ctx, Lenv, ["l", "w", "lk"]), T.Tensor.from_host(ctx, W, ["w", "m", "o1", "i1"]),
+          T.Tensor.from_host(ctx, W, ["m", "wr", "o2", "i2"]), T.Tensor.from_host(ctx, Renv, ["r", "wr", "rk"])]
+    for col in range(16):
+        e = np.zeros(16, complex); e[col] = 1
+        y = heff_apply(*tl, T.Tensor.from_host(ctx, e.reshape(2, 2, 2, 2, order="F"), ["lk", "i1", "i2", "rk"]))
+        M[:, col] = y.to_host().reshape(-1, order="F")
+    H = O.mpo_dense([W] * 4, lv, rv)
+    assert np.allclose(np.linalg.eigvalsh(M), np.linalg.eigvalsh(H), atol=1e-12)
+
+
+# ---------------------------------------------------------------- synthetic inputs and helpers
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+def test_fill_gaussian_matches_host_mirror(ctx, dtype):
+    """tenet_fill_gaussian against oracle/synth.py element for element,
+    including an offset slice of the same stream."""
+    import tenet_hip as T
+    from oracle import synth
+
+    d = T.DeviceArray(ctx, (1000, 37), dtype).gaussian_(11)
+    ref = synth.gaussian(11, 37000, dtype=dtype).reshape(1000, 37, order="F")
+    tol = 1e-13 if dtype == np.complex128 else 1e-6
+    assert _rel(d.to_host(), ref) < tol
+    s = T.DeviceArray(ctx, (500,), dtype).gaussian_(11, offset=1234)
+    assert _rel(s.to_host(), ref.reshape(-1, order="F")[1234:1734]) < tol
+
+
+def test_scale_and_nrm2sq(ctx):
+    import tenet_hip as T
+
+    rng = np.random.default_rng(5)
+    a = O.complex_gaussian(rng, (333, 77))
+    d = T.DeviceArray.from_host(ctx, a)
+    assert abs(d.nrm2sq() - np.linalg.norm(a) ** 2) < 1e-12 * np.linalg.norm(a) ** 2
+    d.scale_(0.5 - 2.0j)
+    assert _rel(d.to_host(), (0.5 - 2.0j) * a) < 1e-15
+    assert T.DeviceArray(ctx, (0,)).nrm2sq() == 0.0
