@@ -229,6 +229,291 @@ def bench_heff(args):
     ctx.close()
 
 
+# ---------------------------------------------------------------- grouped batch
+GROUPED_COUNT, GROUPED_SEED = 8192, 20261202 + 2
+
+
+def bench_grouped(args):
+    """8192 uneven pairwise contractions (ranks 3-5, extents log-uniform
+    2..128, 1-3 contracted labels, operands and result <= 2^20 elements,
+    complex64 Gaussians generated on the device) in one persistent grouped
+    launch, plus 8192 truncated SVDs (sides log-uniform 16..512, complex64
+    input, FP64 factorisation, maxdim = ceil(min(m, n) / 2), threshold 1e-10)
+    in one grouped SVD.  A step is both batches; inputs stay resident."""
+    import ctypes as C
+
+    import numpy as np
+    import tenet_hip as T
+    from tenet_hip import _lib as L
+    from tenet_hip.grouped import GroupedPlan, autotune_tiles, grouped_svd
+    from tenet_hip.workloads import contraction_work, grouped_contractions, grouped_svds, svd_work
+
+    ctx = T.Context(0)
+    t0 = time.perf_counter()
+    specs = grouped_contractions(GROUPED_SEED, GROUPED_COUNT)
+    fl_c, by_c = contraction_work(specs)
+    sizes = [sp.sizes for sp in specs]
+    in_off, out_off = [], []
+    ai = ao = 0
+    for a, b, c in sizes:
+        in_off.append((ai, ai + a))
+        ai += a + b
+        out_off.append(ao)
+        ao += c
+    arena_in = T.DeviceArray(ctx, (ai,), np.complex64).gaussian_(GROUPED_SEED)
+    arena_out = T.DeviceArray(ctx, (ao,), np.complex64)
+    costs = autotune_tiles(ctx, np.complex64)
+    probs = [(sp.shape(sp.la), sp.la, sp.shape(sp.lb), sp.lb, sp.shape(sp.lc), sp.lc) for sp in specs]
+    plan = GroupedPlan(ctx, probs, np.complex64)
+    info = plan.info()
+    ptrs = np.zeros(3 * len(specs), dtype=np.uint64)
+    for i, ((oa, ob), oc) in enumerate(zip(in_off, out_off)):
+        ptrs[3 * i] = arena_in.ptr + 8 * oa
+        ptrs[3 * i + 1] = arena_in.ptr + 8 * ob
+        ptrs[3 * i + 2] = arena_out.ptr + 8 * oc
+    ptrs_dev = T.DeviceArray(ctx, (len(ptrs) // 2 + 1,), np.complex128)
+    L.check(L.lib.tenet_memcpy_h2d(ctx.h, C.c_void_p(ptrs_dev.ptr), ptrs.ctypes.data_as(C.c_void_p), ptrs.nbytes,
+                                   None))
+    log(f"grouped: plan built in {time.perf_counter() - t0:.1f}s, {info}, tile costs {costs}")
+
+    ms, ns, maxdims = grouped_svds(GROUPED_SEED + 1000, GROUPED_COUNT)
+    fl_s = svd_work(ms, ns)
+    svd_in = T.DeviceArray(ctx, (int(np.sum(ms * ns)),), np.complex64).gaussian_(GROUPED_SEED + 1000)
+    mats = []
+    o = 0
+    for m, n in zip(ms, ns):
+        v = T.DeviceArray(ctx, (int(m), int(n)), np.complex64, ptr=svd_in.ptr + 8 * o, owner=False)
+        v._base = svd_in
+        mats.append(v)
+        o += int(m * n)
+
+    def contract():
+        plan.exec_dev(ptrs_dev.ptr)
+        ctx.sync()
+
+    for _ in range(args.warmup):
+        contract()
+    ctx.stats_sample_every(1)
+    ctx.stats_enable(L.KSTAT_GEMM)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        contract()
+    t_c = (time.perf_counter() - t0) / args.steps
+    kst = ctx.stats_read()
+    ctx.stats_enable(L.KSTAT_OFF)
+    log(f"grouped contractions: {t_c * 1e3:.2f} ms, {fl_c / t_c / 1e12:.2f} TFLOP/s, {by_c / t_c / 1e9:.0f} GB/s")
+
+    svd_res = None
+    ctx.stats_enable(L.KSTAT_SVD_ROUND)
+    t0 = time.perf_counter()
+    for _ in range(max(1, args.steps)):
+        svd_res = grouped_svd(ctx, mats, maxdims, 1e-10)
+        ctx.sync()
+    t_s = (time.perf_counter() - t0) / max(1, args.steps)
+    kst_s = ctx.stats_read()
+    ctx.stats_enable(L.KSTAT_OFF)
+    ctx.stats_sample_every(17)
+    log(f"grouped SVDs: {t_s * 1e3:.1f} ms, {fl_s / t_s / 1e12:.2f} TFLOP/s nominal")
+
+    # accuracy on a sample, against the FP64 oracle on regenerated inputs
+    rel_c = rel_s = None
+    cpu = None
+    if not args.no_cpu:
+        from oracle import synth
+        from oracle import tenet_oracle as O
+
+        rng = np.random.default_rng(1)
+        errs = []
+        out_h = None
+        for i in rng.choice(len(specs), 48, replace=False):
+            sp = specs[int(i)]
+            a, b, c = sizes[int(i)]
+            oa, ob = in_off[int(i)]
+            A = synth.gaussian(GROUPED_SEED, a, oa).reshape(sp.shape(sp.la), order="F")
+            B = synth.gaussian(GROUPED_SEED, b, ob).reshape(sp.shape(sp.lb), order="F")
+            ref = O.binary_einsum(O.OTensor(A, sp.la), O.OTensor(B, sp.lb), blas=True).permute(sp.lc).data
+            got = np.empty(c, dtype=np.complex64)
+            L.check(L.lib.tenet_memcpy_d2h(ctx.h, got.ctypes.data_as(C.c_void_p),
+                                           C.c_void_p(arena_out.ptr + 8 * out_off[int(i)]), got.nbytes, None))
+            errs.append(np.linalg.norm(got.reshape(sp.shape(sp.lc), order="F") - ref) / np.linalg.norm(ref))
+        rel_c = float(max(errs))
+        serr = []
+        offs = np.concatenate([[0], np.cumsum(ms * ns)])
+        for i in rng.choice(len(ms), 16, replace=False):
+            m, n = int(ms[i]), int(ns[i])
+            Ah = synth.gaussian(GROUPED_SEED + 1000, m * n, int(offs[i])).reshape(m, n, order="F")
+            s_ref = np.linalg.svd(Ah.astype(np.complex128), compute_uv=False)
+            k = svd_res[int(i)][3]
+            serr.append(float(np.max(np.abs(svd_res[int(i)][1][:k] - s_ref[:k])) / s_ref[0]))
+        rel_s = max(serr)
+        # CPU: the oracle (BLAS-routed einsum; LAPACK gesdd) on a bounded sample
+        fl_done = 0.0
+        done = 0
+        t_cpu_c = 0.0
+        budget = min(args.cpu_seconds, 20.0)
+        for sp, (a, b, c), (oa, ob) in zip(specs, sizes, in_off):
+            A = synth.gaussian(GROUPED_SEED, a, oa, np.complex64).reshape(sp.shape(sp.la), order="F")
+            B = synth.gaussian(GROUPED_SEED, b, ob, np.complex64).reshape(sp.shape(sp.lb), order="F")
+            ts = time.perf_counter()
+            O.binary_einsum(O.OTensor(A, sp.la), O.OTensor(B, sp.lb), blas=True)
+            t_cpu_c += time.perf_counter() - ts
+            fl_done += 8.0 * float(np.prod(sp.mnk))
+            done += 1
+            if t_cpu_c > budget / 2:
+                break
+        sv_fl = 0.0
+        sv_done = 0
+        t_cpu_s = 0.0
+        for i in range(len(ms)):
+            m, n = int(ms[i]), int(ns[i])
+            Ah = synth.gaussian(GROUPED_SEED + 1000, m * n, int(offs[i])).reshape(m, n, order="F")
+            ts = time.perf_counter()
+            np.linalg.svd(Ah, full_matrices=False)
+            t_cpu_s += time.perf_counter() - ts
+            sv_fl += svd_work([m], [n])
+            sv_done += 1
+            if t_cpu_s > budget / 2:
+                break
+        threads = os.environ.get("OPENBLAS_NUM_THREADS") or str(os.cpu_count())
+        cpu = {"value": round((fl_done + sv_fl) / (t_cpu_c + t_cpu_s) / 1e9, 3), "unit": "GFLOP/s",
+               "cores": int(threads), "kind": "port",
+               "sample": f"oracle: first {done} contractions (complex64 numpy einsum via BLAS, {t_cpu_c:.1f} s) "
+                         f"and first {sv_done} SVDs (complex128 LAPACK gesdd, {t_cpu_s:.1f} s), {threads} threads"}
+
+    total = t_c + t_s
+    roof = None
+    if kst["launches"]:
+        ach = kst["bytes"] / (kst["ms"] * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": round(ach, 2), "peak": 8000.0, "unit": "GB/s",
+                "frac": round(ach / 8000.0, 4), "traffic": None,
+                "kernel": "grouped_gemm_c64_kernel (8192 contractions, one persistent launch)",
+                "launches": kst["launches"], "avg_launch_us": round(kst["ms"] * 1e3 / kst["launches"], 1),
+                "fp32_mfma_tflops": round(fl_c / (kst["ms"] / kst["launches"] * 1e-3) / 1e12, 3),
+                "fp32_mfma_frac": round(fl_c / (kst["ms"] / kst["launches"] * 1e-3) / 1e12 / 157.3, 4)}
+    line = {
+        "metric": "grouped batch throughput (8192 contractions + 8192 truncated SVDs)",
+        "value": round((fl_c + fl_s) / total / 1e9, 3), "unit": "GFLOP/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(total * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "c64 (fp32 mfma contractions, fp64 svd)",
+        "data": "synthetic (device-generated seeded Gaussians)",
+        "config": {"workload": "grouped 8192 contractions (rank 3-5, extents 2..128, <=2^20 elems) + 8192 SVDs "
+                               "(16..512, maxdim ceil(min/2), threshold 1e-10)",
+                   "contractions": {"ms": round(t_c * 1e3, 3), "gflop": round(fl_c / 1e9, 2),
+                                    "gbytes": round(by_c / 1e9, 2), "plan": info, "tile_costs": costs,
+                                    "max_relerr_vs_fp64_oracle_48_sampled": rel_c},
+                   "svds": {"ms": round(t_s * 1e3, 3), "gflop_nominal": round(fl_s / 1e9, 1),
+                            "svd_round_launches": kst_s["launches"],
+                            "max_sigma_err_rel_sigma1_16_sampled": rel_s}},
+        "roofline": roof, "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    plan.close()
+    ctx.close()
+
+
+# ---------------------------------------------------------------- serving
+SERVE_STATES, SERVE_SEED, SERVE_P99_BOUND_MS = 32, 20261202 + 4, 50.0
+
+
+def bench_serving(args):
+    """32 device-resident MPS states (n uniform 16..128, d = 2, chi
+    log-uniform 16..512 capped at 2^min(b, n-b), complex64 Gaussians,
+    canonize!(site 1) + normalize!), every left environment prefilled.
+    Requests: 50% amplitudes <x|psi> (uniform x), 50% Pauli strings of weight
+    1-4 on consecutive sites.  Poisson arrivals at increasing rates; the
+    reported rate is the highest whose p99 latency stays within 50 ms."""
+    import numpy as np
+    import tenet_hip as T
+    from tenet_hip import _lib as L
+    from tenet_hip.serving import Server, latency_summary
+    from tenet_hip.workloads import build_serving_states, serving_requests
+
+    ctx = T.Context(0)
+    t0 = time.perf_counter()
+    states = build_serving_states(ctx, SERVE_SEED, SERVE_STATES)
+    ctx.sync()
+    log(f"serving: {len(states)} states built in {time.perf_counter() - t0:.1f}s, "
+        f"{sum(s.bytes for s in states) / 1e9:.2f} GB, n {[s.n for s in states]}")
+    t0 = time.perf_counter()
+    srv = Server(ctx, states, n_streams=4, max_inflight=1024, max_amp_batch=512, prefill_envs=True)
+    log(f"serving: {len(srv.env_fixed)} environments prefilled in {time.perf_counter() - t0:.1f}s")
+    warm = serving_requests(SERVE_SEED + 1, states, 400)
+    srv.run(warm)
+    sweep = []
+    best = None
+    rates = [float(x) for x in args.rates.split(",")]
+    for i, rate in enumerate(rates):
+        nreq = int(min(max(rate * args.serve_seconds, 200), 40000))
+        reqs, arr = serving_requests(SERVE_SEED + 10 + i, states, nreq, rate=rate)
+        ctx.stats_sample_every(1)
+        ctx.stats_enable(L.KSTAT_AMPLITUDE)
+        wall = srv.run(reqs, arr)
+        kst = ctx.stats_read()
+        ctx.stats_enable(L.KSTAT_OFF)
+        lat = latency_summary(reqs)
+        done_rate = nreq / max(r.t_done for r in reqs)
+        amp_bytes = 0.0
+        for r in reqs:
+            if r.kind == "amp":
+                c = states[r.state].chis
+                amp_bytes += 8.0 * float(np.sum(c[:-1] * c[1:]))
+        ent = {"offered_rps": rate, "requests": nreq, "served_rps": round(done_rate, 1), **{k: round(v, 3)
+               for k, v in lat.items()}, "amp_kernel_ms": round(kst["ms"], 2), "amp_launches": kst["launches"],
+               "amp_bytes": amp_bytes}
+        sweep.append(ent)
+        log(f"serving rate {rate}: {ent}")
+        if lat["p99_ms"] <= SERVE_P99_BOUND_MS:
+            best = ent
+        else:
+            break
+    ctx.stats_sample_every(17)
+    stats = dict(srv.stats)
+    srv.close()
+    r = best if best is not None else sweep[0]
+    roof = None
+    if r["amp_launches"] and r["amp_kernel_ms"] > 0:
+        ach = r["amp_bytes"] / (r["amp_kernel_ms"] * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": round(ach, 2), "peak": 8000.0, "unit": "GB/s", "frac": round(ach / 8000, 5),
+                "traffic": None, "kernel": "amplitude_kernel (one workgroup per request, batched per loop pass)",
+                "launches": r["amp_launches"], "avg_launch_us": round(r["amp_kernel_ms"] * 1e3 / r["amp_launches"], 1)}
+    cpu = None
+    if not args.no_cpu:
+        from oracle import tenet_oracle as O
+
+        hosts = [O.OMPS([a.to_host().astype(np.complex64) for a in st.mps.sites]) for st in states[:8]]
+        reqs = serving_requests(SERVE_SEED + 99, states[:8], 400)
+        t0 = time.perf_counter()
+        done = 0
+        for q in reqs:
+            if q.kind == "amp":
+                O.amplitude(q.bits, hosts[q.state])
+            else:
+                O.expectation_pauli(hosts[q.state], q.start, q.paulis)
+            done += 1
+            if time.perf_counter() - t0 > min(args.cpu_seconds, 15.0):
+                break
+        dt = time.perf_counter() - t0
+        threads = os.environ.get("OPENBLAS_NUM_THREADS") or str(os.cpu_count())
+        cpu = {"value": round(done / dt, 2), "unit": "req/s", "cores": int(threads), "kind": "port",
+               "sample": f"oracle amplitude / full-sweep expectation on {done} requests over the first 8 states, "
+                         f"one at a time ({dt:.1f} s, numpy/OpenBLAS, {threads} threads)"}
+    line = {
+        "metric": "serving req/s at p99 <= 50 ms (amplitudes + Pauli expectations, continuous batching)",
+        "value": r["served_rps"], "unit": "req/s", "n_gpus": 1, "steps": len(sweep), "warmup": 1,
+        "ms_per_step": round(1e3 / max(r["served_rps"], 1e-9), 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "c64", "data": "synthetic (device-generated seeded Gaussian MPS)",
+        "config": {"workload": "serving 32 MPS states n 16..128 chi 16..512 c64; 50% amplitude / 50% Pauli weight 1-4;"
+                               " Poisson arrivals", "p99_bound_ms": SERVE_P99_BOUND_MS, "p50_ms": r["p50_ms"],
+                   "p99_ms": r["p99_ms"], "sweep": sweep, "server_stats": stats,
+                   "state_gb": round(sum(s.bytes for s in states) / 1e9, 3)},
+        "roofline": roof, "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    for st in states:
+        st.close()
+    ctx.close()
+
+
 # ---------------------------------------------------------------- plain small
 PLAIN_N, PLAIN_CHI, PLAIN_SEED = 20, 32, 20261202 + 0
 
@@ -347,7 +632,9 @@ def main():
     ap.add_argument("--forms", default="fp64,split_bf16")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="tebd", choices=["tebd", "heff", "plain"])
+    ap.add_argument("--workload", default="tebd", choices=["tebd", "heff", "plain", "grouped", "serving"])
+    ap.add_argument("--rates", default="250,500,1000,2000,4000,8000,16000")
+    ap.add_argument("--serve-seconds", type=float, default=2.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: this engine has no multi-GPU path (the TEBD sweep is "
@@ -359,6 +646,10 @@ def main():
         return bench_heff(args)
     if args.workload == "plain":
         return bench_plain(args)
+    if args.workload == "grouped":
+        return bench_grouped(args)
+    if args.workload == "serving":
+        return bench_serving(args)
 
     import numpy as np
     import tenet_hip as T

@@ -91,6 +91,23 @@ int tenet_memcpy_d2d(tenet_ctx ctx, void* dst, const void* src, size_t bytes, te
 int tenet_memset_zero(tenet_ctx ctx, void* dst, size_t bytes, tenet_stream stream); /* fill kernel */
 int tenet_mem_info(tenet_ctx ctx, size_t* free_bytes, size_t* total_bytes, size_t* pool_bytes);
 
+/* ---- asynchronous request loops (no reference counterpart: the serving
+ * loop's plumbing) --------------------------------------------------------
+ * Pinned host buffers (hipHostMalloc) for copies that return before they
+ * complete; events to poll completion and order streams.  Pool memory freed
+ * with tenet_free is reused at once, so free a buffer only after the work
+ * using it is known complete. */
+typedef void* tenet_event; /* hipEvent_t */
+int tenet_host_alloc(tenet_ctx ctx, size_t bytes, void** hptr);
+int tenet_host_free(tenet_ctx ctx, void* hptr);
+/* to_host = 1: device -> pinned host; 0: pinned host -> device.  Asynchronous. */
+int tenet_memcpy_async(tenet_ctx ctx, void* dst, const void* src, size_t bytes, int to_host, tenet_stream stream);
+int tenet_event_create(tenet_ctx ctx, tenet_event* ev);
+int tenet_event_destroy(tenet_ctx ctx, tenet_event ev);
+int tenet_event_record(tenet_ctx ctx, tenet_event ev, tenet_stream stream);
+int tenet_event_query(tenet_ctx ctx, tenet_event ev, int* done); /* done = 1 when complete */
+int tenet_stream_wait_event(tenet_ctx ctx, tenet_stream stream, tenet_event ev);
+
 /* ---- pairwise contraction (K1) ------------------------------------------
  * Replaces Muscle.binary_einsum(a, b; dims) at Tenet call sites
  * src/Operations/canonize.jl:44,47,61,100; evolve.jl:29,57,76,86,108,118;
@@ -178,12 +195,49 @@ int tenet_amplitudes(tenet_ctx ctx, tenet_stream stream, int dtype, int64_t nreq
 int tenet_expect_pauli(tenet_ctx ctx, tenet_stream stream, int dtype, int prec, int64_t start, int64_t w,
                        int64_t d, const int64_t* chi, const void* const* sites, const void* env_l,
                        const char* paulis, double* out);
+/* Left environment E(r, r') = sum over sites [0, upto) of conj(A) A (the
+ * prefix of the overlap.jl:36-50 sweep), for reuse as env_l: env_out gets E
+ * divided by exp(out[2]) (chi[upto]^2 elements, device); out: device, 3
+ * doubles with the log-scale in out[2].  upto = 0 gives the 1 x 1 identity. */
+int tenet_left_env(tenet_ctx ctx, tenet_stream stream, int dtype, int prec, int64_t upto, int64_t d,
+                   const int64_t* chi, const void* const* sites, void* env_out, double* out);
+/* Every left environment of one sweep: env_outs (HOST array of upto+1
+ * device pointers, NULL entries skipped) receive E(j) / exp(logs[j]) for
+ * j = 0..upto (chi[j]^2 elements each); logs: device, upto+1 doubles.
+ * Synchronises the stream before returning. */
+int tenet_left_envs(tenet_ctx ctx, tenet_stream stream, int dtype, int prec, int64_t upto, int64_t d,
+                    const int64_t* chi, const void* const* sites, void* const* env_outs, double* logs);
 
 /* ---- grouped (persistent) launches ---------------------------------------
- * Many independent contractions in one launch.  descs holds 3*count
- * descriptors (a, b, c per problem); ptrs holds 3*count device pointers. */
+ * Many independent contractions (each one Muscle.binary_einsum, as at the
+ * call sites of canonize.jl:44-61 / evolve.jl:29-119, batched) in one
+ * persistent launch.  descs holds 3*count descriptors (a, b, c per problem).
+ *
+ * The plan analyses every problem once: offset tables in one device arena,
+ * a tile shape per problem (64x64, 32x32 or 16x16 outputs, chosen by a cost
+ * model whose per-shape costs tenet_autotune_tiles measures), long K ranges
+ * cut into chunks, and a work queue of (problem, tile, chunk) units sorted
+ * by decreasing cost that resident workgroups drain.  Chunk partials are
+ * summed in chunk order by the workgroup finishing a tile, so results do
+ * not depend on the schedule.  The plan owns its device memory. */
+typedef struct tenet_gplan_s* tenet_gplan;
+int tenet_grouped_plan(tenet_ctx ctx, int dtype, int prec, int64_t count, const tenet_tdesc* descs,
+                       tenet_gplan* plan);
+/* ptrs: HOST array of 3*count device pointers (A, B, C per problem); returns
+ * after the launch completes */
+int tenet_grouped_exec(tenet_gplan plan, tenet_stream stream, void* const* ptrs);
+/* ptrs_dev: DEVICE array of 3*count pointers; asynchronous, graph-capturable */
+int tenet_grouped_exec_dev(tenet_gplan plan, tenet_stream stream, const uint64_t* ptrs_dev);
+int tenet_grouped_plan_info(tenet_gplan plan, int64_t* units, int64_t* split_tiles, int64_t shapes[3],
+                            double* flops, double* bytes);
+int tenet_grouped_plan_destroy(tenet_gplan plan);
+/* plan + exec + destroy */
 int tenet_grouped_einsum(tenet_ctx ctx, tenet_stream stream, int dtype, int prec, int64_t count,
                          const tenet_tdesc* descs, void* const* ptrs);
+/* Measure the three tile shapes (64 problems of (256x1024).(1024x256)) and
+ * keep their relative cost per (tile, K step) in the context; costs_out
+ * (may be NULL) receives them, 64x64 = 1. */
+int tenet_autotune_tiles(tenet_ctx ctx, int dtype, double costs_out[3]);
 /* Many independent thin SVDs (column-major, lda = m) with per-problem
  * truncation.  All pointer arrays are HOST arrays of device pointers;
  * ms, ns, maxdims are host arrays; k_dev / disc_dev device arrays (count). */

@@ -70,14 +70,16 @@ class OTensor:
         return OTensor(self.data, [mapping.get(l, l) for l in self.labels])
 
 
-def binary_einsum(a: OTensor, b: OTensor, dims=None) -> OTensor:
+def binary_einsum(a: OTensor, b: OTensor, dims=None, blas=False) -> OTensor:
     """Pairwise contraction over ``dims`` (default: all shared labels).
 
     Shared labels not in ``dims`` are kept once and multiplied elementwise
     (batch/Hadamard); ``dims=[]`` is how Tenet absorbs singular values
     (canonize.jl:44, evolve.jl:76).  Output labels are a's labels in order
     followed by b's labels not in a (Muscle's own order is unpinned; callers
-    realign by label, DMRG.jl:320).
+    realign by label, DMRG.jl:320).  ``blas=True`` lets numpy route the
+    product through BLAS (permute + gemm, as Muscle's CPU path does) instead
+    of its plain summation loop; the checker uses the loop.
     """
     shared = [l for l in a.labels if l in b.labels]
     dims = shared if dims is None else list(dims)
@@ -91,7 +93,7 @@ def binary_einsum(a: OTensor, b: OTensor, dims=None) -> OTensor:
             letters[l] = string.ascii_letters[len(letters)]
     spec = "".join(letters[l] for l in a.labels) + "," + "".join(letters[l] for l in b.labels) \
         + "->" + "".join(letters[l] for l in out)
-    return OTensor(np.einsum(spec, a.data, b.data, optimize=False), out)
+    return OTensor(np.einsum(spec, a.data, b.data, optimize="greedy" if blas else False), out)
 
 
 def _matricize(A: OTensor, rows, cols):
@@ -572,7 +574,7 @@ PAULI = {
 def expectation_pauli(psi: OMPS, start: int, paulis: str):
     """<psi|P|psi> / 1 for a Pauli string on sites start..start+w-1.
 
-    Composition from the reference's pieces (SURVEY 3.4): apply one-site
+    Composition from the reference's pieces: apply one-site
     Paulis by the 1-site shortcut (simple_update.jl:28-37), then the
     transfer-matrix overlap (overlap.jl:36-50).  Pauli matrices follow
     ext/TenetPythonCallExt/pytket.jl:23-31.
@@ -645,7 +647,7 @@ def tebd_gates(seed, n, depth):
 
 
 def tebd_run(n, sched, gates, maxdim, threshold, progress=None):
-    """TEBD as the user loop of simple_update! over the brickwork."""
+    """TEBD as the user loop of simple_update! over the brickwork (simple_update.jl:82-92 per gate)."""
     psi = product_state_mps([[1.0, 0.0]] * n)
     for g, (a, U) in enumerate(zip(sched, gates)):
         simple_update_mps(psi, U, a, maxdim, threshold)

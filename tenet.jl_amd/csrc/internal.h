@@ -50,19 +50,38 @@ int launch_gemm_c128(const GemmLaunch& g, int prec, hipStream_t s, void* ws, siz
 int launch_gemm_c64(const GemmLaunch& g, int prec, hipStream_t s, void* ws, size_t ws_bytes);
 size_t gemm_workspace_bytes(const GemmLaunch& g, int dtype, int prec);
 
-// Grouped GEMM: an array of problems in device memory, one launch.
+// Grouped GEMM: problems and work units in device memory, one persistent
+// launch.  Operand pointers come separately (3 per problem: A, B, C) so one
+// plan serves many launches.
+enum { GROUPED_TILE_64 = 0, GROUPED_TILE_32 = 1, GROUPED_TILE_16 = 2 };
 struct GroupedProblem {
-  const void* A;
-  const void* B;
-  void* C;
   GemmTables t;
   int64_t M, N, K, batch;
   int32_t a_mfast, b_nfast;
+  int32_t shape;            // GROUPED_TILE_*
+  int32_t kchunks;          // K chunks per tile (1: no split)
   int64_t tiles_m, tiles_n;
-  int64_t tile_begin;  // prefix sum of tiles over problems
+  int64_t kc;               // chunk length (multiple of 16)
+  int64_t part_base;        // first partial element (complex) of this problem
+  int64_t ctr_base;         // first tile arrival counter of this problem
 };
-int launch_grouped_gemm(const GroupedProblem* probs_dev, int64_t nprob, int64_t total_tiles, int dtype,
-                        hipStream_t s);
+struct GroupedUnit {
+  int32_t prob;
+  int32_t chunk;
+  int64_t tile;  // z * tiles_m * tiles_n + tn * tiles_m + tm
+};
+struct GroupedLaunch {
+  const GroupedProblem* probs;
+  const GroupedUnit* units;
+  int64_t nunits;
+  const uint64_t* ptrs;
+  void* partials;
+  uint32_t* counters;  // [0] queue head, [64 + i] tile arrival counters
+  int64_t grid;
+  int dtype, prec;
+};
+int launch_grouped_gemm(const GroupedLaunch& g, hipStream_t s);
+int grouped_occupancy(int dtype);  // resident workgroups per CU of the grouped kernel
 
 // ---- K2: Householder QR ----------------------------------------------------
 // In-place QR of column-major W (m x n, ld = m) for `batch` matrices laid out

@@ -81,6 +81,8 @@ struct tenet_ctx_s {
   std::vector<hipEvent_t> stat_events;  // pairs
   size_t stat_used = 0;
   int64_t stat_launches = 0, stat_sampled = 0, stat_every = 17;
+  // grouped-launch tile costs per (tile, K step) relative to 64 x 64, per dtype (0 = unset)
+  double tile_cost[2][3] = {{0, 0, 0}, {0, 0, 0}};
   double stat_ms = 0.0, stat_flops = 0.0, stat_bytes = 0.0;
 };
 // Every stat_every-th launch of the tracked family is bracketed by events
@@ -328,6 +330,70 @@ extern "C" int tenet_device_sync(tenet_ctx ctx) {
 }
 
 // ---------------------------------------------------------------------------
+// events and pinned host staging (asynchronous request loops)
+// ---------------------------------------------------------------------------
+extern "C" int tenet_event_create(tenet_ctx ctx, tenet_event* ev) {
+  TRY(use_device(ctx));
+  CHECK_ARG(ev, "null event out");
+  hipEvent_t e;
+  TENET_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  *ev = (tenet_event)e;
+  return TENET_OK;
+}
+extern "C" int tenet_event_destroy(tenet_ctx ctx, tenet_event ev) {
+  TRY(use_device(ctx));
+  if (ev) TENET_HIP_CHECK(hipEventDestroy((hipEvent_t)ev));
+  return TENET_OK;
+}
+extern "C" int tenet_event_record(tenet_ctx ctx, tenet_event ev, tenet_stream stream) {
+  TRY(use_device(ctx));
+  CHECK_ARG(ev, "null event");
+  TENET_HIP_CHECK(hipEventRecord((hipEvent_t)ev, pick(ctx, stream)));
+  return TENET_OK;
+}
+extern "C" int tenet_event_query(tenet_ctx ctx, tenet_event ev, int* done) {
+  TRY(use_device(ctx));
+  CHECK_ARG(ev && done, "bad argument");
+  const hipError_t e = hipEventQuery((hipEvent_t)ev);
+  if (e == hipSuccess) {
+    *done = 1;
+    return TENET_OK;
+  }
+  if (e == hipErrorNotReady) {
+    *done = 0;
+    return TENET_OK;
+  }
+  TENET_HIP_CHECK(e);
+  return TENET_OK;
+}
+extern "C" int tenet_stream_wait_event(tenet_ctx ctx, tenet_stream stream, tenet_event ev) {
+  TRY(use_device(ctx));
+  CHECK_ARG(ev, "null event");
+  TENET_HIP_CHECK(hipStreamWaitEvent(pick(ctx, stream), (hipEvent_t)ev, 0));
+  return TENET_OK;
+}
+extern "C" int tenet_host_alloc(tenet_ctx ctx, size_t bytes, void** hptr) {
+  TRY(use_device(ctx));
+  CHECK_ARG(hptr, "null pointer out");
+  TENET_HIP_CHECK(hipHostMalloc(hptr, std::max<size_t>(bytes, 1), hipHostMallocDefault));
+  return TENET_OK;
+}
+extern "C" int tenet_host_free(tenet_ctx ctx, void* hptr) {
+  TRY(use_device(ctx));
+  if (hptr) TENET_HIP_CHECK(hipHostFree(hptr));
+  return TENET_OK;
+}
+extern "C" int tenet_memcpy_async(tenet_ctx ctx, void* dst, const void* src, size_t bytes, int to_host,
+                                  tenet_stream stream) {
+  TRY(use_device(ctx));
+  if (bytes == 0) return TENET_OK;
+  TRY(check_span(ctx, to_host ? src : dst, bytes, to_host ? "async d2h src" : "async h2d dst"));
+  TENET_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, to_host ? hipMemcpyDeviceToHost : hipMemcpyHostToDevice,
+                                 pick(ctx, stream)));
+  return TENET_OK;
+}
+
+// ---------------------------------------------------------------------------
 // memory pool
 // ---------------------------------------------------------------------------
 static size_t size_class(size_t b) {
@@ -468,8 +534,16 @@ static void radix_table(const std::vector<LabelInfo*>& ls, int which, std::vecto
   }
 }
 
-static int build_plan(tenet_ctx ctx, int dtype, int prec, const tenet_tdesc* a, const tenet_tdesc* b,
-                      const tenet_tdesc* c, tenet_plan p) {
+// Label analysis of one contraction: the nine offset tables (rowA, kA, bA,
+// kB, colB, bB, rowC, colC, bC) and the GEMM shape, on the host.
+struct HostPlan {
+  int64_t M = 0, N = 0, K = 0, batch = 0;
+  int a_mfast = 0, b_nfast = 0;
+  int64_t spanA = 0, spanB = 0, spanC = 0;
+  std::vector<int64_t> v[9];
+};
+
+static int analyse_plan(const tenet_tdesc* a, const tenet_tdesc* b, const tenet_tdesc* c, HostPlan& hp) {
   for (const tenet_tdesc* d : {a, b, c}) {
     if (d->rank < 0 || d->rank > TENET_MAX_RANK) {
       set_error("rank out of range");
@@ -550,25 +624,21 @@ static int build_plan(tenet_ctx ctx, int dtype, int prec, const tenet_tdesc* a, 
   std::stable_sort(kk.begin(), kk.end(), by(0));
   std::stable_sort(bt.begin(), bt.end(), by(2));
 
-  std::vector<int64_t> rowA, kA, bA, kB, colB, bB, rowC, colC, bC;
-  radix_table(fa, 0, rowA);
-  radix_table(fa, 2, rowC);
-  radix_table(kk, 0, kA);
-  radix_table(kk, 1, kB);
-  radix_table(fb, 1, colB);
-  radix_table(fb, 2, colC);
-  radix_table(bt, 0, bA);
-  radix_table(bt, 1, bB);
-  radix_table(bt, 2, bC);
-  p->ctx = ctx;
-  p->dtype = dtype;
-  p->prec = prec;
-  p->M = (int64_t)rowA.size();
-  p->N = (int64_t)colB.size();
-  p->K = (int64_t)kA.size();
-  p->batch = (int64_t)bA.size();
-  p->a_mfast = (!fa.empty() && fa[0]->sa == 1) ? 1 : 0;
-  p->b_nfast = (!fb.empty() && fb[0]->sb == 1) ? 1 : 0;
+  radix_table(fa, 0, hp.v[0]);  // rowA
+  radix_table(kk, 0, hp.v[1]);  // kA
+  radix_table(bt, 0, hp.v[2]);  // bA
+  radix_table(kk, 1, hp.v[3]);  // kB
+  radix_table(fb, 1, hp.v[4]);  // colB
+  radix_table(bt, 1, hp.v[5]);  // bB
+  radix_table(fa, 2, hp.v[6]);  // rowC
+  radix_table(fb, 2, hp.v[7]);  // colC
+  radix_table(bt, 2, hp.v[8]);  // bC
+  hp.M = (int64_t)hp.v[0].size();
+  hp.N = (int64_t)hp.v[4].size();
+  hp.K = (int64_t)hp.v[1].size();
+  hp.batch = (int64_t)hp.v[2].size();
+  hp.a_mfast = (!fa.empty() && fa[0]->sa == 1) ? 1 : 0;
+  hp.b_nfast = (!fb.empty() && fb[0]->sb == 1) ? 1 : 0;
   auto span = [](const tenet_tdesc* d) {
     int64_t s = 0;
     for (int i = 0; i < d->rank; ++i) {
@@ -577,31 +647,50 @@ static int build_plan(tenet_ctx ctx, int dtype, int prec, const tenet_tdesc* a, 
     }
     return s + 1;
   };
-  p->spanA = span(a);
-  p->spanB = span(b);
-  p->spanC = span(c);
-  const size_t total = rowA.size() + kA.size() + bA.size() + kB.size() + colB.size() + bB.size() + rowC.size() +
-                       colC.size() + bC.size();
+  hp.spanA = span(a);
+  hp.spanB = span(b);
+  hp.spanC = span(c);
+  return TENET_OK;
+}
+
+static void set_tables(GemmTables& t, int64_t* T, const size_t* off) {
+  t.rowA = T + off[0];
+  t.kA = T + off[1];
+  t.bA = T + off[2];
+  t.kB = T + off[3];
+  t.colB = T + off[4];
+  t.bB = T + off[5];
+  t.rowC = T + off[6];
+  t.colC = T + off[7];
+  t.bC = T + off[8];
+}
+
+static int build_plan(tenet_ctx ctx, int dtype, int prec, const tenet_tdesc* a, const tenet_tdesc* b,
+                      const tenet_tdesc* c, tenet_plan p) {
+  HostPlan hp;
+  TRY(analyse_plan(a, b, c, hp));
+  p->ctx = ctx;
+  p->dtype = dtype;
+  p->prec = prec;
+  p->M = hp.M;
+  p->N = hp.N;
+  p->K = hp.K;
+  p->batch = hp.batch;
+  p->a_mfast = hp.a_mfast;
+  p->b_nfast = hp.b_nfast;
+  p->spanA = hp.spanA;
+  p->spanB = hp.spanB;
+  p->spanC = hp.spanC;
   std::vector<int64_t> host;
-  host.reserve(total);
-  std::vector<size_t> off;
-  for (auto* v : {&rowA, &kA, &bA, &kB, &colB, &bB, &rowC, &colC, &bC}) {
-    off.push_back(host.size());
-    host.insert(host.end(), v->begin(), v->end());
+  size_t off[9];
+  for (int i = 0; i < 9; ++i) {
+    off[i] = host.size();
+    host.insert(host.end(), hp.v[i].begin(), hp.v[i].end());
   }
   if (host.empty()) host.push_back(0);
   TENET_HIP_CHECK(hipMalloc((void**)&p->tables, host.size() * sizeof(int64_t)));
   TENET_HIP_CHECK(hipMemcpy(p->tables, host.data(), host.size() * sizeof(int64_t), hipMemcpyHostToDevice));
-  int64_t* T = p->tables;
-  p->t.rowA = T + off[0];
-  p->t.kA = T + off[1];
-  p->t.bA = T + off[2];
-  p->t.kB = T + off[3];
-  p->t.colB = T + off[4];
-  p->t.bB = T + off[5];
-  p->t.rowC = T + off[6];
-  p->t.colC = T + off[7];
-  p->t.bC = T + off[8];
+  set_tables(p->t, p->tables, off);
   GemmLaunch g{};
   g.M = p->M;
   g.N = p->N;
@@ -1101,18 +1190,13 @@ extern "C" int tenet_amplitudes(tenet_ctx ctx, tenet_stream stream, int dtype, i
   return launch_amplitudes(nreq, n_dev, site_ptrs_dev, chis_dev, bits_dev, d, out, dtype, 2048, s);
 }
 
-extern "C" int tenet_expect_pauli(tenet_ctx ctx, tenet_stream stream, int dtype, int prec, int64_t start, int64_t w,
-                                  int64_t d, const int64_t* chi, const void* const* sites, const void* env_l,
-                                  const char* paulis, double* out) {
-  TRY(use_device(ctx));
-  TRY(valid_dtype_prec(dtype, prec));
-  CHECK_ARG(d == 2, "Pauli strings need d = 2");
-  CHECK_ARG(w >= 1 && start >= 0 && chi && sites && paulis && out, "bad argument");
-  for (int64_t j = 0; j < w; ++j) {
-    const char c = paulis[j];
-    CHECK_ARG(c == 'I' || c == 'X' || c == 'Y' || c == 'Z', "Pauli letters must be I, X, Y or Z");
-  }
-  hipStream_t s = pick(ctx, stream);
+// Transfer-matrix sweep over sites [start, start + w): env <- conj(A) (env .
+// P_j A) with P_j the Pauli letter j (NULL letters: identity, the site is
+// used directly), renormalised after every site with the scale's log added
+// to out[2].  Leaves the final environment in *env_final (workspace).
+static int env_sweep(tenet_ctx ctx, hipStream_t s, int dtype, int prec, int64_t start, int64_t w, int64_t d,
+                     const int64_t* chi, const void* const* sites, const void* env_l, const char* paulis,
+                     double* out, void** env_final, void* const* env_each = nullptr, double* log_each = nullptr) {
   const size_t es = ELEM[dtype];
   int64_t max_env = 1, max_site = 1;
   for (int64_t j = 0; j <= w; ++j) max_env = std::max(max_env, chi[start + j] * chi[start + j]);
@@ -1133,61 +1217,421 @@ extern "C" int tenet_expect_pauli(tenet_ctx ctx, tenet_stream stream, int dtype,
   }
   for (int64_t j = 0; j < w; ++j) {
     const int64_t i = start + j, l = chi[i], r = chi[i + 1];
-    TRY(launch_apply_pauli(sites[i], l, r, paulis[j], PA, dtype, s));
+    const void* op_site = sites[i];
+    if (paulis && paulis[j] != 'I') {
+      TRY(launch_apply_pauli(sites[i], l, r, paulis[j], PA, dtype, s));
+      op_site = PA;
+    }
     // T(a, o, r') = env(a, a') PA(a', o, r')
     const tenet_tdesc dE = mkdesc({l, l}, {1, 2});
     const tenet_tdesc dP = mkdesc({l, d, r}, {2, 3, 4});
     const tenet_tdesc dT = mkdesc({l, d, r}, {1, 3, 4});
-    TRY(einsum_internal(ctx, s, dtype, prec, dE, env[cur], dP, PA, dT, T, 0, 0));
+    TRY(einsum_internal(ctx, s, dtype, prec, dE, env[cur], dP, op_site, dT, T, 0, 0));
     // env'(r, r') = conj(A(a, o, r)) T(a, o, r')
     const tenet_tdesc dA = mkdesc({l, d, r}, {1, 3, 5});
     const tenet_tdesc dE2 = mkdesc({r, r}, {5, 4});
     TRY(einsum_internal(ctx, s, dtype, prec, dA, sites[i], dT, T, dE2, env[cur ^ 1], 1, 0));
     cur ^= 1;
     TRY(launch_rescale_log(env[cur], r * r, out, dtype, s));
+    if (env_each && env_each[j + 1])
+      TENET_HIP_CHECK(hipMemcpyAsync(env_each[j + 1], env[cur], (size_t)r * r * es, hipMemcpyDeviceToDevice, s));
+    if (log_each)
+      TENET_HIP_CHECK(hipMemcpyAsync(log_each + j + 1, out + 2, sizeof(double), hipMemcpyDeviceToDevice, s));
   }
-  return launch_trace_diag(env[cur], chi[start + w], out, dtype, s);
+  *env_final = env[cur];
+  return TENET_OK;
+}
+
+extern "C" int tenet_expect_pauli(tenet_ctx ctx, tenet_stream stream, int dtype, int prec, int64_t start, int64_t w,
+                                  int64_t d, const int64_t* chi, const void* const* sites, const void* env_l,
+                                  const char* paulis, double* out) {
+  TRY(use_device(ctx));
+  TRY(valid_dtype_prec(dtype, prec));
+  CHECK_ARG(d == 2, "Pauli strings need d = 2");
+  CHECK_ARG(w >= 1 && start >= 0 && chi && sites && paulis && out, "bad argument");
+  for (int64_t j = 0; j < w; ++j) {
+    const char c = paulis[j];
+    CHECK_ARG(c == 'I' || c == 'X' || c == 'Y' || c == 'Z', "Pauli letters must be I, X, Y or Z");
+  }
+  hipStream_t s = pick(ctx, stream);
+  void* env = nullptr;
+  TRY(env_sweep(ctx, s, dtype, prec, start, w, d, chi, sites, env_l, paulis, out, &env));
+  return launch_trace_diag(env, chi[start + w], out, dtype, s);
+}
+
+extern "C" int tenet_left_envs(tenet_ctx ctx, tenet_stream stream, int dtype, int prec, int64_t upto, int64_t d,
+                               const int64_t* chi, const void* const* sites, void* const* env_outs, double* logs) {
+  TRY(use_device(ctx));
+  TRY(valid_dtype_prec(dtype, prec));
+  CHECK_ARG(upto >= 0 && d >= 1 && chi && sites && env_outs && logs, "bad argument");
+  hipStream_t s = pick(ctx, stream);
+  const size_t es = ELEM[dtype];
+  for (int64_t j = 0; j <= upto; ++j)
+    if (env_outs[j]) TRY(check_span(ctx, env_outs[j], (size_t)chi[j] * chi[j] * es, "env_outs"));
+  TRY(check_span(ctx, logs, (size_t)(upto + 1) * sizeof(double), "logs"));
+  TRY(launch_fill_zero(logs, sizeof(double), s));
+  if (env_outs[0]) TRY(launch_set_identity(env_outs[0], chi[0], dtype, s));
+  if (upto == 0) return TENET_OK;
+  void* wsp = nullptr;
+  // the sweep's running (re, im, log) triple lives in its own small buffer
+  TRY(tenet_malloc(ctx, 64, &wsp));
+  void* env = nullptr;
+  const int st = env_sweep(ctx, s, dtype, prec, 0, upto, d, chi, sites, nullptr, nullptr, (double*)wsp, &env,
+                           env_outs, logs);
+  // the triple is consumed by copies already on the stream; keep it until they ran
+  TENET_HIP_CHECK(hipStreamSynchronize(s));
+  (void)tenet_free(ctx, wsp);
+  return st;
+}
+
+extern "C" int tenet_left_env(tenet_ctx ctx, tenet_stream stream, int dtype, int prec, int64_t upto, int64_t d,
+                              const int64_t* chi, const void* const* sites, void* env_out, double* out) {
+  TRY(use_device(ctx));
+  TRY(valid_dtype_prec(dtype, prec));
+  CHECK_ARG(upto >= 0 && d >= 1 && chi && sites && env_out && out, "bad argument");
+  hipStream_t s = pick(ctx, stream);
+  const size_t es = ELEM[dtype];
+  TRY(check_span(ctx, env_out, (size_t)chi[upto] * chi[upto] * es, "env_out"));
+  if (upto == 0) {
+    TRY(launch_fill_zero(out, 3 * sizeof(double), s));
+    return launch_set_identity(env_out, chi[0], dtype, s);
+  }
+  void* env = nullptr;
+  TRY(env_sweep(ctx, s, dtype, prec, 0, upto, d, chi, sites, nullptr, nullptr, out, &env));
+  TENET_HIP_CHECK(hipMemcpyAsync(env_out, env, (size_t)chi[upto] * chi[upto] * es, hipMemcpyDeviceToDevice, s));
+  return TENET_OK;
 }
 
 // ---------------------------------------------------------------------------
 // grouped launches
 // ---------------------------------------------------------------------------
-extern "C" int tenet_grouped_einsum(tenet_ctx ctx, tenet_stream stream, int dtype, int prec, int64_t count,
-                                    const tenet_tdesc* descs, void* const* ptrs) {
+// ---------------------------------------------------------------------------
+// grouped contractions: plan (tables, tile shapes, K chunks, unit queue) once,
+// launch many times
+// ---------------------------------------------------------------------------
+struct tenet_gplan_s {
+  tenet_ctx ctx = nullptr;
+  int dtype = 0, prec = 0;
+  int64_t count = 0, nunits = 0, grid = 0;
+  void* dev = nullptr;  // one allocation: tables, problems, units, counters, partials, pointer staging
+  GroupedProblem* probs = nullptr;
+  GroupedUnit* units = nullptr;
+  uint32_t* counters = nullptr;
+  void* partials = nullptr;
+  uint64_t* ptrs_dev = nullptr;
+  std::vector<int64_t> spanA, spanB, spanC;
+  double flops = 0.0, bytes = 0.0;
+  int64_t shape_count[3] = {0, 0, 0};
+  int64_t split_tiles = 0;
+};
+
+static const int TILE_EDGE[3] = {64, 32, 16};
+
+// Relative cost of one (tile, 16-wide K step) per tile shape.  Measured by
+// tenet_autotune_tiles; until then a model: the three shapes run 64, 16 and
+// 4 MFMA blocks per K step on four waves, with a fixed per-step overhead.
+static void tile_costs(tenet_ctx ctx, int dtype, double c[3]) {
+  const int d = dtype == TENET_C128 ? 0 : 1;
+  if (ctx->tile_cost[d][0] > 0) {
+    for (int i = 0; i < 3; ++i) c[i] = ctx->tile_cost[d][i];
+    return;
+  }
+  c[0] = 1.0;
+  c[1] = 0.40;
+  c[2] = 0.25;
+}
+
+static int grouped_plan_build(tenet_ctx ctx, int dtype, int prec, int64_t count, const tenet_tdesc* descs,
+                              int force_shape, tenet_gplan g) {
+  g->ctx = ctx;
+  g->dtype = dtype;
+  g->prec = prec;
+  g->count = count;
+  const size_t es = ELEM[dtype];
+  std::vector<HostPlan> hps((size_t)count);
+  std::vector<GroupedProblem> probs((size_t)count);
+  g->spanA.resize((size_t)count);
+  g->spanB.resize((size_t)count);
+  g->spanC.resize((size_t)count);
+  double c[3];
+  tile_costs(ctx, dtype, c);
+  size_t table_elems = 0;
+  double total_cost = 0.0;
+  std::vector<double> tile_step_cost((size_t)count);
+  for (int64_t i = 0; i < count; ++i) {
+    HostPlan& hp = hps[(size_t)i];
+    TRY(analyse_plan(&descs[3 * i], &descs[3 * i + 1], &descs[3 * i + 2], hp));
+    for (auto& v : hp.v) table_elems += v.size();
+    GroupedProblem& q = probs[(size_t)i];
+    q.M = hp.M;
+    q.N = hp.N;
+    q.K = hp.K;
+    q.batch = hp.batch;
+    q.a_mfast = hp.a_mfast;
+    q.b_nfast = hp.b_nfast;
+    g->spanA[(size_t)i] = hp.spanA;
+    g->spanB[(size_t)i] = hp.spanB;
+    g->spanC[(size_t)i] = hp.spanC;
+    g->flops += 8.0 * hp.M * hp.N * hp.K * hp.batch;
+    g->bytes += (double)es * ((double)hp.M * hp.K + (double)hp.K * hp.N + (double)hp.M * hp.N) * hp.batch;
+    // tile shape with the least modelled time for this problem
+    const int64_t ksteps = std::max<int64_t>(1, (hp.K + 15) / 16);
+    int best = 0;
+    double best_t = 0.0;
+    for (int v = 0; v < 3; ++v) {
+      if (force_shape >= 0 && v != force_shape) continue;
+      const int64_t e = TILE_EDGE[v];
+      const double t = c[v] * (double)((hp.M + e - 1) / e) * (double)((hp.N + e - 1) / e) * hp.batch * ksteps;
+      if (force_shape >= 0 || t < best_t || best_t == 0.0) {
+        best = v;
+        best_t = t;
+      }
+    }
+    q.shape = best;
+    g->shape_count[best] += 1;
+    q.tiles_m = (hp.M + TILE_EDGE[best] - 1) / TILE_EDGE[best];
+    q.tiles_n = (hp.N + TILE_EDGE[best] - 1) / TILE_EDGE[best];
+    tile_step_cost[(size_t)i] = c[best];
+    if (hp.M * hp.N * hp.batch > 0) total_cost += best_t;
+  }
+  // persistent grid: resident workgroups on every CU
+  int dev = 0, ncu = 256;
+  TENET_HIP_CHECK(hipGetDevice(&dev));
+  TENET_HIP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  const int occ = grouped_occupancy(dtype);
+  g->grid = (int64_t)ncu * std::max(1, occ);
+  // K chunking: no unit above ~1/8 of one workgroup's fair share
+  const double unit_cap = std::max(total_cost / ((double)g->grid * 8.0), 16.0 * c[0]);
+  int64_t part_elems = 0, ctr_elems = 0, nunits = 0;
+  for (int64_t i = 0; i < count; ++i) {
+    GroupedProblem& q = probs[(size_t)i];
+    const int64_t ksteps = std::max<int64_t>(1, (q.K + 15) / 16);
+    const double per_tile = tile_step_cost[(size_t)i] * ksteps;
+    int64_t ch = 1;
+    if (per_tile > 1.5 * unit_cap) ch = std::min<int64_t>((int64_t)std::ceil(per_tile / unit_cap), 256);
+    ch = std::min<int64_t>(ch, ksteps);
+    const int64_t kcs = (ksteps + ch - 1) / ch;  // K steps per chunk
+    ch = (ksteps + kcs - 1) / kcs;
+    q.kchunks = (int32_t)ch;
+    q.kc = kcs * 16;
+    const int64_t tiles = q.tiles_m * q.tiles_n * q.batch;
+    q.part_base = -1;
+    q.ctr_base = -1;
+    if (ch > 1) {
+      const int64_t e = TILE_EDGE[q.shape];
+      q.part_base = part_elems;
+      part_elems += tiles * ch * e * e;
+      q.ctr_base = ctr_elems;
+      ctr_elems += tiles;
+      g->split_tiles += tiles;
+    }
+    nunits += tiles * ch;
+  }
+  CHECK_ARG(nunits < (int64_t)0xffffffffLL, "too many work units for one grouped launch");
+  g->nunits = nunits;
+  // unit queue, most expensive first (chunks of one tile stay adjacent)
+  std::vector<GroupedUnit> units;
+  units.reserve((size_t)nunits);
+  std::vector<std::pair<double, int64_t>> order;
+  order.reserve((size_t)count);
+  for (int64_t i = 0; i < count; ++i) {
+    const GroupedProblem& q = probs[(size_t)i];
+    order.push_back({tile_step_cost[(size_t)i] * (double)((q.K + 15) / 16) / q.kchunks, i});
+  }
+  std::stable_sort(order.begin(), order.end(),
+                   [](const std::pair<double, int64_t>& x, const std::pair<double, int64_t>& y) { return x.first > y.first; });
+  for (auto& o : order) {
+    const GroupedProblem& q = probs[(size_t)o.second];
+    const int64_t tiles = q.tiles_m * q.tiles_n * q.batch;
+    for (int64_t t = 0; t < tiles; ++t)
+      for (int32_t ch = 0; ch < q.kchunks; ++ch) units.push_back({(int32_t)o.second, ch, t});
+  }
+  // one device allocation
+  const size_t rs = dtype == TENET_C128 ? 8 : 4;  // partial accumulators in the MFMA precision
+  const size_t b_tab = align_up(std::max<size_t>(table_elems, 1) * 8, 256);
+  const size_t b_prob = align_up((size_t)count * sizeof(GroupedProblem), 256);
+  const size_t b_unit = align_up(std::max<size_t>(units.size(), 1) * sizeof(GroupedUnit), 256);
+  const size_t b_ctr = align_up((size_t)(64 + ctr_elems) * 4, 256);
+  const size_t b_part = align_up((size_t)std::max<int64_t>(part_elems, 1) * 2 * rs, 256);
+  const size_t b_ptr = align_up((size_t)count * 3 * 8, 256);
+  TENET_HIP_CHECK(hipMalloc(&g->dev, b_tab + b_prob + b_unit + b_ctr + b_part + b_ptr));
+  char* base = (char*)g->dev;
+  int64_t* tables = (int64_t*)base;
+  g->probs = (GroupedProblem*)(base + b_tab);
+  g->units = (GroupedUnit*)(base + b_tab + b_prob);
+  g->counters = (uint32_t*)(base + b_tab + b_prob + b_unit);
+  g->partials = (void*)(base + b_tab + b_prob + b_unit + b_ctr);
+  g->ptrs_dev = (uint64_t*)(base + b_tab + b_prob + b_unit + b_ctr + b_part);
+  std::vector<int64_t> host;
+  host.reserve(table_elems);
+  for (int64_t i = 0; i < count; ++i) {
+    size_t off[9];
+    for (int k = 0; k < 9; ++k) {
+      off[k] = host.size();
+      auto& v = hps[(size_t)i].v[k];
+      host.insert(host.end(), v.begin(), v.end());
+      std::vector<int64_t>().swap(v);
+    }
+    set_tables(probs[(size_t)i].t, tables, off);
+  }
+  if (!host.empty())
+    TENET_HIP_CHECK(hipMemcpy(tables, host.data(), host.size() * 8, hipMemcpyHostToDevice));
+  TENET_HIP_CHECK(hipMemcpy(g->probs, probs.data(), probs.size() * sizeof(GroupedProblem), hipMemcpyHostToDevice));
+  if (!units.empty())
+    TENET_HIP_CHECK(hipMemcpy(g->units, units.data(), units.size() * sizeof(GroupedUnit), hipMemcpyHostToDevice));
+  TENET_HIP_CHECK(hipMemset(g->counters, 0, b_ctr));
+  return TENET_OK;
+}
+
+extern "C" int tenet_grouped_plan(tenet_ctx ctx, int dtype, int prec, int64_t count, const tenet_tdesc* descs,
+                                  tenet_gplan* plan) {
   TRY(use_device(ctx));
   TRY(valid_dtype_prec(dtype, prec));
-  CHECK_ARG(count >= 0 && (count == 0 || (descs && ptrs)), "bad argument");
-  if (count == 0) return TENET_OK;
+  CHECK_ARG(plan && count >= 0 && (count == 0 || descs), "bad argument");
   CHECK_ARG(prec == TENET_PREC_NATIVE || (prec == TENET_PREC_FP32 && dtype == TENET_C64),
             "grouped contractions run in the dtype's native precision");
-  hipStream_t s = pick(ctx, stream);
-  std::vector<GroupedProblem> probs((size_t)count);
-  int64_t tiles = 0;
-  for (int64_t i = 0; i < count; ++i) {
-    tenet_plan p = nullptr;
-    TRY(cached_plan(ctx, s, dtype, prec, &descs[3 * i], &descs[3 * i + 1], &descs[3 * i + 2], &p));
-    GroupedProblem& g = probs[(size_t)i];
-    g.A = ptrs[3 * i];
-    g.B = ptrs[3 * i + 1];
-    g.C = ptrs[3 * i + 2];
-    g.t = p->t;
-    g.M = p->M;
-    g.N = p->N;
-    g.K = p->K;
-    g.batch = p->batch;
-    g.a_mfast = p->a_mfast;
-    g.b_nfast = p->b_nfast;
-    g.tiles_m = (p->M + 63) / 64;
-    g.tiles_n = (p->N + 63) / 64;
-    g.tile_begin = tiles;
-    tiles += g.tiles_m * g.tiles_n * p->batch;
+  auto* g = new tenet_gplan_s();
+  const int st = grouped_plan_build(ctx, dtype, prec, count, descs, -1, g);
+  if (st != TENET_OK) {
+    if (g->dev) (void)hipFree(g->dev);
+    delete g;
+    return st;
   }
-  void* w = nullptr;
-  TRY(ws_get(ctx, s, probs.size() * sizeof(GroupedProblem) + 256, &w));
-  TENET_HIP_CHECK(hipMemcpyAsync(w, probs.data(), probs.size() * sizeof(GroupedProblem), hipMemcpyHostToDevice, s));
-  TRY(launch_grouped_gemm((const GroupedProblem*)w, count, tiles, dtype, s));
-  // the host staging vector must outlive the copy
+  *plan = g;
+  return TENET_OK;
+}
+
+extern "C" int tenet_grouped_plan_info(tenet_gplan plan, int64_t* units, int64_t* split_tiles, int64_t shapes[3],
+                                       double* flops, double* bytes) {
+  CHECK_ARG(plan, "null plan");
+  if (units) *units = plan->nunits;
+  if (split_tiles) *split_tiles = plan->split_tiles;
+  if (shapes)
+    for (int i = 0; i < 3; ++i) shapes[i] = plan->shape_count[i];
+  if (flops) *flops = plan->flops;
+  if (bytes) *bytes = plan->bytes;
+  return TENET_OK;
+}
+
+static int grouped_launch(tenet_gplan g, hipStream_t s) {
+  GroupedLaunch L{};
+  L.probs = g->probs;
+  L.units = g->units;
+  L.nunits = g->nunits;
+  L.ptrs = g->ptrs_dev;
+  L.partials = g->partials;
+  L.counters = g->counters;
+  L.grid = g->grid;
+  L.dtype = g->dtype;
+  L.prec = g->prec;
+  StatScope st(g->ctx, s, TENET_KSTAT_GEMM, g->flops, g->bytes);
+  return launch_grouped_gemm(L, s);
+}
+
+extern "C" int tenet_grouped_exec(tenet_gplan plan, tenet_stream stream, void* const* ptrs) {
+  CHECK_ARG(plan, "null plan");
+  tenet_ctx ctx = plan->ctx;
+  TRY(use_device(ctx));
+  if (plan->count == 0) return TENET_OK;
+  CHECK_ARG(ptrs, "null pointer array");
+  hipStream_t s = pick(ctx, stream);
+  CHECK_ARG(!capturing(s), "tenet_grouped_exec copies host pointers: capture tenet_grouped_exec_dev instead");
+  const size_t es = ELEM[plan->dtype];
+  for (int64_t i = 0; i < plan->count; ++i) {
+    if (plan->spanC[(size_t)i] == 0) continue;
+    TRY(check_span(ctx, ptrs[3 * i + 2], (size_t)plan->spanC[(size_t)i] * es, "C"));
+    if (plan->spanA[(size_t)i]) TRY(check_span(ctx, ptrs[3 * i], (size_t)plan->spanA[(size_t)i] * es, "A"));
+    if (plan->spanB[(size_t)i]) TRY(check_span(ctx, ptrs[3 * i + 1], (size_t)plan->spanB[(size_t)i] * es, "B"));
+  }
+  // pointers travel with the launch; the host array may be reused on return
+  TENET_HIP_CHECK(hipMemcpyAsync(plan->ptrs_dev, ptrs, (size_t)plan->count * 3 * 8, hipMemcpyHostToDevice, s));
+  TRY(grouped_launch(plan, s));
   TENET_HIP_CHECK(hipStreamSynchronize(s));
+  return TENET_OK;
+}
+
+extern "C" int tenet_grouped_exec_dev(tenet_gplan plan, tenet_stream stream, const uint64_t* ptrs_dev) {
+  CHECK_ARG(plan, "null plan");
+  tenet_ctx ctx = plan->ctx;
+  TRY(use_device(ctx));
+  if (plan->count == 0) return TENET_OK;
+  CHECK_ARG(ptrs_dev, "null pointer array");
+  hipStream_t s = pick(ctx, stream);
+  TRY(check_span(ctx, ptrs_dev, (size_t)plan->count * 3 * 8, "pointer array"));
+  if ((const void*)ptrs_dev != (const void*)plan->ptrs_dev)
+    TENET_HIP_CHECK(hipMemcpyAsync(plan->ptrs_dev, ptrs_dev, (size_t)plan->count * 3 * 8, hipMemcpyDeviceToDevice, s));
+  return grouped_launch(plan, s);
+}
+
+extern "C" int tenet_grouped_plan_destroy(tenet_gplan plan) {
+  if (!plan) return TENET_OK;
+  (void)hipSetDevice(plan->ctx->device);
+  (void)hipDeviceSynchronize();
+  if (plan->dev) (void)hipFree(plan->dev);
+  delete plan;
+  return TENET_OK;
+}
+
+extern "C" int tenet_grouped_einsum(tenet_ctx ctx, tenet_stream stream, int dtype, int prec, int64_t count,
+                                    const tenet_tdesc* descs, void* const* ptrs) {
+  if (count == 0) return TENET_OK;
+  tenet_gplan g = nullptr;
+  TRY(tenet_grouped_plan(ctx, dtype, prec, count, descs, &g));
+  const int st = tenet_grouped_exec(g, stream, ptrs);
+  (void)tenet_grouped_plan_destroy(g);
+  return st;
+}
+
+// Autotuner: time each tile shape on a fixed synthetic batch (every problem
+// (256 x 1024) . (1024 x 256), inputs from the Gaussian generator) and keep
+// the measured cost per (tile, K step) for the shape choice of later plans.
+extern "C" int tenet_autotune_tiles(tenet_ctx ctx, int dtype, double costs_out[3]) {
+  TRY(use_device(ctx));
+  TRY(valid_dtype_prec(dtype, 0));
+  hipStream_t s = ctx->stream;
+  const int64_t M = 256, N = 256, K = 1024, P = 64;
+  const size_t es = ELEM[dtype];
+  void* buf = nullptr;
+  TENET_HIP_CHECK(hipMalloc(&buf, (size_t)P * (M * K + K * N + M * N) * es));
+  int st = launch_gaussian(buf, dtype, P * (M * K + K * N), 12345, 0, s);
+  std::vector<tenet_tdesc> descs;
+  std::vector<void*> ptrs;
+  for (int64_t p = 0; p < P; ++p) {
+    descs.push_back(mkdesc({M, K}, {1, 2}));
+    descs.push_back(mkdesc({K, N}, {2, 3}));
+    descs.push_back(mkdesc({M, N}, {1, 3}));
+    char* b = (char*)buf;
+    ptrs.push_back(b + (size_t)p * M * K * es);
+    ptrs.push_back(b + ((size_t)P * M * K + (size_t)p * K * N) * es);
+    ptrs.push_back(b + ((size_t)P * (M * K + K * N) + (size_t)p * M * N) * es);
+  }
+  double cost[3] = {0, 0, 0};
+  for (int v = 0; v < 3 && st == TENET_OK; ++v) {
+    tenet_gplan_s g;
+    st = grouped_plan_build(ctx, dtype, TENET_PREC_NATIVE, P, descs.data(), v, &g);
+    if (st == TENET_OK) {
+      (void)hipMemcpy(g.ptrs_dev, ptrs.data(), ptrs.size() * 8, hipMemcpyHostToDevice);
+      float best = 1e30f;
+      for (int rep = 0; rep < 4 && st == TENET_OK; ++rep) {
+        (void)hipEventRecord(ctx->ev0, s);
+        st = grouped_launch(&g, s);
+        (void)hipEventRecord(ctx->ev1, s);
+        (void)hipEventSynchronize(ctx->ev1);
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+        if (rep > 0) best = std::min(best, ms);
+      }
+      const int64_t e = TILE_EDGE[v];
+      cost[v] = (double)best / ((double)P * ((M + e - 1) / e) * ((N + e - 1) / e) * (K / 16));
+    }
+    if (g.dev) (void)hipFree(g.dev);
+  }
+  (void)hipFree(buf);
+  TRY(st);
+  const int d = dtype == TENET_C128 ? 0 : 1;
+  for (int v = 0; v < 3; ++v) ctx->tile_cost[d][v] = cost[v] / cost[0];
+  if (costs_out)
+    for (int v = 0; v < 3; ++v) costs_out[v] = ctx->tile_cost[d][v];
   return TENET_OK;
 }
 

@@ -83,6 +83,14 @@ SIGNATURES = {
     "tenet_memcpy_d2d": [_p, _p, _p, _sz, _p],
     "tenet_memset_zero": [_p, _p, _sz, _p],
     "tenet_mem_info": [_p, C.POINTER(_sz), C.POINTER(_sz), C.POINTER(_sz)],
+    "tenet_host_alloc": [_p, _sz, C.POINTER(_p)],
+    "tenet_host_free": [_p, _p],
+    "tenet_memcpy_async": [_p, _p, _p, _sz, _i, _p],
+    "tenet_event_create": [_p, C.POINTER(_p)],
+    "tenet_event_destroy": [_p, _p],
+    "tenet_event_record": [_p, _p, _p],
+    "tenet_event_query": [_p, _p, C.POINTER(C.c_int)],
+    "tenet_stream_wait_event": [_p, _p, _p],
     "tenet_einsum_plan": [_p, _i, _i, C.POINTER(TDesc), C.POINTER(TDesc), C.POINTER(TDesc), C.POINTER(_p)],
     "tenet_einsum_exec": [_p, _p, _p, _p, _p, _i, _i, _pd, _pd],
     "tenet_plan_destroy": [_p],
@@ -95,7 +103,15 @@ SIGNATURES = {
     "tenet_overlap": [_p, _p, _i, _i, _i64, _i64, _pi64, C.POINTER(_p), _pi64, C.POINTER(_p), _p],
     "tenet_amplitudes": [_p, _p, _i, _i64, _p, _p, _p, _p, _i64, _p],
     "tenet_expect_pauli": [_p, _p, _i, _i, _i64, _i64, _i64, _pi64, C.POINTER(_p), _p, C.c_char_p, _p],
+    "tenet_left_env": [_p, _p, _i, _i, _i64, _i64, _pi64, C.POINTER(_p), _p, _p],
+    "tenet_left_envs": [_p, _p, _i, _i, _i64, _i64, _pi64, C.POINTER(_p), C.POINTER(_p), _p],
     "tenet_grouped_einsum": [_p, _p, _i, _i, _i64, C.POINTER(TDesc), C.POINTER(_p)],
+    "tenet_grouped_plan": [_p, _i, _i, _i64, C.POINTER(TDesc), C.POINTER(_p)],
+    "tenet_grouped_exec": [_p, _p, C.POINTER(_p)],
+    "tenet_grouped_exec_dev": [_p, _p, _p],
+    "tenet_grouped_plan_info": [_p, _pi64, _pi64, _pi64, _pd, _pd],
+    "tenet_grouped_plan_destroy": [_p],
+    "tenet_autotune_tiles": [_p, _i, _pd],
     "tenet_grouped_svd": [_p, _p, _i, _i64, _pi64, _pi64, C.POINTER(_p), C.POINTER(_p), C.POINTER(_p),
                           C.POINTER(_p), _pi64, _d, _p, _p],
     "tenet_fill_gaussian": [_p, _p, _i, _i64, _p, C.c_uint64, C.c_uint64],

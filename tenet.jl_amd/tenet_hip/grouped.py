@@ -1,0 +1,131 @@
+"""Grouped launches: many independent contractions (or thin SVDs) in one
+persistent launch.
+
+A ``GroupedPlan`` is built once from the problems' shapes and labels
+(offset tables, per-problem tile shape, K chunking, cost-sorted work queue,
+all in one device allocation) and executed many times on fresh operand
+pointers.  Each problem has the semantics of ``binary_einsum``
+(Muscle.binary_einsum at src/Operations/canonize.jl:44-61, evolve.jl:29-119):
+labels in a and b but not in c are contracted, labels in all three are
+batched.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+from .device import DTYPES, Context, DeviceArray, tdesc
+
+
+def _codes(problems):
+    m = {}
+
+    def code(lab):
+        if lab not in m:
+            m[lab] = len(m) + 1
+        return m[lab]
+
+    return code
+
+
+class GroupedPlan:
+    """problems: sequence of (shape_a, labels_a, shape_b, labels_b,
+    shape_c, labels_c) with column-major contiguous operands."""
+
+    def __init__(self, ctx: Context, problems, dtype=np.complex64, prec=L.PREC_NATIVE):
+        self.ctx = ctx
+        self.dtype = np.dtype(dtype)
+        self.code = DTYPES[self.dtype]
+        self.count = len(problems)
+        descs = (L.TDesc * max(3 * self.count, 1))()
+        for i, (sa, la, sb, lb, sc, lc) in enumerate(problems):
+            code = _codes(None)
+            descs[3 * i] = tdesc(sa, [code(x) for x in la])
+            descs[3 * i + 1] = tdesc(sb, [code(x) for x in lb])
+            descs[3 * i + 2] = tdesc(sc, [code(x) for x in lc])
+        self.shapes = [(tuple(p[0]), tuple(p[2]), tuple(p[4])) for p in problems]
+        h = C.c_void_p()
+        L.check(L.lib.tenet_grouped_plan(ctx.h, self.code, prec, self.count, descs, C.byref(h)), "grouped_plan")
+        self.h = h
+
+    def info(self):
+        u, st = C.c_int64(), C.c_int64()
+        shapes = (C.c_int64 * 3)()
+        fl, by = C.c_double(), C.c_double()
+        L.check(L.lib.tenet_grouped_plan_info(self.h, C.byref(u), C.byref(st), shapes, C.byref(fl), C.byref(by)))
+        return {"units": u.value, "split_tiles": st.value, "tiles_64_32_16": list(shapes), "flops": fl.value,
+                "bytes": by.value}
+
+    def exec(self, ptrs, stream=None):
+        """ptrs: 3 * count device addresses (A, B, C per problem).  Returns
+        when the launch has completed."""
+        arr = (C.c_void_p * (3 * self.count))(*[int(p) for p in ptrs])
+        L.check(L.lib.tenet_grouped_exec(self.h, stream, arr), "grouped_exec")
+
+    def exec_dev(self, ptrs_dev: int, stream=None):
+        """ptrs_dev: device address of 3 * count uint64 pointers; asynchronous."""
+        L.check(L.lib.tenet_grouped_exec_dev(self.h, stream, C.c_void_p(int(ptrs_dev))), "grouped_exec_dev")
+
+    def close(self):
+        if getattr(self, "h", None) is not None:
+            L.lib.tenet_grouped_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def autotune_tiles(ctx: Context, dtype=np.complex64):
+    """Measure the per-(tile, K step) cost of the 64x64, 32x32 and 16x16 tile
+    shapes on the device; later plans choose shapes with these costs."""
+    out = (C.c_double * 3)()
+    L.check(L.lib.tenet_autotune_tiles(ctx.h, DTYPES[np.dtype(dtype)], out), "autotune_tiles")
+    return list(out)
+
+
+def grouped_svd(ctx: Context, mats, maxdims=None, threshold=0.0):
+    """Thin SVDs of device matrices (DeviceArray, column-major m x n) in one
+    grouped launch with per-problem truncation (compress.jl:46-59 rule).
+    Returns per problem (U (m x kmax), S (kmax,), Vh (kmax x n), k, discarded
+    weight) with only the first k columns / entries / rows meaningful."""
+    cnt = len(mats)
+    if cnt == 0:
+        return []
+    dt = mats[0].dtype
+    code = DTYPES[dt]
+    ms = np.array([a.shape[0] for a in mats], dtype=np.int64)
+    ns = np.array([a.shape[1] for a in mats], dtype=np.int64)
+    kmax = np.minimum(ms, ns)
+    md = np.zeros(cnt, dtype=np.int64) if maxdims is None else np.asarray(maxdims, dtype=np.int64)
+    Us = [DeviceArray(ctx, (int(m), int(k)), dt) for m, k in zip(ms, kmax)]
+    Vs = [DeviceArray(ctx, (int(k), int(n)), dt) for k, n in zip(kmax, ns)]
+    Ss = [ctx.malloc(int(k) * 8) for k in kmax]
+    kd = ctx.malloc(cnt * 8)
+    dd = ctx.malloc(cnt * 8)
+    try:
+        P = C.c_void_p
+        L.check(L.lib.tenet_grouped_svd(
+            ctx.h, None, code, cnt, ms.ctypes.data_as(C.POINTER(C.c_int64)), ns.ctypes.data_as(C.POINTER(C.c_int64)),
+            (P * cnt)(*[a.ptr for a in mats]), (P * cnt)(*[u.ptr for u in Us]), (P * cnt)(*Ss),
+            (P * cnt)(*[v.ptr for v in Vs]), md.ctypes.data_as(C.POINTER(C.c_int64)), float(threshold), P(kd), P(dd)),
+            "grouped_svd")
+        kh = np.zeros(cnt, dtype=np.int64)
+        dh = np.zeros(cnt)
+        L.check(L.lib.tenet_memcpy_d2h(ctx.h, kh.ctypes.data_as(P), P(kd), kh.nbytes, None))
+        L.check(L.lib.tenet_memcpy_d2h(ctx.h, dh.ctypes.data_as(P), P(dd), dh.nbytes, None))
+        out = []
+        for i in range(cnt):
+            s = np.zeros(int(kmax[i]))
+            L.check(L.lib.tenet_memcpy_d2h(ctx.h, s.ctypes.data_as(P), P(Ss[i]), s.nbytes, None))
+            out.append((Us[i], s, Vs[i], int(kh[i]), float(dh[i])))
+        return out
+    finally:
+        for p in Ss:
+            ctx.free(p)
+        ctx.free(kd)
+        ctx.free(dd)

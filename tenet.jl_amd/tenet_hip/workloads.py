@@ -112,3 +112,55 @@ def serving_state_shapes(rng, n_lo=16, n_hi=128, chi_lo=16, chi_hi=512, d=2):
         e = min(b, n - b)
         bonds.append(chi if e >= 20 else min(chi, d ** e))
     return n, chi, bonds
+
+
+def build_serving_states(ctx, seed: int, count: int, n_lo=16, n_hi=128, chi_lo=16, chi_hi=512,
+                         dtype=np.complex64):
+    """``count`` device-resident MPS states: shapes from serving_state_shapes,
+    complex Gaussian entries generated on the device (state j uses seed
+    seed + j, its sites consecutive offsets of that stream), then
+    canonize!(psi, site 1) and normalize!.  Returns ServedState objects."""
+    from .device import DeviceArray
+    from .mps import MPS
+    from .serving import ServedState
+
+    rng = np.random.default_rng(seed)
+    out = []
+    for j in range(count):
+        n, chi, bonds = serving_state_shapes(rng, n_lo, n_hi, chi_lo, chi_hi)
+        dims = [1] + bonds + [1]
+        sites = []
+        off = 0
+        for i in range(n):
+            a = DeviceArray(ctx, (dims[i], 2, dims[i + 1]), dtype).gaussian_(seed + j, off)
+            off += a.size
+            sites.append(a)
+        psi = MPS(ctx, sites, dtype=dtype)
+        psi.canonize_(0)
+        psi.normalize_()
+        out.append(ServedState(ctx, psi))
+    return out
+
+
+def serving_requests(seed: int, states, count: int, amp_fraction=0.5, rate=None):
+    """``count`` requests over the states: with probability amp_fraction a
+    bitstring amplitude (uniform x), else a Pauli string of weight 1-4 on
+    consecutive sites at a uniform start with letters uniform in X, Y, Z.
+    With ``rate`` (requests/s) also returns Poisson arrival times."""
+    from .serving import Request
+
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for i in range(count):
+        sid = int(rng.integers(0, len(states)))
+        n = states[sid].n
+        if rng.random() < amp_fraction:
+            reqs.append(Request(i, sid, "amp", bits=rng.integers(0, 2, n).astype(np.uint8)))
+        else:
+            w = int(rng.integers(1, 5))
+            start = int(rng.integers(0, n - w + 1))
+            reqs.append(Request(i, sid, "pauli", start=start, paulis="".join(rng.choice(list("XYZ"), w))))
+    if rate is None:
+        return reqs
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, count))
+    return reqs, arrivals

@@ -1,0 +1,136 @@
+"""Grouped launches against the oracle: many uneven contractions in one
+persistent launch (every output against a per-problem binary_einsum of the
+oracle), K-chunked tiles (deterministic chunk-order sums, repeated launches
+of one plan), the tile autotuner, and grouped truncated SVDs.
+
+Tolerances: complex64 problems within 2e-5 relative (norm-wise, FP32 MFMA
+accumulation over K <= ~1e5 terms); complex128 within 1e-12.
+"""
+import numpy as np
+import pytest
+
+from oracle import tenet_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(np.asarray(b)), 1e-300)
+
+
+def _run_grouped(ctx, specs, dtype, seed):
+    import tenet_hip as T
+    from tenet_hip.grouped import GroupedPlan
+
+    rng = np.random.default_rng(seed)
+    probs, hosts, outs, ptrs = [], [], [], []
+    for sp in specs:
+        a = O.complex_gaussian(rng, sp.shape(sp.la), dtype)
+        b = O.complex_gaussian(rng, sp.shape(sp.lb), dtype)
+        da, db = T.DeviceArray.from_host(ctx, a, dtype), T.DeviceArray.from_host(ctx, b, dtype)
+        dc = T.DeviceArray(ctx, sp.shape(sp.lc), dtype)
+        probs.append((sp.shape(sp.la), sp.la, sp.shape(sp.lb), sp.lb, sp.shape(sp.lc), sp.lc))
+        hosts.append((a, b))
+        outs.append((da, db, dc))
+        ptrs += [da.ptr, db.ptr, dc.ptr]
+    plan = GroupedPlan(ctx, probs, dtype)
+    return plan, hosts, outs, ptrs
+
+
+def _check(specs, hosts, outs, tol):
+    for sp, (a, b), (_, _, dc) in zip(specs, hosts, outs):
+        dims = [l for l in sp.la if l in sp.lb and l not in sp.lc]
+        ref = O.binary_einsum(O.OTensor(a.astype(np.complex128), sp.la), O.OTensor(b.astype(np.complex128), sp.lb),
+                              dims)
+        ref = ref.permute(sp.lc).data
+        assert _rel(dc.to_host(), ref) < tol, (sp.la, sp.lb, sp.extent)
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.complex64, 2e-5), (np.complex128, 1e-12)])
+def test_grouped_random_problems(ctx, dtype, tol):
+    """300 problems from the grouped workload generator with a small size
+    cap: ranks 3-5, 1-3 contracted labels, extents log-uniform 2..128."""
+    from tenet_hip.workloads import grouped_contractions
+
+    specs = grouped_contractions(11, 300, cap=1 << 12)
+    plan, hosts, outs, ptrs = _run_grouped(ctx, specs, dtype, 5)
+    info = plan.info()
+    assert sum(info["tiles_64_32_16"]) == len(specs)
+    plan.exec(ptrs)
+    _check(specs, hosts, outs, tol)
+    plan.close()
+
+
+def test_grouped_k_chunks_deterministic(ctx):
+    """A few long-K problems among many small ones force K chunking; the
+    chunk-order reduction makes repeated launches of one plan bit-identical."""
+    from tenet_hip.workloads import ContractionSpec, grouped_contractions
+
+    specs = grouped_contractions(12, 400, cap=1 << 12)
+    for j in range(6):
+        specs.append(ContractionSpec(["m", "k1", "k2"], ["k1", "k2", "n"], ["m", "n"],
+                                     {"m": 3 + 17 * j, "n": 5 + 9 * j, "k1": 97, "k2": 523}))
+    plan, hosts, outs, ptrs = _run_grouped(ctx, specs, np.complex64, 6)
+    info = plan.info()
+    assert info["split_tiles"] > 0, info
+    plan.exec(ptrs)
+    first = [o[2].to_host() for o in outs]
+    _check(specs, hosts, outs, 2e-5)
+    for o in outs:
+        o[2].zero_()
+    plan.exec(ptrs)
+    for f, o in zip(first, outs):
+        assert np.array_equal(f, o[2].to_host())
+    plan.close()
+
+
+def test_grouped_batch_labels_and_empty(ctx):
+    """Batch (Hadamard) labels kept in the output, an output with permuted
+    labels, and a contraction over an empty extent (zero result)."""
+    from tenet_hip.workloads import ContractionSpec
+
+    specs = [ContractionSpec(["i", "k", "z"], ["k", "j", "z"], ["i", "j", "z"], {"i": 20, "j": 33, "k": 40, "z": 7}),
+             ContractionSpec(["i", "k"], ["k", "j"], ["j", "i"], {"i": 70, "j": 3, "k": 1000}),
+             ContractionSpec(["i", "k"], ["k", "j"], ["i", "j"], {"i": 4, "j": 5, "k": 0})]
+    plan, hosts, outs, ptrs = _run_grouped(ctx, specs, np.complex128, 7)
+    plan.exec(ptrs)
+    _check(specs, hosts, outs, 1e-12)
+    assert np.all(outs[2][2].to_host() == 0)
+    plan.close()
+
+
+def test_autotune_then_plan(ctx):
+    from tenet_hip.grouped import autotune_tiles
+    from tenet_hip.workloads import grouped_contractions
+
+    costs = autotune_tiles(ctx, np.complex64)
+    assert len(costs) == 3 and costs[0] == pytest.approx(1.0) and all(c > 0 for c in costs)
+    specs = grouped_contractions(13, 100, cap=1 << 12)
+    plan, hosts, outs, ptrs = _run_grouped(ctx, specs, np.complex64, 8)
+    plan.exec(ptrs)
+    _check(specs, hosts, outs, 2e-5)
+    plan.close()
+
+
+def test_grouped_svd_vs_lapack(ctx):
+    """Grouped truncated SVDs: singular values against LAPACK, the kept
+    count from the truncation rule (maxdim = ceil(min(m, n) / 2)), the
+    discarded weight, and U diag(S) Vh against the best rank-k product."""
+    import tenet_hip as T
+    from tenet_hip.grouped import grouped_svd
+
+    rng = np.random.default_rng(9)
+    shapes = [(16, 16), (40, 17), (17, 40), (100, 64), (64, 100), (33, 33), (200, 20), (20, 200)]
+    hosts = [O.complex_gaussian(rng, s) for s in shapes]
+    mats = [T.DeviceArray.from_host(ctx, h) for h in hosts]
+    maxdims = [(min(s) + 1) // 2 for s in shapes]
+    res = grouped_svd(ctx, mats, maxdims, 1e-10)
+    for h, md, (U, S, Vh, k, disc) in zip(hosts, maxdims, res):
+        s_ref = np.linalg.svd(h, compute_uv=False)
+        assert k == md
+        assert np.allclose(S[:k], s_ref[:k], rtol=1e-12, atol=1e-12 * s_ref[0])
+        assert disc == pytest.approx(np.sum(s_ref[k:] ** 2), rel=1e-10)
+        Uh, Vhh = U.to_host()[:, :k], Vh.to_host()[:k, :]
+        u, s, vh = np.linalg.svd(h, full_matrices=False)
+        best = (u[:, :k] * s[:k]) @ vh[:k]
+        assert _rel((Uh * S[:k]) @ Vhh, best) < 1e-11
