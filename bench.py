@@ -304,14 +304,11 @@ def bench_grouped(args):
     log(f"grouped contractions: {t_c * 1e3:.2f} ms, {fl_c / t_c / 1e12:.2f} TFLOP/s, {by_c / t_c / 1e9:.0f} GB/s")
 
     svd_res = None
-    ctx.stats_enable(L.KSTAT_SVD_ROUND)
     t0 = time.perf_counter()
     for _ in range(max(1, args.steps)):
         svd_res = grouped_svd(ctx, mats, maxdims, 1e-10)
         ctx.sync()
     t_s = (time.perf_counter() - t0) / max(1, args.steps)
-    kst_s = ctx.stats_read()
-    ctx.stats_enable(L.KSTAT_OFF)
     ctx.stats_sample_every(17)
     log(f"grouped SVDs: {t_s * 1e3:.1f} ms, {fl_s / t_s / 1e12:.2f} TFLOP/s nominal")
 
@@ -341,8 +338,8 @@ def bench_grouped(args):
         offs = np.concatenate([[0], np.cumsum(ms * ns)])
         for i in rng.choice(len(ms), 16, replace=False):
             m, n = int(ms[i]), int(ns[i])
-            Ah = synth.gaussian(GROUPED_SEED + 1000, m * n, int(offs[i])).reshape(m, n, order="F")
-            s_ref = np.linalg.svd(Ah.astype(np.complex128), compute_uv=False)
+            Ah = synth.gaussian(GROUPED_SEED + 1000, m * n, int(offs[i]), np.complex64).reshape(m, n, order="F")
+            s_ref = np.linalg.svd(Ah.astype(np.complex128), compute_uv=False)  # the same complex64 input
             k = svd_res[int(i)][3]
             serr.append(float(np.max(np.abs(svd_res[int(i)][1][:k] - s_ref[:k])) / s_ref[0]))
         rel_s = max(serr)
@@ -402,7 +399,6 @@ def bench_grouped(args):
                                     "gbytes": round(by_c / 1e9, 2), "plan": info, "tile_costs": costs,
                                     "max_relerr_vs_fp64_oracle_48_sampled": rel_c},
                    "svds": {"ms": round(t_s * 1e3, 3), "gflop_nominal": round(fl_s / 1e9, 1),
-                            "svd_round_launches": kst_s["launches"],
                             "max_sigma_err_rel_sigma1_16_sampled": rel_s}},
         "roofline": roof, "cpu_baseline": cpu,
     }
@@ -511,6 +507,50 @@ def bench_serving(args):
     print(json.dumps(line), flush=True)
     for st in states:
         st.close()
+    ctx.close()
+
+
+# ---------------------------------------------------------------- single SVD (profiling unit)
+def bench_svd(args):
+    """tenet_svd_thin of one seeded 512 x 512 complex128 Gaussian matrix with
+    maxdim 256 and threshold 1e-10 (the shape of a saturated TEBD bond),
+    repeated --steps times: the unit the SVD kernels' counters are read on."""
+    import tenet_hip as T
+    from tenet_hip.tensor import tensor_svd_thin
+
+    ctx = T.Context(0)
+    A = T.Tensor(T.DeviceArray(ctx, (512, 512)).gaussian_(77), ["r", "c"])
+    for _ in range(args.warmup):
+        tensor_svd_thin(A, ["r"], "s", ["c"], maxdim=256, threshold=1e-10)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        tensor_svd_thin(A, ["r"], "s", ["c"], maxdim=256, threshold=1e-10)
+    ctx.sync()
+    dt = (time.perf_counter() - t0) / args.steps
+    print(json.dumps({"metric": "svd 512x512 c128 maxdim 256", "ms": round(dt * 1e3, 3)}), flush=True)
+    ctx.close()
+
+
+# ---------------------------------------------------------------- traffic calibration
+def bench_copy(args):
+    """x *= alpha over --copy-mib MiB of complex128, five times (tenet_scale:
+    every element read once and written once).  Its FETCH_SIZE / WRITE_SIZE
+    counters per dispatch against the known bytes calibrate the traffic
+    figures of the other kernels."""
+    import tenet_hip as T
+
+    ctx = T.Context(0)
+    n = args.copy_mib * (1 << 20) // 16
+    a = T.DeviceArray(ctx, (n,)).gaussian_(1)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        a.scale_(1.0)
+    ctx.sync()
+    dt = (time.perf_counter() - t0) / 5
+    print(json.dumps({"metric": "scale kernel traffic calibration", "read_bytes": a.nbytes, "write_bytes": a.nbytes,
+                      "ms": round(dt * 1e3, 3), "gbps": round(2 * a.nbytes / dt / 1e9, 1)}), flush=True)
     ctx.close()
 
 
@@ -632,7 +672,10 @@ def main():
     ap.add_argument("--forms", default="fp64,split_bf16")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="tebd", choices=["tebd", "heff", "plain", "grouped", "serving"])
+    ap.add_argument("--gates", type=int, default=0,
+                    help="profile only: apply the first N gates of the sweep (amplitude check skipped)")
+    ap.add_argument("--copy-mib", type=int, default=1024, help="traffic calibration buffer (workload copy)")
+    ap.add_argument("--workload", default="tebd", choices=["tebd", "heff", "plain", "grouped", "serving", "copy", "svd"])
     ap.add_argument("--rates", default="250,500,1000,2000,4000,8000,16000")
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     args = ap.parse_args()
@@ -650,6 +693,10 @@ def main():
         return bench_grouped(args)
     if args.workload == "serving":
         return bench_serving(args)
+    if args.workload == "copy":
+        return bench_copy(args)
+    if args.workload == "svd":
+        return bench_svd(args)
 
     import numpy as np
     import tenet_hip as T
@@ -660,6 +707,17 @@ def main():
     ref_amps = fx["amps"]
     sched = brickwork_schedule(N_SITES, DEPTH)
     gates = haar_gates(SEED, len(sched))
+    if args.gates:
+        # profiling runs: a prefix of the sweep, no accuracy check, no JSON line
+        import tenet_hip.tebd as TB
+        ctx = T.Context(0)
+        gv = upload_gates(ctx, gates[:args.gates])
+        t0 = time.perf_counter()
+        psi = TB.tebd_sweep(ctx, N_SITES, sched[:args.gates], gv, CHI, THR)
+        ctx.sync()
+        log(f"{args.gates} gates in {time.perf_counter() - t0:.2f}s, bonds {psi.bondsizes()}")
+        ctx.close()
+        return
     ctx = T.Context(0)
     gate_views = upload_gates(ctx, gates)
     results = {}

@@ -104,7 +104,7 @@ struct SvdProblem {
   double* Gpart;  // npairs * q partial Gram blocks (2b x 2b complex)
   double* sigma;  // ncp
   int32_t* perm;  // ncp
-  int32_t* flag;  // [0] = rotated in this sweep, [1] = converged, [2] = sweeps
+  int32_t* flag;  // [0] rotated in this sweep, [1] converged, [2] sweeps; flag + 4: the sweep's max relative off-diagonal (double)
   double* fro2;   // ||A||_F^2 (absolute rotation floor eps^2 * ||A||_F^2)
   int32_t nr, ncols, ncp, b, nblk, q;
   int32_t m, n, transposed;
@@ -126,7 +126,7 @@ struct SvdBatch {
   const SvdProblem* arr;
   SvdProblem one;
 };
-int launch_svd_init(const SvdBatch& b, int64_t nprob, int64_t max_elems, int dtype, hipStream_t s);
+int launch_svd_init(const SvdBatch& b, int64_t nprob, int64_t max_elems, int max_ncp, int dtype, hipStream_t s);
 int launch_svd_round(const SvdBatch& b, int64_t nprob, int max_items, int round, int phase, int mode,
                      hipStream_t s);
 int launch_svd_sweep_end(const SvdBatch& b, int64_t nprob, hipStream_t s);

@@ -703,6 +703,183 @@ __global__ __launch_bounds__(QP_THREADS) void qr_panel_reg_kernel(double* __rest
   }
 }
 
+// Register-resident panel: thread t owns panel row t (up to 512 rows x QB
+// columns, one complex double per column in registers; the column index is
+// unrolled so every register index is a compile-time constant).  Per
+// reflector: one block reduction for the column norm and one for the 2*QB
+// dot products (v^H x_c for the trailing columns, v_c^H v for T), the latter
+// a butterfly transpose-reduction inside each wave (32 shuffles for 32
+// values) plus an 8-way LDS sum.  Three barriers per column in all.
+// Same inputs and outputs as qr_panel_reg_kernel.
+template <int H, int BIT>
+__device__ __forceinline__ void halve_step(double (&v)[2 * QB], int lane) {
+  const bool hi = (lane & BIT) != 0;
+#pragma unroll
+  for (int i = 0; i < H; ++i) {
+    const double keep = hi ? v[H + i] : v[i];
+    const double send = hi ? v[i] : v[H + i];
+    v[i] = keep + __shfl_xor(send, BIT, 64);
+  }
+}
+__device__ __forceinline__ void wave_reduce32(double (&v)[2 * QB], int lane) {
+  // after the five halving steps lane l holds the wave total of value l >> 1
+  static_assert(2 * QB == 32, "the butterfly below reduces 32 values");
+  halve_step<16, 32>(v, lane);
+  halve_step<8, 16>(v, lane);
+  halve_step<4, 8>(v, lane);
+  halve_step<2, 4>(v, lane);
+  halve_step<1, 2>(v, lane);
+  v[0] += __shfl_xor(v[0], 1, 64);
+}
+
+__global__ __launch_bounds__(QP_THREADS) void qr_panel_v2_kernel(double* __restrict__ W, double* __restrict__ tau,
+                                                                  double* __restrict__ Tout, int m, int j0, int jb) {
+  constexpr int NW = QP_THREADS / 64;
+  extern __shared__ double pan[];  // pr[QB][512], pi[QB][512]
+  __shared__ double n_part[NW];
+  __shared__ double alpha_s[2];
+  __shared__ double d_part[NW][2 * QB];
+  __shared__ double d_all[2 * QB];
+  __shared__ double Tr_s[QB][QB], Ti_s[QB][QB];
+  double* pr = pan;
+  double* pi = pan + QB * QP_MAX_ROWS;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int mr = m - j0;
+  const int r = tid;
+  const bool have = r < mr;
+  double2* Wc = reinterpret_cast<double2*>(W);
+#pragma unroll
+  for (int c = 0; c < QB; ++c) {
+    double xr = 0.0, xi = 0.0;
+    if (have && c < jb) {
+      const double2 v = Wc[(size_t)(j0 + c) * m + j0 + r];
+      xr = v.x;
+      xi = v.y;
+    }
+    pr[c * QP_MAX_ROWS + r] = xr;
+    pi[c * QP_MAX_ROWS + r] = xi;
+  }
+  for (int e = tid; e < QB * QB; e += QP_THREADS) {
+    Tr_s[e % QB][e / QB] = 0.0;
+    Ti_s[e % QB][e / QB] = 0.0;
+  }
+  // this thread reads and writes only its own row of the panel, so the
+  // barriers below only order the reductions
+#pragma unroll 1
+  for (int jj = 0; jj < jb; ++jj) {
+    const double ajr = pr[jj * QP_MAX_ROWS + r], aji = pi[jj * QP_MAX_ROWS + r];
+    // ---- reflector for column jj
+    const double part = wave_sum((have && r > jj) ? ajr * ajr + aji * aji : 0.0);
+    if (lane == 0) n_part[wave] = part;
+    if (r == jj) {
+      alpha_s[0] = ajr;
+      alpha_s[1] = aji;
+    }
+    __syncthreads();
+    double xn2 = 0.0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) xn2 += n_part[w];
+    const double ar = alpha_s[0], ai = alpha_s[1];
+    double t_r = 0.0, t_i = 0.0, sr = 0.0, si = 0.0, beta = ar;
+    const bool ident = (xn2 == 0.0 && ai == 0.0);
+    if (!ident) {
+      const double nrm = sqrt(ar * ar + ai * ai + xn2);
+      beta = (ar >= 0.0) ? -nrm : nrm;
+      t_r = (beta - ar) / beta;
+      t_i = -ai / beta;
+      const double dr = ar - beta, di = ai;
+      const double den = dr * dr + di * di;
+      sr = dr / den;
+      si = -di / den;
+    }
+    double vr = 0.0, vi = 0.0;
+    if (have && r == jj) {
+      vr = 1.0;
+    } else if (have && r > jj) {
+      vr = ident ? ajr : ajr * sr - aji * si;
+      vi = ident ? aji : ajr * si + aji * sr;
+    }
+    // ---- dots: c > jj: v^H x_c ; c < jj: v_c^H v (v_c: 1 at row c, stored below)
+    double dv[2 * QB];
+#pragma unroll
+    for (int c = 0; c < QB; ++c) {
+      const double xr = pr[c * QP_MAX_ROWS + r], xi = pi[c * QP_MAX_ROWS + r];
+      double a = 0.0, b = 0.0;
+      if (c > jj) {
+        a = vr * xr + vi * xi;
+        b = vr * xi - vi * xr;
+      } else if (c < jj) {
+        const double ur = (r == c) ? 1.0 : (r > c ? xr : 0.0);
+        const double ui = (r > c) ? xi : 0.0;
+        a = ur * vr + ui * vi;
+        b = ur * vi - ui * vr;
+      }
+      dv[2 * c] = a;
+      dv[2 * c + 1] = b;
+    }
+    wave_reduce32(dv, lane);
+    if ((lane & 1) == 0) d_part[wave][lane >> 1] = dv[0];
+    __syncthreads();
+    if (tid < 2 * QB) {
+      double t = 0.0;
+#pragma unroll
+      for (int w = 0; w < NW; ++w) t += d_part[w][tid];
+      d_all[tid] = t;
+    }
+    __syncthreads();
+    // ---- trailing update of this row, then the reflector into column jj
+    if (!ident && have && r >= jj) {
+#pragma unroll
+      for (int c = 0; c < QB; ++c) {
+        if (c > jj) {
+          const double dr = d_all[2 * c], di = d_all[2 * c + 1];
+          const double fr = t_r * dr + t_i * di;
+          const double fi = t_r * di - t_i * dr;
+          pr[c * QP_MAX_ROWS + r] -= vr * fr - vi * fi;
+          pi[c * QP_MAX_ROWS + r] -= vr * fi + vi * fr;
+        }
+      }
+    }
+    if (have && r > jj) {
+      pr[jj * QP_MAX_ROWS + r] = vr;
+      pi[jj * QP_MAX_ROWS + r] = vi;
+    } else if (r == jj && !ident) {
+      pr[jj * QP_MAX_ROWS + r] = beta;
+      pi[jj * QP_MAX_ROWS + r] = 0.0;
+    }
+    // T[a][jj] = -tau * sum_{c=a}^{jj-1} T[a][c] (v_c^H v), thread a < jj (its own row of T)
+    if (tid < jj) {
+      double wr = 0.0, wi = 0.0;
+      for (int c = tid; c < jj; ++c) {
+        wr += Tr_s[tid][c] * d_all[2 * c] - Ti_s[tid][c] * d_all[2 * c + 1];
+        wi += Tr_s[tid][c] * d_all[2 * c + 1] + Ti_s[tid][c] * d_all[2 * c];
+      }
+      Tr_s[tid][jj] = -(t_r * wr - t_i * wi);
+      Ti_s[tid][jj] = -(t_r * wi + t_i * wr);
+    }
+    if (tid == jj) {
+      Tr_s[jj][jj] = t_r;
+      Ti_s[jj][jj] = t_i;
+      tau[2 * (j0 + jj)] = t_r;
+      tau[2 * (j0 + jj) + 1] = t_i;
+    }
+  }
+  __syncthreads();
+  if (have) {
+    for (int c = 0; c < jb; ++c) {
+      double2 v;
+      v.x = pr[c * QP_MAX_ROWS + r];
+      v.y = pi[c * QP_MAX_ROWS + r];
+      Wc[(size_t)(j0 + c) * m + j0 + r] = v;
+    }
+  }
+  for (int e = tid; e < QB * QB; e += QP_THREADS) {
+    const int i = e % QB, j = e / QB;
+    Tout[2 * e] = (i < jb && j < jb) ? Tr_s[i][j] : 0.0;
+    Tout[2 * e + 1] = (i < jb && j < jb) ? Ti_s[i][j] : 0.0;
+  }
+}
+
 // M[j0:m, c0:c0+16] <- (I - V T' V^H) M, register-blocked.  Thread t: row
 // group g = t / 16 (rows g, g+16, ... of each staged chunk) and a 4 x 4 block
 // (a0 = 4*((t%16)/4), c0 = 4*(t%4)) of Y = V^H X; the 16 row-group partials
@@ -938,6 +1115,8 @@ int qr_setup_attributes() {
   if (done) return TENET_OK;
   TENET_HIP_CHECK(hipFuncSetAttribute((const void*)qr_panel_reg_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                       2 * QB * QP_MAX_ROWS * (int)sizeof(double)));
+  TENET_HIP_CHECK(hipFuncSetAttribute((const void*)qr_panel_v2_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      2 * QB * QP_MAX_ROWS * (int)sizeof(double)));
   done = 1;
   return TENET_OK;
 }
@@ -959,7 +1138,7 @@ int launch_qr_blocked(double* W, int64_t m, int64_t n, void* Qout, int64_t ldq, 
   double* Tb = tau + (size_t)k * 2;
   for (int64_t p = 0; p < np; ++p) {
     const int j0 = (int)(p * QB), jb = (int)std::min<int64_t>(QB, k - j0);
-    hipLaunchKernelGGL(qr_panel_reg_kernel, dim3(1), dim3(QP_THREADS), (size_t)2 * QB * QP_MAX_ROWS * sizeof(double),
+    hipLaunchKernelGGL(qr_panel_v2_kernel, dim3(1), dim3(QP_THREADS), (size_t)2 * QB * QP_MAX_ROWS * sizeof(double),
                        s, W, tau, Tb + (size_t)p * QB * QB * 2, (int)m, j0, jb);
     TENET_HIP_CHECK(hipGetLastError());
     const int cb = j0 + jb;

@@ -38,6 +38,7 @@ struct SvSmem {
   double xi[RS][B2MAX + 1];
   double rc[B2MAX / 2], rsr[B2MAX / 2], rsi[B2MAX / 2];
   int rp[B2MAX / 2], rq[B2MAX / 2], ract[B2MAX / 2];
+  double rmax[B2MAX / 2];  // per rotation slot: max |g_pq| / sqrt(g_pp g_qq) seen
   int any_rot, any_outer;
 };
 
@@ -176,6 +177,11 @@ __device__ void jacobi_inner(SvSmem& sm, int n2, double tol, double abs_tol, int
         const double a = sm.gr[p][p], d = sm.gr[q][q];
         const double gr = sm.gr[p][q], gi = sm.gi[p][q];
         const double g2 = gr * gr + gi * gi;
+        if (sweep == 0) {
+          const double ad = fmax(a, 0.0) * fmax(d, 0.0);
+          const double rel = ad > 0.0 ? sqrt(g2 / ad) : 0.0;
+          sm.rmax[tid] = fmax(step == 0 ? 0.0 : sm.rmax[tid], rel);
+        }
         int act = 0;
         double c = 1.0, sr = 0.0, si = 0.0;
         // |g| > tol sqrt(a d) and |g| > abs_tol, compared squared
@@ -379,6 +385,13 @@ __global__ __launch_bounds__(SV_THREADS) void svd_round_kernel(SvdBatch Bt, int 
   // absolute floor (e.g. eps ||A||_F^2) stops the late small-angle rotations
   // that give one-sided Jacobi its accuracy on small singular values.
   jacobi_inner(sm, n2, P.tol, fmax(4.930380657631324e-32 * (*P.fro2), 1e-300), 1, mode == 1);
+  // largest relative off-diagonal of this pair visit -> the sweep's maximum
+  // (non-negative doubles order like their bit patterns)
+  if (threadIdx.x == 0 && c == 0) {
+    double mx = 0.0;
+    for (int k = 0; k < n2 / 2; ++k) mx = fmax(mx, sm.rmax[k]);
+    atomicMax(reinterpret_cast<unsigned long long*>(P.flag + 4), (unsigned long long)__double_as_longlong(mx));
+  }
   if (!sm.any_outer) return;  // pair already orthogonal: W = I (uniform branch)
   if (threadIdx.x == 0 && c == 0) P.flag[0] = 1;
   const int rx0 = (phase == 2) ? 0 : r0, rx1 = (phase == 2) ? P.nr : r1;
@@ -394,9 +407,16 @@ __global__ void svd_sweep_end_kernel(SvdBatch Bt, int64_t nprob) {
   if (i >= nprob) return;
   const SvdProblem& P = Bt.arr ? Bt.arr[i] : Bt.one;
   if (P.flag[1]) return;
-  if (P.flag[0] == 0) P.flag[1] = 1;
+  // Converged when a sweep rotated nothing, or when its largest relative
+  // off-diagonal was below sqrt(tol) / 2: one-sided Jacobi converges
+  // quadratically, so after such a sweep every pair's cosine is ~ that
+  // squared, below tol, and the verification sweep would rotate nothing.
+  unsigned long long* om = reinterpret_cast<unsigned long long*>(P.flag + 4);
+  const double mx = __longlong_as_double((long long)*om);
+  if (P.flag[0] == 0 || mx < 0.5 * sqrt(P.tol)) P.flag[1] = 1;
   P.flag[0] = 0;
   P.flag[2] += 1;
+  *om = 0ull;
 }
 
 // X (nr x ncp) from A or A^H with zero padding; V = I.
@@ -434,21 +454,35 @@ __global__ void svd_init_kernel(SvdBatch Bt) {
     P.flag[0] = 0;
     P.flag[1] = 0;
     P.flag[2] = 0;
+    *reinterpret_cast<unsigned long long*>(P.flag + 4) = 0ull;
   }
 }
 
 // ||X||_F^2, one workgroup per problem, fixed summation order
+// Column norms sqrt(sum_r |X[r][j]|^2) into sigma[j], one wave per column
+// (grid.x covers ncp / 4 columns per problem).  Deterministic.
+__global__ __launch_bounds__(SV_THREADS) void svd_colnorm_kernel(SvdBatch Bt) {
+  const SvdProblem& P = Bt.arr ? Bt.arr[blockIdx.y] : Bt.one;
+  const int lane = threadIdx.x & 63;
+  const int j = blockIdx.x * (SV_THREADS / 64) + (threadIdx.x >> 6);
+  if (j >= P.ncp) return;
+  const double2* col = reinterpret_cast<const double2*>(P.X) + (size_t)j * P.nr;
+  double s = 0.0;
+  for (int r = lane; r < P.nr; r += 64) {
+    const double2 v = col[r];
+    s += v.x * v.x + v.y * v.y;
+  }
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if (lane == 0) P.sigma[j] = sqrt(s);
+}
+
+// ||X||_F^2 from the column norms, one workgroup per problem, fixed order
 __global__ __launch_bounds__(SV_THREADS) void svd_fro_kernel(SvdBatch Bt) {
   __shared__ double red[SV_THREADS / 64];
   const SvdProblem& P = Bt.arr ? Bt.arr[blockIdx.x] : Bt.one;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const size_t total = (size_t)P.nr * P.ncp;
-  const double2* X = reinterpret_cast<const double2*>(P.X);
   double s = 0.0;
-  for (size_t e = tid; e < total; e += SV_THREADS) {
-    const double2 v = X[e];
-    s += v.x * v.x + v.y * v.y;
-  }
+  for (int j = tid; j < P.ncp; j += SV_THREADS) s += P.sigma[j] * P.sigma[j];
   for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
   if (lane == 0) red[wave] = s;
   __syncthreads();
@@ -466,20 +500,10 @@ __global__ __launch_bounds__(SV_THREADS) void svd_sort_kernel(SvdBatch Bt) {
   const int ncp = P.ncp;
   int np2 = 1;
   while (np2 < ncp) np2 <<= 1;
-  for (int j = wave; j < np2; j += SV_THREADS / 64) {
-    double s = 0.0;
-    if (j < ncp) {
-      const double2* col = reinterpret_cast<const double2*>(P.X) + (size_t)j * P.nr;
-      for (int r = lane; r < P.nr; r += 64) {
-        const double2 v = col[r];
-        s += v.x * v.x + v.y * v.y;
-      }
-      for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-    }
-    if (lane == 0) {
-      key[j] = (j < ncp) ? sqrt(s) : -1.0;
-      idx[j] = j;
-    }
+  // column norms were written to sigma by svd_colnorm_kernel
+  for (int j = tid; j < np2; j += SV_THREADS) {
+    key[j] = (j < ncp) ? P.sigma[j] : -1.0;
+    idx[j] = j;
   }
   __syncthreads();
   // bitonic sort, descending by key, ties by index
@@ -582,13 +606,22 @@ __global__ void svd_write_kernel(SvdBatch Bt) {
   }
 }
 
-int launch_svd_init(const SvdBatch& b, int64_t nprob, int64_t max_elems, int dtype, hipStream_t s) {
+#define TRY_HIP_COLNORM(b, nprob, max_ncp, s)                                                             \
+  do {                                                                                                    \
+    hipLaunchKernelGGL(svd_colnorm_kernel, dim3((unsigned)(((max_ncp) + SV_THREADS / 64 - 1) / (SV_THREADS / 64)), \
+                                                (unsigned)(nprob)),                                       \
+                       dim3(SV_THREADS), 0, s, b);                                                        \
+    TENET_HIP_CHECK(hipGetLastError());                                                                   \
+  } while (0)
+
+int launch_svd_init(const SvdBatch& b, int64_t nprob, int64_t max_elems, int max_ncp, int dtype, hipStream_t s) {
   const unsigned gx = (unsigned)std::max<int64_t>(1, std::min<int64_t>((max_elems + 255) / 256, 1024));
   if (dtype == TENET_C128)
     hipLaunchKernelGGL(svd_init_kernel<double2>, dim3(gx, (unsigned)nprob), dim3(256), 0, s, b);
   else
     hipLaunchKernelGGL(svd_init_kernel<float2>, dim3(gx, (unsigned)nprob), dim3(256), 0, s, b);
   TENET_HIP_CHECK(hipGetLastError());
+  TRY_HIP_COLNORM(b, nprob, max_ncp, s);
   hipLaunchKernelGGL(svd_fro_kernel, dim3((unsigned)nprob), dim3(SV_THREADS), 0, s, b);
   TENET_HIP_CHECK(hipGetLastError());
   return TENET_OK;
@@ -614,6 +647,7 @@ int launch_svd_finalize(const SvdBatch& b, int64_t nprob, int max_ncp, int64_t m
     set_error("SVD: min(m, n) exceeds the sort limit (4096)");
     return TENET_ERR_UNSUPPORTED;
   }
+  TRY_HIP_COLNORM(b, nprob, max_ncp, s);
   hipLaunchKernelGGL(svd_sort_kernel, dim3((unsigned)nprob), dim3(SV_THREADS), 0, s, b);
   TENET_HIP_CHECK(hipGetLastError());
   const unsigned gx = (unsigned)std::max<int64_t>(1, std::min<int64_t>((max_elems + 255) / 256, 2048));

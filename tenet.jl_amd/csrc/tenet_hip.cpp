@@ -1004,7 +1004,7 @@ static int svd_run(tenet_ctx ctx, hipStream_t s, int dtype, int64_t m, int64_t n
   SvdBatch B{};
   B.arr = nullptr;
   B.one = P;
-  TRY(launch_svd_init(B, 1, sh.nr * sh.ncp + sh.ncp * sh.ncp, dtype, s));
+  TRY(launch_svd_init(B, 1, sh.nr * sh.ncp + sh.ncp * sh.ncp, (int)sh.ncp, dtype, s));
   const bool sync_ok = !capturing(s);
   const int max_sweeps = sync_ok ? 40 : 14;
   bool converged = false;
@@ -1033,7 +1033,7 @@ static int svd_run(tenet_ctx ctx, hipStream_t s, int dtype, int64_t m, int64_t n
     if (sh.nblk > 2)
       for (int r = 0; r < sh.nblk - 1; ++r) TRY(round(r, 1));
     TRY(launch_svd_sweep_end(B, 1, s));
-    if (sync_ok && sw >= 1) {
+    if (sync_ok) {
       TENET_HIP_CHECK(hipMemcpyAsync(ctx->host_flag, P.flag, 16, hipMemcpyDeviceToHost, s));
       TENET_HIP_CHECK(hipStreamSynchronize(s));
       converged = ctx->host_flag[1] != 0;
@@ -1646,7 +1646,7 @@ extern "C" int tenet_grouped_svd(tenet_ctx ctx, tenet_stream stream, int dtype, 
   CHECK_ARG(ms && ns && As && Us && Vhs, "null argument");
   hipStream_t s = pick(ctx, stream);
   std::vector<SvdShape> shapes((size_t)count);
-  size_t total = align_up((size_t)count * sizeof(SvdProblem), 256) + align_up((size_t)count * 16, 256) +
+  size_t total = align_up((size_t)count * sizeof(SvdProblem), 256) + align_up((size_t)count * 32, 256) +
                  align_up((size_t)count * 8, 256);
   int64_t max_rounds = 0, max_pairs = 0, max_ncp = 0, max_elems = 0, max_out = 0;
   for (int64_t i = 0; i < count; ++i) {
@@ -1696,27 +1696,29 @@ extern "C" int tenet_grouped_svd(tenet_ctx ctx, tenet_stream stream, int dtype, 
     P.disc_dev = disc_dev ? disc_dev + i : nullptr;
     P.tol = svd_tol(sh.nr);
   }
-  flags_base = (int32_t*)bm.take((size_t)count * 16);
+  // per problem 8 int32: rotated, converged, sweeps, spare, then the sweep's
+  // largest relative off-diagonal (8 bytes at flag + 4)
+  flags_base = (int32_t*)bm.take((size_t)count * 32);
   double* fro_base = (double*)bm.take((size_t)count * 8);
   for (int64_t i = 0; i < count; ++i) {
-    probs[(size_t)i].flag = flags_base + 4 * i;
+    probs[(size_t)i].flag = flags_base + 8 * i;
     probs[(size_t)i].fro2 = fro_base + i;
   }
   TENET_HIP_CHECK(hipMemcpyAsync(dprobs, probs.data(), probs.size() * sizeof(SvdProblem), hipMemcpyHostToDevice, s));
   SvdBatch B{};
   B.arr = dprobs;
-  TRY(launch_svd_init(B, count, max_elems, dtype, s));
-  std::vector<int32_t> hflags((size_t)count * 4);
+  TRY(launch_svd_init(B, count, max_elems, (int)max_ncp, dtype, s));
+  std::vector<int32_t> hflags((size_t)count * 8);
   bool all = false;
   for (int sw = 0; sw < 40 && !all; ++sw) {
     TRY(launch_svd_round(B, count, (int)max_pairs, 0, 2, 0, s));
     for (int r = 0; r < max_rounds; ++r) TRY(launch_svd_round(B, count, (int)max_pairs, r, 2, 1, s));
     TRY(launch_svd_sweep_end(B, count, s));
-    if (sw >= 1) {
-      TENET_HIP_CHECK(hipMemcpyAsync(hflags.data(), flags_base, (size_t)count * 16, hipMemcpyDeviceToHost, s));
+    {
+      TENET_HIP_CHECK(hipMemcpyAsync(hflags.data(), flags_base, (size_t)count * 32, hipMemcpyDeviceToHost, s));
       TENET_HIP_CHECK(hipStreamSynchronize(s));
       all = true;
-      for (int64_t i = 0; i < count; ++i) all = all && hflags[(size_t)(4 * i + 1)] != 0;
+      for (int64_t i = 0; i < count; ++i) all = all && hflags[(size_t)(8 * i + 1)] != 0;
     }
   }
   TRY(launch_svd_finalize(B, count, (int)max_ncp, max_out, dtype, s));

@@ -110,7 +110,9 @@ class DeviceArray:
     def from_host(ctx: Context, arr, dtype=None):
         a = np.asarray(arr)
         dt = np.dtype(dtype) if dtype is not None else (a.dtype if a.dtype in DTYPES else np.dtype(np.complex128))
-        a = np.asfortranarray(a.astype(dt, copy=False))
+        a = a.astype(dt, copy=False)
+        # np.asfortranarray promotes a 0-d array to shape (1,): keep scalars 0-d
+        a = np.asfortranarray(a) if a.ndim else a.reshape(())
         d = DeviceArray(ctx, a.shape, dt)
         if d.nbytes:
             L.check(L.lib.tenet_memcpy_h2d(ctx.h, C.c_void_p(d.ptr), a.ctypes.data_as(C.c_void_p),

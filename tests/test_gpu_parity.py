@@ -475,3 +475,21 @@ def test_scale_and_nrm2sq(ctx):
     d.scale_(0.5 - 2.0j)
     assert _rel(d.to_host(), (0.5 - 2.0j) * a) < 1e-15
     assert T.DeviceArray(ctx, (0,)).nrm2sq() == 0.0
+
+
+def test_normalize_and_norm_c64_and_c128(ctx):
+    """normalize!(::MPS) (normalize.jl:15-37) scales the canonical centre by a
+    0-d device scalar: norm 1 afterwards, state direction unchanged."""
+    import tenet_hip as T
+
+    rng = np.random.default_rng(31)
+    for dtype, tol in ((np.complex128, 1e-12), (np.complex64, 2e-6)):
+        arrs = [O.complex_gaussian(rng, s) for s in [(1, 2, 4), (4, 2, 5), (5, 2, 3), (3, 2, 1)]]
+        psi = T.MPS.from_host(ctx, arrs, dtype)
+        ref = O.OMPS([a.astype(np.complex128) for a in arrs])
+        nref = O.norm(ref.copy())
+        assert abs(psi.norm() - nref) < tol * nref
+        psi.normalize_()
+        assert abs(psi.norm() - 1.0) < tol
+        v = O.OMPS([a.astype(np.complex128) for a in psi.to_host()]).to_dense()
+        assert _rel(v, ref.to_dense() / nref) < tol
