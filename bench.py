@@ -31,6 +31,9 @@ N_SITES, DEPTH, CHI, THR = 64, 20, 256, 1e-10
 SEED = 20261202 + 1
 FIXTURE = os.path.join(ROOT, "tests", "golden", "tebd_n64_d20_chi256.npz")
 PEAK_FP64_TFLOPS = 78.6
+# HBM bytes per svd_round_kernel dispatch over one TEBD sweep (n=64, chi_max=256), from two counter
+# passes: FETCH_SIZE 6301.1 KiB x 2.0 + WRITE_SIZE 3985.6 KiB x 1.0 = 16587.8 KiB.
+SVD_ROUND_TRAFFIC_BYTES = 16_985_907
 
 FORMS = {"fp64": 0, "fp32": 1, "split_bf16": 2, "split_fp16": 3}
 
@@ -519,6 +522,7 @@ def bench_svd(args):
     from tenet_hip.tensor import tensor_svd_thin
 
     ctx = T.Context(0)
+    ctx.svd_precondition(bool(args.svd_precond))
     A = T.Tensor(T.DeviceArray(ctx, (512, 512)).gaussian_(77), ["r", "c"])
     for _ in range(args.warmup):
         tensor_svd_thin(A, ["r"], "s", ["c"], maxdim=256, threshold=1e-10)
@@ -528,7 +532,32 @@ def bench_svd(args):
         tensor_svd_thin(A, ["r"], "s", ["c"], maxdim=256, threshold=1e-10)
     ctx.sync()
     dt = (time.perf_counter() - t0) / args.steps
-    print(json.dumps({"metric": "svd 512x512 c128 maxdim 256", "ms": round(dt * 1e3, 3)}), flush=True)
+    print(json.dumps({"metric": "svd 512x512 c128 maxdim 256", "ms": round(dt * 1e3, 3),
+                      "qr_precondition": bool(args.svd_precond)}), flush=True)
+    ctx.close()
+
+
+# ---------------------------------------------------------------- single QR (profiling unit)
+def bench_qr(args):
+    """tenet_qr_thin of seeded 512 x 256 and 512 x 512 complex128 Gaussian
+    matrices (a canonize move at chi = 256, and a square bond matrix),
+    repeated --steps times each."""
+    import tenet_hip as T
+    from tenet_hip.tensor import tensor_qr_thin
+
+    ctx = T.Context(0)
+    res = {}
+    for shp in ((512, 256), (512, 512)):
+        A = T.Tensor(T.DeviceArray(ctx, shp).gaussian_(78), ["r", "c"])
+        for _ in range(args.warmup):
+            tensor_qr_thin(A, ["r"], ["c"], "v")
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            tensor_qr_thin(A, ["r"], ["c"], "v")
+        ctx.sync()
+        res[f"{shp[0]}x{shp[1]}"] = round((time.perf_counter() - t0) / args.steps * 1e3, 3)
+    print(json.dumps({"metric": "thin QR c128 ms", **res}), flush=True)
     ctx.close()
 
 
@@ -672,10 +701,11 @@ def main():
     ap.add_argument("--forms", default="fp64,split_bf16")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--svd-precond", type=int, default=1, help="QR-preconditioned Jacobi SVD (1) or plain (0)")
     ap.add_argument("--gates", type=int, default=0,
                     help="profile only: apply the first N gates of the sweep (amplitude check skipped)")
     ap.add_argument("--copy-mib", type=int, default=1024, help="traffic calibration buffer (workload copy)")
-    ap.add_argument("--workload", default="tebd", choices=["tebd", "heff", "plain", "grouped", "serving", "copy", "svd"])
+    ap.add_argument("--workload", default="tebd", choices=["tebd", "heff", "plain", "grouped", "serving", "copy", "svd", "qr"])
     ap.add_argument("--rates", default="250,500,1000,2000,4000,8000,16000")
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     args = ap.parse_args()
@@ -697,6 +727,8 @@ def main():
         return bench_copy(args)
     if args.workload == "svd":
         return bench_svd(args)
+    if args.workload == "qr":
+        return bench_qr(args)
 
     import numpy as np
     import tenet_hip as T
@@ -719,6 +751,7 @@ def main():
         ctx.close()
         return
     ctx = T.Context(0)
+    ctx.svd_precondition(bool(args.svd_precond))
     gate_views = upload_gates(ctx, gates)
     results = {}
     for form in [f for f in args.forms.split(",") if f]:
@@ -738,7 +771,9 @@ def main():
     if k["launches"] > 0 and k["ms"] > 0:
         ach = k["flops"] / (k["ms"] * 1e-3) / 1e12
         roof = {"bound": "mfma", "achieved": round(ach, 4), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(ach / PEAK_FP64_TFLOPS, 5), "traffic": None,
+                "frac": round(ach / PEAK_FP64_TFLOPS, 5), "traffic": SVD_ROUND_TRAFFIC_BYTES,
+                "traffic_source": "rocprofv3 --pmc FETCH_SIZE x2.0 + WRITE_SIZE x1.0 per dispatch (factors from "
+                                  "scale_kernel over 256 MiB), averaged over one sweep of this workload",
                 "kernel": "svd_round_kernel (one-sided Jacobi round: pair Grams, rotations, apply)",
                 "launches": k["launches"], "avg_launch_us": round(k["ms"] * 1e3 / k["launches"], 2),
                 "kernel_share_of_step": round(k["ms"] / (r["ms_per_step"] * args.steps), 3),
@@ -765,7 +800,7 @@ def main():
         "dtype": "c128" if main_form == "fp64" else f"c128 ({main_form} contractions, fp64 qr/svd)",
         "data": "synthetic (seeded Haar gates, |0...0>)",
         "config": {"workload": "TEBD brickwork n=64 d=2 depth=20 chi_max=256 threshold=1e-10 (630 gates)",
-                   "form": main_form, "relerr_vs_fp64_oracle": r["relerr"],
+                   "form": main_form, "relerr_vs_fp64_oracle": r["relerr"], "svd_qr_precondition": bool(args.svd_precond),
                    "forms": {f: {"ms_per_sweep": round(v["ms_per_step"], 3), "relerr": v["relerr"],
                                  "gflop_nominal": round(v["flops"] / 1e9, 2), "bonds_match_oracle": v["bonds_match"]}
                              for f, v in results.items()}},

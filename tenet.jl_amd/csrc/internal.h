@@ -153,5 +153,6 @@ int launch_set_identity(void* out, int64_t chi, int dtype, hipStream_t s);
 int launch_gaussian(void* out, int dtype, int64_t count, uint64_t seed, uint64_t offset, hipStream_t s);
 int launch_scale(void* x, int dtype, int64_t count, double ar, double ai, hipStream_t s);
 int launch_nrm2(const void* x, int dtype, int64_t count, double* part, double* out, hipStream_t s);
+int launch_sigma_scales(const double* S, int64_t k, void* sq, void* isq, int dtype, hipStream_t s);
 
 }  // namespace tenet

@@ -202,11 +202,27 @@ int launch_convert(const void* src, int src_dtype, void* dst, int dst_dtype, int
 // in LDS as FP64; per site v'[r] = sum_l v[l] A[l, x, r] with A stored
 // (chi_l, d, chi_r) column-major, so the slice A[:, x, :] is a chi_l x chi_r
 // matrix of leading dimension chi_l*d whose columns are contiguous.  Each wave
-// takes columns r; its lanes stride over l (coalesced) and reduce with
-// cross-lane shuffles.  v is rescaled by max|v| after each site with the log
-// of the scale accumulated in FP64 (unnormalised states overflow otherwise).
+// takes 8 columns at a time: its lanes stride over l (coalesced loads, eight
+// independent streams in flight) and the 16 partial sums are combined by one
+// butterfly transpose-reduction (17 shuffles, not 16 full reductions).  The
+// next vector's maximum comes out of the same pass; v is rescaled lazily by
+// its inverse when it is read at the next site, and the log of every scale
+// is accumulated in FP64 (unnormalised states overflow otherwise).  One
+// barrier per site.
 constexpr int AMP_THREADS = 256;
 constexpr int AMP_MAX_CHI = 2048;
+constexpr int AMP_COLS = 8;
+
+template <int H, int BIT>
+__device__ __forceinline__ void amp_halve(double (&v)[2 * AMP_COLS], int lane) {
+  const bool hi = (lane & BIT) != 0;
+#pragma unroll
+  for (int i = 0; i < H; ++i) {
+    const double keep = hi ? v[H + i] : v[i];
+    const double send = hi ? v[i] : v[H + i];
+    v[i] = keep + __shfl_xor(send, BIT, 64);
+  }
+}
 
 template <typename S2>
 __global__ __launch_bounds__(AMP_THREADS) void amplitude_kernel(const int64_t* __restrict__ n_dev,
@@ -215,8 +231,9 @@ __global__ __launch_bounds__(AMP_THREADS) void amplitude_kernel(const int64_t* _
                                                                  const uint64_t* __restrict__ bits, int64_t d,
                                                                  double* __restrict__ out) {
   __shared__ double vr[2][AMP_MAX_CHI], vi[2][AMP_MAX_CHI];
-  __shared__ double red[AMP_THREADS / 64];
+  __shared__ double wmax[2][AMP_THREADS / 64];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  constexpr int NWAVE = AMP_THREADS / 64;
   const int64_t req = blockIdx.x;
   const int64_t n = n_dev[req];
   const uint64_t* const sites = reinterpret_cast<const uint64_t*>(site_ptrs[req]);
@@ -226,7 +243,7 @@ __global__ __launch_bounds__(AMP_THREADS) void amplitude_kernel(const int64_t* _
     vr[0][0] = 1.0;
     vi[0][0] = 0.0;
   }
-  double logscale = 0.0;
+  double logscale = 0.0, scale = 1.0;
   int cur = 0;
   __syncthreads();
   for (int64_t i = 0; i < n; ++i) {
@@ -242,45 +259,57 @@ __global__ __launch_bounds__(AMP_THREADS) void amplitude_kernel(const int64_t* _
     }
     const S2* A = reinterpret_cast<const S2*>(sites[i]) + (int64_t)x[i] * cl;
     const int64_t ld = cl * d;
-    for (int64_t r = wave; r < cr; r += AMP_THREADS / 64) {
-      const S2* col = A + r * ld;
-      double sr = 0.0, si = 0.0;
+    double lmax = 0.0;
+    for (int64_t cb = (int64_t)wave * AMP_COLS; cb < cr; cb += (int64_t)NWAVE * AMP_COLS) {
+      double acc[2 * AMP_COLS];
+#pragma unroll
+      for (int j = 0; j < 2 * AMP_COLS; ++j) acc[j] = 0.0;
       for (int64_t l = lane; l < cl; l += 64) {
-        const S2 a = col[l];
-        const double ar = (double)a.x, ai = (double)a.y;
-        const double pr = vr[cur][l], pi = vi[cur][l];
-        sr += pr * ar - pi * ai;
-        si += pr * ai + pi * ar;
+        const double pr = vr[cur][l] * scale, pi = vi[cur][l] * scale;
+#pragma unroll
+        for (int j = 0; j < AMP_COLS; ++j) {
+          if (cb + j < cr) {
+            const S2 a = A[l + (cb + j) * ld];
+            const double ar = (double)a.x, ai = (double)a.y;
+            acc[2 * j] += pr * ar - pi * ai;
+            acc[2 * j + 1] += pr * ai + pi * ar;
+          }
+        }
       }
-      sr = wave_add(sr);
-      si = wave_add(si);
-      if (lane == 0) {
-        vr[cur ^ 1][r] = sr;
-        vi[cur ^ 1][r] = si;
+      // lane l ends with the wave total of value l >> 2 (four lanes each)
+      amp_halve<8, 32>(acc, lane);
+      amp_halve<4, 16>(acc, lane);
+      amp_halve<2, 8>(acc, lane);
+      amp_halve<1, 4>(acc, lane);
+      acc[0] += __shfl_xor(acc[0], 2, 64);
+      acc[0] += __shfl_xor(acc[0], 1, 64);
+      const int k = lane >> 2;
+      const int64_t col = cb + (k >> 1);
+      if ((lane & 3) == 0 && col < cr) {
+        if (k & 1)
+          vi[cur ^ 1][col] = acc[0];
+        else
+          vr[cur ^ 1][col] = acc[0];
+        lmax = fmax(lmax, fabs(acc[0]));
       }
     }
-    __syncthreads();
-    cur ^= 1;
-    double mx = 0.0;
-    for (int64_t r = tid; r < cr; r += AMP_THREADS) mx = fmax(mx, fmax(fabs(vr[cur][r]), fabs(vi[cur][r])));
-    mx = wave_max(mx);
-    if (lane == 0) red[wave] = mx;
+    lmax = wave_max(lmax);
+    if (lane == 0) wmax[cur][wave] = lmax;
     __syncthreads();
     double m = 0.0;
-    for (int w = 0; w < AMP_THREADS / 64; ++w) m = fmax(m, red[w]);
+#pragma unroll
+    for (int w = 0; w < NWAVE; ++w) m = fmax(m, wmax[cur][w]);
+    cur ^= 1;
     if (m > 0.0) {
-      const double inv = 1.0 / m;
-      for (int64_t r = tid; r < cr; r += AMP_THREADS) {
-        vr[cur][r] *= inv;
-        vi[cur][r] *= inv;
-      }
+      scale = 1.0 / m;
       logscale += log(m);
+    } else {
+      scale = 1.0;
     }
-    __syncthreads();
   }
   if (tid == 0) {
-    out[3 * req + 0] = vr[cur][0];
-    out[3 * req + 1] = vi[cur][0];
+    out[3 * req + 0] = vr[cur][0] * scale;
+    out[3 * req + 1] = vi[cur][0] * scale;
     out[3 * req + 2] = logscale;
   }
 }
@@ -494,6 +523,31 @@ int launch_nrm2(const void* x, int dtype, int64_t count, double* part, double* o
     hipLaunchKernelGGL(nrm2_partial_kernel<float2>, dim3(NB), dim3(256), 0, s, (const float2*)x, count, part);
   TENET_HIP_CHECK(hipGetLastError());
   hipLaunchKernelGGL(nrm2_final_kernel, dim3(1), dim3(256), 0, s, part, NB, out);
+  TENET_HIP_CHECK(hipGetLastError());
+  return TENET_OK;
+}
+
+// sq[j] = sqrt(S[j]), isq[j] = 1 / sqrt(S[j]) (0 where S[j] = 0), as complex
+template <typename S2>
+__global__ void sigma_scales_kernel(const double* __restrict__ S, int64_t k, S2* __restrict__ sq, S2* __restrict__ isq) {
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < k; j += (int64_t)gridDim.x * blockDim.x) {
+    const double r = sqrt(fmax(S[j], 0.0));
+    S2 a, b;
+    a.x = r;
+    a.y = 0.0;
+    b.x = r > 0.0 ? 1.0 / r : 0.0;
+    b.y = 0.0;
+    sq[j] = a;
+    isq[j] = b;
+  }
+}
+int launch_sigma_scales(const double* S, int64_t k, void* sq, void* isq, int dtype, hipStream_t s) {
+  if (k <= 0) return TENET_OK;
+  const unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>((k + 255) / 256, 64));
+  if (dtype == TENET_C128)
+    hipLaunchKernelGGL(sigma_scales_kernel<double2>, dim3(g), dim3(256), 0, s, S, k, (double2*)sq, (double2*)isq);
+  else
+    hipLaunchKernelGGL(sigma_scales_kernel<float2>, dim3(g), dim3(256), 0, s, S, k, (float2*)sq, (float2*)isq);
   TENET_HIP_CHECK(hipGetLastError());
   return TENET_OK;
 }

@@ -1108,6 +1108,219 @@ __global__ void qr_store_kernel(const double* __restrict__ W, const double* __re
 
 bool qr_blocked_fits(int64_t m) { return m <= QP_MAX_ROWS; }
 
+// Row-chunked application of one block reflector, M <- (I - V T' V^H) M on
+// rows [j0, m) and columns [col_begin, col_end): two launches, each with a
+// workgroup per (16-column block, 64-row chunk), so a 512-row update runs on
+// 8x the workgroups of the one-pass kernel above.
+//   phase Y: Ypart[chunk][block] = V[chunk]^H M[chunk, block]   (16 x 16)
+//   phase X: Z = T' sum_chunks Ypart (each workgroup, fixed order);
+//            M[chunk, block] -= V[chunk] Z
+constexpr int QA_RC = 64;
+
+__device__ __forceinline__ void qa_load_v(const double2* __restrict__ Wc, int m, int j0, int jb, int rb, int rows,
+                                          double (&vr)[QA_RC][QB + 1], double (&vi)[QA_RC][QB + 1]) {
+  for (int e = threadIdx.x; e < QA_RC * QB; e += QR_THREADS) {
+    const int r = e % QA_RC, cc = e / QA_RC;
+    const int gr = rb + r, gc = j0 + cc;
+    double v_r = 0.0, v_i = 0.0;
+    if (r < rows && cc < jb) {
+      if (gr == gc) {
+        v_r = 1.0;
+      } else if (gr > gc) {
+        const double2 w = Wc[(size_t)gc * m + gr];
+        v_r = w.x;
+        v_i = w.y;
+      }
+    }
+    vr[r][cc] = v_r;
+    vi[r][cc] = v_i;
+  }
+}
+
+__global__ __launch_bounds__(QR_THREADS) void qr_apply_y_kernel(const double* __restrict__ W,
+                                                                 const double* __restrict__ M, double* __restrict__ Yp,
+                                                                 int m, int ldm, int j0, int jb, int col_begin,
+                                                                 int col_end) {
+  __shared__ double vr[QA_RC][QB + 1], vi[QA_RC][QB + 1];
+  __shared__ double xr[QA_RC][QB + 1], xi[QA_RC][QB + 1];
+  __shared__ double yred[4][QB * QB][2];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int blk = blockIdx.x, chunk = blockIdx.y;
+  const int cb0 = col_begin + blk * QB;
+  const int nc = min(QB, col_end - cb0);
+  const int rb = j0 + chunk * QA_RC;
+  const int rows = min(QA_RC, m - rb);
+  if (nc <= 0 || rows <= 0) return;
+  const double2* Wc = reinterpret_cast<const double2*>(W);
+  const double2* Mc = reinterpret_cast<const double2*>(M);
+  qa_load_v(Wc, m, j0, jb, rb, rows, vr, vi);
+  for (int e = tid; e < QA_RC * QB; e += QR_THREADS) {
+    const int r = e % QA_RC, cc = e / QA_RC;
+    double x_r = 0.0, x_i = 0.0;
+    if (r < rows && cc < nc) {
+      const double2 x = Mc[(size_t)(cb0 + cc) * ldm + rb + r];
+      x_r = x.x;
+      x_i = x.y;
+    }
+    xr[r][cc] = x_r;
+    xi[r][cc] = x_i;
+  }
+  __syncthreads();
+  // thread: row group g (rows g, g+16, g+32, g+48) and a 4 x 4 block of Y
+  const int g = tid / 16, a0 = 4 * ((tid % 16) / 4), c0 = 4 * (tid % 4);
+  double yr_[4][4], yi_[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      yr_[i][j] = 0.0;
+      yi_[i][j] = 0.0;
+    }
+#pragma unroll
+  for (int rr = 0; rr < QA_RC / 16; ++rr) {
+    const int r = g + 16 * rr;
+    double pr[4], pi[4], qr[4], qi[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      pr[i] = vr[r][a0 + i];
+      pi[i] = vi[r][a0 + i];
+      qr[i] = xr[r][c0 + i];
+      qi[i] = xi[r][c0 + i];
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {  // conj(V[r][a]) X[r][c]
+        yr_[i][j] += pr[i] * qr[j] + pi[i] * qi[j];
+        yi_[i][j] += pr[i] * qi[j] - pi[i] * qr[j];
+      }
+  }
+  // row groups of one wave: lanes l, l^16, l^32, l^48
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      double a = yr_[i][j], b = yi_[i][j];
+      a += __shfl_xor(a, 16, 64);
+      b += __shfl_xor(b, 16, 64);
+      a += __shfl_xor(a, 32, 64);
+      b += __shfl_xor(b, 32, 64);
+      yr_[i][j] = a;
+      yi_[i][j] = b;
+    }
+  if (lane < 16) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        yred[wave][(a0 + i) * QB + c0 + j][0] = yr_[i][j];
+        yred[wave][(a0 + i) * QB + c0 + j][1] = yi_[i][j];
+      }
+  }
+  __syncthreads();
+  // Ypart[chunk][blk][a][c], fixed summation order over the four waves
+  const int nblk = gridDim.x;
+  double* out = Yp + ((size_t)chunk * nblk + blk) * QB * QB * 2;
+  {
+    const int e = tid;  // QB * QB = 256 = QR_THREADS
+    out[2 * e] = yred[0][e][0] + yred[1][e][0] + yred[2][e][0] + yred[3][e][0];
+    out[2 * e + 1] = yred[0][e][1] + yred[1][e][1] + yred[2][e][1] + yred[3][e][1];
+  }
+}
+
+__global__ __launch_bounds__(QR_THREADS) void qr_apply_x_kernel(const double* __restrict__ W,
+                                                                 const double* __restrict__ T,
+                                                                 const double* __restrict__ Yp, double* __restrict__ M,
+                                                                 int m, int ldm, int j0, int jb, int col_begin,
+                                                                 int col_end, int nchunks, int adj) {
+  __shared__ double vr[QA_RC][QB + 1], vi[QA_RC][QB + 1];
+  __shared__ double yr[QB][QB + 1], yi[QB][QB + 1];
+  __shared__ double zr[QB][QB + 1], zi[QB][QB + 1];
+  const int tid = threadIdx.x;
+  const int blk = blockIdx.x, chunk = blockIdx.y;
+  const int cb0 = col_begin + blk * QB;
+  const int nc = min(QB, col_end - cb0);
+  const int rb = j0 + chunk * QA_RC;
+  const int rows = min(QA_RC, m - rb);
+  if (nc <= 0 || rows <= 0) return;
+  const double2* Wc = reinterpret_cast<const double2*>(W);
+  double2* Mc = reinterpret_cast<double2*>(M);
+  qa_load_v(Wc, m, j0, jb, rb, rows, vr, vi);
+  {
+    // Y = sum over chunks (fixed order), one entry per thread
+    const int nblk = gridDim.x;
+    const int a = tid / QB, c = tid % QB;
+    double sr = 0.0, si = 0.0;
+    for (int ch = 0; ch < nchunks; ++ch) {
+      const double* in = Yp + ((size_t)ch * nblk + blk) * QB * QB * 2;
+      sr += in[2 * (a * QB + c)];
+      si += in[2 * (a * QB + c) + 1];
+    }
+    yr[a][c] = sr;
+    yi[a][c] = si;
+  }
+  __syncthreads();
+  {
+    // Z = T' Y
+    const int a = tid / QB, c = tid % QB;
+    double sr = 0.0, si = 0.0;
+    for (int b = 0; b < jb; ++b) {
+      double tr, ti;
+      if (adj) {
+        tr = T[2 * (a * QB + b)];
+        ti = -T[2 * (a * QB + b) + 1];
+      } else {
+        tr = T[2 * (b * QB + a)];
+        ti = T[2 * (b * QB + a) + 1];
+      }
+      sr += tr * yr[b][c] - ti * yi[b][c];
+      si += tr * yi[b][c] + ti * yr[b][c];
+    }
+    zr[a][c] = (a < jb) ? sr : 0.0;
+    zi[a][c] = (a < jb) ? si : 0.0;
+  }
+  __syncthreads();
+  // M[chunk rows] -= V Z: thread owns row tid % 64 and 4 columns
+  const int rsub = tid % QA_RC, cq = 4 * (tid / QA_RC);
+  if (rsub < rows) {
+    double sr[4] = {0, 0, 0, 0}, si[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int b = 0; b < QB; ++b) {
+      const double pr = vr[rsub][b], pi = vi[rsub][b];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        sr[j] += pr * zr[b][cq + j] - pi * zi[b][cq + j];
+        si[j] += pr * zi[b][cq + j] + pi * zr[b][cq + j];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if (cq + j < nc) {
+        double2 x = Mc[(size_t)(cb0 + cq + j) * ldm + rb + rsub];
+        x.x -= sr[j];
+        x.y -= si[j];
+        Mc[(size_t)(cb0 + cq + j) * ldm + rb + rsub] = x;
+      }
+    }
+  }
+}
+
+// M <- (I - V T' V^H) M (adj = 1: T' = T^H, forward application during the
+// factorisation; adj = 0: T' = T, backward accumulation of Q)
+static int qr_apply_chunked(const double* W, const double* T, double* M, int m, int ldm, int j0, int jb, int c_begin,
+                            int c_end, int adj, double* Yp, hipStream_t s) {
+  if (c_end <= c_begin || m <= j0) return TENET_OK;
+  const unsigned nblk = (unsigned)((c_end - c_begin + QB - 1) / QB);
+  const unsigned nch = (unsigned)((m - j0 + QA_RC - 1) / QA_RC);
+  hipLaunchKernelGGL(qr_apply_y_kernel, dim3(nblk, nch), dim3(QR_THREADS), 0, s, W, M, Yp, m, ldm, j0, jb, c_begin,
+                     c_end);
+  TENET_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(qr_apply_x_kernel, dim3(nblk, nch), dim3(QR_THREADS), 0, s, W, T, Yp, M, m, ldm, j0, jb, c_begin,
+                     c_end, (int)nch, adj);
+  TENET_HIP_CHECK(hipGetLastError());
+  return TENET_OK;
+}
+
 // The panel kernel keeps up to 512 x 16 complex doubles (128 KiB) in dynamic
 // LDS, above HIP's default dynamic limit; raise it once per process.
 int qr_setup_attributes() {
@@ -1124,7 +1337,9 @@ int qr_setup_attributes() {
 size_t qr_blocked_ws_bytes(int64_t m, int64_t n) {
   const int64_t k = std::min(m, n);
   const int64_t np = (k + QB - 1) / QB;
-  return (size_t)m * n * 16 + (size_t)m * k * 16 + (size_t)k * 16 + (size_t)np * QB * QB * 16 + 1024;
+  const int64_t ych = (m + QA_RC - 1) / QA_RC, yblk = (std::max(n, k) + QB - 1) / QB;
+  return (size_t)m * n * 16 + (size_t)m * k * 16 + (size_t)k * 16 + (size_t)np * QB * QB * 16 +
+         (size_t)ych * yblk * QB * QB * 16 + 2048;
 }
 
 // W (m x n, FP64 complex, already packed) -> Q, R outputs.  Workspace after W:
@@ -1136,6 +1351,7 @@ int launch_qr_blocked(double* W, int64_t m, int64_t n, void* Qout, int64_t ldq, 
   double* Q = W + (size_t)m * n * 2;
   double* tau = Q + (size_t)m * k * 2;
   double* Tb = tau + (size_t)k * 2;
+  double* Yp = Tb + (size_t)np * QB * QB * 2;  // chunk partials of V^H M
   for (int64_t p = 0; p < np; ++p) {
     const int j0 = (int)(p * QB), jb = (int)std::min<int64_t>(QB, k - j0);
     hipLaunchKernelGGL(qr_panel_v2_kernel, dim3(1), dim3(QP_THREADS), (size_t)2 * QB * QP_MAX_ROWS * sizeof(double),
@@ -1143,10 +1359,8 @@ int launch_qr_blocked(double* W, int64_t m, int64_t n, void* Qout, int64_t ldq, 
     TENET_HIP_CHECK(hipGetLastError());
     const int cb = j0 + jb;
     if (cb < n) {
-      const unsigned blocks = (unsigned)((n - cb + QB - 1) / QB);
-      hipLaunchKernelGGL(qr_apply_rb_kernel, dim3(blocks), dim3(QR_THREADS), 0, s, W,
-                         Tb + (size_t)p * QB * QB * 2, W, (int)m, (int)m, j0, jb, cb, (int)n, 1);
-      TENET_HIP_CHECK(hipGetLastError());
+      const int st = qr_apply_chunked(W, Tb + (size_t)p * QB * QB * 2, W, (int)m, (int)m, j0, jb, cb, (int)n, 1, Yp, s);
+      if (st != TENET_OK) return st;
     }
   }
   {
@@ -1157,10 +1371,8 @@ int launch_qr_blocked(double* W, int64_t m, int64_t n, void* Qout, int64_t ldq, 
   }
   for (int64_t p = np - 1; p >= 0; --p) {
     const int j0 = (int)(p * QB), jb = (int)std::min<int64_t>(QB, k - j0);
-    const unsigned blocks = (unsigned)((k - j0 + QB - 1) / QB);
-    hipLaunchKernelGGL(qr_apply_rb_kernel, dim3(blocks), dim3(QR_THREADS), 0, s, W, Tb + (size_t)p * QB * QB * 2,
-                       Q, (int)m, (int)m, j0, jb, j0, (int)k, 0);
-    TENET_HIP_CHECK(hipGetLastError());
+    const int st = qr_apply_chunked(W, Tb + (size_t)p * QB * QB * 2, Q, (int)m, (int)m, j0, jb, j0, (int)k, 0, Yp, s);
+    if (st != TENET_OK) return st;
   }
   const size_t total = (size_t)m * k + (size_t)k * n;
   const unsigned g = (unsigned)std::max<size_t>(1, std::min<size_t>((total + 255) / 256, 4096));

@@ -81,6 +81,7 @@ struct tenet_ctx_s {
   std::vector<hipEvent_t> stat_events;  // pairs
   size_t stat_used = 0;
   int64_t stat_launches = 0, stat_sampled = 0, stat_every = 17;
+  bool svd_precond = true;  // QR-preconditioned Jacobi for complex128 bond matrices (tenet_svd_set_precondition)
   // grouped-launch tile costs per (tile, K step) relative to 64 x 64, per dtype (0 = unset)
   double tile_cost[2][3] = {{0, 0, 0}, {0, 0, 0}};
   double stat_ms = 0.0, stat_flops = 0.0, stat_bytes = 0.0;
@@ -931,9 +932,9 @@ struct SvdShape {
   bool transposed;
 };
 
-static SvdShape svd_shape(int64_t m, int64_t n, bool allow_split) {
+static SvdShape svd_shape(int64_t m, int64_t n, bool allow_split, bool force_adj = false) {
   SvdShape sh{};
-  sh.transposed = m < n;
+  sh.transposed = force_adj || m < n;
   sh.nr = std::max(m, n);
   sh.ncols = std::min(m, n);
   if (sh.ncols <= 32) {
@@ -968,8 +969,8 @@ static double svd_tol(int64_t nr) { return 1e-14 * std::max(1.0, std::sqrt((doub
 
 static int svd_run(tenet_ctx ctx, hipStream_t s, int dtype, int64_t m, int64_t n, const void* A, int64_t lda, void* U,
                    int64_t ldu, double* S, void* Vh, int64_t ldvh, int64_t maxdim, double thr, int64_t* k_dev,
-                   double* disc_dev, void* wsp, bool allow_split) {
-  const SvdShape sh = svd_shape(m, n, allow_split);
+                   double* disc_dev, void* wsp, bool allow_split, bool force_adj = false) {
+  const SvdShape sh = svd_shape(m, n, allow_split, force_adj);
   Bump bm(wsp);
   SvdProblem P{};
   P.X = (double*)bm.take((size_t)sh.nr * sh.ncp * 16);
@@ -1047,6 +1048,88 @@ static int svd_run(tenet_ctx ctx, hipStream_t s, int dtype, int64_t m, int64_t n
   return TENET_OK;
 }
 
+// ---- QR-preconditioned SVD -----------------------------------------------
+// For the tall form (p x q, p = max(m, n), q = min(m, n)) take the thin
+// Householder QR P = Q R, then run the one-sided Jacobi on R^H (Drmac and
+// Veselic, SIAM J. Matrix Anal. Appl. 29 (2008): the rows of R are graded,
+// so the sweeps converge in roughly half as many rounds, with the same
+// relative accuracy).  SVD(R) = U_R S V_R^H with the Jacobi on X = R^H gives
+//   m >= n (A = Q R):        U = Q U_R,          V^H = V_R^H
+//   m <  n (A^H = Q R):      U = (V_R^H)^H,      V^H = (Q U_R)^H
+static bool svd_precond_applies(tenet_ctx ctx, int dtype, int64_t m, int64_t n) {
+  const int64_t p = std::max(m, n), q = std::min(m, n);
+  return ctx->svd_precond && dtype == TENET_C128 && q >= 64 && qr_blocked_fits(p);
+}
+
+static size_t svd_precond_ws_bytes(int64_t m, int64_t n) {
+  const int64_t p = std::max(m, n), q = std::min(m, n);
+  return align_up((size_t)p * q * 16, 256) + 3 * align_up((size_t)q * q * 16, 256) + align_up(qr_ws_bytes(p, q), 256) +
+         svd_ws_bytes(svd_shape(q, q, true, true)) + 1024;
+}
+
+// workspace any SVD path of an m x n matrix may need
+static size_t svd_any_ws_bytes(tenet_ctx ctx, int dtype, int64_t m, int64_t n) {
+  size_t b = svd_ws_bytes(svd_shape(m, n, true));
+  if (svd_precond_applies(ctx, dtype, m, n)) b = std::max(b, svd_precond_ws_bytes(m, n));
+  return b;
+}
+
+static tenet_tdesc mkdesc2(int64_t rows, int64_t cols, int64_t ld, int64_t lr, int64_t lc) {
+  tenet_tdesc d{};
+  d.rank = 2;
+  d.extent[0] = rows;
+  d.extent[1] = cols;
+  d.stride[0] = 1;
+  d.stride[1] = ld;
+  d.label[0] = lr;
+  d.label[1] = lc;
+  return d;
+}
+
+static int svd_any(tenet_ctx ctx, hipStream_t s, int dtype, int64_t m, int64_t n, const void* A, int64_t lda, void* U,
+                   int64_t ldu, double* S, void* Vh, int64_t ldvh, int64_t maxdim, double thr, int64_t* k_dev,
+                   double* disc_dev, void* wsp) {
+  if (!svd_precond_applies(ctx, dtype, m, n))
+    return svd_run(ctx, s, dtype, m, n, A, lda, U, ldu, S, Vh, ldvh, maxdim, thr, k_dev, disc_dev, wsp, true);
+  const int64_t p = std::max(m, n), q = std::min(m, n);
+  const bool tall = m >= n;
+  Bump bm(wsp);
+  void* Q = bm.take((size_t)p * q * 16);
+  void* R = bm.take((size_t)q * q * 16);
+  void* UR = bm.take((size_t)q * q * 16);
+  void* VRh = bm.take((size_t)q * q * 16);
+  void* qr_ws = bm.take(align_up(qr_ws_bytes(p, q), 256));
+  void* svd_ws = bm.take(svd_ws_bytes(svd_shape(q, q, true, true)));
+  void* one = bm.take(256);
+  // P = Q R with P = A (tall) or A^H (wide)
+  TRY(qr_run(ctx, s, TENET_C128, tall ? 0 : TENET_QR_IN_ADJ, p, q, A, lda, Q, p, R, q, qr_ws));
+  if (tall) {
+    // V^H straight into the caller's Vh; U_R to scratch
+    int st = svd_run(ctx, s, TENET_C128, q, q, R, q, UR, q, S, Vh, ldvh, maxdim, thr, k_dev, disc_dev, svd_ws, true,
+                     true);
+    if (st != TENET_OK && st != TENET_ERR_NOCONV) return st;
+    const tenet_tdesc dQ = mkdesc2(p, q, p, 1, 2), dU = mkdesc2(q, q, q, 2, 3), dO = mkdesc2(m, q, ldu, 1, 3);
+    TRY(einsum_internal(ctx, s, TENET_C128, TENET_PREC_NATIVE, dQ, Q, dU, UR, dO, U, 0, 0));
+    return st;
+  }
+  int st = svd_run(ctx, s, TENET_C128, q, q, R, q, UR, q, S, VRh, q, maxdim, thr, k_dev, disc_dev, svd_ws, true, true);
+  if (st != TENET_OK && st != TENET_ERR_NOCONV) return st;
+  // V^H[j, c] = sum_i conj(U_R[i, j]) conj(Q[c, i])
+  const tenet_tdesc dUR = mkdesc2(q, q, q, 1, 2), dQ = mkdesc2(p, q, p, 3, 1), dVh = mkdesc2(q, n, ldvh, 2, 3);
+  TRY(einsum_internal(ctx, s, TENET_C128, TENET_PREC_NATIVE, dUR, UR, dQ, Q, dVh, Vh, 1, 1));
+  // U[r, j] = conj(V_R^H[j, r])
+  TRY(launch_set_identity(one, 1, TENET_C128, s));
+  const tenet_tdesc dV = mkdesc2(q, q, q, 1, 2), dOne{}, dUo = mkdesc2(m, q, ldu, 2, 1);
+  TRY(einsum_internal(ctx, s, TENET_C128, TENET_PREC_NATIVE, dV, VRh, dOne, one, dUo, U, 1, 0));
+  return st;
+}
+
+extern "C" int tenet_svd_set_precondition(tenet_ctx ctx, int on) {
+  TRY(use_device(ctx));
+  ctx->svd_precond = on != 0;
+  return TENET_OK;
+}
+
 extern "C" int tenet_svd_thin(tenet_ctx ctx, tenet_stream stream, int dtype, int64_t m, int64_t n, const void* A,
                               int64_t lda, void* U, int64_t ldu, double* S, void* Vh, int64_t ldvh, int64_t maxdim,
                               double threshold, int64_t* k_dev, double* discarded_dev) {
@@ -1062,10 +1145,9 @@ extern "C" int tenet_svd_thin(tenet_ctx ctx, tenet_stream stream, int dtype, int
   TRY(check_span(ctx, Vh, ((size_t)(n - 1) * ldvh + k) * es, "Vh"));
   if (S) TRY(check_span(ctx, S, (size_t)k * 8, "S"));
   hipStream_t s = pick(ctx, stream);
-  const SvdShape sh = svd_shape(m, n, true);
   void* w = nullptr;
-  TRY(ws_get(ctx, s, svd_ws_bytes(sh), &w));
-  return svd_run(ctx, s, dtype, m, n, A, lda, U, ldu, S, Vh, ldvh, maxdim, threshold, k_dev, discarded_dev, w, true);
+  TRY(ws_get(ctx, s, svd_any_ws_bytes(ctx, dtype, m, n), &w));
+  return svd_any(ctx, s, dtype, m, n, A, lda, U, ldu, S, Vh, ldvh, maxdim, threshold, k_dev, discarded_dev, w);
 }
 
 // ---------------------------------------------------------------------------
@@ -1089,10 +1171,10 @@ extern "C" int tenet_simple_update(tenet_ctx ctx, tenet_stream stream, int dtype
   TRY(check_span(ctx, Aout, (size_t)m * kmax * es, "Aout"));
   TRY(check_span(ctx, Bout, (size_t)kmax * n * es, "Bout"));
   if (S_out) TRY(check_span(ctx, S_out, (size_t)kmax * 8, "S_out"));
-  const SvdShape sh = svd_shape(m, n, true);
+  const size_t svd_b = align_up(svd_any_ws_bytes(ctx, dtype, m, n), 256);
   const size_t theta_b = align_up((size_t)m * n * es, 256);
   const size_t need = 2 * theta_b + align_up((size_t)m * kmax * es, 256) + align_up((size_t)kmax * n * es, 256) +
-                      align_up((size_t)kmax * 8, 256) + 512 + svd_ws_bytes(sh);
+                      align_up((size_t)kmax * 8, 256) + 512 + svd_b;
   void* w = nullptr;
   TRY(ws_get(ctx, s, need, &w));
   Bump bm(w);
@@ -1104,7 +1186,7 @@ extern "C" int tenet_simple_update(tenet_ctx ctx, tenet_stream stream, int dtype
   int64_t* kd = (int64_t*)bm.take(256);
   int64_t* kuse = k_dev ? k_dev : kd;
   double* duse = disc_dev ? disc_dev : (double*)(kd + 1);
-  void* svd_ws = bm.take(svd_ws_bytes(sh));
+  void* svd_ws = bm.take(svd_b);
   // theta(l, ia, ib, r) = A(l, ia, c) B(c, ib, r)
   const tenet_tdesc dA = mkdesc({chil, d1, chi}, {1, 2, 3});
   const tenet_tdesc dB = mkdesc({chi, d2, chir}, {3, 4, 5});
@@ -1115,7 +1197,7 @@ extern "C" int tenet_simple_update(tenet_ctx ctx, tenet_stream stream, int dtype
   const tenet_tdesc dT2 = mkdesc({chil, d1, d2, chir}, {1, 6, 7, 5});
   const int gprec = (prec == TENET_PREC_SPLIT_BF16 || prec == TENET_PREC_SPLIT_FP16) ? TENET_PREC_NATIVE : prec;
   TRY(einsum_internal(ctx, s, dtype, gprec, dT, theta, dG, G, dT2, theta2, 0, 0));
-  int st = svd_run(ctx, s, dtype, m, n, theta2, m, U, m, S, Vh, kmax, maxdim, threshold, kuse, duse, svd_ws, true);
+  int st = svd_any(ctx, s, dtype, m, n, theta2, m, U, m, S, Vh, kmax, maxdim, threshold, kuse, duse, svd_ws);
   if (st != TENET_OK && st != TENET_ERR_NOCONV) return st;
   TRY(launch_absorb(U, m, Vh, kmax, n, S, kuse, absorb, Aout, Bout, kmax, dtype, s));
   if (k_host) {
@@ -1751,6 +1833,17 @@ extern "C" int tenet_scale(tenet_ctx ctx, tenet_stream stream, int dtype, int64_
   if (count == 0) return TENET_OK;
   TRY(check_span(ctx, x, (size_t)count * ELEM[dtype], "x"));
   return launch_scale(x, dtype, count, alpha[0], alpha[1], pick(ctx, stream));
+}
+
+extern "C" int tenet_sigma_scales(tenet_ctx ctx, tenet_stream stream, int dtype, int64_t k, const double* S,
+                                  void* sqrt_out, void* inv_sqrt_out) {
+  TRY(use_device(ctx));
+  TRY(valid_dtype_prec(dtype, 0));
+  CHECK_ARG(k >= 0 && S && sqrt_out && inv_sqrt_out, "bad argument");
+  if (k == 0) return TENET_OK;
+  TRY(check_span(ctx, sqrt_out, (size_t)k * ELEM[dtype], "sqrt_out"));
+  TRY(check_span(ctx, inv_sqrt_out, (size_t)k * ELEM[dtype], "inv_sqrt_out"));
+  return launch_sigma_scales(S, k, sqrt_out, inv_sqrt_out, dtype, pick(ctx, stream));
 }
 
 extern "C" int tenet_nrm2sq(tenet_ctx ctx, tenet_stream stream, int dtype, int64_t count, const void* x,

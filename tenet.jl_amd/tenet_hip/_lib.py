@@ -98,6 +98,7 @@ SIGNATURES = {
     "tenet_qr_thin": [_p, _p, _i, _i64, _i64, _p, _i64, _p, _i64, _p, _i64],
     "tenet_qr_thin_ex": [_p, _p, _i, _i, _i64, _i64, _p, _i64, _p, _i64, _p, _i64],
     "tenet_svd_thin": [_p, _p, _i, _i64, _i64, _p, _i64, _p, _i64, _p, _p, _i64, _i64, _d, _p, _p],
+    "tenet_svd_set_precondition": [_p, _i],
     "tenet_simple_update": [_p, _p, _i, _i, _i64, _i64, _i64, _i64, _i64, _p, _p, _p, _p, _p, _p, _i64, _d, _i,
                             _p, _p, _pi64],
     "tenet_overlap": [_p, _p, _i, _i, _i64, _i64, _pi64, C.POINTER(_p), _pi64, C.POINTER(_p), _p],
@@ -117,6 +118,7 @@ SIGNATURES = {
     "tenet_fill_gaussian": [_p, _p, _i, _i64, _p, C.c_uint64, C.c_uint64],
     "tenet_scale": [_p, _p, _i, _i64, _p, _pd],
     "tenet_nrm2sq": [_p, _p, _i, _i64, _p, _p],
+    "tenet_sigma_scales": [_p, _p, _i, _i64, _p, _p, _p],
     "tenet_graph_begin": [_p, _p],
     "tenet_graph_end": [_p, _p, C.POINTER(_p)],
     "tenet_graph_launch": [_p, _p],

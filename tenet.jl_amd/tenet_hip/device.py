@@ -57,6 +57,10 @@ class Context:
         L.check(L.lib.tenet_stream_create(self.h, C.byref(s)), "stream_create")
         return s
 
+    def svd_precondition(self, on: bool):
+        """QR-preconditioned Jacobi SVD for complex128 bond matrices (default on)."""
+        L.check(L.lib.tenet_svd_set_precondition(self.h, 1 if on else 0), "svd_set_precondition")
+
     def stats_enable(self, family):
         L.check(L.lib.tenet_stats_enable(self.h, int(family)), "stats_enable")
 

@@ -54,6 +54,11 @@ class MPS:
         # nominal work (real flops): complex MAC = 8; thin QR (p >= q) =
         # 4(4pq^2 - 4q^3/3); thin SVD (p >= q) = 4(4p^2q + 8pq^2 + 9q^3)
         self.flops = {"qr": 0.0, "svd": 0.0, "contract": 0.0}
+        # after simple_update_ (AbsorbEqually) at bond (a, a+1): sites a = U sqrt(S),
+        # a+1 = sqrt(S) V^H.  (a, site a, site a+1, sqrt(S), 1/sqrt(S)) while both
+        # sites are unchanged; the next canonize! move across that bond is then a
+        # diagonal rescaling (see _diag_move)
+        self._gate_bond = None
 
     # ---- construction ---------------------------------------------------
     @staticmethod
@@ -113,8 +118,33 @@ class MPS:
                 "einsum")
         return c
 
+    def _diag_move(self, i, right):
+        """The canonize! move across the bond of the last gate, when its two
+        sites are still U sqrt(S) and sqrt(S) V^H: the thin QR of U sqrt(S) is
+        U times sqrt(S) (Householder's Q and R differ only by diagonal phases,
+        a gauge), so the move rescales the two sites (for a right move:
+        Q = U, R = sqrt(S), next site sqrt(S) sqrt(S) V^H; mirrored for a
+        left move).  Returns False when the shortcut does not apply."""
+        rec = self._gate_bond
+        if rec is None:
+            return False
+        a, sa, sb, sq, isq = rec
+        if (right and i != a) or (not right and i != a + 1) or self.sites[a] is not sa or self.sites[a + 1] is not sb:
+            return False
+        A, B = self.sites[a], self.sites[a + 1]
+        if right:  # site a <- A / sqrt(S) on its right bond; site a+1 <- sqrt(S) B on its left bond
+            self.sites[a] = self._einsum(A, [1, 2, 3], isq, [3], A.shape, [1, 2, 3])
+            self.sites[a + 1] = self._einsum(sq, [1], B, [1, 2, 3], B.shape, [1, 2, 3])
+        else:
+            self.sites[a + 1] = self._einsum(isq, [1], B, [1, 2, 3], B.shape, [1, 2, 3])
+            self.sites[a] = self._einsum(A, [1, 2, 3], sq, [3], A.shape, [1, 2, 3])
+        self._gate_bond = None
+        return True
+
     def _qr_right(self, i):
         """generic_canonize_site!(site i, bond i-(i+1); :qr), canonize.jl:23-76."""
+        if self._diag_move(i, True):
+            return
         A, B = self.sites[i], self.sites[i + 1]
         chil, d, chi = A.shape
         if chi == 1:  # canonize.jl:38
@@ -134,6 +164,8 @@ class MPS:
     def _qr_left(self, i):
         """generic_canonize_site!(site i, bond (i-1)-i; :qr): QR of A^H, so
         A = R^H Q^H with Q^H a right isometry."""
+        if self._diag_move(i - 1, False):
+            return
         A, B = self.sites[i], self.sites[i - 1]
         chi, d, chir = A.shape
         if chi == 1:
@@ -337,11 +369,16 @@ class MPS:
         kmax = min(m, n)
         Aout = DeviceArray(self.ctx, (m * kmax,), self.dtype)
         Bout = DeviceArray(self.ctx, (kmax * n,), self.dtype)
+        # the diagonal shortcut needs every kept S > 0: guaranteed by a positive threshold
+        keep_s = absorb == L.ABSORB_EQUAL and (threshold or 0.0) > 0.0
+        S = self.ctx.malloc(8 * kmax) if keep_s else None
         k = C.c_int64(0)
         st = L.lib.tenet_simple_update(self.ctx.h, None, self.code, self.prec if prec is None else prec, chil, d1, chi,
                                        d2, chir, C.c_void_p(A.ptr), C.c_void_p(B.ptr), C.c_void_p(G.ptr),
-                                       C.c_void_p(Aout.ptr), C.c_void_p(Bout.ptr), None, int(maxdim or 0),
-                                       float(threshold or 0.0), int(absorb), None, None, C.byref(k))
+                                       C.c_void_p(Aout.ptr), C.c_void_p(Bout.ptr), None if S is None else C.c_void_p(S),
+                                       int(maxdim or 0), float(threshold or 0.0), int(absorb), None, None, C.byref(k))
+        if st != L.TENET_OK and S is not None:
+            self.ctx.free(S)
         L.check(st, "simple_update")
         k = int(k.value)
         self.flops["contract"] += 8.0 * m * chi * n + 8.0 * chil * chir * (d1 * d2) ** 2
@@ -349,6 +386,15 @@ class MPS:
         self.sites[a] = Aout.view((chil, d1, k))
         self.sites[a + 1] = Bout.view((k, d2, chir))
         self.form = ("mixed", a, a + 1)
+        self._gate_bond = None
+        if S is not None:
+            sq = DeviceArray(self.ctx, (k,), self.dtype)
+            isq = DeviceArray(self.ctx, (k,), self.dtype)
+            L.check(L.lib.tenet_sigma_scales(self.ctx.h, None, self.code, k, C.c_void_p(S), C.c_void_p(sq.ptr),
+                                             C.c_void_p(isq.ptr)), "sigma_scales")
+            # single-stream use: any later reuse of S's pool block is ordered after the kernel above
+            self.ctx.free(S)
+            self._gate_bond = (a, self.sites[a], self.sites[a + 1], sq, isq)
         return k
 
     def evolve_(self, gate, a):

@@ -493,3 +493,68 @@ def test_normalize_and_norm_c64_and_c128(ctx):
         assert abs(psi.norm() - 1.0) < tol
         v = O.OMPS([a.astype(np.complex128) for a in psi.to_host()]).to_dense()
         assert _rel(v, ref.to_dense() / nref) < tol
+
+
+@pytest.mark.parametrize("m,n,plain", [(100, 64, True), (64, 100, True), (300, 128, True), (128, 300, True),
+                                       (512, 512, False), (200, 512, False)])
+def test_svd_preconditioned_matches_plain(ctx, m, n, plain):
+    """The QR-preconditioned Jacobi (Jacobi on R^H) on a spectrum graded from
+    1 to 1e-13: σ, the truncation count and the rank-k product against the
+    construction, and against the plain one-sided Jacobi where that one
+    converges (at 512 columns and this grading the plain sweeps do not
+    settle within 40 sweeps: the reason for the QR step)."""
+    import tenet_hip as T
+
+    rng = np.random.default_rng(m * 7 + n)
+    k = min(m, n)
+    qa, _ = np.linalg.qr(O.complex_gaussian(rng, (m, k)))
+    qb, _ = np.linalg.qr(O.complex_gaussian(rng, (n, k)))
+    sv = np.logspace(0, -13, k)
+    A = (qa * sv) @ qb.conj().T
+    res = {}
+    for on in ((True, False) if plain else (True,)):
+        ctx.svd_precondition(on)
+        U, S, V = T.tensor_svd_thin(T.Tensor.from_host(ctx, A, ["i", "j"]), ["i"], "s", ["j"], maxdim=k - 3,
+                                    threshold=1e-10)
+        res[on] = (U.to_host(), S, V.to_host())
+    ctx.svd_precondition(True)
+    U1, S1, V1 = res[True]
+    assert len(S1) == int(np.count_nonzero(sv[:k - 3] > 1e-10))
+    assert np.allclose(S1, sv[:len(S1)], rtol=1e-10, atol=1e-15)
+    if plain:
+        S0 = res[False][1]
+        assert len(S0) == len(S1)
+        assert np.allclose(S1, S0, rtol=1e-10, atol=1e-15)
+    best = (qa[:, :len(S1)] * sv[:len(S1)]) @ qb[:, :len(S1)].conj().T
+    assert _rel((U1 * S1) @ V1, best) < 1e-11
+    # vectors formed as X / σ (one side or the other, per path) are orthogonal
+    # to ~eps σ1 / σ: check the well-separated part, σ > 1e-5
+    g = int(np.count_nonzero(S1 > 1e-5))
+    assert np.allclose(U1[:, :g].conj().T @ U1[:, :g], np.eye(g), atol=1e-11)
+    assert np.allclose(V1[:g] @ V1[:g].conj().T, np.eye(g), atol=1e-11)
+
+
+def test_gate_bond_diagonal_move_matches_qr(ctx):
+    """After simple_update! (AbsorbEqually) the next canonize! move across the
+    gate's bond is a diagonal rescaling; it must give the same state and the
+    same canonical form as the Householder move."""
+    import tenet_hip as T
+    from tenet_hip.tebd import brickwork_schedule, haar_gates
+
+    n = 10
+    sched = brickwork_schedule(n, 6)
+    gates = haar_gates(123, len(sched))
+    psi_fast = T.MPS.product_state(ctx, [[1.0, 0.0]] * n)
+    psi_ref = T.MPS.product_state(ctx, [[1.0, 0.0]] * n)
+    for a, U in zip(sched, gates):
+        psi_fast.simple_update_(U, a, maxdim=16, threshold=1e-12)
+        psi_ref.simple_update_(U, a, maxdim=16, threshold=1e-12)
+        psi_ref._gate_bond = None  # force the Householder move
+    for psi in (psi_fast, psi_ref):
+        psi.canonize_(0)
+    assert psi_fast.bondsizes() == psi_ref.bondsizes()
+    vf = O.OMPS([a for a in psi_fast.to_host()]).to_dense()
+    vr = O.OMPS([a for a in psi_ref.to_host()]).to_dense()
+    assert _rel(vf, vr) < 1e-12
+    for a in psi_fast.to_host()[1:]:
+        assert O.is_right_isometry(a, atol=1e-12)
