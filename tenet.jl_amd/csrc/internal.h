@@ -104,7 +104,7 @@ struct SvdProblem {
   double* Gpart;  // npairs * q partial Gram blocks (2b x 2b complex)
   double* sigma;  // ncp
   int32_t* perm;  // ncp
-  int32_t* flag;  // [0] rotated in this sweep, [1] converged, [2] sweeps; flag + 4: the sweep's max relative off-diagonal (double)
+  int32_t* flag;  // [0] rotated in this sweep, [1] converged, [2] sweeps; flag + 4: the sweep's max squared relative off-diagonal (double)
   double* fro2;   // ||A||_F^2 (absolute rotation floor eps^2 * ||A||_F^2)
   int32_t nr, ncols, ncp, b, nblk, q;
   int32_t m, n, transposed;

@@ -34,12 +34,22 @@ struct SvSmem {
   double gi[B2MAX][B2MAX + 1];
   double wr[B2MAX][B2MAX + 1];
   double wi[B2MAX][B2MAX + 1];
-  double xr[RS][B2MAX + 1];
-  double xi[RS][B2MAX + 1];
+  union {
+    struct {  // row staging (Gram accumulation, rotation of X and V)
+      double xr[RS][B2MAX + 1];
+      double xi[RS][B2MAX + 1];
+    };
+    struct {  // second Gram buffer of the inner solve
+      double hr[B2MAX][B2MAX + 1];
+      double hi[B2MAX][B2MAX + 1];
+    };
+  };
   double rc[B2MAX / 2], rsr[B2MAX / 2], rsi[B2MAX / 2];
   int rp[B2MAX / 2], rq[B2MAX / 2], ract[B2MAX / 2];
-  double rmax[B2MAX / 2];  // per rotation slot: max |g_pq| / sqrt(g_pp g_qq) seen
-  int any_rot, any_outer;
+  double rmax[B2MAX / 2];  // per rotation slot: max |g_pq|^2 / (g_pp g_qq) seen
+  double rcs[SV_THREADS / 64][B2MAX / 2][4];  // per wave, per slot: c, Re s, Im s, (pad)
+  int rpq[SV_THREADS / 64][B2MAX / 2];         // per wave, per slot: rotation active
+  int any_outer;
 };
 
 __device__ __forceinline__ int pair_col(int l, int I, int J, int b) { return l < b ? I * b + l : J * b + (l - b); }
@@ -121,23 +131,39 @@ __device__ void gram_over_rows(SvSmem& sm, const double* X, int64_t ld, int r0, 
   }
 }
 
-// Diagonalise the Hermitian sm.g (n2 x n2, n2 <= 32) in place by cyclic
-// Jacobi in the round-robin order; W accumulates the rotations.  Per step the
-// n2/2 disjoint rotations J_k are formed, then G <- J^H G J is applied one
-// 2x2 block at a time, B(k,l) <- J_k^H B(k,l) J_l for k <= l (the mirror
-// block is its conjugate transpose), so G passes through LDS once per step
-// with two barriers.  sm.any_outer reports whether the first sweep rotated.
-// cross_only: the b = n2/2 rotations of step s pair local column k (block I)
-// with column b + (k + s) % b (block J), b steps per sweep, so every I-J pair
-// is visited once and pairs inside a block are left to the full rounds.
-__device__ __forceinline__ void inner_pair(int k, int step, int n2, bool cross_only, int& p, int& q) {
-  if (cross_only) {
+// Diagonalise the Hermitian Gram (n2 x n2, n2 <= 32) by one cyclic Jacobi
+// sweep; W accumulates the rotations.  Step s applies n2/2 disjoint
+// rotations J_k (slot k pairs columns p_k < q_k), G <- J^H G J and W <- W J.
+// CROSS: slot k pairs local column k (block I) with b + (k + s) % b
+// (block J), b steps per sweep, so every I-J pair is visited once and pairs
+// inside a block are left to the full rounds; otherwise the round-robin
+// (circle) order over all n2 columns, n2 - 1 steps.
+//
+// One workgroup barrier per step.  G is double-buffered (the second buffer
+// lies over the row-staging area, idle during the solve).  Every wave
+// computes the step's rotations in its lanes 0..n2/2-1 from the current
+// buffer and passes them to its other lanes through its own LDS slots
+// (the same code on the same operands in every wave, so all waves hold the
+// same rotations; LDS operations of one wave complete in order, so no
+// barrier is needed for that exchange).  Thread t owns the block
+// {p_k, q_k} x {p_l, q_l} with k = t % (n2/2), l = t / (n2/2) (all
+// (n2/2)^2 <= 256 ordered slot pairs, no mirror writes) and two W elements.
+template <bool CROSS>
+__device__ __forceinline__ void inner_pair(int k, int step, int n2, int& p, int& q) {
+  if (CROSS) {
     const int b = n2 / 2;
+    const int j = k + step;  // < 2b
     p = k;
-    q = b + (k + step) % b;
+    q = b + (j >= b ? j - b : j);
   } else {
-    p = circle_pos(k, step, n2);
-    q = circle_pos(n2 - 1 - k, step, n2);
+    // circle_pos without the modulo: i - 1 + r < 2 (n2 - 1)
+    const int m = n2 - 1;
+    const int i0 = k, i1 = n2 - 1 - k;
+    int a0 = i0 - 1 + step, a1 = i1 - 1 + step;
+    a0 = a0 >= m ? a0 - m : a0;
+    a1 = a1 >= m ? a1 - m : a1;
+    p = i0 == 0 ? 0 : 1 + a0;
+    q = i1 == 0 ? 0 : 1 + a1;
     if (p > q) {
       const int t = p;
       p = q;
@@ -146,129 +172,197 @@ __device__ __forceinline__ void inner_pair(int k, int step, int n2, bool cross_o
   }
 }
 
-__device__ void jacobi_inner(SvSmem& sm, int n2, double tol, double abs_tol, int max_sweeps, bool cross_only) {
+typedef double GramRow[B2MAX + 1];
+
+// 1/sqrt(x) for x > 0: hardware estimate and two Newton steps in correction
+// form y <- y + y (1 - x y^2) / 2 (each squares the relative error; the
+// estimate is good to ~2^-23, so two steps reach double precision).
+__device__ __forceinline__ double rsqrt_nr(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  y = fma(0.5 * y, fma(-x * y, y, 1.0), y);
+  y = fma(0.5 * y, fma(-x * y, y, 1.0), y);
+  return y;
+}
+
+// 1/x for x > 0: hardware estimate and two Newton steps y <- y + y (1 - x y)
+__device__ __forceinline__ double rcp_nr(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  y = fma(y, fma(-x, y, 1.0), y);
+  y = fma(y, fma(-x, y, 1.0), y);
+  return y;
+}
+
+// Per-thread ownership in the inner solve (fixed over the steps).
+struct InnerOwn {
+  int bk, bl;  // G block {p_bk, q_bk} x {p_bl, q_bl}
+  bool blk;
+  int wrow[2], wslot[2];  // W elements (row, slot), wrow < 0: none
+};
+
+// One step; wave w = threadIdx.x / 64 publishes its rotations in rcs[w], rpq[w].
+// The G and W entries a thread touches depend only on the step's pairing,
+// so they are read before the rotations are formed (the rotation lanes'
+// own operands first).
+template <bool CROSS>
+__device__ __forceinline__ void inner_step(SvSmem& sm, const GramRow* cr, const GramRow* ci, GramRow* nr, GramRow* ni,
+                                           int step, int n2, const InnerOwn& o, double tol2, double abs2,
+                                           double& rmax2, int& any) {
+  const int tid = threadIdx.x, w = tid >> 6, ln = tid & 63;
+  const int npair = n2 / 2;
+  int rp = 0, rq = 0;
+  double ra = 0.0, rd = 0.0, rxr = 0.0, rxi = 0.0;
+  if (ln < npair) {
+    inner_pair<CROSS>(ln, step, n2, rp, rq);
+    ra = cr[rp][rp];
+    rd = cr[rq][rq];
+    rxr = cr[rp][rq];
+    rxi = ci[rp][rq];
+  }
+  int p0 = 0, q0 = 0, p1 = 0, q1 = 0;
+  double b00r = 0, b00i = 0, b01r = 0, b01i = 0, b10r = 0, b10i = 0, b11r = 0, b11i = 0;
+  if (o.blk) {
+    inner_pair<CROSS>(o.bk, step, n2, p0, q0);
+    inner_pair<CROSS>(o.bl, step, n2, p1, q1);
+    // B = G[{p0,q0}][{p1,q1}]
+    b00r = cr[p0][p1];
+    b00i = ci[p0][p1];
+    b01r = cr[p0][q1];
+    b01i = ci[p0][q1];
+    b10r = cr[q0][p1];
+    b10i = ci[q0][p1];
+    b11r = cr[q0][q1];
+    b11i = ci[q0][q1];
+  }
+  int wp[2], wq[2];
+  double wpr[2], wpi[2], wqr[2], wqi[2];
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    wp[u] = wq[u] = 0;
+    wpr[u] = wpi[u] = wqr[u] = wqi[u] = 0.0;
+    if (o.wrow[u] >= 0) {
+      inner_pair<CROSS>(o.wslot[u], step, n2, wp[u], wq[u]);
+      wpr[u] = sm.wr[o.wrow[u]][wp[u]];
+      wpi[u] = sm.wi[o.wrow[u]][wp[u]];
+      wqr[u] = sm.wr[o.wrow[u]][wq[u]];
+      wqi[u] = sm.wi[o.wrow[u]][wq[u]];
+    }
+  }
+  // Rotation of slot ln: zeroes G[p][q] of [[a, g], [conj(g), d]].  With
+  // delta = (d - a) / 2 and r = sqrt(delta^2 + |g|^2) the angle has
+  // cos 2theta = |delta| / r and sin 2theta = |g| / r, so c = cos theta =
+  // sqrt((1 + |delta| / r) / 2) and s = sign(delta) g / (2 r c) (the
+  // smaller rotation of Rutishauser's formulas, tan theta = |g| / (|delta| + r)).
+  // Identity when |g| <= tol sqrt(a d) or |g| <= abs_tol (compared squared).
+  if (ln < npair) {
+    const double g2 = rxr * rxr + rxi * rxi;
+    const double ad = fmax(ra, 0.0) * fmax(rd, 0.0);
+    const int act = (g2 > tol2 * ad && g2 > abs2) ? 1 : 0;
+    double c = 1.0, sr = 0.0, si = 0.0;
+    if (act) {
+      const double dl = 0.5 * (rd - ra);
+      const double ir = rsqrt_nr(fma(dl, dl, g2));  // 1 / r
+      const double v = fma(0.5 * fabs(dl), ir, 0.5);  // c^2
+      const double y = rsqrt_nr(v);                   // 1 / c
+      c = v * y;
+      const double sn = 0.5 * ir * y;
+      sr = dl >= 0.0 ? sn * rxr : -(sn * rxr);
+      si = dl >= 0.0 ? sn * rxi : -(sn * rxi);
+    }
+    if (w == 0) {  // squared relative off-diagonal, for the convergence test
+      rmax2 = fmax(rmax2, ad > 0.0 ? g2 * rcp_nr(ad) : 0.0);
+      any |= act;
+    }
+    sm.rcs[w][ln][0] = c;
+    sm.rcs[w][ln][1] = sr;
+    sm.rcs[w][ln][2] = si;
+    sm.rpq[w][ln] = act;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  if (o.blk) {
+    const int bk = o.bk, bl = o.bl;
+    const double ck = sm.rcs[w][bk][0], ekr = sm.rcs[w][bk][1], eki = sm.rcs[w][bk][2];
+    const double cl = sm.rcs[w][bl][0], elr = sm.rcs[w][bl][1], eli = sm.rcs[w][bl][2];
+    // T = J_k^H B: T0 = c B0 - e B1 ; T1 = conj(e) B0 + c B1
+    const double t00r = ck * b00r - (ekr * b10r - eki * b10i), t00i = ck * b00i - (ekr * b10i + eki * b10r);
+    const double t01r = ck * b01r - (ekr * b11r - eki * b11i), t01i = ck * b01i - (ekr * b11i + eki * b11r);
+    const double t10r = (ekr * b00r + eki * b00i) + ck * b10r, t10i = (ekr * b00i - eki * b00r) + ck * b10i;
+    const double t11r = (ekr * b01r + eki * b01i) + ck * b11r, t11i = (ekr * b01i - eki * b01r) + ck * b11i;
+    // B' = T J_l: col0 = c T0 - conj(e) T1 ; col1 = e T0 + c T1 (columns of T)
+    double n00r = cl * t00r - (elr * t01r + eli * t01i), n00i = cl * t00i - (elr * t01i - eli * t01r);
+    double n10r = cl * t10r - (elr * t11r + eli * t11i), n10i = cl * t10i - (elr * t11i - eli * t11r);
+    double n01r = (elr * t00r - eli * t00i) + cl * t01r, n01i = (elr * t00i + eli * t00r) + cl * t01i;
+    double n11r = (elr * t10r - eli * t10i) + cl * t11r, n11i = (elr * t10i + eli * t10r) + cl * t11i;
+    if (bk == bl) {  // diagonal block: exactly diagonal and real after the rotation
+      n00i = 0.0;
+      n11i = 0.0;
+      if (sm.rpq[w][bk]) {
+        n01r = n01i = n10r = n10i = 0.0;
+      }
+    }
+    nr[p0][p1] = n00r;
+    ni[p0][p1] = n00i;
+    nr[p0][q1] = n01r;
+    ni[p0][q1] = n01i;
+    nr[q0][p1] = n10r;
+    ni[q0][p1] = n10i;
+    nr[q0][q1] = n11r;
+    ni[q0][q1] = n11i;
+  }
+  // W <- W J on this thread's elements
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    if (o.wrow[u] < 0) continue;
+    const int k = o.wslot[u];
+    if (!sm.rpq[w][k]) continue;
+    const double c = sm.rcs[w][k][0], er = sm.rcs[w][k][1], ei = sm.rcs[w][k][2];
+    const double pr = wpr[u], pi = wpi[u], qr = wqr[u], qi = wqi[u];
+    const int row = o.wrow[u];
+    sm.wr[row][wp[u]] = c * pr - (er * qr + ei * qi);
+    sm.wi[row][wp[u]] = c * pi - (er * qi - ei * qr);
+    sm.wr[row][wq[u]] = (er * pr - ei * pi) + c * qr;
+    sm.wi[row][wq[u]] = (er * pi + ei * pr) + c * qi;
+  }
+  __syncthreads();
+}
+
+template <bool CROSS>
+__device__ void jacobi_inner(SvSmem& sm, int n2, double tol, double abs_tol) {
   const int tid = threadIdx.x;
   const int npair = n2 / 2;
-  const int nblk2 = npair * (npair + 1) / 2;
   for (int e = tid; e < n2 * n2; e += SV_THREADS) {
     const int i = e % n2, j = e / n2;
     sm.wr[i][j] = (i == j) ? 1.0 : 0.0;
     sm.wi[i][j] = 0.0;
   }
   if (tid == 0) sm.any_outer = 0;
-  // block (k, l), k <= l, owned by thread tid < nblk2
-  int bk = 0, bl = 0;
-  if (tid < nblk2) {
-    int rem = tid;
-    while (rem >= npair - bk) {
-      rem -= npair - bk;
-      ++bk;
-    }
-    bl = bk + rem;
+  // every ordered slot pair (k, l) is one block: all npair^2 <= 256 blocks,
+  // each written by its owner (no mirror writes)
+  InnerOwn o;
+  o.blk = tid < npair * npair;
+  o.bk = tid % npair;
+  o.bl = tid / npair;
+  // W element e = tid + 256 u: slot e % npair, row e / npair (npair n2 <= 512)
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const int e = tid + SV_THREADS * u;
+    o.wrow[u] = e < npair * n2 ? e / npair : -1;
+    o.wslot[u] = e % npair;
   }
-  for (int sweep = 0; sweep < max_sweeps; ++sweep) {
-    if (tid == 0) sm.any_rot = 0;
-    const int nsteps = cross_only ? npair : n2 - 1;
-    for (int step = 0; step < nsteps; ++step) {
-      __syncthreads();
-      if (tid < npair) {
-        int p, q;
-        inner_pair(tid, step, n2, cross_only, p, q);
-        const double a = sm.gr[p][p], d = sm.gr[q][q];
-        const double gr = sm.gr[p][q], gi = sm.gi[p][q];
-        const double g2 = gr * gr + gi * gi;
-        if (sweep == 0) {
-          const double ad = fmax(a, 0.0) * fmax(d, 0.0);
-          const double rel = ad > 0.0 ? sqrt(g2 / ad) : 0.0;
-          sm.rmax[tid] = fmax(step == 0 ? 0.0 : sm.rmax[tid], rel);
-        }
-        int act = 0;
-        double c = 1.0, sr = 0.0, si = 0.0;
-        // |g| > tol sqrt(a d) and |g| > abs_tol, compared squared
-        if (g2 > tol * tol * fmax(a, 0.0) * fmax(d, 0.0) && g2 > abs_tol * abs_tol) {
-          act = 1;
-          const double rg = rsqrt(g2);  // 1/|g|
-          const double tau = 0.5 * (d - a) * rg;
-          const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-          c = rsqrt(1.0 + t * t);
-          const double sn = t * c * rg;
-          sr = sn * gr;
-          si = sn * gi;
-          sm.any_rot = 1;
-          if (sweep == 0) sm.any_outer = 1;
-        }
-        sm.rp[tid] = p;
-        sm.rq[tid] = q;
-        sm.rc[tid] = c;
-        sm.rsr[tid] = sr;
-        sm.rsi[tid] = si;
-        sm.ract[tid] = act;
-      }
-      __syncthreads();
-      if (tid < nblk2 && (sm.ract[bk] || sm.ract[bl])) {
-        const int p0 = sm.rp[bk], q0 = sm.rq[bk], p1 = sm.rp[bl], q1 = sm.rq[bl];
-        const double ck = sm.rc[bk], ekr = sm.rsr[bk], eki = sm.rsi[bk];
-        const double cl = sm.rc[bl], elr = sm.rsr[bl], eli = sm.rsi[bl];
-        // B = G[{p0,q0}][{p1,q1}]
-        const double b00r = sm.gr[p0][p1], b00i = sm.gi[p0][p1];
-        const double b01r = sm.gr[p0][q1], b01i = sm.gi[p0][q1];
-        const double b10r = sm.gr[q0][p1], b10i = sm.gi[q0][p1];
-        const double b11r = sm.gr[q0][q1], b11i = sm.gi[q0][q1];
-        // T = J_k^H B: T0 = c B0 - e B1 ; T1 = conj(e) B0 + c B1
-        const double t00r = ck * b00r - (ekr * b10r - eki * b10i), t00i = ck * b00i - (ekr * b10i + eki * b10r);
-        const double t01r = ck * b01r - (ekr * b11r - eki * b11i), t01i = ck * b01i - (ekr * b11i + eki * b11r);
-        const double t10r = (ekr * b00r + eki * b00i) + ck * b10r, t10i = (ekr * b00i - eki * b00r) + ck * b10i;
-        const double t11r = (ekr * b01r + eki * b01i) + ck * b11r, t11i = (ekr * b01i - eki * b01r) + ck * b11i;
-        // B' = T J_l: col0 = c T0 - conj(e) T1 ; col1 = e T0 + c T1 (columns of T)
-        double n00r = cl * t00r - (elr * t01r + eli * t01i), n00i = cl * t00i - (elr * t01i - eli * t01r);
-        double n10r = cl * t10r - (elr * t11r + eli * t11i), n10i = cl * t10i - (elr * t11i - eli * t11r);
-        double n01r = (elr * t00r - eli * t00i) + cl * t01r, n01i = (elr * t00i + eli * t00r) + cl * t01i;
-        double n11r = (elr * t10r - eli * t10i) + cl * t11r, n11i = (elr * t10i + eli * t10r) + cl * t11i;
-        if (bk == bl) {  // diagonal block: exactly diagonal and real after the rotation
-          n00i = 0.0;
-          n11i = 0.0;
-          if (sm.ract[bk]) {
-            n01r = n01i = n10r = n10i = 0.0;
-          }
-        }
-        sm.gr[p0][p1] = n00r;
-        sm.gi[p0][p1] = n00i;
-        sm.gr[p0][q1] = n01r;
-        sm.gi[p0][q1] = n01i;
-        sm.gr[q0][p1] = n10r;
-        sm.gi[q0][p1] = n10i;
-        sm.gr[q0][q1] = n11r;
-        sm.gi[q0][q1] = n11i;
-        if (bk != bl) {
-          sm.gr[p1][p0] = n00r;
-          sm.gi[p1][p0] = -n00i;
-          sm.gr[q1][p0] = n01r;
-          sm.gi[q1][p0] = -n01i;
-          sm.gr[p1][q0] = n10r;
-          sm.gi[p1][q0] = -n10i;
-          sm.gr[q1][q0] = n11r;
-          sm.gi[q1][q0] = -n11i;
-        }
-      }
-      // W <- W J
-      for (int e = tid; e < npair * n2; e += SV_THREADS) {
-        const int k = e / n2, row = e % n2;
-        if (!sm.ract[k]) continue;
-        const int p = sm.rp[k], q = sm.rq[k];
-        const double c = sm.rc[k], er = sm.rsr[k], ei = sm.rsi[k];
-        const double pr = sm.wr[row][p], pi = sm.wi[row][p];
-        const double qr = sm.wr[row][q], qi = sm.wi[row][q];
-        sm.wr[row][p] = c * pr - (er * qr + ei * qi);
-        sm.wi[row][p] = c * pi - (er * qi - ei * qr);
-        sm.wr[row][q] = (er * pr - ei * pi) + c * qr;
-        sm.wi[row][q] = (er * pi + ei * pr) + c * qi;
-      }
-    }
-    __syncthreads();
-    const int rotated = sm.any_rot;
-    __syncthreads();  // everyone has read any_rot before thread 0 resets it
-    if (!rotated) break;
+  const double tol2 = tol * tol, abs2 = abs_tol * abs_tol;
+  double rmax2 = 0.0;
+  int any = 0;
+  const int nsteps = CROSS ? npair : n2 - 1;
+  __syncthreads();
+  // even steps read (gr, gi) and write (hr, hi); odd steps the reverse
+  for (int step = 0; step < nsteps; step += 2) {
+    inner_step<CROSS>(sm, sm.gr, sm.gi, sm.hr, sm.hi, step, n2, o, tol2, abs2, rmax2, any);
+    if (step + 1 < nsteps) inner_step<CROSS>(sm, sm.hr, sm.hi, sm.gr, sm.gi, step + 1, n2, o, tol2, abs2, rmax2, any);
   }
+  if (tid < npair) sm.rmax[tid] = rmax2;
+  if (any) sm.any_outer = 1;
+  __syncthreads();
 }
 
 // rows [r0, r1) of the pair's columns of M <- M W.  Thread t: row t % 64 of
@@ -384,8 +478,12 @@ __global__ __launch_bounds__(SV_THREADS) void svd_round_kernel(SvdBatch Bt, int 
   // converge regardless; a full inner diagonalisation costs ~5x more).  Rotations are skipped only below eps^2 ||A||_F^2: a larger
   // absolute floor (e.g. eps ||A||_F^2) stops the late small-angle rotations
   // that give one-sided Jacobi its accuracy on small singular values.
-  jacobi_inner(sm, n2, P.tol, fmax(4.930380657631324e-32 * (*P.fro2), 1e-300), 1, mode == 1);
-  // largest relative off-diagonal of this pair visit -> the sweep's maximum
+  const double abs_tol = fmax(4.930380657631324e-32 * (*P.fro2), 1e-300);
+  if (mode == 1)
+    jacobi_inner<true>(sm, n2, P.tol, abs_tol);
+  else
+    jacobi_inner<false>(sm, n2, P.tol, abs_tol);
+  // largest squared relative off-diagonal of this pair visit -> the sweep's maximum
   // (non-negative doubles order like their bit patterns)
   if (threadIdx.x == 0 && c == 0) {
     double mx = 0.0;
@@ -408,12 +506,13 @@ __global__ void svd_sweep_end_kernel(SvdBatch Bt, int64_t nprob) {
   const SvdProblem& P = Bt.arr ? Bt.arr[i] : Bt.one;
   if (P.flag[1]) return;
   // Converged when a sweep rotated nothing, or when its largest relative
-  // off-diagonal was below sqrt(tol) / 2: one-sided Jacobi converges
-  // quadratically, so after such a sweep every pair's cosine is ~ that
-  // squared, below tol, and the verification sweep would rotate nothing.
+  // off-diagonal was below sqrt(tol) / 2 (squared: tol / 4): one-sided
+  // Jacobi converges quadratically, so after such a sweep every pair's
+  // cosine is ~ that squared, below tol, and the verification sweep would
+  // rotate nothing.
   unsigned long long* om = reinterpret_cast<unsigned long long*>(P.flag + 4);
-  const double mx = __longlong_as_double((long long)*om);
-  if (P.flag[0] == 0 || mx < 0.5 * sqrt(P.tol)) P.flag[1] = 1;
+  const double mx2 = __longlong_as_double((long long)*om);
+  if (P.flag[0] == 0 || mx2 < 0.25 * P.tol) P.flag[1] = 1;
   P.flag[0] = 0;
   P.flag[2] += 1;
   *om = 0ull;

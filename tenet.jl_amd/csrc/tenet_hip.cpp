@@ -1779,7 +1779,7 @@ extern "C" int tenet_grouped_svd(tenet_ctx ctx, tenet_stream stream, int dtype, 
     P.tol = svd_tol(sh.nr);
   }
   // per problem 8 int32: rotated, converged, sweeps, spare, then the sweep's
-  // largest relative off-diagonal (8 bytes at flag + 4)
+  // largest squared relative off-diagonal (8 bytes at flag + 4)
   flags_base = (int32_t*)bm.take((size_t)count * 32);
   double* fro_base = (double*)bm.take((size_t)count * 8);
   for (int64_t i = 0; i < count; ++i) {
