@@ -13,6 +13,8 @@
 #include "../../include/tenet_hip.h"
 #include "internal.h"
 
+#include <utility>
+
 namespace tenet {
 
 constexpr int QR_THREADS = 256;
@@ -880,6 +882,247 @@ __global__ __launch_bounds__(QP_THREADS) void qr_panel_v2_kernel(double* __restr
   }
 }
 
+// Panel v3: 256 threads, thread t owns panel rows t and t + 256 in
+// registers (up to 512 rows x QB columns), the column loop fully unrolled so
+// every register index is a compile-time constant and the trailing work is
+// triangular.  One block reduction per column: with x the column below the
+// diagonal and P the pivot row (written to LDS by its owner), every dot the
+// column needs follows from e_c = x^H a_c (rows below the diagonal):
+//   ||x||^2 = e_jj,  v^H a_c = conj(s) e_c + P_c  (v = s x below, 1 on the
+//   diagonal),  v_c^H v = conj(P_c) + s conj(e_c)  (c < jj, for T).
+// The 32 real values of e are transpose-reduced inside each wave (levels
+// pairing lanes l ^ 15, l ^ 7, l ^ 2, l ^ 1 by DPP, l ^ 16 by swizzle, l ^ 32
+// by permute; lane l ends with the wave total of value 16 b3 + 8 b2 + 4 b1 +
+// 2 b0 + b4, b_i the bits of l), the four wave totals meet in LDS (double
+// buffered by column parity): one barrier per column.
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+__device__ __forceinline__ double swz16_d(double x) {  // lane l ^ 16 within each 32-lane half
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_ds_swizzle((int)(b & 0xffffffffll), 0x401F);
+  const int hi = __builtin_amdgcn_ds_swizzle((int)(b >> 32), 0x401F);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+template <int H, int CTRL, int BIT>
+__device__ __forceinline__ void halve_dpp(double (&v)[2 * QB], int lane) {
+  const bool hi = (lane & BIT) != 0;
+#pragma unroll
+  for (int i = 0; i < H; ++i) {
+    const double keep = hi ? v[H + i] : v[i];
+    const double send = hi ? v[i] : v[H + i];
+    v[i] = keep + dpp_d<CTRL>(send);
+  }
+}
+
+__device__ __forceinline__ void wave_transpose_reduce32(double (&v)[2 * QB], int lane) {
+  static_assert(2 * QB == 32, "the network below reduces 32 values");
+  halve_dpp<16, 0x140, 8>(v, lane);  // row_mirror: l ^ 15
+  halve_dpp<8, 0x141, 4>(v, lane);   // row_half_mirror: l ^ 7
+  halve_dpp<4, 0x4E, 2>(v, lane);    // quad_perm [2,3,0,1]: l ^ 2
+  halve_dpp<2, 0xB1, 1>(v, lane);    // quad_perm [1,0,3,2]: l ^ 1
+  {
+    const bool hi = (lane & 16) != 0;
+    const double keep = hi ? v[1] : v[0];
+    const double send = hi ? v[0] : v[1];
+    v[0] = keep + swz16_d(send);
+  }
+  v[0] += __shfl_xor(v[0], 32, 64);
+}
+
+constexpr int QP3_THREADS = 256;
+
+// One column of the v3 panel (JJ compile-time: static register indices).
+template <int JJ>
+__device__ __forceinline__ void qr_panel_v3_step(double (&ar)[2][QB], double (&ai)[2][QB],
+                                                 double (*part)[QP3_THREADS / 64][2 * QB],
+                                                 double (*prow)[2 * QB], double (*Tr_s)[QB], double (*Ti_s)[QB],
+                                                 double* __restrict__ tau, int tid, int lane, int wave, int idx, int row0,
+                                                 int row1, int j0) {
+  constexpr int jj = JJ;
+  const int par = jj & 1;
+  if (tid == jj) {  // pivot row jj is row slot 0 of thread jj
+#pragma unroll
+    for (int c = 0; c < QB; ++c) {
+      prow[par][2 * c] = ar[0][c];
+      prow[par][2 * c + 1] = ai[0][c];
+    }
+  }
+  // e_c = sum over this thread's rows below the diagonal of conj(x_r) a_c[r]
+  double e[2 * QB];
+#pragma unroll
+  for (int c = 0; c < QB; ++c) e[2 * c] = e[2 * c + 1] = 0.0;
+#pragma unroll
+  for (int sl = 0; sl < 2; ++sl) {
+    const int row = sl ? row1 : row0;
+    if (row > jj) {
+      const double xr = ar[sl][jj], xi = ai[sl][jj];
+#pragma unroll
+      for (int c = 0; c < QB; ++c) {
+        e[2 * c] += xr * ar[sl][c] + xi * ai[sl][c];
+        e[2 * c + 1] += xr * ai[sl][c] - xi * ar[sl][c];
+      }
+    }
+  }
+  wave_transpose_reduce32(e, lane);
+  if (lane < 32) part[par][wave][idx] = e[0];
+  __syncthreads();
+  // column totals (fixed wave order) and the pivot row
+  auto E = [&](int i) {
+    return ((part[par][0][i] + part[par][1][i]) + part[par][2][i]) + part[par][3][i];
+  };
+  const double alr = prow[par][2 * jj], ali = prow[par][2 * jj + 1];
+  const double xn2 = E(2 * jj);
+  double t_r = 0.0, t_i = 0.0, sr = 1.0, si = 0.0, beta = alr;
+  const bool ident = (xn2 == 0.0 && ali == 0.0);
+  if (!ident) {
+    const double nrm = sqrt(alr * alr + ali * ali + xn2);
+    beta = (alr >= 0.0) ? -nrm : nrm;
+    t_r = (beta - alr) / beta;
+    t_i = -ali / beta;
+    const double dr = alr - beta, di = ali;
+    const double den = dr * dr + di * di;
+    sr = dr / den;
+    si = -di / den;
+  }
+  // trailing columns: w_c = v^H a_c = conj(s) e_c + P_c, f_c = conj(tau) w_c,
+  // a_c[r] -= v_r f_c on rows r >= jj
+  if (!ident) {
+#pragma unroll
+    for (int c = jj + 1; c < QB; ++c) {
+      const double er = E(2 * c), ei = E(2 * c + 1);
+      const double wr = sr * er + si * ei + prow[par][2 * c];
+      const double wi = sr * ei - si * er + prow[par][2 * c + 1];
+      const double fr = t_r * wr + t_i * wi;
+      const double fi = t_r * wi - t_i * wr;
+#pragma unroll
+      for (int sl = 0; sl < 2; ++sl) {
+        const int row = sl ? row1 : row0;
+        if (row == jj) {
+          ar[sl][c] -= fr;
+          ai[sl][c] -= fi;
+        } else if (row > jj) {
+          const double xr = ar[sl][jj], xi = ai[sl][jj];
+          const double vr = xr * sr - xi * si, vi = xr * si + xi * sr;
+          ar[sl][c] -= vr * fr - vi * fi;
+          ai[sl][c] -= vr * fi + vi * fr;
+        }
+      }
+    }
+  }
+  // the reflector into column jj (below the diagonal), beta on it
+#pragma unroll
+  for (int sl = 0; sl < 2; ++sl) {
+    const int row = sl ? row1 : row0;
+    if (row > jj && !ident) {
+      const double xr = ar[sl][jj], xi = ai[sl][jj];
+      ar[sl][jj] = xr * sr - xi * si;
+      ai[sl][jj] = xr * si + xi * sr;
+    } else if (row == jj && !ident) {
+      ar[sl][jj] = beta;
+      ai[sl][jj] = 0.0;
+    }
+  }
+  // T[a][jj] = -tau sum_{c=a}^{jj-1} T[a][c] (v_c^H v), v_c^H v = conj(P_c) + s conj(e_c)
+  if (tid < jj) {
+    double wr = 0.0, wi = 0.0;
+#pragma unroll
+    for (int c = 0; c < jj; ++c) {
+      if (c >= tid) {
+        const double er = E(2 * c), ei = E(2 * c + 1);
+        const double dr = prow[par][2 * c] + (sr * er + si * ei);
+        const double di = -prow[par][2 * c + 1] + (si * er - sr * ei);
+        wr += Tr_s[tid][c] * dr - Ti_s[tid][c] * di;
+        wi += Tr_s[tid][c] * di + Ti_s[tid][c] * dr;
+      }
+    }
+    Tr_s[tid][jj] = -(t_r * wr - t_i * wi);
+    Ti_s[tid][jj] = -(t_r * wi + t_i * wr);
+  }
+  if (tid == jj) {
+    Tr_s[jj][jj] = t_r;
+    Ti_s[jj][jj] = t_i;
+    tau[2 * (j0 + jj)] = t_r;
+    tau[2 * (j0 + jj) + 1] = t_i;
+  }
+  }
+
+template <int... J>
+__device__ __forceinline__ void qr_panel_v3_cols(std::integer_sequence<int, J...>, int jb, double (&ar)[2][QB],
+                                                 double (&ai)[2][QB], double (*part)[QP3_THREADS / 64][2 * QB],
+                                                 double (*prow)[2 * QB], double (*Tr_s)[QB], double (*Ti_s)[QB],
+                                                 double* __restrict__ tau, int tid, int lane, int wave, int idx,
+                                                 int row0, int row1, int j0) {
+  ((J < jb ? qr_panel_v3_step<J>(ar, ai, part, prow, Tr_s, Ti_s, tau, tid, lane, wave, idx, row0, row1, j0) : void()),
+   ...);
+}
+
+__global__ __launch_bounds__(QP3_THREADS) void qr_panel_v3_kernel(double* __restrict__ W, double* __restrict__ tau,
+                                                                   double* __restrict__ Tout, int m, int j0, int jb) {
+  __shared__ double part[2][QP3_THREADS / 64][2 * QB];
+  __shared__ double prow[2][2 * QB];
+  __shared__ double Tr_s[QB][QB], Ti_s[QB][QB];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int mr = m - j0;
+  const int row0 = tid, row1 = tid + QP3_THREADS;
+  const bool h0 = row0 < mr, h1 = row1 < mr;
+  double2* Wc = reinterpret_cast<double2*>(W);
+  double ar[2][QB], ai[2][QB];
+#pragma unroll
+  for (int c = 0; c < QB; ++c) {
+    ar[0][c] = ai[0][c] = ar[1][c] = ai[1][c] = 0.0;
+    if (c < jb) {
+      if (h0) {
+        const double2 v = Wc[(size_t)(j0 + c) * m + j0 + row0];
+        ar[0][c] = v.x;
+        ai[0][c] = v.y;
+      }
+      if (h1) {
+        const double2 v = Wc[(size_t)(j0 + c) * m + j0 + row1];
+        ar[1][c] = v.x;
+        ai[1][c] = v.y;
+      }
+    }
+  }
+  for (int e = tid; e < QB * QB; e += QP3_THREADS) {
+    Tr_s[e % QB][e / QB] = 0.0;
+    Ti_s[e % QB][e / QB] = 0.0;
+  }
+  const int idx = 16 * ((lane >> 3) & 1) + 8 * ((lane >> 2) & 1) + 4 * ((lane >> 1) & 1) + 2 * (lane & 1) +
+                  ((lane >> 4) & 1);
+  qr_panel_v3_cols(std::make_integer_sequence<int, QB>{}, jb, ar, ai, part, prow, Tr_s, Ti_s, tau, tid, lane, wave, idx,
+                   row0, row1, j0);
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < QB; ++c) {
+    if (c < jb) {
+      if (h0) {
+        double2 v;
+        v.x = ar[0][c];
+        v.y = ai[0][c];
+        Wc[(size_t)(j0 + c) * m + j0 + row0] = v;
+      }
+      if (h1) {
+        double2 v;
+        v.x = ar[1][c];
+        v.y = ai[1][c];
+        Wc[(size_t)(j0 + c) * m + j0 + row1] = v;
+      }
+    }
+  }
+  for (int e = tid; e < QB * QB; e += QP3_THREADS) {
+    const int i = e % QB, j = e / QB;
+    Tout[2 * e] = (i < jb && j < jb) ? Tr_s[i][j] : 0.0;
+    Tout[2 * e + 1] = (i < jb && j < jb) ? Ti_s[i][j] : 0.0;
+  }
+}
+
 // M[j0:m, c0:c0+16] <- (I - V T' V^H) M, register-blocked.  Thread t: row
 // group g = t / 16 (rows g, g+16, ... of each staged chunk) and a 4 x 4 block
 // (a0 = 4*((t%16)/4), c0 = 4*(t%4)) of Y = V^H X; the 16 row-group partials
@@ -1354,8 +1597,8 @@ int launch_qr_blocked(double* W, int64_t m, int64_t n, void* Qout, int64_t ldq, 
   double* Yp = Tb + (size_t)np * QB * QB * 2;  // chunk partials of V^H M
   for (int64_t p = 0; p < np; ++p) {
     const int j0 = (int)(p * QB), jb = (int)std::min<int64_t>(QB, k - j0);
-    hipLaunchKernelGGL(qr_panel_v2_kernel, dim3(1), dim3(QP_THREADS), (size_t)2 * QB * QP_MAX_ROWS * sizeof(double),
-                       s, W, tau, Tb + (size_t)p * QB * QB * 2, (int)m, j0, jb);
+    hipLaunchKernelGGL(qr_panel_v3_kernel, dim3(1), dim3(QP3_THREADS), 0, s, W, tau, Tb + (size_t)p * QB * QB * 2,
+                       (int)m, j0, jb);
     TENET_HIP_CHECK(hipGetLastError());
     const int cb = j0 + jb;
     if (cb < n) {
