@@ -55,19 +55,30 @@ struct SvSmem {
 __device__ __forceinline__ int pair_col(int l, int I, int J, int b) { return l < b ? I * b + l : J * b + (l - b); }
 
 // Stage rows [r0, r0+rows) of the pair's 2b columns of M (ld) into LDS.
+// All of a thread's loads (at most 64 x 32 / 256 = 8) are issued before any
+// is written to LDS, so the staging costs one memory latency, not eight.
+constexpr int STAGE_PER = RS * B2MAX / SV_THREADS;
 __device__ __forceinline__ void stage_rows(SvSmem& sm, const double* M, int64_t ld, int r0, int rows, int I, int J,
                                            int b) {
   const int n2 = 2 * b;
-  for (int e = threadIdx.x; e < RS * n2; e += SV_THREADS) {
+  const double2* M2 = reinterpret_cast<const double2*>(M);
+  double2 v[STAGE_PER];
+#pragma unroll
+  for (int u = 0; u < STAGE_PER; ++u) {
+    const int e = threadIdx.x + u * SV_THREADS;
     const int r = e % RS, l = e / RS;
-    double vr = 0.0, vi = 0.0;
-    if (r < rows) {
-      const double2 v = reinterpret_cast<const double2*>(M)[(size_t)pair_col(l, I, J, b) * ld + r0 + r];
-      vr = v.x;
-      vi = v.y;
+    v[u].x = 0.0;
+    v[u].y = 0.0;
+    if (l < n2 && r < rows) v[u] = M2[(size_t)pair_col(l, I, J, b) * ld + r0 + r];
+  }
+#pragma unroll
+  for (int u = 0; u < STAGE_PER; ++u) {
+    const int e = threadIdx.x + u * SV_THREADS;
+    const int r = e % RS, l = e / RS;
+    if (l < n2) {
+      sm.xr[r][l] = v[u].x;
+      sm.xi[r][l] = v[u].y;
     }
-    sm.xr[r][l] = vr;
-    sm.xi[r][l] = vi;
   }
 }
 
@@ -449,17 +460,34 @@ __global__ __launch_bounds__(SV_THREADS) void svd_round_kernel(SvdBatch Bt, int 
       ar[t] = 0.0;
       ai[t] = 0.0;
     }
-    for (int cc = 0; cc < q; ++cc) {
-      const double* in = P.Gpart + ((size_t)p * q + cc) * (size_t)n2 * n2 * 2;
-      if ((int)threadIdx.x < (n2 / 2) * (n2 / 2)) {
+    // the q chunk partials in chunk order, four chunks' loads in flight at once
+    if ((int)threadIdx.x < (n2 / 2) * (n2 / 2)) {
+      int eo[GE_PER_THREAD];
 #pragma unroll
-        for (int t = 0; t < GE_PER_THREAD; ++t) {
-          int row, col;
-          gram_slot(n2, t, row, col);
-          const int e = col * n2 + row;
-          ar[t] += in[2 * e];
-          ai[t] += in[2 * e + 1];
-        }
+      for (int t = 0; t < GE_PER_THREAD; ++t) {
+        int row, col;
+        gram_slot(n2, t, row, col);
+        eo[t] = col * n2 + row;
+      }
+      const double2* base = reinterpret_cast<const double2*>(P.Gpart) + (size_t)p * q * n2 * n2;
+      for (int cc = 0; cc < q; cc += 4) {
+        double2 g[4][GE_PER_THREAD];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+          for (int t = 0; t < GE_PER_THREAD; ++t) {
+            g[u][t].x = 0.0;
+            g[u][t].y = 0.0;
+            if (cc + u < q) g[u][t] = base[(size_t)(cc + u) * n2 * n2 + eo[t]];
+          }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          if (cc + u < q)
+#pragma unroll
+            for (int t = 0; t < GE_PER_THREAD; ++t) {
+              ar[t] += g[u][t].x;
+              ai[t] += g[u][t].y;
+            }
       }
     }
   }
