@@ -30,6 +30,7 @@ for _p in (ROOT, PKG):
 N_SITES, DEPTH, CHI, THR = 64, 20, 256, 1e-10
 SEED = 20261202 + 1
 FIXTURE = os.path.join(ROOT, "tests", "golden", "tebd_n64_d20_chi256.npz")
+VIDAL_FIXTURE = os.path.join(ROOT, "tests", "golden", "tebd_vidal_n64_d20_chi256.npz")
 PEAK_FP64_TFLOPS = 78.6
 # HBM bytes per svd_round_kernel dispatch over one TEBD sweep (n=64, chi_max=256), from two counter
 # passes: FETCH_SIZE 6301.1 KiB x 2.0 + WRITE_SIZE 3985.6 KiB x 1.0 = 16587.8 KiB.
@@ -537,6 +538,121 @@ def bench_svd(args):
     ctx.close()
 
 
+# ---------------------------------------------------------------- Vidal-gauge TEBD
+def layers_of(sched, n):
+    """Split the serpentine brickwork schedule into its layers (order kept,
+    so gate g still acts on bond sched[g])."""
+    out, i, parity = [], 0, 0
+    while i < len(sched):
+        cnt = len(range(parity, n - 1, 2))
+        out.append(list(sched[i:i + cnt]))
+        i += cnt
+        parity ^= 1
+    return out
+
+
+def _vidal_gate_flops(chil, d1, chi, d2, chir):
+    return 8.0 * chil * d1 * chi * d2 * chir + 8.0 * chil * chir * (d1 * d2) ** 2 + _svd_fl(chil * d1, d2 * chir)
+
+
+def cpu_baseline_vidal(layers, gates, budget_s):
+    """The oracle's Vidal-gauge update on the host cores, layer after layer of
+    the same sweep until the time budget is spent."""
+    from oracle import tenet_oracle as O
+
+    v = O.vidal_product_state([[1.0, 0.0]] * N_SITES)
+    flops, done, g = 0.0, 0, 0
+    t0 = time.perf_counter()
+    for bonds in layers:
+        for a in bonds:
+            chil, d1, chi = v.B[a].shape
+            _, d2, chir = v.B[a + 1].shape
+            flops += _vidal_gate_flops(chil, d1, chi, d2, chir)
+            O.vidal_gate(v, gates[g], a, CHI, THR)
+            g += 1
+            done += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    return flops / dt / 1e9, done, dt
+
+
+def bench_tebd_vidal(args):
+    """The TEBD sweep of the default workload (same Haar gates on the same
+    bonds) in the Vidal gauge: each brickwork layer's gates are independent,
+    so a layer is four grouped contraction launches and one grouped SVD
+    (tenet_hip.vidal).  Accuracy against the oracle's Vidal restatement
+    (tests/golden/tebd_vidal_n64_d20_chi256.npz)."""
+    import numpy as np
+    import tenet_hip as T
+    from tenet_hip.tebd import brickwork_schedule, haar_gates, upload_gates
+    from tenet_hip.vidal import tebd_vidal_sweep
+
+    fx = np.load(VIDAL_FIXTURE)
+    bits = np.unpackbits(fx["bits"], axis=1)[:, :N_SITES]
+    sched = brickwork_schedule(N_SITES, DEPTH)
+    gates = haar_gates(SEED, len(sched))
+    layers = layers_of(list(sched), N_SITES)
+    ctx = T.Context(0)
+    gv = upload_gates(ctx, gates)
+
+    def one():
+        v = tebd_vidal_sweep(ctx, N_SITES, layers, gv, CHI, THR)
+        ctx.sync()
+        return v
+
+    for w in range(args.warmup):
+        t0 = time.perf_counter()
+        one().close()
+        log(f"vidal warmup {w} {time.perf_counter() - t0:.3f}s")
+    ctx.stats_enable(T._lib.KSTAT_SVD_ROUND)
+    t_all = time.perf_counter()
+    v = None
+    for s in range(args.steps):
+        if v is not None:
+            v.close()
+        t0 = time.perf_counter()
+        v = one()
+        log(f"vidal step {s} {(time.perf_counter() - t0) * 1e3:.1f} ms")
+    wall = (time.perf_counter() - t_all) / args.steps
+    k = ctx.stats_read()
+    ctx.stats_enable(T._lib.KSTAT_OFF)
+    amps = v.to_mps().amplitudes(bits)
+    rel = float(np.linalg.norm(amps - fx["amps"]) / np.linalg.norm(fx["amps"]))
+    bonds_ok = bool(list(v.bondsizes()) == list(fx["bonds"]))
+    flops = sum(v.flops.values())
+    gflops = flops / wall / 1e9
+    log(f"vidal sweep {wall * 1e3:.1f} ms, relerr {rel:.2e}, bonds match {bonds_ok}")
+    roof = None
+    if k["launches"] > 0 and k["ms"] > 0:
+        ach = k["flops"] / (k["ms"] * 1e-3) / 1e12
+        roof = {"bound": "mfma", "achieved": round(ach, 4), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(ach / PEAK_FP64_TFLOPS, 5), "traffic": None,
+                "kernel": "svd_round_kernel (grouped: every SVD of a layer in each launch)",
+                "launches": k["launches"], "avg_launch_us": round(k["ms"] * 1e3 / k["launches"], 2),
+                "kernel_share_of_step": round(k["ms"] / (wall * 1e3 * args.steps), 3)}
+    cpu = None
+    if not args.no_cpu:
+        threads = os.environ.get("OPENBLAS_NUM_THREADS") or str(os.cpu_count())
+        cg, ng, dt = cpu_baseline_vidal(layers, gates, args.cpu_seconds)
+        cpu = {"value": round(cg, 3), "unit": "GFLOP/s", "cores": int(threads), "kind": "port",
+               "sample": f"oracle Vidal-gauge update on the first {ng} of 630 gates of the same sweep "
+                         f"({dt:.1f} s, numpy/OpenBLAS LAPACK, {threads} threads)"}
+    line = {
+        "metric": "TEBD sweep time & GFLOP/s, Vidal gauge (brickwork layer = grouped launches)",
+        "value": round(gflops, 3), "unit": "GFLOP/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c128", "data": "synthetic (seeded Haar gates, |0...0>)",
+        "config": {"workload": "TEBD brickwork n=64 d=2 depth=20 chi_max=256 threshold=1e-10 (630 gates), Vidal gauge",
+                   "relerr_vs_fp64_oracle": rel, "bonds_match_oracle": bonds_ok,
+                   "gflop_nominal": round(flops / 1e9, 2)},
+        "roofline": roof, "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    v.close()
+    ctx.close()
+
+
 # ---------------------------------------------------------------- single QR (profiling unit)
 def bench_qr(args):
     """tenet_qr_thin of seeded 512 x 256 and 512 x 512 complex128 Gaussian
@@ -705,7 +821,7 @@ def main():
     ap.add_argument("--gates", type=int, default=0,
                     help="profile only: apply the first N gates of the sweep (amplitude check skipped)")
     ap.add_argument("--copy-mib", type=int, default=1024, help="traffic calibration buffer (workload copy)")
-    ap.add_argument("--workload", default="tebd", choices=["tebd", "heff", "plain", "grouped", "serving", "copy", "svd", "qr"])
+    ap.add_argument("--workload", default="tebd", choices=["tebd", "tebd-vidal", "heff", "plain", "grouped", "serving", "copy", "svd", "qr"])
     ap.add_argument("--rates", default="250,500,1000,2000,4000,8000,16000")
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     args = ap.parse_args()
@@ -727,6 +843,8 @@ def main():
         return bench_copy(args)
     if args.workload == "svd":
         return bench_svd(args)
+    if args.workload == "tebd-vidal":
+        return bench_tebd_vidal(args)
     if args.workload == "qr":
         return bench_qr(args)
 

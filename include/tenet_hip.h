@@ -228,6 +228,11 @@ int tenet_left_envs(tenet_ctx ctx, tenet_stream stream, int dtype, int prec, int
 typedef struct tenet_gplan_s* tenet_gplan;
 int tenet_grouped_plan(tenet_ctx ctx, int dtype, int prec, int64_t count, const tenet_tdesc* descs,
                        tenet_gplan* plan);
+/* The same with per-problem conjugation: conj[2*i] conjugates problem i's
+ * a, conj[2*i + 1] its b (binary_einsum on conj(a) / conj(b), as the
+ * transfer-matrix and Vidal-gauge updates need; NULL = none). */
+int tenet_grouped_plan_conj(tenet_ctx ctx, int dtype, int prec, int64_t count, const tenet_tdesc* descs,
+                            const int32_t* conj, tenet_gplan* plan);
 /* ptrs: HOST array of 3*count device pointers (A, B, C per problem); returns
  * after the launch completes */
 int tenet_grouped_exec(tenet_gplan plan, tenet_stream stream, void* const* ptrs);

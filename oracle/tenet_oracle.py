@@ -28,6 +28,12 @@ Parity pins (what checks each function; see tests/test_oracle.py):
   overlap/norm       dense state-vector brute force, norm^2 = overlap
   heff_apply         exact diagonalisation energy of the Ising chain
                      (test/unit/dmrg.jl:38) through a two-site H_eff
+  vidal_* (Vidal-gauge TEBD; the reference's Vidal update is a TODO stub,
+                     so the published algorithm is restated)
+                     dense state-vector brute force without truncation,
+                     equality with the sequential simple_update! TEBD when
+                     nothing is truncated, Lambda = the Schmidt values of
+                     the MixedCanonical state, right-isometric B
 Parity unpinned (no reference-side check exists without Julia): Muscle's
 output label order of binary_einsum, Muscle.simple_update's internal
 normalisation, raw Q/R/U/V gauges (only gauge-invariant quantities are
@@ -654,6 +660,112 @@ def tebd_run(n, sched, gates, maxdim, threshold, progress=None):
         if progress is not None:
             progress(g)
     return psi
+
+
+# --------------------------------------------------------------------------
+# Vidal-gauge TEBD (Gamma / Lambda form)
+# --------------------------------------------------------------------------
+# Tenet holds the Gamma / Lambda containers (VidalMPS.jl:19-204) but its
+# Vidal update and conversions are unfinished (simple_update.jl:94-112 and
+# canonize.jl:258-310 are TODO stubs; no test covers them).  This restates
+# the published algorithm: G. Vidal, Phys. Rev. Lett. 91, 147902 (2003), in
+# the inverse-free form of M. B. Hastings, J. Math. Phys. 50, 095207 (2009):
+# sites are kept as B_i = Gamma_i Lambda_i (right-canonical), Lambda_i is the
+# Schmidt spectrum of bond i-(i+1).  A gate on (a, a+1):
+#   theta~ = G . (B_a B_{a+1}),  X = Lambda_{a-1} theta~ = U S V^H,
+#   keep k by the compress! rule (compress.jl:46-59: maxdim, then the
+#   absolute threshold on raw S),
+#   B_{a+1} <- V^H[:k],  Lambda_a <- S[:k],  B_a <- theta~ V[:, :k]
+# (B_a = Lambda_{a-1}^{-1} U S without dividing by Lambda).  The gates of a
+# brickwork layer act on disjoint bonds and commute, so a layer's updates
+# are independent.
+
+
+class OVidal:
+    """Vidal-gauge MPS: B[i] = Gamma_i Lambda_i as (chi_l, d, chi_r) arrays,
+    lam[i] the Lambda vector of bond i-(i+1) (n - 1 of them)."""
+
+    def __init__(self, B, lam):
+        self.B = [np.asarray(b) for b in B]
+        self.lam = [np.asarray(x, dtype=np.float64) for x in lam]
+
+    @property
+    def n(self):
+        return len(self.B)
+
+    def bondsizes(self):
+        return [b.shape[2] for b in self.B[:-1]]
+
+    def to_mps(self) -> OMPS:
+        """The B sites are right-canonical: an MPS with its centre at site 0."""
+        return OMPS([b.copy() for b in self.B], form=("mixed", 0, 0))
+
+
+def vidal_product_state(vectors) -> OVidal:
+    B = [np.asarray(v, dtype=np.complex128).reshape(1, -1, 1) for v in vectors]
+    nrm = [np.linalg.norm(b) for b in B]
+    B = [b / x for b, x in zip(B, nrm)]
+    B[0] = B[0] * float(np.prod(nrm))
+    return OVidal(B, [np.ones(1) for _ in range(len(B) - 1)])
+
+
+def vidal_from_mps(psi: OMPS) -> OVidal:
+    """Right-canonicalise (centre at site 0), then one left-to-right SVD
+    sweep: X_i = Lambda_{i-1} Bt_i = U S V^H with Bt_0 = A_0 and
+    Bt_{i+1} = V^H A_{i+1}; B_i = Bt_i V, Lambda_i = S (all nonzero kept)."""
+    p = psi.copy()
+    canonize_mixed(p, 0, 0)
+    n = p.n
+    B, lam = [], []
+    bt = p.A[0]
+    prev = np.ones(1)
+    for i in range(n - 1):
+        l, d, r = bt.shape
+        u, s, vh = np.linalg.svd((prev[:, None, None] * bt).reshape(l * d, r), full_matrices=False)
+        k = int(np.count_nonzero(s > 0.0))
+        vh = vh[:k]
+        B.append(np.tensordot(bt, vh.conj(), axes=([2], [1])))  # Bt V
+        lam.append(s[:k])
+        bt = np.tensordot(vh, p.A[i + 1], axes=([1], [0]))
+        prev = s[:k]
+    B.append(bt)
+    return OVidal(B, lam)
+
+
+def vidal_gate(v: OVidal, U4: np.ndarray, a: int, maxdim=None, threshold=None):
+    """One two-site update on (a, a+1); U4 as in simple_update_mps."""
+    A, Bn = v.B[a], v.B[a + 1]
+    chil, d, _ = A.shape
+    _, d2, chir = Bn.shape
+    G = U4.reshape(d, d2, d, d2)  # (oa, ob, ia, ib)
+    th = np.tensordot(A, Bn, axes=([2], [0]))  # (l, ia, ib, r)
+    th = np.einsum("xyij,lijr->lxyr", G, th, optimize=False)
+    left = v.lam[a - 1] if a > 0 else np.ones(1)
+    X = (left[:, None, None, None] * th).reshape(chil * d, d2 * chir)
+    _, s, vh = np.linalg.svd(X, full_matrices=False)
+    k = truncation_keep(s, maxdim, threshold)
+    vh = vh[:k]
+    v.B[a] = (th.reshape(chil * d, d2 * chir) @ vh.conj().T).reshape(chil, d, k)
+    v.B[a + 1] = vh.reshape(k, d2, chir)
+    v.lam[a] = s[:k]
+    return s[:k]
+
+
+def brickwork_layers(n, depth):
+    """The brickwork schedule as layers of disjoint bonds (left sites)."""
+    return [list(range(0, n - 1, 2)) if layer % 2 == 0 else list(range(1, n - 1, 2)) for layer in range(depth)]
+
+
+def vidal_tebd_run(n, layers, gates, maxdim, threshold):
+    """Vidal-gauge TEBD from |0...0>: gates consumed layer by layer in the
+    order of ``layers`` (one gate per bond entry)."""
+    v = vidal_product_state([[1.0, 0.0]] * n)
+    g = 0
+    for bonds in layers:
+        for a in bonds:
+            vidal_gate(v, gates[g], a, maxdim, threshold)
+            g += 1
+    return v
 
 
 # --------------------------------------------------------------------------

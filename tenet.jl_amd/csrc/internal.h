@@ -58,6 +58,7 @@ struct GroupedProblem {
   GemmTables t;
   int64_t M, N, K, batch;
   int32_t a_mfast, b_nfast;
+  int32_t conjA, conjB;     // conjugate operand A / B
   int32_t shape;            // GROUPED_TILE_*
   int32_t kchunks;          // K chunks per tile (1: no split)
   int64_t tiles_m, tiles_n;

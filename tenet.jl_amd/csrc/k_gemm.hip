@@ -451,8 +451,8 @@ __device__ __forceinline__ void grouped_unit(const GroupedProblem& q, const uint
   p.K = q.K;
   p.a_mfast = q.a_mfast;
   p.b_nfast = q.b_nfast;
-  p.conjA = 0;
-  p.conjB = 0;
+  p.conjA = q.conjA;
+  p.conjB = q.conjB;
   p.alpha_re = 1.0;
   p.alpha_im = 0.0;
   p.beta_re = 0.0;

@@ -1424,7 +1424,7 @@ static void tile_costs(tenet_ctx ctx, int dtype, double c[3]) {
 }
 
 static int grouped_plan_build(tenet_ctx ctx, int dtype, int prec, int64_t count, const tenet_tdesc* descs,
-                              int force_shape, tenet_gplan g) {
+                              int force_shape, tenet_gplan g, const int32_t* conj = nullptr) {
   g->ctx = ctx;
   g->dtype = dtype;
   g->prec = prec;
@@ -1451,6 +1451,8 @@ static int grouped_plan_build(tenet_ctx ctx, int dtype, int prec, int64_t count,
     q.batch = hp.batch;
     q.a_mfast = hp.a_mfast;
     q.b_nfast = hp.b_nfast;
+    q.conjA = conj ? (conj[2 * i] != 0) : 0;
+    q.conjB = conj ? (conj[2 * i + 1] != 0) : 0;
     g->spanA[(size_t)i] = hp.spanA;
     g->spanB[(size_t)i] = hp.spanB;
     g->spanC[(size_t)i] = hp.spanC;
@@ -1567,13 +1569,18 @@ static int grouped_plan_build(tenet_ctx ctx, int dtype, int prec, int64_t count,
 
 extern "C" int tenet_grouped_plan(tenet_ctx ctx, int dtype, int prec, int64_t count, const tenet_tdesc* descs,
                                   tenet_gplan* plan) {
+  return tenet_grouped_plan_conj(ctx, dtype, prec, count, descs, nullptr, plan);
+}
+
+extern "C" int tenet_grouped_plan_conj(tenet_ctx ctx, int dtype, int prec, int64_t count, const tenet_tdesc* descs,
+                                       const int32_t* conj, tenet_gplan* plan) {
   TRY(use_device(ctx));
   TRY(valid_dtype_prec(dtype, prec));
   CHECK_ARG(plan && count >= 0 && (count == 0 || descs), "bad argument");
   CHECK_ARG(prec == TENET_PREC_NATIVE || (prec == TENET_PREC_FP32 && dtype == TENET_C64),
             "grouped contractions run in the dtype's native precision");
   auto* g = new tenet_gplan_s();
-  const int st = grouped_plan_build(ctx, dtype, prec, count, descs, -1, g);
+  const int st = grouped_plan_build(ctx, dtype, prec, count, descs, -1, g, conj);
   if (st != TENET_OK) {
     if (g->dev) (void)hipFree(g->dev);
     delete g;
@@ -1790,11 +1797,27 @@ extern "C" int tenet_grouped_svd(tenet_ctx ctx, tenet_stream stream, int dtype, 
   SvdBatch B{};
   B.arr = dprobs;
   TRY(launch_svd_init(B, count, max_elems, (int)max_ncp, dtype, s));
-  std::vector<int32_t> hflags((size_t)count * 8);
+  std::vector<int32_t> hflags((size_t)count * 8, 0);
   bool all = false;
   for (int sw = 0; sw < 40 && !all; ++sw) {
-    TRY(launch_svd_round(B, count, (int)max_pairs, 0, 2, 0, s));
-    for (int r = 0; r < max_rounds; ++r) TRY(launch_svd_round(B, count, (int)max_pairs, r, 2, 1, s));
+    // nominal work of one round over the problems still sweeping: the Gram of
+    // every pair over its rows and the rotation of its X and V rows
+    double fl = 0.0, by = 0.0;
+    for (int64_t i = 0; i < count; ++i) {
+      if (hflags[(size_t)(8 * i + 1)]) continue;
+      const SvdShape& sh = shapes[(size_t)i];
+      const double n2 = 2.0 * sh.b;
+      fl += 8.0 * sh.npairs * n2 * n2 * (2.0 * sh.nr + sh.ncp);
+      by += 16.0 * sh.npairs * n2 * (3.0 * sh.nr + 2.0 * sh.ncp);
+    }
+    {
+      StatScope st(ctx, s, TENET_KSTAT_SVD_ROUND, fl, by);
+      TRY(launch_svd_round(B, count, (int)max_pairs, 0, 2, 0, s));
+    }
+    for (int r = 0; r < max_rounds; ++r) {
+      StatScope st(ctx, s, TENET_KSTAT_SVD_ROUND, fl, by);
+      TRY(launch_svd_round(B, count, (int)max_pairs, r, 2, 1, s));
+    }
     TRY(launch_svd_sweep_end(B, count, s));
     {
       TENET_HIP_CHECK(hipMemcpyAsync(hflags.data(), flags_base, (size_t)count * 32, hipMemcpyDeviceToHost, s));

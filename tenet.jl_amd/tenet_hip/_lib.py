@@ -108,6 +108,7 @@ SIGNATURES = {
     "tenet_left_envs": [_p, _p, _i, _i, _i64, _i64, _pi64, C.POINTER(_p), C.POINTER(_p), _p],
     "tenet_grouped_einsum": [_p, _p, _i, _i, _i64, C.POINTER(TDesc), C.POINTER(_p)],
     "tenet_grouped_plan": [_p, _i, _i, _i64, C.POINTER(TDesc), C.POINTER(_p)],
+    "tenet_grouped_plan_conj": [_p, _i, _i, _i64, C.POINTER(TDesc), C.POINTER(C.c_int32), C.POINTER(_p)],
     "tenet_grouped_exec": [_p, _p, C.POINTER(_p)],
     "tenet_grouped_exec_dev": [_p, _p, _p],
     "tenet_grouped_plan_info": [_p, _pi64, _pi64, _pi64, _pd, _pd],

@@ -32,22 +32,32 @@ def _codes(problems):
 
 class GroupedPlan:
     """problems: sequence of (shape_a, labels_a, shape_b, labels_b,
-    shape_c, labels_c) with column-major contiguous operands."""
+    shape_c, labels_c) with column-major contiguous operands, or with an
+    eighth and ninth entry: strides of a and b (elements).  conj: optional
+    per-problem (conj_a, conj_b) pairs."""
 
-    def __init__(self, ctx: Context, problems, dtype=np.complex64, prec=L.PREC_NATIVE):
+    def __init__(self, ctx: Context, problems, dtype=np.complex64, prec=L.PREC_NATIVE, conj=None):
         self.ctx = ctx
         self.dtype = np.dtype(dtype)
         self.code = DTYPES[self.dtype]
         self.count = len(problems)
         descs = (L.TDesc * max(3 * self.count, 1))()
-        for i, (sa, la, sb, lb, sc, lc) in enumerate(problems):
+        for i, pr in enumerate(problems):
+            sa, la, sb, lb, sc, lc = pr[:6]
+            st_a = pr[6] if len(pr) > 6 else None
+            st_b = pr[7] if len(pr) > 7 else None
             code = _codes(None)
-            descs[3 * i] = tdesc(sa, [code(x) for x in la])
-            descs[3 * i + 1] = tdesc(sb, [code(x) for x in lb])
+            descs[3 * i] = tdesc(sa, [code(x) for x in la], st_a)
+            descs[3 * i + 1] = tdesc(sb, [code(x) for x in lb], st_b)
             descs[3 * i + 2] = tdesc(sc, [code(x) for x in lc])
         self.shapes = [(tuple(p[0]), tuple(p[2]), tuple(p[4])) for p in problems]
         h = C.c_void_p()
-        L.check(L.lib.tenet_grouped_plan(ctx.h, self.code, prec, self.count, descs, C.byref(h)), "grouped_plan")
+        if conj is None:
+            L.check(L.lib.tenet_grouped_plan(ctx.h, self.code, prec, self.count, descs, C.byref(h)), "grouped_plan")
+        else:
+            flags = (C.c_int32 * max(2 * self.count, 1))(*[int(bool(x)) for pair in conj for x in pair])
+            L.check(L.lib.tenet_grouped_plan_conj(ctx.h, self.code, prec, self.count, descs, flags, C.byref(h)),
+                    "grouped_plan_conj")
         self.h = h
 
     def info(self):

@@ -6,6 +6,11 @@
                             1024 seeded bitstrings and their FP64 oracle
                             amplitudes, the final bond dimensions, and the
                             nominal flop counts of the sweep.
+  tebd_vidal_n64_d20_chi256.npz   the same gates on the same bonds (each
+                            brickwork layer's gates together) in the Vidal
+                            gauge (oracle.vidal_gate): the same 1024
+                            bitstrings' amplitudes, bond dimensions and the
+                            Lambda spectrum of the middle bond (--vidal).
 
 Run:  OPENBLAS_NUM_THREADS=8 python tests/golden/make_golden.py [--tag T]
 A second run with another thread count gives the oracle's own
@@ -38,8 +43,11 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=os.path.join(HERE, "tebd_n64_d20_chi256.npz"))
     ap.add_argument("--compare", default=None, help="an earlier fixture to compare against")
+    ap.add_argument("--vidal", action="store_true", help="the Vidal-gauge fixture")
     args = ap.parse_args()
     sched, gates = O.tebd_gates(SEED, N, DEPTH)
+    if args.vidal:
+        return vidal(args, sched, gates)
     t0 = time.perf_counter()
     last = [t0]
 
@@ -62,6 +70,32 @@ def main():
         print("relative l2 difference to", args.compare, ":", rel)
     np.savez(args.out, **out)
     print(f"sweep {dt:.1f}s, max bond {max(psi.bondsizes())}, wrote {args.out}")
+
+
+def layers_of(sched, n):
+    """Split the serpentine schedule into its brickwork layers (order kept)."""
+    out, i, parity = [], 0, 0
+    while i < len(sched):
+        cnt = len(range(parity, n - 1, 2))
+        out.append(list(sched[i:i + cnt]))
+        i += cnt
+        parity ^= 1
+    return out
+
+
+def vidal(args, sched, gates):
+    out_path = args.out if args.out != os.path.join(HERE, "tebd_n64_d20_chi256.npz") else \
+        os.path.join(HERE, "tebd_vidal_n64_d20_chi256.npz")
+    layers = layers_of(sched, N)
+    t0 = time.perf_counter()
+    v = O.vidal_tebd_run(N, layers, gates, CHI, THR)
+    dt = time.perf_counter() - t0
+    psi = v.to_mps()
+    bits = bitstrings(N, NAMP, SEED + 7)
+    amps = np.array([O.amplitude(b, psi) for b in bits])
+    np.savez(out_path, bits=np.packbits(bits, axis=1), amps=amps, bonds=np.array(v.bondsizes()),
+             lam_mid=v.lam[N // 2 - 1], seconds=dt, threads=os.environ.get("OPENBLAS_NUM_THREADS", "default"))
+    print(f"vidal sweep {dt:.1f}s, max bond {max(v.bondsizes())}, wrote {out_path}")
 
 
 if __name__ == "__main__":

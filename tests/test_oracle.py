@@ -325,3 +325,54 @@ def test_synth_splitmix_known_values():
     with np.errstate(over="ignore"):
         got = synth._mix64(k * synth._G)
     assert [int(v) for v in got] == [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
+
+
+# ---- Vidal-gauge TEBD (restated algorithm; the reference's update is a stub)
+def _dense_brickwork(n, layers, gates):
+    psi = np.zeros(2 ** n, complex)
+    psi[0] = 1.0
+    g = 0
+    for bonds in layers:
+        for a in bonds:
+            psi = np.kron(np.kron(np.eye(2 ** a), gates[g]), np.eye(2 ** (n - a - 2))) @ psi
+            g += 1
+    return psi
+
+
+def test_vidal_tebd_untruncated_equals_dense_and_sequential():
+    """Without truncation the Vidal-gauge TEBD is exact: its state equals the
+    dense state vector and the sequential simple_update! TEBD's; every Lambda
+    equals the Schmidt spectrum of that bond; every B is right-isometric."""
+    rng = np.random.default_rng(11)
+    n, depth = 8, 6
+    layers = O.brickwork_layers(n, depth)
+    gates = np.stack([O.haar_unitary(rng, 4) for _ in range(sum(map(len, layers)))])
+    v = O.vidal_tebd_run(n, layers, gates, None, None)
+    dense = _dense_brickwork(n, layers, gates)
+    assert np.allclose(v.to_mps().to_dense(), dense, atol=1e-13)
+    seq = O.tebd_run(n, [a for b in layers for a in b], gates, None, None)
+    assert np.allclose(seq.to_dense(), dense, atol=1e-13)
+    for i in range(n - 1):
+        sv = np.sort(O.schmidt_values(seq.copy(), i))[::-1]
+        assert np.allclose(sv[:len(v.lam[i])], v.lam[i], atol=1e-12)
+        assert np.allclose(sv[len(v.lam[i]):], 0.0, atol=1e-12)
+    assert all(O.is_right_isometry(b, 1e-12) for b in v.B)
+
+
+def test_vidal_truncation_rule_and_from_mps():
+    """A layer keeps min(maxdim, #S > threshold) per bond (compress.jl:46-59);
+    vidal_from_mps reproduces the state and its Schmidt spectra."""
+    rng = np.random.default_rng(12)
+    n = 8
+    psi = O.random_gaussian_mps(rng, n, 2, 6)
+    O.normalize(psi)
+    v = O.vidal_from_mps(psi)
+    assert np.allclose(v.to_mps().to_dense(), psi.to_dense(), atol=1e-12)
+    for i in range(n - 1):
+        sv = np.sort(O.schmidt_values(psi.copy(), i))[::-1]
+        assert np.allclose(sv[:len(v.lam[i])], v.lam[i], atol=1e-12)
+    gates = [O.haar_unitary(rng, 4) for _ in range(4)]
+    bonds = [0, 2, 4, 6]
+    for U, a in zip(gates, bonds):
+        s = O.vidal_gate(v, U, a, maxdim=3, threshold=1e-3)
+        assert len(s) == min(3, int(np.count_nonzero(s > 1e-3))) and v.bondsizes()[a] == len(s)
