@@ -28,6 +28,8 @@ Parity pins (what checks each function; see tests/test_oracle.py):
   overlap/norm       dense state-vector brute force, norm^2 = overlap
   heff_apply         exact diagonalisation energy of the Ising chain
                      (test/unit/dmrg.jl:38) through a two-site H_eff
+  evolve_mpo_*       dense MPO x state product (direct exact; zip-up exact
+                     without truncation), truncation rule, left isometries
   vidal_* (Vidal-gauge TEBD; the reference's Vidal update is a TODO stub,
                      so the published algorithm is restated)
                      dense state-vector brute force without truncation,
@@ -766,6 +768,75 @@ def vidal_tebd_run(n, layers, gates, maxdim, threshold):
             vidal_gate(v, gates[g], a, maxdim, threshold)
             g += 1
     return v
+
+
+# --------------------------------------------------------------------------
+# MPO application (evolve!(::MPS, ::AbstractMPO))
+# --------------------------------------------------------------------------
+# MPO sites as W[l, r, o, i] arrays with size-1 edge bonds (MPO.jl:7 order;
+# Tenet stores the edge sites without the l / r leg).  The MPS's physical
+# index is contracted with the MPO's input i, the output o becomes the new
+# physical index (align!(mps, :outputs, op, :inputs), evolve.jl:24-25).
+
+
+def mpo_sites(W, n, left_vec, right_vec):
+    """Site tensors of a uniform MPO from its bulk W[l, r, o, i] and boundary vectors."""
+    first = np.einsum("l,lroi->roi", np.asarray(left_vec, W.dtype), W, optimize=False)[None]
+    last = np.einsum("lroi,r->loi", W, np.asarray(right_vec, W.dtype), optimize=False)[:, None]
+    return [first] + [W.copy() for _ in range(n - 2)] + [last]
+
+
+def mpo_sites_dense(Ws):
+    """Dense operator (site 1 the most significant digit) from MPO site tensors."""
+    M = Ws[0][0]  # (r, o, i)
+    for W in Ws[1:]:
+        M = np.einsum("aOI,abxy->bOxIy", M, W, optimize=False)
+        D = M.shape[1] * M.shape[2]
+        M = M.reshape(M.shape[0], D, D)
+    return M[0]
+
+
+def evolve_mpo_direct(psi: OMPS, Ws) -> OMPS:
+    """generic_evolve_mps_mpo_direct! (evolve.jl:19-46) + the canonical-form
+    reset of evolve!(::MPS, ::AbstractMPO) (evolve.jl:131-138): per site
+    A'[(l, L), o, (r, R)] = sum_p A[l, p, r] W[L, R, o, p], the virtual
+    pairs fused (mps bond major, MPO bond minor), form MixedCanonical(all)."""
+    out = []
+    for A, W in zip(psi.A, Ws):
+        c = np.einsum("lpr,LRop->lLorR", A, W, optimize=False)
+        l, L_, o, r, R = c.shape
+        out.append(c.reshape(l * L_, o, r * R))
+    return OMPS(out, form=("mixed", 0, psi.n - 1))
+
+
+def evolve_mpo_zipup(psi: OMPS, Ws, maxdim=None, threshold=None) -> OMPS:
+    """generic_evolve_mps_mpo_zipup! (evolve.jl:49-129): site 1 gives
+    C[o, (r, R)] = U S V, truncated by the compress! rule, site 1 <- U,
+    carry R = S V; site i: R . A_i . W_i as [(o, s), (r, R)] = U S V, site i
+    <- U as (s, o, s'), carry S V; the last site takes R . A_n . W_n.  The
+    reference spells the keyword ``threhold`` (evolve.jl:49).  The result's
+    centre is the last site (the U's are left-isometric); the reference
+    leaves the canonical-form trait as it was."""
+    n = psi.n
+    A, W = psi.A[0], Ws[0]
+    C = np.einsum("pr,Rop->orR", A[0], W[0], optimize=False)  # (o, r, R)
+    d, r, R = C.shape
+    u, s, vh = np.linalg.svd(C.reshape(d, r * R), full_matrices=False)
+    k = truncation_keep(s, maxdim, threshold)
+    sites = [u[:, :k].reshape(1, d, k)]
+    carry = (s[:k, None] * vh[:k]).reshape(k, r, R)
+    for i in range(1, n - 1):
+        t = np.einsum("srR,rpq->sRpq", carry, psi.A[i], optimize=False)
+        t = np.einsum("sRpq,RQop->osqQ", t, Ws[i], optimize=False)  # (o, s, r2, R2)
+        o, sd, q, Q = t.shape
+        u, s, vh = np.linalg.svd(t.reshape(o * sd, q * Q), full_matrices=False)
+        k = truncation_keep(s, maxdim, threshold)
+        sites.append(np.transpose(u[:, :k].reshape(o, sd, k), (1, 0, 2)))
+        carry = (s[:k, None] * vh[:k]).reshape(k, q, Q)
+    t = np.einsum("srR,rp->sRp", carry, psi.A[n - 1][:, :, 0], optimize=False)
+    t = np.einsum("sRp,Rop->so", t, Ws[n - 1][:, 0], optimize=False)
+    sites.append(t[:, :, None])
+    return OMPS(sites, form=("mixed", n - 1, n - 1))
 
 
 # --------------------------------------------------------------------------

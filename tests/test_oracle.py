@@ -376,3 +376,24 @@ def test_vidal_truncation_rule_and_from_mps():
     for U, a in zip(gates, bonds):
         s = O.vidal_gate(v, U, a, maxdim=3, threshold=1e-3)
         assert len(s) == min(3, int(np.count_nonzero(s > 1e-3))) and v.bondsizes()[a] == len(s)
+
+
+# ---- MPO application (evolve!(::MPS, ::AbstractMPO), evolve.jl:19-138)
+def test_evolve_mpo_direct_and_zipup_equal_dense_product():
+    rng = np.random.default_rng(31)
+    n = 7
+    psi = O.random_gaussian_mps(rng, n, 2, 5)
+    Ws = O.mpo_sites(O.xxz_mpo_W(1.0, 0.5), n, np.eye(5)[4], np.eye(5)[0])
+    H = O.mpo_sites_dense(Ws)
+    assert np.allclose(H, O.mpo_dense([O.xxz_mpo_W(1.0, 0.5)] * n, np.eye(5)[4], np.eye(5)[0]))
+    ref = H @ psi.to_dense()
+    d = O.evolve_mpo_direct(psi, Ws)
+    assert d.bondsizes() == [5 * x for x in psi.bondsizes()] and d.form == ("mixed", 0, n - 1)
+    assert np.allclose(d.to_dense(), ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+    z = O.evolve_mpo_zipup(psi, Ws, None, None)
+    assert np.allclose(z.to_dense(), ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+    assert all(O.is_left_isometry(a, 1e-12) for a in z.A[:-1])
+    # truncation: each cut keeps min(maxdim, #S > threshold)
+    zt = O.evolve_mpo_zipup(psi, Ws, 3, 1e-8)
+    assert max(zt.bondsizes()) <= 3
+    assert np.linalg.norm(zt.to_dense() - ref) < np.linalg.norm(ref)
