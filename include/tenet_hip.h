@@ -266,6 +266,17 @@ int tenet_fill_gaussian(tenet_ctx ctx, tenet_stream stream, int dtype, int64_t c
 /* x *= alpha (normalize!(tensor) at src/Operations/normalize.jl:9,26 scales
  * the centre tensor this way) */
 int tenet_scale(tenet_ctx ctx, tenet_stream stream, int dtype, int64_t count, void* x, const double alpha[2]);
+/* One step of Tenet's sample(psi, nsamples) (src/Operations/sample.jl:20-72)
+ * at site `site` of an n-site MPS canonised to its first site.  T (d x chir
+ * x nsamp, column-major) holds every sample's projected site tensor
+ * T[x, r, s] = sum_l V[l, s] A[l, x, r] (tenet_einsum of the previous
+ * projections V with the site).  Per sample: p_x = sum_r |T[x, r, s]|^2,
+ * x* drawn from p / sum(p) with the uniform of generator element
+ * s * n + site (the tenet_fill_gaussian stream's u1, seed `seed`),
+ * bits[s * n + site] = x* (int32), and, unless Vn is NULL (last site),
+ * Vn[r, s] = T[x*, r, s] / sqrt(p_x*).  d <= 8. */
+int tenet_sample_select(tenet_ctx ctx, tenet_stream stream, int dtype, const void* T, int d, int chir, int64_t nsamp,
+                        int n, int site, uint64_t seed, int32_t* bits, void* Vn);
 /* sqrt_out[j] = sqrt(S[j]) and inv_sqrt_out[j] = 1 / sqrt(S[j]) (0 where
  * S[j] = 0) as complex vectors of length k: the diagonal factors of a bond
  * after simple_update's AbsorbEqually (simple_update.jl:63), with which the

@@ -28,6 +28,9 @@ Parity pins (what checks each function; see tests/test_oracle.py):
   overlap/norm       dense state-vector brute force, norm^2 = overlap
   heff_apply         exact diagonalisation energy of the Ising chain
                      (test/unit/dmrg.jl:38) through a two-site H_eff
+  sample             the reference's boolean-MPS test (test/unit/mps.jl:445-463:
+                     every sample in {|0010>, |1101>}), frequencies against
+                     |psi(x)|^2 (chi-square) on a dense-checkable state
   evolve_mpo_*       dense MPO x state product (direct exact; zip-up exact
                      without truncation), truncation rule, left isometries
   vidal_* (Vidal-gauge TEBD; the reference's Vidal update is a TODO stub,
@@ -601,6 +604,40 @@ def expectation_pauli(psi: OMPS, start: int, paulis: str):
 # --------------------------------------------------------------------------
 
 BASE_SEED = 20261202
+
+
+def sample(psi: OMPS, nsamples: int, seed: int):
+    """sample(psi, nsamples) (sample.jl:20-72) restated per sample: on a copy
+    canonised to the first site, v = [1]; at site i t = v . A_i (x, r),
+    p_x = sum_r |t[x, r]|^2, x* = the first x whose running sum exceeds
+    u * sum(p) (the last x with p_x > 0 if none), v = t[x*] / sqrt(p_x*).
+    The uniform u of (sample s, site i) is u1 of generator element s n + i
+    (synth.uniforms; Julia's RNG stream is not reproducible here, so this is
+    the device's stream).  Returns 0-based values (Tenet's are 1-based)."""
+    from .synth import uniforms
+
+    p = psi.copy()
+    canonize_mixed(p, 0, 0)
+    n = p.n
+    u = uniforms(seed, 0, nsamples * n)[0].reshape(nsamples, n)
+    out = np.zeros((nsamples, n), dtype=np.int32)
+    for s in range(nsamples):
+        v = np.ones(1, dtype=p.A[0].dtype)
+        for i in range(n):
+            t = np.tensordot(v, p.A[i], axes=([0], [0]))  # (x, r)
+            pr = np.sum(np.abs(t) ** 2, axis=1)
+            target = u[s, i] * pr.sum()
+            cum, pick, last = 0.0, -1, 0
+            for x in range(len(pr)):
+                if pr[x] > 0.0:
+                    last = x
+                cum += pr[x]
+                if pick < 0 and pr[x] > 0.0 and cum > target:
+                    pick = x
+            pick = last if pick < 0 else pick
+            out[s, i] = pick
+            v = t[pick] / np.sqrt(pr[pick])
+    return out
 
 
 def complex_gaussian(rng, shape, dtype=np.complex128):

@@ -153,6 +153,8 @@ int launch_apply_pauli(const void* A, int64_t chil, int64_t chir, char which, vo
 int launch_set_identity(void* out, int64_t chi, int dtype, hipStream_t s);
 int launch_gaussian(void* out, int dtype, int64_t count, uint64_t seed, uint64_t offset, hipStream_t s);
 int launch_scale(void* x, int dtype, int64_t count, double ar, double ai, hipStream_t s);
+int launch_sample_select(const void* T, int dtype, int d, int chir, int64_t nsamp, int n, int site, uint64_t seed,
+                         int32_t* bits, void* Vn, hipStream_t s);
 int launch_nrm2(const void* x, int dtype, int64_t count, double* part, double* out, hipStream_t s);
 int launch_sigma_scales(const double* S, int64_t k, void* sq, void* isq, int dtype, hipStream_t s);
 

@@ -470,6 +470,77 @@ int launch_gaussian(void* out, int dtype, int64_t count, uint64_t seed, uint64_t
   return TENET_OK;
 }
 
+// One sampling step at one site (Tenet's sample, src/Operations/sample.jl:20-72).
+// T holds every sample's projected site tensor, T[x, r, s] = sum_l V[l, s]
+// A[l, x, r] (column-major d x chir x nsamp).  One wave per sample:
+// p_x = sum_r |T[x, r, s]|^2 in FP64 (all lanes end with the same totals: an
+// xor butterfly adds the same pairs in every lane), the sample's uniform u is
+// element s * n + site of the generator above (u1 of that element), x* is the
+// first x whose running sum of p exceeds u * sum_x p_x (the last x with
+// p_x > 0 when rounding leaves none), and V'[r, s] = T[x*, r, s] / sqrt(p_x*).
+constexpr int SAMPLE_MAX_D = 8;
+template <typename S2>
+__global__ __launch_bounds__(256) void sample_select_kernel(const S2* __restrict__ T, int d, int chir, int64_t nsamp,
+                                                            int n, int site, uint64_t seed, int32_t* __restrict__ bits,
+                                                            S2* __restrict__ Vn) {
+  const int lane = threadIdx.x & 63;
+  const int64_t s = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (s >= nsamp) return;  // whole waves; the kernel has no barrier
+  const S2* t = T + s * (int64_t)d * chir;
+  double p[SAMPLE_MAX_D];
+#pragma unroll
+  for (int x = 0; x < SAMPLE_MAX_D; ++x) p[x] = 0.0;
+  for (int r = lane; r < chir; r += 64) {
+#pragma unroll
+    for (int x = 0; x < SAMPLE_MAX_D; ++x) {
+      if (x < d) {
+        const S2 v = t[x + (int64_t)d * r];
+        p[x] += (double)v.x * v.x + (double)v.y * v.y;
+      }
+    }
+  }
+#pragma unroll
+  for (int x = 0; x < SAMPLE_MAX_D; ++x)
+    for (int off = 32; off > 0; off >>= 1) p[x] += __shfl_xor(p[x], off, 64);
+  double total = 0.0;
+  for (int x = 0; x < d; ++x) total += p[x];
+  const uint64_t k = 2ull * ((uint64_t)s * (uint64_t)n + (uint64_t)site);
+  const double u = (double)(mix64(k * 0x9E3779B97F4A7C15ull + seed * 0xD1B54A32D192ED03ull) >> 11) * 0x1p-53;
+  const double target = u * total;
+  int pick = -1, last = 0;
+  double cum = 0.0;
+  for (int x = 0; x < d; ++x) {
+    if (p[x] > 0.0) last = x;
+    cum += p[x];
+    if (pick < 0 && p[x] > 0.0 && cum > target) pick = x;
+  }
+  if (pick < 0) pick = last;
+  if (lane == 0) bits[s * n + site] = pick;
+  if (Vn) {
+    const double sc = p[pick] > 0.0 ? 1.0 / sqrt(p[pick]) : 0.0;
+    for (int r = lane; r < chir; r += 64) {
+      const S2 v = t[pick + (int64_t)d * r];
+      S2 o;
+      o.x = v.x * sc;
+      o.y = v.y * sc;
+      Vn[r + (int64_t)chir * s] = o;
+    }
+  }
+}
+int launch_sample_select(const void* T, int dtype, int d, int chir, int64_t nsamp, int n, int site, uint64_t seed,
+                         int32_t* bits, void* Vn, hipStream_t s) {
+  if (nsamp <= 0) return TENET_OK;
+  const unsigned g = (unsigned)((nsamp + 3) / 4);
+  if (dtype == TENET_C128)
+    hipLaunchKernelGGL(sample_select_kernel<double2>, dim3(g), dim3(256), 0, s, (const double2*)T, d, chir, nsamp, n,
+                       site, seed, bits, (double2*)Vn);
+  else
+    hipLaunchKernelGGL(sample_select_kernel<float2>, dim3(g), dim3(256), 0, s, (const float2*)T, d, chir, nsamp, n,
+                       site, seed, bits, (float2*)Vn);
+  TENET_HIP_CHECK(hipGetLastError());
+  return TENET_OK;
+}
+
 // x *= alpha (complex) for count elements
 template <typename S2>
 __global__ void scale_kernel(S2* __restrict__ x, int64_t count, double ar, double ai) {

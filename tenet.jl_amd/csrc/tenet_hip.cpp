@@ -1858,6 +1858,19 @@ extern "C" int tenet_scale(tenet_ctx ctx, tenet_stream stream, int dtype, int64_
   return launch_scale(x, dtype, count, alpha[0], alpha[1], pick(ctx, stream));
 }
 
+extern "C" int tenet_sample_select(tenet_ctx ctx, tenet_stream stream, int dtype, const void* T, int d, int chir,
+                                   int64_t nsamp, int n, int site, uint64_t seed, int32_t* bits, void* Vn) {
+  TRY(use_device(ctx));
+  TRY(valid_dtype_prec(dtype, 0));
+  CHECK_ARG(T && bits && d >= 1 && d <= 8 && chir >= 1 && nsamp >= 0 && n >= 1 && site >= 0 && site < n,
+            "bad argument");
+  if (nsamp == 0) return TENET_OK;
+  TRY(check_span(ctx, T, (size_t)d * chir * nsamp * ELEM[dtype], "T"));
+  TRY(check_span(ctx, bits, (size_t)nsamp * n * 4, "bits"));
+  if (Vn) TRY(check_span(ctx, Vn, (size_t)chir * nsamp * ELEM[dtype], "Vn"));
+  return launch_sample_select(T, dtype, d, chir, nsamp, n, site, seed, bits, Vn, pick(ctx, stream));
+}
+
 extern "C" int tenet_sigma_scales(tenet_ctx ctx, tenet_stream stream, int dtype, int64_t k, const double* S,
                                   void* sqrt_out, void* inv_sqrt_out) {
   TRY(use_device(ctx));

@@ -117,6 +117,7 @@ SIGNATURES = {
     "tenet_grouped_svd": [_p, _p, _i, _i64, _pi64, _pi64, C.POINTER(_p), C.POINTER(_p), C.POINTER(_p),
                           C.POINTER(_p), _pi64, _d, _p, _p],
     "tenet_fill_gaussian": [_p, _p, _i, _i64, _p, C.c_uint64, C.c_uint64],
+    "tenet_sample_select": [_p, _p, _i, _p, _i, _i, _i64, _i, _i, C.c_uint64, _p, _p],
     "tenet_scale": [_p, _p, _i, _i64, _p, _pd],
     "tenet_nrm2sq": [_p, _p, _i, _i64, _p, _p],
     "tenet_sigma_scales": [_p, _p, _i, _i64, _p, _p, _p],

@@ -493,6 +493,39 @@ class MPS:
             ctx.free(buf)
         return (h[:, 0] + 1j * h[:, 1]) * np.exp(h[:, 2])
 
+    def sample(self, nsamples, seed=0):
+        """Tenet's ``sample(psi, nsamples)`` (src/Operations/sample.jl:20-72):
+        on a copy canonised to the first site, every sample walks the chain
+        with a projected vector V (1 at the first site): T = V . A_i for all
+        samples in one contraction, then tenet_sample_select draws each
+        sample's value from p_x = sum_r |T[x, r]|^2 with the uniform of
+        generator element (sample * n + site) and renormalises its
+        projection.  Returns an (nsamples, n) int array of 0-based values
+        (Tenet's are 1-based)."""
+        p = self.copy()
+        p.canonize_(0)
+        ctx, n = p.ctx, p.n
+        ns = int(nsamples)
+        out = np.zeros((ns, n), dtype=np.int32)
+        if ns == 0:
+            return out
+        bits = ctx.malloc(ns * n * 4)
+        try:
+            V = DeviceArray.from_host(ctx, np.ones((1, ns)), self.dtype)
+            for i in range(n):
+                A = p.sites[i]
+                l, d, r = A.shape
+                T = p._einsum(V, [1, 4], A, [1, 2, 3], (d, r, ns), [2, 3, 4])
+                Vn = DeviceArray(ctx, (r, ns), self.dtype) if i < n - 1 else None
+                L.check(L.lib.tenet_sample_select(ctx.h, None, self.code, C.c_void_p(T.ptr), d, r, ns, n, i,
+                                                  C.c_uint64(int(seed)), C.c_void_p(bits),
+                                                  C.c_void_p(Vn.ptr) if Vn is not None else None), "sample_select")
+                V = Vn
+            L.check(L.lib.tenet_memcpy_d2h(ctx.h, out.ctypes.data_as(C.c_void_p), C.c_void_p(bits), out.nbytes, None))
+        finally:
+            ctx.free(bits)
+        return out
+
     def expect_pauli(self, start, paulis, prec=L.PREC_NATIVE):
         """<psi|P|psi> for a Pauli string on sites start..start+w-1, using the
         canonical form: sites left of the centre are left isometries and

@@ -397,3 +397,42 @@ def test_evolve_mpo_direct_and_zipup_equal_dense_product():
     zt = O.evolve_mpo_zipup(psi, Ws, 3, 1e-8)
     assert max(zt.bondsizes()) <= 3
     assert np.linalg.norm(zt.to_dense() - ref) < np.linalg.norm(ref)
+
+
+# ---- sampling (sample.jl:20-72)
+def boolean_mps_0010_1101():
+    """The reference's test MPS (test/unit/mps.jl:445-463): |0010> + |1101>."""
+    s1 = np.array([[1.0, 0.0], [0.0, 1.0]])  # (r, o)
+    s2 = np.zeros((2, 2, 2))  # (l, r, o)
+    s2[0, 0, 0] = s2[1, 1, 1] = 1.0
+    s3 = np.zeros((2, 2, 2))
+    s3[0, 1, 1] = s3[1, 0, 0] = 1.0
+    s4 = np.array([[0.0, 1.0], [1.0, 0.0]]) / np.sqrt(2.0)  # (l, o)
+    return O.OMPS.from_tenet_arrays([s1, s2, s3, s4], order=("l", "r", "o"))
+
+
+def test_sample_boolean_mps_support():
+    psi = boolean_mps_0010_1101()
+    dense = psi.to_dense()
+    assert np.isclose(abs(dense[0b0010]), 1 / np.sqrt(2)) and np.isclose(abs(dense[0b1101]), 1 / np.sqrt(2))
+    smp = O.sample(psi, 128, seed=5)
+    rows = {tuple(r) for r in smp}
+    assert rows <= {(0, 0, 1, 0), (1, 1, 0, 1)} and len(rows) == 2
+
+
+def test_sample_frequencies_match_born_rule():
+    rng = np.random.default_rng(17)
+    n = 5
+    psi = O.random_gaussian_mps(rng, n, 2, 3)
+    O.normalize(psi)
+    prob = np.abs(psi.to_dense()) ** 2
+    prob /= prob.sum()
+    N = 20000
+    smp = O.sample(psi, N, seed=9)
+    idx = smp @ (1 << np.arange(n - 1, -1, -1))
+    counts = np.bincount(idx, minlength=2 ** n)
+    exp = N * prob
+    m = exp > 5
+    chi2 = np.sum((counts[m] - exp[m]) ** 2 / exp[m])
+    dof = int(m.sum()) - 1
+    assert chi2 < dof + 5 * np.sqrt(2 * dof)
