@@ -1538,7 +1538,9 @@ static int grouped_plan_build(tenet_ctx ctx, int dtype, int prec, int64_t count,
   const size_t b_ctr = align_up((size_t)(64 + ctr_elems) * 4, 256);
   const size_t b_part = align_up((size_t)std::max<int64_t>(part_elems, 1) * 2 * rs, 256);
   const size_t b_ptr = align_up((size_t)count * 3 * 8, 256);
-  TENET_HIP_CHECK(hipMalloc(&g->dev, b_tab + b_prob + b_unit + b_ctr + b_part + b_ptr));
+  // from the context's caching pool: plans are built per batch shape, and a
+  // device allocation per plan would cost more than the plan's analysis
+  TRY(tenet_malloc(ctx, b_tab + b_prob + b_unit + b_ctr + b_part + b_ptr, &g->dev));
   char* base = (char*)g->dev;
   int64_t* tables = (int64_t*)base;
   g->probs = (GroupedProblem*)(base + b_tab);
@@ -1582,7 +1584,7 @@ extern "C" int tenet_grouped_plan_conj(tenet_ctx ctx, int dtype, int prec, int64
   auto* g = new tenet_gplan_s();
   const int st = grouped_plan_build(ctx, dtype, prec, count, descs, -1, g, conj);
   if (st != TENET_OK) {
-    if (g->dev) (void)hipFree(g->dev);
+    if (g->dev) (void)tenet_free(ctx, g->dev);
     delete g;
     return st;
   }
@@ -1655,8 +1657,8 @@ extern "C" int tenet_grouped_exec_dev(tenet_gplan plan, tenet_stream stream, con
 extern "C" int tenet_grouped_plan_destroy(tenet_gplan plan) {
   if (!plan) return TENET_OK;
   (void)hipSetDevice(plan->ctx->device);
-  (void)hipDeviceSynchronize();
-  if (plan->dev) (void)hipFree(plan->dev);
+  (void)hipDeviceSynchronize();  // no launch still reads the arena when the pool hands it out again
+  if (plan->dev) (void)tenet_free(plan->ctx, plan->dev);
   delete plan;
   return TENET_OK;
 }
@@ -1713,7 +1715,8 @@ extern "C" int tenet_autotune_tiles(tenet_ctx ctx, int dtype, double costs_out[3
       const int64_t e = TILE_EDGE[v];
       cost[v] = (double)best / ((double)P * ((M + e - 1) / e) * ((N + e - 1) / e) * (K / 16));
     }
-    if (g.dev) (void)hipFree(g.dev);
+    (void)hipStreamSynchronize(s);
+    if (g.dev) (void)tenet_free(ctx, g.dev);
   }
   (void)hipFree(buf);
   TRY(st);

@@ -22,9 +22,19 @@ class Context:
         self.h = h
         self.device = device
         self._closed = False
+        # host-side plan objects shared by everything on this context (grouped
+        # plans of the Vidal-gauge layers, ...), keyed by the caller; closed
+        # before the context
+        self.plan_cache = {}
 
     def close(self):
         if not self._closed:
+            for p in self.plan_cache.values():
+                try:
+                    p.close()
+                except Exception:
+                    pass
+            self.plan_cache = {}
             L.lib.tenet_destroy(self.h)
             self._closed = True
 

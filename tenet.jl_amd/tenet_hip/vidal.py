@@ -29,8 +29,9 @@ a layer as four grouped contraction launches and one grouped SVD:
   5. B_a <- theta~_k conj(V^H[:k])^T and B_{a+1} <- V^H[:k] (a conjugated
      contraction and a strided copy, one launch)
 
-Plans are cached by the layer's shapes (after the bonds saturate every
-layer of a parity reuses its plans).
+Plans are cached on the context by the layer's shapes (after the bonds
+saturate every layer of a parity reuses its plans, and so does every later
+sweep).
 """
 from __future__ import annotations
 
@@ -128,10 +129,13 @@ class VidalMPS:
 
     # ---- one brickwork layer ------------------------------------------------
     def _plan(self, key, problems, conj=None):
-        p = self._plans.get(key)
+        # cached on the context: every sweep (and every VidalMPS) with the
+        # same layer shapes reuses the plans
+        key = ("vidal", self.dtype.str) + key
+        p = self.ctx.plan_cache.get(key)
         if p is None:
             p = GroupedPlan(self.ctx, problems, self.dtype, L.PREC_NATIVE, conj)
-            self._plans[key] = p
+            self.ctx.plan_cache[key] = p
         return p
 
     def apply_layer_(self, gates, bonds, maxdim=None, threshold=None):
@@ -236,8 +240,7 @@ class VidalMPS:
         return [int(k) for k in ks]
 
     def close(self):
-        for p in self._plans.values():
-            p.close()
+        """Plans live on the context (Context.close releases them)."""
         self._plans = {}
 
 
