@@ -21,6 +21,10 @@
 namespace tenet {
 
 constexpr int SV_THREADS = 256;
+// the round kernel: two waves per SIMD, the Gram, the inner solve and the
+// rotation split so that each thread carries half the work of a 256-thread
+// workgroup's thread
+constexpr int SR_THREADS = 512;
 constexpr int B2MAX = 32;  // 2b
 constexpr int RS = 64;     // staged rows per LDS pass
 constexpr int SORT_MAX = 4096;
@@ -47,8 +51,8 @@ struct SvSmem {
   double rc[B2MAX / 2], rsr[B2MAX / 2], rsi[B2MAX / 2];
   int rp[B2MAX / 2], rq[B2MAX / 2], ract[B2MAX / 2];
   double rmax[B2MAX / 2];  // per rotation slot: max |g_pq|^2 / (g_pp g_qq) seen
-  double rcs[SV_THREADS / 64][B2MAX / 2][4];  // per wave, per slot: c, Re s, Im s, (pad)
-  int rpq[SV_THREADS / 64][B2MAX / 2];         // per wave, per slot: rotation active
+  double rcs[SR_THREADS / 64][B2MAX / 2][4];  // per wave, per slot: c, Re s, Im s, (pad)
+  int rpq[SR_THREADS / 64][B2MAX / 2];         // per wave, per slot: rotation active
   int any_outer;
 };
 
@@ -57,7 +61,7 @@ __device__ __forceinline__ int pair_col(int l, int I, int J, int b) { return l <
 // Stage rows [r0, r0+rows) of the pair's 2b columns of M (ld) into LDS.
 // All of a thread's loads (at most 64 x 32 / 256 = 8) are issued before any
 // is written to LDS, so the staging costs one memory latency, not eight.
-constexpr int STAGE_PER = RS * B2MAX / SV_THREADS;
+constexpr int STAGE_PER = RS * B2MAX / SR_THREADS;
 __device__ __forceinline__ void stage_rows(SvSmem& sm, const double* M, int64_t ld, int r0, int rows, int I, int J,
                                            int b) {
   const int n2 = 2 * b;
@@ -65,7 +69,7 @@ __device__ __forceinline__ void stage_rows(SvSmem& sm, const double* M, int64_t 
   double2 v[STAGE_PER];
 #pragma unroll
   for (int u = 0; u < STAGE_PER; ++u) {
-    const int e = threadIdx.x + u * SV_THREADS;
+    const int e = threadIdx.x + u * SR_THREADS;
     const int r = e % RS, l = e / RS;
     v[u].x = 0.0;
     v[u].y = 0.0;
@@ -73,7 +77,7 @@ __device__ __forceinline__ void stage_rows(SvSmem& sm, const double* M, int64_t 
   }
 #pragma unroll
   for (int u = 0; u < STAGE_PER; ++u) {
-    const int e = threadIdx.x + u * SV_THREADS;
+    const int e = threadIdx.x + u * SR_THREADS;
     const int r = e % RS, l = e / RS;
     if (l < n2) {
       sm.xr[r][l] = v[u].x;
@@ -83,17 +87,21 @@ __device__ __forceinline__ void stage_rows(SvSmem& sm, const double* M, int64_t 
 }
 
 // acc += staged^H staged for this thread's 2x2 block of the Gram matrix:
-// thread t < (n2/2)^2 owns rows {2i, 2i+1} x cols {2j, 2j+1}, i = t % (n2/2),
-// j = t / (n2/2); acc[0..3] = (2i,2j), (2i+1,2j), (2i,2j+1), (2i+1,2j+1).
+// slot t' = t % 256 < (n2/2)^2 owns rows {2i, 2i+1} x cols {2j, 2j+1},
+// i = t' % (n2/2), j = t' / (n2/2); acc[0..3] = (2i,2j), (2i+1,2j),
+// (2i,2j+1), (2i+1,2j+1).  Threads t < 256 take the staged rows [0, 32),
+// threads t >= 256 the rows [32, 64); gram_combine adds the two halves.
 constexpr int GE_PER_THREAD = 4;
+constexpr int GRAM_SLOTS = 256;
 
 __device__ __forceinline__ void gram_accum(SvSmem& sm, int n2, int rows, double* ar, double* ai) {
   const int h = n2 / 2;
-  const int t = threadIdx.x;
+  const int t = threadIdx.x % GRAM_SLOTS, half = threadIdx.x / GRAM_SLOTS;
   if (t >= h * h) return;
   const int i0 = 2 * (t % h), j0 = 2 * (t / h);
+  const int rlo = half * (RS / 2), rhi = min(rows, rlo + RS / 2);
   double s0r = 0, s0i = 0, s1r = 0, s1i = 0, s2r = 0, s2i = 0, s3r = 0, s3i = 0;
-  for (int r = 0; r < rows; ++r) {
+  for (int r = rlo; r < rhi; ++r) {
     const double ar0 = sm.xr[r][i0], ai0 = sm.xi[r][i0];
     const double ar1 = sm.xr[r][i0 + 1], ai1 = sm.xi[r][i0 + 1];
     const double br0 = sm.xr[r][j0], bi0 = sm.xi[r][j0];
@@ -118,6 +126,30 @@ __device__ __forceinline__ void gram_accum(SvSmem& sm, int n2, int rows, double*
   ai[3] += s3i;
 }
 
+// Threads t < 256 end with the full sums: the upper half's partials pass
+// through the (then idle) staging rows.
+__device__ __forceinline__ void gram_combine(SvSmem& sm, int n2, double* ar, double* ai) {
+  const int h = n2 / 2;
+  const int t = threadIdx.x % GRAM_SLOTS, half = threadIdx.x / GRAM_SLOTS;
+  double* buf = &sm.xr[0][0];
+  __syncthreads();
+  if (half == 1 && t < h * h) {
+#pragma unroll
+    for (int u = 0; u < GE_PER_THREAD; ++u) {
+      buf[(2 * u) * GRAM_SLOTS + t] = ar[u];
+      buf[(2 * u + 1) * GRAM_SLOTS + t] = ai[u];
+    }
+  }
+  __syncthreads();
+  if (half == 0 && t < h * h) {
+#pragma unroll
+    for (int u = 0; u < GE_PER_THREAD; ++u) {
+      ar[u] += buf[(2 * u) * GRAM_SLOTS + t];
+      ai[u] += buf[(2 * u + 1) * GRAM_SLOTS + t];
+    }
+  }
+}
+
 // Gram entry index (row, col) of this thread's slot t (0..3)
 __device__ __forceinline__ void gram_slot(int n2, int slot, int& row, int& col) {
   const int h = n2 / 2;
@@ -140,6 +172,7 @@ __device__ void gram_over_rows(SvSmem& sm, const double* X, int64_t ld, int r0, 
     __syncthreads();
     gram_accum(sm, n2, rows, ar, ai);
   }
+  gram_combine(sm, n2, ar, ai);
 }
 
 // Diagonalise the Hermitian Gram (n2 x n2, n2 <= 32) by one cyclic Jacobi
@@ -156,9 +189,10 @@ __device__ void gram_over_rows(SvSmem& sm, const double* X, int64_t ld, int r0, 
 // buffer and passes them to its other lanes through its own LDS slots
 // (the same code on the same operands in every wave, so all waves hold the
 // same rotations; LDS operations of one wave complete in order, so no
-// barrier is needed for that exchange).  Thread t owns the block
-// {p_k, q_k} x {p_l, q_l} with k = t % (n2/2), l = t / (n2/2) (all
-// (n2/2)^2 <= 256 ordered slot pairs, no mirror writes) and two W elements.
+// barrier is needed for that exchange).  Threads 2j and 2j + 1 own the two
+// output rows of the block {p_k, q_k} x {p_l, q_l}, k = j % (n2/2),
+// l = j / (n2/2) (all (n2/2)^2 <= 256 ordered slot pairs, no mirror writes);
+// each thread also owns one W element (512 threads, n2 <= 32).
 template <bool CROSS>
 __device__ __forceinline__ void inner_pair(int k, int step, int n2, int& p, int& q) {
   if (CROSS) {
@@ -206,6 +240,7 @@ __device__ __forceinline__ double rcp_nr(double x) {
 // Per-thread ownership in the inner solve (fixed over the steps).
 struct InnerOwn {
   int bk, bl;  // G block {p_bk, q_bk} x {p_bl, q_bl}
+  int half;    // output row of the block: 0 -> p_bk, 1 -> q_bk
   bool blk;
   int wrow[2], wslot[2];  // W elements (row, slot), wrow < 0: none
 };
@@ -295,31 +330,32 @@ __device__ __forceinline__ void inner_step(SvSmem& sm, const GramRow* cr, const 
     const int bk = o.bk, bl = o.bl;
     const double ck = sm.rcs[w][bk][0], ekr = sm.rcs[w][bk][1], eki = sm.rcs[w][bk][2];
     const double cl = sm.rcs[w][bl][0], elr = sm.rcs[w][bl][1], eli = sm.rcs[w][bl][2];
-    // T = J_k^H B: T0 = c B0 - e B1 ; T1 = conj(e) B0 + c B1
-    const double t00r = ck * b00r - (ekr * b10r - eki * b10i), t00i = ck * b00i - (ekr * b10i + eki * b10r);
-    const double t01r = ck * b01r - (ekr * b11r - eki * b11i), t01i = ck * b01i - (ekr * b11i + eki * b11r);
-    const double t10r = (ekr * b00r + eki * b00i) + ck * b10r, t10i = (ekr * b00i - eki * b00r) + ck * b10i;
-    const double t11r = (ekr * b01r + eki * b01i) + ck * b11r, t11i = (ekr * b01i - eki * b01r) + ck * b11i;
-    // B' = T J_l: col0 = c T0 - conj(e) T1 ; col1 = e T0 + c T1 (columns of T)
-    double n00r = cl * t00r - (elr * t01r + eli * t01i), n00i = cl * t00i - (elr * t01i - eli * t01r);
-    double n10r = cl * t10r - (elr * t11r + eli * t11i), n10i = cl * t10i - (elr * t11i - eli * t11r);
-    double n01r = (elr * t00r - eli * t00i) + cl * t01r, n01i = (elr * t00i + eli * t00r) + cl * t01i;
-    double n11r = (elr * t10r - eli * t10i) + cl * t11r, n11i = (elr * t10i + eli * t10r) + cl * t11i;
+    // row h of T = J_k^H B: T0 = c B0 - e B1 ; T1 = conj(e) B0 + c B1, i.e.
+    // T_h = a B0 + b B1 with (a, b) = (c, -e) for h = 0 and (conj(e), c) for h = 1
+    const bool h1 = o.half != 0;
+    const double a_r = h1 ? ekr : ck, a_i = h1 ? -eki : 0.0;
+    const double b_r = h1 ? ck : -ekr, b_i = h1 ? 0.0 : -eki;
+    const double t0r = (a_r * b00r - a_i * b00i) + (b_r * b10r - b_i * b10i);
+    const double t0i = (a_r * b00i + a_i * b00r) + (b_r * b10i + b_i * b10r);
+    const double t1r = (a_r * b01r - a_i * b01i) + (b_r * b11r - b_i * b11i);
+    const double t1i = (a_r * b01i + a_i * b01r) + (b_r * b11i + b_i * b11r);
+    // row h of B' = T J_l: col0 = c T0 - conj(e) T1 ; col1 = e T0 + c T1
+    double n0r = cl * t0r - (elr * t1r + eli * t1i), n0i = cl * t0i - (elr * t1i - eli * t1r);
+    double n1r = (elr * t0r - eli * t0i) + cl * t1r, n1i = (elr * t0i + eli * t0r) + cl * t1i;
     if (bk == bl) {  // diagonal block: exactly diagonal and real after the rotation
-      n00i = 0.0;
-      n11i = 0.0;
-      if (sm.rpq[w][bk]) {
-        n01r = n01i = n10r = n10i = 0.0;
+      if (h1) {
+        n1i = 0.0;
+        if (sm.rpq[w][bk]) n0r = n0i = 0.0;
+      } else {
+        n0i = 0.0;
+        if (sm.rpq[w][bk]) n1r = n1i = 0.0;
       }
     }
-    nr[p0][p1] = n00r;
-    ni[p0][p1] = n00i;
-    nr[p0][q1] = n01r;
-    ni[p0][q1] = n01i;
-    nr[q0][p1] = n10r;
-    ni[q0][p1] = n10i;
-    nr[q0][q1] = n11r;
-    ni[q0][q1] = n11i;
+    const int ph = h1 ? q0 : p0;
+    nr[ph][p1] = n0r;
+    ni[ph][p1] = n0i;
+    nr[ph][q1] = n1r;
+    ni[ph][q1] = n1i;
   }
   // W <- W J on this thread's elements
 #pragma unroll
@@ -342,22 +378,25 @@ template <bool CROSS>
 __device__ void jacobi_inner(SvSmem& sm, int n2, double tol, double abs_tol) {
   const int tid = threadIdx.x;
   const int npair = n2 / 2;
-  for (int e = tid; e < n2 * n2; e += SV_THREADS) {
-    const int i = e % n2, j = e / n2;
-    sm.wr[i][j] = (i == j) ? 1.0 : 0.0;
-    sm.wi[i][j] = 0.0;
+  for (int e = tid; e < n2 * n2; e += SR_THREADS) {
+    const int i = e % n2, c = e / n2;
+    sm.wr[i][c] = (i == c) ? 1.0 : 0.0;
+    sm.wi[i][c] = 0.0;
   }
   if (tid == 0) sm.any_outer = 0;
-  // every ordered slot pair (k, l) is one block: all npair^2 <= 256 blocks,
-  // each written by its owner (no mirror writes)
+  // every ordered slot pair (k, l) is one block (all npair^2 <= 256 blocks,
+  // each written by its owners, no mirror writes); threads 2j and 2j + 1
+  // compute its two output rows
   InnerOwn o;
-  o.blk = tid < npair * npair;
-  o.bk = tid % npair;
-  o.bl = tid / npair;
-  // W element e = tid + 256 u: slot e % npair, row e / npair (npair n2 <= 512)
+  const int j = tid >> 1;
+  o.half = tid & 1;
+  o.blk = j < npair * npair;
+  o.bk = j % npair;
+  o.bl = j / npair;
+  // W element e = tid + 512 u: slot e % npair, row e / npair (npair n2 <= 512)
 #pragma unroll
   for (int u = 0; u < 2; ++u) {
-    const int e = tid + SV_THREADS * u;
+    const int e = tid + SR_THREADS * u;
     o.wrow[u] = e < npair * n2 ? e / npair : -1;
     o.wslot[u] = e % npair;
   }
@@ -377,28 +416,31 @@ __device__ void jacobi_inner(SvSmem& sm, int n2, double tol, double abs_tol) {
 }
 
 // rows [r0, r1) of the pair's columns of M <- M W.  Thread t: row t % 64 of
-// the staged chunk and 8 consecutive output columns (group t / 64).
+// the staged chunk and 4 consecutive output columns (group t / 64).
+constexpr int APPLY_COLS = SR_THREADS / RS > 0 ? B2MAX / (SR_THREADS / RS) : B2MAX;
 __device__ void apply_rows(SvSmem& sm, double* M, int64_t ld, int r0, int r1, int I, int J, int b) {
   const int n2 = 2 * b;
-  const int r = threadIdx.x % RS, cg = 8 * (threadIdx.x / RS);
+  const int r = threadIdx.x % RS, cg = APPLY_COLS * (threadIdx.x / RS);
   for (int rb = r0; rb < r1; rb += RS) {
     const int rows = min(RS, r1 - rb);
     __syncthreads();
     stage_rows(sm, M, ld, rb, rows, I, J, b);
     __syncthreads();
     if (r < rows && cg < n2) {
-      double sr[8] = {0, 0, 0, 0, 0, 0, 0, 0}, si[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      double sr[APPLY_COLS], si[APPLY_COLS];
+#pragma unroll
+      for (int j = 0; j < APPLY_COLS; ++j) sr[j] = si[j] = 0.0;
       for (int t = 0; t < n2; ++t) {
         const double xr = sm.xr[r][t], xi = sm.xi[r][t];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
+        for (int j = 0; j < APPLY_COLS; ++j) {
           const double wr = sm.wr[t][cg + j], wi = sm.wi[t][cg + j];
           sr[j] += xr * wr - xi * wi;
           si[j] += xr * wi + xi * wr;
         }
       }
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
+      for (int j = 0; j < APPLY_COLS; ++j) {
         if (cg + j < n2) {
           double2 v;
           v.x = sr[j];
@@ -413,7 +455,7 @@ __device__ void apply_rows(SvSmem& sm, double* M, int64_t ld, int r0, int r1, in
 
 // mode 0: full inner sweep on the fixed pairs (2p, 2p+1) (one such round per
 // sweep); mode 1: cross-block inner steps on the circle pairs of `round`.
-__global__ __launch_bounds__(SV_THREADS) void svd_round_kernel(SvdBatch Bt, int round, int phase, int mode) {
+__global__ __launch_bounds__(SR_THREADS) void svd_round_kernel(SvdBatch Bt, int round, int phase, int mode) {
   __shared__ SvSmem sm;
   const SvdProblem& P = Bt.arr ? Bt.arr[blockIdx.y] : Bt.one;
   if (P.flag[1]) return;  // converged
@@ -756,7 +798,7 @@ int launch_svd_init(const SvdBatch& b, int64_t nprob, int64_t max_elems, int max
 
 int launch_svd_round(const SvdBatch& b, int64_t nprob, int max_items, int round, int phase, int mode,
                      hipStream_t s) {
-  hipLaunchKernelGGL(svd_round_kernel, dim3((unsigned)max_items, (unsigned)nprob), dim3(SV_THREADS), 0, s, b, round,
+  hipLaunchKernelGGL(svd_round_kernel, dim3((unsigned)max_items, (unsigned)nprob), dim3(SR_THREADS), 0, s, b, round,
                      phase, mode);
   TENET_HIP_CHECK(hipGetLastError());
   return TENET_OK;
