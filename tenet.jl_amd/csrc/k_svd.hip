@@ -59,14 +59,13 @@ struct SvSmem {
 __device__ __forceinline__ int pair_col(int l, int I, int J, int b) { return l < b ? I * b + l : J * b + (l - b); }
 
 // Stage rows [r0, r0+rows) of the pair's 2b columns of M (ld) into LDS.
-// All of a thread's loads (at most 64 x 32 / 256 = 8) are issued before any
-// is written to LDS, so the staging costs one memory latency, not eight.
+// All of a thread's loads (64 x 32 / 512 = 4) are issued before any is
+// written to LDS, so the staging costs one memory latency, not four.
 constexpr int STAGE_PER = RS * B2MAX / SR_THREADS;
-__device__ __forceinline__ void stage_rows(SvSmem& sm, const double* M, int64_t ld, int r0, int rows, int I, int J,
-                                           int b) {
+__device__ __forceinline__ void load_rows(double2 (&v)[STAGE_PER], const double* M, int64_t ld, int r0, int rows,
+                                          int I, int J, int b) {
   const int n2 = 2 * b;
   const double2* M2 = reinterpret_cast<const double2*>(M);
-  double2 v[STAGE_PER];
 #pragma unroll
   for (int u = 0; u < STAGE_PER; ++u) {
     const int e = threadIdx.x + u * SR_THREADS;
@@ -75,6 +74,9 @@ __device__ __forceinline__ void stage_rows(SvSmem& sm, const double* M, int64_t 
     v[u].y = 0.0;
     if (l < n2 && r < rows) v[u] = M2[(size_t)pair_col(l, I, J, b) * ld + r0 + r];
   }
+}
+
+__device__ __forceinline__ void store_rows(SvSmem& sm, const double2 (&v)[STAGE_PER], int n2) {
 #pragma unroll
   for (int u = 0; u < STAGE_PER; ++u) {
     const int e = threadIdx.x + u * SR_THREADS;
@@ -84,6 +86,13 @@ __device__ __forceinline__ void stage_rows(SvSmem& sm, const double* M, int64_t 
       sm.xi[r][l] = v[u].y;
     }
   }
+}
+
+__device__ __forceinline__ void stage_rows(SvSmem& sm, const double* M, int64_t ld, int r0, int rows, int I, int J,
+                                           int b) {
+  double2 v[STAGE_PER];
+  load_rows(v, M, ld, r0, rows, I, J, b);
+  store_rows(sm, v, 2 * b);
 }
 
 // acc += staged^H staged for this thread's 2x2 block of the Gram matrix:
@@ -189,10 +198,9 @@ __device__ void gram_over_rows(SvSmem& sm, const double* X, int64_t ld, int r0, 
 // buffer and passes them to its other lanes through its own LDS slots
 // (the same code on the same operands in every wave, so all waves hold the
 // same rotations; LDS operations of one wave complete in order, so no
-// barrier is needed for that exchange).  Threads 2j and 2j + 1 own the two
-// output rows of the block {p_k, q_k} x {p_l, q_l}, k = j % (n2/2),
-// l = j / (n2/2) (all (n2/2)^2 <= 256 ordered slot pairs, no mirror writes);
-// each thread also owns one W element (512 threads, n2 <= 32).
+// barrier is needed for that exchange).  G stays exactly Hermitian: only
+// the blocks above the diagonal are computed (their mirrors are written as
+// conjugates), and the diagonal 2x2 blocks take the closed-form update.
 template <bool CROSS>
 __device__ __forceinline__ void inner_pair(int k, int step, int n2, int& p, int& q) {
   if (CROSS) {
@@ -237,139 +245,163 @@ __device__ __forceinline__ double rcp_nr(double x) {
   return y;
 }
 
-// Per-thread ownership in the inner solve (fixed over the steps).
+// Per-thread ownership in the inner solve (fixed over the steps).  G is kept
+// Hermitian in full: waves 0-3 update the off-diagonal 2x2 blocks above the
+// diagonal (bk < bl, one output row per thread) and write their conjugate
+// mirrors; the rotation lanes of wave 0 write the diagonal blocks; waves 4-7
+// rotate W.  Lanes < npair of every wave form the step's rotations, so no
+// wave waits on another before its update.
+constexpr int GW_THREADS = SR_THREADS / 2;
 struct InnerOwn {
-  int bk, bl;  // G block {p_bk, q_bk} x {p_bl, q_bl}
+  int rslot;  // rotation slot of this lane (lanes >= npair repeat slot 0)
+  bool rown;  // lane < npair
+  bool gwave;
+  int bk, bl;  // G block {p_bk, q_bk} x {p_bl, q_bl}, bk < bl
   int half;    // output row of the block: 0 -> p_bk, 1 -> q_bk
-  bool blk;
-  int wrow[2], wslot[2];  // W elements (row, slot), wrow < 0: none
+  bool gown;
+  int wrow[2], wslot[2];  // W elements (row, slot)
+  bool wown[2];
 };
+
+// row h of B' = J_k^H B J_l for the 2x2 block B = [B0; B1] and the rotations
+// J = [[c, e], [-conj(e), c]]: row h of T = J_k^H B is a B0 + b B1 with
+// (a, b) = (c_k, -e_k) for h = 0 and (conj(e_k), c_k) for h = 1, and row h of
+// B' is (c_l T0 - conj(e_l) T1, e_l T0 + c_l T1).
+__device__ __forceinline__ void block_row(bool h1, double ck, double ekr, double eki, double cl, double elr,
+                                          double eli, const double (&b)[8], double& n0r, double& n0i, double& n1r,
+                                          double& n1i) {
+  const double a_r = h1 ? ekr : ck, a_i = h1 ? -eki : 0.0;
+  const double b_r = h1 ? ck : -ekr, b_i = h1 ? 0.0 : -eki;
+  // b = {B00 re, im, B01 re, im, B10 re, im, B11 re, im}
+  const double t0r = (a_r * b[0] - a_i * b[1]) + (b_r * b[4] - b_i * b[5]);
+  const double t0i = (a_r * b[1] + a_i * b[0]) + (b_r * b[5] + b_i * b[4]);
+  const double t1r = (a_r * b[2] - a_i * b[3]) + (b_r * b[6] - b_i * b[7]);
+  const double t1i = (a_r * b[3] + a_i * b[2]) + (b_r * b[7] + b_i * b[6]);
+  n0r = cl * t0r - (elr * t1r + eli * t1i);
+  n0i = cl * t0i - (elr * t1i - eli * t1r);
+  n1r = (elr * t0r - eli * t0i) + cl * t1r;
+  n1i = (elr * t0i + eli * t0r) + cl * t1i;
+}
 
 // One step; wave w = threadIdx.x / 64 publishes its rotations in rcs[w], rpq[w].
 // The G and W entries a thread touches depend only on the step's pairing,
-// so they are read before the rotations are formed (the rotation lanes'
-// own operands first).
+// so they are read before the rotations are formed.  Every thread reads
+// valid entries (unowned units repeat unit 0); only owners store.
 template <bool CROSS>
 __device__ __forceinline__ void inner_step(SvSmem& sm, const GramRow* cr, const GramRow* ci, GramRow* nr, GramRow* ni,
                                            int step, int n2, const InnerOwn& o, double tol2, double abs2,
                                            double& rmax2, int& any) {
-  const int tid = threadIdx.x, w = tid >> 6, ln = tid & 63;
-  const int npair = n2 / 2;
-  int rp = 0, rq = 0;
-  double ra = 0.0, rd = 0.0, rxr = 0.0, rxi = 0.0;
-  if (ln < npair) {
-    inner_pair<CROSS>(ln, step, n2, rp, rq);
-    ra = cr[rp][rp];
-    rd = cr[rq][rq];
-    rxr = cr[rp][rq];
-    rxi = ci[rp][rq];
-  }
+  const int w = threadIdx.x >> 6;
+  int rp, rq;
+  inner_pair<CROSS>(o.rslot, step, n2, rp, rq);
+  const double ra = cr[rp][rp], rd = cr[rq][rq], rxr = cr[rp][rq], rxi = ci[rp][rq];
   int p0 = 0, q0 = 0, p1 = 0, q1 = 0;
-  double b00r = 0, b00i = 0, b01r = 0, b01i = 0, b10r = 0, b10i = 0, b11r = 0, b11i = 0;
-  if (o.blk) {
+  double b[8];
+  int wp[2], wq[2];
+  double wv[2][4];
+  if (o.gwave) {
     inner_pair<CROSS>(o.bk, step, n2, p0, q0);
     inner_pair<CROSS>(o.bl, step, n2, p1, q1);
-    // B = G[{p0,q0}][{p1,q1}]
-    b00r = cr[p0][p1];
-    b00i = ci[p0][p1];
-    b01r = cr[p0][q1];
-    b01i = ci[p0][q1];
-    b10r = cr[q0][p1];
-    b10i = ci[q0][p1];
-    b11r = cr[q0][q1];
-    b11i = ci[q0][q1];
-  }
-  int wp[2], wq[2];
-  double wpr[2], wpi[2], wqr[2], wqi[2];
+    b[0] = cr[p0][p1];
+    b[1] = ci[p0][p1];
+    b[2] = cr[p0][q1];
+    b[3] = ci[p0][q1];
+    b[4] = cr[q0][p1];
+    b[5] = ci[q0][p1];
+    b[6] = cr[q0][q1];
+    b[7] = ci[q0][q1];
+  } else {
 #pragma unroll
-  for (int u = 0; u < 2; ++u) {
-    wp[u] = wq[u] = 0;
-    wpr[u] = wpi[u] = wqr[u] = wqi[u] = 0.0;
-    if (o.wrow[u] >= 0) {
+    for (int u = 0; u < 2; ++u) {
       inner_pair<CROSS>(o.wslot[u], step, n2, wp[u], wq[u]);
-      wpr[u] = sm.wr[o.wrow[u]][wp[u]];
-      wpi[u] = sm.wi[o.wrow[u]][wp[u]];
-      wqr[u] = sm.wr[o.wrow[u]][wq[u]];
-      wqi[u] = sm.wi[o.wrow[u]][wq[u]];
+      wv[u][0] = sm.wr[o.wrow[u]][wp[u]];
+      wv[u][1] = sm.wi[o.wrow[u]][wp[u]];
+      wv[u][2] = sm.wr[o.wrow[u]][wq[u]];
+      wv[u][3] = sm.wi[o.wrow[u]][wq[u]];
     }
   }
-  // Rotation of slot ln: zeroes G[p][q] of [[a, g], [conj(g), d]].  With
+  // Rotation of slot rslot: zeroes G[p][q] of [[a, g], [conj(g), d]].  With
   // delta = (d - a) / 2 and r = sqrt(delta^2 + |g|^2) the angle has
   // cos 2theta = |delta| / r and sin 2theta = |g| / r, so c = cos theta =
   // sqrt((1 + |delta| / r) / 2) and s = sign(delta) g / (2 r c) (the
   // smaller rotation of Rutishauser's formulas, tan theta = |g| / (|delta| + r)).
+  // The new diagonal is a - sign(delta) t |g|, d + sign(delta) t |g| with
+  // t |g| = |g|^2 / (|delta| + r) = |g|^2 / (2 r c^2).
   // Identity when |g| <= tol sqrt(a d) or |g| <= abs_tol (compared squared).
-  if (ln < npair) {
-    const double g2 = rxr * rxr + rxi * rxi;
-    const double ad = fmax(ra, 0.0) * fmax(rd, 0.0);
-    const int act = (g2 > tol2 * ad && g2 > abs2) ? 1 : 0;
-    double c = 1.0, sr = 0.0, si = 0.0;
-    if (act) {
-      const double dl = 0.5 * (rd - ra);
-      const double ir = rsqrt_nr(fma(dl, dl, g2));  // 1 / r
-      const double v = fma(0.5 * fabs(dl), ir, 0.5);  // c^2
-      const double y = rsqrt_nr(v);                   // 1 / c
-      c = v * y;
-      const double sn = 0.5 * ir * y;
-      sr = dl >= 0.0 ? sn * rxr : -(sn * rxr);
-      si = dl >= 0.0 ? sn * rxi : -(sn * rxi);
-    }
-    if (w == 0) {  // squared relative off-diagonal, for the convergence test
+  const double g2 = rxr * rxr + rxi * rxi;
+  const double ad = fmax(ra, 0.0) * fmax(rd, 0.0);
+  const int act = (g2 > tol2 * ad && g2 > abs2) ? 1 : 0;
+  double c = 1.0, sr = 0.0, si = 0.0, an = ra, dn = rd;
+  if (act) {
+    const double dl = 0.5 * (rd - ra);
+    const double ir = rsqrt_nr(fma(dl, dl, g2));    // 1 / r
+    const double v = fma(0.5 * fabs(dl), ir, 0.5);  // c^2
+    const double y = rsqrt_nr(v);                   // 1 / c
+    c = v * y;
+    const double sn = 0.5 * ir * y;
+    const double tg = (0.5 * g2) * (ir * (y * y));
+    const bool pos = dl >= 0.0;
+    sr = pos ? sn * rxr : -(sn * rxr);
+    si = pos ? sn * rxi : -(sn * rxi);
+    an = pos ? ra - tg : ra + tg;
+    dn = pos ? rd + tg : rd - tg;
+  }
+  if (o.rown) {
+    sm.rcs[w][o.rslot][0] = c;
+    sm.rcs[w][o.rslot][1] = sr;
+    sm.rcs[w][o.rslot][2] = si;
+    sm.rpq[w][o.rslot] = act;
+    if (w == 0) {
+      // squared relative off-diagonal, for the convergence test
       rmax2 = fmax(rmax2, ad > 0.0 ? g2 * rcp_nr(ad) : 0.0);
       any |= act;
+      // the pair's diagonal block: real diagonal, off-diagonal zeroed when rotated
+      const double gr = act ? 0.0 : rxr, gi = act ? 0.0 : rxi;
+      nr[rp][rp] = an;
+      ni[rp][rp] = 0.0;
+      nr[rq][rq] = dn;
+      ni[rq][rq] = 0.0;
+      nr[rp][rq] = gr;
+      ni[rp][rq] = gi;
+      nr[rq][rp] = gr;
+      ni[rq][rp] = -gi;
     }
-    sm.rcs[w][ln][0] = c;
-    sm.rcs[w][ln][1] = sr;
-    sm.rcs[w][ln][2] = si;
-    sm.rpq[w][ln] = act;
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  if (o.blk) {
-    const int bk = o.bk, bl = o.bl;
-    const double ck = sm.rcs[w][bk][0], ekr = sm.rcs[w][bk][1], eki = sm.rcs[w][bk][2];
-    const double cl = sm.rcs[w][bl][0], elr = sm.rcs[w][bl][1], eli = sm.rcs[w][bl][2];
-    // row h of T = J_k^H B: T0 = c B0 - e B1 ; T1 = conj(e) B0 + c B1, i.e.
-    // T_h = a B0 + b B1 with (a, b) = (c, -e) for h = 0 and (conj(e), c) for h = 1
-    const bool h1 = o.half != 0;
-    const double a_r = h1 ? ekr : ck, a_i = h1 ? -eki : 0.0;
-    const double b_r = h1 ? ck : -ekr, b_i = h1 ? 0.0 : -eki;
-    const double t0r = (a_r * b00r - a_i * b00i) + (b_r * b10r - b_i * b10i);
-    const double t0i = (a_r * b00i + a_i * b00r) + (b_r * b10i + b_i * b10r);
-    const double t1r = (a_r * b01r - a_i * b01i) + (b_r * b11r - b_i * b11i);
-    const double t1i = (a_r * b01i + a_i * b01r) + (b_r * b11i + b_i * b11r);
-    // row h of B' = T J_l: col0 = c T0 - conj(e) T1 ; col1 = e T0 + c T1
-    double n0r = cl * t0r - (elr * t1r + eli * t1i), n0i = cl * t0i - (elr * t1i - eli * t1r);
-    double n1r = (elr * t0r - eli * t0i) + cl * t1r, n1i = (elr * t0i + eli * t0r) + cl * t1i;
-    if (bk == bl) {  // diagonal block: exactly diagonal and real after the rotation
-      if (h1) {
-        n1i = 0.0;
-        if (sm.rpq[w][bk]) n0r = n0i = 0.0;
-      } else {
-        n0i = 0.0;
-        if (sm.rpq[w][bk]) n1r = n1i = 0.0;
-      }
+  if (o.gwave) {
+    if (o.gown) {
+      const int bk = o.bk, bl = o.bl;
+      const double ck = sm.rcs[w][bk][0], ekr = sm.rcs[w][bk][1], eki = sm.rcs[w][bk][2];
+      const double cl = sm.rcs[w][bl][0], elr = sm.rcs[w][bl][1], eli = sm.rcs[w][bl][2];
+      const bool h1 = o.half != 0;
+      double n0r, n0i, n1r, n1i;
+      block_row(h1, ck, ekr, eki, cl, elr, eli, b, n0r, n0i, n1r, n1i);
+      const int ph = h1 ? q0 : p0;
+      nr[ph][p1] = n0r;
+      ni[ph][p1] = n0i;
+      nr[ph][q1] = n1r;
+      ni[ph][q1] = n1i;
+      nr[p1][ph] = n0r;
+      ni[p1][ph] = -n0i;
+      nr[q1][ph] = n1r;
+      ni[q1][ph] = -n1i;
     }
-    const int ph = h1 ? q0 : p0;
-    nr[ph][p1] = n0r;
-    ni[ph][p1] = n0i;
-    nr[ph][q1] = n1r;
-    ni[ph][q1] = n1i;
-  }
-  // W <- W J on this thread's elements
+  } else {
+    // W <- W J on this thread's elements
 #pragma unroll
-  for (int u = 0; u < 2; ++u) {
-    if (o.wrow[u] < 0) continue;
-    const int k = o.wslot[u];
-    if (!sm.rpq[w][k]) continue;
-    const double c = sm.rcs[w][k][0], er = sm.rcs[w][k][1], ei = sm.rcs[w][k][2];
-    const double pr = wpr[u], pi = wpi[u], qr = wqr[u], qi = wqi[u];
-    const int row = o.wrow[u];
-    sm.wr[row][wp[u]] = c * pr - (er * qr + ei * qi);
-    sm.wi[row][wp[u]] = c * pi - (er * qi - ei * qr);
-    sm.wr[row][wq[u]] = (er * pr - ei * pi) + c * qr;
-    sm.wi[row][wq[u]] = (er * pi + ei * pr) + c * qi;
+    for (int u = 0; u < 2; ++u) {
+      const int k = o.wslot[u];
+      if (!o.wown[u] || !sm.rpq[w][k]) continue;
+      const double cw = sm.rcs[w][k][0], er = sm.rcs[w][k][1], ei = sm.rcs[w][k][2];
+      const double pr = wv[u][0], pi = wv[u][1], qr = wv[u][2], qi = wv[u][3];
+      const int row = o.wrow[u];
+      sm.wr[row][wp[u]] = cw * pr - (er * qr + ei * qi);
+      sm.wi[row][wp[u]] = cw * pi - (er * qi - ei * qr);
+      sm.wr[row][wq[u]] = (er * pr - ei * pi) + cw * qr;
+      sm.wi[row][wq[u]] = (er * pi + ei * pr) + cw * qi;
+    }
   }
   __syncthreads();
 }
@@ -384,21 +416,34 @@ __device__ void jacobi_inner(SvSmem& sm, int n2, double tol, double abs_tol) {
     sm.wi[i][c] = 0.0;
   }
   if (tid == 0) sm.any_outer = 0;
-  // every ordered slot pair (k, l) is one block (all npair^2 <= 256 blocks,
-  // each written by its owners, no mirror writes); threads 2j and 2j + 1
-  // compute its two output rows
   InnerOwn o;
-  const int j = tid >> 1;
-  o.half = tid & 1;
-  o.blk = j < npair * npair;
-  o.bk = j % npair;
-  o.bl = j / npair;
-  // W element e = tid + 512 u: slot e % npair, row e / npair (npair n2 <= 512)
+  const int ln = tid & 63;
+  o.rown = ln < npair;
+  o.rslot = o.rown ? ln : 0;
+  o.gwave = tid < GW_THREADS;
+  // G: the npair (npair - 1) / 2 blocks bk < bl in row-major order, two
+  // threads (output rows) each: at most 240 threads
+  {
+    const int nblk = npair * (npair - 1) / 2;
+    const int u = tid >> 1;
+    o.half = tid & 1;
+    o.gown = o.gwave && u < nblk;
+    int rem = o.gown ? u : 0, k = 0;
+    while (k < npair - 1 && rem >= npair - 1 - k) {
+      rem -= npair - 1 - k;
+      ++k;
+    }
+    o.bk = k;
+    o.bl = min(k + 1 + rem, npair - 1);
+  }
+  // W: element e = (tid - 256) + 256 u is (row e / npair, slot e % npair); npair n2 <= 512
 #pragma unroll
   for (int u = 0; u < 2; ++u) {
-    const int e = tid + SR_THREADS * u;
-    o.wrow[u] = e < npair * n2 ? e / npair : -1;
-    o.wslot[u] = e % npair;
+    const int e = (tid - GW_THREADS) + GW_THREADS * u;
+    o.wown[u] = !o.gwave && e < npair * n2;
+    const int ec = o.wown[u] ? e : 0;
+    o.wrow[u] = ec / npair;
+    o.wslot[u] = ec % npair;
   }
   const double tol2 = tol * tol, abs2 = abs_tol * abs_tol;
   double rmax2 = 0.0;
@@ -415,40 +460,47 @@ __device__ void jacobi_inner(SvSmem& sm, int n2, double tol, double abs_tol) {
   __syncthreads();
 }
 
-// rows [r0, r1) of the pair's columns of M <- M W.  Thread t: row t % 64 of
-// the staged chunk and 4 consecutive output columns (group t / 64).
+// The staged rows [rb, rb + rows) of the pair's columns of M <- M W.  Thread
+// t: row t % 64 of the staged chunk and 4 consecutive output columns
+// (group t / 64).
 constexpr int APPLY_COLS = SR_THREADS / RS > 0 ? B2MAX / (SR_THREADS / RS) : B2MAX;
-__device__ void apply_rows(SvSmem& sm, double* M, int64_t ld, int r0, int r1, int I, int J, int b) {
+__device__ __forceinline__ void apply_staged(SvSmem& sm, double* M, int64_t ld, int rb, int rows, int I, int J,
+                                             int b) {
   const int n2 = 2 * b;
   const int r = threadIdx.x % RS, cg = APPLY_COLS * (threadIdx.x / RS);
+  if (r < rows && cg < n2) {
+    double sr[APPLY_COLS], si[APPLY_COLS];
+#pragma unroll
+    for (int j = 0; j < APPLY_COLS; ++j) sr[j] = si[j] = 0.0;
+    for (int t = 0; t < n2; ++t) {
+      const double xr = sm.xr[r][t], xi = sm.xi[r][t];
+#pragma unroll
+      for (int j = 0; j < APPLY_COLS; ++j) {
+        const double wr = sm.wr[t][cg + j], wi = sm.wi[t][cg + j];
+        sr[j] += xr * wr - xi * wi;
+        si[j] += xr * wi + xi * wr;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < APPLY_COLS; ++j) {
+      if (cg + j < n2) {
+        double2 v;
+        v.x = sr[j];
+        v.y = si[j];
+        reinterpret_cast<double2*>(M)[(size_t)pair_col(cg + j, I, J, b) * ld + rb + r] = v;
+      }
+    }
+  }
+}
+
+// rows [r0, r1) of the pair's columns of M <- M W, one staged chunk at a time
+__device__ void apply_rows(SvSmem& sm, double* M, int64_t ld, int r0, int r1, int I, int J, int b) {
   for (int rb = r0; rb < r1; rb += RS) {
     const int rows = min(RS, r1 - rb);
     __syncthreads();
     stage_rows(sm, M, ld, rb, rows, I, J, b);
     __syncthreads();
-    if (r < rows && cg < n2) {
-      double sr[APPLY_COLS], si[APPLY_COLS];
-#pragma unroll
-      for (int j = 0; j < APPLY_COLS; ++j) sr[j] = si[j] = 0.0;
-      for (int t = 0; t < n2; ++t) {
-        const double xr = sm.xr[r][t], xi = sm.xi[r][t];
-#pragma unroll
-        for (int j = 0; j < APPLY_COLS; ++j) {
-          const double wr = sm.wr[t][cg + j], wi = sm.wi[t][cg + j];
-          sr[j] += xr * wr - xi * wi;
-          si[j] += xr * wi + xi * wr;
-        }
-      }
-#pragma unroll
-      for (int j = 0; j < APPLY_COLS; ++j) {
-        if (cg + j < n2) {
-          double2 v;
-          v.x = sr[j];
-          v.y = si[j];
-          reinterpret_cast<double2*>(M)[(size_t)pair_col(cg + j, I, J, b) * ld + rb + r] = v;
-        }
-      }
-    }
+    apply_staged(sm, M, ld, rb, rows, I, J, b);
   }
   __syncthreads();
 }
