@@ -86,9 +86,12 @@ struct TileAcc {
 // Accumulate the (tm, tn) tile of batch entry z over k in [k_lo, k_hi).
 // On return every wave holds its TM x TN blocks; for WGK > 1 the sum over
 // the K-splitting waves has been folded into wave 0 (the others hold zeros).
-template <typename S2, typename R, typename Cfg>
+// CONJ = false: the operands are never conjugated (p.conjA / p.conjB are
+// not read), so the loads carry no per-element sign select.
+template <typename S2, typename R, typename Cfg, bool CONJ = true>
 __device__ __forceinline__ void gemm_accumulate(const TileView& p, int64_t tm, int64_t tn, int64_t z, int64_t k_lo,
                                                 int64_t k_hi, GemmSmem<R>& sm, TileAcc<R, Cfg>& acc) {
+  const bool conjA = CONJ && p.conjA, conjB = CONJ && p.conjB;
   constexpr int BM = Cfg::BM, BN = Cfg::BN, NA = Cfg::NA, NB = Cfg::NB;
   const S2* __restrict__ A = static_cast<const S2*>(p.A);
   const S2* __restrict__ B = static_cast<const S2*>(p.B);
@@ -142,7 +145,7 @@ __device__ __forceinline__ void gemm_accumulate(const TileView& p, int64_t tm, i
       if (la_ok[j] && gka < k_hi) {
         const S2 v = A[la_fixed[j] + p.t.kA[gka]];
         ra_r[j] = (R)v.x;
-        ra_i[j] = p.conjA ? -(R)v.y : (R)v.y;
+        ra_i[j] = conjA ? -(R)v.y : (R)v.y;
       } else {
         ra_r[j] = 0;
         ra_i[j] = 0;
@@ -154,7 +157,7 @@ __device__ __forceinline__ void gemm_accumulate(const TileView& p, int64_t tm, i
       if (lb_ok[j] && gkb < k_hi) {
         const S2 v = B[lb_fixed[j] + p.t.kB[gkb]];
         rb_r[j] = (R)v.x;
-        rb_i[j] = p.conjB ? -(R)v.y : (R)v.y;
+        rb_i[j] = conjB ? -(R)v.y : (R)v.y;
       } else {
         rb_r[j] = 0;
         rb_i[j] = 0;
@@ -464,7 +467,10 @@ __device__ __forceinline__ void grouped_unit(const GroupedProblem& q, const uint
   const int64_t k_lo = (int64_t)u.chunk * q.kc;
   const int64_t k_hi = std::min<int64_t>(q.K, k_lo + q.kc);
   TileAcc<R, Cfg> acc;
-  gemm_accumulate<S2, R, Cfg>(p, tm, tn, z, k_lo, k_hi, sm, acc);
+  if (q.conjA || q.conjB)
+    gemm_accumulate<S2, R, Cfg, true>(p, tm, tn, z, k_lo, k_hi, sm, acc);
+  else
+    gemm_accumulate<S2, R, Cfg, false>(p, tm, tn, z, k_lo, k_hi, sm, acc);
   if (q.kchunks == 1) {
     gemm_store<S2, R, Cfg>(p, tm, tn, z, acc);
   } else {
