@@ -1,8 +1,10 @@
 """ctypes binding of libtenet_hip.so (declarations mirror include/tenet_hip.h).
 
-The library is loaded from the package tree (``tenet.jl_amd/lib``).  There
-is no CPU fallback: a missing library raises at import of this module, and
-creating a context on a host without a gfx950 device raises TenetError.
+The library is loaded from the package tree (``tenet.jl_amd/lib``), or from
+the path in the environment variable TENET_HIP_LIB (another build of the
+same sources, e.g. ``make OUT=...``).  There is no CPU fallback: a missing
+library raises at import of this module, and creating a context on a host
+without a gfx950 device raises TenetError.
 """
 from __future__ import annotations
 
@@ -10,7 +12,7 @@ import ctypes as C
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.normpath(os.path.join(_HERE, "..", "lib", "libtenet_hip.so"))
+LIB_PATH = os.environ.get("TENET_HIP_LIB") or os.path.normpath(os.path.join(_HERE, "..", "lib", "libtenet_hip.so"))
 
 TENET_OK = 0
 TENET_ERR_HIP = 1
