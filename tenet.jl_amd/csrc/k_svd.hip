@@ -245,6 +245,49 @@ __device__ __forceinline__ double rcp_nr(double x) {
   return y;
 }
 
+// Rotation zeroing G[p][q] of [[a, g], [conj(g), d]].  With
+// delta = (d - a) / 2 and r = sqrt(delta^2 + |g|^2) the angle has
+// cos 2theta = |delta| / r and sin 2theta = |g| / r, so c = cos theta =
+// sqrt((1 + |delta| / r) / 2) and s = sign(delta) g / (2 r c) (the
+// smaller rotation of Rutishauser's formulas, tan theta = |g| / (|delta| + r)).
+// The new diagonal is a - sign(delta) t |g|, d + sign(delta) t |g| with
+// t |g| = |g|^2 / (|delta| + r) = |g|^2 / (2 r c^2).
+// Identity when |g| <= tol sqrt(a d) or |g| <= abs_tol (compared squared).
+struct Rot {
+  double c, sr, si;  // J = [[c, s], [-conj(s), c]]
+  double an, dn;     // new diagonal
+  double rel2;       // |g|^2 / (a d), for the convergence test
+  int act;
+};
+
+__device__ __forceinline__ Rot form_rotation(double ra, double rd, double rxr, double rxi, double tol2, double abs2) {
+  Rot o;
+  const double g2 = rxr * rxr + rxi * rxi;
+  const double ad = fmax(ra, 0.0) * fmax(rd, 0.0);
+  o.act = (g2 > tol2 * ad && g2 > abs2) ? 1 : 0;
+  o.c = 1.0;
+  o.sr = 0.0;
+  o.si = 0.0;
+  o.an = ra;
+  o.dn = rd;
+  if (o.act) {
+    const double dl = 0.5 * (rd - ra);
+    const double ir = rsqrt_nr(fma(dl, dl, g2));    // 1 / r
+    const double v = fma(0.5 * fabs(dl), ir, 0.5);  // c^2
+    const double y = rsqrt_nr(v);                   // 1 / c
+    o.c = v * y;
+    const double sn = 0.5 * ir * y;
+    const double tg = (0.5 * g2) * (ir * (y * y));
+    const bool pos = dl >= 0.0;
+    o.sr = pos ? sn * rxr : -(sn * rxr);
+    o.si = pos ? sn * rxi : -(sn * rxi);
+    o.an = pos ? ra - tg : ra + tg;
+    o.dn = pos ? rd + tg : rd - tg;
+  }
+  o.rel2 = ad > 0.0 ? g2 * rcp_nr(ad) : 0.0;
+  return o;
+}
+
 // Per-thread ownership in the inner solve (fixed over the steps).  G is kept
 // Hermitian in full: waves 0-3 update the off-diagonal 2x2 blocks above the
 // diagonal (bk < bl, one output row per thread) and write their conjugate
@@ -320,32 +363,9 @@ __device__ __forceinline__ void inner_step(SvSmem& sm, const GramRow* cr, const 
       wv[u][3] = sm.wi[o.wrow[u]][wq[u]];
     }
   }
-  // Rotation of slot rslot: zeroes G[p][q] of [[a, g], [conj(g), d]].  With
-  // delta = (d - a) / 2 and r = sqrt(delta^2 + |g|^2) the angle has
-  // cos 2theta = |delta| / r and sin 2theta = |g| / r, so c = cos theta =
-  // sqrt((1 + |delta| / r) / 2) and s = sign(delta) g / (2 r c) (the
-  // smaller rotation of Rutishauser's formulas, tan theta = |g| / (|delta| + r)).
-  // The new diagonal is a - sign(delta) t |g|, d + sign(delta) t |g| with
-  // t |g| = |g|^2 / (|delta| + r) = |g|^2 / (2 r c^2).
-  // Identity when |g| <= tol sqrt(a d) or |g| <= abs_tol (compared squared).
-  const double g2 = rxr * rxr + rxi * rxi;
-  const double ad = fmax(ra, 0.0) * fmax(rd, 0.0);
-  const int act = (g2 > tol2 * ad && g2 > abs2) ? 1 : 0;
-  double c = 1.0, sr = 0.0, si = 0.0, an = ra, dn = rd;
-  if (act) {
-    const double dl = 0.5 * (rd - ra);
-    const double ir = rsqrt_nr(fma(dl, dl, g2));    // 1 / r
-    const double v = fma(0.5 * fabs(dl), ir, 0.5);  // c^2
-    const double y = rsqrt_nr(v);                   // 1 / c
-    c = v * y;
-    const double sn = 0.5 * ir * y;
-    const double tg = (0.5 * g2) * (ir * (y * y));
-    const bool pos = dl >= 0.0;
-    sr = pos ? sn * rxr : -(sn * rxr);
-    si = pos ? sn * rxi : -(sn * rxi);
-    an = pos ? ra - tg : ra + tg;
-    dn = pos ? rd + tg : rd - tg;
-  }
+  const Rot rt = form_rotation(ra, rd, rxr, rxi, tol2, abs2);
+  const int act = rt.act;
+  const double c = rt.c, sr = rt.sr, si = rt.si, an = rt.an, dn = rt.dn;
   if (o.rown) {
     sm.rcs[w][o.rslot][0] = c;
     sm.rcs[w][o.rslot][1] = sr;
@@ -353,7 +373,7 @@ __device__ __forceinline__ void inner_step(SvSmem& sm, const GramRow* cr, const 
     sm.rpq[w][o.rslot] = act;
     if (w == 0) {
       // squared relative off-diagonal, for the convergence test
-      rmax2 = fmax(rmax2, ad > 0.0 ? g2 * rcp_nr(ad) : 0.0);
+      rmax2 = fmax(rmax2, rt.rel2);
       any |= act;
       // the pair's diagonal block: real diagonal, off-diagonal zeroed when rotated
       const double gr = act ? 0.0 : rxr, gi = act ? 0.0 : rxi;
@@ -407,7 +427,7 @@ __device__ __forceinline__ void inner_step(SvSmem& sm, const GramRow* cr, const 
 }
 
 template <bool CROSS>
-__device__ void jacobi_inner(SvSmem& sm, int n2, double tol, double abs_tol) {
+__device__ __forceinline__ void jacobi_inner(SvSmem& sm, int n2, double tol, double abs_tol) {
   const int tid = threadIdx.x;
   const int npair = n2 / 2;
   for (int e = tid; e < n2 * n2; e += SR_THREADS) {
@@ -457,6 +477,180 @@ __device__ void jacobi_inner(SvSmem& sm, int n2, double tol, double abs_tol) {
   }
   if (tid < npair) sm.rmax[tid] = rmax2;
   if (any) sm.any_outer = 1;
+  __syncthreads();
+}
+
+// Cross-block inner solve for 2b = 32 (every cross round: a matrix with more
+// than one block pair has b = 16) with the rotations formed one step ahead.
+// In the cross order p_k = k at every step and q_k(s + 1) = q_{k+1}(s)
+// (slots mod 16), so the Gram entries rotation k of step s + 1 needs are
+//   G'[k][k]       = the new diagonal a'_k of rotation k of step s,
+//   G'[q][q]       = the new diagonal d'_{k+1} of rotation k + 1 of step s,
+//   G'[k][q_{k+1}] = row p_k, column q_{k+1} of J_k^H B J_{k+1}, B the 2x2
+//                    block {k, q_k} x {k+1, q_{k+1}} of G before step s,
+// all known from G_s and step s's rotations.  Wave 7 (lanes 0-15: one slot
+// each) forms them and step s + 1's rotations while waves 0-3 update the G
+// blocks above the diagonal and waves 4-6 rotate W with step s's rotations;
+// the rotations pass through LDS slots double-buffered by step parity, so a
+// step still takes one workgroup barrier.  Wave 7 also writes the diagonal
+// blocks (closed form).
+constexpr int XB = 16;         // b
+constexpr int XROT_WAVE = 7;   // rotation wave
+constexpr int XW_THREADS = 192;  // W threads: waves 4-6, 512 elements, 3 each
+
+__device__ __forceinline__ void jacobi_cross32(SvSmem& sm, double tol, double abs_tol) {
+  const int tid = threadIdx.x, wave = tid >> 6, ln = tid & 63;
+  for (int e = tid; e < 2 * XB * 2 * XB; e += SR_THREADS) {
+    const int i = e % (2 * XB), c = e / (2 * XB);
+    sm.wr[i][c] = (i == c) ? 1.0 : 0.0;
+    sm.wi[i][c] = 0.0;
+  }
+  if (tid == 0) sm.any_outer = 0;
+  const double tol2 = tol * tol, abs2 = abs_tol * abs_tol;
+  // roles (wave-uniform)
+  const bool gwave = wave < 4, wwave = wave >= 4 && wave < XROT_WAVE, rwave = wave == XROT_WAVE;
+  // G: blocks bk < bl in row-major order, two threads (output rows) each
+  int bk = 0, bl = 1, half = tid & 1;
+  bool gown = false;
+  if (gwave) {
+    const int u = tid >> 1;
+    gown = u < XB * (XB - 1) / 2;
+    int rem = gown ? u : 0, k = 0;
+    while (k < XB - 1 && rem >= XB - 1 - k) {
+      rem -= XB - 1 - k;
+      ++k;
+    }
+    bk = k;
+    bl = k + 1 + rem;
+  }
+  // W: element e = (tid - 256) + 192 u is (row e / 16, slot e % 16)
+  int wrow[3], wslot[3];
+  bool wown[3];
+#pragma unroll
+  for (int u = 0; u < 3; ++u) {
+    const int e = (tid - 4 * 64) + XW_THREADS * u;
+    wown[u] = wwave && e < 2 * XB * XB;
+    const int ec = wown[u] ? e : 0;
+    wrow[u] = ec / XB;
+    wslot[u] = ec % XB;
+  }
+  // rotation wave: lane k < 16 owns slot k
+  const int k = ln & (XB - 1), kn = (k + 1) & (XB - 1);
+  const bool rown = rwave && ln < XB;
+  Rot rt;
+  double gr = 0.0, gi = 0.0;  // G[k][q_k] the current rotation was formed from
+  double rmax2 = 0.0;
+  int any = 0;
+  if (rwave) {
+    const int q = XB + k;
+    gr = sm.gr[k][q];
+    gi = sm.gi[k][q];
+    rt = form_rotation(sm.gr[k][k], sm.gr[q][q], gr, gi, tol2, abs2);
+    if (rown) {
+      sm.rcs[0][k][0] = rt.c;
+      sm.rcs[0][k][1] = rt.sr;
+      sm.rcs[0][k][2] = rt.si;
+      sm.rpq[0][k] = rt.act;
+      rmax2 = fmax(rmax2, rt.rel2);
+      any |= rt.act;
+    }
+  }
+  __syncthreads();
+#pragma unroll 2
+  for (int step = 0; step < XB; ++step) {
+    const int par = step & 1;
+    const GramRow* cr = par ? sm.hr : sm.gr;
+    const GramRow* ci = par ? sm.hi : sm.gi;
+    GramRow* nr = par ? sm.gr : sm.hr;
+    GramRow* ni = par ? sm.gi : sm.hi;
+    if (gwave) {
+      const int p0 = bk, q0 = XB + ((bk + step) & (XB - 1));
+      const int p1 = bl, q1 = XB + ((bl + step) & (XB - 1));
+      double b[8];
+      b[0] = cr[p0][p1];
+      b[1] = ci[p0][p1];
+      b[2] = cr[p0][q1];
+      b[3] = ci[p0][q1];
+      b[4] = cr[q0][p1];
+      b[5] = ci[q0][p1];
+      b[6] = cr[q0][q1];
+      b[7] = ci[q0][q1];
+      const double ck = sm.rcs[par][bk][0], ekr = sm.rcs[par][bk][1], eki = sm.rcs[par][bk][2];
+      const double cl = sm.rcs[par][bl][0], elr = sm.rcs[par][bl][1], eli = sm.rcs[par][bl][2];
+      if (gown) {
+        const bool h1 = half != 0;
+        double n0r, n0i, n1r, n1i;
+        block_row(h1, ck, ekr, eki, cl, elr, eli, b, n0r, n0i, n1r, n1i);
+        const int ph = h1 ? q0 : p0;
+        nr[ph][p1] = n0r;
+        ni[ph][p1] = n0i;
+        nr[ph][q1] = n1r;
+        ni[ph][q1] = n1i;
+        nr[p1][ph] = n0r;
+        ni[p1][ph] = -n0i;
+        nr[q1][ph] = n1r;
+        ni[q1][ph] = -n1i;
+      }
+    } else if (wwave) {
+#pragma unroll
+      for (int u = 0; u < 3; ++u) {
+        const int sl = wslot[u], row = wrow[u];
+        const int wp = sl, wq = XB + ((sl + step) & (XB - 1));
+        const double pr = sm.wr[row][wp], pi = sm.wi[row][wp], qr = sm.wr[row][wq], qi = sm.wi[row][wq];
+        if (!wown[u] || !sm.rpq[par][sl]) continue;
+        const double c = sm.rcs[par][sl][0], er = sm.rcs[par][sl][1], ei = sm.rcs[par][sl][2];
+        sm.wr[row][wp] = c * pr - (er * qr + ei * qi);
+        sm.wi[row][wp] = c * pi - (er * qi - ei * qr);
+        sm.wr[row][wq] = (er * pr - ei * pi) + c * qr;
+        sm.wi[row][wq] = (er * pi + ei * pr) + c * qi;
+      }
+    } else {
+      // rotation wave: this step's diagonal blocks, then the next step's rotations
+      const int q = XB + ((k + step) & (XB - 1));
+      const int qn = XB + ((kn + step) & (XB - 1));
+      double b[8];
+      b[0] = cr[k][kn];
+      b[1] = ci[k][kn];
+      b[2] = cr[k][qn];
+      b[3] = ci[k][qn];
+      b[4] = cr[q][kn];
+      b[5] = ci[q][kn];
+      b[6] = cr[q][qn];
+      b[7] = ci[q][qn];
+      if (rown) {
+        const double ogr = rt.act ? 0.0 : gr, ogi = rt.act ? 0.0 : gi;
+        nr[k][k] = rt.an;
+        ni[k][k] = 0.0;
+        nr[q][q] = rt.dn;
+        ni[q][q] = 0.0;
+        nr[k][q] = ogr;
+        ni[k][q] = ogi;
+        nr[q][k] = ogr;
+        ni[q][k] = -ogi;
+      }
+      if (step + 1 < XB) {
+        // slot k + 1's rotation and new diagonal d' (lanes 0-15 hold slots 0-15)
+        const double cn = __shfl(rt.c, kn), srn = __shfl(rt.sr, kn), sin_ = __shfl(rt.si, kn);
+        const double dnn = __shfl(rt.dn, kn);
+        double n0r, n0i, n1r, n1i;
+        block_row(false, rt.c, rt.sr, rt.si, cn, srn, sin_, b, n0r, n0i, n1r, n1i);
+        gr = n1r;
+        gi = n1i;
+        rt = form_rotation(rt.an, dnn, gr, gi, tol2, abs2);
+        if (rown) {
+          sm.rcs[par ^ 1][k][0] = rt.c;
+          sm.rcs[par ^ 1][k][1] = rt.sr;
+          sm.rcs[par ^ 1][k][2] = rt.si;
+          sm.rpq[par ^ 1][k] = rt.act;
+          rmax2 = fmax(rmax2, rt.rel2);
+          any |= rt.act;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (rown) sm.rmax[k] = rmax2;
+  if (rown && any) sm.any_outer = 1;
   __syncthreads();
 }
 
@@ -512,7 +706,9 @@ __global__ __launch_bounds__(SR_THREADS) void svd_round_kernel(SvdBatch Bt, int 
   const SvdProblem& P = Bt.arr ? Bt.arr[blockIdx.y] : Bt.one;
   if (P.flag[1]) return;  // converged
   const int nb = P.nblk;
-  if (mode == 1 && round >= nb - 1) return;
+  // a single block pair (nb = 2) is swept whole in the mode-0 round; cross
+  // rounds exist only for nb > 2, where every block is b = 16 wide
+  if (mode == 1 && (nb <= 2 || round >= nb - 1)) return;
   const int q = P.q;
   const int item = blockIdx.x;
   const int p = item / q, c = item % q;
@@ -602,7 +798,7 @@ __global__ __launch_bounds__(SR_THREADS) void svd_round_kernel(SvdBatch Bt, int 
   // that give one-sided Jacobi its accuracy on small singular values.
   const double abs_tol = fmax(4.930380657631324e-32 * (*P.fro2), 1e-300);
   if (mode == 1)
-    jacobi_inner<true>(sm, n2, P.tol, abs_tol);
+    jacobi_cross32(sm, P.tol, abs_tol);
   else
     jacobi_inner<false>(sm, n2, P.tol, abs_tol);
   // largest squared relative off-diagonal of this pair visit -> the sweep's maximum
