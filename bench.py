@@ -33,8 +33,8 @@ FIXTURE = os.path.join(ROOT, "tests", "golden", "tebd_n64_d20_chi256.npz")
 VIDAL_FIXTURE = os.path.join(ROOT, "tests", "golden", "tebd_vidal_n64_d20_chi256.npz")
 PEAK_FP64_TFLOPS = 78.6
 # HBM bytes per svd_round_kernel dispatch over one TEBD sweep (n=64, chi_max=256), from two counter
-# passes: FETCH_SIZE 6301.1 KiB x 2.0 + WRITE_SIZE 3985.6 KiB x 1.0 = 16587.8 KiB.
-SVD_ROUND_TRAFFIC_BYTES = 16_985_907
+# passes: FETCH_SIZE 6311.0 KiB x 2.0 + WRITE_SIZE 3985.0 KiB x 1.0 = 16607.0 KiB.
+SVD_ROUND_TRAFFIC_BYTES = 17_005_568
 
 FORMS = {"fp64": 0, "fp32": 1, "split_bf16": 2, "split_fp16": 3}
 

@@ -128,8 +128,9 @@ struct SvdBatch {
   SvdProblem one;
 };
 int launch_svd_init(const SvdBatch& b, int64_t nprob, int64_t max_elems, int max_ncp, int dtype, hipStream_t s);
+// ev0 / ev1 (optional): stamped at the kernel's start and end by the dispatch
 int launch_svd_round(const SvdBatch& b, int64_t nprob, int max_items, int round, int phase, int mode,
-                     hipStream_t s);
+                     hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 int launch_svd_sweep_end(const SvdBatch& b, int64_t nprob, hipStream_t s);
 int launch_svd_finalize(const SvdBatch& b, int64_t nprob, int max_ncp, int64_t max_elems, int dtype,
                         hipStream_t s);

@@ -15,6 +15,8 @@
 // per row chunk; phase 1: every workgroup of the pair sums the q partials in
 // a fixed order, solves the same 2b x 2b problem, and rotates its own row
 // chunk).  With q = 1 the two phases fuse (phase 2).
+#include <hip/hip_ext.h>
+
 #include "../../include/tenet_hip.h"
 #include "internal.h"
 
@@ -1045,9 +1047,12 @@ int launch_svd_init(const SvdBatch& b, int64_t nprob, int64_t max_elems, int max
 }
 
 int launch_svd_round(const SvdBatch& b, int64_t nprob, int max_items, int round, int phase, int mode,
-                     hipStream_t s) {
-  hipLaunchKernelGGL(svd_round_kernel, dim3((unsigned)max_items, (unsigned)nprob), dim3(SR_THREADS), 0, s, b, round,
-                     phase, mode);
+                     hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
+  const dim3 grid((unsigned)max_items, (unsigned)nprob);
+  if (ev0)
+    hipExtLaunchKernelGGL(svd_round_kernel, grid, dim3(SR_THREADS), 0, s, ev0, ev1, 0, b, round, phase, mode);
+  else
+    hipLaunchKernelGGL(svd_round_kernel, grid, dim3(SR_THREADS), 0, s, b, round, phase, mode);
   TENET_HIP_CHECK(hipGetLastError());
   return TENET_OK;
 }

@@ -107,7 +107,12 @@ struct StatScope {
   tenet_ctx ctx;
   hipStream_t s;
   bool on = false;
-  StatScope(tenet_ctx c, hipStream_t st, int family, double flops, double bytes) : ctx(c), s(st) {
+  // launch_records: the sampled pair is handed to the launch
+  // (start_event / stop_event), which stamps it in the dispatch itself
+  // (hipExtLaunchKernelGGL), so the sample is the kernel's own duration
+  bool by_launch = false, recorded = false;
+  StatScope(tenet_ctx c, hipStream_t st, int family, double flops, double bytes, bool launch_records = false)
+      : ctx(c), s(st) {
     if (!ctx || ctx->stat_family != family || family == 0) return;
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     if (hipStreamIsCapturing(s, &cs) == hipSuccess && cs == hipStreamCaptureStatusActive) return;
@@ -126,15 +131,39 @@ struct StatScope {
         return;
       }
     }
-    if (hipEventRecord(ctx->stat_events[ctx->stat_used], s) != hipSuccess) return;
+    if (launch_records) {
+      by_launch = true;
+    } else if (hipEventRecord(ctx->stat_events[ctx->stat_used], s) != hipSuccess) {
+      return;
+    }
     on = true;
     ctx->stat_sampled += 1;
   }
+  hipEvent_t start_event() const { return on && by_launch ? ctx->stat_events[ctx->stat_used] : nullptr; }
+  hipEvent_t stop_event() const { return on && by_launch ? ctx->stat_events[ctx->stat_used + 1] : nullptr; }
+  // the launch with start_event / stop_event succeeded
+  void launched() { recorded = true; }
   ~StatScope() {
     if (!on) return;
+    if (by_launch) {
+      if (recorded)
+        ctx->stat_used += 2;
+      else
+        ctx->stat_sampled -= 1;
+      return;
+    }
     if (hipEventRecord(ctx->stat_events[ctx->stat_used + 1], s) == hipSuccess) ctx->stat_used += 2;
   }
 };
+
+// launch_svd_round with its timing sample stamped by the dispatch
+static int svd_round_timed(tenet_ctx ctx, hipStream_t s, double flops, double bytes, const SvdBatch& B, int64_t nprob,
+                           int max_items, int round, int phase, int mode) {
+  StatScope st(ctx, s, TENET_KSTAT_SVD_ROUND, flops, bytes, true);
+  const int rc = launch_svd_round(B, nprob, max_items, round, phase, mode, s, st.start_event(), st.stop_event());
+  if (rc == TENET_OK) st.launched();
+  return rc;
+}
 
 static const size_t ELEM[2] = {16, 8};
 
@@ -1018,16 +1047,11 @@ static int svd_run(tenet_ctx ctx, hipStream_t s, int dtype, int64_t m, int64_t n
   const double apply_fl = 8.0 * sh.npairs * n2 * n2 * (sh.nr + sh.ncp);
   const double xv_bytes = 16.0 * sh.npairs * n2 * (sh.nr + sh.ncp);
   auto round = [&](int r, int mode) -> int {
-    if (sh.q == 1) {
-      StatScope st(ctx, s, TENET_KSTAT_SVD_ROUND, gram_fl + apply_fl, 16.0 * sh.npairs * n2 * sh.nr + 2.0 * xv_bytes);
-      return launch_svd_round(B, 1, (int)sh.npairs, r, 2, mode, s);
-    }
-    {
-      StatScope st(ctx, s, TENET_KSTAT_SVD_ROUND, gram_fl, 16.0 * sh.npairs * n2 * sh.nr);
-      TRY(launch_svd_round(B, 1, (int)(sh.npairs * sh.q), r, 0, mode, s));
-    }
-    StatScope st(ctx, s, TENET_KSTAT_SVD_ROUND, apply_fl, 2.0 * xv_bytes);
-    return launch_svd_round(B, 1, (int)(sh.npairs * sh.q), r, 1, mode, s);
+    if (sh.q == 1)
+      return svd_round_timed(ctx, s, gram_fl + apply_fl, 16.0 * sh.npairs * n2 * sh.nr + 2.0 * xv_bytes, B, 1,
+                             (int)sh.npairs, r, 2, mode);
+    TRY(svd_round_timed(ctx, s, gram_fl, 16.0 * sh.npairs * n2 * sh.nr, B, 1, (int)(sh.npairs * sh.q), r, 0, mode));
+    return svd_round_timed(ctx, s, apply_fl, 2.0 * xv_bytes, B, 1, (int)(sh.npairs * sh.q), r, 1, mode);
   };
   for (int sw = 0; sw < max_sweeps && !converged; ++sw) {
     TRY(round(0, 0));
@@ -1813,14 +1837,8 @@ extern "C" int tenet_grouped_svd(tenet_ctx ctx, tenet_stream stream, int dtype, 
       fl += 8.0 * sh.npairs * n2 * n2 * (2.0 * sh.nr + sh.ncp);
       by += 16.0 * sh.npairs * n2 * (3.0 * sh.nr + 2.0 * sh.ncp);
     }
-    {
-      StatScope st(ctx, s, TENET_KSTAT_SVD_ROUND, fl, by);
-      TRY(launch_svd_round(B, count, (int)max_pairs, 0, 2, 0, s));
-    }
-    for (int r = 0; r < max_rounds; ++r) {
-      StatScope st(ctx, s, TENET_KSTAT_SVD_ROUND, fl, by);
-      TRY(launch_svd_round(B, count, (int)max_pairs, r, 2, 1, s));
-    }
+    TRY(svd_round_timed(ctx, s, fl, by, B, count, (int)max_pairs, 0, 2, 0));
+    for (int r = 0; r < max_rounds; ++r) TRY(svd_round_timed(ctx, s, fl, by, B, count, (int)max_pairs, r, 2, 1));
     TRY(launch_svd_sweep_end(B, count, s));
     {
       TENET_HIP_CHECK(hipMemcpyAsync(hflags.data(), flags_base, (size_t)count * 32, hipMemcpyDeviceToHost, s));
