@@ -269,8 +269,8 @@ int launch_qr_extract(const double* W, const double* tau, int64_t m, int64_t n, 
 // ---------------------------------------------------------------------------
 // Blocked Householder QR (compact WY, Golub & Van Loan 5.2.3; LAPACK zlarft
 // conventions): panels of QB columns are factored by one workgroup with the
-// panel resident in LDS, and the trailing matrix and Q are updated by many
-// workgroups, one 16-column block each:
+// panel resident in registers, and the trailing matrix and Q are updated by
+// many workgroups, one 16-column block each:
 //   panel:    H_1 ... H_jb = I - V T V^H      (T upper triangular)
 //   trailing: A <- (I - V T V^H)^H A = A - V (T^H (V^H A))
 //   Q build:  Q <- (I - V T V^H) Q, panels in reverse order, Q = [I; 0]
@@ -278,609 +278,7 @@ int launch_qr_extract(const double* W, const double* tau, int64_t m, int64_t n, 
 namespace tenet {
 
 constexpr int QB = 16;            // panel width
-constexpr int QP_MAX_ROWS = 512;  // panel rows held in LDS (512 x 16 complex = 128 KiB)
-
-__global__ __launch_bounds__(QR_THREADS) void qr_panel_kernel(double* __restrict__ W, double* __restrict__ tau,
-                                                               double* __restrict__ Tout, int m, int j0, int jb) {
-  extern __shared__ double qsm[];  // pr[QB][mr], pi[QB][mr]
-  __shared__ double red[QR_THREADS / 64][2];
-  __shared__ double s_par[4];
-  __shared__ double tr_s[QB][QB], ti_s[QB][QB];
-  __shared__ double taur[QB], taui[QB];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int mr = m - j0;
-  double* pr = qsm;
-  double* pi = qsm + (size_t)QB * mr;
-  const double2* Wc = reinterpret_cast<const double2*>(W);
-  for (int e = tid; e < jb * mr; e += QR_THREADS) {
-    const int c = e / mr, r = e % mr;
-    const double2 v = Wc[(size_t)(j0 + c) * m + j0 + r];
-    pr[c * mr + r] = v.x;
-    pi[c * mr + r] = v.y;
-  }
-  __syncthreads();
-#pragma clang loop unroll(disable)
-  for (int jj = 0; jj < jb; ++jj) {
-    double* cr = pr + jj * mr;
-    double* ci = pi + jj * mr;
-    double part = 0.0;
-    for (int r = jj + 1 + tid; r < mr; r += QR_THREADS) part += cr[r] * cr[r] + ci[r] * ci[r];
-    part = wave_sum(part);
-    if (lane == 0) red[wave][0] = part;
-    __syncthreads();
-    if (tid == 0) {
-      double xn2 = 0.0;
-      for (int w = 0; w < QR_THREADS / 64; ++w) xn2 += red[w][0];
-      const double ar = cr[jj], ai = ci[jj];
-      double t_r = 0.0, t_i = 0.0, sr = 0.0, si = 0.0, beta = ar;
-      if (!(xn2 == 0.0 && ai == 0.0)) {
-        const double nrm = sqrt(ar * ar + ai * ai + xn2);
-        beta = (ar >= 0.0) ? -nrm : nrm;
-        t_r = (beta - ar) / beta;
-        t_i = -ai / beta;
-        const double dr = ar - beta, di = ai;
-        const double den = dr * dr + di * di;
-        sr = dr / den;
-        si = -di / den;
-      }
-      s_par[0] = sr;
-      s_par[1] = si;
-      s_par[2] = beta;
-      taur[jj] = t_r;
-      taui[jj] = t_i;
-    }
-    __syncthreads();
-    const double sr = s_par[0], si = s_par[1], beta = s_par[2];
-    const double t_r = taur[jj], t_i = taui[jj];
-    const bool ident = (t_r == 0.0 && t_i == 0.0);
-    if (!ident) {
-      for (int r = jj + 1 + tid; r < mr; r += QR_THREADS) {
-        const double xr = cr[r], xi = ci[r];
-        cr[r] = xr * sr - xi * si;
-        ci[r] = xr * si + xi * sr;
-      }
-    }
-    __syncthreads();
-    // apply H^H to the panel's later columns: one wave per column
-    if (!ident) {
-      for (int c = jj + 1 + wave; c < jb; c += QR_THREADS / 64) {
-        double* xr_ = pr + c * mr;
-        double* xi_ = pi + c * mr;
-        double dr = 0.0, di = 0.0;
-        for (int r = jj + lane; r < mr; r += 64) {
-          const double vr = (r == jj) ? 1.0 : cr[r], vi = (r == jj) ? 0.0 : ci[r];
-          dr += vr * xr_[r] + vi * xi_[r];
-          di += vr * xi_[r] - vi * xr_[r];
-        }
-        dr = wave_sum(dr);
-        di = wave_sum(di);
-        const double fr = t_r * dr + t_i * di;  // conj(tau) * dot
-        const double fi = t_r * di - t_i * dr;
-        for (int r = jj + lane; r < mr; r += 64) {
-          const double vr = (r == jj) ? 1.0 : cr[r], vi = (r == jj) ? 0.0 : ci[r];
-          xr_[r] -= vr * fr - vi * fi;
-          xi_[r] -= vr * fi + vi * fr;
-        }
-      }
-    }
-    __syncthreads();
-    if (tid == 0 && !ident) {
-      cr[jj] = beta;
-      ci[jj] = 0.0;
-    }
-    __syncthreads();
-  }
-  // T (zlarft, forward/columnwise): T[i][i] = tau_i,
-  // T[0:i, i] = -tau_i * T[0:i, 0:i] * (V[:, 0:i]^H v_i).  First the dots.
-  for (int e = wave; e < jb * jb; e += QR_THREADS / 64) {
-    const int a = e % jb, b = e / jb;  // tr_s[a][b] = v_a^H v_b for a < b
-    if (a >= b) continue;
-    double dr = 0.0, di = 0.0;
-    for (int r = b + lane; r < mr; r += 64) {
-      const double var = (r == a) ? 1.0 : (r < a ? 0.0 : pr[a * mr + r]);
-      const double vai = (r == a) ? 0.0 : (r < a ? 0.0 : pi[a * mr + r]);
-      const double vbr = (r == b) ? 1.0 : pr[b * mr + r];
-      const double vbi = (r == b) ? 0.0 : pi[b * mr + r];
-      dr += var * vbr + vai * vbi;
-      di += var * vbi - vai * vbr;
-    }
-    dr = wave_sum(dr);
-    di = wave_sum(di);
-    if (lane == 0) {
-      tr_s[a][b] = dr;
-      ti_s[a][b] = di;
-    }
-  }
-  __syncthreads();
-  if (tid == 0) {
-    double Tr[QB][QB], Ti[QB][QB];
-    for (int i = 0; i < jb; ++i)
-      for (int j = 0; j < jb; ++j) {
-        Tr[i][j] = 0.0;
-        Ti[i][j] = 0.0;
-      }
-    for (int i = 0; i < jb; ++i) {
-      const double t_r = taur[i], t_i = taui[i];
-      for (int a = 0; a < i; ++a) {
-        // w_a = sum_{c=a}^{i-1} T[a][c] * d[c][i]
-        double wr = 0.0, wi = 0.0;
-        for (int c = a; c < i; ++c) {
-          wr += Tr[a][c] * tr_s[c][i] - Ti[a][c] * ti_s[c][i];
-          wi += Tr[a][c] * ti_s[c][i] + Ti[a][c] * tr_s[c][i];
-        }
-        Tr[a][i] = -(t_r * wr - t_i * wi);
-        Ti[a][i] = -(t_r * wi + t_i * wr);
-      }
-      Tr[i][i] = t_r;
-      Ti[i][i] = t_i;
-    }
-    for (int i = 0; i < jb; ++i)
-      for (int j = 0; j < jb; ++j) {
-        Tout[2 * (j * QB + i)] = Tr[i][j];
-        Tout[2 * (j * QB + i) + 1] = Ti[i][j];
-      }
-    for (int i = 0; i < jb; ++i) {
-      tau[2 * (j0 + i)] = taur[i];
-      tau[2 * (j0 + i) + 1] = taui[i];
-    }
-  }
-  double2* Wo = reinterpret_cast<double2*>(W);
-  for (int e = tid; e < jb * mr; e += QR_THREADS) {
-    const int c = e / mr, r = e % mr;
-    double2 v;
-    v.x = pr[c * mr + r];
-    v.y = pi[c * mr + r];
-    Wo[(size_t)(j0 + c) * m + j0 + r] = v;
-  }
-}
-
-// M[j0:m, c0:c0+16] <- (I - V T' V^H) M[...], T' = T^H (adj) or T.
-// V = unit lower-trapezoidal columns j0..j0+jb-1 of W (rows j0..m-1).
-// One workgroup per 16-column block starting at col_begin.
-__global__ __launch_bounds__(QR_THREADS) void qr_apply_block_kernel(const double* __restrict__ W,
-                                                                     const double* __restrict__ T,
-                                                                     double* __restrict__ M, int m, int ldm, int j0,
-                                                                     int jb, int col_begin, int col_end, int adj) {
-  constexpr int RSQ = 64;
-  __shared__ double vr[RSQ][QB + 1], vi[RSQ][QB + 1];
-  __shared__ double xr[RSQ][QB + 1], xi[RSQ][QB + 1];
-  __shared__ double yr[QB][QB + 1], yi[QB][QB + 1];
-  __shared__ double zr[QB][QB + 1], zi[QB][QB + 1];
-  const int tid = threadIdx.x;
-  const int c0 = col_begin + blockIdx.x * QB;
-  const int nc = min(QB, col_end - c0);
-  if (nc <= 0) return;
-  const double2* Wc = reinterpret_cast<const double2*>(W);
-  double2* Mc = reinterpret_cast<double2*>(M);
-  // Y = V^H X : thread owns (a = tid % 16, c = tid / 16)
-  const int a = tid % QB, c = tid / QB;
-  double accr = 0.0, acci = 0.0;
-  for (int rb = j0; rb < m; rb += RSQ) {
-    const int rows = min(RSQ, m - rb);
-    __syncthreads();
-    for (int e = tid; e < RSQ * QB; e += QR_THREADS) {
-      const int r = e % RSQ, cc = e / RSQ;
-      const int gr = rb + r;
-      double v_r = 0.0, v_i = 0.0, x_r = 0.0, x_i = 0.0;
-      if (r < rows) {
-        if (cc < jb) {
-          const int gc = j0 + cc;  // V column cc: unit at row gc, zero above
-          if (gr == gc) {
-            v_r = 1.0;
-          } else if (gr > gc) {
-            const double2 w = Wc[(size_t)gc * m + gr];
-            v_r = w.x;
-            v_i = w.y;
-          }
-        }
-        if (cc < nc) {
-          const double2 x = Mc[(size_t)(c0 + cc) * ldm + gr];
-          x_r = x.x;
-          x_i = x.y;
-        }
-      }
-      vr[r][cc] = v_r;
-      vi[r][cc] = v_i;
-      xr[r][cc] = x_r;
-      xi[r][cc] = x_i;
-    }
-    __syncthreads();
-    for (int r = 0; r < rows; ++r) {
-      // conj(V[r][a]) * X[r][c]
-      accr += vr[r][a] * xr[r][c] + vi[r][a] * xi[r][c];
-      acci += vr[r][a] * xi[r][c] - vi[r][a] * xr[r][c];
-    }
-  }
-  yr[a][c] = accr;
-  yi[a][c] = acci;
-  __syncthreads();
-  // Z = T' Y, T' = T^H when adj else T (T stored column-major QB x QB)
-  {
-    double sr = 0.0, si = 0.0;
-    for (int b = 0; b < jb; ++b) {
-      double tr, ti;
-      if (adj) {  // (T^H)[a][b] = conj(T[b][a])
-        tr = T[2 * (a * QB + b)];
-        ti = -T[2 * (a * QB + b) + 1];
-      } else {
-        tr = T[2 * (b * QB + a)];
-        ti = T[2 * (b * QB + a) + 1];
-      }
-      sr += tr * yr[b][c] - ti * yi[b][c];
-      si += tr * yi[b][c] + ti * yr[b][c];
-    }
-    __syncthreads();
-    zr[a][c] = (a < jb) ? sr : 0.0;
-    zi[a][c] = (a < jb) ? si : 0.0;
-  }
-  __syncthreads();
-  // X -= V Z
-  for (int rb = j0; rb < m; rb += RSQ) {
-    const int rows = min(RSQ, m - rb);
-    __syncthreads();
-    for (int e = tid; e < RSQ * QB; e += QR_THREADS) {
-      const int r = e % RSQ, cc = e / RSQ;
-      const int gr = rb + r;
-      double v_r = 0.0, v_i = 0.0;
-      if (r < rows && cc < jb) {
-        const int gc = j0 + cc;
-        if (gr == gc) {
-          v_r = 1.0;
-        } else if (gr > gc) {
-          const double2 w = Wc[(size_t)gc * m + gr];
-          v_r = w.x;
-          v_i = w.y;
-        }
-      }
-      vr[r][cc] = v_r;
-      vi[r][cc] = v_i;
-    }
-    __syncthreads();
-    for (int e = tid; e < RSQ * QB; e += QR_THREADS) {
-      const int r = e % RSQ, cc = e / RSQ;
-      if (r >= rows || cc >= nc) continue;
-      double sr = 0.0, si = 0.0;
-      for (int b = 0; b < jb; ++b) {
-        sr += vr[r][b] * zr[b][cc] - vi[r][b] * zi[b][cc];
-        si += vr[r][b] * zi[b][cc] + vi[r][b] * zr[b][cc];
-      }
-      double2 x = Mc[(size_t)(c0 + cc) * ldm + rb + r];
-      x.x -= sr;
-      x.y -= si;
-      Mc[(size_t)(c0 + cc) * ldm + rb + r] = x;
-    }
-  }
-}
-
-// LDS-resident panel (up to 512 rows x QB columns, 128 KiB), 512 threads.
-// Thread t owns panel row t for the norm, scaling and update steps; for the
-// QB dot products of each reflector, wave w reduces columns 2w and 2w+1 over
-// all rows, so a reflector costs two workgroup barriers and two complex wave
-// reductions per wave.  Requires m - j0 <= 512.
-constexpr int QP_THREADS = 512;
-
-__global__ __launch_bounds__(QP_THREADS) void qr_panel_reg_kernel(double* __restrict__ W, double* __restrict__ tau,
-                                                                   double* __restrict__ Tout, int m, int j0, int jb) {
-  constexpr int NW = QP_THREADS / 64;
-  extern __shared__ double pan[];  // pr[QB][512], pi[QB][512]
-  __shared__ double nred[NW];
-  __shared__ double alpha_s[2];
-  __shared__ double dots[2 * QB];
-  __shared__ double Tr_s[QB][QB], Ti_s[QB][QB];
-  double* pr = pan;
-  double* pi = pan + QB * QP_MAX_ROWS;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int mr = m - j0;
-  const int r = tid;
-  const bool have = r < mr;
-  double2* Wc = reinterpret_cast<double2*>(W);
-  for (int c = 0; c < QB; ++c) {
-    double xr = 0.0, xi = 0.0;
-    if (have && c < jb) {
-      const double2 v = Wc[(size_t)(j0 + c) * m + j0 + r];
-      xr = v.x;
-      xi = v.y;
-    }
-    pr[c * QP_MAX_ROWS + r] = xr;
-    pi[c * QP_MAX_ROWS + r] = xi;
-  }
-  for (int e = tid; e < QB * QB; e += QP_THREADS) {
-    Tr_s[e % QB][e / QB] = 0.0;
-    Ti_s[e % QB][e / QB] = 0.0;
-  }
-  __syncthreads();
-  for (int jj = 0; jj < jb; ++jj) {
-    double* cr = pr + jj * QP_MAX_ROWS;
-    double* ci = pi + jj * QP_MAX_ROWS;
-    const double xr_ = cr[r], xi_ = ci[r];
-    double part = (have && r > jj) ? xr_ * xr_ + xi_ * xi_ : 0.0;
-    part = wave_sum(part);
-    if (lane == 0) nred[wave] = part;
-    if (r == jj) {
-      alpha_s[0] = xr_;
-      alpha_s[1] = xi_;
-    }
-    __syncthreads();
-    double xn2 = 0.0;
-#pragma unroll
-    for (int w = 0; w < NW; ++w) xn2 += nred[w];
-    const double ar = alpha_s[0], ai = alpha_s[1];
-    double t_r = 0.0, t_i = 0.0, sr = 0.0, si = 0.0, beta = ar;
-    const bool ident = (xn2 == 0.0 && ai == 0.0);
-    if (!ident) {
-      const double nrm = sqrt(ar * ar + ai * ai + xn2);
-      beta = (ar >= 0.0) ? -nrm : nrm;
-      t_r = (beta - ar) / beta;
-      t_i = -ai / beta;
-      const double dr = ar - beta, di = ai;
-      const double den = dr * dr + di * di;
-      sr = dr / den;
-      si = -di / den;
-    }
-    // v in column jj: scaled x below the diagonal; the diagonal reads as 1
-    double vr = 0.0, vi = 0.0;
-    if (have && r == jj) {
-      vr = 1.0;
-    } else if (have && r > jj) {
-      vr = ident ? xr_ : xr_ * sr - xi_ * si;
-      vi = ident ? xi_ : xr_ * si + xi_ * sr;
-      cr[r] = vr;
-      ci[r] = vi;
-    }
-    __syncthreads();
-    // dots: wave w reduces columns 2w, 2w+1 over all rows.
-    //   c > jj: v^H x_c (for the update);  c < jj: v_c^H v (column jj of T)
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int c = 2 * wave + h;
-      double a = 0.0, b = 0.0;
-      if (c < jb && c != jj) {
-        const double* xcr = pr + c * QP_MAX_ROWS;
-        const double* xci = pi + c * QP_MAX_ROWS;
-        for (int rr = lane; rr < mr; rr += 64) {
-          if (rr < jj) continue;
-          const double v_r = (rr == jj) ? 1.0 : cr[rr];
-          const double v_i = (rr == jj) ? 0.0 : ci[rr];
-          if (c > jj) {
-            a += v_r * xcr[rr] + v_i * xci[rr];
-            b += v_r * xci[rr] - v_i * xcr[rr];
-          } else if (rr > c) {
-            a += xcr[rr] * v_r + xci[rr] * v_i;
-            b += xcr[rr] * v_i - xci[rr] * v_r;
-          }
-        }
-      }
-      a = wave_sum(a);
-      b = wave_sum(b);
-      if (lane == 0) {
-        dots[2 * c] = a;
-        dots[2 * c + 1] = b;
-      }
-    }
-    __syncthreads();
-    // trailing update of this thread's row: x_c -= conj(tau) v (v^H x_c)
-    if (!ident && have && r >= jj) {
-      for (int c = jj + 1; c < jb; ++c) {
-        const double dr = dots[2 * c], di = dots[2 * c + 1];
-        const double fr = t_r * dr + t_i * di;
-        const double fi = t_r * di - t_i * dr;
-        pr[c * QP_MAX_ROWS + r] -= vr * fr - vi * fi;
-        pi[c * QP_MAX_ROWS + r] -= vr * fi + vi * fr;
-      }
-    }
-    // T[a][jj] = -tau * sum_{c=a}^{jj-1} T[a][c] (v_c^H v), thread a < jj
-    if (tid < jj) {
-      double wr = 0.0, wi = 0.0;
-      for (int c = tid; c < jj; ++c) {
-        wr += Tr_s[tid][c] * dots[2 * c] - Ti_s[tid][c] * dots[2 * c + 1];
-        wi += Tr_s[tid][c] * dots[2 * c + 1] + Ti_s[tid][c] * dots[2 * c];
-      }
-      Tr_s[tid][jj] = -(t_r * wr - t_i * wi);
-      Ti_s[tid][jj] = -(t_r * wi + t_i * wr);
-    }
-    if (tid == jj) {
-      Tr_s[jj][jj] = t_r;
-      Ti_s[jj][jj] = t_i;
-      tau[2 * (j0 + jj)] = t_r;
-      tau[2 * (j0 + jj) + 1] = t_i;
-      if (!ident) {
-        cr[jj] = beta;
-        ci[jj] = 0.0;
-      }
-    }
-    __syncthreads();
-  }
-  if (have) {
-    for (int c = 0; c < jb; ++c) {
-      double2 v;
-      v.x = pr[c * QP_MAX_ROWS + r];
-      v.y = pi[c * QP_MAX_ROWS + r];
-      Wc[(size_t)(j0 + c) * m + j0 + r] = v;
-    }
-  }
-  for (int e = tid; e < QB * QB; e += QP_THREADS) {
-    const int i = e % QB, j = e / QB;
-    Tout[2 * e] = (i < jb && j < jb) ? Tr_s[i][j] : 0.0;
-    Tout[2 * e + 1] = (i < jb && j < jb) ? Ti_s[i][j] : 0.0;
-  }
-}
-
-// Register-resident panel: thread t owns panel row t (up to 512 rows x QB
-// columns, one complex double per column in registers; the column index is
-// unrolled so every register index is a compile-time constant).  Per
-// reflector: one block reduction for the column norm and one for the 2*QB
-// dot products (v^H x_c for the trailing columns, v_c^H v for T), the latter
-// a butterfly transpose-reduction inside each wave (32 shuffles for 32
-// values) plus an 8-way LDS sum.  Three barriers per column in all.
-// Same inputs and outputs as qr_panel_reg_kernel.
-template <int H, int BIT>
-__device__ __forceinline__ void halve_step(double (&v)[2 * QB], int lane) {
-  const bool hi = (lane & BIT) != 0;
-#pragma unroll
-  for (int i = 0; i < H; ++i) {
-    const double keep = hi ? v[H + i] : v[i];
-    const double send = hi ? v[i] : v[H + i];
-    v[i] = keep + __shfl_xor(send, BIT, 64);
-  }
-}
-__device__ __forceinline__ void wave_reduce32(double (&v)[2 * QB], int lane) {
-  // after the five halving steps lane l holds the wave total of value l >> 1
-  static_assert(2 * QB == 32, "the butterfly below reduces 32 values");
-  halve_step<16, 32>(v, lane);
-  halve_step<8, 16>(v, lane);
-  halve_step<4, 8>(v, lane);
-  halve_step<2, 4>(v, lane);
-  halve_step<1, 2>(v, lane);
-  v[0] += __shfl_xor(v[0], 1, 64);
-}
-
-__global__ __launch_bounds__(QP_THREADS) void qr_panel_v2_kernel(double* __restrict__ W, double* __restrict__ tau,
-                                                                  double* __restrict__ Tout, int m, int j0, int jb) {
-  constexpr int NW = QP_THREADS / 64;
-  extern __shared__ double pan[];  // pr[QB][512], pi[QB][512]
-  __shared__ double n_part[NW];
-  __shared__ double alpha_s[2];
-  __shared__ double d_part[NW][2 * QB];
-  __shared__ double d_all[2 * QB];
-  __shared__ double Tr_s[QB][QB], Ti_s[QB][QB];
-  double* pr = pan;
-  double* pi = pan + QB * QP_MAX_ROWS;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int mr = m - j0;
-  const int r = tid;
-  const bool have = r < mr;
-  double2* Wc = reinterpret_cast<double2*>(W);
-#pragma unroll
-  for (int c = 0; c < QB; ++c) {
-    double xr = 0.0, xi = 0.0;
-    if (have && c < jb) {
-      const double2 v = Wc[(size_t)(j0 + c) * m + j0 + r];
-      xr = v.x;
-      xi = v.y;
-    }
-    pr[c * QP_MAX_ROWS + r] = xr;
-    pi[c * QP_MAX_ROWS + r] = xi;
-  }
-  for (int e = tid; e < QB * QB; e += QP_THREADS) {
-    Tr_s[e % QB][e / QB] = 0.0;
-    Ti_s[e % QB][e / QB] = 0.0;
-  }
-  // this thread reads and writes only its own row of the panel, so the
-  // barriers below only order the reductions
-#pragma unroll 1
-  for (int jj = 0; jj < jb; ++jj) {
-    const double ajr = pr[jj * QP_MAX_ROWS + r], aji = pi[jj * QP_MAX_ROWS + r];
-    // ---- reflector for column jj
-    const double part = wave_sum((have && r > jj) ? ajr * ajr + aji * aji : 0.0);
-    if (lane == 0) n_part[wave] = part;
-    if (r == jj) {
-      alpha_s[0] = ajr;
-      alpha_s[1] = aji;
-    }
-    __syncthreads();
-    double xn2 = 0.0;
-#pragma unroll
-    for (int w = 0; w < NW; ++w) xn2 += n_part[w];
-    const double ar = alpha_s[0], ai = alpha_s[1];
-    double t_r = 0.0, t_i = 0.0, sr = 0.0, si = 0.0, beta = ar;
-    const bool ident = (xn2 == 0.0 && ai == 0.0);
-    if (!ident) {
-      const double nrm = sqrt(ar * ar + ai * ai + xn2);
-      beta = (ar >= 0.0) ? -nrm : nrm;
-      t_r = (beta - ar) / beta;
-      t_i = -ai / beta;
-      const double dr = ar - beta, di = ai;
-      const double den = dr * dr + di * di;
-      sr = dr / den;
-      si = -di / den;
-    }
-    double vr = 0.0, vi = 0.0;
-    if (have && r == jj) {
-      vr = 1.0;
-    } else if (have && r > jj) {
-      vr = ident ? ajr : ajr * sr - aji * si;
-      vi = ident ? aji : ajr * si + aji * sr;
-    }
-    // ---- dots: c > jj: v^H x_c ; c < jj: v_c^H v (v_c: 1 at row c, stored below)
-    double dv[2 * QB];
-#pragma unroll
-    for (int c = 0; c < QB; ++c) {
-      const double xr = pr[c * QP_MAX_ROWS + r], xi = pi[c * QP_MAX_ROWS + r];
-      double a = 0.0, b = 0.0;
-      if (c > jj) {
-        a = vr * xr + vi * xi;
-        b = vr * xi - vi * xr;
-      } else if (c < jj) {
-        const double ur = (r == c) ? 1.0 : (r > c ? xr : 0.0);
-        const double ui = (r > c) ? xi : 0.0;
-        a = ur * vr + ui * vi;
-        b = ur * vi - ui * vr;
-      }
-      dv[2 * c] = a;
-      dv[2 * c + 1] = b;
-    }
-    wave_reduce32(dv, lane);
-    if ((lane & 1) == 0) d_part[wave][lane >> 1] = dv[0];
-    __syncthreads();
-    if (tid < 2 * QB) {
-      double t = 0.0;
-#pragma unroll
-      for (int w = 0; w < NW; ++w) t += d_part[w][tid];
-      d_all[tid] = t;
-    }
-    __syncthreads();
-    // ---- trailing update of this row, then the reflector into column jj
-    if (!ident && have && r >= jj) {
-#pragma unroll
-      for (int c = 0; c < QB; ++c) {
-        if (c > jj) {
-          const double dr = d_all[2 * c], di = d_all[2 * c + 1];
-          const double fr = t_r * dr + t_i * di;
-          const double fi = t_r * di - t_i * dr;
-          pr[c * QP_MAX_ROWS + r] -= vr * fr - vi * fi;
-          pi[c * QP_MAX_ROWS + r] -= vr * fi + vi * fr;
-        }
-      }
-    }
-    if (have && r > jj) {
-      pr[jj * QP_MAX_ROWS + r] = vr;
-      pi[jj * QP_MAX_ROWS + r] = vi;
-    } else if (r == jj && !ident) {
-      pr[jj * QP_MAX_ROWS + r] = beta;
-      pi[jj * QP_MAX_ROWS + r] = 0.0;
-    }
-    // T[a][jj] = -tau * sum_{c=a}^{jj-1} T[a][c] (v_c^H v), thread a < jj (its own row of T)
-    if (tid < jj) {
-      double wr = 0.0, wi = 0.0;
-      for (int c = tid; c < jj; ++c) {
-        wr += Tr_s[tid][c] * d_all[2 * c] - Ti_s[tid][c] * d_all[2 * c + 1];
-        wi += Tr_s[tid][c] * d_all[2 * c + 1] + Ti_s[tid][c] * d_all[2 * c];
-      }
-      Tr_s[tid][jj] = -(t_r * wr - t_i * wi);
-      Ti_s[tid][jj] = -(t_r * wi + t_i * wr);
-    }
-    if (tid == jj) {
-      Tr_s[jj][jj] = t_r;
-      Ti_s[jj][jj] = t_i;
-      tau[2 * (j0 + jj)] = t_r;
-      tau[2 * (j0 + jj) + 1] = t_i;
-    }
-  }
-  __syncthreads();
-  if (have) {
-    for (int c = 0; c < jb; ++c) {
-      double2 v;
-      v.x = pr[c * QP_MAX_ROWS + r];
-      v.y = pi[c * QP_MAX_ROWS + r];
-      Wc[(size_t)(j0 + c) * m + j0 + r] = v;
-    }
-  }
-  for (int e = tid; e < QB * QB; e += QP_THREADS) {
-    const int i = e % QB, j = e / QB;
-    Tout[2 * e] = (i < jb && j < jb) ? Tr_s[i][j] : 0.0;
-    Tout[2 * e + 1] = (i < jb && j < jb) ? Ti_s[i][j] : 0.0;
-  }
-}
+constexpr int QP_MAX_ROWS = 512;  // panel rows held in registers (two per thread of 256)
 
 // Panel v3: 256 threads, thread t owns panel rows t and t + 256 in
 // registers (up to 512 rows x QB columns), the column loop fully unrolled so
@@ -1120,182 +518,6 @@ __global__ __launch_bounds__(QP3_THREADS) void qr_panel_v3_kernel(double* __rest
     const int i = e % QB, j = e / QB;
     Tout[2 * e] = (i < jb && j < jb) ? Tr_s[i][j] : 0.0;
     Tout[2 * e + 1] = (i < jb && j < jb) ? Ti_s[i][j] : 0.0;
-  }
-}
-
-// M[j0:m, c0:c0+16] <- (I - V T' V^H) M, register-blocked.  Thread t: row
-// group g = t / 16 (rows g, g+16, ... of each staged chunk) and a 4 x 4 block
-// (a0 = 4*((t%16)/4), c0 = 4*(t%4)) of Y = V^H X; the 16 row-group partials
-// are reduced through shuffles (4 per wave) and LDS (4 waves).
-__global__ __launch_bounds__(QR_THREADS) void qr_apply_rb_kernel(const double* __restrict__ W,
-                                                                  const double* __restrict__ T, double* __restrict__ M,
-                                                                  int m, int ldm, int j0, int jb, int col_begin,
-                                                                  int col_end, int adj) {
-  constexpr int RC = 64;
-  __shared__ double vr[RC][QB + 1], vi[RC][QB + 1];
-  __shared__ double xr[RC][QB + 1], xi[RC][QB + 1];
-  __shared__ double yred[4][QB * QB][2];
-  __shared__ double zr[QB][QB + 1], zi[QB][QB + 1];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int cb0 = col_begin + blockIdx.x * QB;
-  const int nc = min(QB, col_end - cb0);
-  if (nc <= 0) return;
-  const double2* Wc = reinterpret_cast<const double2*>(W);
-  double2* Mc = reinterpret_cast<double2*>(M);
-  const int g = tid / 16, a0 = 4 * ((tid % 16) / 4), c0 = 4 * (tid % 4);
-  double yr_[4][4], yi_[4][4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      yr_[i][j] = 0.0;
-      yi_[i][j] = 0.0;
-    }
-  // Chunk staging with one-chunk register prefetch: thread loads 4 entries of
-  // V and 4 of X per 64-row chunk.
-  double pv_r[4], pv_i[4], px_r[4], px_i[4];
-  auto gload = [&](int rb, bool withx) {
-    const int rows = min(RC, m - rb);
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int e = tid + q * QR_THREADS;
-      const int r = e % RC, cc = e / RC;
-      const int gr = rb + r;
-      double v_r = 0.0, v_i = 0.0, x_r = 0.0, x_i = 0.0;
-      if (r < rows && cc < jb) {
-        const int gc = j0 + cc;
-        if (gr == gc) {
-          v_r = 1.0;
-        } else if (gr > gc) {
-          const double2 w = Wc[(size_t)gc * m + gr];
-          v_r = w.x;
-          v_i = w.y;
-        }
-      }
-      if (withx && r < rows && cc < nc) {
-        const double2 x = Mc[(size_t)(cb0 + cc) * ldm + gr];
-        x_r = x.x;
-        x_i = x.y;
-      }
-      pv_r[q] = v_r;
-      pv_i[q] = v_i;
-      px_r[q] = x_r;
-      px_i[q] = x_i;
-    }
-  };
-  auto sstore = [&](bool withx) {
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int e = tid + q * QR_THREADS;
-      const int r = e % RC, cc = e / RC;
-      vr[r][cc] = pv_r[q];
-      vi[r][cc] = pv_i[q];
-      if (withx) {
-        xr[r][cc] = px_r[q];
-        xi[r][cc] = px_i[q];
-      }
-    }
-  };
-  gload(j0, true);
-  for (int rb = j0; rb < m; rb += RC) {
-    __syncthreads();
-    sstore(true);
-    __syncthreads();
-    if (rb + RC < m) gload(rb + RC, true);
-#pragma unroll
-    for (int rr = 0; rr < RC / 16; ++rr) {
-      const int r = g + 16 * rr;
-      double pr[4], pi[4], qr[4], qi[4];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        pr[i] = vr[r][a0 + i];
-        pi[i] = vi[r][a0 + i];
-        qr[i] = xr[r][c0 + i];
-        qi[i] = xi[r][c0 + i];
-      }
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {  // conj(V[r][a]) X[r][c]
-          yr_[i][j] += pr[i] * qr[j] + pi[i] * qi[j];
-          yi_[i][j] += pr[i] * qi[j] - pi[i] * qr[j];
-        }
-    }
-  }
-  // reduce the 4 row groups of this wave (lanes l, l^16, l^32, l^48)
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      double a = yr_[i][j], b = yi_[i][j];
-      a += __shfl_xor(a, 16, 64);
-      b += __shfl_xor(b, 16, 64);
-      a += __shfl_xor(a, 32, 64);
-      b += __shfl_xor(b, 32, 64);
-      yr_[i][j] = a;
-      yi_[i][j] = b;
-    }
-  if (lane < 16) {
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        yred[wave][(a0 + i) * QB + c0 + j][0] = yr_[i][j];
-        yred[wave][(a0 + i) * QB + c0 + j][1] = yi_[i][j];
-      }
-  }
-  __syncthreads();
-  {
-    // Z = T' Y, one entry per thread
-    const int a = tid / QB, c = tid % QB;
-    double sr = 0.0, si = 0.0;
-    for (int b = 0; b < jb; ++b) {
-      const double ybr = yred[0][b * QB + c][0] + yred[1][b * QB + c][0] + yred[2][b * QB + c][0] + yred[3][b * QB + c][0];
-      const double ybi = yred[0][b * QB + c][1] + yred[1][b * QB + c][1] + yred[2][b * QB + c][1] + yred[3][b * QB + c][1];
-      double tr, ti;
-      if (adj) {
-        tr = T[2 * (a * QB + b)];
-        ti = -T[2 * (a * QB + b) + 1];
-      } else {
-        tr = T[2 * (b * QB + a)];
-        ti = T[2 * (b * QB + a) + 1];
-      }
-      sr += tr * ybr - ti * ybi;
-      si += tr * ybi + ti * ybr;
-    }
-    zr[a][c] = (a < jb) ? sr : 0.0;
-    zi[a][c] = (a < jb) ? si : 0.0;
-  }
-  // X -= V Z: thread owns row r = tid % 64 of each chunk and 4 columns
-  const int rsub = tid % RC, cq = 4 * (tid / RC);
-  gload(j0, false);
-  for (int rb = j0; rb < m; rb += RC) {
-    const int rows = min(RC, m - rb);
-    __syncthreads();
-    sstore(false);
-    __syncthreads();
-    if (rb + RC < m) gload(rb + RC, false);
-    if (rsub < rows) {
-      double sr[4] = {0, 0, 0, 0}, si[4] = {0, 0, 0, 0};
-#pragma unroll
-      for (int b = 0; b < QB; ++b) {
-        const double pr = vr[rsub][b], pi = vi[rsub][b];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          sr[j] += pr * zr[b][cq + j] - pi * zi[b][cq + j];
-          si[j] += pr * zi[b][cq + j] + pi * zr[b][cq + j];
-        }
-      }
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        if (cq + j < nc) {
-          double2 x = Mc[(size_t)(cb0 + cq + j) * ldm + rb + rsub];
-          x.x -= sr[j];
-          x.y -= si[j];
-          Mc[(size_t)(cb0 + cq + j) * ldm + rb + rsub] = x;
-        }
-      }
-    }
   }
 }
 
@@ -1566,17 +788,6 @@ static int qr_apply_chunked(const double* W, const double* T, double* M, int m, 
 
 // The panel kernel keeps up to 512 x 16 complex doubles (128 KiB) in dynamic
 // LDS, above HIP's default dynamic limit; raise it once per process.
-int qr_setup_attributes() {
-  static int done = 0;
-  if (done) return TENET_OK;
-  TENET_HIP_CHECK(hipFuncSetAttribute((const void*)qr_panel_reg_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      2 * QB * QP_MAX_ROWS * (int)sizeof(double)));
-  TENET_HIP_CHECK(hipFuncSetAttribute((const void*)qr_panel_v2_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      2 * QB * QP_MAX_ROWS * (int)sizeof(double)));
-  done = 1;
-  return TENET_OK;
-}
-
 size_t qr_blocked_ws_bytes(int64_t m, int64_t n) {
   const int64_t k = std::min(m, n);
   const int64_t np = (k + QB - 1) / QB;
@@ -1587,6 +798,292 @@ size_t qr_blocked_ws_bytes(int64_t m, int64_t n) {
 
 // W (m x n, FP64 complex, already packed) -> Q, R outputs.  Workspace after W:
 // Q scratch (m x k), tau (k), T blocks.
+// ---------------------------------------------------------------------------
+// Small QR in one workgroup: A (m x n) and Q (m x k) resident in LDS for
+// m, n <= 64 (the canonize! moves at small bond dimensions: 64 x 32,
+// 2 x 32, ...), one launch instead of the blocked path's panel / apply /
+// Q-build sequence.  The reflectors are those of
+// qr_factor_kernel (v(0) = 1, beta = -sign(Re alpha) ||(alpha, x)||, H^H
+// applied from the left, Q = H_1 ... H_k [I; 0] accumulated backwards).
+// Lanes run over rows and the sixteen waves over columns (two columns per
+// pass, so two reductions are in flight).  Wave 0 owns column j + 1 at step j: it
+// updates that column first and then forms the next reflector, so a
+// reflector costs one barrier; column j's scaled vector is written during
+// step j + 1, when no wave reads that column.
+// ---------------------------------------------------------------------------
+constexpr int QS_THREADS = 1024;
+constexpr int QS_ELEMS = 8192;  // complex entries of A and Q together (128 KiB)
+constexpr int QS_MAX_K = 64;    // m (n + k) <= QS_ELEMS and k = min(m, n) give k <= 64
+
+// Sum over the 64 lanes, every lane receiving it: quad, half-row and row
+// levels by DPP (register latency), then the 32-lane and 64-lane halves.
+__device__ __forceinline__ double wave_sum_dpp(double v) {
+  v += dpp_d<0xB1>(v);   // quad_perm [1,0,3,2]: l ^ 1
+  v += dpp_d<0x4E>(v);   // quad_perm [2,3,0,1]: l ^ 2
+  v += dpp_d<0x141>(v);  // row_half_mirror: the other quad of the 8
+  v += dpp_d<0x140>(v);  // row_mirror: the other 8 of the 16
+  v += swz16_d(v);       // l ^ 16
+  v += __shfl_xor(v, 32, 64);
+  return v;
+}
+
+struct QsScal {
+  double tr, ti;  // tau
+  double sr, si;  // 1 / (alpha - beta)
+  double beta;
+  int ident;  // H = I
+};
+
+// Reflector of column j (rows j..m-1), one wave; lane 0 writes *out.
+__device__ __forceinline__ void qs_reflector(const double2* A, int m, int j, int lane, QsScal* out) {
+  double part = 0.0;
+  for (int i = j + 1 + lane; i < m; i += 64) {
+    const double2 x = A[(size_t)j * m + i];
+    part += x.x * x.x + x.y * x.y;
+  }
+  part = wave_sum_dpp(part);
+  if (lane == 0) {
+    const double2 a = A[(size_t)j * m + j];
+    const double ar = a.x, ai = a.y;
+    QsScal o;
+    o.tr = 0.0;
+    o.ti = 0.0;
+    o.sr = 0.0;
+    o.si = 0.0;
+    o.beta = ar;
+    o.ident = 1;
+    if (!(part == 0.0 && ai == 0.0)) {
+      const double nrm = sqrt(ar * ar + ai * ai + part);
+      const double beta = (ar >= 0.0) ? -nrm : nrm;
+      o.tr = (beta - ar) / beta;
+      o.ti = -ai / beta;
+      const double dr = ar - beta, di = ai;
+      const double den = dr * dr + di * di;
+      o.sr = dr / den;
+      o.si = -di / den;
+      o.beta = beta;
+      o.ident = 0;
+    }
+    *out = o;
+  }
+}
+
+template <typename S2>
+__global__ __launch_bounds__(QS_THREADS) void qr_small_kernel(const double* __restrict__ W, int m, int n, void* Qout,
+                                                              int64_t ldq, void* Rout, int64_t ldr, int flags) {
+  __shared__ double2 A[QS_ELEMS];  // A (ld = m), then Q (ld = m)
+  __shared__ QsScal sc[QS_MAX_K];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int k = min(m, n);
+  double2* Q = A + (size_t)m * n;
+  const double2* W2 = reinterpret_cast<const double2*>(W);
+  for (int e = tid; e < m * n; e += QS_THREADS) A[e] = W2[e];
+  __syncthreads();
+  if (wave == 0) qs_reflector(A, m, 0, lane, &sc[0]);
+  __syncthreads();
+  for (int j = 0; j < k; ++j) {
+    if (j > 0 && !sc[j - 1].ident) {
+      // column j - 1: beta on the diagonal, the scaled vector below
+      const QsScal p = sc[j - 1];
+      for (int i = j - 1 + tid; i < m; i += QS_THREADS) {
+        double2 x = A[(size_t)(j - 1) * m + i];
+        if (i == j - 1) {
+          x.x = p.beta;
+          x.y = 0.0;
+        } else {
+          const double xr = x.x, xi = x.y;
+          x.x = xr * p.sr - xi * p.si;
+          x.y = xr * p.si + xi * p.sr;
+        }
+        A[(size_t)(j - 1) * m + i] = x;
+      }
+    }
+    const QsScal r = sc[j];
+    bool next_done = false;
+    if (!r.ident) {
+      // A(j:, c) -= conj(tau) v (v^H A(j:, c)), v = [1; x / (alpha - beta)]
+      for (int c = j + 1 + wave; c < n; c += 2 * (QS_THREADS / 64)) {
+        const int c2 = c + QS_THREADS / 64;
+        const bool two = c2 < n;
+        double d1r = 0.0, d1i = 0.0, d2r = 0.0, d2i = 0.0;
+        for (int i = j + lane; i < m; i += 64) {
+          double vr = 1.0, vi = 0.0;
+          if (i != j) {
+            const double2 x = A[(size_t)j * m + i];
+            vr = x.x * r.sr - x.y * r.si;
+            vi = x.x * r.si + x.y * r.sr;
+          }
+          const double2 a1 = A[(size_t)c * m + i];
+          d1r += vr * a1.x + vi * a1.y;
+          d1i += vr * a1.y - vi * a1.x;
+          if (two) {
+            const double2 a2 = A[(size_t)c2 * m + i];
+            d2r += vr * a2.x + vi * a2.y;
+            d2i += vr * a2.y - vi * a2.x;
+          }
+        }
+        d1r = wave_sum_dpp(d1r);
+        d1i = wave_sum_dpp(d1i);
+        d2r = wave_sum_dpp(d2r);
+        d2i = wave_sum_dpp(d2i);
+        const double f1r = r.tr * d1r + r.ti * d1i, f1i = r.tr * d1i - r.ti * d1r;  // conj(tau) * dot
+        const double f2r = r.tr * d2r + r.ti * d2i, f2i = r.tr * d2i - r.ti * d2r;
+        for (int i = j + lane; i < m; i += 64) {
+          double vr = 1.0, vi = 0.0;
+          if (i != j) {
+            const double2 x = A[(size_t)j * m + i];
+            vr = x.x * r.sr - x.y * r.si;
+            vi = x.x * r.si + x.y * r.sr;
+          }
+          double2 a1 = A[(size_t)c * m + i];
+          a1.x -= vr * f1r - vi * f1i;
+          a1.y -= vr * f1i + vi * f1r;
+          A[(size_t)c * m + i] = a1;
+          if (two) {
+            double2 a2 = A[(size_t)c2 * m + i];
+            a2.x -= vr * f2r - vi * f2i;
+            a2.y -= vr * f2i + vi * f2r;
+            A[(size_t)c2 * m + i] = a2;
+          }
+        }
+        if (wave == 0 && !next_done && j + 1 < k) {
+          // column j + 1 (this wave's first) is final: the next reflector
+          qs_reflector(A, m, j + 1, lane, &sc[j + 1]);
+          next_done = true;
+        }
+      }
+    }
+    if (wave == 0 && !next_done && j + 1 < k) qs_reflector(A, m, j + 1, lane, &sc[j + 1]);
+    __syncthreads();
+  }
+  if (k > 0 && !sc[k - 1].ident) {
+    const QsScal p = sc[k - 1];
+    for (int i = k - 1 + tid; i < m; i += QS_THREADS) {
+      double2 x = A[(size_t)(k - 1) * m + i];
+      if (i == k - 1) {
+        x.x = p.beta;
+        x.y = 0.0;
+      } else {
+        const double xr = x.x, xi = x.y;
+        x.x = xr * p.sr - xi * p.si;
+        x.y = xr * p.si + xi * p.sr;
+      }
+      A[(size_t)(k - 1) * m + i] = x;
+    }
+  }
+  // Q = H_1 ... H_k [I; 0], backwards
+  for (int e = tid; e < m * k; e += QS_THREADS) {
+    double2 z;
+    z.x = (e % m == e / m) ? 1.0 : 0.0;
+    z.y = 0.0;
+    Q[e] = z;
+  }
+  __syncthreads();
+  for (int j = k - 1; j >= 0; --j) {
+    const QsScal r = sc[j];
+    if (r.ident) continue;  // uniform
+    // Q(j:, c) -= tau v (v^H Q(j:, c)) for c = j..k-1 (columns < j are still e_c)
+    for (int c = j + wave; c < k; c += 2 * (QS_THREADS / 64)) {
+      const int c2 = c + QS_THREADS / 64;
+      const bool two = c2 < k;
+      double d1r = 0.0, d1i = 0.0, d2r = 0.0, d2i = 0.0;
+      for (int i = j + lane; i < m; i += 64) {
+        double vr = 1.0, vi = 0.0;
+        if (i != j) {
+          const double2 x = A[(size_t)j * m + i];
+          vr = x.x;
+          vi = x.y;
+        }
+        const double2 q1 = Q[(size_t)c * m + i];
+        d1r += vr * q1.x + vi * q1.y;
+        d1i += vr * q1.y - vi * q1.x;
+        if (two) {
+          const double2 q2 = Q[(size_t)c2 * m + i];
+          d2r += vr * q2.x + vi * q2.y;
+          d2i += vr * q2.y - vi * q2.x;
+        }
+      }
+      d1r = wave_sum_dpp(d1r);
+      d1i = wave_sum_dpp(d1i);
+      d2r = wave_sum_dpp(d2r);
+      d2i = wave_sum_dpp(d2i);
+      const double f1r = r.tr * d1r - r.ti * d1i, f1i = r.tr * d1i + r.ti * d1r;  // tau * dot
+      const double f2r = r.tr * d2r - r.ti * d2i, f2i = r.tr * d2i + r.ti * d2r;
+      for (int i = j + lane; i < m; i += 64) {
+        double vr = 1.0, vi = 0.0;
+        if (i != j) {
+          const double2 x = A[(size_t)j * m + i];
+          vr = x.x;
+          vi = x.y;
+        }
+        double2 q1 = Q[(size_t)c * m + i];
+        q1.x -= vr * f1r - vi * f1i;
+        q1.y -= vr * f1i + vi * f1r;
+        Q[(size_t)c * m + i] = q1;
+        if (two) {
+          double2 q2 = Q[(size_t)c2 * m + i];
+          q2.x -= vr * f2r - vi * f2i;
+          q2.y -= vr * f2i + vi * f2r;
+          Q[(size_t)c2 * m + i] = q2;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // outputs: Q (m x k) and R (k x n, upper triangle of A), adjoints on request
+  const bool qadj = (flags & TENET_QR_Q_ADJ) != 0, radj = (flags & TENET_QR_R_ADJ) != 0;
+  S2* Qo = static_cast<S2*>(Qout);
+  S2* Ro = static_cast<S2*>(Rout);
+  for (int e = tid; e < m * k; e += QS_THREADS) {
+    const int r = e % m, c = e / m;
+    const double2 v = Q[e];
+    S2 o;
+    o.x = v.x;
+    o.y = qadj ? -v.y : v.y;
+    if (qadj)
+      Qo[(size_t)r * ldq + c] = o;
+    else
+      Qo[(size_t)c * ldq + r] = o;
+  }
+  for (int e = tid; e < k * n; e += QS_THREADS) {
+    const int r = e % k, c = e / k;
+    double2 v;
+    v.x = 0.0;
+    v.y = 0.0;
+    if (r <= c) v = A[(size_t)c * m + r];
+    S2 o;
+    o.x = v.x;
+    o.y = radj ? -v.y : v.y;
+    if (radj)
+      Ro[(size_t)r * ldr + c] = o;
+    else
+      Ro[(size_t)c * ldr + r] = o;
+  }
+}
+
+// one row per lane and at most two column pairs per wave and step; taller or
+// wider matrices are faster on the blocked path
+bool qr_small_fits(int64_t m, int64_t n) {
+  const int64_t k = std::min(m, n);
+  return m > 0 && n > 0 && m <= 64 && n <= 64 && m * (n + k) <= QS_ELEMS;
+}
+
+int launch_qr_small(const double* W, int64_t m, int64_t n, void* Qout, int64_t ldq, void* Rout, int64_t ldr,
+                    int dtype, int flags, hipStream_t s) {
+  if (!qr_small_fits(m, n)) {
+    set_error("tenet_qr_thin: matrix too large for the single-workgroup QR");
+    return TENET_ERR_UNSUPPORTED;
+  }
+  if (dtype == TENET_C128)
+    hipLaunchKernelGGL(qr_small_kernel<double2>, dim3(1), dim3(QS_THREADS), 0, s, W, (int)m, (int)n, Qout, ldq, Rout,
+                       ldr, flags);
+  else
+    hipLaunchKernelGGL(qr_small_kernel<float2>, dim3(1), dim3(QS_THREADS), 0, s, W, (int)m, (int)n, Qout, ldq, Rout,
+                       ldr, flags);
+  TENET_HIP_CHECK(hipGetLastError());
+  return TENET_OK;
+}
+
 int launch_qr_blocked(double* W, int64_t m, int64_t n, void* Qout, int64_t ldq, void* Rout, int64_t ldr, int dtype,
                       int flags, hipStream_t s) {
   const int64_t k = std::min(m, n);

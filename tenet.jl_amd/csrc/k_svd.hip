@@ -187,12 +187,10 @@ __device__ void gram_over_rows(SvSmem& sm, const double* X, int64_t ld, int r0, 
 }
 
 // Diagonalise the Hermitian Gram (n2 x n2, n2 <= 32) by one cyclic Jacobi
-// sweep; W accumulates the rotations.  Step s applies n2/2 disjoint
-// rotations J_k (slot k pairs columns p_k < q_k), G <- J^H G J and W <- W J.
-// CROSS: slot k pairs local column k (block I) with b + (k + s) % b
-// (block J), b steps per sweep, so every I-J pair is visited once and pairs
-// inside a block are left to the full rounds; otherwise the round-robin
-// (circle) order over all n2 columns, n2 - 1 steps.
+// sweep in the round-robin (circle) order over all n2 columns: n2 - 1
+// steps, step s applies n2/2 disjoint rotations J_k (slot k pairs columns
+// p_k < q_k), G <- J^H G J and W <- W J.  (The cross rounds, which pair only
+// columns of different blocks, run jacobi_cross32 below.)
 //
 // One workgroup barrier per step.  G is double-buffered (the second buffer
 // lies over the row-staging area, idle during the solve).  Every wave
@@ -203,27 +201,20 @@ __device__ void gram_over_rows(SvSmem& sm, const double* X, int64_t ld, int r0, 
 // barrier is needed for that exchange).  G stays exactly Hermitian: only
 // the blocks above the diagonal are computed (their mirrors are written as
 // conjugates), and the diagonal 2x2 blocks take the closed-form update.
-template <bool CROSS>
-__device__ __forceinline__ void inner_pair(int k, int step, int n2, int& p, int& q) {
-  if (CROSS) {
-    const int b = n2 / 2;
-    const int j = k + step;  // < 2b
-    p = k;
-    q = b + (j >= b ? j - b : j);
-  } else {
-    // circle_pos without the modulo: i - 1 + r < 2 (n2 - 1)
-    const int m = n2 - 1;
-    const int i0 = k, i1 = n2 - 1 - k;
-    int a0 = i0 - 1 + step, a1 = i1 - 1 + step;
-    a0 = a0 >= m ? a0 - m : a0;
-    a1 = a1 >= m ? a1 - m : a1;
-    p = i0 == 0 ? 0 : 1 + a0;
-    q = i1 == 0 ? 0 : 1 + a1;
-    if (p > q) {
-      const int t = p;
-      p = q;
-      q = t;
-    }
+// columns of slot k at step s of the circle order
+__device__ __forceinline__ void circle_pair(int k, int step, int n2, int& p, int& q) {
+  // circle_pos without the modulo: i - 1 + r < 2 (n2 - 1)
+  const int m = n2 - 1;
+  const int i0 = k, i1 = n2 - 1 - k;
+  int a0 = i0 - 1 + step, a1 = i1 - 1 + step;
+  a0 = a0 >= m ? a0 - m : a0;
+  a1 = a1 >= m ? a1 - m : a1;
+  p = i0 == 0 ? 0 : 1 + a0;
+  q = i1 == 0 ? 0 : 1 + a1;
+  if (p > q) {
+    const int t = p;
+    p = q;
+    q = t;
   }
 }
 
@@ -332,21 +323,20 @@ __device__ __forceinline__ void block_row(bool h1, double ck, double ekr, double
 // The G and W entries a thread touches depend only on the step's pairing,
 // so they are read before the rotations are formed.  Every thread reads
 // valid entries (unowned units repeat unit 0); only owners store.
-template <bool CROSS>
 __device__ __forceinline__ void inner_step(SvSmem& sm, const GramRow* cr, const GramRow* ci, GramRow* nr, GramRow* ni,
                                            int step, int n2, const InnerOwn& o, double tol2, double abs2,
                                            double& rmax2, int& any) {
   const int w = threadIdx.x >> 6;
   int rp, rq;
-  inner_pair<CROSS>(o.rslot, step, n2, rp, rq);
+  circle_pair(o.rslot, step, n2, rp, rq);
   const double ra = cr[rp][rp], rd = cr[rq][rq], rxr = cr[rp][rq], rxi = ci[rp][rq];
   int p0 = 0, q0 = 0, p1 = 0, q1 = 0;
   double b[8];
   int wp[2], wq[2];
   double wv[2][4];
   if (o.gwave) {
-    inner_pair<CROSS>(o.bk, step, n2, p0, q0);
-    inner_pair<CROSS>(o.bl, step, n2, p1, q1);
+    circle_pair(o.bk, step, n2, p0, q0);
+    circle_pair(o.bl, step, n2, p1, q1);
     b[0] = cr[p0][p1];
     b[1] = ci[p0][p1];
     b[2] = cr[p0][q1];
@@ -358,7 +348,7 @@ __device__ __forceinline__ void inner_step(SvSmem& sm, const GramRow* cr, const 
   } else {
 #pragma unroll
     for (int u = 0; u < 2; ++u) {
-      inner_pair<CROSS>(o.wslot[u], step, n2, wp[u], wq[u]);
+      circle_pair(o.wslot[u], step, n2, wp[u], wq[u]);
       wv[u][0] = sm.wr[o.wrow[u]][wp[u]];
       wv[u][1] = sm.wi[o.wrow[u]][wp[u]];
       wv[u][2] = sm.wr[o.wrow[u]][wq[u]];
@@ -428,7 +418,6 @@ __device__ __forceinline__ void inner_step(SvSmem& sm, const GramRow* cr, const 
   __syncthreads();
 }
 
-template <bool CROSS>
 __device__ __forceinline__ void jacobi_inner(SvSmem& sm, int n2, double tol, double abs_tol) {
   const int tid = threadIdx.x;
   const int npair = n2 / 2;
@@ -470,12 +459,12 @@ __device__ __forceinline__ void jacobi_inner(SvSmem& sm, int n2, double tol, dou
   const double tol2 = tol * tol, abs2 = abs_tol * abs_tol;
   double rmax2 = 0.0;
   int any = 0;
-  const int nsteps = CROSS ? npair : n2 - 1;
+  const int nsteps = n2 - 1;
   __syncthreads();
   // even steps read (gr, gi) and write (hr, hi); odd steps the reverse
   for (int step = 0; step < nsteps; step += 2) {
-    inner_step<CROSS>(sm, sm.gr, sm.gi, sm.hr, sm.hi, step, n2, o, tol2, abs2, rmax2, any);
-    if (step + 1 < nsteps) inner_step<CROSS>(sm, sm.hr, sm.hi, sm.gr, sm.gi, step + 1, n2, o, tol2, abs2, rmax2, any);
+    inner_step(sm, sm.gr, sm.gi, sm.hr, sm.hi, step, n2, o, tol2, abs2, rmax2, any);
+    if (step + 1 < nsteps) inner_step(sm, sm.hr, sm.hi, sm.gr, sm.gi, step + 1, n2, o, tol2, abs2, rmax2, any);
   }
   if (tid < npair) sm.rmax[tid] = rmax2;
   if (any) sm.any_outer = 1;
@@ -802,7 +791,7 @@ __global__ __launch_bounds__(SR_THREADS) void svd_round_kernel(SvdBatch Bt, int 
   if (mode == 1)
     jacobi_cross32(sm, P.tol, abs_tol);
   else
-    jacobi_inner<false>(sm, n2, P.tol, abs_tol);
+    jacobi_inner(sm, n2, P.tol, abs_tol);
   // largest squared relative off-diagonal of this pair visit -> the sweep's maximum
   // (non-negative doubles order like their bit patterns)
   if (threadIdx.x == 0 && c == 0) {

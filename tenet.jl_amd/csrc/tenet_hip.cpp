@@ -918,8 +918,9 @@ static int qr_run_inner(hipStream_t s, int dtype, int flags, int64_t m, int64_t 
   double* W = (double*)b.take((size_t)m * n * 16 + (size_t)m * k * 16);
   double* tau = (double*)b.take((size_t)k * 16);
   TRY(launch_pack_c128(A, dtype, m, n, lda, (flags & TENET_QR_IN_ADJ) ? 1 : 0, W, s));
+  if (qr_small_fits(m, n))
+    return launch_qr_small(W, m, n, Q, ldq, R, ldr, dtype, flags & (TENET_QR_Q_ADJ | TENET_QR_R_ADJ), s);
   if (qr_blocked_fits(m)) {
-    TRY(qr_setup_attributes());
     return launch_qr_blocked(W, m, n, Q, ldq, R, ldr, dtype, flags & (TENET_QR_Q_ADJ | TENET_QR_R_ADJ), s);
   }
   TRY(launch_qr_factor(W, tau, m, n, 1, s));
