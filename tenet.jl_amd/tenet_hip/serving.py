@@ -1,12 +1,14 @@
 """Request loop over device-resident MPS states: bitstring amplitudes
-<x|psi> (overlap(::ProductState, ::MPS), overlap.jl:22-32) and Pauli-string
+<x|psi> (overlap(::ProductState, ::MPS), overlap.jl:22-32), Pauli-string
 expectations <psi|P|psi> (the overlap.jl:36-50 sweep with one-site Paulis,
-ext/TenetPythonCallExt/pytket.jl:23-31 conventions) with continuous
-batching.
+ext/TenetPythonCallExt/pytket.jl:23-31 conventions) and batches of
+samples (sample(psi, nsamples), src/Operations/sample.jl:20-72) with
+continuous batching.
 
 * States stay on the device in MixedCanonical(site 1) form (canonize!, then
   normalize!), so every site right of a Pauli string is a right isometry and
-  the sweep stops at the string's end.
+  the sweep stops at the string's end, and a sample request walks the
+  chain from site 1 without re-canonising.
 * Expectations reuse a cached left environment E(start) per (state, start)
   (tenet_left_env); the first request that needs one computes it on its own
   stream and later requests on other streams wait on its event.
@@ -30,17 +32,20 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _lib as L
-from .device import Context, DeviceArray
+from .device import Context, DeviceArray, tdesc
 
 
 @dataclass
 class Request:
     rid: int
     state: int
-    kind: str  # "amp" or "pauli"
+    kind: str  # "amp", "pauli" or "sample"
     bits: np.ndarray = None
     start: int = 0
     paulis: str = ""
+    nsamples: int = 0
+    seed: int = 0
+    samples: np.ndarray = None  # (nsamples, n) 0-based values of a "sample" request
     t_arrive: float = 0.0
     t_done: float = -1.0
     value: complex = None
@@ -48,7 +53,9 @@ class Request:
     t_admit: float = -1.0
 
     @property
-    def result(self) -> complex:
+    def result(self):
+        if self.kind == "sample":
+            return self.samples
         return self.value * math.exp(self.log_scale)
 
 
@@ -90,6 +97,10 @@ def request_footprint(st: ServedState, r) -> int:
     or the sweep's two environments and two site buffers)."""
     if r.kind == "amp":
         return AMP_REQ_BYTES + st.n
+    if r.kind == "sample":
+        mx = int(np.max(st.chis))
+        d = max(s.shape[1] for s in st.mps.sites)
+        return (2 * mx + d * mx) * r.nsamples * st.dtype.itemsize + 4 * r.nsamples * st.n + 3 * 256
     w = len(r.paulis)
     chi = st.chis[r.start:r.start + w + 1]
     mx_env = int(np.max(chi)) ** 2
@@ -202,7 +213,8 @@ class Server:
         self.env_fixed = {}  # (state, start) -> {"env": ptr, "logval": float}, resident for the server's life
         if prefill_envs:
             self._prefill()
-        self.stats = {"amp_batches": 0, "pauli_launches": 0, "env_built": 0, "env_hits": 0, "admission_waits": 0}
+        self.stats = {"amp_batches": 0, "pauli_launches": 0, "sample_launches": 0, "env_built": 0, "env_hits": 0,
+                      "admission_waits": 0}
 
     def _prefill(self):
         """Every left environment of every state, one sweep per state
@@ -327,8 +339,51 @@ class Server:
         self.stats["pauli_launches"] += 1
         return b
 
+    def _launch_sample(self, r):
+        """The sampling sweep of MPS.sample on one stream: V (1 at site 1) .
+        A_i for all samples, then tenet_sample_select per site; the values
+        land in the request's pinned buffer."""
+        ctx = self.ctx
+        st = self.states[r.state]
+        if st.mps.form != ("mixed", 0, 0):
+            raise ValueError("sample requests need the served state canonised to its first site")
+        ns, n = int(r.nsamples), st.n
+        item = st.dtype.itemsize
+        mx = int(np.max(st.chis))
+        d = max(x.shape[1] for x in st.mps.sites)
+        vbuf = [ctx.malloc(mx * ns * item + 64), ctx.malloc(mx * ns * item + 64)]
+        tbuf = ctx.malloc(d * mx * ns * item + 64)
+        bits = ctx.malloc(4 * ns * n + 64)
+        pin = self.pins.get(max(4 * ns * n, ns * item) + 64)
+        s = self._stream()
+        ones = pin.view(st.dtype, ns)
+        ones[:] = 1.0
+        L.check(L.lib.tenet_memcpy_async(ctx.h, C.c_void_p(vbuf[0]), C.c_void_p(pin.ptr), ns * item, 0, s))
+        v = vbuf[0]
+        for i in range(n):
+            l, dd, rr = st.mps.sites[i].shape
+            da, db, dc = tdesc((l, ns), [1, 4]), tdesc((l, dd, rr), [1, 2, 3]), tdesc((dd, rr, ns), [2, 3, 4])
+            L.check(L.lib.tenet_einsum(ctx.h, s, st.code, L.PREC_NATIVE, C.byref(da), C.c_void_p(v), C.byref(db),
+                                       C.c_void_p(st.mps.sites[i].ptr), C.byref(dc), C.c_void_p(tbuf), 0, 0),
+                    "einsum")
+            vn = vbuf[(i + 1) % 2] if i < n - 1 else None
+            L.check(L.lib.tenet_sample_select(ctx.h, s, st.code, C.c_void_p(tbuf), dd, rr, ns, n, i,
+                                              C.c_uint64(int(r.seed)), C.c_void_p(bits),
+                                              C.c_void_p(vn) if vn is not None else None), "sample_select")
+            v = vn
+        # the values overwrite the ones in the pinned buffer (same stream, after the upload)
+        L.check(L.lib.tenet_memcpy_async(ctx.h, C.c_void_p(pin.ptr), C.c_void_p(bits), 4 * ns * n, 1, s))
+        ev = self._event()
+        ev.record(s)
+        self.stats["sample_launches"] += 1
+        return _Batch([r], "sample", ev, pin, vbuf + [tbuf, bits], self.footprint(r))
+
     def _finish(self, b: _Batch, now):
-        if b.kind == "amp":
+        if b.kind == "sample":
+            r = b.reqs[0]
+            r.samples = b.pinned.view(np.int32, r.nsamples * self.states[r.state].n).reshape(r.nsamples, -1).copy()
+            r.t_done = now
+        elif b.kind == "amp":
             nreq = len(b.reqs)
             nb = sum(self.states[r.state].n for r in b.reqs)
             res = b.pinned.view(np.float64, 3 * nreq, offset=32 * nreq + nb).reshape(nreq, 3)
@@ -401,6 +456,10 @@ class Server:
                         inflight.append(self._launch_amps(amps, now))
                         n_inflight += len(amps)
                         amps = []
+                elif r.kind == "sample":
+                    self.used += self.footprint(r)
+                    inflight.append(self._launch_sample(r))
+                    n_inflight += 1
                 else:
                     self.used += self.footprint(r)
                     inflight.append(self._launch_pauli(r))

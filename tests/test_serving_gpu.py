@@ -106,3 +106,34 @@ def test_prefilled_environments(ctx, served):
     for r in reqs:
         assert abs(r.result - ref[r.rid]) <= 1e-5 * max(abs(ref[r.rid]), 1e-30) + 1e-7
     full.close()
+
+
+def test_sample_requests_in_the_mix(ctx, served):
+    """Sample requests (sample(psi, nsamples), src/Operations/sample.jl:20-72)
+    served together with amplitudes and expectations: each equals MPS.sample
+    of the same device state value for value (the same kernels on the same
+    data and uniform stream), and the oracle's draws on the downloaded state
+    except where a uniform falls within complex64 rounding of a cumulative
+    boundary (at most 1% of the rows)."""
+    from tenet_hip.serving import Request, Server
+    from tenet_hip.workloads import serving_requests
+
+    states, host = served
+    reqs, arr = serving_requests(7, states, 60, rate=20000.0)
+    samp = [Request(rid=len(reqs) + i, state=i % len(states), kind="sample", nsamples=64 + 32 * i, seed=1000 + i)
+            for i in range(6)]
+    arrivals = np.concatenate([np.asarray(arr, dtype=float), np.linspace(0.0, float(np.max(arr)), len(samp))])
+    srv = Server(ctx, states, n_streams=4, max_inflight=64, max_amp_batch=32)
+    srv.run(reqs + samp, arrivals)
+    assert srv.stats["sample_launches"] == len(samp)
+    assert all(r.t_done >= r.t_arrive for r in reqs + samp)
+    for r in samp:
+        got = r.result
+        assert got.shape == (r.nsamples, states[r.state].n)
+        assert np.array_equal(got, states[r.state].mps.sample(r.nsamples, seed=r.seed))
+        ora = O.sample(host[r.state], r.nsamples, r.seed)
+        assert (ora != got).any(axis=1).mean() <= 0.01
+    for r in reqs:
+        ref = _oracle_value(host, r)
+        assert abs(r.result - ref) <= 1e-4 * max(abs(ref), 1e-30) + 1e-6 * (r.kind == "pauli"), (r.kind, r.rid)
+    srv.close()
