@@ -143,8 +143,10 @@ def test_qr_rank_deficient_integer(ctx):
 
 
 # ---------------------------------------------------------------- K3: SVD
+# (1500, 1100): 70 blocks, row split capped at 16 chunks of 94 rows (two staged passes per workgroup), no QR
+# preconditioning (p > 512)
 @pytest.mark.parametrize("m,n", [(6, 4), (4, 6), (33, 20), (64, 64), (100, 300), (512, 512), (1, 1), (1, 9), (9, 1),
-                                 (40, 70)])
+                                 (40, 70), (1500, 1100)])
 def test_svd_values_and_factors(ctx, m, n):
     import tenet_hip as T
 
