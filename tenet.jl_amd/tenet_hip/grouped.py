@@ -112,30 +112,41 @@ def grouped_svd(ctx: Context, mats, maxdims=None, threshold=0.0):
     ns = np.array([a.shape[1] for a in mats], dtype=np.int64)
     kmax = np.minimum(ms, ns)
     md = np.zeros(cnt, dtype=np.int64) if maxdims is None else np.asarray(maxdims, dtype=np.int64)
-    Us = [DeviceArray(ctx, (int(m), int(k)), dt) for m, k in zip(ms, kmax)]
-    Vs = [DeviceArray(ctx, (int(k), int(n)), dt) for k, n in zip(kmax, ns)]
-    Ss = [ctx.malloc(int(k) * 8) for k in kmax]
+    # one arena each for the U's, the V^H's and the singular values (views per
+    # problem), so the outputs cost three allocations and one download
+    item = np.dtype(dt).itemsize
+    u_off = np.concatenate([[0], np.cumsum(ms * kmax)])
+    v_off = np.concatenate([[0], np.cumsum(kmax * ns)])
+    s_off = np.concatenate([[0], np.cumsum(kmax)])
+    u_arena = DeviceArray(ctx, (int(u_off[-1]),), dt)
+    v_arena = DeviceArray(ctx, (int(v_off[-1]),), dt)
+    Us, Vs = [], []
+    for i in range(cnt):
+        u = DeviceArray(ctx, (int(ms[i]), int(kmax[i])), dt, ptr=u_arena.ptr + item * int(u_off[i]), owner=False)
+        u._owner = u_arena
+        v = DeviceArray(ctx, (int(kmax[i]), int(ns[i])), dt, ptr=v_arena.ptr + item * int(v_off[i]), owner=False)
+        v._owner = v_arena
+        Us.append(u)
+        Vs.append(v)
+    s_all = ctx.malloc(int(s_off[-1]) * 8 + 64)
     kd = ctx.malloc(cnt * 8)
     dd = ctx.malloc(cnt * 8)
     try:
         P = C.c_void_p
         L.check(L.lib.tenet_grouped_svd(
             ctx.h, None, code, cnt, ms.ctypes.data_as(C.POINTER(C.c_int64)), ns.ctypes.data_as(C.POINTER(C.c_int64)),
-            (P * cnt)(*[a.ptr for a in mats]), (P * cnt)(*[u.ptr for u in Us]), (P * cnt)(*Ss),
-            (P * cnt)(*[v.ptr for v in Vs]), md.ctypes.data_as(C.POINTER(C.c_int64)), float(threshold), P(kd), P(dd)),
+            (P * cnt)(*[a.ptr for a in mats]), (P * cnt)(*[u.ptr for u in Us]),
+            (P * cnt)(*[s_all + 8 * int(o) for o in s_off[:-1]]), (P * cnt)(*[v.ptr for v in Vs]),
+            md.ctypes.data_as(C.POINTER(C.c_int64)), float(threshold), P(kd), P(dd)),
             "grouped_svd")
         kh = np.zeros(cnt, dtype=np.int64)
         dh = np.zeros(cnt)
+        sh = np.zeros(int(s_off[-1]))
         L.check(L.lib.tenet_memcpy_d2h(ctx.h, kh.ctypes.data_as(P), P(kd), kh.nbytes, None))
         L.check(L.lib.tenet_memcpy_d2h(ctx.h, dh.ctypes.data_as(P), P(dd), dh.nbytes, None))
-        out = []
-        for i in range(cnt):
-            s = np.zeros(int(kmax[i]))
-            L.check(L.lib.tenet_memcpy_d2h(ctx.h, s.ctypes.data_as(P), P(Ss[i]), s.nbytes, None))
-            out.append((Us[i], s, Vs[i], int(kh[i]), float(dh[i])))
-        return out
+        L.check(L.lib.tenet_memcpy_d2h(ctx.h, sh.ctypes.data_as(P), P(s_all), sh.nbytes, None))
+        return [(Us[i], sh[s_off[i]:s_off[i + 1]].copy(), Vs[i], int(kh[i]), float(dh[i])) for i in range(cnt)]
     finally:
-        for p in Ss:
-            ctx.free(p)
+        ctx.free(s_all)
         ctx.free(kd)
         ctx.free(dd)
