@@ -149,6 +149,16 @@ int launch_rescale_log(void* x, int64_t count, double* acc, int dtype, hipStream
 int launch_amplitudes(int64_t nreq, const int64_t* n_dev, const uint64_t* site_ptrs, const uint64_t* chis,
                       const uint64_t* bits, int64_t d, double* out, int dtype, int max_chi, hipStream_t s);
 int launch_write_scalar_from(const void* env, double* out, int dtype, hipStream_t s);
+// <b|a> in one workgroup when n <= 96, every bond <= 32 and d chi^2 <= 2048; the
+// bond sizes and site pointers travel as the kernel's argument (3 KiB)
+constexpr int OS_NMAX = 96;
+struct OverlapTab {
+  int32_t n, d;
+  int64_t chi_a[OS_NMAX + 1], chi_b[OS_NMAX + 1];
+  uint64_t a[OS_NMAX], b[OS_NMAX];
+};
+bool overlap_small_fits(int64_t n, int64_t d, const int64_t* chi_a, const int64_t* chi_b);
+int launch_overlap_small(const OverlapTab& tab, double* out, int dtype, hipStream_t s);
 int launch_trace_diag(const void* env, int64_t chi, double* out, int dtype, hipStream_t s);
 int launch_convert(const void* src, int src_dtype, void* dst, int dst_dtype, int64_t count, hipStream_t s);
 int launch_pack_c128(const void* src, int dtype, int64_t m, int64_t n, int64_t lda, int adj, void* dst,

@@ -2,6 +2,8 @@
 // absorbs singular values (Muscle.simple_update's AbsorbEqually,
 // simple_update.jl:63), the running log-scale of transfer-matrix sweeps, and
 // K4's batched amplitude sweep <x|psi> (overlap.jl:22-32).
+#include <algorithm>
+
 #include "../../include/tenet_hip.h"
 #include "internal.h"
 
@@ -133,6 +135,188 @@ int launch_write_scalar_from(const void* env, double* out, int dtype, hipStream_
     hipLaunchKernelGGL(write_scalar_kernel<double2>, dim3(1), dim3(1), 0, s, (const double2*)env, out);
   else
     hipLaunchKernelGGL(write_scalar_kernel<float2>, dim3(1), dim3(1), 0, s, (const float2*)env, out);
+  TENET_HIP_CHECK(hipGetLastError());
+  return TENET_OK;
+}
+
+// <b|a> for small bonds in one workgroup (overlap.jl:36-50): the environment
+// E (chi_a x chi_b) stays in LDS while the sites stream through, so a chain
+// costs one launch instead of two contractions and a rescale per site.
+// Per site i (E = [1] before site 0):
+//   T(lb, s, ra)  = sum_la E(la, lb) A(la, s, ra)           (lb x d*ra, K = la)
+//   E'(ra, rb)    = sum_{lb, s} T(lb, s, ra) conj(B(lb, s, rb))  (K = lb*d)
+// then E' is divided by its largest |re| / |im| and the log of that scale is
+// added to out[2], as the multi-launch sweep does.  FP64 accumulation; each
+// thread holds a 2 x 4 block of T and a 2 x 2 block of E'; A, B and T sit in
+// LDS transposed so that the lanes of a wave read consecutive entries.
+constexpr int OS_THREADS = 256;
+constexpr int OS_CHI = 32;     // every bond <= 32
+constexpr int OS_SITE = 2048;  // d * chi^2 <= 2048 (A, B and T staged in LDS)
+
+template <typename S2>
+__global__ __launch_bounds__(OS_THREADS) void overlap_small_kernel(const OverlapTab tab, double* __restrict__ out) {
+  __shared__ double Er[OS_CHI][OS_CHI + 1], Ei[OS_CHI][OS_CHI + 1];  // E(la, lb)
+  __shared__ double Ar[OS_SITE], Ai[OS_SITE], Br[OS_SITE], Bi[OS_SITE], Tr[OS_SITE], Ti[OS_SITE];
+  __shared__ double red[OS_THREADS / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int n = tab.n, d = tab.d;
+  const int64_t* chi_a = tab.chi_a;
+  const int64_t* chi_b = tab.chi_b;
+  const uint64_t* pa = tab.a;
+  const uint64_t* pb = tab.b;
+  if (tid == 0) {
+    Er[0][0] = 1.0;
+    Ei[0][0] = 0.0;
+  }
+  double logacc = 0.0;
+  __syncthreads();
+  for (int i = 0; i < n; ++i) {
+    const int la = (int)chi_a[i], lb = (int)chi_b[i], ra = (int)chi_a[i + 1], rb = (int)chi_b[i + 1];
+    const S2* A = reinterpret_cast<const S2*>(pa[i]);
+    const S2* B = reinterpret_cast<const S2*>(pb[i]);
+    const int na = la * d * ra, nb = lb * d * rb;
+    const int N = d * ra, K = lb * d;
+    // all of a thread's loads of both sites in flight before the first store;
+    // stored transposed so that the index running across lanes in the
+    // products below is the contiguous one (no LDS bank conflicts):
+    //   A'[c + N a] = A(a, c),  c = s + d ra     B'[rb + rb_n k] = B(k, rb),  k = lb + lb_n s
+    S2 va[OS_SITE / OS_THREADS], vb[OS_SITE / OS_THREADS];
+#pragma unroll
+    for (int u = 0; u < OS_SITE / OS_THREADS; ++u) {
+      const int e = tid + u * OS_THREADS;
+      if (e < na) va[u] = A[e];
+      if (e < nb) vb[u] = B[e];
+    }
+#pragma unroll
+    for (int u = 0; u < OS_SITE / OS_THREADS; ++u) {
+      const int e = tid + u * OS_THREADS;
+      if (e < na) {
+        const int ia = e % la, ic = e / la;
+        Ar[ic + N * ia] = (double)va[u].x;
+        Ai[ic + N * ia] = (double)va[u].y;
+      }
+      if (e < nb) {
+        const int ik = e % K, ib = e / K;
+        Br[ib + rb * ik] = (double)vb[u].x;
+        Bi[ib + rb * ik] = (double)vb[u].y;
+      }
+    }
+    __syncthreads();
+    // T = E^T A: rows lb, columns c = s + d * ra; stored as T'[ra + ra_n (lb + lb_n s)]
+    {
+      const int M = lb, mb = (M + 1) / 2, nbk = (N + 3) / 4;
+      for (int blk = tid; blk < mb * nbk; blk += OS_THREADS) {
+        const int r0 = 2 * (blk % mb), c0 = 4 * (blk / mb);
+        double tr[2][4], ti[2][4];
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+          for (int y = 0; y < 4; ++y) tr[x][y] = ti[x][y] = 0.0;
+        for (int a = 0; a < la; ++a) {
+          double er[2], ei[2], xr[4], xi[4];
+#pragma unroll
+          for (int x = 0; x < 2; ++x) {
+            const int r = min(r0 + x, M - 1);
+            er[x] = Er[a][r];
+            ei[x] = Ei[a][r];
+          }
+#pragma unroll
+          for (int y = 0; y < 4; ++y) {
+            const int c = min(c0 + y, N - 1);
+            xr[y] = Ar[c + N * a];
+            xi[y] = Ai[c + N * a];
+          }
+#pragma unroll
+          for (int x = 0; x < 2; ++x)
+#pragma unroll
+            for (int y = 0; y < 4; ++y) {
+              tr[x][y] += er[x] * xr[y] - ei[x] * xi[y];
+              ti[x][y] += er[x] * xi[y] + ei[x] * xr[y];
+            }
+        }
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+          for (int y = 0; y < 4; ++y)
+            if (r0 + x < M && c0 + y < N) {
+              const int c = c0 + y, sidx = c % d, rai = c / d;
+              const int t = rai + ra * ((r0 + x) + lb * sidx);
+              Tr[t] = tr[x][y];
+              Ti[t] = ti[x][y];
+            }
+      }
+    }
+    __syncthreads();
+    // E'(ra, rb) = sum_k T'[ra + ra_n k] conj(B'[rb + rb_n k]), k = lb + lb_n s
+    const int mb = (ra + 1) / 2, nbk = (rb + 1) / 2;
+    double outr[2][2] = {{0.0, 0.0}, {0.0, 0.0}}, outi[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+    const bool has = tid < mb * nbk;  // mb * nbk <= 256
+    const int a0 = 2 * (tid % max(mb, 1)), b0 = 2 * (tid / max(mb, 1));
+    double mx = 0.0;
+    if (has) {
+      for (int k = 0; k < K; ++k) {
+        double tr[2], ti[2], br[2], bi[2];
+#pragma unroll
+        for (int x = 0; x < 2; ++x) {
+          const int ai = min(a0 + x, ra - 1), bj = min(b0 + x, rb - 1);
+          tr[x] = Tr[ai + ra * k];
+          ti[x] = Ti[ai + ra * k];
+          br[x] = Br[bj + rb * k];
+          bi[x] = Bi[bj + rb * k];
+        }
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+          for (int y = 0; y < 2; ++y) {  // T * conj(B)
+            outr[x][y] += tr[x] * br[y] + ti[x] * bi[y];
+            outi[x][y] += ti[x] * br[y] - tr[x] * bi[y];
+          }
+      }
+#pragma unroll
+      for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+          if (a0 + x < ra && b0 + y < rb) mx = fmax(mx, fmax(fabs(outr[x][y]), fabs(outi[x][y])));
+    }
+    mx = wave_max(mx);
+    if (lane == 0) red[wave] = mx;
+    __syncthreads();  // also: every read of E, A, T above is done
+    double m = red[0];
+#pragma unroll
+    for (int w = 1; w < OS_THREADS / 64; ++w) m = fmax(m, red[w]);
+    const double inv = m > 0.0 ? 1.0 / m : 1.0;
+    if (has) {
+#pragma unroll
+      for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+          if (a0 + x < ra && b0 + y < rb) {
+            Er[a0 + x][b0 + y] = outr[x][y] * inv;
+            Ei[a0 + x][b0 + y] = outi[x][y] * inv;
+          }
+    }
+    if (m > 0.0) logacc += log(m);
+    __syncthreads();
+  }
+  if (tid == 0) {
+    out[0] = Er[0][0];
+    out[1] = Ei[0][0];
+    out[2] = logacc;
+  }
+}
+
+bool overlap_small_fits(int64_t n, int64_t d, const int64_t* chi_a, const int64_t* chi_b) {
+  if (n < 1 || n > OS_NMAX || d < 1) return false;
+  int64_t mx = 1;
+  for (int64_t i = 0; i <= n; ++i) mx = std::max(mx, std::max(chi_a[i], chi_b[i]));
+  return mx <= OS_CHI && d * mx * mx <= OS_SITE;
+}
+
+int launch_overlap_small(const OverlapTab& tab, double* out, int dtype, hipStream_t s) {
+  if (dtype == TENET_C128)
+    hipLaunchKernelGGL(overlap_small_kernel<double2>, dim3(1), dim3(OS_THREADS), 0, s, tab, out);
+  else
+    hipLaunchKernelGGL(overlap_small_kernel<float2>, dim3(1), dim3(OS_THREADS), 0, s, tab, out);
   TENET_HIP_CHECK(hipGetLastError());
   return TENET_OK;
 }

@@ -1245,13 +1245,28 @@ extern "C" int tenet_overlap(tenet_ctx ctx, tenet_stream stream, int dtype, int 
   CHECK_ARG(chi_a[0] == 1 && chi_a[n] == 1 && chi_b[0] == 1 && chi_b[n] == 1, "open boundary bonds must be 1");
   hipStream_t s = pick(ctx, stream);
   const size_t es = ELEM[dtype];
-  int64_t max_env = 1, max_t = 1;
-  for (int64_t i = 0; i <= n; ++i) max_env = std::max(max_env, chi_a[i] * chi_b[i]);
-  for (int64_t i = 0; i < n; ++i) max_t = std::max(max_t, chi_b[i] * d * chi_a[i + 1]);
   for (int64_t i = 0; i < n; ++i) {
     TRY(check_span(ctx, a_sites[i], (size_t)chi_a[i] * d * chi_a[i + 1] * es, "a site"));
     TRY(check_span(ctx, b_sites[i], (size_t)chi_b[i] * d * chi_b[i + 1] * es, "b site"));
   }
+  if (prec == TENET_PREC_NATIVE && overlap_small_fits(n, d, chi_a, chi_b)) {
+    // small bonds: the whole chain in one workgroup, the environment in LDS
+    OverlapTab tab{};
+    tab.n = (int32_t)n;
+    tab.d = (int32_t)d;
+    for (int64_t i = 0; i <= n; ++i) {
+      tab.chi_a[i] = chi_a[i];
+      tab.chi_b[i] = chi_b[i];
+    }
+    for (int64_t i = 0; i < n; ++i) {
+      tab.a[i] = (uint64_t)(uintptr_t)a_sites[i];
+      tab.b[i] = (uint64_t)(uintptr_t)b_sites[i];
+    }
+    return launch_overlap_small(tab, out, dtype, s);
+  }
+  int64_t max_env = 1, max_t = 1;
+  for (int64_t i = 0; i <= n; ++i) max_env = std::max(max_env, chi_a[i] * chi_b[i]);
+  for (int64_t i = 0; i < n; ++i) max_t = std::max(max_t, chi_b[i] * d * chi_a[i + 1]);
   void* w = nullptr;
   TRY(ws_get(ctx, s, 2 * align_up((size_t)max_env * es, 256) + align_up((size_t)max_t * es, 256) + 256, &w));
   Bump bm(w);

@@ -340,6 +340,23 @@ def test_overlap_norm_amplitudes_expectation(ctx):
         assert abs(got - ref) < 1e-10 * abs(np.vdot(vec, vec))
 
 
+# bonds <= 32 with d chi^2 <= 2048 take the one-workgroup sweep; chi 40 the per-site launches
+@pytest.mark.parametrize("n,d,chi_a,chi_b", [(9, 2, 6, 5), (6, 3, 8, 7), (7, 2, 32, 32), (7, 2, 40, 24),
+                                             (1, 2, 1, 1)])
+def test_overlap_of_two_states(ctx, n, d, chi_a, chi_b):
+    """overlap(a, b) = <b|a> (overlap.jl:36-50) against the dense vectors."""
+    import tenet_hip as T
+
+    rng = np.random.default_rng(100 * n + chi_a)
+    oa = O.random_gaussian_mps(rng, n, d, chi_a)
+    ob = O.random_gaussian_mps(rng, n, d, chi_b)
+    a, b = T.MPS.from_host(ctx, oa.A), T.MPS.from_host(ctx, ob.A)
+    ref = np.vdot(ob.to_dense(), oa.to_dense())
+    got = a.overlap(b)
+    scale = np.linalg.norm(oa.to_dense()) * np.linalg.norm(ob.to_dense())
+    assert abs(got - ref) < 1e-12 * scale
+
+
 def test_overlap_large_dynamic_range(ctx):
     """Unnormalised Gaussian MPS (n=20, chi=32): <psi|psi> ~ 1e36 needs the
     running log-scale."""
