@@ -95,7 +95,7 @@ int launch_qr_extract(const double* W, const double* tau, int64_t m, int64_t n, 
 // single-workgroup QR with A and Q in LDS (m (n + min(m, n)) <= 8192)
 bool qr_small_fits(int64_t m, int64_t n);
 int launch_qr_small(const double* W, int64_t m, int64_t n, void* Qout, int64_t ldq, void* Rout, int64_t ldr,
-                    int dtype, int flags, hipStream_t s);
+                    int dtype, int flags, hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 bool qr_blocked_fits(int64_t m);
 size_t qr_blocked_ws_bytes(int64_t m, int64_t n);
 int launch_qr_blocked(double* W, int64_t m, int64_t n, void* Qout, int64_t ldq, void* Rout, int64_t ldr, int dtype,

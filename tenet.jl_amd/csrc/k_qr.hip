@@ -10,6 +10,8 @@
 // lives in a FP64 work copy in global memory; the current reflector is
 // staged in LDS and each wave owns a set of trailing columns, so the dot
 // products need only wave-level (64-lane) reductions.
+#include <hip/hip_ext.h>
+
 #include "../../include/tenet_hip.h"
 #include "internal.h"
 
@@ -1068,18 +1070,19 @@ bool qr_small_fits(int64_t m, int64_t n) {
   return m > 0 && n > 0 && m <= 64 && n <= 64 && m * (n + k) <= QS_ELEMS;
 }
 
+// ev0 / ev1 (optional): stamped at the kernel's start and end by the dispatch
 int launch_qr_small(const double* W, int64_t m, int64_t n, void* Qout, int64_t ldq, void* Rout, int64_t ldr,
-                    int dtype, int flags, hipStream_t s) {
+                    int dtype, int flags, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
   if (!qr_small_fits(m, n)) {
     set_error("tenet_qr_thin: matrix too large for the single-workgroup QR");
     return TENET_ERR_UNSUPPORTED;
   }
   if (dtype == TENET_C128)
-    hipLaunchKernelGGL(qr_small_kernel<double2>, dim3(1), dim3(QS_THREADS), 0, s, W, (int)m, (int)n, Qout, ldq, Rout,
-                       ldr, flags);
+    hipExtLaunchKernelGGL(qr_small_kernel<double2>, dim3(1), dim3(QS_THREADS), 0, s, ev0, ev1, 0, W, (int)m, (int)n,
+                          Qout, ldq, Rout, ldr, flags);
   else
-    hipLaunchKernelGGL(qr_small_kernel<float2>, dim3(1), dim3(QS_THREADS), 0, s, W, (int)m, (int)n, Qout, ldq, Rout,
-                       ldr, flags);
+    hipExtLaunchKernelGGL(qr_small_kernel<float2>, dim3(1), dim3(QS_THREADS), 0, s, ev0, ev1, 0, W, (int)m, (int)n,
+                          Qout, ldq, Rout, ldr, flags);
   TENET_HIP_CHECK(hipGetLastError());
   return TENET_OK;
 }
