@@ -280,7 +280,7 @@ int launch_qr_extract(const double* W, const double* tau, int64_t m, int64_t n, 
 namespace tenet {
 
 constexpr int QB = 16;            // panel width
-constexpr int QP_MAX_ROWS = 512;  // panel rows held in registers (two per thread of 256)
+constexpr int QP_MAX_ROWS = 1024;  // panel rows held in registers (up to four per thread of 256)
 
 // Panel v3: 256 threads, thread t owns panel rows t and t + 256 in
 // registers (up to 512 rows x QB columns), the column loop fully unrolled so
@@ -338,13 +338,14 @@ __device__ __forceinline__ void wave_transpose_reduce32(double (&v)[2 * QB], int
 
 constexpr int QP3_THREADS = 256;
 
-// One column of the v3 panel (JJ compile-time: static register indices).
-template <int JJ>
-__device__ __forceinline__ void qr_panel_v3_step(double (&ar)[2][QB], double (&ai)[2][QB],
+// One column of the v3 panel (JJ compile-time: static register indices;
+// thread t holds panel rows t + 256 r, r < R).
+template <int JJ, int R>
+__device__ __forceinline__ void qr_panel_v3_step(double (&ar)[R][QB], double (&ai)[R][QB],
                                                  double (*part)[QP3_THREADS / 64][2 * QB],
                                                  double (*prow)[2 * QB], double (*Tr_s)[QB], double (*Ti_s)[QB],
-                                                 double* __restrict__ tau, int tid, int lane, int wave, int idx, int row0,
-                                                 int row1, int j0) {
+                                                 double* __restrict__ tau, int tid, int lane, int wave, int idx, const int (&rowv)[R],
+                                                 int j0) {
   constexpr int jj = JJ;
   const int par = jj & 1;
   if (tid == jj) {  // pivot row jj is row slot 0 of thread jj
@@ -359,8 +360,8 @@ __device__ __forceinline__ void qr_panel_v3_step(double (&ar)[2][QB], double (&a
 #pragma unroll
   for (int c = 0; c < QB; ++c) e[2 * c] = e[2 * c + 1] = 0.0;
 #pragma unroll
-  for (int sl = 0; sl < 2; ++sl) {
-    const int row = sl ? row1 : row0;
+  for (int sl = 0; sl < R; ++sl) {
+    const int row = rowv[sl];
     if (row > jj) {
       const double xr = ar[sl][jj], xi = ai[sl][jj];
 #pragma unroll
@@ -402,8 +403,8 @@ __device__ __forceinline__ void qr_panel_v3_step(double (&ar)[2][QB], double (&a
       const double fr = t_r * wr + t_i * wi;
       const double fi = t_r * wi - t_i * wr;
 #pragma unroll
-      for (int sl = 0; sl < 2; ++sl) {
-        const int row = sl ? row1 : row0;
+      for (int sl = 0; sl < R; ++sl) {
+        const int row = rowv[sl];
         if (row == jj) {
           ar[sl][c] -= fr;
           ai[sl][c] -= fi;
@@ -418,8 +419,8 @@ __device__ __forceinline__ void qr_panel_v3_step(double (&ar)[2][QB], double (&a
   }
   // the reflector into column jj (below the diagonal), beta on it
 #pragma unroll
-  for (int sl = 0; sl < 2; ++sl) {
-    const int row = sl ? row1 : row0;
+  for (int sl = 0; sl < R; ++sl) {
+    const int row = rowv[sl];
     if (row > jj && !ident) {
       const double xr = ar[sl][jj], xi = ai[sl][jj];
       ar[sl][jj] = xr * sr - xi * si;
@@ -453,16 +454,17 @@ __device__ __forceinline__ void qr_panel_v3_step(double (&ar)[2][QB], double (&a
   }
   }
 
-template <int... J>
-__device__ __forceinline__ void qr_panel_v3_cols(std::integer_sequence<int, J...>, int jb, double (&ar)[2][QB],
-                                                 double (&ai)[2][QB], double (*part)[QP3_THREADS / 64][2 * QB],
+template <int R, int... J>
+__device__ __forceinline__ void qr_panel_v3_cols(std::integer_sequence<int, J...>, int jb, double (&ar)[R][QB],
+                                                 double (&ai)[R][QB], double (*part)[QP3_THREADS / 64][2 * QB],
                                                  double (*prow)[2 * QB], double (*Tr_s)[QB], double (*Ti_s)[QB],
                                                  double* __restrict__ tau, int tid, int lane, int wave, int idx,
-                                                 int row0, int row1, int j0) {
-  ((J < jb ? qr_panel_v3_step<J>(ar, ai, part, prow, Tr_s, Ti_s, tau, tid, lane, wave, idx, row0, row1, j0) : void()),
+                                                 const int (&rowv)[R], int j0) {
+  ((J < jb ? qr_panel_v3_step<J, R>(ar, ai, part, prow, Tr_s, Ti_s, tau, tid, lane, wave, idx, rowv, j0) : void()),
    ...);
 }
 
+template <int R>
 __global__ __launch_bounds__(QP3_THREADS) void qr_panel_v3_kernel(double* __restrict__ W, double* __restrict__ tau,
                                                                    double* __restrict__ Tout, int m, int j0, int jb) {
   __shared__ double part[2][QP3_THREADS / 64][2 * QB];
@@ -470,52 +472,44 @@ __global__ __launch_bounds__(QP3_THREADS) void qr_panel_v3_kernel(double* __rest
   __shared__ double Tr_s[QB][QB], Ti_s[QB][QB];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int mr = m - j0;
-  const int row0 = tid, row1 = tid + QP3_THREADS;
-  const bool h0 = row0 < mr, h1 = row1 < mr;
   double2* Wc = reinterpret_cast<double2*>(W);
-  double ar[2][QB], ai[2][QB];
+  double ar[R][QB], ai[R][QB];
 #pragma unroll
-  for (int c = 0; c < QB; ++c) {
-    ar[0][c] = ai[0][c] = ar[1][c] = ai[1][c] = 0.0;
-    if (c < jb) {
-      if (h0) {
-        const double2 v = Wc[(size_t)(j0 + c) * m + j0 + row0];
-        ar[0][c] = v.x;
-        ai[0][c] = v.y;
-      }
-      if (h1) {
-        const double2 v = Wc[(size_t)(j0 + c) * m + j0 + row1];
-        ar[1][c] = v.x;
-        ai[1][c] = v.y;
+  for (int c = 0; c < QB; ++c)
+#pragma unroll
+    for (int sl = 0; sl < R; ++sl) {
+      const int row = tid + sl * QP3_THREADS;
+      ar[sl][c] = ai[sl][c] = 0.0;
+      if (c < jb && row < mr) {
+        const double2 v = Wc[(size_t)(j0 + c) * m + j0 + row];
+        ar[sl][c] = v.x;
+        ai[sl][c] = v.y;
       }
     }
-  }
   for (int e = tid; e < QB * QB; e += QP3_THREADS) {
     Tr_s[e % QB][e / QB] = 0.0;
     Ti_s[e % QB][e / QB] = 0.0;
   }
   const int idx = 16 * ((lane >> 3) & 1) + 8 * ((lane >> 2) & 1) + 4 * ((lane >> 1) & 1) + 2 * (lane & 1) +
                   ((lane >> 4) & 1);
-  qr_panel_v3_cols(std::make_integer_sequence<int, QB>{}, jb, ar, ai, part, prow, Tr_s, Ti_s, tau, tid, lane, wave, idx,
-                   row0, row1, j0);
+  int rowv[R];
+#pragma unroll
+  for (int sl = 0; sl < R; ++sl) rowv[sl] = tid + sl * QP3_THREADS;
+  qr_panel_v3_cols<R>(std::make_integer_sequence<int, QB>{}, jb, ar, ai, part, prow, Tr_s, Ti_s, tau, tid, lane, wave,
+                      idx, rowv, j0);
   __syncthreads();
 #pragma unroll
-  for (int c = 0; c < QB; ++c) {
-    if (c < jb) {
-      if (h0) {
+  for (int c = 0; c < QB; ++c)
+#pragma unroll
+    for (int sl = 0; sl < R; ++sl) {
+      const int row = tid + sl * QP3_THREADS;
+      if (c < jb && row < mr) {
         double2 v;
-        v.x = ar[0][c];
-        v.y = ai[0][c];
-        Wc[(size_t)(j0 + c) * m + j0 + row0] = v;
-      }
-      if (h1) {
-        double2 v;
-        v.x = ar[1][c];
-        v.y = ai[1][c];
-        Wc[(size_t)(j0 + c) * m + j0 + row1] = v;
+        v.x = ar[sl][c];
+        v.y = ai[sl][c];
+        Wc[(size_t)(j0 + c) * m + j0 + row] = v;
       }
     }
-  }
   for (int e = tid; e < QB * QB; e += QP3_THREADS) {
     const int i = e % QB, j = e / QB;
     Tout[2 * e] = (i < jb && j < jb) ? Tr_s[i][j] : 0.0;
@@ -1097,8 +1091,12 @@ int launch_qr_blocked(double* W, int64_t m, int64_t n, void* Qout, int64_t ldq, 
   double* Yp = Tb + (size_t)np * QB * QB * 2;  // chunk partials of V^H M
   for (int64_t p = 0; p < np; ++p) {
     const int j0 = (int)(p * QB), jb = (int)std::min<int64_t>(QB, k - j0);
-    hipLaunchKernelGGL(qr_panel_v3_kernel, dim3(1), dim3(QP3_THREADS), 0, s, W, tau, Tb + (size_t)p * QB * QB * 2,
-                       (int)m, j0, jb);
+    if (m - j0 <= 2 * QP3_THREADS)
+      hipLaunchKernelGGL(qr_panel_v3_kernel<2>, dim3(1), dim3(QP3_THREADS), 0, s, W, tau, Tb + (size_t)p * QB * QB * 2,
+                         (int)m, j0, jb);
+    else
+      hipLaunchKernelGGL(qr_panel_v3_kernel<4>, dim3(1), dim3(QP3_THREADS), 0, s, W, tau, Tb + (size_t)p * QB * QB * 2,
+                         (int)m, j0, jb);
     TENET_HIP_CHECK(hipGetLastError());
     const int cb = j0 + jb;
     if (cb < n) {

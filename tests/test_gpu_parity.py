@@ -112,9 +112,11 @@ def test_einsum_shape_errors(ctx):
 
 
 # ---------------------------------------------------------------- K2: QR
-# single-workgroup path for m, n <= 64 (64x64 at its limit); blocked otherwise (65x63, 66x64, 128x32, ...)
+# single-workgroup path for m, n <= 64 (64x64 at its limit); blocked otherwise (65x63, 66x64, 128x32, ...), with
+# two register rows per panel thread up to 512 rows and four up to 1024 (1024x512, 777x300); taller: unblocked
 @pytest.mark.parametrize("m,n", [(6, 4), (4, 6), (64, 32), (2, 32), (512, 256), (1, 5), (7, 1), (130, 130), (64, 64),
-                                 (65, 63), (128, 32), (4096, 1), (1, 4096), (8, 1000), (66, 64)])
+                                 (65, 63), (128, 32), (4096, 1), (1, 4096), (8, 1000), (66, 64), (1024, 512),
+                                 (777, 300), (1500, 40)])
 def test_qr_invariants_and_r_diagonal(ctx, m, n):
     import tenet_hip as T
 
