@@ -1087,9 +1087,15 @@ static int svd_run(tenet_ctx ctx, hipStream_t s, int dtype, int64_t m, int64_t n
 // relative accuracy).  SVD(R) = U_R S V_R^H with the Jacobi on X = R^H gives
 //   m >= n (A = Q R):        U = Q U_R,          V^H = V_R^H
 //   m <  n (A^H = Q R):      U = (V_R^H)^H,      V^H = (Q U_R)^H
+// Preconditioning is available for every form with q >= 64.  Up to 1024
+// rows (the blocked QR) it runs first; above, the QR is the single-workgroup
+// kernel (~20x the plain sweeps' time on a Gaussian 1500 x 300), so the
+// plain sweeps run first and the preconditioned path only when they do not
+// converge — strongly graded matrices (sigma over 12 decades, mixing
+// singular vectors) need more sweeps than the limit without it.
 static bool svd_precond_applies(tenet_ctx ctx, int dtype, int64_t m, int64_t n) {
-  const int64_t p = std::max(m, n), q = std::min(m, n);
-  return ctx->svd_precond && dtype == TENET_C128 && q >= 64 && qr_blocked_fits(p);
+  const int64_t q = std::min(m, n);
+  return ctx->svd_precond && dtype == TENET_C128 && q >= 64;
 }
 
 static size_t svd_precond_ws_bytes(int64_t m, int64_t n) {
@@ -1122,6 +1128,11 @@ static int svd_any(tenet_ctx ctx, hipStream_t s, int dtype, int64_t m, int64_t n
                    double* disc_dev, void* wsp) {
   if (!svd_precond_applies(ctx, dtype, m, n))
     return svd_run(ctx, s, dtype, m, n, A, lda, U, ldu, S, Vh, ldvh, maxdim, thr, k_dev, disc_dev, wsp, true);
+  if (!qr_blocked_fits(std::max(m, n))) {
+    // the Jacobi works on its own copy of A, so a retry starts from the untouched input
+    const int st0 = svd_run(ctx, s, dtype, m, n, A, lda, U, ldu, S, Vh, ldvh, maxdim, thr, k_dev, disc_dev, wsp, true);
+    if (st0 != TENET_ERR_NOCONV) return st0;
+  }
   const int64_t p = std::max(m, n), q = std::min(m, n);
   const bool tall = m >= n;
   Bump bm(wsp);
