@@ -1883,8 +1883,22 @@ extern "C" int tenet_grouped_svd(tenet_ctx ctx, tenet_stream stream, int dtype, 
   TRY(launch_svd_finalize(B, count, (int)max_ncp, max_out, dtype, s));
   TENET_HIP_CHECK(hipStreamSynchronize(s));
   if (!all) {
-    set_error("grouped Jacobi SVD: some problems did not converge");
-    return TENET_ERR_NOCONV;
+    // Problems the plain sweeps left unconverged (strongly graded spectra) go
+    // one at a time through the single-matrix path, which preconditions
+    // complex128 by a QR, from their untouched inputs (the stream is idle, so
+    // its workspace may be reused).
+    for (int64_t i = 0; i < count; ++i) {
+      if (hflags[(size_t)(8 * i + 1)]) continue;
+      const int64_t m = ms[i], n = ns[i];
+      void* w = nullptr;
+      TRY(ws_get(ctx, s, svd_any_ws_bytes(ctx, dtype, m, n), &w));
+      const int st = svd_any(ctx, s, dtype, m, n, As[i], m, Us[i], m, Ss[i], Vhs[i], std::min(m, n),
+                             maxdims ? maxdims[i] : 0, threshold, k_dev + i, disc_dev + i, w);
+      if (st != TENET_OK) {
+        if (st == TENET_ERR_NOCONV) set_error("grouped Jacobi SVD: some problems did not converge");
+        return st;
+      }
+    }
   }
   return TENET_OK;
 }
