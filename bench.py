@@ -806,9 +806,10 @@ def bench_plain(args):
         ach = kst["flops"] / (kst["ms"] * 1e-3) / 1e12
         roof = {"bound": "mfma", "achieved": round(ach, 5), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(ach / PEAK_FP64_TFLOPS, 6), "traffic": None,
-                "kernel": "qr_small_kernel (one 64x32 / 2x32 Householder QR per move, one workgroup)",
+                "kernel": "qr_reg_kernel (one canonize move per launch: 64x32 / 2x32 Householder QR of A^H with "
+                          "R^H absorbed into the neighbour, one workgroup)",
                 "launches": kst["launches"], "avg_launch_us": round(kst["ms"] * 1e3 / kst["launches"], 2),
-                "note": "latency-bound: 19 dependent QR moves of 64x32, 64 barrier-separated reflector steps each"}
+                "note": "latency-bound: 19 dependent moves, 32 barrier-separated reflector steps each"}
     line = {
         "metric": "plain canonize!+norm latency (n=20, chi=32)", "value": round(wall * 1e6, 1), "unit": "us/step",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall * 1e3, 4),

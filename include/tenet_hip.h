@@ -138,6 +138,19 @@ int tenet_qr_thin(tenet_ctx ctx, tenet_stream stream, int dtype, int64_t m, int6
 typedef enum { TENET_QR_IN_ADJ = 1, TENET_QR_Q_ADJ = 2, TENET_QR_R_ADJ = 4 } tenet_qr_flags;
 int tenet_qr_thin_ex(tenet_ctx ctx, tenet_stream stream, int dtype, int flags, int64_t m, int64_t n, const void* A,
                      int64_t lda, void* Q, int64_t ldq, void* R, int64_t ldr);
+/* One canonize move in one launch: the QR above followed by R's contraction
+ * into the neighbour tensor, replacing generic_canonize_site!'s
+ * tensor_qr_thin + binary_einsum(R, B) (src/Operations/canonize.jl:58-70).
+ * flags: TENET_QR_IN_ADJ, TENET_QR_Q_ADJ.  R itself is not returned.
+ * side TENET_ABSORB_R_B: Bout (k x nb, ldbo >= k) = R B, B n x nb (ldb >= n);
+ * side TENET_ABSORB_B_RH: Bout (nb x k, ldbo >= nb) = B R^H, B nb x n
+ * (ldb >= nb).  Bout must not overlap A, Q or B.  m, n <= 64 (the
+ * single-workgroup QR); larger shapes return TENET_ERR_UNSUPPORTED (use
+ * tenet_qr_thin_ex + tenet_einsum). */
+typedef enum { TENET_ABSORB_R_B = 0, TENET_ABSORB_B_RH = 1 } tenet_absorb_side;
+int tenet_qr_absorb(tenet_ctx ctx, tenet_stream stream, int dtype, int flags, int64_t m, int64_t n, const void* A,
+                    int64_t lda, void* Q, int64_t ldq, int side, int64_t nb, const void* B, int64_t ldb, void* Bout,
+                    int64_t ldbo);
 
 /* ---- thin SVD + truncation (K3) -------------------------------------------
  * Replaces Muscle.tensor_svd_thin at canonize.jl:41,97; evolve.jl:62,92;

@@ -92,10 +92,20 @@ int launch_qr_factor(double* W, double* tau, int64_t m, int64_t n, int64_t batch
 int launch_qr_extract(const double* W, const double* tau, int64_t m, int64_t n, int64_t batch, void* Q,
                       int64_t ldq, void* R, int64_t ldr, int dtype, int flags, hipStream_t s);
 
-// single-workgroup QR with A and Q in LDS (m (n + min(m, n)) <= 8192)
+// single-workgroup register-resident QR (m, n <= 64), reading A in place;
+// flags: TENET_QR_IN_ADJ, TENET_QR_Q_ADJ, TENET_QR_R_ADJ.  Rout may be null.
+// abs (optional): R absorbed into a neighbour in the same launch, side 0:
+// Bout (k x nb) = R B (B n x nb), side 1: Bout (nb x k) = B R^H (B nb x n).
+struct QrAbsorbArgs {
+  const void* B;
+  void* Bout;
+  int64_t ldb, ldbo, nb;
+  int side;
+};
 bool qr_small_fits(int64_t m, int64_t n);
-int launch_qr_small(const double* W, int64_t m, int64_t n, void* Qout, int64_t ldq, void* Rout, int64_t ldr,
-                    int dtype, int flags, hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+int launch_qr_small(const void* A, int64_t lda, int64_t m, int64_t n, void* Qout, int64_t ldq, void* Rout,
+                    int64_t ldr, int dtype, int flags, const QrAbsorbArgs* abs, hipStream_t s,
+                    hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 bool qr_blocked_fits(int64_t m);
 size_t qr_blocked_ws_bytes(int64_t m, int64_t n);
 int launch_qr_blocked(double* W, int64_t m, int64_t n, void* Qout, int64_t ldq, void* Rout, int64_t ldr, int dtype,

@@ -782,8 +782,6 @@ static int qr_apply_chunked(const double* W, const double* T, double* M, int m, 
   return TENET_OK;
 }
 
-// The panel kernel keeps up to 512 x 16 complex doubles (128 KiB) in dynamic
-// LDS, above HIP's default dynamic limit; raise it once per process.
 size_t qr_blocked_ws_bytes(int64_t m, int64_t n) {
   const int64_t k = std::min(m, n);
   const int64_t np = (k + QB - 1) / QB;
@@ -792,295 +790,398 @@ size_t qr_blocked_ws_bytes(int64_t m, int64_t n) {
          (size_t)ych * yblk * QB * QB * 16 + 2048;
 }
 
-// W (m x n, FP64 complex, already packed) -> Q, R outputs.  Workspace after W:
-// Q scratch (m x k), tau (k), T blocks.
 // ---------------------------------------------------------------------------
-// Small QR in one workgroup: A (m x n) and Q (m x k) resident in LDS for
-// m, n <= 64 (the canonize! moves at small bond dimensions: 64 x 32,
-// 2 x 32, ...), one launch instead of the blocked path's panel / apply /
-// Q-build sequence.  The reflectors are those of
-// qr_factor_kernel (v(0) = 1, beta = -sign(Re alpha) ||(alpha, x)||, H^H
-// applied from the left, Q = H_1 ... H_k [I; 0] accumulated backwards).
-// Lanes run over rows and the sixteen waves over columns (two columns per
-// pass, so two reductions are in flight).  Wave 0 owns column j + 1 at step j: it
-// updates that column first and then forms the next reflector, so a
-// reflector costs one barrier; column j's scaled vector is written during
-// step j + 1, when no wave reads that column.
+// Register-resident QR for m, n <= 64 (the canonize! moves at small bond
+// dimensions: 64 x 32, 2 x 32, ...): one workgroup, A read in place (adjoint
+// and complex64 input converted on the load), no workspace.  With RL rows per
+// lane (RL = 4 or 8), lane l of wave w holds rows RL rb .. RL rb + RL - 1,
+// rb = l / RL, of column RL w + (l % RL) in registers: a wave's RL columns
+// are updated by the same instructions and a column's v^H a is an in-lane
+// sum over RL rows plus the exchange levels over the 64 / RL row blocks.
+// Step j: every wave applies reflector j (v, tau from LDS) to its columns
+// right of j; the wave owning column j + 1 then forms reflector j + 1 (every
+// lane of it computes the scalars; no lane-serial section) and publishes it:
+// one barrier per reflector.  Q = H_1 ... H_k [I; 0] is accumulated
+// afterwards without barriers (all reflectors are in LDS), each wave on its
+// own columns.  The reflectors are qr_factor_kernel's (v(0) = 1,
+// beta = -sign(Re alpha) ||(alpha, x)||, H^H applied from the left).
+// RL = 4 halves each wave's work per step and doubles the waves; it is used
+// while the workgroup stays within 1024 threads.
 // ---------------------------------------------------------------------------
-constexpr int QS_THREADS = 1024;
-constexpr int QS_ELEMS = 8192;  // complex entries of A and Q together (128 KiB)
-constexpr int QS_MAX_K = 64;    // m (n + k) <= QS_ELEMS and k = min(m, n) give k <= 64
+constexpr int QW_MAX = 64;  // m, n <= 64
+constexpr int QW_PROD = 4;  // waves that absorb R into the neighbour
 
-// Sum over the 64 lanes, every lane receiving it: quad, half-row and row
-// levels by DPP (register latency), then the 32-lane and 64-lane halves.
-__device__ __forceinline__ double wave_sum_dpp(double v) {
-  v += dpp_d<0xB1>(v);   // quad_perm [1,0,3,2]: l ^ 1
-  v += dpp_d<0x4E>(v);   // quad_perm [2,3,0,1]: l ^ 2
-  v += dpp_d<0x141>(v);  // row_half_mirror: the other quad of the 8
-  v += dpp_d<0x140>(v);  // row_mirror: the other 8 of the 16
-  v += swz16_d(v);       // l ^ 16
-  v += __shfl_xor(v, 32, 64);
-  return v;
+// Row i of a reflector sits at (i % RL) * (64 / RL) + i / RL: the row blocks a
+// wave reads with one instruction are then adjacent in LDS.
+
+// x(l) + x(l ^ 16) and x(l) + x(l ^ 32) in every lane, on the VALU
+// (permlane swaps of the 16-lane rows and the 32-lane halves): both lanes of
+// a pair add the same two values in the same order.
+__device__ __forceinline__ double xsum16_d(double x) {
+  const long long b = __double_as_longlong(x);
+  const unsigned lo = (unsigned)(b & 0xffffffffll), hi = (unsigned)(b >> 32);
+  const auto pl = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+  const auto ph = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+  const double e = __longlong_as_double(((long long)ph[0] << 32) | (long long)pl[0]);
+  const double o = __longlong_as_double(((long long)ph[1] << 32) | (long long)pl[1]);
+  return e + o;
 }
 
-struct QsScal {
-  double tr, ti;  // tau
-  double sr, si;  // 1 / (alpha - beta)
-  double beta;
-  int ident;  // H = I
+__device__ __forceinline__ double xsum32_d(double x) {
+  const long long b = __double_as_longlong(x);
+  const unsigned lo = (unsigned)(b & 0xffffffffll), hi = (unsigned)(b >> 32);
+  const auto pl = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+  const auto ph = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+  const double e = __longlong_as_double(((long long)ph[0] << 32) | (long long)pl[0]);
+  const double o = __longlong_as_double(((long long)ph[1] << 32) | (long long)pl[1]);
+  return e + o;
+}
+
+// Sum over the row blocks of a column: lanes l, l + RL, l + 2 RL, ... of the
+// 16-lane row by rotations (row_ror:8, and row_ror:4 first when RL = 4),
+// then across rows and halves.
+template <int RL>
+__device__ __forceinline__ double qw_rowsum(double v) {
+  if (RL == 4) v += dpp_d<0x124>(v);  // row_ror:4
+  v += dpp_d<0x128>(v);                // row_ror:8
+  return xsum32_d(xsum16_d(v));
+}
+
+__device__ __forceinline__ double readlane_d(double x, int src) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), src);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// v^H a over this lane's RL rows (two accumulator chains), then the column sum
+template <int RL>
+__device__ __forceinline__ void qw_dot(const double (&vr)[RL], const double (&vi)[RL], const double (&xr)[RL],
+                                       const double (&xi)[RL], double& dr, double& di) {
+  constexpr int H = RL / 2;
+  double r0 = 0.0, i0 = 0.0, r1 = 0.0, i1 = 0.0;
+#pragma unroll
+  for (int r = 0; r < H; ++r) {
+    r0 = fma(vr[r], xr[r], fma(vi[r], xi[r], r0));
+    i0 = fma(vr[r], xi[r], fma(-vi[r], xr[r], i0));
+    r1 = fma(vr[r + H], xr[r + H], fma(vi[r + H], xi[r + H], r1));
+    i1 = fma(vr[r + H], xi[r + H], fma(-vi[r + H], xr[r + H], i1));
+  }
+  dr = qw_rowsum<RL>(r0 + r1);
+  di = qw_rowsum<RL>(i0 + i1);
+}
+
+// x -= v f over this lane's RL rows
+template <int RL>
+__device__ __forceinline__ void qw_axpy(const double (&vr)[RL], const double (&vi)[RL], double fr, double fi,
+                                        double (&xr)[RL], double (&xi)[RL]) {
+#pragma unroll
+  for (int r = 0; r < RL; ++r) {
+    xr[r] = fma(-vr[r], fr, fma(vi[r], fi, xr[r]));
+    xi[r] = fma(-vr[r], fi, fma(-vi[r], fr, xi[r]));
+  }
+}
+
+template <int RL>
+__device__ __forceinline__ void qw_load_v(const double2* Vj, int rb, double (&vr)[RL], double (&vi)[RL]) {
+#pragma unroll
+  for (int r = 0; r < RL; ++r) {
+    const double2 x = Vj[r * (64 / RL) + rb];  // row RL rb + r
+    vr[r] = x.x;
+    vi[r] = x.y;
+  }
+}
+
+// Reflector of column jj, called by the whole wave that owns it (wave jj / RL;
+// jj uniform); on return that column's registers hold R(:, jj) (beta on the
+// diagonal, zeros below) and V / T hold v and tau.  Branch-free apart from
+// the stores.
+template <int RL>
+__device__ __forceinline__ void qw_reflector(double (&ar)[RL], double (&ai)[RL], int jj, int cl, int rb,
+                                             double2 (*V)[QW_MAX], double2* T) {
+  constexpr int H = RL / 2;
+  const int jr = jj % RL;
+  double p0 = 0.0, p1 = 0.0;
+#pragma unroll
+  for (int r = 0; r < H; ++r) {
+    const double t0 = fma(ar[r], ar[r], ai[r] * ai[r]);
+    const double t1 = fma(ar[r + H], ar[r + H], ai[r + H] * ai[r + H]);
+    p0 += (RL * rb + r > jj) ? t0 : 0.0;
+    p1 += (RL * rb + r + H > jj) ? t1 : 0.0;
+  }
+  const double part = qw_rowsum<RL>(p0 + p1);
+  double sr = ar[0], si = ai[0];
+#pragma unroll
+  for (int r = 1; r < RL; ++r) {
+    sr = (r == jr) ? ar[r] : sr;
+    si = (r == jr) ? ai[r] : si;
+  }
+  const double xn = readlane_d(part, jr);  // ||x||^2 of column jj (lane jr: row block 0 of that column)
+  const double alr = readlane_d(sr, jj), ali = readlane_d(si, jj);  // A(jj, jj) sits in lane jj
+  double tr = 0.0, ti = 0.0, scr = 0.0, sci = 0.0, beta = alr;
+  if (!(xn == 0.0 && ali == 0.0)) {
+    const double nrm = sqrt(fma(alr, alr, fma(ali, ali, xn)));
+    beta = (alr >= 0.0) ? -nrm : nrm;
+    const double dr = alr - beta, di = ali;
+    const double ib = 1.0 / beta, iden = 1.0 / fma(dr, dr, di * di);
+    tr = (beta - alr) * ib;
+    ti = -ali * ib;
+    scr = dr * iden;  // 1 / (alpha - beta)
+    sci = -di * iden;
+  }
+  const bool own = cl == jr;
+  double2 v[RL];
+#pragma unroll
+  for (int r = 0; r < RL; ++r) {
+    const int i = RL * rb + r;
+    const bool below = i > jj, diag = i == jj;
+    v[r].x = below ? fma(ar[r], scr, -ai[r] * sci) : (diag ? 1.0 : 0.0);
+    v[r].y = below ? fma(ar[r], sci, ai[r] * scr) : 0.0;
+    ar[r] = (own && below) ? 0.0 : ((own && diag) ? beta : ar[r]);
+    ai[r] = (own && (below || diag)) ? 0.0 : ai[r];
+  }
+  if (own) {
+#pragma unroll
+    for (int r = 0; r < RL; ++r) V[jj][r * (64 / RL) + rb] = v[r];
+    if (rb == 0) {
+      double2 t;
+      t.x = tr;
+      t.y = ti;
+      T[jj] = t;
+    }
+  }
+}
+
+// Neighbour absorption (generic_canonize_site!'s R = binary_einsum(R, B)):
+// side 0: Bo (k x nb) = R B with B n x nb; side 1: Bo (nb x k) = B R^H with
+// B nb x n.  B == nullptr: none.
+template <typename S2>
+struct QwAbsorb {
+  const S2* B;
+  S2* Bo;
+  int64_t ldb, ldbo;
+  int nb, side;
 };
 
-// Reflector of column j (rows j..m-1), one wave; lane 0 writes *out.
-__device__ __forceinline__ void qs_reflector(const double2* A, int m, int j, int lane, QsScal* out) {
-  double part = 0.0;
-  for (int i = j + 1 + lane; i < m; i += 64) {
-    const double2 x = A[(size_t)j * m + i];
-    part += x.x * x.x + x.y * x.y;
-  }
-  part = wave_sum_dpp(part);
-  if (lane == 0) {
-    const double2 a = A[(size_t)j * m + j];
-    const double ar = a.x, ai = a.y;
-    QsScal o;
-    o.tr = 0.0;
-    o.ti = 0.0;
-    o.sr = 0.0;
-    o.si = 0.0;
-    o.beta = ar;
-    o.ident = 1;
-    if (!(part == 0.0 && ai == 0.0)) {
-      const double nrm = sqrt(ar * ar + ai * ai + part);
-      const double beta = (ar >= 0.0) ? -nrm : nrm;
-      o.tr = (beta - ar) / beta;
-      o.ti = -ai / beta;
-      const double dr = ar - beta, di = ai;
-      const double den = dr * dr + di * di;
-      o.sr = dr / den;
-      o.si = -di / den;
-      o.beta = beta;
-      o.ident = 0;
+// Bo from R (Rs[c * 64 + q] = R(q, c), zero below the diagonal), by the
+// QW_PROD product waves (thread pt of them) while the column waves form Q.
+// A thread owns one neighbour index p and eight consecutive q: each B
+// element it loads serves eight outputs.
+template <typename S2>
+__device__ __forceinline__ void qw_absorb(const QwAbsorb<S2>& ab, const double2* Rs, int n, int k, int pt) {
+  const int nb = ab.nb, total = nb * ((k + 7) >> 3);
+  for (int e = pt; e < total; e += 64 * QW_PROD) {
+    const int p = e % nb, q0 = 8 * (e / nb);
+    double sr[8], si[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      sr[t] = 0.0;
+      si[t] = 0.0;
     }
-    *out = o;
+    // rows q0 + t > c of R are zero; rows >= k are not stored below
+    if (ab.side == 0) {  // Bo(q, p) = sum_c R(q, c) B(c, p)
+      const S2* b = ab.B + (size_t)p * ab.ldb;
+#pragma unroll 4
+      for (int c = q0; c < n; ++c) {
+        const S2 x = b[c];
+        const double xr = x.x, xi = x.y;
+        const double2* r = Rs + c * QW_MAX + q0;
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+          const double2 y = r[t];
+          sr[t] = fma(y.x, xr, fma(-y.y, xi, sr[t]));
+          si[t] = fma(y.x, xi, fma(y.y, xr, si[t]));
+        }
+      }
+    } else {  // Bo(p, q) = sum_c B(p, c) conj(R(q, c))
+      const S2* b = ab.B + p;
+#pragma unroll 4
+      for (int c = q0; c < n; ++c) {
+        const S2 x = b[(size_t)c * ab.ldb];
+        const double xr = x.x, xi = x.y;
+        const double2* r = Rs + c * QW_MAX + q0;
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+          const double2 y = r[t];
+          sr[t] = fma(xr, y.x, fma(xi, y.y, sr[t]));
+          si[t] = fma(xi, y.x, fma(-xr, y.y, si[t]));
+        }
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      if (q0 + t < k) {
+        S2 o;
+        o.x = sr[t];
+        o.y = si[t];
+        if (ab.side == 0)
+          ab.Bo[(size_t)p * ab.ldbo + q0 + t] = o;
+        else
+          ab.Bo[(size_t)(q0 + t) * ab.ldbo + p] = o;
+      }
+    }
   }
 }
 
-template <typename S2>
-__global__ __launch_bounds__(QS_THREADS) void qr_small_kernel(const double* __restrict__ W, int m, int n, void* Qout,
-                                                              int64_t ldq, void* Rout, int64_t ldr, int flags) {
-  __shared__ double2 A[QS_ELEMS];  // A (ld = m), then Q (ld = m)
-  __shared__ QsScal sc[QS_MAX_K];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+template <typename S2, int RL>
+__global__ __launch_bounds__(RL == 4 ? 1024 : 768) void qr_reg_kernel(const S2* __restrict__ A, int64_t lda, int m,
+                                                                      int n, S2* __restrict__ Qo, int64_t ldq,
+                                                                      S2* __restrict__ Ro, int64_t ldr, int flags,
+                                                                      const QwAbsorb<S2> ab) {
+  __shared__ double2 V[QW_MAX][QW_MAX];    // V[j][(i % RL) * (64 / RL) + i / RL] = v_j(i)
+  __shared__ double2 T[QW_MAX];            // tau_j
+  __shared__ double2 Rs[QW_MAX * QW_MAX];  // R for the absorption
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int ncw = (n + RL - 1) / RL;  // column waves; the QW_PROD waves after them absorb R
+  const int cl = lane % RL, rb = lane / RL;
+  const int c = RL * wave + cl;  // this lane's column; rows RL rb + r
   const int k = min(m, n);
-  double2* Q = A + (size_t)m * n;
-  const double2* W2 = reinterpret_cast<const double2*>(W);
-  for (int e = tid; e < m * n; e += QS_THREADS) A[e] = W2[e];
-  __syncthreads();
-  if (wave == 0) qs_reflector(A, m, 0, lane, &sc[0]);
+  const bool in_adj = (flags & TENET_QR_IN_ADJ) != 0;
+  double ar[RL], ai[RL];
+#pragma unroll
+  for (int r = 0; r < RL; ++r) {
+    const int i = RL * rb + r;
+    ar[r] = 0.0;
+    ai[r] = 0.0;
+    if (i < m && c < n) {
+      if (in_adj) {  // element (i, c) of A^H: conj(A(c, i))
+        const S2 x = A[(size_t)i * lda + c];
+        ar[r] = x.x;
+        ai[r] = -(double)x.y;
+      } else {
+        const S2 x = A[(size_t)c * lda + i];
+        ar[r] = x.x;
+        ai[r] = x.y;
+      }
+    }
+  }
+  if (wave == 0) qw_reflector<RL>(ar, ai, 0, cl, rb, V, T);
   __syncthreads();
   for (int j = 0; j < k; ++j) {
-    if (j > 0 && !sc[j - 1].ident) {
-      // column j - 1: beta on the diagonal, the scaled vector below
-      const QsScal p = sc[j - 1];
-      for (int i = j - 1 + tid; i < m; i += QS_THREADS) {
-        double2 x = A[(size_t)(j - 1) * m + i];
-        if (i == j - 1) {
-          x.x = p.beta;
-          x.y = 0.0;
-        } else {
-          const double xr = x.x, xi = x.y;
-          x.x = xr * p.sr - xi * p.si;
-          x.y = xr * p.si + xi * p.sr;
-        }
-        A[(size_t)(j - 1) * m + i] = x;
-      }
-    }
-    const QsScal r = sc[j];
-    bool next_done = false;
-    if (!r.ident) {
-      // A(j:, c) -= conj(tau) v (v^H A(j:, c)), v = [1; x / (alpha - beta)]
-      for (int c = j + 1 + wave; c < n; c += 2 * (QS_THREADS / 64)) {
-        const int c2 = c + QS_THREADS / 64;
-        const bool two = c2 < n;
-        double d1r = 0.0, d1i = 0.0, d2r = 0.0, d2i = 0.0;
-        for (int i = j + lane; i < m; i += 64) {
-          double vr = 1.0, vi = 0.0;
-          if (i != j) {
-            const double2 x = A[(size_t)j * m + i];
-            vr = x.x * r.sr - x.y * r.si;
-            vi = x.x * r.si + x.y * r.sr;
-          }
-          const double2 a1 = A[(size_t)c * m + i];
-          d1r += vr * a1.x + vi * a1.y;
-          d1i += vr * a1.y - vi * a1.x;
-          if (two) {
-            const double2 a2 = A[(size_t)c2 * m + i];
-            d2r += vr * a2.x + vi * a2.y;
-            d2i += vr * a2.y - vi * a2.x;
-          }
-        }
-        d1r = wave_sum_dpp(d1r);
-        d1i = wave_sum_dpp(d1i);
-        d2r = wave_sum_dpp(d2r);
-        d2i = wave_sum_dpp(d2i);
-        const double f1r = r.tr * d1r + r.ti * d1i, f1i = r.tr * d1i - r.ti * d1r;  // conj(tau) * dot
-        const double f2r = r.tr * d2r + r.ti * d2i, f2i = r.tr * d2i - r.ti * d2r;
-        for (int i = j + lane; i < m; i += 64) {
-          double vr = 1.0, vi = 0.0;
-          if (i != j) {
-            const double2 x = A[(size_t)j * m + i];
-            vr = x.x * r.sr - x.y * r.si;
-            vi = x.x * r.si + x.y * r.sr;
-          }
-          double2 a1 = A[(size_t)c * m + i];
-          a1.x -= vr * f1r - vi * f1i;
-          a1.y -= vr * f1i + vi * f1r;
-          A[(size_t)c * m + i] = a1;
-          if (two) {
-            double2 a2 = A[(size_t)c2 * m + i];
-            a2.x -= vr * f2r - vi * f2i;
-            a2.y -= vr * f2i + vi * f2r;
-            A[(size_t)c2 * m + i] = a2;
-          }
-        }
-        if (wave == 0 && !next_done && j + 1 < k) {
-          // column j + 1 (this wave's first) is final: the next reflector
-          qs_reflector(A, m, j + 1, lane, &sc[j + 1]);
-          next_done = true;
-        }
-      }
-    }
-    if (wave == 0 && !next_done && j + 1 < k) qs_reflector(A, m, j + 1, lane, &sc[j + 1]);
-    __syncthreads();
-  }
-  if (k > 0 && !sc[k - 1].ident) {
-    const QsScal p = sc[k - 1];
-    for (int i = k - 1 + tid; i < m; i += QS_THREADS) {
-      double2 x = A[(size_t)(k - 1) * m + i];
-      if (i == k - 1) {
-        x.x = p.beta;
-        x.y = 0.0;
-      } else {
-        const double xr = x.x, xi = x.y;
-        x.x = xr * p.sr - xi * p.si;
-        x.y = xr * p.si + xi * p.sr;
-      }
-      A[(size_t)(k - 1) * m + i] = x;
-    }
-  }
-  // Q = H_1 ... H_k [I; 0], backwards
-  for (int e = tid; e < m * k; e += QS_THREADS) {
-    double2 z;
-    z.x = (e % m == e / m) ? 1.0 : 0.0;
-    z.y = 0.0;
-    Q[e] = z;
-  }
-  __syncthreads();
-  for (int j = k - 1; j >= 0; --j) {
-    const QsScal r = sc[j];
-    if (r.ident) continue;  // uniform
-    // Q(j:, c) -= tau v (v^H Q(j:, c)) for c = j..k-1 (columns < j are still e_c)
-    for (int c = j + wave; c < k; c += 2 * (QS_THREADS / 64)) {
-      const int c2 = c + QS_THREADS / 64;
-      const bool two = c2 < k;
-      double d1r = 0.0, d1i = 0.0, d2r = 0.0, d2i = 0.0;
-      for (int i = j + lane; i < m; i += 64) {
-        double vr = 1.0, vi = 0.0;
-        if (i != j) {
-          const double2 x = A[(size_t)j * m + i];
-          vr = x.x;
-          vi = x.y;
-        }
-        const double2 q1 = Q[(size_t)c * m + i];
-        d1r += vr * q1.x + vi * q1.y;
-        d1i += vr * q1.y - vi * q1.x;
-        if (two) {
-          const double2 q2 = Q[(size_t)c2 * m + i];
-          d2r += vr * q2.x + vi * q2.y;
-          d2i += vr * q2.y - vi * q2.x;
-        }
-      }
-      d1r = wave_sum_dpp(d1r);
-      d1i = wave_sum_dpp(d1i);
-      d2r = wave_sum_dpp(d2r);
-      d2i = wave_sum_dpp(d2i);
-      const double f1r = r.tr * d1r - r.ti * d1i, f1i = r.tr * d1i + r.ti * d1r;  // tau * dot
-      const double f2r = r.tr * d2r - r.ti * d2i, f2i = r.tr * d2i + r.ti * d2r;
-      for (int i = j + lane; i < m; i += 64) {
-        double vr = 1.0, vi = 0.0;
-        if (i != j) {
-          const double2 x = A[(size_t)j * m + i];
-          vr = x.x;
-          vi = x.y;
-        }
-        double2 q1 = Q[(size_t)c * m + i];
-        q1.x -= vr * f1r - vi * f1i;
-        q1.y -= vr * f1i + vi * f1r;
-        Q[(size_t)c * m + i] = q1;
-        if (two) {
-          double2 q2 = Q[(size_t)c2 * m + i];
-          q2.x -= vr * f2r - vi * f2i;
-          q2.y -= vr * f2i + vi * f2r;
-          Q[(size_t)c2 * m + i] = q2;
-        }
-      }
+    if (wave < ncw && RL * wave + RL - 1 > j) {  // a column wave with columns right of j (uniform)
+      const double2 t = T[j];
+      double vr[RL], vi[RL], dr, di;
+      qw_load_v<RL>(V[j], rb, vr, vi);
+      qw_dot<RL>(vr, vi, ar, ai, dr, di);
+      if (c > j && c < n) qw_axpy<RL>(vr, vi, t.x * dr + t.y * di, t.x * di - t.y * dr, ar, ai);  // conj(tau) v^H a
+      if (j + 1 < k && wave == (j + 1) / RL) qw_reflector<RL>(ar, ai, j + 1, cl, rb, V, T);
     }
     __syncthreads();
   }
-  // outputs: Q (m x k) and R (k x n, upper triangle of A), adjoints on request
+  // R (k x n): rows i < k of the registers (zero below the diagonal)
   const bool qadj = (flags & TENET_QR_Q_ADJ) != 0, radj = (flags & TENET_QR_R_ADJ) != 0;
-  S2* Qo = static_cast<S2*>(Qout);
-  S2* Ro = static_cast<S2*>(Rout);
-  for (int e = tid; e < m * k; e += QS_THREADS) {
-    const int r = e % m, c = e / m;
-    const double2 v = Q[e];
-    S2 o;
-    o.x = v.x;
-    o.y = qadj ? -v.y : v.y;
-    if (qadj)
-      Qo[(size_t)r * ldq + c] = o;
-    else
-      Qo[(size_t)c * ldq + r] = o;
+  if (ab.B != nullptr) {  // uniform: R to LDS, then the product waves absorb it
+    if (c < n) {
+#pragma unroll
+      for (int r = 0; r < RL; ++r) {
+        double2 x;
+        x.x = ar[r];
+        x.y = ai[r];
+        if (RL * rb + r < k) Rs[c * QW_MAX + RL * rb + r] = x;
+      }
+    }
+    __syncthreads();
+    if (wave >= ncw) {
+      qw_absorb(ab, Rs, n, k, tid - 64 * ncw);
+      return;
+    }
   }
-  for (int e = tid; e < k * n; e += QS_THREADS) {
-    const int r = e % k, c = e / k;
-    double2 v;
-    v.x = 0.0;
-    v.y = 0.0;
-    if (r <= c) v = A[(size_t)c * m + r];
-    S2 o;
-    o.x = v.x;
-    o.y = radj ? -v.y : v.y;
-    if (radj)
-      Ro[(size_t)r * ldr + c] = o;
-    else
-      Ro[(size_t)c * ldr + r] = o;
+  if (Ro != nullptr && c < n) {
+#pragma unroll
+    for (int r = 0; r < RL; ++r) {
+      const int i = RL * rb + r;
+      if (i < k) {
+        S2 o;
+        o.x = ar[r];
+        o.y = radj ? -ai[r] : ai[r];
+        if (radj)
+          Ro[(size_t)i * ldr + c] = o;
+        else
+          Ro[(size_t)c * ldr + i] = o;
+      }
+    }
+  }
+  // Q = H_1 ... H_k [I; 0], backwards; a column left of j is still e_c and
+  // v_j vanishes above row j, so H_j leaves it alone.  The next reflector is
+  // read while the current one is applied.
+  if (RL * wave >= k) return;  // no barrier follows
+  double qr[RL], qi[RL];
+#pragma unroll
+  for (int r = 0; r < RL; ++r) {
+    qr[r] = (RL * rb + r == c) ? 1.0 : 0.0;
+    qi[r] = 0.0;
+  }
+  int j = min(k - 1, RL * wave + RL - 1);
+  double vr[RL], vi[RL];
+  qw_load_v<RL>(V[j], rb, vr, vi);
+  for (; j >= 0; --j) {
+    const double2 t = T[j];
+    double nr[RL], ni[RL], dr, di;
+    qw_load_v<RL>(V[j > 0 ? j - 1 : 0], rb, nr, ni);
+    qw_dot<RL>(vr, vi, qr, qi, dr, di);
+    qw_axpy<RL>(vr, vi, t.x * dr - t.y * di, t.x * di + t.y * dr, qr, qi);  // tau v^H q_c
+#pragma unroll
+    for (int r = 0; r < RL; ++r) {
+      vr[r] = nr[r];
+      vi[r] = ni[r];
+    }
+  }
+  if (c < k) {
+#pragma unroll
+    for (int r = 0; r < RL; ++r) {
+      const int i = RL * rb + r;
+      if (i < m) {
+        S2 o;
+        o.x = qr[r];
+        o.y = qadj ? -qi[r] : qi[r];
+        if (qadj)
+          Qo[(size_t)i * ldq + c] = o;
+        else
+          Qo[(size_t)c * ldq + i] = o;
+      }
+    }
   }
 }
 
-// one row per lane and at most two column pairs per wave and step; taller or
-// wider matrices are faster on the blocked path
-bool qr_small_fits(int64_t m, int64_t n) {
-  const int64_t k = std::min(m, n);
-  return m > 0 && n > 0 && m <= 64 && n <= 64 && m * (n + k) <= QS_ELEMS;
+bool qr_small_fits(int64_t m, int64_t n) { return m > 0 && n > 0 && m <= QW_MAX && n <= QW_MAX; }
+
+template <typename S2>
+static void launch_qr_reg(const void* A, int64_t lda, int64_t m, int64_t n, void* Qout, int64_t ldq, void* Rout,
+                          int64_t ldr, int flags, const QrAbsorbArgs* abs, hipStream_t s, hipEvent_t ev0,
+                          hipEvent_t ev1) {
+  QwAbsorb<S2> ab{nullptr, nullptr, 0, 0, 0, 0};
+  if (abs != nullptr) ab = QwAbsorb<S2>{(const S2*)abs->B, (S2*)abs->Bout, abs->ldb, abs->ldbo, (int)abs->nb, abs->side};
+  const int prod = abs != nullptr ? QW_PROD : 0;
+  const int w4 = (int)((n + 3) / 4) + prod;
+  if (w4 <= 16) {  // four rows per lane
+    hipExtLaunchKernelGGL(qr_reg_kernel<S2, 4>, dim3(1), dim3(64 * w4), 0, s, ev0, ev1, 0, (const S2*)A, lda, (int)m,
+                          (int)n, (S2*)Qout, ldq, (S2*)Rout, ldr, flags, ab);
+  } else {
+    const int w8 = (int)((n + 7) / 8) + prod;
+    hipExtLaunchKernelGGL(qr_reg_kernel<S2, 8>, dim3(1), dim3(64 * w8), 0, s, ev0, ev1, 0, (const S2*)A, lda, (int)m,
+                          (int)n, (S2*)Qout, ldq, (S2*)Rout, ldr, flags, ab);
+  }
 }
 
 // ev0 / ev1 (optional): stamped at the kernel's start and end by the dispatch
-int launch_qr_small(const double* W, int64_t m, int64_t n, void* Qout, int64_t ldq, void* Rout, int64_t ldr,
-                    int dtype, int flags, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
+int launch_qr_small(const void* A, int64_t lda, int64_t m, int64_t n, void* Qout, int64_t ldq, void* Rout,
+                    int64_t ldr, int dtype, int flags, const QrAbsorbArgs* abs, hipStream_t s, hipEvent_t ev0,
+                    hipEvent_t ev1) {
   if (!qr_small_fits(m, n)) {
     set_error("tenet_qr_thin: matrix too large for the single-workgroup QR");
     return TENET_ERR_UNSUPPORTED;
   }
   if (dtype == TENET_C128)
-    hipExtLaunchKernelGGL(qr_small_kernel<double2>, dim3(1), dim3(QS_THREADS), 0, s, ev0, ev1, 0, W, (int)m, (int)n,
-                          Qout, ldq, Rout, ldr, flags);
+    launch_qr_reg<double2>(A, lda, m, n, Qout, ldq, Rout, ldr, flags, abs, s, ev0, ev1);
   else
-    hipExtLaunchKernelGGL(qr_small_kernel<float2>, dim3(1), dim3(QS_THREADS), 0, s, ev0, ev1, 0, W, (int)m, (int)n,
-                          Qout, ldq, Rout, ldr, flags);
+    launch_qr_reg<float2>(A, lda, m, n, Qout, ldq, Rout, ldr, flags, abs, s, ev0, ev1);
   TENET_HIP_CHECK(hipGetLastError());
   return TENET_OK;
 }
 
+// W (m x n, FP64 complex, already packed) -> Q, R outputs.  Workspace after W:
+// Q scratch (m x k), tau (k), T blocks.
 int launch_qr_blocked(double* W, int64_t m, int64_t n, void* Qout, int64_t ldq, void* Rout, int64_t ldr, int dtype,
                       int flags, hipStream_t s) {
   const int64_t k = std::min(m, n);

@@ -908,12 +908,10 @@ static int qr_run(tenet_ctx ctx, hipStream_t s, int dtype, int flags, int64_t m,
   const double fl = 4.0 * (4.0 * p * q * q - 4.0 * q * q * q / 3.0);
   const double by = (double)ELEM[dtype] * ((double)m * n + (double)m * k + (double)k * n);
   if (qr_small_fits(m, n)) {
-    // one-workgroup QR: the timing sample is that kernel alone, stamped by its dispatch
-    double* W = (double*)b.take((size_t)m * n * 16);
-    TRY(launch_pack_c128(A, dtype, m, n, lda, (flags & TENET_QR_IN_ADJ) ? 1 : 0, W, s));
+    // one-launch QR reading A in place: the timing sample is that kernel, stamped by its dispatch
     StatScope st(ctx, s, TENET_KSTAT_QR, fl, by, true);
-    const int rc = launch_qr_small(W, m, n, Q, ldq, R, ldr, dtype, flags & (TENET_QR_Q_ADJ | TENET_QR_R_ADJ), s,
-                                   st.start_event(), st.stop_event());
+    const int rc =
+        launch_qr_small(A, lda, m, n, Q, ldq, R, ldr, dtype, flags, nullptr, s, st.start_event(), st.stop_event());
     if (rc == TENET_OK) st.launched();
     return rc;
   }
@@ -953,6 +951,55 @@ extern "C" int tenet_qr_thin_ex(tenet_ctx ctx, tenet_stream stream, int dtype, i
   void* w = nullptr;
   TRY(ws_get(ctx, s, qr_ws_bytes(m, n), &w));
   return qr_run(ctx, s, dtype, flags, m, n, A, lda, Q, ldq, R, ldr, w);
+}
+
+static bool spans_overlap(const void* a, size_t na, const void* b, size_t nb) {
+  const char* pa = static_cast<const char*>(a);
+  const char* pb = static_cast<const char*>(b);
+  return pa < pb + nb && pb < pa + na;
+}
+
+extern "C" int tenet_qr_absorb(tenet_ctx ctx, tenet_stream stream, int dtype, int flags, int64_t m, int64_t n,
+                               const void* A, int64_t lda, void* Q, int64_t ldq, int side, int64_t nb,
+                               const void* B, int64_t ldb, void* Bout, int64_t ldbo) {
+  TRY(use_device(ctx));
+  TRY(valid_dtype_prec(dtype, 0));
+  CHECK_ARG(m > 0 && n > 0 && nb > 0, "empty matrix");
+  CHECK_ARG((flags & ~(TENET_QR_IN_ADJ | TENET_QR_Q_ADJ)) == 0, "flags: only TENET_QR_IN_ADJ and TENET_QR_Q_ADJ");
+  CHECK_ARG(side == TENET_ABSORB_R_B || side == TENET_ABSORB_B_RH, "unknown absorb side");
+  CHECK_ARG(nb <= INT32_MAX / 64, "nb too large");
+  if (!qr_small_fits(m, n)) {
+    set_error("tenet_qr_absorb: m, n <= 64 only (use tenet_qr_thin_ex and tenet_einsum)");
+    return TENET_ERR_UNSUPPORTED;
+  }
+  const int64_t k = std::min(m, n);
+  const bool in_adj = flags & TENET_QR_IN_ADJ, q_adj = flags & TENET_QR_Q_ADJ;
+  const bool rb = side == TENET_ABSORB_R_B;
+  CHECK_ARG(lda >= (in_adj ? n : m), "lda too small");
+  CHECK_ARG(ldq >= (q_adj ? k : m), "ldq too small");
+  CHECK_ARG(ldb >= (rb ? n : nb), "ldb too small");
+  CHECK_ARG(ldbo >= (rb ? k : nb), "ldbo too small");
+  const size_t es = ELEM[dtype];
+  const size_t sa = (in_adj ? ((size_t)(m - 1) * lda + n) : ((size_t)(n - 1) * lda + m)) * es;
+  const size_t sq = (q_adj ? ((size_t)(m - 1) * ldq + k) : ((size_t)(k - 1) * ldq + m)) * es;
+  const size_t sb = (rb ? ((size_t)(nb - 1) * ldb + n) : ((size_t)(n - 1) * ldb + nb)) * es;
+  const size_t so = (rb ? ((size_t)(nb - 1) * ldbo + k) : ((size_t)(k - 1) * ldbo + nb)) * es;
+  TRY(check_span(ctx, A, sa, "A"));
+  TRY(check_span(ctx, Q, sq, "Q"));
+  TRY(check_span(ctx, B, sb, "B"));
+  TRY(check_span(ctx, Bout, so, "Bout"));
+  CHECK_ARG(!spans_overlap(Bout, so, A, sa) && !spans_overlap(Bout, so, Q, sq) && !spans_overlap(Bout, so, B, sb),
+            "Bout overlaps A, Q or B");
+  hipStream_t s = pick(ctx, stream);
+  const double p = (double)std::max(m, n), q = (double)k;
+  const double fl = 4.0 * (4.0 * p * q * q - 4.0 * q * q * q / 3.0) + 8.0 * (double)nb * q * (double)n;
+  const double by = (double)es * ((double)m * n + (double)m * k + 2.0 * (double)nb * n);
+  const QrAbsorbArgs abs{B, Bout, ldb, ldbo, nb, side};
+  StatScope st(ctx, s, TENET_KSTAT_QR, fl, by, true);
+  const int rc =
+      launch_qr_small(A, lda, m, n, Q, ldq, nullptr, 0, dtype, flags, &abs, s, st.start_event(), st.stop_event());
+  if (rc == TENET_OK) st.launched();
+  return rc;
 }
 
 extern "C" int tenet_qr_thin(tenet_ctx ctx, tenet_stream stream, int dtype, int64_t m, int64_t n, const void* A,

@@ -31,6 +31,7 @@ C128, C64 = 0, 1
 PREC_NATIVE, PREC_FP32, PREC_SPLIT_BF16, PREC_SPLIT_FP16 = 0, 1, 2, 3
 ABSORB_NONE, ABSORB_EQUAL, ABSORB_LEFT, ABSORB_RIGHT = 0, 1, 2, 3
 QR_IN_ADJ, QR_Q_ADJ, QR_R_ADJ = 1, 2, 4
+ABSORB_R_B, ABSORB_B_RH = 0, 1
 KSTAT_OFF, KSTAT_SVD_ROUND, KSTAT_GEMM, KSTAT_QR, KSTAT_AMPLITUDE = 0, 1, 2, 3, 4
 MAX_RANK = 8
 
@@ -99,6 +100,7 @@ SIGNATURES = {
     "tenet_einsum": [_p, _p, _i, _i, C.POINTER(TDesc), _p, C.POINTER(TDesc), _p, C.POINTER(TDesc), _p, _i, _i],
     "tenet_qr_thin": [_p, _p, _i, _i64, _i64, _p, _i64, _p, _i64, _p, _i64],
     "tenet_qr_thin_ex": [_p, _p, _i, _i, _i64, _i64, _p, _i64, _p, _i64, _p, _i64],
+    "tenet_qr_absorb": [_p, _p, _i, _i, _i64, _i64, _p, _i64, _p, _i64, _i, _i64, _p, _i64, _p, _i64],
     "tenet_svd_thin": [_p, _p, _i, _i64, _i64, _p, _i64, _p, _i64, _p, _p, _i64, _i64, _d, _p, _p],
     "tenet_svd_set_precondition": [_p, _i],
     "tenet_simple_update": [_p, _p, _i, _i, _i64, _i64, _i64, _i64, _i64, _p, _p, _p, _p, _p, _p, _i64, _d, _i,

@@ -36,6 +36,11 @@ def svd_flops(m, n):
     return 4.0 * (4.0 * p * p * q + 8.0 * p * q * q + 9.0 * q ** 3)
 
 
+def _qr_absorb_fits(m, n):
+    """tenet_qr_absorb's shape range (the single-workgroup QR)."""
+    return m <= 64 and n <= 64
+
+
 def _gate_tensor(U4, d):
     """4x4 gate with site a as the most significant digit, U4[d*oa+ob, d*ia+ib]
     (test/unit/simple_update.jl:19-33) -> column-major G[oa, ob, ia, ib]."""
@@ -152,13 +157,20 @@ class MPS:
         m = chil * d
         k = min(m, chi)
         Q = DeviceArray(self.ctx, (chil, d, k), self.dtype)
-        R = DeviceArray(self.ctx, (k, chi), self.dtype)
-        L.check(L.lib.tenet_qr_thin(self.ctx.h, None, self.code, m, chi, C.c_void_p(A.ptr), m, C.c_void_p(Q.ptr), m,
-                                    C.c_void_p(R.ptr), k), "qr")
         _, d2, chir = B.shape
         self.flops["qr"] += qr_flops(m, chi)
         self.flops["contract"] += 8.0 * k * chi * d2 * chir
-        self.sites[i + 1] = self._einsum(R, [1, 2], B, [2, 3, 4], (k, d2, chir), [1, 3, 4])
+        if _qr_absorb_fits(m, chi):  # QR and R's contraction into B in one launch
+            Bn = DeviceArray(self.ctx, (k, d2, chir), self.dtype)
+            L.check(L.lib.tenet_qr_absorb(self.ctx.h, None, self.code, 0, m, chi, C.c_void_p(A.ptr), m,
+                                          C.c_void_p(Q.ptr), m, L.ABSORB_R_B, d2 * chir, C.c_void_p(B.ptr), chi,
+                                          C.c_void_p(Bn.ptr), k), "qr_absorb")
+            self.sites[i + 1] = Bn
+        else:
+            R = DeviceArray(self.ctx, (k, chi), self.dtype)
+            L.check(L.lib.tenet_qr_thin(self.ctx.h, None, self.code, m, chi, C.c_void_p(A.ptr), m,
+                                        C.c_void_p(Q.ptr), m, C.c_void_p(R.ptr), k), "qr")
+            self.sites[i + 1] = self._einsum(R, [1, 2], B, [2, 3, 4], (k, d2, chir), [1, 3, 4])
         self.sites[i] = Q
 
     def _qr_left(self, i):
@@ -173,14 +185,22 @@ class MPS:
         m = d * chir  # rows of A^H
         k = min(m, chi)
         Qh = DeviceArray(self.ctx, (k, d, chir), self.dtype)
-        Rh = DeviceArray(self.ctx, (chi, k), self.dtype)
-        L.check(L.lib.tenet_qr_thin_ex(self.ctx.h, None, self.code, L.QR_IN_ADJ | L.QR_Q_ADJ | L.QR_R_ADJ, m, chi,
-                                       C.c_void_p(A.ptr), chi, C.c_void_p(Qh.ptr), k, C.c_void_p(Rh.ptr), chi),
-                "qr (adjoint)")
         chil2, d2, _ = B.shape
         self.flops["qr"] += qr_flops(m, chi)
         self.flops["contract"] += 8.0 * chil2 * d2 * chi * k
-        self.sites[i - 1] = self._einsum(B, [1, 2, 3], Rh, [3, 4], (chil2, d2, k), [1, 2, 4])
+        if _qr_absorb_fits(m, chi):  # QR of A^H and B R^H in one launch
+            Bn = DeviceArray(self.ctx, (chil2, d2, k), self.dtype)
+            L.check(L.lib.tenet_qr_absorb(self.ctx.h, None, self.code, L.QR_IN_ADJ | L.QR_Q_ADJ, m, chi,
+                                          C.c_void_p(A.ptr), chi, C.c_void_p(Qh.ptr), k, L.ABSORB_B_RH, chil2 * d2,
+                                          C.c_void_p(B.ptr), chil2 * d2, C.c_void_p(Bn.ptr), chil2 * d2),
+                    "qr_absorb (adjoint)")
+            self.sites[i - 1] = Bn
+        else:
+            Rh = DeviceArray(self.ctx, (chi, k), self.dtype)
+            L.check(L.lib.tenet_qr_thin_ex(self.ctx.h, None, self.code, L.QR_IN_ADJ | L.QR_Q_ADJ | L.QR_R_ADJ, m,
+                                           chi, C.c_void_p(A.ptr), chi, C.c_void_p(Qh.ptr), k, C.c_void_p(Rh.ptr),
+                                           chi), "qr (adjoint)")
+            self.sites[i - 1] = self._einsum(B, [1, 2, 3], Rh, [3, 4], (chil2, d2, k), [1, 2, 4])
         self.sites[i] = Qh
 
     def _svd(self, M: DeviceArray, m, n, maxdim=0, threshold=0.0):

@@ -133,6 +133,122 @@ def test_qr_invariants_and_r_diagonal(ctx, m, n):
     assert np.allclose(np.abs(np.diag(r)), np.abs(np.diag(r_ref)), rtol=1e-11, atol=1e-13)
 
 
+@pytest.mark.parametrize("m,n", [(64, 32), (17, 9), (3, 50), (64, 64), (1, 1), (40, 1), (9, 64)])
+@pytest.mark.parametrize("flags", [0, 1, 2, 4, 7])
+@pytest.mark.parametrize("dt", [np.complex128, np.complex64])
+def test_qr_small_flags_strides_dtypes(ctx, m, n, flags, dt):
+    """tenet_qr_thin_ex on the single-workgroup path (m, n <= 64) with every
+    adjoint flag, leading dimensions larger than the matrix and complex64:
+    A^H read in place for QR_IN_ADJ, Q^H / R^H written for QR_Q_ADJ / QR_R_ADJ."""
+    import ctypes as C
+
+    import tenet_hip as T
+    from tenet_hip import _lib as L
+    from tenet_hip.device import DTYPES
+
+    rng = np.random.default_rng(m * 131 + n * 7 + flags)
+    A = O.complex_gaussian(rng, (m, n))
+    k = min(m, n)
+    in_adj, q_adj, r_adj = flags & 1, flags & 2, flags & 4
+    pad = 3
+    src = A.conj().T if in_adj else A  # stored matrix
+    lda = src.shape[0] + pad
+    Ah = np.zeros((lda, src.shape[1]), dtype=dt)
+    Ah[: src.shape[0]] = src
+    qshape = (k, m) if q_adj else (m, k)
+    rshape = (n, k) if r_adj else (k, n)
+    ldq, ldr = qshape[0] + pad, rshape[0] + pad
+    dA = T.DeviceArray.from_host(ctx, Ah)
+    dQ = T.DeviceArray(ctx, (ldq, qshape[1]), dt).zero_()
+    dR = T.DeviceArray(ctx, (ldr, rshape[1]), dt).zero_()
+    L.check(L.lib.tenet_qr_thin_ex(ctx.h, None, DTYPES[np.dtype(dt)], flags, m, n, C.c_void_p(dA.ptr), lda,
+                                   C.c_void_p(dQ.ptr), ldq, C.c_void_p(dR.ptr), ldr), "qr_thin_ex")
+    qh, rh = dQ.to_host(), dR.to_host()
+    assert not np.any(qh[qshape[0]:]) and not np.any(rh[rshape[0]:])  # padding rows untouched
+    q = qh[: qshape[0]].astype(complex)
+    r = rh[: rshape[0]].astype(complex)
+    q = q.conj().T if q_adj else q
+    r = r.conj().T if r_adj else r
+    tol = 1e-13 if dt == np.complex128 else 1e-5
+    a_in = Ah[: src.shape[0]].astype(complex)
+    a_in = a_in.conj().T if in_adj else a_in
+    assert np.linalg.norm(q.conj().T @ q - np.eye(k)) < tol * max(1, k)
+    assert _rel(q @ r, a_in) < tol
+    assert not np.any(np.tril(r, -1))
+    _, r_ref = np.linalg.qr(a_in)
+    assert np.allclose(np.abs(np.diag(r)), np.abs(np.diag(r_ref)), rtol=10 * tol, atol=tol)
+
+
+@pytest.mark.parametrize("m,n,nb", [(64, 32, 64), (2, 2, 8), (17, 9, 5), (64, 64, 130), (8, 40, 3), (1, 1, 1)])
+@pytest.mark.parametrize("side", [0, 1])
+@pytest.mark.parametrize("dt", [np.complex128, np.complex64])
+def test_qr_absorb_matches_qr_then_contraction(ctx, m, n, nb, side, dt):
+    """tenet_qr_absorb = tensor_qr_thin + binary_einsum(R, B) (canonize.jl:58-70)
+    in one launch: Bout equals R B (side 0) or B R^H (side 1) with the R of
+    tenet_qr_thin_ex on the same input, and Q Bout = A B / Bout Q^H = B A."""
+    import ctypes as C
+
+    import tenet_hip as T
+    from tenet_hip import _lib as L
+    from tenet_hip.device import DTYPES
+
+    rng = np.random.default_rng(m * 31 + n * 17 + nb + side)
+    k = min(m, n)
+    code = DTYPES[np.dtype(dt)]
+    tol = 1e-13 if dt == np.complex128 else 1e-5
+    if side == 0:  # right move: A is m x n, Q m x k, B n x nb
+        flags = 0
+        A = O.complex_gaussian(rng, (m, n)).astype(dt)
+        Bm = O.complex_gaussian(rng, (n, nb)).astype(dt)
+        lda, ldq, ldb, ldbo = m, m, n, k
+        qshape, oshape = (m, k), (k, nb)
+    else:  # left move: A^H is factorised (A stored n x m), Q^H written (k x m), B nb x n
+        flags = L.QR_IN_ADJ | L.QR_Q_ADJ
+        A = O.complex_gaussian(rng, (n, m)).astype(dt)
+        Bm = O.complex_gaussian(rng, (nb, n)).astype(dt)
+        lda, ldq, ldb, ldbo = n, k, nb, nb
+        qshape, oshape = (k, m), (nb, k)
+    dA, dB = T.DeviceArray.from_host(ctx, A), T.DeviceArray.from_host(ctx, Bm)
+    dQ, dO = T.DeviceArray(ctx, qshape, dt), T.DeviceArray(ctx, oshape, dt)
+    L.check(L.lib.tenet_qr_absorb(ctx.h, None, code, flags, m, n, C.c_void_p(dA.ptr), lda, C.c_void_p(dQ.ptr), ldq,
+                                  side, nb, C.c_void_p(dB.ptr), ldb, C.c_void_p(dO.ptr), ldbo), "qr_absorb")
+    q, out = dQ.to_host().astype(complex), dO.to_host().astype(complex)
+    # the same factorisation through tenet_qr_thin_ex: its R, contracted on the host
+    rshape = (n, k) if side else (k, n)
+    dQ2, dR2 = T.DeviceArray(ctx, qshape, dt), T.DeviceArray(ctx, rshape, dt)
+    L.check(L.lib.tenet_qr_thin_ex(ctx.h, None, code, flags | (L.QR_R_ADJ if side else 0), m, n, C.c_void_p(dA.ptr),
+                                   lda, C.c_void_p(dQ2.ptr), ldq, C.c_void_p(dR2.ptr), rshape[0]), "qr_thin_ex")
+    r2 = dR2.to_host().astype(complex)
+    assert _rel(dQ2.to_host(), dQ.to_host()) < tol  # same reflectors (rows per lane may differ: rounding only)
+    A64, B64 = A.astype(complex), Bm.astype(complex)
+    if side == 0:
+        assert _rel(out, r2 @ B64) < tol
+        assert _rel(q @ out, A64 @ B64) < 10 * tol
+        assert np.linalg.norm(q.conj().T @ q - np.eye(k)) < tol * max(1, k)
+    else:
+        assert _rel(out, B64 @ r2) < tol  # r2 holds R^H (n x k)
+        assert _rel(out @ q, B64 @ A64) < 10 * tol
+        assert np.linalg.norm(q @ q.conj().T - np.eye(k)) < tol * max(1, k)
+
+
+def test_qr_absorb_rejects_large_and_overlap(ctx):
+    import ctypes as C
+
+    import tenet_hip as T
+    from tenet_hip import _lib as L
+
+    a = T.DeviceArray(ctx, (65, 8)).zero_()
+    q = T.DeviceArray(ctx, (65, 8))
+    b = T.DeviceArray(ctx, (8, 4)).zero_()
+    o = T.DeviceArray(ctx, (8, 4))
+    st = L.lib.tenet_qr_absorb(ctx.h, None, L.C128, 0, 65, 8, C.c_void_p(a.ptr), 65, C.c_void_p(q.ptr), 65, 0, 4,
+                               C.c_void_p(b.ptr), 8, C.c_void_p(o.ptr), 8)
+    assert st == L.TENET_ERR_UNSUPPORTED
+    st = L.lib.tenet_qr_absorb(ctx.h, None, L.C128, 0, 64, 8, C.c_void_p(a.ptr), 65, C.c_void_p(q.ptr), 65, 0, 4,
+                               C.c_void_p(b.ptr), 8, C.c_void_p(b.ptr), 8)
+    assert st == L.TENET_ERR_ARG
+
+
 def test_qr_rank_deficient_integer(ctx):
     """reshape(1:64, 16, 4) has rank 2: Q must still be an isometry."""
     import tenet_hip as T
