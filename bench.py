@@ -765,11 +765,12 @@ def bench_plain(args):
         nr = psi.norm()
         return v, nr
 
+    ctx.stats_sample_every(1)
+    ctx.stats_enable(L.KSTAT_QR)  # during the warmup too: its timing events are created there
     for w in range(args.warmup):
         one(states[w])
     ctx.sync()
-    ctx.stats_sample_every(1)
-    ctx.stats_enable(L.KSTAT_QR)
+    ctx.stats_enable(L.KSTAT_QR)  # counters restart for the timed steps
     t0 = time.perf_counter()
     for s in range(args.steps):
         vals.append(one(states[args.warmup + s]))
@@ -827,8 +828,8 @@ def bench_plain(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default 2; 20 for the plain workload)")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed steps (default 1; 3 for plain)")
     ap.add_argument("--forms", default="fp64,split_bf16")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -840,6 +841,11 @@ def main():
     ap.add_argument("--rates", default="250,500,1000,2000,4000,8000,16000")
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     args = ap.parse_args()
+    latency = args.workload == "plain"  # a ~1 ms step: more samples
+    if args.steps is None:
+        args.steps = 20 if latency else 2
+    if args.warmup is None:
+        args.warmup = 3 if latency else 1
     if args.gpus != 1:
         print("bench.py measures one MI355X: this engine has no multi-GPU path (the TEBD sweep is "
               "sequential in MixedCanonical form); run with --gpus 1", file=sys.stderr)

@@ -147,11 +147,28 @@ int launch_write_scalar_from(const void* env, double* out, int dtype, hipStream_
 //   E'(ra, rb)    = sum_{lb, s} T(lb, s, ra) conj(B(lb, s, rb))  (K = lb*d)
 // then E' is divided by its largest |re| / |im| and the log of that scale is
 // added to out[2], as the multi-launch sweep does.  FP64 accumulation; each
-// thread holds a 2 x 4 block of T and a 2 x 2 block of E'; A, B and T sit in
-// LDS transposed so that the lanes of a wave read consecutive entries.
-constexpr int OS_THREADS = 256;
+// of the 512 threads holds a 2 x 2 block of T and a 2 x 1 block of E' (two
+// waves per SIMD); A, B and T sit in LDS transposed so that the lanes of a
+// wave read consecutive entries, and the next site's loads are in flight
+// while the current one is contracted.
+constexpr int OS_THREADS = 512;
 constexpr int OS_CHI = 32;     // every bond <= 32
 constexpr int OS_SITE = 2048;  // d * chi^2 <= 2048 (A, B and T staged in LDS)
+constexpr int OS_PER = OS_SITE / OS_THREADS;
+
+template <typename S2>
+__device__ __forceinline__ void os_load(const OverlapTab& tab, int i, int tid, S2 (&va)[OS_PER], S2 (&vb)[OS_PER]) {
+  const int d = tab.d;
+  const int na = (int)(tab.chi_a[i] * d * tab.chi_a[i + 1]), nb = (int)(tab.chi_b[i] * d * tab.chi_b[i + 1]);
+  const S2* A = reinterpret_cast<const S2*>(tab.a[i]);
+  const S2* B = reinterpret_cast<const S2*>(tab.b[i]);
+#pragma unroll
+  for (int u = 0; u < OS_PER; ++u) {
+    const int e = tid + u * OS_THREADS;
+    if (e < na) va[u] = A[e];
+    if (e < nb) vb[u] = B[e];
+  }
+}
 
 template <typename S2>
 __global__ __launch_bounds__(OS_THREADS) void overlap_small_kernel(const OverlapTab tab, double* __restrict__ out) {
@@ -162,33 +179,23 @@ __global__ __launch_bounds__(OS_THREADS) void overlap_small_kernel(const Overlap
   const int n = tab.n, d = tab.d;
   const int64_t* chi_a = tab.chi_a;
   const int64_t* chi_b = tab.chi_b;
-  const uint64_t* pa = tab.a;
-  const uint64_t* pb = tab.b;
   if (tid == 0) {
     Er[0][0] = 1.0;
     Ei[0][0] = 0.0;
   }
   double logacc = 0.0;
+  S2 va[OS_PER], vb[OS_PER];
+  os_load(tab, 0, tid, va, vb);
   __syncthreads();
   for (int i = 0; i < n; ++i) {
     const int la = (int)chi_a[i], lb = (int)chi_b[i], ra = (int)chi_a[i + 1], rb = (int)chi_b[i + 1];
-    const S2* A = reinterpret_cast<const S2*>(pa[i]);
-    const S2* B = reinterpret_cast<const S2*>(pb[i]);
     const int na = la * d * ra, nb = lb * d * rb;
     const int N = d * ra, K = lb * d;
-    // all of a thread's loads of both sites in flight before the first store;
     // stored transposed so that the index running across lanes in the
     // products below is the contiguous one (no LDS bank conflicts):
     //   A'[c + N a] = A(a, c),  c = s + d ra     B'[rb + rb_n k] = B(k, rb),  k = lb + lb_n s
-    S2 va[OS_SITE / OS_THREADS], vb[OS_SITE / OS_THREADS];
 #pragma unroll
-    for (int u = 0; u < OS_SITE / OS_THREADS; ++u) {
-      const int e = tid + u * OS_THREADS;
-      if (e < na) va[u] = A[e];
-      if (e < nb) vb[u] = B[e];
-    }
-#pragma unroll
-    for (int u = 0; u < OS_SITE / OS_THREADS; ++u) {
+    for (int u = 0; u < OS_PER; ++u) {
       const int e = tid + u * OS_THREADS;
       if (e < na) {
         const int ia = e % la, ic = e / la;
@@ -202,18 +209,19 @@ __global__ __launch_bounds__(OS_THREADS) void overlap_small_kernel(const Overlap
       }
     }
     __syncthreads();
+    if (i + 1 < n) os_load(tab, i + 1, tid, va, vb);  // in flight during the products
     // T = E^T A: rows lb, columns c = s + d * ra; stored as T'[ra + ra_n (lb + lb_n s)]
     {
-      const int M = lb, mb = (M + 1) / 2, nbk = (N + 3) / 4;
+      const int M = lb, mb = (M + 1) / 2, nbk = (N + 1) / 2;
       for (int blk = tid; blk < mb * nbk; blk += OS_THREADS) {
-        const int r0 = 2 * (blk % mb), c0 = 4 * (blk / mb);
-        double tr[2][4], ti[2][4];
+        const int r0 = 2 * (blk % mb), c0 = 2 * (blk / mb);
+        double tr[2][2], ti[2][2];
 #pragma unroll
         for (int x = 0; x < 2; ++x)
 #pragma unroll
-          for (int y = 0; y < 4; ++y) tr[x][y] = ti[x][y] = 0.0;
+          for (int y = 0; y < 2; ++y) tr[x][y] = ti[x][y] = 0.0;
         for (int a = 0; a < la; ++a) {
-          double er[2], ei[2], xr[4], xi[4];
+          double er[2], ei[2], xr[2], xi[2];
 #pragma unroll
           for (int x = 0; x < 2; ++x) {
             const int r = min(r0 + x, M - 1);
@@ -221,7 +229,7 @@ __global__ __launch_bounds__(OS_THREADS) void overlap_small_kernel(const Overlap
             ei[x] = Ei[a][r];
           }
 #pragma unroll
-          for (int y = 0; y < 4; ++y) {
+          for (int y = 0; y < 2; ++y) {
             const int c = min(c0 + y, N - 1);
             xr[y] = Ar[c + N * a];
             xi[y] = Ai[c + N * a];
@@ -229,15 +237,15 @@ __global__ __launch_bounds__(OS_THREADS) void overlap_small_kernel(const Overlap
 #pragma unroll
           for (int x = 0; x < 2; ++x)
 #pragma unroll
-            for (int y = 0; y < 4; ++y) {
-              tr[x][y] += er[x] * xr[y] - ei[x] * xi[y];
-              ti[x][y] += er[x] * xi[y] + ei[x] * xr[y];
+            for (int y = 0; y < 2; ++y) {
+              tr[x][y] = fma(er[x], xr[y], fma(-ei[x], xi[y], tr[x][y]));
+              ti[x][y] = fma(er[x], xi[y], fma(ei[x], xr[y], ti[x][y]));
             }
         }
 #pragma unroll
         for (int x = 0; x < 2; ++x)
 #pragma unroll
-          for (int y = 0; y < 4; ++y)
+          for (int y = 0; y < 2; ++y)
             if (r0 + x < M && c0 + y < N) {
               const int c = c0 + y, sidx = c % d, rai = c / d;
               const int t = rai + ra * ((r0 + x) + lb * sidx);
@@ -248,35 +256,25 @@ __global__ __launch_bounds__(OS_THREADS) void overlap_small_kernel(const Overlap
     }
     __syncthreads();
     // E'(ra, rb) = sum_k T'[ra + ra_n k] conj(B'[rb + rb_n k]), k = lb + lb_n s
-    const int mb = (ra + 1) / 2, nbk = (rb + 1) / 2;
-    double outr[2][2] = {{0.0, 0.0}, {0.0, 0.0}}, outi[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
-    const bool has = tid < mb * nbk;  // mb * nbk <= 256
-    const int a0 = 2 * (tid % max(mb, 1)), b0 = 2 * (tid / max(mb, 1));
+    const int mb = (ra + 1) / 2;
+    double outr[2] = {0.0, 0.0}, outi[2] = {0.0, 0.0};
+    const bool has = tid < mb * rb;  // mb * rb <= 512
+    const int a0 = 2 * (tid % max(mb, 1)), b0 = tid / max(mb, 1);
     double mx = 0.0;
     if (has) {
       for (int k = 0; k < K; ++k) {
-        double tr[2], ti[2], br[2], bi[2];
+        const double brr = Br[b0 + rb * k], bii = Bi[b0 + rb * k];
 #pragma unroll
-        for (int x = 0; x < 2; ++x) {
-          const int ai = min(a0 + x, ra - 1), bj = min(b0 + x, rb - 1);
-          tr[x] = Tr[ai + ra * k];
-          ti[x] = Ti[ai + ra * k];
-          br[x] = Br[bj + rb * k];
-          bi[x] = Bi[bj + rb * k];
+        for (int x = 0; x < 2; ++x) {  // T * conj(B)
+          const int ai = min(a0 + x, ra - 1);
+          const double trr = Tr[ai + ra * k], tii = Ti[ai + ra * k];
+          outr[x] = fma(trr, brr, fma(tii, bii, outr[x]));
+          outi[x] = fma(tii, brr, fma(-trr, bii, outi[x]));
         }
-#pragma unroll
-        for (int x = 0; x < 2; ++x)
-#pragma unroll
-          for (int y = 0; y < 2; ++y) {  // T * conj(B)
-            outr[x][y] += tr[x] * br[y] + ti[x] * bi[y];
-            outi[x][y] += ti[x] * br[y] - tr[x] * bi[y];
-          }
       }
 #pragma unroll
       for (int x = 0; x < 2; ++x)
-#pragma unroll
-        for (int y = 0; y < 2; ++y)
-          if (a0 + x < ra && b0 + y < rb) mx = fmax(mx, fmax(fabs(outr[x][y]), fabs(outi[x][y])));
+        if (a0 + x < ra) mx = fmax(mx, fmax(fabs(outr[x]), fabs(outi[x])));
     }
     mx = wave_max(mx);
     if (lane == 0) red[wave] = mx;
@@ -288,12 +286,10 @@ __global__ __launch_bounds__(OS_THREADS) void overlap_small_kernel(const Overlap
     if (has) {
 #pragma unroll
       for (int x = 0; x < 2; ++x)
-#pragma unroll
-        for (int y = 0; y < 2; ++y)
-          if (a0 + x < ra && b0 + y < rb) {
-            Er[a0 + x][b0 + y] = outr[x][y] * inv;
-            Ei[a0 + x][b0 + y] = outi[x][y] * inv;
-          }
+        if (a0 + x < ra) {
+          Er[a0 + x][b0] = outr[x] * inv;
+          Ei[a0 + x][b0] = outi[x] * inv;
+        }
     }
     if (m > 0.0) logacc += log(m);
     __syncthreads();

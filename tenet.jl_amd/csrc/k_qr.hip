@@ -291,8 +291,8 @@ constexpr int QP_MAX_ROWS = 1024;  // panel rows held in registers (up to four p
 //   ||x||^2 = e_jj,  v^H a_c = conj(s) e_c + P_c  (v = s x below, 1 on the
 //   diagonal),  v_c^H v = conj(P_c) + s conj(e_c)  (c < jj, for T).
 // The 32 real values of e are transpose-reduced inside each wave (levels
-// pairing lanes l ^ 15, l ^ 7, l ^ 2, l ^ 1 by DPP, l ^ 16 by swizzle, l ^ 32
-// by permute; lane l ends with the wave total of value 16 b3 + 8 b2 + 4 b1 +
+// pairing lanes l ^ 15, l ^ 7, l ^ 2, l ^ 1 by DPP, l ^ 16 and l ^ 32 by
+// permlane swaps; lane l ends with the wave total of value 16 b3 + 8 b2 + 4 b1 +
 // 2 b0 + b4, b_i the bits of l), the four wave totals meet in LDS (double
 // buffered by column parity): one barrier per column.
 template <int CTRL>
@@ -303,11 +303,31 @@ __device__ __forceinline__ double dpp_d(double x) {
   return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
 
-__device__ __forceinline__ double swz16_d(double x) {  // lane l ^ 16 within each 32-lane half
+// Lane exchanges on the VALU (gfx950 permlane swaps of the 16-lane rows and
+// the 32-lane halves).  xpair16_d(a, b): lanes of even rows get a(l) +
+// a(l ^ 16), lanes of odd rows b(l) + b(l ^ 16); xsum16_d / xsum32_d:
+// x(l) + x(l ^ 16) / x(l) + x(l ^ 32).  Both lanes of a pair add the same two
+// values in the same order.
+__device__ __forceinline__ double xpair16_d(double a, double b) {
+  const long long ba = __double_as_longlong(a), bb = __double_as_longlong(b);
+  const auto pl = __builtin_amdgcn_permlane16_swap((unsigned)(ba & 0xffffffffll), (unsigned)(bb & 0xffffffffll),
+                                                   false, false);
+  const auto ph = __builtin_amdgcn_permlane16_swap((unsigned)(ba >> 32), (unsigned)(bb >> 32), false, false);
+  const double e = __longlong_as_double(((long long)ph[0] << 32) | (long long)pl[0]);
+  const double o = __longlong_as_double(((long long)ph[1] << 32) | (long long)pl[1]);
+  return e + o;
+}
+
+__device__ __forceinline__ double xsum16_d(double x) { return xpair16_d(x, x); }
+
+__device__ __forceinline__ double xsum32_d(double x) {
   const long long b = __double_as_longlong(x);
-  const int lo = __builtin_amdgcn_ds_swizzle((int)(b & 0xffffffffll), 0x401F);
-  const int hi = __builtin_amdgcn_ds_swizzle((int)(b >> 32), 0x401F);
-  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+  const unsigned lo = (unsigned)(b & 0xffffffffll), hi = (unsigned)(b >> 32);
+  const auto pl = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+  const auto ph = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+  const double e = __longlong_as_double(((long long)ph[0] << 32) | (long long)pl[0]);
+  const double o = __longlong_as_double(((long long)ph[1] << 32) | (long long)pl[1]);
+  return e + o;
 }
 
 template <int H, int CTRL, int BIT>
@@ -327,13 +347,7 @@ __device__ __forceinline__ void wave_transpose_reduce32(double (&v)[2 * QB], int
   halve_dpp<8, 0x141, 4>(v, lane);   // row_half_mirror: l ^ 7
   halve_dpp<4, 0x4E, 2>(v, lane);    // quad_perm [2,3,0,1]: l ^ 2
   halve_dpp<2, 0xB1, 1>(v, lane);    // quad_perm [1,0,3,2]: l ^ 1
-  {
-    const bool hi = (lane & 16) != 0;
-    const double keep = hi ? v[1] : v[0];
-    const double send = hi ? v[0] : v[1];
-    v[0] = keep + swz16_d(send);
-  }
-  v[0] += __shfl_xor(v[0], 32, 64);
+  v[0] = xsum32_d(xpair16_d(v[0], v[1]));  // l ^ 16 (keeping v[0] on even rows, v[1] on odd), then l ^ 32
 }
 
 constexpr int QP3_THREADS = 256;
@@ -813,29 +827,6 @@ constexpr int QW_PROD = 4;  // waves that absorb R into the neighbour
 
 // Row i of a reflector sits at (i % RL) * (64 / RL) + i / RL: the row blocks a
 // wave reads with one instruction are then adjacent in LDS.
-
-// x(l) + x(l ^ 16) and x(l) + x(l ^ 32) in every lane, on the VALU
-// (permlane swaps of the 16-lane rows and the 32-lane halves): both lanes of
-// a pair add the same two values in the same order.
-__device__ __forceinline__ double xsum16_d(double x) {
-  const long long b = __double_as_longlong(x);
-  const unsigned lo = (unsigned)(b & 0xffffffffll), hi = (unsigned)(b >> 32);
-  const auto pl = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
-  const auto ph = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
-  const double e = __longlong_as_double(((long long)ph[0] << 32) | (long long)pl[0]);
-  const double o = __longlong_as_double(((long long)ph[1] << 32) | (long long)pl[1]);
-  return e + o;
-}
-
-__device__ __forceinline__ double xsum32_d(double x) {
-  const long long b = __double_as_longlong(x);
-  const unsigned lo = (unsigned)(b & 0xffffffffll), hi = (unsigned)(b >> 32);
-  const auto pl = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
-  const auto ph = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
-  const double e = __longlong_as_double(((long long)ph[0] << 32) | (long long)pl[0]);
-  const double o = __longlong_as_double(((long long)ph[1] << 32) | (long long)pl[1]);
-  return e + o;
-}
 
 // Sum over the row blocks of a column: lanes l, l + RL, l + 2 RL, ... of the
 // 16-lane row by rotations (row_ror:8, and row_ror:4 first when RL = 4),
