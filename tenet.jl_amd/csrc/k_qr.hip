@@ -330,6 +330,23 @@ __device__ __forceinline__ double xsum32_d(double x) {
   return e + o;
 }
 
+__device__ __forceinline__ double readlane_d(double x, int src) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), src);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Sum over the 64 lanes, the same value in every lane: quads and the two
+// halves of each 16-lane row by DPP, then the rows and halves of the wave.
+__device__ __forceinline__ double wsum64(double v) {
+  v += dpp_d<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += dpp_d<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp_d<0x141>(v);  // row_half_mirror: the other quad of the 8
+  v += dpp_d<0x140>(v);  // row_mirror: the other 8 of the 16
+  return xsum32_d(xsum16_d(v));
+}
+
 template <int H, int CTRL, int BIT>
 __device__ __forceinline__ void halve_dpp(double (&v)[2 * QB], int lane) {
   const bool hi = (lane & BIT) != 0;
@@ -528,6 +545,175 @@ __global__ __launch_bounds__(QP3_THREADS) void qr_panel_v3_kernel(double* __rest
     const int i = e % QB, j = e / QB;
     Tout[2 * e] = (i < jb && j < jb) ? Tr_s[i][j] : 0.0;
     Tout[2 * e + 1] = (i < jb && j < jb) ? Ti_s[i][j] : 0.0;
+  }
+}
+
+
+// Panel v4 (panel rows <= 256): one wave per panel column (QB waves), lane l
+// of wave c holding rows l + 64 r (r < RL, panel-relative) of column c in
+// registers.  Step j: every wave right of j applies reflector j (v, tau from
+// LDS) to its column, with a 64-lane sum for v^H a_c; the wave of column
+// j + 1 then forms reflector j + 1 and publishes it; the waves left of j form
+// w_c = v_c^H v_j (their registers hold v_c below the diagonal) for T's
+// column j, which the then idle wave j + 1 assembles one step later
+// (T(a, j) = -tau_j sum_{c=a}^{j-1} T(a, c) w_c, T(j, j) = tau_j: zlarft's
+// forward columnwise rule).  One barrier per column; the outputs are those of
+// qr_panel_v3_kernel.
+template <int RL>
+__device__ __forceinline__ void qp4_reflector(double (&ar)[RL], double (&ai)[RL], int jj, int lane,
+                                              double2 (*Vs)[64 * RL], double2* taus, double* __restrict__ tau,
+                                              int j0) {
+  double part = 0.0;
+#pragma unroll
+  for (int r = 0; r < RL; ++r) {
+    const double t = fma(ar[r], ar[r], ai[r] * ai[r]);
+    part += (lane + 64 * r > jj) ? t : 0.0;
+  }
+  const double xn = readlane_d(wsum64(part), 0);
+  const double alr = readlane_d(ar[0], jj), ali = readlane_d(ai[0], jj);  // row jj < 64: lane jj, slot 0
+  double tr = 0.0, ti = 0.0, scr = 0.0, sci = 0.0, beta = alr;
+  if (!(xn == 0.0 && ali == 0.0)) {
+    const double nrm = sqrt(fma(alr, alr, fma(ali, ali, xn)));
+    beta = (alr >= 0.0) ? -nrm : nrm;
+    const double dr = alr - beta, di = ali;
+    const double ib = 1.0 / beta, iden = 1.0 / fma(dr, dr, di * di);
+    tr = (beta - alr) * ib;
+    ti = -ali * ib;
+    scr = dr * iden;  // 1 / (alpha - beta)
+    sci = -di * iden;
+  }
+#pragma unroll
+  for (int r = 0; r < RL; ++r) {
+    const int row = lane + 64 * r;
+    const bool below = row > jj, diag = row == jj;
+    double2 v;
+    v.x = below ? fma(ar[r], scr, -ai[r] * sci) : (diag ? 1.0 : 0.0);
+    v.y = below ? fma(ar[r], sci, ai[r] * scr) : 0.0;
+    ar[r] = below ? v.x : (diag ? beta : ar[r]);
+    ai[r] = below ? v.y : (diag ? 0.0 : ai[r]);
+    Vs[jj & 1][row] = v;  // double buffered: waves may still read reflector jj - 1
+  }
+  if (lane == 0) {
+    double2 t;
+    t.x = tr;
+    t.y = ti;
+    taus[jj] = t;
+    tau[2 * (j0 + jj)] = tr;
+    tau[2 * (j0 + jj) + 1] = ti;
+  }
+}
+
+// T column jj from the w_c = v_c^H v_jj (lanes a <= jj of one wave)
+__device__ __forceinline__ void qp4_tcol(int jj, int lane, const double2* taus, const double2* w,
+                                         double2 (*Ts)[QB]) {
+  if (lane > jj) return;
+  const double2 t = taus[jj];
+  if (lane == jj) {
+    Ts[jj][jj] = t;
+    return;
+  }
+  double sr = 0.0, si = 0.0;
+  for (int c = lane; c < jj; ++c) {  // T(a, c) w_c, a = lane
+    const double2 x = Ts[c][lane], y = w[c];
+    sr = fma(x.x, y.x, fma(-x.y, y.y, sr));
+    si = fma(x.x, y.y, fma(x.y, y.x, si));
+  }
+  double2 o;
+  o.x = -(t.x * sr - t.y * si);
+  o.y = -(t.x * si + t.y * sr);
+  Ts[jj][lane] = o;
+}
+
+template <int RL>
+__global__ __launch_bounds__(64 * QB) void qr_panel_v4_kernel(double* __restrict__ W, double* __restrict__ tau,
+                                                             double* __restrict__ Tout, int m, int j0, int jb) {
+  __shared__ double2 Vs[2][64 * RL];  // reflector j in Vs[j & 1], panel rows
+  __shared__ double2 taus[QB];
+  __shared__ double2 Ws[2][QB];    // w_c = v_c^H v_j by the parity of j
+  __shared__ double2 Ts[QB][QB];   // Ts[c][a] = T(a, c)
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // panel column
+  const int mr = m - j0;
+  double2* Wc = reinterpret_cast<double2*>(W);
+  double ar[RL], ai[RL];
+#pragma unroll
+  for (int r = 0; r < RL; ++r) {
+    const int row = lane + 64 * r;
+    ar[r] = 0.0;
+    ai[r] = 0.0;
+    if (wave < jb && row < mr) {
+      const double2 v = Wc[(size_t)(j0 + wave) * m + j0 + row];
+      ar[r] = v.x;
+      ai[r] = v.y;
+    }
+  }
+  for (int e = tid; e < QB * QB; e += 64 * QB) {
+    double2 z;
+    z.x = 0.0;
+    z.y = 0.0;
+    Ts[e / QB][e % QB] = z;
+  }
+  if (wave == 0) qp4_reflector<RL>(ar, ai, 0, lane, Vs, taus, tau, j0);
+  __syncthreads();
+  for (int j = 0; j < jb; ++j) {
+    const double2 t = taus[j];
+    const double2* Vj = Vs[j & 1];  // read per use, not held in registers
+    double r0 = 0.0, i0 = 0.0;      // v_j^H a (trailing) or v_c^H v_j (left of j)
+    if (wave > j && wave < jb) {
+#pragma unroll
+      for (int r = 0; r < RL; ++r) {
+        const double2 v = Vj[lane + 64 * r];
+        r0 = fma(v.x, ar[r], fma(v.y, ai[r], r0));
+        i0 = fma(v.x, ai[r], fma(-v.y, ar[r], i0));
+      }
+      const double dr = wsum64(r0), di = wsum64(i0);
+      const double fr = t.x * dr + t.y * di, fi = t.x * di - t.y * dr;  // conj(tau) v^H a
+#pragma unroll
+      for (int r = 0; r < RL; ++r) {
+        const double2 v = Vj[lane + 64 * r];
+        ar[r] = fma(-v.x, fr, fma(v.y, fi, ar[r]));
+        ai[r] = fma(-v.x, fi, fma(-v.y, fr, ai[r]));
+      }
+      if (wave == j + 1) qp4_reflector<RL>(ar, ai, j + 1, lane, Vs, taus, tau, j0);
+    } else if (wave < j) {
+      // v_c is 0 above row c and v_j above row j > c: the registers' R part meets zeros
+#pragma unroll
+      for (int r = 0; r < RL; ++r) {
+        const double2 v = Vj[lane + 64 * r];
+        r0 = fma(ar[r], v.x, fma(ai[r], v.y, r0));
+        i0 = fma(ar[r], v.y, fma(-ai[r], v.x, i0));
+      }
+      const double wr = wsum64(r0), wi = wsum64(i0);
+      if (lane == 0) {
+        double2 w;
+        w.x = wr;
+        w.y = wi;
+        Ws[j & 1][wave] = w;
+      }
+    } else if (wave == j && j > 0) {
+      qp4_tcol(j - 1, lane, taus, Ws[(j - 1) & 1], Ts);
+    }
+    __syncthreads();
+  }
+  if (wave == 0 && jb > 0) qp4_tcol(jb - 1, lane, taus, Ws[(jb - 1) & 1], Ts);
+  __syncthreads();
+  if (wave < jb) {
+#pragma unroll
+    for (int r = 0; r < RL; ++r) {
+      const int row = lane + 64 * r;
+      if (row < mr) {
+        double2 v;
+        v.x = ar[r];
+        v.y = ai[r];
+        Wc[(size_t)(j0 + wave) * m + j0 + row] = v;
+      }
+    }
+  }
+  for (int e = tid; e < QB * QB; e += 64 * QB) {
+    const int i = e % QB, jc = e / QB;
+    const double2 x = Ts[jc][i];
+    Tout[2 * e] = (i < jb && jc < jb) ? x.x : 0.0;
+    Tout[2 * e + 1] = (i < jb && jc < jb) ? x.y : 0.0;
   }
 }
 
@@ -836,13 +1022,6 @@ __device__ __forceinline__ double qw_rowsum(double v) {
   if (RL == 4) v += dpp_d<0x124>(v);  // row_ror:4
   v += dpp_d<0x128>(v);                // row_ror:8
   return xsum32_d(xsum16_d(v));
-}
-
-__device__ __forceinline__ double readlane_d(double x, int src) {
-  const long long b = __double_as_longlong(x);
-  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), src);
-  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
-  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
 
 // v^H a over this lane's RL rows (two accumulator chains), then the column sum
@@ -1183,9 +1362,16 @@ int launch_qr_blocked(double* W, int64_t m, int64_t n, void* Qout, int64_t ldq, 
   double* Yp = Tb + (size_t)np * QB * QB * 2;  // chunk partials of V^H M
   for (int64_t p = 0; p < np; ++p) {
     const int j0 = (int)(p * QB), jb = (int)std::min<int64_t>(QB, k - j0);
-    if (m - j0 <= 2 * QP3_THREADS)
-      hipLaunchKernelGGL(qr_panel_v3_kernel<2>, dim3(1), dim3(QP3_THREADS), 0, s, W, tau, Tb + (size_t)p * QB * QB * 2,
-                         (int)m, j0, jb);
+    double* Tp = Tb + (size_t)p * QB * QB * 2;
+    const int mr = (int)m - j0;
+    if (mr <= 64)
+      hipLaunchKernelGGL(qr_panel_v4_kernel<1>, dim3(1), dim3(64 * QB), 0, s, W, tau, Tp, (int)m, j0, jb);
+    else if (mr <= 128)
+      hipLaunchKernelGGL(qr_panel_v4_kernel<2>, dim3(1), dim3(64 * QB), 0, s, W, tau, Tp, (int)m, j0, jb);
+    else if (mr <= 256)
+      hipLaunchKernelGGL(qr_panel_v4_kernel<4>, dim3(1), dim3(64 * QB), 0, s, W, tau, Tp, (int)m, j0, jb);
+    else if (mr <= 2 * QP3_THREADS)
+      hipLaunchKernelGGL(qr_panel_v3_kernel<2>, dim3(1), dim3(QP3_THREADS), 0, s, W, tau, Tp, (int)m, j0, jb);
     else
       hipLaunchKernelGGL(qr_panel_v3_kernel<4>, dim3(1), dim3(QP3_THREADS), 0, s, W, tau, Tb + (size_t)p * QB * QB * 2,
                          (int)m, j0, jb);
