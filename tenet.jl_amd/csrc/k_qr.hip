@@ -549,7 +549,7 @@ __global__ __launch_bounds__(QP3_THREADS) void qr_panel_v3_kernel(double* __rest
 }
 
 
-// Panel v4 (panel rows <= 256): one wave per panel column (QB waves), lane l
+// Panel v4 (panel rows <= 512): one wave per panel column (QB waves), lane l
 // of wave c holding rows l + 64 r (r < RL, panel-relative) of column c in
 // registers.  Step j: every wave right of j applies reflector j (v, tau from
 // LDS) to its column, with a 64-lane sum for v^H a_c; the wave of column
@@ -637,15 +637,12 @@ __global__ __launch_bounds__(64 * QB) void qr_panel_v4_kernel(double* __restrict
   double2* Wc = reinterpret_cast<double2*>(W);
   double ar[RL], ai[RL];
 #pragma unroll
-  for (int r = 0; r < RL; ++r) {
+  for (int r = 0; r < RL; ++r) {  // branch-free: lanes outside the panel read element 0 and drop it
     const int row = lane + 64 * r;
-    ar[r] = 0.0;
-    ai[r] = 0.0;
-    if (wave < jb && row < mr) {
-      const double2 v = Wc[(size_t)(j0 + wave) * m + j0 + row];
-      ar[r] = v.x;
-      ai[r] = v.y;
-    }
+    const bool ok = wave < jb && row < mr;
+    const double2 v = Wc[ok ? (size_t)(j0 + wave) * m + j0 + row : 0];
+    ar[r] = ok ? v.x : 0.0;
+    ai[r] = ok ? v.y : 0.0;
   }
   for (int e = tid; e < QB * QB; e += 64 * QB) {
     double2 z;
@@ -1210,21 +1207,12 @@ __global__ __launch_bounds__(RL == 4 ? 1024 : 768) void qr_reg_kernel(const S2* 
   const bool in_adj = (flags & TENET_QR_IN_ADJ) != 0;
   double ar[RL], ai[RL];
 #pragma unroll
-  for (int r = 0; r < RL; ++r) {
+  for (int r = 0; r < RL; ++r) {  // branch-free: lanes outside A read element 0 and drop it
     const int i = RL * rb + r;
-    ar[r] = 0.0;
-    ai[r] = 0.0;
-    if (i < m && c < n) {
-      if (in_adj) {  // element (i, c) of A^H: conj(A(c, i))
-        const S2 x = A[(size_t)i * lda + c];
-        ar[r] = x.x;
-        ai[r] = -(double)x.y;
-      } else {
-        const S2 x = A[(size_t)c * lda + i];
-        ar[r] = x.x;
-        ai[r] = x.y;
-      }
-    }
+    const bool ok = i < m && c < n;
+    const S2 x = A[ok ? (in_adj ? (size_t)i * lda + c : (size_t)c * lda + i) : 0];  // A^H: conj(A(c, i))
+    ar[r] = ok ? (double)x.x : 0.0;
+    ai[r] = ok ? (in_adj ? -(double)x.y : (double)x.y) : 0.0;
   }
   if (wave == 0) qw_reflector<RL>(ar, ai, 0, cl, rb, V, T);
   __syncthreads();
@@ -1370,8 +1358,8 @@ int launch_qr_blocked(double* W, int64_t m, int64_t n, void* Qout, int64_t ldq, 
       hipLaunchKernelGGL(qr_panel_v4_kernel<2>, dim3(1), dim3(64 * QB), 0, s, W, tau, Tp, (int)m, j0, jb);
     else if (mr <= 256)
       hipLaunchKernelGGL(qr_panel_v4_kernel<4>, dim3(1), dim3(64 * QB), 0, s, W, tau, Tp, (int)m, j0, jb);
-    else if (mr <= 2 * QP3_THREADS)
-      hipLaunchKernelGGL(qr_panel_v3_kernel<2>, dim3(1), dim3(QP3_THREADS), 0, s, W, tau, Tp, (int)m, j0, jb);
+    else if (mr <= 512)
+      hipLaunchKernelGGL(qr_panel_v4_kernel<8>, dim3(1), dim3(64 * QB), 0, s, W, tau, Tp, (int)m, j0, jb);
     else
       hipLaunchKernelGGL(qr_panel_v3_kernel<4>, dim3(1), dim3(QP3_THREADS), 0, s, W, tau, Tb + (size_t)p * QB * QB * 2,
                          (int)m, j0, jb);
