@@ -36,9 +36,11 @@ def svd_flops(m, n):
     return 4.0 * (4.0 * p * p * q + 8.0 * p * q * q + 9.0 * q ** 3)
 
 
-def _qr_absorb_fits(m, n):
-    """tenet_qr_absorb's shape range (the single-workgroup QR)."""
-    return m <= 64 and n <= 64
+def _qr_absorb_fits(m, n, nb):
+    """tenet_qr_absorb's shape range (the single-workgroup QR) and a neighbour
+    small enough for its four product waves (a wider one contracts faster as
+    a separate many-workgroup GEMM)."""
+    return m <= 64 and n <= 64 and nb <= 128
 
 
 def _gate_tensor(U4, d):
@@ -160,7 +162,7 @@ class MPS:
         _, d2, chir = B.shape
         self.flops["qr"] += qr_flops(m, chi)
         self.flops["contract"] += 8.0 * k * chi * d2 * chir
-        if _qr_absorb_fits(m, chi):  # QR and R's contraction into B in one launch
+        if _qr_absorb_fits(m, chi, d2 * chir):  # QR and R's contraction into B in one launch
             Bn = DeviceArray(self.ctx, (k, d2, chir), self.dtype)
             L.check(L.lib.tenet_qr_absorb(self.ctx.h, None, self.code, 0, m, chi, C.c_void_p(A.ptr), m,
                                           C.c_void_p(Q.ptr), m, L.ABSORB_R_B, d2 * chir, C.c_void_p(B.ptr), chi,
@@ -188,7 +190,7 @@ class MPS:
         chil2, d2, _ = B.shape
         self.flops["qr"] += qr_flops(m, chi)
         self.flops["contract"] += 8.0 * chil2 * d2 * chi * k
-        if _qr_absorb_fits(m, chi):  # QR of A^H and B R^H in one launch
+        if _qr_absorb_fits(m, chi, chil2 * d2):  # QR of A^H and B R^H in one launch
             Bn = DeviceArray(self.ctx, (chil2, d2, k), self.dtype)
             L.check(L.lib.tenet_qr_absorb(self.ctx.h, None, self.code, L.QR_IN_ADJ | L.QR_Q_ADJ, m, chi,
                                           C.c_void_p(A.ptr), chi, C.c_void_p(Qh.ptr), k, L.ABSORB_B_RH, chil2 * d2,
