@@ -155,6 +155,33 @@ function Muscle.tensor_svd_thin(A::HIPTensor{T}; inds_u, inds_v=setdiff(inds(A),
            Tensor(reshape_view(V, (kmax, dv...)), [ind_s, inds_v...])
 end
 
+# ---- one canonize move in one launch (generic_canonize_site!, canonize.jl:58-70) ----
+# tensor_qr_thin(A) followed by binary_einsum(R, B) across `ind_bond`, both
+# matricised dimensions of A at most 64; returns (Q, R B) on the bond's label
+# (the reference renames the virtual index back to it), or `nothing` when the
+# shapes are out of range and the two calls above are the route.
+function qr_absorb(A::HIPTensor{T}, B::HIPTensor{T}, ind_bond) where {T}
+    inds_a = filter(!=(ind_bond), inds(A))
+    inds_b = filter(!=(ind_bond), inds(B))
+    m = prod(size(A, i) for i in inds_a; init=1)
+    n = size(A, ind_bond)
+    (m <= 64 && n <= 64) || return nothing
+    M, = matricize(A, inds_a, [ind_bond])
+    Bm, _, nb = matricize(B, [ind_bond], inds_b)
+    k = min(m, n)
+    Q = HIPArray{T,2}(undef, (m, k))
+    Bo = HIPArray{T,2}(undef, (k, nb))
+    check(ccall((:tenet_qr_absorb, libtenet), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Cint, Int64, Int64, Ptr{Cvoid}, Int64, Ptr{Cvoid}, Int64, Cint, Int64,
+                 Ptr{Cvoid}, Int64, Ptr{Cvoid}, Int64),
+                context().h, C_NULL, dtype_code(T), Cint(0), m, n, M.ptr, m, Q.ptr, m, Cint(0), nb, Bm.ptr, n,
+                Bo.ptr, k))
+    da = Tuple(size(A, i) for i in inds_a)
+    db = Tuple(size(B, i) for i in inds_b)
+    return Tensor(reshape_view(Q, (da..., k)), [inds_a..., ind_bond]),
+           Tensor(reshape_view(Bo, (k, db...)), [ind_bond, inds_b...])
+end
+
 # ---- Muscle.simple_update (simple_update.jl:53-64) ----------------------------
 function Muscle.simple_update(A::HIPTensor{T}, ind_a, B::HIPTensor{T}, ind_b, ind_bond, G::HIPTensor{T}, ind_op_a,
                               ind_op_b; maxdim=0, threshold=0.0, absorb=Muscle.AbsorbEqually()) where {T}
