@@ -151,7 +151,9 @@ int launch_write_scalar_from(const void* env, double* out, int dtype, hipStream_
 // waves per SIMD); A, B and T sit in LDS transposed so that the lanes of a
 // wave read consecutive entries, and the next site's loads are in flight
 // while the current one is contracted.
-constexpr int OS_THREADS = 512;
+constexpr int OS_THREADS = 512;  // 1024 threads with 1 x 2 / 1 x 1 blocks measured no faster
+constexpr int OS_TR = 2;         // T block: OS_TR rows x 2 columns per thread
+constexpr int OS_ER = 2;         // E' block: OS_ER rows x 1 column per thread
 constexpr int OS_CHI = 32;     // every bond <= 32
 constexpr int OS_SITE = 2048;  // d * chi^2 <= 2048 (A, B and T staged in LDS)
 constexpr int OS_PER = OS_SITE / OS_THREADS;
@@ -212,18 +214,18 @@ __global__ __launch_bounds__(OS_THREADS) void overlap_small_kernel(const Overlap
     if (i + 1 < n) os_load(tab, i + 1, tid, va, vb);  // in flight during the products
     // T = E^T A: rows lb, columns c = s + d * ra; stored as T'[ra + ra_n (lb + lb_n s)]
     {
-      const int M = lb, mb = (M + 1) / 2, nbk = (N + 1) / 2;
+      const int M = lb, mb = (M + OS_TR - 1) / OS_TR, nbk = (N + 1) / 2;
       for (int blk = tid; blk < mb * nbk; blk += OS_THREADS) {
-        const int r0 = 2 * (blk % mb), c0 = 2 * (blk / mb);
-        double tr[2][2], ti[2][2];
+        const int r0 = OS_TR * (blk % mb), c0 = 2 * (blk / mb);
+        double tr[OS_TR][2], ti[OS_TR][2];
 #pragma unroll
-        for (int x = 0; x < 2; ++x)
+        for (int x = 0; x < OS_TR; ++x)
 #pragma unroll
           for (int y = 0; y < 2; ++y) tr[x][y] = ti[x][y] = 0.0;
         for (int a = 0; a < la; ++a) {
-          double er[2], ei[2], xr[2], xi[2];
+          double er[OS_TR], ei[OS_TR], xr[2], xi[2];
 #pragma unroll
-          for (int x = 0; x < 2; ++x) {
+          for (int x = 0; x < OS_TR; ++x) {
             const int r = min(r0 + x, M - 1);
             er[x] = Er[a][r];
             ei[x] = Ei[a][r];
@@ -235,7 +237,7 @@ __global__ __launch_bounds__(OS_THREADS) void overlap_small_kernel(const Overlap
             xi[y] = Ai[c + N * a];
           }
 #pragma unroll
-          for (int x = 0; x < 2; ++x)
+          for (int x = 0; x < OS_TR; ++x)
 #pragma unroll
             for (int y = 0; y < 2; ++y) {
               tr[x][y] = fma(er[x], xr[y], fma(-ei[x], xi[y], tr[x][y]));
@@ -243,7 +245,7 @@ __global__ __launch_bounds__(OS_THREADS) void overlap_small_kernel(const Overlap
             }
         }
 #pragma unroll
-        for (int x = 0; x < 2; ++x)
+        for (int x = 0; x < OS_TR; ++x)
 #pragma unroll
           for (int y = 0; y < 2; ++y)
             if (r0 + x < M && c0 + y < N) {
@@ -256,16 +258,18 @@ __global__ __launch_bounds__(OS_THREADS) void overlap_small_kernel(const Overlap
     }
     __syncthreads();
     // E'(ra, rb) = sum_k T'[ra + ra_n k] conj(B'[rb + rb_n k]), k = lb + lb_n s
-    const int mb = (ra + 1) / 2;
-    double outr[2] = {0.0, 0.0}, outi[2] = {0.0, 0.0};
-    const bool has = tid < mb * rb;  // mb * rb <= 512
-    const int a0 = 2 * (tid % max(mb, 1)), b0 = tid / max(mb, 1);
+    const int mb = (ra + OS_ER - 1) / OS_ER;
+    double outr[OS_ER], outi[OS_ER];
+#pragma unroll
+    for (int x = 0; x < OS_ER; ++x) outr[x] = outi[x] = 0.0;
+    const bool has = tid < mb * rb;  // mb * rb <= OS_THREADS
+    const int a0 = OS_ER * (tid % max(mb, 1)), b0 = tid / max(mb, 1);
     double mx = 0.0;
     if (has) {
       for (int k = 0; k < K; ++k) {
         const double brr = Br[b0 + rb * k], bii = Bi[b0 + rb * k];
 #pragma unroll
-        for (int x = 0; x < 2; ++x) {  // T * conj(B)
+        for (int x = 0; x < OS_ER; ++x) {  // T * conj(B)
           const int ai = min(a0 + x, ra - 1);
           const double trr = Tr[ai + ra * k], tii = Ti[ai + ra * k];
           outr[x] = fma(trr, brr, fma(tii, bii, outr[x]));
@@ -273,7 +277,7 @@ __global__ __launch_bounds__(OS_THREADS) void overlap_small_kernel(const Overlap
         }
       }
 #pragma unroll
-      for (int x = 0; x < 2; ++x)
+      for (int x = 0; x < OS_ER; ++x)
         if (a0 + x < ra) mx = fmax(mx, fmax(fabs(outr[x]), fabs(outi[x])));
     }
     mx = wave_max(mx);
@@ -285,7 +289,7 @@ __global__ __launch_bounds__(OS_THREADS) void overlap_small_kernel(const Overlap
     const double inv = m > 0.0 ? 1.0 / m : 1.0;
     if (has) {
 #pragma unroll
-      for (int x = 0; x < 2; ++x)
+      for (int x = 0; x < OS_ER; ++x)
         if (a0 + x < ra) {
           Er[a0 + x][b0] = outr[x] * inv;
           Ei[a0 + x][b0] = outi[x] * inv;
