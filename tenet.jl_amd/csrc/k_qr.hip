@@ -858,13 +858,8 @@ __global__ __launch_bounds__(QR_THREADS) void qr_apply_y_kernel(const double* __
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      double a = yr_[i][j], b = yi_[i][j];
-      a += __shfl_xor(a, 16, 64);
-      b += __shfl_xor(b, 16, 64);
-      a += __shfl_xor(a, 32, 64);
-      b += __shfl_xor(b, 32, 64);
-      yr_[i][j] = a;
-      yi_[i][j] = b;
+      yr_[i][j] = xsum32_d(xsum16_d(yr_[i][j]));
+      yi_[i][j] = xsum32_d(xsum16_d(yi_[i][j]));
     }
   if (lane < 16) {
 #pragma unroll

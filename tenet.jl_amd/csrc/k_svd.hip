@@ -246,6 +246,15 @@ __device__ __forceinline__ double rcp_nr(double x) {
 // The new diagonal is a - sign(delta) t |g|, d + sign(delta) t |g| with
 // t |g| = |g|^2 / (|delta| + r) = |g|^2 / (2 r c^2).
 // Identity when |g| <= tol sqrt(a d) or |g| <= abs_tol (compared squared).
+// x of lane (l + 1) mod 16 within each 16-lane row (DPP row_ror:15, on the
+// VALU; lanes 0-15 hold the slots of the cross-block inner solve)
+__device__ __forceinline__ double from_next_lane(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), 0x12F, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x12F, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
 struct Rot {
   double c, sr, si;  // J = [[c, s], [-conj(s), c]]
   double an, dn;     // new diagonal
@@ -621,8 +630,8 @@ __device__ __forceinline__ void jacobi_cross32(SvSmem& sm, double tol, double ab
       }
       if (step + 1 < XB) {
         // slot k + 1's rotation and new diagonal d' (lanes 0-15 hold slots 0-15)
-        const double cn = __shfl(rt.c, kn), srn = __shfl(rt.sr, kn), sin_ = __shfl(rt.si, kn);
-        const double dnn = __shfl(rt.dn, kn);
+        const double cn = from_next_lane(rt.c), srn = from_next_lane(rt.sr), sin_ = from_next_lane(rt.si);
+        const double dnn = from_next_lane(rt.dn);
         double n0r, n0i, n1r, n1i;
         block_row(false, rt.c, rt.sr, rt.si, cn, srn, sin_, b, n0r, n0i, n1r, n1i);
         gr = n1r;
