@@ -654,31 +654,35 @@ __global__ __launch_bounds__(64 * QB) void qr_panel_v4_kernel(double* __restrict
   __syncthreads();
   for (int j = 0; j < jb; ++j) {
     const double2 t = taus[j];
-    const double2* Vj = Vs[j & 1];  // read per use, not held in registers
-    double r0 = 0.0, i0 = 0.0;      // v_j^H a (trailing) or v_c^H v_j (left of j)
+    const double2* Vj = Vs[j & 1];
+    double vr[RL], vi[RL];
+#pragma unroll
+    for (int r = 0; r < RL; ++r) {
+      const double2 v = Vj[lane + 64 * r];
+      vr[r] = v.x;
+      vi[r] = v.y;
+    }
+    double r0 = 0.0, i0 = 0.0;  // v_j^H a (trailing) or v_c^H v_j (left of j)
     if (wave > j && wave < jb) {
 #pragma unroll
       for (int r = 0; r < RL; ++r) {
-        const double2 v = Vj[lane + 64 * r];
-        r0 = fma(v.x, ar[r], fma(v.y, ai[r], r0));
-        i0 = fma(v.x, ai[r], fma(-v.y, ar[r], i0));
+        r0 = fma(vr[r], ar[r], fma(vi[r], ai[r], r0));
+        i0 = fma(vr[r], ai[r], fma(-vi[r], ar[r], i0));
       }
       const double dr = wsum64(r0), di = wsum64(i0);
       const double fr = t.x * dr + t.y * di, fi = t.x * di - t.y * dr;  // conj(tau) v^H a
 #pragma unroll
       for (int r = 0; r < RL; ++r) {
-        const double2 v = Vj[lane + 64 * r];
-        ar[r] = fma(-v.x, fr, fma(v.y, fi, ar[r]));
-        ai[r] = fma(-v.x, fi, fma(-v.y, fr, ai[r]));
+        ar[r] = fma(-vr[r], fr, fma(vi[r], fi, ar[r]));
+        ai[r] = fma(-vr[r], fi, fma(-vi[r], fr, ai[r]));
       }
       if (wave == j + 1) qp4_reflector<RL>(ar, ai, j + 1, lane, Vs, taus, tau, j0);
     } else if (wave < j) {
       // v_c is 0 above row c and v_j above row j > c: the registers' R part meets zeros
 #pragma unroll
       for (int r = 0; r < RL; ++r) {
-        const double2 v = Vj[lane + 64 * r];
-        r0 = fma(ar[r], v.x, fma(ai[r], v.y, r0));
-        i0 = fma(ar[r], v.y, fma(-ai[r], v.x, i0));
+        r0 = fma(ar[r], vr[r], fma(ai[r], vi[r], r0));
+        i0 = fma(ar[r], vi[r], fma(-ai[r], vr[r], i0));
       }
       const double wr = wsum64(r0), wi = wsum64(i0);
       if (lane == 0) {
