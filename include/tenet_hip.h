@@ -144,7 +144,7 @@ int tenet_qr_thin_ex(tenet_ctx ctx, tenet_stream stream, int dtype, int flags, i
  * flags: TENET_QR_IN_ADJ, TENET_QR_Q_ADJ.  R itself is not returned.
  * side TENET_ABSORB_R_B: Bout (k x nb, ldbo >= k) = R B, B n x nb (ldb >= n);
  * side TENET_ABSORB_B_RH: Bout (nb x k, ldbo >= nb) = B R^H, B nb x n
- * (ldb >= nb).  Bout must not overlap A, Q or B.  m, n <= 64 (the
+ * (ldb >= nb).  Bout must not overlap A, Q or B, nor Q overlap B.  m, n <= 64 (the
  * single-workgroup QR); larger shapes return TENET_ERR_UNSUPPORTED (use
  * tenet_qr_thin_ex + tenet_einsum). */
 typedef enum { TENET_ABSORB_R_B = 0, TENET_ABSORB_B_RH = 1 } tenet_absorb_side;

@@ -990,6 +990,7 @@ extern "C" int tenet_qr_absorb(tenet_ctx ctx, tenet_stream stream, int dtype, in
   TRY(check_span(ctx, Bout, so, "Bout"));
   CHECK_ARG(!spans_overlap(Bout, so, A, sa) && !spans_overlap(Bout, so, Q, sq) && !spans_overlap(Bout, so, B, sb),
             "Bout overlaps A, Q or B");
+  CHECK_ARG(!spans_overlap(Q, sq, B, sb), "Q overlaps B (B is read while Q is written)");
   hipStream_t s = pick(ctx, stream);
   const double p = (double)std::max(m, n), q = (double)k;
   const double fl = 4.0 * (4.0 * p * q * q - 4.0 * q * q * q / 3.0) + 8.0 * (double)nb * q * (double)n;
