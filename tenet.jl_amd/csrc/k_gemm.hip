@@ -343,12 +343,12 @@ __device__ __forceinline__ void partial_sum(const R* __restrict__ part, int nchu
       }
 }
 
-// One 64x64 output tile (tm, tn) of batch entry z over the whole K range.
-template <typename S2, typename R>
+// One Cfg::BM x Cfg::BN output tile (tm, tn) of batch entry z over the whole K range.
+template <typename S2, typename R, typename Cfg>
 __device__ __forceinline__ void gemm_tile(const TileView& p, int64_t tm, int64_t tn, int64_t z, GemmSmem<R>& sm) {
-  TileAcc<R, Tile64> acc;
-  gemm_accumulate<S2, R, Tile64>(p, tm, tn, z, 0, p.K, sm, acc);
-  gemm_store<S2, R, Tile64>(p, tm, tn, z, acc);
+  TileAcc<R, Cfg> acc;
+  gemm_accumulate<S2, R, Cfg>(p, tm, tn, z, 0, p.K, sm, acc);
+  gemm_store<S2, R, Cfg>(p, tm, tn, z, acc);
 }
 
 // Map a physical workgroup id to a logical tile so that the tiles one XCD
@@ -360,7 +360,7 @@ __device__ __forceinline__ int64_t xcd_remap(int64_t pid, int64_t total) {
   return (pid % NXCD) * (total / NXCD) + pid / NXCD;
 }
 
-template <typename S2, typename R>
+template <typename S2, typename R, typename Cfg>
 __global__ __launch_bounds__(256) void gemm_kernel(TileView p, int64_t tiles_m, int64_t tiles_n, int64_t batch) {
   __shared__ GemmSmem<R> sm;
   const int64_t per_batch = tiles_m * tiles_n;
@@ -368,7 +368,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(TileView p, int64_t tiles_m, 
   const int64_t lid = xcd_remap((int64_t)blockIdx.x, total);
   const int64_t z = lid / per_batch;
   const int64_t t = lid - z * per_batch;
-  gemm_tile<S2, R>(p, t % tiles_m, t / tiles_m, z, sm);
+  gemm_tile<S2, R, Cfg>(p, t % tiles_m, t / tiles_m, z, sm);
 }
 
 template <typename S2, typename R>
@@ -389,14 +389,21 @@ static int launch_gemm_t(const GemmLaunch& g, hipStream_t s) {
   p.alpha_im = g.alpha_im;
   p.beta_re = g.beta_re;
   p.beta_im = g.beta_im;
-  const int64_t tm = (g.M + 63) / 64, tn = (g.N + 63) / 64;
+  // 64 x 64 tiles, or 32 x 32 when 64 x 64 would leave most of the 256 CUs idle
+  // (a 512 x 512 product: 64 workgroups against 256)
+  const bool small = ((g.M + 63) / 64) * ((g.N + 63) / 64) * g.batch < 128;
+  const int64_t bt = small ? 32 : 64;
+  const int64_t tm = (g.M + bt - 1) / bt, tn = (g.N + bt - 1) / bt;
   const int64_t total = tm * tn * g.batch;
   if (total <= 0) return TENET_OK;
   if (total > 0x7fffffffLL) {
     set_error("contraction too large for one launch");
     return TENET_ERR_UNSUPPORTED;
   }
-  hipLaunchKernelGGL((gemm_kernel<S2, R>), dim3((unsigned)total), dim3(256), 0, s, p, tm, tn, g.batch);
+  if (small)
+    hipLaunchKernelGGL((gemm_kernel<S2, R, Tile32>), dim3((unsigned)total), dim3(256), 0, s, p, tm, tn, g.batch);
+  else
+    hipLaunchKernelGGL((gemm_kernel<S2, R, Tile64>), dim3((unsigned)total), dim3(256), 0, s, p, tm, tn, g.batch);
   TENET_HIP_CHECK(hipGetLastError());
   return TENET_OK;
 }
