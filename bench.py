@@ -720,16 +720,9 @@ PLAIN_N, PLAIN_CHI, PLAIN_SEED = 20, 32, 20261202 + 0
 def plain_arrays():
     """n = 20, d = 2, uniform chi = 32 (edges included: site 1 is (1, 2, 32)
     as a 3-way array), standard complex Gaussian entries from numpy PCG64."""
-    import numpy as np
+    from tenet_hip.workloads import plain_state_arrays
 
-    rng = np.random.default_rng(PLAIN_SEED)
-    out = []
-    for i in range(PLAIN_N):
-        chil = 1 if i == 0 else PLAIN_CHI
-        chir = 1 if i == PLAIN_N - 1 else PLAIN_CHI
-        shp = (chil, 2, chir)
-        out.append((rng.standard_normal(shp) + 1j * rng.standard_normal(shp)) / np.sqrt(2.0))
-    return out
+    return plain_state_arrays(PLAIN_SEED, PLAIN_N, PLAIN_CHI)
 
 
 def bench_plain(args):

@@ -262,8 +262,12 @@ int tenet_grouped_einsum(tenet_ctx ctx, tenet_stream stream, int dtype, int prec
  * (may be NULL) receives them, 64x64 = 1. */
 int tenet_autotune_tiles(tenet_ctx ctx, int dtype, double costs_out[3]);
 /* Many independent thin SVDs (column-major, lda = m) with per-problem
- * truncation.  All pointer arrays are HOST arrays of device pointers;
- * ms, ns, maxdims are host arrays; k_dev / disc_dev device arrays (count). */
+ * truncation (the tensor_svd_thin calls of canonize.jl:97 / evolve.jl:62,
+ * batched).  All pointer arrays are HOST arrays of device pointers;
+ * ms, ns, maxdims are host arrays; k_dev / disc_dev device arrays (count).
+ * Optional: Us[i] = NULL (U not formed; a problem with m <= n then runs the
+ * Jacobi on A^H and accumulates no rotations), Ss = NULL, maxdims = NULL
+ * (no cap), k_dev = NULL, disc_dev = NULL.  As[i] and Vhs[i] are required. */
 int tenet_grouped_svd(tenet_ctx ctx, tenet_stream stream, int dtype, int64_t count, const int64_t* ms,
                       const int64_t* ns, const void* const* As, void* const* Us, double* const* Ss,
                       void* const* Vhs, const int64_t* maxdims, double threshold, int64_t* k_dev, double* disc_dev);

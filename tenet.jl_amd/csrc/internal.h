@@ -122,6 +122,7 @@ struct SvdProblem {
   double* fro2;   // ||A||_F^2 (absolute rotation floor eps^2 * ||A||_F^2)
   int32_t nr, ncols, ncp, b, nblk, q;
   int32_t m, n, transposed;
+  int32_t accum;  // apply the rotations to V (0: V is never needed, U == NULL with X = A^H)
   const void* A;  // input (m x n, lda)
   int64_t lda;
   void* U;

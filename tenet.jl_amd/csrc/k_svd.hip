@@ -430,9 +430,11 @@ __device__ __forceinline__ void inner_step(SvSmem& sm, const GramRow* cr, const 
 __device__ __forceinline__ void jacobi_inner(SvSmem& sm, int n2, double tol, double abs_tol) {
   const int tid = threadIdx.x;
   const int npair = n2 / 2;
-  for (int e = tid; e < n2 * n2; e += SR_THREADS) {
-    const int i = e % n2, c = e / n2;
-    sm.wr[i][c] = (i == c) ? 1.0 : 0.0;
+  // W = I on the pair's n2 columns and 0 elsewhere in the 32 x 32 block (the
+  // MFMA rotation runs over all 32 columns)
+  for (int e = tid; e < B2MAX * B2MAX; e += SR_THREADS) {
+    const int i = e % B2MAX, c = e / B2MAX;
+    sm.wr[i][c] = (i == c && i < n2) ? 1.0 : 0.0;
     sm.wi[i][c] = 0.0;
   }
   if (tid == 0) sm.any_outer = 0;
@@ -699,6 +701,219 @@ __device__ void apply_rows(SvSmem& sm, double* M, int64_t ld, int r0, int r1, in
   __syncthreads();
 }
 
+// ---- FP64 MFMA forms of the Gram and of the rotation (v_mfma_f64_16x16x4_f64)
+//
+// Operand layout of the 16x16x4 FP64 product (lane l): A[i = l % 16][k = l / 16],
+// B[k = l / 16][j = l % 16]; accumulator register r of lane l holds
+// D[i = 4 r + l / 16][j = l % 16].  The pair's 2b <= 32 columns are two
+// 16-column blocks (columns >= 2b staged as zeros).
+typedef double svd_d4v __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ svd_d4v mfma_f64(double a, double b, svd_d4v c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// All 32 staged columns (zeros beyond 2b and below `rows`), so that the MFMA
+// operands never read stale LDS.
+__device__ __forceinline__ void store_rows_all(SvSmem& sm, const double2 (&v)[STAGE_PER]) {
+#pragma unroll
+  for (int u = 0; u < STAGE_PER; ++u) {
+    const int e = threadIdx.x + u * SR_THREADS;
+    const int r = e % RS, l = e / RS;
+    sm.xr[r][l] = v[u].x;
+    sm.xi[r][l] = v[u].y;
+  }
+}
+
+// Gram G = X^H X of the pair's columns over rows [r0, r1) into sm.gr / sm.gi.
+// With P = Xr^T Xi:  Re G = Xr^T Xr + Xi^T Xi (symmetric), Im G = P - P^T.
+// For a 4-row K step the A operand of X^T and the B operand of X are the same
+// register (X[k = l / 16][c = l % 16]).  Waves 0-3 take the staged rows
+// [0, 32) of every chunk, waves 4-7 the rows [32, 64); within a group the
+// four waves own: Re G(0,0) + P(0,0); Re G(1,1) + P(1,1); Re G(0,1);
+// P(0,1) + P(1,0) (3, 3, 2 and 2 products per K step).  The two row groups
+// are added in LDS at the end (fixed order: deterministic).
+__device__ void gram_mfma(SvSmem& sm, const double* X, int64_t ld, int r0, int r1, int I, int J, int b) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int role = wave & 3, grp = wave >> 2;
+  const int kr = lane >> 4, kc = lane & 15;
+  svd_d4v a0 = {0, 0, 0, 0}, a1 = {0, 0, 0, 0};
+  double2 v[STAGE_PER];
+  load_rows(v, X, ld, r0, min(RS, r1 - r0), I, J, b);
+  for (int rb = r0; rb < r1; rb += RS) {
+    __syncthreads();
+    store_rows_all(sm, v);
+    __syncthreads();
+    if (rb + RS < r1) load_rows(v, X, ld, rb + RS, min(RS, r1 - rb - RS), I, J, b);
+#pragma unroll
+    for (int s = 0; s < RS / 8; ++s) {
+      const int row = grp * (RS / 2) + 4 * s + kr;
+      const double xr0 = sm.xr[row][kc], xi0 = sm.xi[row][kc];
+      const double xr1 = sm.xr[row][16 + kc], xi1 = sm.xi[row][16 + kc];
+      if (role == 0) {
+        a0 = mfma_f64(xr0, xr0, a0);
+        a0 = mfma_f64(xi0, xi0, a0);
+        a1 = mfma_f64(xr0, xi0, a1);
+      } else if (role == 1) {
+        a0 = mfma_f64(xr1, xr1, a0);
+        a0 = mfma_f64(xi1, xi1, a0);
+        a1 = mfma_f64(xr1, xi1, a1);
+      } else if (role == 2) {
+        a0 = mfma_f64(xr0, xr1, a0);
+        a0 = mfma_f64(xi0, xi1, a0);
+      } else {
+        a0 = mfma_f64(xr0, xi1, a0);
+        a1 = mfma_f64(xr1, xi0, a1);
+      }
+    }
+  }
+  // row group 1 -> LDS (staging area), group 0 adds
+  double* red = &sm.xr[0][0];
+  __syncthreads();
+  if (grp == 1) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      red[((role * 2 + 0) * 4 + r) * 64 + lane] = a0[r];
+      red[((role * 2 + 1) * 4 + r) * 64 + lane] = a1[r];
+    }
+  }
+  __syncthreads();
+  if (grp == 0) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      a0[r] += red[((role * 2 + 0) * 4 + r) * 64 + lane];
+      a1[r] += red[((role * 2 + 1) * 4 + r) * 64 + lane];
+    }
+    // Re G into gr (mirrored), P into wr (scratch until the inner solve sets W)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = 4 * r + kr, j = kc;
+      if (role == 0) {
+        sm.gr[i][j] = a0[r];
+        sm.wr[i][j] = a1[r];
+      } else if (role == 1) {
+        sm.gr[16 + i][16 + j] = a0[r];
+        sm.wr[16 + i][16 + j] = a1[r];
+      } else if (role == 2) {
+        sm.gr[i][16 + j] = a0[r];
+        sm.gr[16 + j][i] = a0[r];
+      } else {
+        sm.wr[i][16 + j] = a0[r];
+        sm.wr[16 + i][j] = a1[r];
+      }
+    }
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < B2MAX * B2MAX; e += SR_THREADS) {
+    const int i = e & (B2MAX - 1), j = e / B2MAX;
+    sm.gi[i][j] = sm.wr[i][j] - sm.wr[j][i];
+  }
+  __syncthreads();
+}
+
+// Rows [r0, r1) of the pair's columns of M <- M W by MFMA: per 64-row chunk
+// wave w computes the 16 x 16 output block (rows 16 (w % 4), columns
+// 16 (w / 4)) over K = 32 (Yr = Xr Wr - Xi Wi, Yi = Xr Wi + Xi Wr), the W
+// operands held in registers for the whole call; the block goes back
+// through LDS so the stores are column-contiguous.  The next chunk's loads
+// are in flight during the products.
+__device__ void apply_mfma(SvSmem& sm, double* M, int64_t ld, int r0, int r1, int I, int J, int b) {
+  const int n2 = 2 * b;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int rblk = wave & 3, cblk = wave >> 2;
+  const int kr = lane >> 4, kc = lane & 15;
+  double2 v[STAGE_PER];
+  load_rows(v, M, ld, r0, min(RS, r1 - r0), I, J, b);
+  for (int rb = r0; rb < r1; rb += RS) {
+    const int rows = min(RS, r1 - rb);
+    __syncthreads();
+    store_rows_all(sm, v);
+    __syncthreads();
+    if (rb + RS < r1) load_rows(v, M, ld, rb + RS, min(RS, r1 - rb - RS), I, J, b);
+    svd_d4v yr = {0, 0, 0, 0}, yi = {0, 0, 0, 0};
+#pragma unroll
+    for (int s = 0; s < B2MAX / 4; ++s) {
+      const double xr = sm.xr[16 * rblk + kc][4 * s + kr], xi = sm.xi[16 * rblk + kc][4 * s + kr];
+      const double wre = sm.wr[4 * s + kr][16 * cblk + kc], wim = sm.wi[4 * s + kr][16 * cblk + kc];
+      yr = mfma_f64(xr, wre, yr);
+      yr = mfma_f64(-xi, wim, yr);
+      yi = mfma_f64(xr, wim, yi);
+      yi = mfma_f64(xi, wre, yi);
+    }
+    __syncthreads();  // every wave has read its X operands
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      sm.xr[16 * rblk + 4 * r + kr][16 * cblk + kc] = yr[r];
+      sm.xi[16 * rblk + 4 * r + kr][16 * cblk + kc] = yi[r];
+    }
+    __syncthreads();
+    double2* M2 = reinterpret_cast<double2*>(M);
+#pragma unroll
+    for (int u = 0; u < STAGE_PER; ++u) {
+      const int e = threadIdx.x + u * SR_THREADS;
+      const int r = e % RS, l = e / RS;
+      if (l < n2 && r < rows) {
+        double2 o;
+        o.x = sm.xr[r][l];
+        o.y = sm.xi[r][l];
+        M2[(size_t)pair_col(l, I, J, b) * ld + rb + r] = o;
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// Pair (I, J) of item p in round `round` (mode 0: the fixed pairs (2p, 2p+1);
+// mode 1: the circle pairs).
+__device__ __forceinline__ void round_pair(int p, int round, int mode, int nb, int& I, int& J) {
+  I = 2 * p;
+  J = 2 * p + 1;
+  if (mode == 1) {
+    I = circle_pos(p, round, nb);
+    J = circle_pos(nb - 1 - p, round, nb);
+    if (I > J) {
+      const int t = I;
+      I = J;
+      J = t;
+    }
+  }
+}
+
+// One round with one workgroup per pair over all rows (the grouped batches
+// and small single problems): the Gram and the rotation of X (and of V when
+// accumulated) on FP64 MFMA, the inner solve as in svd_round_kernel.
+// mode 0: full inner sweep on the fixed pairs; mode 1: cross-block inner steps.
+__global__ __launch_bounds__(SR_THREADS, 4) void svd_round_mfma_kernel(SvdBatch Bt, int round, int mode) {
+  __shared__ SvSmem sm;
+  const SvdProblem& P = Bt.arr ? Bt.arr[blockIdx.y] : Bt.one;
+  if (P.flag[1]) return;  // converged
+  const int nb = P.nblk;
+  if (mode == 1 && (nb <= 2 || round >= nb - 1)) return;
+  const int p = blockIdx.x;
+  if (p >= nb / 2) return;
+  int I, J;
+  round_pair(p, round, mode, nb, I, J);
+  const int b = P.b, n2 = 2 * b;
+  gram_mfma(sm, P.X, P.nr, 0, P.nr, I, J, b);
+  const double abs_tol = fmax(4.930380657631324e-32 * (*P.fro2), 1e-300);
+  if (mode == 1)
+    jacobi_cross32(sm, P.tol, abs_tol);
+  else
+    jacobi_inner(sm, n2, P.tol, abs_tol);
+  if (threadIdx.x == 0) {
+    double mx = 0.0;
+    for (int k = 0; k < n2 / 2; ++k) mx = fmax(mx, sm.rmax[k]);
+    atomicMax(reinterpret_cast<unsigned long long*>(P.flag + 4), (unsigned long long)__double_as_longlong(mx));
+  }
+  if (!sm.any_outer) return;  // pair already orthogonal: W = I (uniform branch)
+  if (threadIdx.x == 0) P.flag[0] = 1;
+  apply_mfma(sm, P.X, P.nr, 0, P.nr, I, J, b);
+  if (P.accum) apply_mfma(sm, P.V, P.ncp, 0, P.ncp, I, J, b);
+}
+
+// Rounds with each pair split over q workgroups by rows (phase 0: partial
+// Grams; phase 1: reduce, inner solve, rotate the workgroup's row chunk), on
+// the VALU with LDS staging.
 // mode 0: full inner sweep on the fixed pairs (2p, 2p+1) (one such round per
 // sweep); mode 1: cross-block inner steps on the circle pairs of `round`.
 __global__ __launch_bounds__(SR_THREADS) void svd_round_kernel(SvdBatch Bt, int round, int phase, int mode) {
@@ -713,16 +928,8 @@ __global__ __launch_bounds__(SR_THREADS) void svd_round_kernel(SvdBatch Bt, int 
   const int item = blockIdx.x;
   const int p = item / q, c = item % q;
   if (p >= nb / 2) return;
-  int I = 2 * p, J = 2 * p + 1;
-  if (mode == 1) {
-    I = circle_pos(p, round, nb);
-    J = circle_pos(nb - 1 - p, round, nb);
-    if (I > J) {
-      const int t = I;
-      I = J;
-      J = t;
-    }
-  }
+  int I, J;
+  round_pair(p, round, mode, nb, I, J);
   const int b = P.b, n2 = 2 * b;
   const int rcx = (P.nr + q - 1) / q;
   const int r0 = c * rcx, r1 = min(P.nr, r0 + rcx);
@@ -743,9 +950,7 @@ __global__ __launch_bounds__(SR_THREADS) void svd_round_kernel(SvdBatch Bt, int 
     }
     return;
   }
-  if (phase == 2) {
-    gram_over_rows(sm, P.X, P.nr, 0, P.nr, I, J, b, ar, ai);
-  } else {
+  {
     for (int t = 0; t < GE_PER_THREAD; ++t) {
       ar[t] = 0.0;
       ai[t] = 0.0;
@@ -810,11 +1015,11 @@ __global__ __launch_bounds__(SR_THREADS) void svd_round_kernel(SvdBatch Bt, int 
   }
   if (!sm.any_outer) return;  // pair already orthogonal: W = I (uniform branch)
   if (threadIdx.x == 0 && c == 0) P.flag[0] = 1;
-  const int rx0 = (phase == 2) ? 0 : r0, rx1 = (phase == 2) ? P.nr : r1;
-  apply_rows(sm, P.X, P.nr, rx0, rx1, I, J, b);
+  apply_rows(sm, P.X, P.nr, r0, r1, I, J, b);
+  if (!P.accum) return;
   const int rcv = (P.ncp + q - 1) / q;
-  const int v0 = (phase == 2) ? 0 : c * rcv;
-  const int v1 = (phase == 2) ? P.ncp : min(P.ncp, v0 + rcv);
+  const int v0 = c * rcv;
+  const int v1 = min(P.ncp, v0 + rcv);
   apply_rows(sm, P.V, P.ncp, v0, v1, I, J, b);
 }
 
@@ -860,7 +1065,7 @@ __global__ void svd_init_kernel(SvdBatch Bt) {
         }
       }
       reinterpret_cast<double2*>(P.X)[e] = v;
-    } else {
+    } else if (P.accum) {
       const size_t f = e - nx;
       const int r = (int)(f % P.ncp), c = (int)(f / P.ncp);
       v.x = (r == c) ? 1.0 : 0.0;
@@ -985,6 +1190,7 @@ __global__ void svd_write_kernel(SvdBatch Bt) {
   for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < nu + nvh; e += (size_t)gridDim.x * blockDim.x) {
     S2 o;
     if (e < nu) {
+      if (!U) continue;  // U not requested
       const int r = (int)(e % P.m), j = (int)(e / P.m);
       const int pj = P.perm[j];
       const double s = P.sigma[j];
@@ -1047,10 +1253,16 @@ int launch_svd_init(const SvdBatch& b, int64_t nprob, int64_t max_elems, int max
 int launch_svd_round(const SvdBatch& b, int64_t nprob, int max_items, int round, int phase, int mode,
                      hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
   const dim3 grid((unsigned)max_items, (unsigned)nprob);
-  if (ev0)
+  if (phase == 2) {
+    if (ev0)
+      hipExtLaunchKernelGGL(svd_round_mfma_kernel, grid, dim3(SR_THREADS), 0, s, ev0, ev1, 0, b, round, mode);
+    else
+      hipLaunchKernelGGL(svd_round_mfma_kernel, grid, dim3(SR_THREADS), 0, s, b, round, mode);
+  } else if (ev0) {
     hipExtLaunchKernelGGL(svd_round_kernel, grid, dim3(SR_THREADS), 0, s, ev0, ev1, 0, b, round, phase, mode);
-  else
+  } else {
     hipLaunchKernelGGL(svd_round_kernel, grid, dim3(SR_THREADS), 0, s, b, round, phase, mode);
+  }
   TENET_HIP_CHECK(hipGetLastError());
   return TENET_OK;
 }

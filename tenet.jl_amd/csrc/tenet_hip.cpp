@@ -1074,6 +1074,7 @@ static int svd_run(tenet_ctx ctx, hipStream_t s, int dtype, int64_t m, int64_t n
   P.m = (int32_t)m;
   P.n = (int32_t)n;
   P.transposed = sh.transposed ? 1 : 0;
+  P.accum = 1;
   P.A = A;
   P.lda = lda;
   P.U = U;
@@ -1846,9 +1847,18 @@ extern "C" int tenet_grouped_svd(tenet_ctx ctx, tenet_stream stream, int dtype, 
   size_t total = align_up((size_t)count * sizeof(SvdProblem), 256) + align_up((size_t)count * 32, 256) +
                  align_up((size_t)count * 8, 256);
   int64_t max_rounds = 0, max_pairs = 0, max_ncp = 0, max_elems = 0, max_out = 0;
+  const size_t es = ELEM[dtype];
   for (int64_t i = 0; i < count; ++i) {
     CHECK_ARG(ms[i] > 0 && ns[i] > 0 && std::min(ms[i], ns[i]) <= 4096, "bad matrix shape");
-    shapes[(size_t)i] = svd_shape(ms[i], ns[i], false);
+    const int64_t m = ms[i], n = ns[i], k = std::min(m, n);
+    CHECK_ARG(As[i] && Vhs[i], "null A or Vh pointer");
+    TRY(check_span(ctx, As[i], (size_t)m * n * es, "A"));
+    if (Us[i]) TRY(check_span(ctx, Us[i], (size_t)m * k * es, "U"));
+    TRY(check_span(ctx, Vhs[i], (size_t)k * n * es, "Vh"));
+    if (Ss && Ss[i]) TRY(check_span(ctx, Ss[i], (size_t)k * 8, "S"));
+    // Without U and with m <= n, the Jacobi runs on X = A^H: V^H comes from X
+    // itself and the rotations need not be accumulated
+    shapes[(size_t)i] = svd_shape(ms[i], ns[i], false, !Us[i] && m <= n);
     const SvdShape& sh = shapes[(size_t)i];
     total += svd_ws_bytes(sh);
     max_rounds = std::max(max_rounds, sh.nblk - 1);
@@ -1880,6 +1890,7 @@ extern "C" int tenet_grouped_svd(tenet_ctx ctx, tenet_stream stream, int dtype, 
     P.m = (int32_t)ms[i];
     P.n = (int32_t)ns[i];
     P.transposed = sh.transposed ? 1 : 0;
+    P.accum = (Us[i] || !sh.transposed) ? 1 : 0;
     P.A = As[i];
     P.lda = ms[i];
     P.U = Us[i];
@@ -1914,9 +1925,9 @@ extern "C" int tenet_grouped_svd(tenet_ctx ctx, tenet_stream stream, int dtype, 
     for (int64_t i = 0; i < count; ++i) {
       if (hflags[(size_t)(8 * i + 1)]) continue;
       const SvdShape& sh = shapes[(size_t)i];
-      const double n2 = 2.0 * sh.b;
-      fl += 8.0 * sh.npairs * n2 * n2 * (2.0 * sh.nr + sh.ncp);
-      by += 16.0 * sh.npairs * n2 * (3.0 * sh.nr + 2.0 * sh.ncp);
+      const double n2 = 2.0 * sh.b, vr = probs[(size_t)i].accum ? (double)sh.ncp : 0.0;
+      fl += 8.0 * sh.npairs * n2 * n2 * (2.0 * sh.nr + vr);
+      by += 16.0 * sh.npairs * n2 * (3.0 * sh.nr + 2.0 * vr);
     }
     TRY(svd_round_timed(ctx, s, fl, by, B, count, (int)max_pairs, 0, 2, 0));
     for (int r = 0; r < max_rounds; ++r) TRY(svd_round_timed(ctx, s, fl, by, B, count, (int)max_pairs, r, 2, 1));
@@ -1939,9 +1950,14 @@ extern "C" int tenet_grouped_svd(tenet_ctx ctx, tenet_stream stream, int dtype, 
       if (hflags[(size_t)(8 * i + 1)]) continue;
       const int64_t m = ms[i], n = ns[i];
       void* w = nullptr;
-      TRY(ws_get(ctx, s, svd_any_ws_bytes(ctx, dtype, m, n), &w));
-      const int st = svd_any(ctx, s, dtype, m, n, As[i], m, Us[i], m, Ss[i], Vhs[i], std::min(m, n),
-                             maxdims ? maxdims[i] : 0, threshold, k_dev + i, disc_dev + i, w);
+      // the single-matrix path always forms U: scratch for a problem without one
+      const size_t ub = Us[i] ? 0 : align_up((size_t)m * std::min(m, n) * ELEM[dtype], 256);
+      TRY(ws_get(ctx, s, svd_any_ws_bytes(ctx, dtype, m, n) + ub, &w));
+      void* u = Us[i] ? Us[i] : (void*)((char*)w + svd_any_ws_bytes(ctx, dtype, m, n));
+      // k_dev, disc_dev and Ss are optional, as on the grouped path above
+      const int st = svd_any(ctx, s, dtype, m, n, As[i], m, u, m, Ss ? Ss[i] : nullptr, Vhs[i], std::min(m, n),
+                             maxdims ? maxdims[i] : 0, threshold, k_dev ? k_dev + i : nullptr,
+                             disc_dev ? disc_dev + i : nullptr, w);
       if (st != TENET_OK) {
         if (st == TENET_ERR_NOCONV) set_error("grouped Jacobi SVD: some problems did not converge");
         return st;

@@ -131,12 +131,14 @@ class MPS:
         U times sqrt(S) (Householder's Q and R differ only by diagonal phases,
         a gauge), so the move rescales the two sites (for a right move:
         Q = U, R = sqrt(S), next site sqrt(S) sqrt(S) V^H; mirrored for a
-        left move).  Returns False when the shortcut does not apply."""
+        left move).  ``i`` is the left site of the bond the move crosses
+        (bond i-(i+1)) in both directions.  Returns False when the shortcut
+        does not apply."""
         rec = self._gate_bond
         if rec is None:
             return False
         a, sa, sb, sq, isq = rec
-        if (right and i != a) or (not right and i != a + 1) or self.sites[a] is not sa or self.sites[a + 1] is not sb:
+        if i != a or self.sites[a] is not sa or self.sites[a + 1] is not sb:
             return False
         A, B = self.sites[a], self.sites[a + 1]
         if right:  # site a <- A / sqrt(S) on its right bond; site a+1 <- sqrt(S) B on its left bond

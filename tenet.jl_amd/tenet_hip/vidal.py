@@ -186,7 +186,8 @@ class VidalMPS:
         ms = np.array([l * d1 for (l, d1, b, d2, r) in shp], dtype=np.int64)
         ns = np.array([d2 * r for (l, d1, b, d2, r) in shp], dtype=np.int64)
         kmax = np.minimum(ms, ns)
-        Us = [DeviceArray(ctx, (int(m), int(k)), dt) for m, k in zip(ms, kmax)]
+        # U is never used (B_a is formed from theta~ and V): no U pointers, so
+        # the SVD of an m <= n problem runs on X^H and accumulates no rotations
         Vs = [DeviceArray(ctx, (int(k), int(n)), dt) for k, n in zip(kmax, ns)]
         Ss = [ctx.malloc(int(k) * 8) for k in kmax]
         kd = ctx.malloc(P * 8)
@@ -196,7 +197,7 @@ class VidalMPS:
             L.check(L.lib.tenet_grouped_svd(
                 ctx.h, None, self.code, P, ms.ctypes.data_as(C.POINTER(C.c_int64)),
                 ns.ctypes.data_as(C.POINTER(C.c_int64)), (Pp * P)(*[x.ptr for x in X]),
-                (Pp * P)(*[u.ptr for u in Us]), (Pp * P)(*Ss), (Pp * P)(*[v.ptr for v in Vs]),
+                (Pp * P)(*([None] * P)), (Pp * P)(*Ss), (Pp * P)(*[v.ptr for v in Vs]),
                 mds.ctypes.data_as(C.POINTER(C.c_int64)), thr, Pp(kd), None), "grouped_svd")
             ks = np.zeros(P, dtype=np.int64)
             L.check(L.lib.tenet_memcpy_d2h(ctx.h, ks.ctypes.data_as(Pp), Pp(kd), ks.nbytes, None))
@@ -236,7 +237,7 @@ class VidalMPS:
         fresh = self._upload_lams([svals[j] for j in range(P)])
         for j, a in enumerate(bonds):
             self.lam[a] = fresh[j]
-        del Us, Vs, th, tt, X
+        del Vs, th, tt, X
         return [int(k) for k in ks]
 
     def close(self):

@@ -2,6 +2,8 @@
 generated on the device (``DeviceArray.gaussian_``), so that multi-GB
 batches never cross PCIe and any slice can be regenerated on the host.
 
+plain_state_arrays     the n = 20, chi = 32 Gaussian MPS of the plain
+                       canonize!/norm workload
 grouped_contractions   8192 pairwise contractions of rank-3..5 tensors,
                        extents log-uniform in 2..128, 1-3 contracted labels
 grouped_svds           8192 truncated SVDs, sides log-uniform in 16..512
@@ -39,6 +41,18 @@ class ContractionSpec:
         M = int(np.prod([self.extent[l] for l in self.la if l not in shared]))
         N = int(np.prod([self.extent[l] for l in self.lb if l not in shared]))
         return M, N, K
+
+
+def plain_state_arrays(seed: int, n: int = 20, chi: int = 32, d: int = 2):
+    """n sites, uniform bond dimension chi (edges included: site 1 is
+    (1, d, chi) as a 3-way array), standard complex Gaussian entries
+    (N(0,1) + i N(0,1)) / sqrt(2) from numpy's PCG64 stream ``seed``."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        shp = (1 if i == 0 else chi, d, 1 if i == n - 1 else chi)
+        out.append((rng.standard_normal(shp) + 1j * rng.standard_normal(shp)) / np.sqrt(2.0))
+    return out
 
 
 def _log_uniform_int(rng, lo, hi):

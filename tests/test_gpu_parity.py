@@ -718,3 +718,46 @@ def test_gate_bond_diagonal_move_matches_qr(ctx):
     assert _rel(vf, vr) < 1e-12
     for a in psi_fast.to_host()[1:]:
         assert O.is_right_isometry(a, atol=1e-12)
+
+
+def _isometry_pattern_ok(psi, lo, hi, atol=1e-11):
+    """Sites left of lo are left isometries and sites right of hi right
+    isometries (the MixedCanonical(lo..hi) pattern, canonize.jl:128-149)."""
+    sites = psi.to_host()
+    return (all(O.is_left_isometry(a, atol=atol) for a in sites[:lo])
+            and all(O.is_right_isometry(a, atol=atol) for a in sites[hi + 1:]))
+
+
+@pytest.mark.parametrize("case", ["left_across_gate", "widen_then_shrink", "right_across_gate"])
+def test_gate_bond_diagonal_move_directions(ctx, case):
+    """The diagonal shortcut may only replace the canonize! move across the
+    gate's own bond, in either direction.  Each case ends in a form whose
+    every site outside the centre is an isometry, with the same state as the
+    Householder moves give."""
+    import tenet_hip as T
+    from tenet_hip.tebd import haar_gates
+
+    n = 8
+    gates = haar_gates(321, 6)
+    fast = T.MPS.product_state(ctx, [[1.0, 0.0]] * n)
+    ref = T.MPS.product_state(ctx, [[1.0, 0.0]] * n)
+    # entangle the chain so every bond is > 1
+    for g, a in enumerate([0, 2, 4, 6, 1, 3]):
+        for psi in (fast, ref):
+            psi.simple_update_(gates[g], a, maxdim=8, threshold=1e-12)
+            if psi is ref:
+                psi._gate_bond = None
+    # the last gate acted on (3, 4)
+    steps = {"left_across_gate": [(3, 3), (2, 2)],
+             "widen_then_shrink": [(3, 6), (3, 4)],
+             "right_across_gate": [(4, 4), (5, 5)]}[case]
+    for lo, hi in steps:
+        for psi in (fast, ref):
+            psi.canonize_(lo, hi)
+            if psi is ref:
+                psi._gate_bond = None
+        assert fast.form == ref.form == ("mixed", lo, hi)
+        assert _isometry_pattern_ok(fast, lo, hi)
+        vf = O.OMPS(fast.to_host()).to_dense()
+        vr = O.OMPS(ref.to_host()).to_dense()
+        assert _rel(vf, vr) < 1e-12

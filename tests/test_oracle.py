@@ -436,3 +436,90 @@ def test_sample_frequencies_match_born_rule():
     chi2 = np.sum((counts[m] - exp[m]) ** 2 / exp[m])
     dof = int(m.sum()) - 1
     assert chi2 < dof + 5 * np.sqrt(2 * dof)
+
+
+# ---------------------------------------------------------------- Muscle.simple_update (tensor level)
+def test_simple_update_tensor_cx_known_answer():
+    """The CX case of test/unit/simple_update.jl:19-43 at the tensor level:
+    |1>|1> under CX is |1>|0> (site a the control, most significant digit)."""
+    A = O.OTensor(np.array([[0.0], [1.0]], dtype=complex), ["pa", "b"])   # (ia, bond)
+    B = O.OTensor(np.array([[0.0, 1.0]], dtype=complex), ["b", "pb"])     # (bond, ib)
+    CX = np.array([[1, 0, 0, 0], [0, 1, 0, 0], [0, 0, 0, 1], [0, 0, 1, 0]], dtype=complex)
+    G = O.OTensor(CX.reshape(2, 2, 2, 2), ["oa", "ob", "pa", "pb"])
+    nA, nB, s = O.simple_update(A, "pa", B, "pb", "b", G, "oa", "ob")
+    assert nA.labels == ["pa", "b"] and nB.labels == ["b", "pb"]
+    theta = np.einsum("ik,kj->ij", nA.data, nB.data)
+    assert np.allclose(theta, [[0, 0], [1, 0]], atol=1e-15)
+    assert np.allclose(s[0], 1.0) and np.all(s[1:] < 1e-15)
+
+
+@pytest.mark.parametrize("maxdim", [None, 3])
+def test_simple_update_tensor_equals_dense_and_best_rank_k(maxdim):
+    """theta~ = G.(A.B) brute-forced with loops; the factors multiply back to
+    theta~ (no cut) or to its best rank-k approximation (Eckart-Young), and
+    AbsorbEqually puts sqrt(s) on both sides: nA^H nA = diag(s) = nB nB^H."""
+    rng = np.random.default_rng(5)
+    chil, d, chi, chir = 3, 2, 4, 3
+    a = O.complex_gaussian(rng, (chil, d, chi))
+    b = O.complex_gaussian(rng, (chi, d, chir))
+    g = O.haar_unitary(rng, 4).reshape(d, d, d, d)  # (oa, ob, ia, ib)
+    dense = np.zeros((chil, d, d, chir), dtype=complex)
+    for l, oa, ob, r, ia, ib, c in itertools.product(range(chil), range(d), range(d), range(chir), range(d),
+                                                     range(d), range(chi)):
+        dense[l, oa, ob, r] += g[oa, ob, ia, ib] * a[l, ia, c] * b[c, ib, r]
+    A = O.OTensor(a, ["l", "pa", "b"])
+    B = O.OTensor(b, ["b", "pb", "r"])
+    G = O.OTensor(g, ["oa", "ob", "pa", "pb"])
+    nA, nB, s = O.simple_update(A, "pa", B, "pb", "b", G, "oa", "ob", maxdim=maxdim)
+    k = len(s)
+    assert k == (min(chil * d, d * chir) if maxdim is None else maxdim)
+    mat = dense.reshape(chil * d, d * chir)
+    u, sv, vh = np.linalg.svd(mat)
+    best = (u[:, :k] * sv[:k]) @ vh[:k]
+    got = np.einsum("lxk,kyr->lxyr", nA.data, nB.data).reshape(chil * d, d * chir)
+    assert np.linalg.norm(got - best) < 1e-12 * np.linalg.norm(mat)
+    na = nA.data.reshape(chil * d, k)
+    nb = nB.data.reshape(k, d * chir)
+    assert np.allclose(na.conj().T @ na, np.diag(s), atol=1e-12)
+    assert np.allclose(nb @ nb.conj().T, np.diag(s), atol=1e-12)
+
+
+@pytest.mark.parametrize("absorb", ["left", "right"])
+def test_simple_update_tensor_absorb_sides(absorb):
+    """AbsorbLeft keeps V^H an exact right isometry and AbsorbRight keeps U a
+    left isometry; the product is unchanged by where s goes."""
+    rng = np.random.default_rng(6)
+    a = O.complex_gaussian(rng, (2, 2, 3))
+    b = O.complex_gaussian(rng, (3, 2, 2))
+    g = O.haar_unitary(rng, 4).reshape(2, 2, 2, 2)
+    A, B = O.OTensor(a, ["l", "pa", "b"]), O.OTensor(b, ["b", "pb", "r"])
+    G = O.OTensor(g, ["oa", "ob", "pa", "pb"])
+    eA, eB, _ = O.simple_update(A, "pa", B, "pb", "b", G, "oa", "ob")
+    nA, nB, s = O.simple_update(A, "pa", B, "pb", "b", G, "oa", "ob", absorb=absorb)
+    k = len(s)
+    na, nb = nA.data.reshape(4, k), nB.data.reshape(k, 4)
+    if absorb == "left":
+        assert np.allclose(nb @ nb.conj().T, np.eye(k), atol=1e-12)
+    else:
+        assert np.allclose(na.conj().T @ na, np.eye(k), atol=1e-12)
+    assert np.allclose(na @ nb, eA.data.reshape(4, k) @ eB.data.reshape(k, 4), atol=1e-12)
+
+
+# ---------------------------------------------------------------- overlap(ProductState, MPS)
+def test_overlap_product_vs_dense_contraction():
+    """<x|psi> for general (non-basis, complex) product vectors, without
+    conjugating x (overlap.jl:22-32), against the dense state contracted with
+    the Kronecker product of the vectors."""
+    rng = np.random.default_rng(7)
+    psi = O.random_gaussian_mps(rng, 6, 2, 4)
+    xs = [O.complex_gaussian(rng, (2,)) for _ in range(6)]
+    kron = xs[0]
+    for x in xs[1:]:
+        kron = np.kron(kron, x)
+    ref = np.dot(kron, psi.to_dense())  # no conjugation
+    assert abs(O.overlap_product(xs, psi) - ref) < 1e-12 * abs(ref) + 1e-14
+    # basis vectors reduce to the amplitude of that bitstring
+    bits = [1, 0, 0, 1, 1, 0]
+    e = [np.eye(2)[b] for b in bits]
+    idx = int("".join(map(str, bits)), 2)
+    assert abs(O.overlap_product(e, psi) - psi.to_dense()[idx]) < 1e-12
