@@ -1,25 +1,41 @@
 #!/usr/bin/env python3
-"""Benchmark: one TEBD sweep on one MI355X.
+"""Benchmark: the TEBD sweep on one MI355X (and, with --workload, the other
+configurations: plain, heff, grouped, serving).
 
-Workload (the configuration the headline metric is quoted on): open MPS,
-n = 64 sites, d = 2, initial |0...0>, depth-20 brickwork of Haar-random
-4x4 gates (seed 20261203, serpentine order: 630 gates), simple_update! with
-maxdim = 256 and absolute threshold 1e-10.  A "step" is one full sweep of
-630 gates; the state starts on the device and only the kept bond dimension
-k crosses back to the host per gate (the plain form).
+Default workload (the configuration the headline metric is quoted on): open
+MPS, n = 64 sites, d = 2, initial |0...0>, depth-20 brickwork of Haar-random
+4x4 gates (seed 20261203, serpentine order: 630 gates), maxdim = 256 and
+absolute threshold 1e-10.  A "step" is one full sweep of 630 gates; the
+gates and the state are resident on the device before the timed region and
+only kept bond dimensions cross back to the host.
 
-Forms: fp64 = contractions in FP64 MFMA; split_* = contractions in 8-bit
-split slices on bf16/fp16 MFMA with exact FP32 accumulation.  QR and SVD are
-FP64 in every form.  Accuracy: relative l2 error of 1024 seeded amplitudes
-<x|psi> against the FP64 oracle's amplitudes (tests/golden fixture).
+Forms (QR and SVD are FP64 in every form):
+  vidal       the batched form: each brickwork layer's 31-32 gates act on
+              disjoint bonds, so a layer is a few grouped contraction
+              launches and one grouped Jacobi SVD (Vidal gauge, Hastings'
+              inverse-free update); the headline line.  Accuracy against the
+              oracle's Vidal-gauge restatement on the same gates.
+  fp64        the plain FP64 GPU form: simple_update! gate by gate in
+              MixedCanonical form, one SVD at a time with the kept bond
+              dimension read back per gate (no grouping, no graph).
+  split_bf16 / split_fp16   the plain form with its contractions on 8-bit
+              bf16 / fp16 split slices (exact FP32 accumulation).
+  fp32        the plain form with ComplexF32-grade (FP32 MFMA) contractions.
+The plain forms are checked against the oracle's MixedCanonical sweep.
+Accuracy: relative l2 error of 1024 seeded amplitudes <x|psi> against the
+FP64 oracle's (tests/golden fixtures), and every bond dimension equal.
 
 Prints one JSON line (rank 0, N = 1).
 """
 import argparse
 import json
 import os
+import platform
+import statistics
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.join(ROOT, "tenet.jl_amd")
@@ -32,84 +48,151 @@ SEED = 20261202 + 1
 FIXTURE = os.path.join(ROOT, "tests", "golden", "tebd_n64_d20_chi256.npz")
 VIDAL_FIXTURE = os.path.join(ROOT, "tests", "golden", "tebd_vidal_n64_d20_chi256.npz")
 PEAK_FP64_TFLOPS = 78.6
-# HBM bytes per svd_round_kernel dispatch over one TEBD sweep (n=64, chi_max=256), from two counter
-# passes: FETCH_SIZE 6311.0 KiB x 2.0 + WRITE_SIZE 3985.0 KiB x 1.0 = 16607.0 KiB.
-SVD_ROUND_TRAFFIC_BYTES = 17_005_568
+# HBM bytes per round-kernel dispatch (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 KiB,
+# factors from the scale kernel over 1 GiB) for each TEBD form, from rocprofv3
+# --pmc passes over one sweep; None where not measured.
+ROUND_TRAFFIC_BYTES = {"fp64": 17_005_568, "vidal": None}
 
-FORMS = {"fp64": 0, "fp32": 1, "split_bf16": 2, "split_fp16": 3}
+FORMS = {"fp64": 0, "fp32": 1, "split_bf16": 2, "split_fp16": 3, "vidal": 0}
+DEFAULT_FORMS = "vidal,fp64,split_bf16,split_fp16,fp32"
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def run_form(ctx, T, sched, gate_views, prec, steps, warmup, bits):
-    import numpy as np
-    from tenet_hip.tebd import tebd_sweep
-
-    def one():
-        psi = tebd_sweep(ctx, N_SITES, sched, gate_views, CHI, THR, prec=prec)
-        ctx.sync()
-        return psi
-
+def _timed_sweeps(ctx, T, one, steps, warmup, tag):
+    """warmup untimed sweeps, then `steps` timed ones with the round kernels'
+    per-launch timing on (HIP events on their stream); returns the last
+    result, the wall time per sweep and the round-kernel statistics."""
     for w in range(warmup):
         t0 = time.perf_counter()
-        one()
-        log(f"prec {prec} warmup {w} {time.perf_counter() - t0:.3f}s")
+        r = one()
+        log(f"{tag} warmup {w} {time.perf_counter() - t0:.3f}s")
+        if hasattr(r, "close"):
+            r.close()
     ctx.sync()
-    ms = []
-    psi = None
-    # live per-launch timing of the dominant kernel (HIP events on its stream)
     ctx.stats_enable(T._lib.KSTAT_SVD_ROUND)
-    ctx.timer_start()
     t_all = time.perf_counter()
+    res = None
     for s in range(steps):
+        if res is not None and hasattr(res, "close"):
+            res.close()
         t0 = time.perf_counter()
-        psi = one()
-        ms.append((time.perf_counter() - t0) * 1e3)
-        log(f"prec {prec} step {s} {ms[-1]:.1f} ms")
-    wall = time.perf_counter() - t_all
-    ev_ms = ctx.timer_stop()
+        res = one()
+        log(f"{tag} step {s} {(time.perf_counter() - t0) * 1e3:.1f} ms")
+    wall = (time.perf_counter() - t_all) / steps
     kst = ctx.stats_read()
     ctx.stats_enable(T._lib.KSTAT_OFF)
+    return res, wall, kst
+
+
+def run_form(ctx, T, form, sched, layers, gate_views, steps, warmup, bits):
+    from tenet_hip.tebd import tebd_sweep
+    from tenet_hip.vidal import tebd_vidal_sweep
+
+    if form == "vidal":
+        def one():
+            v = tebd_vidal_sweep(ctx, N_SITES, layers, gate_views, CHI, THR)
+            ctx.sync()
+            return v
+    else:
+        def one():
+            psi = tebd_sweep(ctx, N_SITES, sched, gate_views, CHI, THR, prec=FORMS[form])
+            ctx.sync()
+            return psi
+
+    res, wall, kst = _timed_sweeps(ctx, T, one, steps, warmup, form)
+    psi = res.to_mps() if form == "vidal" else res
     amps = psi.amplitudes(bits)
-    flops = sum(psi.flops.values())
-    return {"ms_per_step": wall * 1e3 / steps, "event_ms_per_step": ev_ms / steps, "step_ms": ms, "amps": amps,
-            "flops": flops, "flops_split": dict(psi.flops), "bonds": psi.bondsizes(), "kstats": kst}
+    out = {"ms_per_step": wall * 1e3, "amps": amps, "flops": sum(res.flops.values()),
+           "flops_split": dict(res.flops), "bonds": res.bondsizes(), "kstats": kst, "steps": steps,
+           "warmup": warmup}
+    if form == "vidal":
+        res.close()
+    return out
 
 
-def cpu_baseline(sched, gates, budget_s):
-    """The oracle (numpy/LAPACK restatement of Tenet's simple_update! loop)
-    on the host cores, on the first gates of the same sweep until the time
-    budget is spent."""
-    import numpy as np
+def _cpu_info():
+    model = platform.processor() or ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return model, os.cpu_count()
+
+
+def _saturated_gate_sample(form, rng):
+    """Host tensors of one gate of the saturated sweep: sites (256, 2, 256),
+    a Haar gate, and (Vidal) the Lambda of the left bond."""
     from oracle import tenet_oracle as O
 
-    psi = O.product_state_mps([[1.0, 0.0]] * N_SITES)
-    flops = 0.0
-    t0 = time.perf_counter()
-    done = 0
-    for a, U in zip(sched, gates):
-        # nominal work of this gate with the same conventions as the device path
-        lo, hi = psi.form[1], psi.form[2]
-        for i in range(lo, a):
-            chil, d, chi = psi.A[i].shape
-            if chi > 1:
-                flops += _qr_fl(chil * d, chi) + 8.0 * min(chil * d, chi) * chi * psi.A[i + 1].shape[1] * psi.A[i + 1].shape[2]
-        for i in range(hi, a + 1, -1):
-            chi, d, chir = psi.A[i].shape
-            if chi > 1:
-                flops += _qr_fl(d * chir, chi) + 8.0 * psi.A[i - 1].shape[0] * psi.A[i - 1].shape[1] * chi * min(d * chir, chi)
-        O.canonize_mixed(psi, a, a + 1)
-        chil, d1, chi = psi.A[a].shape
-        _, d2, chir = psi.A[a + 1].shape
-        flops += 8.0 * chil * d1 * chi * d2 * chir + 8.0 * chil * chir * (d1 * d2) ** 2 + _svd_fl(chil * d1, d2 * chir)
-        O.simple_update_mps(psi, U, a, CHI, THR)
-        done += 1
-        if time.perf_counter() - t0 > budget_s:
-            break
-    dt = time.perf_counter() - t0
-    return flops / dt / 1e9, done, dt
+    A = O.complex_gaussian(rng, (CHI, 2, CHI))
+    B = O.complex_gaussian(rng, (CHI, 2, CHI))
+    lam = np.sort(rng.random(CHI))[::-1] if form == "vidal" else None
+    return A, B, O.haar_unitary(rng, 4), lam
+
+
+def _cpu_gate(form, A, B, U, lam):
+    """One saturated gate as the oracle runs it: the Vidal-gauge update
+    (oracle.vidal_gate), or for the plain form the canonize! move onto the
+    gate (a 512 x 256 thin QR and R's contraction into the next site) and
+    Muscle.simple_update (oracle.simple_update: theta, gate, 512 x 512 SVD,
+    truncation to 256, sqrt(s) into both sides)."""
+    from oracle import tenet_oracle as O
+
+    if form == "vidal":
+        # sites 1, 2 of a three-site chain: X = Lambda_0 theta~ is 512 x 512
+        O.vidal_gate(O.OVidal([A[:1].copy(), A, B], [lam, np.ones(CHI)]), U, 1, CHI, THR)
+        return
+    ta = O.OTensor(A, ["l", "pa", "b"])
+    q, r = O.tensor_qr_thin(ta, ["l", "pa"], ["b"], "v")
+    tb = O.binary_einsum(r, O.OTensor(B, ["b", "pb", "r"]), dims=["b"]).rename({"v": "b"})
+    G = O.OTensor(U.reshape(2, 2, 2, 2), ["oa", "ob", "pa", "pb"])
+    O.simple_update(q.rename({"v": "b"}), "pa", tb, "pb", "b", G, "oa", "ob", maxdim=CHI, threshold=THR)
+
+
+def _gate_flops(form):
+    m = 2 * CHI
+    fl = 8.0 * m * CHI * m + 8.0 * CHI * CHI * 16 * 4 + _svd_fl(m, m)  # theta, gate, SVD
+    if form == "vidal":
+        return fl + 8.0 * m * m * CHI  # B_a = theta~ V
+    return fl + _qr_fl(m, CHI) + 8.0 * CHI * CHI * m  # the canonize! move's QR and R . B
+
+
+def cpu_baseline_tebd(form, budget_s):
+    """The oracle (numpy / OpenBLAS LAPACK: gesdd, Householder QR) on the
+    host cores, on a bounded sample of the same workload: gates at the
+    sweep's saturated shape (chi = 256: 512 x 512 bond matrices, 49 of the
+    63 bonds end there), one warm-up block, then the median of 5 timed
+    blocks, at all host threads and at 1 thread (threadpoolctl)."""
+    from threadpoolctl import threadpool_limits
+
+    rng = np.random.default_rng(5)
+    samples = [_saturated_gate_sample(form, rng) for _ in range(4)]
+    model, ncpu = _cpu_info()
+    out = {}
+    for label, threads in (("all", None), ("one", 1)):
+        ctx_lim = threadpool_limits(limits=threads) if threads else threadpool_limits(limits=None)
+        with ctx_lim:
+            t0 = time.perf_counter()
+            _cpu_gate(form, *samples[0])  # warm-up
+            per_gate = time.perf_counter() - t0
+            # block size from the warm-up: ~budget / 12 per block
+            nblk = max(1, min(8, int(budget_s / 12.0 / max(per_gate, 1e-3))))
+            times = []
+            for rep in range(5):
+                t0 = time.perf_counter()
+                for g in range(nblk):
+                    _cpu_gate(form, *samples[g % len(samples)])
+                times.append((time.perf_counter() - t0) / nblk)
+            med = statistics.median(times)
+        out[label] = {"s_per_gate": med, "gflops": _gate_flops(form) / med / 1e9, "gates_per_block": nblk}
+    threads = int(os.environ.get("OPENBLAS_NUM_THREADS") or ncpu)
+    return out, model, threads
 
 
 def _qr_fl(m, n):
@@ -398,10 +481,14 @@ def bench_grouped(args):
             "fp32_mfma_tflops": round(fl_c / (kst["ms"] / kst["launches"] * 1e-3) / 1e12, 3),
             "fp32_mfma_frac": round(fl_c / (kst["ms"] / kst["launches"] * 1e-3) / 1e12 / 157.3, 4)}
     if kss["launches"] and kss["ms"] > 0:
-        ach = kss["flops"] / (kss["ms"] * 1e-3) / 1e12
+        ach = fl_s * max(1, args.steps) / (kss["ms"] * 1e-3) / 1e12
+        executed = kss["flops"] / (kss["ms"] * 1e-3) / 1e12
         roof = {"bound": "mfma", "achieved": round(ach, 4), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(ach / PEAK_FP64_TFLOPS, 5), "traffic": None,
-                "kernel": "svd_round_kernel (grouped: a round of every unconverged SVD in each launch)",
+                "achieved_basis": "nominal SVD flops 4(4p^2q+8pq^2+9q^3) of the batch / round-kernel time",
+                "executed_tflops": round(executed, 3), "executed_frac": round(executed / PEAK_FP64_TFLOPS, 4),
+                "kernel": "svd_round_mfma_kernel (grouped: a round of every unconverged SVD in each launch; Gram "
+                          "and rotation on v_mfma_f64_16x16x4)",
                 "launches": kss["launches"], "avg_launch_us": round(kss["ms"] * 1e3 / kss["launches"], 2),
                 "kernel_share_of_step": round(kss["ms"] / (total * 1e3 * max(1, args.steps)), 3),
                 "contraction_kernel": contraction_kernel}
@@ -528,31 +615,7 @@ def bench_serving(args):
     ctx.close()
 
 
-# ---------------------------------------------------------------- single SVD (profiling unit)
-def bench_svd(args):
-    """tenet_svd_thin of one seeded 512 x 512 complex128 Gaussian matrix with
-    maxdim 256 and threshold 1e-10 (the shape of a saturated TEBD bond),
-    repeated --steps times: the unit the SVD kernels' counters are read on."""
-    import tenet_hip as T
-    from tenet_hip.tensor import tensor_svd_thin
-
-    ctx = T.Context(0)
-    ctx.svd_precondition(bool(args.svd_precond))
-    A = T.Tensor(T.DeviceArray(ctx, (512, 512)).gaussian_(77), ["r", "c"])
-    for _ in range(args.warmup):
-        tensor_svd_thin(A, ["r"], "s", ["c"], maxdim=256, threshold=1e-10)
-    ctx.sync()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        tensor_svd_thin(A, ["r"], "s", ["c"], maxdim=256, threshold=1e-10)
-    ctx.sync()
-    dt = (time.perf_counter() - t0) / args.steps
-    print(json.dumps({"metric": "svd 512x512 c128 maxdim 256", "ms": round(dt * 1e3, 3),
-                      "qr_precondition": bool(args.svd_precond)}), flush=True)
-    ctx.close()
-
-
-# ---------------------------------------------------------------- Vidal-gauge TEBD
+# ---------------------------------------------------------------- TEBD (default)
 def layers_of(sched, n):
     """Split the serpentine brickwork schedule into its layers (order kept,
     so gate g still acts on bond sched[g])."""
@@ -565,129 +628,105 @@ def layers_of(sched, n):
     return out
 
 
-def _vidal_gate_flops(chil, d1, chi, d2, chir):
-    return 8.0 * chil * d1 * chi * d2 * chir + 8.0 * chil * chir * (d1 * d2) ** 2 + _svd_fl(chil * d1, d2 * chir)
-
-
-def cpu_baseline_vidal(layers, gates, budget_s):
-    """The oracle's Vidal-gauge update on the host cores, layer after layer of
-    the same sweep until the time budget is spent."""
-    from oracle import tenet_oracle as O
-
-    v = O.vidal_product_state([[1.0, 0.0]] * N_SITES)
-    flops, done, g = 0.0, 0, 0
-    t0 = time.perf_counter()
-    for bonds in layers:
-        for a in bonds:
-            chil, d1, chi = v.B[a].shape
-            _, d2, chir = v.B[a + 1].shape
-            flops += _vidal_gate_flops(chil, d1, chi, d2, chir)
-            O.vidal_gate(v, gates[g], a, CHI, THR)
-            g += 1
-            done += 1
-        if time.perf_counter() - t0 > budget_s:
-            break
-    dt = time.perf_counter() - t0
-    return flops / dt / 1e9, done, dt
-
-
-def bench_tebd_vidal(args):
-    """The TEBD sweep of the default workload (same Haar gates on the same
-    bonds) in the Vidal gauge: each brickwork layer's gates are independent,
-    so a layer is four grouped contraction launches and one grouped SVD
-    (tenet_hip.vidal).  Accuracy against the oracle's Vidal restatement
-    (tests/golden/tebd_vidal_n64_d20_chi256.npz)."""
-    import numpy as np
+def bench_tebd(args):
     import tenet_hip as T
     from tenet_hip.tebd import brickwork_schedule, haar_gates, upload_gates
-    from tenet_hip.vidal import tebd_vidal_sweep
 
-    fx = np.load(VIDAL_FIXTURE)
-    bits = np.unpackbits(fx["bits"], axis=1)[:, :N_SITES]
+    fx = {"plain": np.load(FIXTURE), "vidal": np.load(VIDAL_FIXTURE)}
+    bits = np.unpackbits(fx["plain"]["bits"], axis=1)[:, :N_SITES]
+    assert np.array_equal(fx["plain"]["bits"], fx["vidal"]["bits"])
     sched = brickwork_schedule(N_SITES, DEPTH)
     gates = haar_gates(SEED, len(sched))
     layers = layers_of(list(sched), N_SITES)
+    forms = [f for f in args.forms.split(",") if f]
+    head = forms[0]
     ctx = T.Context(0)
-    gv = upload_gates(ctx, gates)
-
-    def one():
-        v = tebd_vidal_sweep(ctx, N_SITES, layers, gv, CHI, THR)
-        ctx.sync()
-        return v
-
-    for w in range(args.warmup):
-        t0 = time.perf_counter()
-        one().close()
-        log(f"vidal warmup {w} {time.perf_counter() - t0:.3f}s")
-    ctx.stats_enable(T._lib.KSTAT_SVD_ROUND)
-    t_all = time.perf_counter()
-    v = None
-    for s in range(args.steps):
-        if v is not None:
-            v.close()
-        t0 = time.perf_counter()
-        v = one()
-        log(f"vidal step {s} {(time.perf_counter() - t0) * 1e3:.1f} ms")
-    wall = (time.perf_counter() - t_all) / args.steps
-    k = ctx.stats_read()
-    ctx.stats_enable(T._lib.KSTAT_OFF)
-    amps = v.to_mps().amplitudes(bits)
-    rel = float(np.linalg.norm(amps - fx["amps"]) / np.linalg.norm(fx["amps"]))
-    bonds_ok = bool(list(v.bondsizes()) == list(fx["bonds"]))
-    flops = sum(v.flops.values())
-    gflops = flops / wall / 1e9
-    log(f"vidal sweep {wall * 1e3:.1f} ms, relerr {rel:.2e}, bonds match {bonds_ok}")
+    gate_views = upload_gates(ctx, gates)
+    results = {}
+    for form in forms:
+        steps = args.steps if form == head else min(args.steps, args.cmp_steps)
+        warm = args.warmup if form == head else min(args.warmup, 1)
+        log(f"form {form}: {warm} warmup + {steps} timed sweeps")
+        r = run_form(ctx, T, form, sched, layers, gate_views, steps, warm, bits)
+        ref = fx["vidal" if form == "vidal" else "plain"]
+        r["relerr"] = float(np.linalg.norm(r["amps"] - ref["amps"]) / np.linalg.norm(ref["amps"]))
+        r["bonds_match"] = bool(list(r["bonds"]) == list(ref["bonds"]))
+        if form == "vidal":  # distance to the MixedCanonical oracle (a different truncation order)
+            pa = fx["plain"]["amps"]
+            r["dev_vs_mixedcanonical_oracle"] = float(np.linalg.norm(r["amps"] - pa) / np.linalg.norm(pa))
+        results[form] = r
+        log(f"form {form}: {r['ms_per_step']:.1f} ms/sweep, relerr {r['relerr']:.2e}, bonds match "
+            f"{r['bonds_match']}, {r['flops'] / 1e9:.1f} GFLOP nominal")
+    r = results[head]
+    base = results.get("fp64")
+    # the round kernel: nominal SVD work of the sweep over the kernel's time
+    k = r["kstats"]
     roof = None
     if k["launches"] > 0 and k["ms"] > 0:
-        ach = k["flops"] / (k["ms"] * 1e-3) / 1e12
+        svd_nominal = r["flops_split"]["svd"] * r["steps"]
+        ach = svd_nominal / (k["ms"] * 1e-3) / 1e12
+        executed = k["flops"] / (k["ms"] * 1e-3) / 1e12
         roof = {"bound": "mfma", "achieved": round(ach, 4), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(ach / PEAK_FP64_TFLOPS, 5), "traffic": None,
-                "kernel": "svd_round_kernel (grouped: every SVD of a layer in each launch)",
+                "frac": round(ach / PEAK_FP64_TFLOPS, 5), "traffic": ROUND_TRAFFIC_BYTES.get(head),
+                "kernel": ("svd_round_mfma_kernel (grouped one-sided Jacobi rounds: every SVD of a brickwork layer "
+                           "per launch; Gram and rotation on v_mfma_f64_16x16x4, inner 32x32 solve on FP64 VALU)"
+                           if head == "vidal" else
+                           "svd_round_kernel (one-sided Jacobi round of one SVD: pair Grams, rotations, apply; "
+                           "FP64 VALU)"),
+                "achieved_basis": "nominal SVD flops 4(4p^2q+8pq^2+9q^3) of the sweep / round-kernel time",
+                "peak_basis": "FP64 (vector and matrix peaks are both 78.6 TFLOP/s on MI355X)",
+                "executed_tflops": round(executed, 3), "executed_frac": round(executed / PEAK_FP64_TFLOPS, 4),
                 "launches": k["launches"], "avg_launch_us": round(k["ms"] * 1e3 / k["launches"], 2),
-                "kernel_share_of_step": round(k["ms"] / (wall * 1e3 * args.steps), 3)}
+                "kernel_share_of_step": round(k["ms"] / (r["ms_per_step"] * r["steps"]), 3),
+                "algorithmic_flops_per_launch": round(svd_nominal / k["launches"])}
     cpu = None
     if not args.no_cpu:
-        threads = os.environ.get("OPENBLAS_NUM_THREADS") or str(os.cpu_count())
-        cg, ng, dt = cpu_baseline_vidal(layers, gates, args.cpu_seconds)
-        cpu = {"value": round(cg, 3), "unit": "GFLOP/s", "cores": int(threads), "kind": "port",
-               "sample": f"oracle Vidal-gauge update on the first {ng} of 630 gates of the same sweep "
-                         f"({dt:.1f} s, numpy/OpenBLAS LAPACK, {threads} threads)"}
+        cform = "vidal" if head == "vidal" else "fp64"
+        res, model, threads = cpu_baseline_tebd(cform, args.cpu_seconds)
+        est_sweep = r["flops"] / (res["all"]["gflops"] * 1e9)
+        cpu = {"value": round(res["all"]["gflops"], 3), "unit": "GFLOP/s", "cores": threads, "kind": "port",
+               "sample": (f"oracle {'Vidal-gauge update' if cform == 'vidal' else 'canonize! move + simple_update'} "
+                          f"at the sweep's saturated shape (chi 256, 512x512 SVD), median of 5 blocks of "
+                          f"{res['all']['gates_per_block']} gates after a warm-up; numpy/OpenBLAS LAPACK on "
+                          f"{threads} threads ({model}); at this rate the full sweep's nominal work takes "
+                          f"{est_sweep:.1f} s"),
+               "one_thread": {"value": round(res["one"]["gflops"], 3), "unit": "GFLOP/s",
+                              "gates_per_block": res["one"]["gates_per_block"]},
+               "cpu_model": model, "host_cpus": os.cpu_count()}
     line = {
-        "metric": "TEBD sweep time & GFLOP/s, Vidal gauge (brickwork layer = grouped launches)",
-        "value": round(gflops, 3), "unit": "GFLOP/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round(wall * 1e3, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "c128", "data": "synthetic (seeded Haar gates, |0...0>)",
-        "config": {"workload": "TEBD brickwork n=64 d=2 depth=20 chi_max=256 threshold=1e-10 (630 gates), Vidal gauge",
-                   "relerr_vs_fp64_oracle": rel, "bonds_match_oracle": bonds_ok,
-                   "gflop_nominal": round(flops / 1e9, 2)},
-        "roofline": roof, "cpu_baseline": cpu,
+        "metric": "TEBD sweep time & GFLOP/s split-prec vs FP64 @relerr<=1e-8; req/s + p99 latency",
+        "value": round(r["flops"] / (r["ms_per_step"] * 1e-3) / 1e9, 3),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": r["steps"],
+        "warmup": r["warmup"],
+        "ms_per_step": round(r["ms_per_step"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c128",
+        "data": "synthetic (seeded Haar gates, |0...0>)",
+        "config": {
+            "workload": "TEBD brickwork n=64 d=2 depth=20 chi_max=256 threshold=1e-10 (630 gates)",
+            "form": head,
+            "relerr_vs_fp64_oracle": r["relerr"],
+            "bonds_match_oracle": r["bonds_match"],
+            "speedup_vs_plain_fp64": round(base["ms_per_step"] / r["ms_per_step"], 3) if base else None,
+            "forms": {f: {"ms_per_sweep": round(v["ms_per_step"], 3), "timed_sweeps": v["steps"],
+                          "relerr_vs_own_fp64_oracle": v["relerr"], "bonds_match_oracle": v["bonds_match"],
+                          "gflop_nominal": round(v["flops"] / 1e9, 2),
+                          "round_kernel_share": (round(v["kstats"]["ms"] / (v["ms_per_step"] * v["steps"]), 3)
+                                                 if v["kstats"]["launches"] else None),
+                          **({"dev_vs_mixedcanonical_oracle": v["dev_vs_mixedcanonical_oracle"]}
+                             if "dev_vs_mixedcanonical_oracle" in v else {})}
+                      for f, v in results.items()},
+            "oracle": {"vidal": "oracle Vidal-gauge sweep (tests/golden/tebd_vidal_n64_d20_chi256.npz)",
+                       "plain forms": "oracle MixedCanonical sweep (tests/golden/tebd_n64_d20_chi256.npz)"},
+        },
+        "roofline": roof,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
-    v.close()
-    ctx.close()
-
-
-# ---------------------------------------------------------------- single QR (profiling unit)
-def bench_qr(args):
-    """tenet_qr_thin of seeded 512 x 256 and 512 x 512 complex128 Gaussian
-    matrices (a canonize move at chi = 256, and a square bond matrix),
-    repeated --steps times each."""
-    import tenet_hip as T
-    from tenet_hip.tensor import tensor_qr_thin
-
-    ctx = T.Context(0)
-    res = {}
-    for shp in ((512, 256), (512, 512)):
-        A = T.Tensor(T.DeviceArray(ctx, shp).gaussian_(78), ["r", "c"])
-        for _ in range(args.warmup):
-            tensor_qr_thin(A, ["r"], ["c"], "v")
-        ctx.sync()
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            tensor_qr_thin(A, ["r"], ["c"], "v")
-        ctx.sync()
-        res[f"{shp[0]}x{shp[1]}"] = round((time.perf_counter() - t0) / args.steps * 1e3, 3)
-    print(json.dumps({"metric": "thin QR c128 ms", **res}), flush=True)
     ctx.close()
 
 
@@ -821,32 +860,31 @@ def bench_plain(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=None, help="timed steps (default 2; 20 for the plain workload)")
+    ap.add_argument("--steps", type=int, default=None, help="timed steps of the headline form (default 5; 20 plain)")
     ap.add_argument("--warmup", type=int, default=None, help="untimed steps (default 1; 3 for plain)")
-    ap.add_argument("--forms", default="fp64,split_bf16")
+    ap.add_argument("--forms", default=None,
+                    help=f"TEBD forms, headline first (default {DEFAULT_FORMS}); heff: fp64,fp32,split_bf16,split_fp16")
+    ap.add_argument("--cmp-steps", type=int, default=2, help="timed sweeps of each comparison TEBD form")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--svd-precond", type=int, default=1, help="QR-preconditioned Jacobi SVD (1) or plain (0)")
-    ap.add_argument("--gates", type=int, default=0,
-                    help="profile only: apply the first N gates of the sweep (amplitude check skipped)")
     ap.add_argument("--copy-mib", type=int, default=1024, help="traffic calibration buffer (workload copy)")
-    ap.add_argument("--workload", default="tebd", choices=["tebd", "tebd-vidal", "heff", "plain", "grouped", "serving", "copy", "svd", "qr"])
+    ap.add_argument("--workload", default="tebd", choices=["tebd", "heff", "plain", "grouped", "serving", "copy"])
     ap.add_argument("--rates", default="250,500,1000,2000,4000,8000,16000")
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     args = ap.parse_args()
     latency = args.workload == "plain"  # a ~1 ms step: more samples
     if args.steps is None:
-        args.steps = 20 if latency else 2
+        args.steps = 20 if latency else 5
     if args.warmup is None:
         args.warmup = 3 if latency else 1
     if args.gpus != 1:
-        print("bench.py measures one MI355X: this engine has no multi-GPU path (the TEBD sweep is "
-              "sequential in MixedCanonical form); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one MI355X and runs no multi-GPU path (every GPU box here has one GPU); "
+              "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     if args.workload == "heff":
-        if args.forms == "fp64,split_bf16":
-            args.forms = "fp64,fp32,split_bf16,split_fp16"
+        args.forms = args.forms or "fp64,fp32,split_bf16,split_fp16"
         return bench_heff(args)
+    args.forms = args.forms or DEFAULT_FORMS
     if args.workload == "plain":
         return bench_plain(args)
     if args.workload == "grouped":
@@ -855,92 +893,7 @@ def main():
         return bench_serving(args)
     if args.workload == "copy":
         return bench_copy(args)
-    if args.workload == "svd":
-        return bench_svd(args)
-    if args.workload == "tebd-vidal":
-        return bench_tebd_vidal(args)
-    if args.workload == "qr":
-        return bench_qr(args)
-
-    import numpy as np
-    import tenet_hip as T
-    from tenet_hip.tebd import brickwork_schedule, haar_gates, upload_gates
-
-    fx = np.load(FIXTURE)
-    bits = np.unpackbits(fx["bits"], axis=1)[:, :N_SITES]
-    ref_amps = fx["amps"]
-    sched = brickwork_schedule(N_SITES, DEPTH)
-    gates = haar_gates(SEED, len(sched))
-    if args.gates:
-        # profiling runs: a prefix of the sweep, no accuracy check, no JSON line
-        import tenet_hip.tebd as TB
-        ctx = T.Context(0)
-        gv = upload_gates(ctx, gates[:args.gates])
-        t0 = time.perf_counter()
-        psi = TB.tebd_sweep(ctx, N_SITES, sched[:args.gates], gv, CHI, THR)
-        ctx.sync()
-        log(f"{args.gates} gates in {time.perf_counter() - t0:.2f}s, bonds {psi.bondsizes()}")
-        ctx.close()
-        return
-    ctx = T.Context(0)
-    ctx.svd_precondition(bool(args.svd_precond))
-    gate_views = upload_gates(ctx, gates)
-    results = {}
-    for form in [f for f in args.forms.split(",") if f]:
-        log(f"form {form}")
-        r = run_form(ctx, T, sched, gate_views, FORMS[form], args.steps, args.warmup, bits)
-        r["relerr"] = float(np.linalg.norm(r["amps"] - ref_amps) / np.linalg.norm(ref_amps))
-        r["bonds_match"] = bool(list(r["bonds"]) == list(fx["bonds"]))
-        results[form] = r
-        log(f"form {form}: {r['ms_per_step']:.1f} ms/sweep, relerr {r['relerr']:.2e}, "
-            f"{r['flops'] / 1e9:.1f} GFLOP nominal")
-    main_form = min(results, key=lambda f: results[f]["ms_per_step"] if results[f]["relerr"] <= 1e-8 else 1e30)
-    r = results[main_form]
-    gflops = r["flops"] / (r["ms_per_step"] * 1e-3) / 1e9
-
-    k = r["kstats"]
-    roof = None
-    if k["launches"] > 0 and k["ms"] > 0:
-        ach = k["flops"] / (k["ms"] * 1e-3) / 1e12
-        roof = {"bound": "mfma", "achieved": round(ach, 4), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(ach / PEAK_FP64_TFLOPS, 5), "traffic": SVD_ROUND_TRAFFIC_BYTES,
-                "traffic_source": "rocprofv3 --pmc FETCH_SIZE x2.0 + WRITE_SIZE x1.0 per dispatch (factors from "
-                                  "scale_kernel over 256 MiB), averaged over one sweep of this workload",
-                "kernel": "svd_round_kernel (one-sided Jacobi round: pair Grams, rotations, apply)",
-                "launches": k["launches"], "avg_launch_us": round(k["ms"] * 1e3 / k["launches"], 2),
-                "kernel_share_of_step": round(k["ms"] / (r["ms_per_step"] * args.steps), 3),
-                "algorithmic_bytes_per_launch": round(k["bytes"] / k["launches"])}
-    cpu = None
-    if not args.no_cpu:
-        threads = os.environ.get("OPENBLAS_NUM_THREADS") or str(os.cpu_count())
-        cg, ng, dt = cpu_baseline(sched, gates, args.cpu_seconds)
-        cpu = {"value": round(cg, 3), "unit": "GFLOP/s", "cores": int(threads), "kind": "port",
-               "sample": f"oracle simple_update! loop on the first {ng} of 630 gates of the same sweep "
-                         f"({dt:.1f} s, numpy/OpenBLAS LAPACK, {threads} threads)"}
-
-    line = {
-        "metric": "TEBD sweep time & GFLOP/s split-prec vs FP64 @relerr<=1e-8; req/s + p99 latency",
-        "value": round(gflops, 3),
-        "unit": "GFLOP/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(r["ms_per_step"], 3),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "c128" if main_form == "fp64" else f"c128 ({main_form} contractions, fp64 qr/svd)",
-        "data": "synthetic (seeded Haar gates, |0...0>)",
-        "config": {"workload": "TEBD brickwork n=64 d=2 depth=20 chi_max=256 threshold=1e-10 (630 gates)",
-                   "form": main_form, "relerr_vs_fp64_oracle": r["relerr"], "svd_qr_precondition": bool(args.svd_precond),
-                   "forms": {f: {"ms_per_sweep": round(v["ms_per_step"], 3), "relerr": v["relerr"],
-                                 "gflop_nominal": round(v["flops"] / 1e9, 2), "bonds_match_oracle": v["bonds_match"]}
-                             for f, v in results.items()}},
-        "roofline": roof,
-        "cpu_baseline": cpu,
-    }
-    print(json.dumps(line), flush=True)
-    ctx.close()
+    return bench_tebd(args)
 
 
 if __name__ == "__main__":
