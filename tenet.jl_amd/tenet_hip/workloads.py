@@ -132,8 +132,13 @@ def build_serving_states(ctx, seed: int, count: int, n_lo=16, n_hi=128, chi_lo=1
                          dtype=np.complex64):
     """``count`` device-resident MPS states: shapes from serving_state_shapes,
     complex Gaussian entries generated on the device (state j uses seed
-    seed + j, its sites consecutive offsets of that stream), then
-    canonize!(psi, site 1) and normalize!.  Returns ServedState objects."""
+    seed + j, its sites consecutive offsets of that stream), each site
+    scaled by 1 / sqrt(d chi_right), then canonize!(psi, site 1) and
+    normalize!.  The scale leaves the normalised state unchanged; without it
+    E<psi|psi> = d^n prod(chi_b) (about 1e66 at n = 32, chi = 213, and
+    1e330 at n = 128, chi = 512), so the centre site that the canonize
+    sweep accumulates the norm into overflows complex64; with it
+    E<psi|psi> = 1.  Returns ServedState objects."""
     from .device import DeviceArray
     from .mps import MPS
     from .serving import ServedState
@@ -147,6 +152,7 @@ def build_serving_states(ctx, seed: int, count: int, n_lo=16, n_hi=128, chi_lo=1
         off = 0
         for i in range(n):
             a = DeviceArray(ctx, (dims[i], 2, dims[i + 1]), dtype).gaussian_(seed + j, off)
+            a.scale_(1.0 / math.sqrt(2.0 * dims[i + 1]))
             off += a.size
             sites.append(a)
         psi = MPS(ctx, sites, dtype=dtype)
