@@ -163,6 +163,21 @@ int tenet_qr_absorb(tenet_ctx ctx, tenet_stream stream, int dtype, int flags, in
 int tenet_svd_thin(tenet_ctx ctx, tenet_stream stream, int dtype, int64_t m, int64_t n, const void* A, int64_t lda,
                    void* U, int64_t ldu, double* S, void* Vh, int64_t ldvh, int64_t maxdim, double threshold,
                    int64_t* k_dev, double* discarded_dev);
+/* Allocation-free variant of tenet_svd_thin.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, outputs and truncation as tenet_svd_thin;
+ * the Jacobi workspace is a per-GPU static device array of
+ * TENET_SVD_NOALLOC_SCRATCH_BYTES (held until the process ends), so no
+ * hipMalloc / hipFree or allocating routine is called.  It runs the plain
+ * sweeps (no QR preconditioning; every shape with min(m, n) <= 512 and
+ * max(m, n) <= 1024 fits); a shape whose workspace does not fit returns
+ * TENET_ERR_UNSUPPORTED.  Every call on one GPU shares that GPU's array:
+ * calls that overlap on one GPU (two streams, two threads) race. */
+#define TENET_SVD_NOALLOC_SCRATCH_BYTES ((size_t)24 << 20)
+int tenet_svd_thin_noalloc(tenet_ctx ctx, tenet_stream stream, int dtype, int64_t m, int64_t n, const void* A,
+                           int64_t lda, void* U, int64_t ldu, double* S, void* Vh, int64_t ldvh, int64_t maxdim,
+                           double threshold, int64_t* k_dev, double* discarded_dev);
 /* Complex128 matrices with 64 <= min(m, n) and max(m, n) <= 512 take a thin
  * Householder QR first and run the Jacobi on R^H (fewer sweeps, same
  * relative accuracy); on by default, 0 turns it off (tenet_svd_thin and

@@ -62,6 +62,33 @@ struct tenet_graph_s {
   int device = 0;
 };
 
+// Workspace of tenet_svd_thin_noalloc: a file-scope device array, one copy
+// per GPU, held until the process ends.  Host code never passes `scratch`
+// itself to a kernel (on the host that name is only a stand-in for the
+// symbol): tenet_create looks up each device's address once, with that
+// device current, and the entry point hands that address to the kernels.
+static __device__ __align__(256) unsigned char scratch[TENET_SVD_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices] = {};
+static hipError_t g_scratch_err[kMaxDevices] = {};
+static bool g_scratch_done[kMaxDevices] = {};
+
+// Device address of `scratch` on `device` (the current device), looked up
+// on the first call for that device; the lookup's status is kept with it.
+static hipError_t scratch_lookup(int device) {
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch_done[device]) {
+    void* p = nullptr;
+    g_scratch_err[device] = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    g_scratch_ptr[device] = g_scratch_err[device] == hipSuccess ? p : nullptr;
+    if (g_scratch_err[device] != hipSuccess) (void)hipGetLastError();
+    g_scratch_done[device] = true;
+  }
+  return g_scratch_err[device];
+}
+
 struct tenet_ctx_s {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -287,6 +314,13 @@ extern "C" int tenet_create(int device, tenet_ctx* out) {
     return TENET_ERR_NODEVICE;
   }
   TENET_HIP_CHECK(hipSetDevice(device));
+  if (device >= kMaxDevices) {
+    set_error("device index above the library's per-device table");
+    return TENET_ERR_ARG;
+  }
+  // this device's copy of the allocation-free SVD's scratch array; a failed
+  // lookup is reported by tenet_svd_thin_noalloc, the only user
+  (void)scratch_lookup(device);
   auto* c = new tenet_ctx_s();
   c->device = device;
   TENET_HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -1239,6 +1273,39 @@ extern "C" int tenet_svd_thin(tenet_ctx ctx, tenet_stream stream, int dtype, int
   void* w = nullptr;
   TRY(ws_get(ctx, s, svd_any_ws_bytes(ctx, dtype, m, n), &w));
   return svd_any(ctx, s, dtype, m, n, A, lda, U, ldu, S, Vh, ldvh, maxdim, threshold, k_dev, discarded_dev, w);
+}
+
+extern "C" int tenet_svd_thin_noalloc(tenet_ctx ctx, tenet_stream stream, int dtype, int64_t m, int64_t n,
+                                      const void* A, int64_t lda, void* U, int64_t ldu, double* S, void* Vh,
+                                      int64_t ldvh, int64_t maxdim, double threshold, int64_t* k_dev,
+                                      double* discarded_dev) {
+  TRY(use_device(ctx));
+  TRY(valid_dtype_prec(dtype, 0));
+  CHECK_ARG(m > 0 && n > 0, "empty matrix");
+  const int64_t k = std::min(m, n);
+  CHECK_ARG(lda >= m && ldu >= m && ldvh >= k, "leading dimension too small");
+  const size_t es = ELEM[dtype];
+  TRY(check_span(ctx, A, ((size_t)(n - 1) * lda + m) * es, "A"));
+  TRY(check_span(ctx, U, ((size_t)(k - 1) * ldu + m) * es, "U"));
+  TRY(check_span(ctx, Vh, ((size_t)(n - 1) * ldvh + k) * es, "Vh"));
+  if (S) TRY(check_span(ctx, S, (size_t)k * 8, "S"));
+  // the plain Jacobi path (no QR preconditioning: its contractions build
+  // plans, which allocate); its whole workspace must fit the scratch array
+  const size_t need = k <= 4096 ? svd_ws_bytes(svd_shape(m, n, true)) : SIZE_MAX;
+  if (need > TENET_SVD_NOALLOC_SCRATCH_BYTES) {
+    set_error("tenet_svd_thin_noalloc: a " + std::to_string(m) + " x " + std::to_string(n) + " SVD needs " +
+              std::to_string(need) + " bytes of workspace, above the " +
+              std::to_string((size_t)TENET_SVD_NOALLOC_SCRATCH_BYTES) +
+              "-byte scratch array (tenet_svd_thin allocates instead)");
+    return TENET_ERR_UNSUPPORTED;
+  }
+  const hipError_t e = scratch_lookup(ctx->device);
+  if (e != hipSuccess) {
+    set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    return TENET_ERR_HIP;
+  }
+  return svd_run(ctx, pick(ctx, stream), dtype, m, n, A, lda, U, ldu, S, Vh, ldvh, maxdim, threshold, k_dev,
+                 discarded_dev, g_scratch_ptr[ctx->device], true);
 }
 
 // ---------------------------------------------------------------------------

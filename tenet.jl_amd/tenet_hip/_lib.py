@@ -102,6 +102,7 @@ SIGNATURES = {
     "tenet_qr_thin_ex": [_p, _p, _i, _i, _i64, _i64, _p, _i64, _p, _i64, _p, _i64],
     "tenet_qr_absorb": [_p, _p, _i, _i, _i64, _i64, _p, _i64, _p, _i64, _i, _i64, _p, _i64, _p, _i64],
     "tenet_svd_thin": [_p, _p, _i, _i64, _i64, _p, _i64, _p, _i64, _p, _p, _i64, _i64, _d, _p, _p],
+    "tenet_svd_thin_noalloc": [_p, _p, _i, _i64, _i64, _p, _i64, _p, _i64, _p, _p, _i64, _i64, _d, _p, _p],
     "tenet_svd_set_precondition": [_p, _i],
     "tenet_simple_update": [_p, _p, _i, _i, _i64, _i64, _i64, _i64, _i64, _p, _p, _p, _p, _p, _p, _i64, _d, _i,
                             _p, _p, _pi64],
