@@ -26,20 +26,95 @@ from .vidal import _leading_rows
 
 
 class MPO:
-    def __init__(self, ctx: Context, sites, dtype=np.complex128):
+    """Canonical forms: Tenet runs the MPS code on an MPO (canonize!(::MPO),
+    canonize.jl:255; compress!(::MPO), compress.jl:72), with the site's
+    non-bond legs (o, i) as the matricised rows.  Here the column-major
+    (chi_l, o, i, chi_r) site is the (chi_l, o i, chi_r) MPS site with a
+    fused physical index, so the MPS methods run on views and the result
+    is viewed back; ``form`` and ``lam`` follow the MPS conventions."""
+
+    def __init__(self, ctx: Context, sites, dtype=np.complex128, form=None):
         self.ctx = ctx
         self.sites = list(sites)
         self.dtype = np.dtype(dtype)
+        self.form = form if form is not None else ("mixed", 0, len(self.sites) - 1)  # MPO.jl: MixedCanonical(all)
+        self.lam = None
 
     @property
     def n(self):
         return len(self.sites)
 
+    def bondsizes(self):
+        return [s.shape[3] for s in self.sites[:-1]]
+
+    def copy(self):
+        o = MPO(self.ctx, [s.copy() for s in self.sites], self.dtype, self.form)
+        o.lam = None if self.lam is None else self.lam.copy()
+        return o
+
+    # ---- canonical forms on the fused (o, i) index -------------------------------
+    def _as_mps(self) -> MPS:
+        m = MPS(self.ctx, [s.view((s.shape[0], s.shape[1] * s.shape[2], s.shape[3])) for s in self.sites],
+                self.form, self.dtype)
+        m.lam = self.lam
+        return m
+
+    def _take(self, m: MPS):
+        dims = [(s.shape[1], s.shape[2]) for s in self.sites]
+        self.sites = [a.view((a.shape[0], o, i, a.shape[2])) for a, (o, i) in zip(m.sites, dims)]
+        self.form = m.form
+        self.lam = m.lam
+        return self
+
+    def canonize_(self, lo, hi=None):
+        """canonize!(::MPO, MixedCanonical) (canonize.jl:121-255)."""
+        return self._take(self._as_mps().canonize_(lo, hi))
+
+    def canonize_bond_(self, i):
+        """canonize!(::MPO, BondCanonical(bond i-(i+1)))."""
+        return self._take(self._as_mps().canonize_bond_(i))
+
+    def compress_bond_(self, i, maxdim=None, threshold=None):
+        """compress!(::MPO, bond) (compress.jl:29-72)."""
+        return self._take(self._as_mps().compress_bond_(i, maxdim, threshold))
+
+    def compress_(self, maxdim=None, threshold=None):
+        """compress!(::MPO) over all bonds (compress.jl:21-27, 72)."""
+        return self._take(self._as_mps().compress_(maxdim, threshold))
+
+    truncate_ = compress_
+
+    def norm(self):
+        """Frobenius norm of the operator (norm.jl:17-37 on an MPO)."""
+        m = self._as_mps()
+        v = m.norm()
+        self._take(m)
+        return v
+
     @staticmethod
-    def from_host(ctx: Context, arrays_lroi, dtype=np.complex128):
+    def from_host(ctx: Context, arrays_lroi, dtype=np.complex128, form=None):
         """From host W[l, r, o, i] site arrays (size-1 edge bonds)."""
         sites = [DeviceArray.from_host(ctx, np.transpose(np.asarray(W), (0, 2, 3, 1)), dtype) for W in arrays_lroi]
-        return MPO(ctx, sites, dtype)
+        return MPO(ctx, sites, dtype, form)
+
+    @staticmethod
+    def from_arrays(ctx: Context, arrays, order=("l", "r", "o", "i"), dtype=np.complex128):
+        """Tenet's ``MPO(arrays; order)`` (MPO.jl:41-132): the edge arrays
+        lack the l (first site) or r (last site) leg."""
+        order = list(order)
+        n = len(arrays)
+        out = []
+        for j, arr in enumerate(arrays):
+            loc = [x for x in order if not ((j == 0 and x == "l") or (j == n - 1 and x == "r"))]
+            a = np.asarray(arr)
+            if j == 0:
+                a = a[None]
+                loc = ["l"] + loc
+            if j == n - 1:
+                a = a[..., None]
+                loc = loc + ["r"]
+            out.append(np.transpose(a, [loc.index(x) for x in ("l", "r", "o", "i")]))
+        return MPO.from_host(ctx, out, dtype)
 
     @staticmethod
     def from_bulk(ctx: Context, W, n, left_vec, right_vec, dtype=np.complex128):
@@ -48,6 +123,10 @@ class MPO:
         first = np.einsum("l,lroi->roi", np.asarray(left_vec, W.dtype), W, optimize=False)[None]
         last = np.einsum("lroi,r->loi", W, np.asarray(right_vec, W.dtype), optimize=False)[:, None]
         return MPO.from_host(ctx, [first] + [W] * (n - 2) + [last], dtype)
+
+    def to_host_sites(self):
+        """Site arrays in the device layout (chi_l, o, i, chi_r)."""
+        return [s.to_host() for s in self.sites]
 
     def to_host(self):
         """Site arrays as W[l, r, o, i]."""
