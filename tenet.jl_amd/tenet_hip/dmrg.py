@@ -283,3 +283,61 @@ def dmrg2_(psi: MPS, H, nsweeps=4, maxdim=128, kdim=8, tol=1e-10, maxiter=60, cu
     psi.lam = None
     psi._gate_bond = None
     return energy
+
+
+def heff1_apply(L_env: Tensor, W: Tensor, R_env: Tensor, x: Tensor, prec=L.PREC_NATIVE, labels=("l", "o", "r")):
+    """One-site EffectiveHamiltonian(le, re, op)(x) (DMRG.jl:10-18 with op =
+    the site's MPO tensor, as Dmrg1 builds it at DMRG.jl:117, 171):
+    y = ((L . x) . W) . R.  Labels: L_env (l, w, lk), x (lk, i, rk),
+    W (w, o, i, wr), R_env (r, wr, rk); y comes back as ``labels``."""
+    t1 = binary_einsum(L_env, x, prec=prec)  # (l, w, i, rk)
+    t2 = binary_einsum(t1, W)  # (l, rk, o, wr)
+    y = binary_einsum(t2, R_env, prec=prec)  # (l, o, r)
+    return permute(y, list(labels))
+
+
+def dmrg1_(psi: MPS, H, nsweeps=4, kdim=8, tol=1e-10, maxiter=60):
+    """One-site DMRG (Dmrg1, src/Algorithms/DMRG.jl:44-212) on ``psi`` with
+    the MPO ``H``: right environments from a right-canonical state (centre
+    at site 0), then per sweep a left-to-right pass over sites 1..n-1 and a
+    right-to-left pass over sites n-2..0 (0-based; each pass skips its first
+    site, DMRG.jl:90, 145).  At each site: canonize! onto it (one QR move),
+    update the environment left behind, and replace the site by the lowest
+    eigenvector of the one-site H_eff (restarted Lanczos on the device).
+    Bond dimensions never change.  Returns the last eigenvalue."""
+    n = psi.n
+    ctx, dt = psi.ctx, psi.dtype
+    psi.canonize_(0)
+    ones = DeviceArray.from_host(ctx, np.ones((1, 1, 1)), dt)
+    R = [None] * (n + 1)
+    Lf = [None] * (n + 1)
+    R[n] = ones
+    Lf[0] = ones
+    for j in range(n - 1, 0, -1):
+        R[j] = _env_right(psi, R[j + 1], psi.sites[j], H.sites[j])
+    energy = 0.0
+
+    def solve(i):
+        x = psi.sites[i]
+        Le = Tensor(Lf[i], ["l", "w", "lk"])
+        Re = Tensor(R[i + 1], ["r", "wr", "rk"])
+        W = Tensor(H.sites[i], ["w", "o", "i", "wr"])
+
+        def apply(v):
+            return heff1_apply(Le, W, Re, Tensor(v, ["lk", "i", "rk"])).data
+
+        return lanczos_ground(apply, x, kdim, tol, maxiter)
+
+    for _ in range(nsweeps):
+        for i in range(1, n):  # left to right
+            psi.canonize_(i)
+            Lf[i] = _env_left(psi, Lf[i - 1], psi.sites[i - 1], H.sites[i - 1])
+            energy, y = solve(i)
+            psi.sites[i] = y
+        for i in range(n - 2, -1, -1):  # right to left
+            psi.canonize_(i)
+            R[i + 1] = _env_right(psi, R[i + 2], psi.sites[i + 1], H.sites[i + 1])
+            energy, y = solve(i)
+            psi.sites[i] = y
+    psi._gate_bond = None
+    return energy

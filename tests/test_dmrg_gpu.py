@@ -69,3 +69,55 @@ def test_dmrg2_xxz_matches_ed(ctx):
     e = dmrg2_(psi, H, nsweeps=4, maxdim=16)
     e_ed = np.linalg.eigvalsh(O.mpo_dense([W] * n, lv, rv))[0]
     assert abs(e - e_ed) < 1e-9 * abs(e_ed)
+
+
+# ---- one-site DMRG (Dmrg1, src/Algorithms/DMRG.jl:44-212) -----------------
+def test_dmrg1_ising_product_start(ctx):
+    """test/unit/dmrg.jl:10-14: from |0...0> (bond dimension 1, where each
+    local problem is 2 x 2 and every eigensolver is exact) ten sweeps end at
+    -1.1902477482849715 per site (reference tolerance 1e-4)."""
+    import tenet_hip as T
+    from tenet_hip.dmrg import dmrg1_
+
+    H, _, _ = _ising(ctx)
+    psi = T.MPS.product_state(ctx, [[1.0, 0.0]] * N)
+    e = dmrg1_(psi, H, nsweeps=10)
+    assert psi.bondsizes() == [1] * (N - 1)
+    assert abs(e / N - (-1.1902477482849715)) < 1e-4
+
+
+def test_dmrg1_ising_ghz_start(ctx):
+    """test/unit/dmrg.jl:16-23: from the bond-dimension-2 state
+    MPS([[1 0; 0 1], delta..., [1 0; 0 1]]) ten sweeps end at
+    -1.2372129315563634 per site (reference tolerance 1e-4)."""
+    import tenet_hip as T
+    from tenet_hip.dmrg import dmrg1_
+
+    H, _, _ = _ising(ctx)
+    delta = np.zeros((2, 2, 2), dtype=np.complex128)
+    delta[0, 0, 0] = 1.0
+    delta[1, 1, 1] = 1.0
+    arrs = [np.eye(2)] + [delta] * (N - 2) + [np.eye(2)]
+    psi = T.MPS.from_arrays(ctx, arrs)
+    e = dmrg1_(psi, H, nsweeps=10)
+    assert psi.bondsizes() == [2] * (N - 1)
+    assert abs(e / N - (-1.2372129315563634)) < 1e-4
+
+
+def test_dmrg1_reaches_ed_from_full_bond_state(ctx):
+    """With the bond dimensions of the exact ground state (2^min(b, n-b))
+    one-site DMRG converges to the exact-diagonalisation energy, and the
+    state it returns has that energy."""
+    import tenet_hip as T
+    from tenet_hip.dmrg import dmrg1_
+
+    H, lv, rv = _ising(ctx)
+    rng = np.random.default_rng(5)
+    arrs = O.random_gaussian_mps(rng, N, 2, 32, cap_edges=True).A
+    psi = T.MPS.from_host(ctx, arrs)
+    e = dmrg1_(psi, H, nsweeps=6)
+    Hd = O.mpo_dense([O.ising_mpo_W(1.0, 1.0)] * N, lv, rv)
+    e_ed = np.linalg.eigvalsh(Hd)[0]
+    assert abs(e - e_ed) < 1e-9 * abs(e_ed)
+    v = O.OMPS(psi.to_host()).to_dense()
+    assert abs(np.vdot(v, Hd @ v).real / np.vdot(v, v).real - e_ed) < 1e-8 * abs(e_ed)
