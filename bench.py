@@ -15,6 +15,8 @@ Forms (QR and SVD are FP64 in every form):
               launches and one grouped Jacobi SVD (Vidal gauge, Hastings'
               inverse-free update); the headline line.  Accuracy against the
               oracle's Vidal-gauge restatement on the same gates.
+  vidal_graph the same batched form with the steady layers (bond
+              dimensions saturated) replayed from captured HIP graphs.
   fp64        the plain FP64 GPU form: simple_update! gate by gate in
               MixedCanonical form, one SVD at a time with the kept bond
               dimension read back per gate (no grouping, no graph).
@@ -49,12 +51,13 @@ FIXTURE = os.path.join(ROOT, "tests", "golden", "tebd_n64_d20_chi256.npz")
 VIDAL_FIXTURE = os.path.join(ROOT, "tests", "golden", "tebd_vidal_n64_d20_chi256.npz")
 PEAK_FP64_TFLOPS = 78.6
 # HBM bytes per round-kernel dispatch (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 KiB,
-# factors from the scale kernel over 1 GiB) for each TEBD form, from rocprofv3
-# --pmc passes over one sweep; None where not measured.
-ROUND_TRAFFIC_BYTES = {"fp64": 17_005_568, "vidal": None}
+# factors from the scale kernel over 256 MiB) for each TEBD form, from
+# rocprofv3 --pmc passes over one sweep; None where not measured.
+ROUND_TRAFFIC_BYTES = {"fp64": 17_005_568, "vidal": 204_619_715, "vidal_graph": 204_619_715}
 
-FORMS = {"fp64": 0, "fp32": 1, "split_bf16": 2, "split_fp16": 3, "vidal": 0}
-DEFAULT_FORMS = "vidal,fp64,split_bf16,split_fp16,fp32"
+FORMS = {"fp64": 0, "fp32": 1, "split_bf16": 2, "split_fp16": 3, "vidal": 0, "vidal_graph": 0}
+DEFAULT_FORMS = "vidal,vidal_graph,fp64,split_bf16,split_fp16,fp32"
+VIDAL_FORMS = ("vidal", "vidal_graph")
 
 
 def log(msg):
@@ -91,9 +94,9 @@ def run_form(ctx, T, form, sched, layers, gate_views, steps, warmup, bits):
     from tenet_hip.tebd import tebd_sweep
     from tenet_hip.vidal import tebd_vidal_sweep
 
-    if form == "vidal":
+    if form in VIDAL_FORMS:
         def one():
-            v = tebd_vidal_sweep(ctx, N_SITES, layers, gate_views, CHI, THR)
+            v = tebd_vidal_sweep(ctx, N_SITES, layers, gate_views, CHI, THR, graphs=(form == "vidal_graph"))
             ctx.sync()
             return v
     else:
@@ -103,12 +106,12 @@ def run_form(ctx, T, form, sched, layers, gate_views, steps, warmup, bits):
             return psi
 
     res, wall, kst = _timed_sweeps(ctx, T, one, steps, warmup, form)
-    psi = res.to_mps() if form == "vidal" else res
+    psi = res.to_mps() if form in VIDAL_FORMS else res
     amps = psi.amplitudes(bits)
     out = {"ms_per_step": wall * 1e3, "amps": amps, "flops": sum(res.flops.values()),
            "flops_split": dict(res.flops), "bonds": res.bondsizes(), "kstats": kst, "steps": steps,
            "warmup": warmup}
-    if form == "vidal":
+    if form in VIDAL_FORMS:
         res.close()
     return out
 
@@ -207,7 +210,7 @@ def _svd_fl(m, n):
 
 # ---------------------------------------------------------------- DMRG H_eff
 HEFF_CHI, HEFF_W, HEFF_SEED = 2048, 5, 20261202 + 3
-HEFF_FORMS = {"fp64": 0, "fp32": 1, "split_bf16": 2, "split_fp16": 3}
+HEFF_FORMS = {"fp64": 0, "fp32": 1, "split_bf16": 2, "split_fp16": 3, "split_i8": 4, "split_i8_fast": 5}
 HEFF_SAMPLE_ROWS = 256
 
 
@@ -296,10 +299,20 @@ def bench_heff(args):
     roof = None
     if k["launches"] > 0 and k["ms"] > 0:
         ach = k["flops"] / (k["ms"] * 1e-3) / 1e12
-        peak = PEAK_FP64_TFLOPS if main_form == "fp64" else (157.3 if main_form == "fp32" else 2500.0)
+        # peak of the format the result is accurate to: FP64 for the split
+        # forms (their executed slice products against the matrix-core peak
+        # of their slice type are reported beside it)
+        peak = 157.3 if main_form == "fp32" else PEAK_FP64_TFLOPS
         roof = {"bound": "mfma", "achieved": round(ach, 4), "peak": peak, "unit": "TFLOP/s",
                 "frac": round(ach / peak, 5), "traffic": None, "kernel": "contraction kernels of the apply",
                 "launches": k["launches"], "avg_launch_us": round(k["ms"] * 1e3 / k["launches"], 2)}
+        pairs = {"split_bf16": 21, "split_fp16": 21, "split_i8": 28, "split_i8_fast": 15}.get(main_form)
+        if pairs:
+            mpeak = 5000.0 if "i8" in main_form else 2500.0
+            ex = ach * pairs
+            roof.update({"executed_slice_products_tops": round(ex, 2), "slice_mfma_peak": mpeak,
+                         "executed_frac": round(ex / mpeak, 4),
+                         "executed_basis": f"{pairs} slice-pair products, each the apply's nominal work"})
     line = {
         "metric": "DMRG two-site H_eff apply throughput (nominal complex flops)",
         "value": round(flops / (r["ms"] * 1e-3) / 1e9, 3), "unit": "GFLOP/s", "n_gpus": 1,
@@ -575,7 +588,9 @@ def bench_serving(args):
     if r["amp_launches"] and r["amp_kernel_ms"] > 0:
         ach = r["amp_bytes"] / (r["amp_kernel_ms"] * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": round(ach, 2), "peak": 8000.0, "unit": "GB/s", "frac": round(ach / 8000, 5),
-                "traffic": None, "kernel": "amplitude_kernel (one workgroup per request, batched per loop pass)",
+                "traffic": None, "kernel": "amp_site_kernel (site-parallel amplitudes: one launch per site position, each "
+                                           "request's slice product over ceil(chi/16) workgroups) and amplitude_kernel "
+                                           "(one workgroup per request, batches whose bonds stay below 64)",
                 "launches": r["amp_launches"], "avg_launch_us": round(r["amp_kernel_ms"] * 1e3 / r["amp_launches"], 1)}
     cpu = None
     if not args.no_cpu:
@@ -648,10 +663,10 @@ def bench_tebd(args):
         warm = args.warmup if form == head else min(args.warmup, 1)
         log(f"form {form}: {warm} warmup + {steps} timed sweeps")
         r = run_form(ctx, T, form, sched, layers, gate_views, steps, warm, bits)
-        ref = fx["vidal" if form == "vidal" else "plain"]
+        ref = fx["vidal" if form in VIDAL_FORMS else "plain"]
         r["relerr"] = float(np.linalg.norm(r["amps"] - ref["amps"]) / np.linalg.norm(ref["amps"]))
         r["bonds_match"] = bool(list(r["bonds"]) == list(ref["bonds"]))
-        if form == "vidal":  # distance to the MixedCanonical oracle (a different truncation order)
+        if form in VIDAL_FORMS:  # distance to the MixedCanonical oracle (a different truncation order)
             pa = fx["plain"]["amps"]
             r["dev_vs_mixedcanonical_oracle"] = float(np.linalg.norm(r["amps"] - pa) / np.linalg.norm(pa))
         results[form] = r
@@ -670,7 +685,7 @@ def bench_tebd(args):
                 "frac": round(ach / PEAK_FP64_TFLOPS, 5), "traffic": ROUND_TRAFFIC_BYTES.get(head),
                 "kernel": ("svd_round_mfma_kernel (grouped one-sided Jacobi rounds: every SVD of a brickwork layer "
                            "per launch; Gram and rotation on v_mfma_f64_16x16x4, inner 32x32 solve on FP64 VALU)"
-                           if head == "vidal" else
+                           if head in VIDAL_FORMS else
                            "svd_round_kernel (one-sided Jacobi round of one SVD: pair Grams, rotations, apply; "
                            "FP64 VALU)"),
                 "achieved_basis": "nominal SVD flops 4(4p^2q+8pq^2+9q^3) of the sweep / round-kernel time",
@@ -681,7 +696,7 @@ def bench_tebd(args):
                 "algorithmic_flops_per_launch": round(svd_nominal / k["launches"])}
     cpu = None
     if not args.no_cpu:
-        cform = "vidal" if head == "vidal" else "fp64"
+        cform = "vidal" if head in VIDAL_FORMS else "fp64"
         res, model, threads = cpu_baseline_tebd(cform, args.cpu_seconds)
         est_sweep = r["flops"] / (res["all"]["gflops"] * 1e9)
         cpu = {"value": round(res["all"]["gflops"], 3), "unit": "GFLOP/s", "cores": threads, "kind": "port",
@@ -812,6 +827,23 @@ def bench_plain(args):
     ctx.stats_sample_every(17)
     ov, nr = vals[-1]
 
+    # the same step as one captured graph (canonize.jl:136-145 sweep, overlap,
+    # norm): every replay re-reads the source state and rewrites the same
+    # output buffers; its values equal the eager step's bit for bit
+    from tenet_hip.graphs import CanonizeNormGraph
+
+    src = MPS.from_host(ctx, arrs)
+    g = CanonizeNormGraph(src, 0)
+    for _ in range(max(1, args.warmup)):
+        g.run()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        gv = g.run()
+    wall_graph = (time.perf_counter() - t0) / args.steps
+    graph_equal = gv == (ov, nr)
+    g.close()
+    log(f"plain: eager {wall * 1e6:.1f} us/step, graph {wall_graph * 1e6:.1f} us/step, equal {graph_equal}")
+
     rel = None
     cpu = None
     if not args.no_cpu:
@@ -843,14 +875,18 @@ def bench_plain(args):
                           "R^H absorbed into the neighbour, one workgroup)",
                 "launches": kst["launches"], "avg_launch_us": round(kst["ms"] * 1e3 / kst["launches"], 2),
                 "note": "latency-bound: 19 dependent moves, 32 barrier-separated reflector steps each"}
+    best = min(wall, wall_graph)
     line = {
-        "metric": "plain canonize!+norm latency (n=20, chi=32)", "value": round(wall * 1e6, 1), "unit": "us/step",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall * 1e3, 4),
+        "metric": "plain canonize!+norm latency (n=20, chi=32)", "value": round(best * 1e6, 1), "unit": "us/step",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(best * 1e3, 4),
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "c128",
         "data": "synthetic (seeded Gaussian MPS)",
         "config": {"workload": "open MPS n=20 d=2 chi=32 Gaussian: canonize!(site 1) + overlap + norm",
                    "gflop_nominal": round((fl_qr + fl_ov) / 1e9, 4), "overlap": [ov.real, ov.imag], "norm": nr,
-                   "overlap_relerr_vs_fp64_oracle": rel},
+                   "overlap_relerr_vs_fp64_oracle": rel,
+                   "forms": {"eager": {"us_per_step": round(wall * 1e6, 1)},
+                             "graph": {"us_per_step": round(wall_graph * 1e6, 1),
+                                       "bit_identical_to_eager": bool(graph_equal)}}},
         "roofline": roof, "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
@@ -882,7 +918,7 @@ def main():
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     if args.workload == "heff":
-        args.forms = args.forms or "fp64,fp32,split_bf16,split_fp16"
+        args.forms = args.forms or "fp64,fp32,split_bf16,split_fp16,split_i8,split_i8_fast"
         return bench_heff(args)
     args.forms = args.forms or DEFAULT_FORMS
     if args.workload == "plain":

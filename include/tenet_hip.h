@@ -43,15 +43,20 @@ typedef enum {
 typedef enum { TENET_C128 = 0, TENET_C64 = 1 } tenet_dtype;
 
 /* Arithmetic of a contraction.  NATIVE = the dtype's own precision
- * (FP64 for C128, FP32 for C64).  SPLIT_* = operands split into 8-bit
- * slices (bf16 / fp16 MFMA), exact FP32 accumulation in short K blocks,
- * FP64 summation across blocks and slices; the result is written in the
- * tensor dtype. */
+ * (FP64 for C128, FP32 for C64).  SPLIT_BF16 / SPLIT_FP16 = operands split
+ * into 8-bit slices (bf16 / fp16 MFMA), exact FP32 accumulation in short K
+ * blocks, FP64 summation across blocks and slices.  SPLIT_I8 = 7-bit slices
+ * on int8 MFMA with exact int32 accumulation (seven slices for complex128
+ * results: FP64 grade, 2^-49; four for complex64); SPLIT_I8_FAST = five
+ * slices for complex128 results (2^-35 relative to the row / column
+ * maxima).  The result is written in the tensor dtype. */
 typedef enum {
   TENET_PREC_NATIVE = 0,
   TENET_PREC_FP32 = 1,
   TENET_PREC_SPLIT_BF16 = 2,
-  TENET_PREC_SPLIT_FP16 = 3
+  TENET_PREC_SPLIT_FP16 = 3,
+  TENET_PREC_SPLIT_I8 = 4,
+  TENET_PREC_SPLIT_I8_FAST = 5
 } tenet_prec;
 
 typedef enum { TENET_ABSORB_NONE = 0, TENET_ABSORB_EQUAL = 1, TENET_ABSORB_LEFT = 2, TENET_ABSORB_RIGHT = 3 } tenet_absorb;
@@ -90,6 +95,22 @@ int tenet_memcpy_d2h(tenet_ctx ctx, void* dst, const void* src, size_t bytes, te
 int tenet_memcpy_d2d(tenet_ctx ctx, void* dst, const void* src, size_t bytes, tenet_stream stream);
 int tenet_memset_zero(tenet_ctx ctx, void* dst, size_t bytes, tenet_stream stream); /* fill kernel */
 int tenet_mem_info(tenet_ctx ctx, size_t* free_bytes, size_t* total_bytes, size_t* pool_bytes);
+/* Stream-ordered free: the block returns to the pool once the work queued on
+ * `stream` (NULL: the context's stream) before this call has completed, so it
+ * may be called while kernels on that stream still use the block. */
+int tenet_free_async(tenet_ctx ctx, void* dptr, tenet_stream stream);
+/* Two-phase workspace: the bytes one call of `op` on an m x n matrix
+ * (TENET_OP_SIMPLE_UPDATE: the bond matrix, m = chil*d1, n = d2*chir) takes
+ * from the per-stream workspace, and a call that grows that workspace now
+ * (so that a later graph capture, which cannot grow it, finds it large
+ * enough). */
+typedef enum { TENET_OP_QR = 1, TENET_OP_SVD = 2, TENET_OP_SIMPLE_UPDATE = 3 } tenet_op;
+int tenet_workspace_size(tenet_ctx ctx, int op, int dtype, int64_t m, int64_t n, size_t* bytes);
+int tenet_workspace_reserve(tenet_ctx ctx, tenet_stream stream, size_t bytes);
+/* Library log: level 0 off, 1 every error message as it is set, 2 also
+ * resource events (workspace growth); to `path` (appended) or stderr when
+ * path is NULL or empty.  Process-wide. */
+int tenet_set_log(int level, const char* path);
 
 /* ---- asynchronous request loops (no reference counterpart: the serving
  * loop's plumbing) --------------------------------------------------------
@@ -219,6 +240,17 @@ int tenet_overlap(tenet_ctx ctx, tenet_stream stream, int dtype, int prec, int64
 int tenet_amplitudes(tenet_ctx ctx, tenet_stream stream, int dtype, int64_t nreq, const int64_t* n_dev,
                      const uint64_t* site_ptrs_dev, const uint64_t* chis_dev, const uint64_t* bits_dev, int64_t d,
                      double* out);
+/* Site-parallel form of tenet_amplitudes (same request tables, same
+ * results within rounding): one launch per site position, each request's
+ * vector-slice product spread over ceil(chi_r / 16) workgroups, so that a
+ * large state (chi 512: 2 MiB per selected slice) is read by many CUs at
+ * once instead of one.  n_max / chi_max bound every request's site count
+ * and bond sizes (a request with more sites or a larger bond reports NaN).
+ * Replaces the same call site, overlap(::ProductState, ::MPS) at
+ * src/Operations/overlap.jl:22-32. */
+int tenet_amplitudes_sites(tenet_ctx ctx, tenet_stream stream, int dtype, int64_t nreq, int64_t n_max,
+                           int64_t chi_max, const int64_t* n_dev, const uint64_t* site_ptrs_dev,
+                           const uint64_t* chis_dev, const uint64_t* bits_dev, int64_t d, double* out);
 
 /* Pauli-string expectation <psi|P|psi> on sites [start, start+w) given the
  * left environment env_l (chi_start x chi_start, device; NULL = identity,
@@ -266,7 +298,9 @@ int tenet_grouped_plan_conj(tenet_ctx ctx, int dtype, int prec, int64_t count, c
 int tenet_grouped_exec(tenet_gplan plan, tenet_stream stream, void* const* ptrs);
 /* ptrs_dev: DEVICE array of 3*count pointers; asynchronous, graph-capturable */
 int tenet_grouped_exec_dev(tenet_gplan plan, tenet_stream stream, const uint64_t* ptrs_dev);
-int tenet_grouped_plan_info(tenet_gplan plan, int64_t* units, int64_t* split_tiles, int64_t shapes[3],
+/* shapes: problems per unit kind (64 x 64, 32 x 32, 16 x 16 MFMA tiles;
+ * element-wise units for K <= 8 with an output edge below 16) */
+int tenet_grouped_plan_info(tenet_gplan plan, int64_t* units, int64_t* split_tiles, int64_t shapes[4],
                             double* flops, double* bytes);
 int tenet_grouped_plan_destroy(tenet_gplan plan);
 /* plan + exec + destroy */
@@ -286,6 +320,22 @@ int tenet_autotune_tiles(tenet_ctx ctx, int dtype, double costs_out[3]);
 int tenet_grouped_svd(tenet_ctx ctx, tenet_stream stream, int dtype, int64_t count, const int64_t* ms,
                       const int64_t* ns, const void* const* As, void* const* Us, double* const* Ss,
                       void* const* Vhs, const int64_t* maxdims, double threshold, int64_t* k_dev, double* disc_dev);
+/* The same grouped SVD bound to its operands once (shapes, problem table and
+ * every Jacobi workspace in one device block), so that a run is kernel
+ * launches only and can be captured (tenet_graph_*).  s_mode
+ * TENET_S_DTYPE writes S as complex values of the matrices' dtype (sigma, 0),
+ * so that it can scale a tensor directly (the Vidal-gauge Lambda); S then
+ * holds kmax complex entries.  Eager exec: as tenet_grouped_svd.  Captured
+ * exec: a fixed 30 sweeps, and the context's device status bit 0
+ * (tenet_graph_status) for problems they did not finish. */
+typedef struct tenet_gsvd_s* tenet_gsvd;
+typedef enum { TENET_S_REAL = 0, TENET_S_DTYPE = 1 } tenet_s_mode;
+int tenet_grouped_svd_plan(tenet_ctx ctx, int dtype, int64_t count, const int64_t* ms, const int64_t* ns,
+                           const void* const* As, void* const* Us, void* const* Ss, void* const* Vhs,
+                           const int64_t* maxdims, double threshold, int64_t* k_dev, double* disc_dev, int s_mode,
+                           tenet_gsvd* plan);
+int tenet_grouped_svd_exec(tenet_gsvd plan, tenet_stream stream);
+int tenet_grouped_svd_destroy(tenet_gsvd plan);
 
 /* ---- elementwise helpers ---------------------------------------------------
  * Synthetic inputs on the device: element i of dst becomes a standard
@@ -319,10 +369,23 @@ int tenet_sigma_scales(tenet_ctx ctx, tenet_stream stream, int dtype, int64_t k,
 /* out (device double) = sum |x_i|^2 in FP64 (norm(tensor), norm.jl:28) */
 int tenet_nrm2sq(tenet_ctx ctx, tenet_stream stream, int dtype, int64_t count, const void* x, double* out);
 
-/* ---- HIP graph capture of a sequence of calls on one stream --------------- */
+/* ---- HIP graph capture of a sequence of calls on one stream ---------------
+ * Captured sweeps: the calls between tenet_graph_begin and tenet_graph_end on
+ * `stream` (NULL: the context's stream) are recorded, not run; a launch
+ * replays them (NULL: on the capture stream).  The canonize! / overlap /
+ * norm sweeps (src/Operations/canonize.jl:136-145, overlap.jl:36-50) capture
+ * whole; run a sequence once eagerly first, so that workspaces and plans
+ * exist (neither may grow during a capture).  Calls that read results back
+ * to the host (tenet_grouped_svd, tenet_simple_update with k_host, the
+ * memcpy helpers) cannot be captured.  A captured tenet_svd_thin runs a
+ * fixed 30 sweeps (rounds of a converged matrix return at once) and, if
+ * they did not converge, sets bit 0 of the context's device status word:
+ * tenet_graph_status synchronises the context stream, returns that word and
+ * clears it. */
 int tenet_graph_begin(tenet_ctx ctx, tenet_stream stream);
 int tenet_graph_end(tenet_ctx ctx, tenet_stream stream, tenet_graph* graph);
 int tenet_graph_launch(tenet_graph graph, tenet_stream stream);
+int tenet_graph_status(tenet_ctx ctx, int* status);
 int tenet_graph_destroy(tenet_graph graph);
 
 /* ---- diagnostics -------------------------------------------------------- */

@@ -45,22 +45,32 @@ struct GemmLaunch {
   double alpha_re, alpha_im, beta_re, beta_im;
 };
 
-// prec: 0 native (fp64 for c128), 1 fp32, 2 split-bf16, 3 split-fp16
+// prec: 0 native (fp64 for c128), 1 fp32, 2 split-bf16, 3 split-fp16, 4 / 5 split-int8 (7 / 5 slices)
 int launch_gemm_c128(const GemmLaunch& g, int prec, hipStream_t s, void* ws, size_t ws_bytes);
 int launch_gemm_c64(const GemmLaunch& g, int prec, hipStream_t s, void* ws, size_t ws_bytes);
 size_t gemm_workspace_bytes(const GemmLaunch& g, int dtype, int prec);
+// int8 split precision (k_split8.hip)
+int launch_split8_gemm(const GemmLaunch& g, int dtype, int prec, hipStream_t s, void* ws, size_t ws_bytes);
+size_t split8_workspace_bytes(const GemmLaunch& g, int dtype, int prec);
 
 // Grouped GEMM: problems and work units in device memory, one persistent
 // launch.  Operand pointers come separately (3 per problem: A, B, C) so one
 // plan serves many launches.
-enum { GROUPED_TILE_64 = 0, GROUPED_TILE_32 = 1, GROUPED_TILE_16 = 2 };
+enum { GROUPED_TILE_64 = 0, GROUPED_TILE_32 = 1, GROUPED_TILE_16 = 2, GROUPED_SMALLK = 3 };
+// GROUPED_SMALLK units: SMALLK_ELEMS consecutive output elements each, one
+// thread per element (contractions with K <= 8 and an output edge below 16:
+// Hadamard scalings, gate applications, strided copies, where an MFMA tile
+// would be mostly padding)
+constexpr int64_t SMALLK_ELEMS = 4096;
 struct GroupedProblem {
   GemmTables t;
   int64_t M, N, K, batch;
   int32_t a_mfast, b_nfast;
   int32_t conjA, conjB;     // conjugate operand A / B
-  int32_t shape;            // GROUPED_TILE_*
+  int32_t shape;            // GROUPED_TILE_* / GROUPED_SMALLK
   int32_t kchunks;          // K chunks per tile (1: no split)
+  int32_t order;            // GROUPED_SMALLK: fastest index class (0: m, 1: n, 2: batch), C's unit-stride one
+  int32_t pad0;
   int64_t tiles_m, tiles_n;
   int64_t kc;               // chunk length (multiple of 16)
   int64_t part_base;        // first partial element (complex) of this problem
@@ -123,6 +133,7 @@ struct SvdProblem {
   int32_t nr, ncols, ncp, b, nblk, q;
   int32_t m, n, transposed;
   int32_t accum;  // apply the rotations to V (0: V is never needed, U == NULL with X = A^H)
+  int32_t s_mode; // S output: 0 double, 1 complex128 (sigma, 0), 2 complex64
   const void* A;  // input (m x n, lda)
   int64_t lda;
   void* U;
@@ -148,6 +159,8 @@ int launch_svd_round(const SvdBatch& b, int64_t nprob, int max_items, int round,
 int launch_svd_sweep_end(const SvdBatch& b, int64_t nprob, hipStream_t s);
 int launch_svd_finalize(const SvdBatch& b, int64_t nprob, int max_ncp, int64_t max_elems, int dtype,
                         hipStream_t s);
+// bit 0 of *status <- 1 for every problem not flagged converged
+int launch_svd_status(const SvdBatch& b, int64_t nprob, int32_t* status, hipStream_t s);
 
 // ---- misc elementwise / K4 ----------------------------------------------------
 int launch_fill_zero(void* dst, size_t bytes, hipStream_t s);
@@ -157,6 +170,23 @@ int launch_absorb(const void* U, int64_t m, const void* Vh, int64_t ldvh, int64_
                   hipStream_t s);
 // scale by 1/max|x| and accumulate log(max) into acc[2]; optionally mantissa out
 int launch_rescale_log(void* x, int64_t count, double* acc, int dtype, hipStream_t s);
+// site-parallel amplitudes (k_misc.hip): device tables as for
+// launch_amplitudes, plus chi_max / d bounds and workspace rows (two
+// 2 chi_max double rows per request, one log-scale per request).
+struct AmpSiteArgs {
+  const int64_t* n;
+  const uint64_t* sites;
+  const uint64_t* chis;
+  const uint64_t* bits;
+  int64_t d, chi_max, n_max;
+  double* vbuf;
+  double* logs;
+  double* out;
+};
+// one site position of every request (a no-op for requests with fewer sites);
+// e0 / e1: optional event pair stamped by the dispatch
+int launch_amp_site(const AmpSiteArgs& a, int64_t nreq, int64_t site, int dtype, hipStream_t s, hipEvent_t e0,
+                    hipEvent_t e1);
 int launch_amplitudes(int64_t nreq, const int64_t* n_dev, const uint64_t* site_ptrs, const uint64_t* chis,
                       const uint64_t* bits, int64_t d, double* out, int dtype, int max_chi, hipStream_t s);
 int launch_write_scalar_from(const void* env, double* out, int dtype, hipStream_t s);

@@ -420,6 +420,9 @@ int launch_gemm_c128(const GemmLaunch& g, int prec, hipStream_t s, void* ws, siz
     case TENET_PREC_SPLIT_BF16:
     case TENET_PREC_SPLIT_FP16:
       return launch_split_gemm(g, TENET_C128, prec, s, ws, ws_bytes);
+    case TENET_PREC_SPLIT_I8:
+    case TENET_PREC_SPLIT_I8_FAST:
+      return launch_split8_gemm(g, TENET_C128, prec, s, ws, ws_bytes);
   }
   set_error("unknown precision");
   return TENET_ERR_ARG;
@@ -434,6 +437,9 @@ int launch_gemm_c64(const GemmLaunch& g, int prec, hipStream_t s, void* ws, size
     case TENET_PREC_SPLIT_BF16:
     case TENET_PREC_SPLIT_FP16:
       return launch_split_gemm(g, TENET_C64, prec, s, ws, ws_bytes);
+    case TENET_PREC_SPLIT_I8:
+    case TENET_PREC_SPLIT_I8_FAST:
+      return launch_split8_gemm(g, TENET_C64, prec, s, ws, ws_bytes);
   }
   set_error("unknown precision");
   return TENET_ERR_ARG;
@@ -505,6 +511,61 @@ __device__ __forceinline__ void grouped_unit(const GroupedProblem& q, const uint
   }
 }
 
+// A GROUPED_SMALLK unit: output elements [tile * SMALLK_ELEMS, +SMALLK_ELEMS)
+// of the flattened (m, n, z) space, the class q.order varying fastest (the
+// one with C's smallest stride, so consecutive threads write consecutive
+// addresses); each thread sums its K <= 8 products directly.
+template <typename S2, typename R>
+__device__ __forceinline__ void grouped_smallk_unit(const GroupedProblem& q, const uint64_t* __restrict__ ptrs,
+                                                    const GroupedUnit& u) {
+  const S2* __restrict__ A = reinterpret_cast<const S2*>(ptrs[3 * u.prob]);
+  const S2* __restrict__ B = reinterpret_cast<const S2*>(ptrs[3 * u.prob + 1]);
+  S2* __restrict__ C = reinterpret_cast<S2*>(ptrs[3 * u.prob + 2]);
+  const int64_t total = q.M * q.N * q.batch;
+  const int64_t e0 = u.tile * SMALLK_ELEMS;
+  const int64_t e1 = std::min<int64_t>(total, e0 + SMALLK_ELEMS);
+  // extents in decode order: fastest, middle, slowest
+  const int64_t ext0 = q.order == 0 ? q.M : (q.order == 1 ? q.N : q.batch);
+  const int64_t ext1 = q.order == 0 ? q.N : q.M;
+  const R sa = q.conjA ? (R)-1 : (R)1, sb = q.conjB ? (R)-1 : (R)1;
+  for (int64_t e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
+    const int64_t i0 = e % ext0, rest = e / ext0;
+    const int64_t i1 = rest % ext1, i2 = rest / ext1;
+    int64_t m, n, z;
+    if (q.order == 0) {
+      m = i0;
+      n = i1;
+      z = i2;
+    } else if (q.order == 1) {
+      n = i0;
+      m = i1;
+      z = i2;
+    } else {
+      z = i0;
+      m = i1;
+      n = i2;
+    }
+    const int64_t oa = q.t.rowA[m] + q.t.bA[z], ob = q.t.colB[n] + q.t.bB[z];
+    R cr = 0, ci = 0;
+    // K <= 8, unrolled: the element loop stays the only loop of the unit
+    // (the persistent loop around it must keep a single nested level)
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      if (k < q.K) {
+        const S2 a = A[oa + q.t.kA[k]];
+        const S2 b = B[ob + q.t.kB[k]];
+        const R ar = (R)a.x, ai = sa * (R)a.y, br = (R)b.x, bi = sb * (R)b.y;
+        cr += ar * br - ai * bi;
+        ci += ar * bi + ai * br;
+      }
+    }
+    S2 o;
+    o.x = cr;
+    o.y = ci;
+    C[q.t.rowC[m] + q.t.colC[n] + q.t.bC[z]] = o;
+  }
+}
+
 template <typename S2, typename R>
 __device__ __forceinline__ void grouped_gemm_body(const GroupedProblem* __restrict__ probs,
                                                   const GroupedUnit* __restrict__ units, int64_t nunits,
@@ -535,6 +596,9 @@ __device__ __forceinline__ void grouped_gemm_body(const GroupedProblem* __restri
     const GroupedUnit u = units[ui];
     const GroupedProblem& q = probs[u.prob];
     switch (q.shape) {
+      case GROUPED_SMALLK:
+        grouped_smallk_unit<S2, R>(q, ptrs, u);
+        break;
       case GROUPED_TILE_16:
         grouped_unit<S2, R, Tile16>(q, ptrs, u, partials, tile_ctr, sm, &bcast);
         break;

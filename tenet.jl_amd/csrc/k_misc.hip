@@ -4,6 +4,8 @@
 // K4's batched amplitude sweep <x|psi> (overlap.jl:22-32).
 #include <algorithm>
 
+#include <hip/hip_ext.h>
+
 #include "../../include/tenet_hip.h"
 #include "internal.h"
 
@@ -496,6 +498,149 @@ __global__ __launch_bounds__(AMP_THREADS) void amplitude_kernel(const int64_t* _
     out[3 * req + 1] = vi[cur][0] * scale;
     out[3 * req + 2] = logscale;
   }
+}
+
+// ---- site-parallel amplitudes ------------------------------------------------
+// One launch per site position: workgroup (column block, request) computes
+// AS_COLS entries of v' = (v / m) A[:, x_i, :] for its request, where v is the
+// request's previous vector (read whole from the workspace into LDS) and m
+// its largest |component| (every workgroup of the request computes the same
+// m, so the rescale is deterministic); the workgroup of column block 0 adds
+// log(m) to the request's log-scale.  A request's vectors alternate between
+// two workspace rows; at its last site (chi_r = 1) block 0 writes
+// (re, im, log-scale) to out.  The chain is sequential over sites, so a
+// request's sites are spread over launches instead of one workgroup, and a
+// 512 x 512 slice is read by 32 workgroups at once.
+constexpr int AS_THREADS = 256;
+constexpr int AS_COLS = 16;  // columns per workgroup: 4 per wave
+
+template <typename S2>
+__global__ __launch_bounds__(AS_THREADS) void amp_site_kernel(AmpSiteArgs a, int64_t site) {
+  __shared__ double vr[AMP_MAX_CHI], vi[AMP_MAX_CHI];
+  __shared__ double wm[AS_THREADS / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t req = blockIdx.y;
+  const int64_t n = a.n[req];
+  if (n > a.n_max) {
+    // more sites than the launches cover: report NaN once
+    if (site == 0 && blockIdx.x == 0 && threadIdx.x == 0) {
+      a.out[3 * req + 0] = __builtin_nan("");
+      a.out[3 * req + 1] = __builtin_nan("");
+      a.out[3 * req + 2] = 0.0;
+    }
+    return;
+  }
+  if (site >= n) return;
+  const int64_t* chi = reinterpret_cast<const int64_t*>(a.chis[req]);
+  const uint8_t* x = reinterpret_cast<const uint8_t*>(a.bits[req]);
+  const int64_t cl = chi[site], cr = chi[site + 1];
+  const int64_t cb0 = (int64_t)blockIdx.x * AS_COLS;
+  if (cb0 >= cr && !(cr < 1 && blockIdx.x == 0)) return;
+  double* out = a.out + 3 * req;
+  if (cl < 1 || cr < 1 || cl > a.chi_max || cr > a.chi_max || (int64_t)x[site] >= a.d || (site == 0 && cl != 1) ||
+      (site == n - 1 && cr != 1)) {
+    // uniform across the request's workgroups: report NaN instead of reading
+    // out of bounds, and mark the request so that its later sites skip it
+    if (blockIdx.x == 0 && tid == 0) {
+      out[0] = __builtin_nan("");
+      out[1] = __builtin_nan("");
+      out[2] = 0.0;
+      a.logs[req] = __builtin_nan("");
+    }
+    return;
+  }
+  if (site > 0 && __builtin_isnan(a.logs[req])) return;  // marked at an earlier site
+  double* vin = a.vbuf + ((size_t)req * 2 + ((site + 1) & 1)) * 2 * a.chi_max;
+  double* vout = a.vbuf + ((size_t)req * 2 + (site & 1)) * 2 * a.chi_max;
+  // previous vector into LDS and its largest component
+  double m = 1.0;
+  if (site == 0) {
+    if (tid == 0) {
+      vr[0] = 1.0;
+      vi[0] = 0.0;
+    }
+  } else {
+    double mx = 0.0;
+    for (int64_t l = tid; l < cl; l += AS_THREADS) {
+      const double re = vin[2 * l], im = vin[2 * l + 1];
+      vr[l] = re;
+      vi[l] = im;
+      mx = fmax(mx, fmax(fabs(re), fabs(im)));
+    }
+    mx = wave_max(mx);
+    if (lane == 0) wm[wave] = mx;
+    __syncthreads();
+    m = fmax(fmax(wm[0], wm[1]), fmax(wm[2], wm[3]));
+    if (!(m > 0.0)) m = 1.0;
+  }
+  __syncthreads();
+  const double sc = 1.0 / m;
+  if (blockIdx.x == 0 && tid == 0) a.logs[req] = (site == 0 ? 0.0 : a.logs[req] + log(m));
+  const S2* A = reinterpret_cast<const S2*>(reinterpret_cast<const uint64_t*>(a.sites[req])[site]) +
+                (int64_t)x[site] * cl;
+  const int64_t ld = cl * a.d;
+  const int64_t c0 = cb0 + 4 * wave;
+  double acc[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) acc[j] = 0.0;
+#pragma unroll 2
+  for (int64_t l = lane; l < cl; l += 64) {
+    const double pr = vr[l] * sc, pi = vi[l] * sc;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if (c0 + j < cr) {
+        const S2 v = A[l + (c0 + j) * ld];
+        const double ar = (double)v.x, ai = (double)v.y;
+        acc[2 * j] += pr * ar - pi * ai;
+        acc[2 * j + 1] += pr * ai + pi * ar;
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc[j] += __shfl_xor(acc[j], off, 64);
+  }
+  if (lane < 4 && c0 + lane < cr) {
+    double re = acc[0], im = acc[1];
+#pragma unroll
+    for (int j = 1; j < 4; ++j)
+      if (lane == j) {
+        re = acc[2 * j];
+        im = acc[2 * j + 1];
+      }
+    vout[2 * (c0 + lane)] = re;
+    vout[2 * (c0 + lane) + 1] = im;
+    if (site == n - 1 && c0 + lane == 0) {
+      // the last site has chi_r = 1: its single entry is the amplitude
+      out[0] = re;
+      out[1] = im;
+      out[2] = a.logs[req];
+    }
+  }
+}
+
+int launch_amp_site(const AmpSiteArgs& a, int64_t nreq, int64_t site, int dtype, hipStream_t s, hipEvent_t e0,
+                    hipEvent_t e1) {
+  if (nreq <= 0) return TENET_OK;
+  if (a.chi_max > AMP_MAX_CHI) {
+    set_error("amplitudes: bond dimension above 2048");
+    return TENET_ERR_UNSUPPORTED;
+  }
+  const dim3 grid((unsigned)((a.chi_max + AS_COLS - 1) / AS_COLS), (unsigned)nreq);
+  if (dtype == TENET_C128) {
+    if (e0)
+      hipExtLaunchKernelGGL(amp_site_kernel<double2>, grid, dim3(AS_THREADS), 0, s, e0, e1, 0, a, site);
+    else
+      hipLaunchKernelGGL(amp_site_kernel<double2>, grid, dim3(AS_THREADS), 0, s, a, site);
+  } else {
+    if (e0)
+      hipExtLaunchKernelGGL(amp_site_kernel<float2>, grid, dim3(AS_THREADS), 0, s, e0, e1, 0, a, site);
+    else
+      hipLaunchKernelGGL(amp_site_kernel<float2>, grid, dim3(AS_THREADS), 0, s, a, site);
+  }
+  TENET_HIP_CHECK(hipGetLastError());
+  return TENET_OK;
 }
 
 int launch_amplitudes(int64_t nreq, const int64_t* n_dev, const uint64_t* site_ptrs, const uint64_t* chis,

@@ -53,7 +53,7 @@ static SplitLayout split_layout(const GemmLaunch& g) {
 static size_t align256(size_t x) { return (x + 255) / 256 * 256; }
 
 size_t gemm_workspace_bytes(const GemmLaunch& g, int dtype, int prec) {
-  (void)dtype;
+  if (prec == TENET_PREC_SPLIT_I8 || prec == TENET_PREC_SPLIT_I8_FAST) return split8_workspace_bytes(g, dtype, prec);
   if (prec != TENET_PREC_SPLIT_BF16 && prec != TENET_PREC_SPLIT_FP16) return 0;
   const SplitLayout L = split_layout(g);
   return align256(L.a_bytes) + align256(L.b_bytes) + align256(L.ea_bytes) + align256(L.eb_bytes) + 1024;

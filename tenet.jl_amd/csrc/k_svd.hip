@@ -1153,7 +1153,15 @@ __global__ __launch_bounds__(SV_THREADS) void svd_sort_kernel(SvdBatch Bt) {
   for (int j = tid; j < kmax; j += SV_THREADS) {
     P.sigma[j] = key[j];
     P.perm[j] = idx[j];
-    if (P.S) P.S[j] = key[j];
+    if (P.S) {
+      if (P.s_mode == 1) {
+        reinterpret_cast<double2*>(P.S)[j] = make_double2(key[j], 0.0);
+      } else if (P.s_mode == 2) {
+        reinterpret_cast<float2*>(P.S)[j] = make_float2((float)key[j], 0.0f);
+      } else {
+        P.S[j] = key[j];
+      }
+    }
   }
   // kept count: min(maxdim, #(s > threshold)) -- a prefix since s descends
   int kcap = (P.maxdim > 0 && P.maxdim < kmax) ? (int)P.maxdim : kmax;
@@ -1229,6 +1237,161 @@ __global__ void svd_write_kernel(SvdBatch Bt) {
   }
 }
 
+// Orthonormal completion of the factor built as X_j / sigma_j (U, or V^H's
+// rows when the Jacobi ran on A^H): for kept singular values (j < k of the
+// truncation) at or below 1e-8 sigma_0 that column carries X's rounding
+// noise, not an orthonormal direction (exactly zero sigma: a zero column).  Each such column j, in
+// order, is projected off columns 0..j-1 twice (classical Gram-Schmidt with
+// reorthogonalisation) and normalised; if too little of it survives (below
+// 1e-3 of its length before), the unit vector e_t (t = j, j+1, ... mod L)
+// is used instead.  The result is an orthonormal completion as LAPACK
+// returns it; the product U diag(S) V^H changes by at most
+// 1e-8 sigma_0 * O(eps) per column.  One workgroup per problem; a problem
+// without such columns returns at once.
+constexpr int CP_THREADS = 256;
+constexpr int CP_MAX_K = 4096;
+
+template <typename S2>
+struct Fac {
+  S2* p;
+  int64_t ld;
+  bool adj;  // column j of F is conj(row j of V^H)
+  __device__ __forceinline__ double2 get(int64_t r, int64_t j) const {
+    const S2 v = adj ? p[r * ld + j] : p[j * ld + r];
+    double2 o;
+    o.x = (double)v.x;
+    o.y = adj ? -(double)v.y : (double)v.y;
+    return o;
+  }
+  __device__ __forceinline__ void put(int64_t r, int64_t j, double re, double im) const {
+    S2 o;
+    o.x = re;
+    o.y = adj ? -im : im;
+    if (adj)
+      p[r * ld + j] = o;
+    else
+      p[j * ld + r] = o;
+  }
+};
+
+__device__ __forceinline__ double cp_wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__device__ __forceinline__ double cp_block_sum(double v, double* red) {
+  v = cp_wave_sum(v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = 0.0;
+#pragma unroll
+  for (int w = 0; w < CP_THREADS / 64; ++w) t += red[w];
+  return t;
+}
+
+template <typename S2>
+__global__ __launch_bounds__(CP_THREADS) void svd_complete_kernel(SvdBatch Bt) {
+  const SvdProblem& P = Bt.arr ? Bt.arr[blockIdx.y] : Bt.one;
+  const int kmax = P.ncols;
+  if (kmax > CP_MAX_K) return;
+  Fac<S2> F;
+  int64_t L;
+  if (!P.transposed) {
+    if (!P.U) return;
+    F.p = static_cast<S2*>(P.U);
+    F.ld = P.ldu;
+    F.adj = false;
+    L = P.m;
+  } else {
+    F.p = static_cast<S2*>(P.Vh);
+    F.ld = P.ldvh;
+    F.adj = true;
+    L = P.n;
+  }
+  const double tau = P.sigma[0] * 1e-8;
+  // only the kept columns (the truncation's k, written by the sort kernel)
+  // are completed: the caller drops the others (a TEBD bond matrix of rank
+  // 256 in 512 columns would otherwise orthogonalise 256 discarded ones)
+  const int klim = P.k_dev ? (int)min((int64_t)kmax, *P.k_dev) : kmax;
+  __shared__ double cr[CP_MAX_K], ci[CP_MAX_K];
+  __shared__ double red[CP_THREADS / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  int64_t next_unit = 0;
+  for (int j = 0; j < klim; ++j) {
+    if (P.sigma[j] > tau) continue;  // uniform: sigma is read by every thread
+    double n0 = 0.0;
+    for (int64_t r = tid; r < L; r += CP_THREADS) {
+      const double2 v = F.get(r, j);
+      n0 += v.x * v.x + v.y * v.y;
+    }
+    n0 = sqrt(cp_block_sum(n0, red));
+    bool unit = !(n0 > 0.0);
+    for (int attempt = 0; attempt < 16; ++attempt) {
+      if (unit) {
+        // e_t, t taken in turn from the unit vectors not yet tried
+        const int64_t t = (j + next_unit++) % L;
+        for (int64_t r = tid; r < L; r += CP_THREADS) F.put(r, j, r == t ? 1.0 : 0.0, 0.0);
+        n0 = 1.0;
+        __syncthreads();
+      }
+      for (int pass = 0; pass < 2; ++pass) {
+        // c = F[:, :j]^H v (one wave per column, lanes over rows)
+        for (int i = wave; i < j; i += CP_THREADS / 64) {
+          double sr = 0.0, si = 0.0;
+          for (int64_t r = lane; r < L; r += 64) {
+            const double2 a = F.get(r, i), v = F.get(r, j);
+            sr += a.x * v.x + a.y * v.y;
+            si += a.x * v.y - a.y * v.x;
+          }
+          sr = cp_wave_sum(sr);
+          si = cp_wave_sum(si);
+          if (lane == 0) {
+            cr[i] = sr;
+            ci[i] = si;
+          }
+        }
+        __syncthreads();
+        // v -= F[:, :j] c (one thread per row)
+        for (int64_t r = tid; r < L; r += CP_THREADS) {
+          double2 v = F.get(r, j);
+          for (int i = 0; i < j; ++i) {
+            const double2 a = F.get(r, i);
+            v.x -= a.x * cr[i] - a.y * ci[i];
+            v.y -= a.x * ci[i] + a.y * cr[i];
+          }
+          F.put(r, j, v.x, v.y);
+        }
+        __syncthreads();
+      }
+      double nn = 0.0;
+      for (int64_t r = tid; r < L; r += CP_THREADS) {
+        const double2 v = F.get(r, j);
+        nn += v.x * v.x + v.y * v.y;
+      }
+      nn = sqrt(cp_block_sum(nn, red));
+      if (nn > 1e-3 * n0) {
+        const double inv = 1.0 / nn;
+        for (int64_t r = tid; r < L; r += CP_THREADS) {
+          const double2 v = F.get(r, j);
+          F.put(r, j, v.x * inv, v.y * inv);
+        }
+        __syncthreads();
+        break;
+      }
+      unit = true;
+    }
+  }
+}
+
+// After a fixed number of captured sweeps: bit 0 of *status is set for a
+// problem whose sweeps did not converge (the graph's caller reads it).
+__global__ void svd_status_kernel(SvdBatch Bt, int32_t* status) {
+  const SvdProblem& P = Bt.arr ? Bt.arr[blockIdx.x] : Bt.one;
+  if (threadIdx.x == 0 && !P.flag[1]) atomicOr(status, 1);
+}
+
 #define TRY_HIP_COLNORM(b, nprob, max_ncp, s)                                                             \
   do {                                                                                                    \
     hipLaunchKernelGGL(svd_colnorm_kernel, dim3((unsigned)(((max_ncp) + SV_THREADS / 64 - 1) / (SV_THREADS / 64)), \
@@ -1287,6 +1450,17 @@ int launch_svd_finalize(const SvdBatch& b, int64_t nprob, int max_ncp, int64_t m
     hipLaunchKernelGGL(svd_write_kernel<double2>, dim3(gx, (unsigned)nprob), dim3(256), 0, s, b);
   else
     hipLaunchKernelGGL(svd_write_kernel<float2>, dim3(gx, (unsigned)nprob), dim3(256), 0, s, b);
+  TENET_HIP_CHECK(hipGetLastError());
+  if (dtype == TENET_C128)
+    hipLaunchKernelGGL(svd_complete_kernel<double2>, dim3(1, (unsigned)nprob), dim3(CP_THREADS), 0, s, b);
+  else
+    hipLaunchKernelGGL(svd_complete_kernel<float2>, dim3(1, (unsigned)nprob), dim3(CP_THREADS), 0, s, b);
+  TENET_HIP_CHECK(hipGetLastError());
+  return TENET_OK;
+}
+
+int launch_svd_status(const SvdBatch& b, int64_t nprob, int32_t* status, hipStream_t s) {
+  hipLaunchKernelGGL(svd_status_kernel, dim3((unsigned)nprob), dim3(64), 0, s, b, status);
   TENET_HIP_CHECK(hipGetLastError());
   return TENET_OK;
 }

@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <list>
 #include <map>
@@ -20,7 +21,21 @@
 
 namespace tenet {
 static thread_local std::string g_err;
-void set_error(const std::string& m) { g_err = m; }
+// library log (tenet_set_log): 0 off, 1 errors, 2 errors and resource events
+static int g_log_level = 0;
+static FILE* g_log_file = nullptr;
+static std::mutex g_log_mu;
+void log_line(int level, const std::string& m) {
+  if (level > g_log_level) return;
+  std::lock_guard<std::mutex> g(g_log_mu);
+  FILE* f = g_log_file ? g_log_file : stderr;
+  std::fprintf(f, "[tenet_hip] %s\n", m.c_str());
+  std::fflush(f);
+}
+void set_error(const std::string& m) {
+  g_err = m;
+  if (g_log_level >= 1) log_line(1, "error: " + m);
+}
 }  // namespace tenet
 
 using namespace tenet;
@@ -60,6 +75,7 @@ struct tenet_graph_s {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int device = 0;
+  hipStream_t stream = nullptr;  // the capture stream: replays go there when launched with NULL
 };
 
 // Workspace of tenet_svd_thin_noalloc: a file-scope device array, one copy
@@ -95,6 +111,14 @@ struct tenet_ctx_s {
   std::mutex mu;
   std::map<size_t, std::vector<void*>> free_blocks;
   std::map<uintptr_t, size_t> live;  // base -> class size
+  // blocks freed by tenet_free_async: reusable once their event completes
+  struct PendingFree {
+    hipEvent_t ev;
+    void* ptr;
+    size_t cls;
+  };
+  std::vector<PendingFree> pending;
+  std::vector<hipEvent_t> spare_events;
   size_t pool_bytes = 0;
   std::unordered_map<hipStream_t, Workspace> ws;
   std::unordered_map<hipStream_t, Workspace> ws_gemm;
@@ -103,6 +127,9 @@ struct tenet_ctx_s {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int32_t* host_flag = nullptr;  // pinned
   int64_t* host_k = nullptr;     // pinned
+  // device status word set by captured work (bit 0: a Jacobi SVD inside a
+  // graph ended unconverged); read and cleared by tenet_graph_status
+  int32_t* dev_status = nullptr;
   // per-kernel timing (tenet_stats_*)
   int stat_family = 0;
   std::vector<hipEvent_t> stat_events;  // pairs
@@ -228,6 +255,8 @@ static int ws_get(tenet_ctx ctx, hipStream_t s, size_t bytes, void** out, int wh
       return TENET_ERR_OOM;
     }
     const size_t nb = align_up(std::max(bytes, w.bytes + w.bytes / 2), 1 << 20);
+    if (g_log_level >= 2)
+      log_line(2, "workspace " + std::to_string(which) + " grows to " + std::to_string(nb) + " bytes");
     TENET_HIP_CHECK(hipStreamSynchronize(s));
     if (w.ptr) TENET_HIP_CHECK(hipFree(w.ptr));
     w.ptr = nullptr;
@@ -328,6 +357,8 @@ extern "C" int tenet_create(int device, tenet_ctx* out) {
   TENET_HIP_CHECK(hipEventCreate(&c->ev1));
   TENET_HIP_CHECK(hipHostMalloc((void**)&c->host_flag, 4096, hipHostMallocDefault));
   TENET_HIP_CHECK(hipHostMalloc((void**)&c->host_k, 64, hipHostMallocDefault));
+  TENET_HIP_CHECK(hipMalloc((void**)&c->dev_status, 256));
+  TENET_HIP_CHECK(hipMemset(c->dev_status, 0, 256));
   *out = c;
   return TENET_OK;
 }
@@ -343,12 +374,18 @@ extern "C" int tenet_destroy(tenet_ctx ctx) {
   for (auto& kv : ctx->free_blocks)
     for (void* p : kv.second) (void)hipFree(p);
   for (auto& kv : ctx->live) (void)hipFree((void*)kv.first);
+  for (auto& pf : ctx->pending) {
+    (void)hipFree(pf.ptr);
+    (void)hipEventDestroy(pf.ev);
+  }
+  for (hipEvent_t e : ctx->spare_events) (void)hipEventDestroy(e);
   for (auto& kv : ctx->ws)
     if (kv.second.ptr) (void)hipFree(kv.second.ptr);
   for (auto& kv : ctx->ws_gemm)
     if (kv.second.ptr) (void)hipFree(kv.second.ptr);
   (void)hipHostFree(ctx->host_flag);
   (void)hipHostFree(ctx->host_k);
+  if (ctx->dev_status) (void)hipFree(ctx->dev_status);
   (void)hipEventDestroy(ctx->ev0);
   (void)hipEventDestroy(ctx->ev1);
   for (hipEvent_t e : ctx->stat_events) (void)hipEventDestroy(e);
@@ -475,6 +512,19 @@ extern "C" int tenet_malloc(tenet_ctx ctx, size_t bytes, void** dptr) {
   CHECK_ARG(dptr, "null output");
   const size_t cls = size_class(std::max<size_t>(bytes, 1));
   std::lock_guard<std::mutex> g(ctx->mu);
+  // blocks whose stream-ordered free has completed become reusable
+  for (size_t i = 0; i < ctx->pending.size();) {
+    auto& pf = ctx->pending[i];
+    if (hipEventQuery(pf.ev) == hipSuccess) {
+      ctx->free_blocks[pf.cls].push_back(pf.ptr);
+      ctx->spare_events.push_back(pf.ev);
+      pf = ctx->pending.back();
+      ctx->pending.pop_back();
+    } else {
+      (void)hipGetLastError();  // hipErrorNotReady is not an error here
+      ++i;
+    }
+  }
   auto it = ctx->free_blocks.find(cls);
   if (it != ctx->free_blocks.end() && !it->second.empty()) {
     void* p = it->second.back();
@@ -520,6 +570,29 @@ extern "C" int tenet_free(tenet_ctx ctx, void* dptr) {
   return TENET_OK;
 }
 
+extern "C" int tenet_free_async(tenet_ctx ctx, void* dptr, tenet_stream stream) {
+  TRY(use_device(ctx));
+  if (!dptr) return TENET_OK;
+  hipStream_t s = pick(ctx, stream);
+  std::lock_guard<std::mutex> g(ctx->mu);
+  auto it = ctx->live.find((uintptr_t)dptr);
+  if (it == ctx->live.end()) {
+    set_error("tenet_free_async: pointer not from tenet_malloc");
+    return TENET_ERR_ARG;
+  }
+  hipEvent_t ev = nullptr;
+  if (!ctx->spare_events.empty()) {
+    ev = ctx->spare_events.back();
+    ctx->spare_events.pop_back();
+  } else {
+    TENET_HIP_CHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  }
+  TENET_HIP_CHECK(hipEventRecord(ev, s));
+  ctx->pending.push_back({ev, dptr, it->second});
+  ctx->live.erase(it);
+  return TENET_OK;
+}
+
 extern "C" int tenet_memcpy_h2d(tenet_ctx ctx, void* dst, const void* src, size_t bytes, tenet_stream s) {
   TRY(use_device(ctx));
   if (bytes == 0) return TENET_OK;
@@ -558,6 +631,25 @@ extern "C" int tenet_mem_info(tenet_ctx ctx, size_t* free_b, size_t* total_b, si
   if (free_b) *free_b = f;
   if (total_b) *total_b = t;
   if (pool_b) *pool_b = ctx->pool_bytes;
+  return TENET_OK;
+}
+
+extern "C" int tenet_set_log(int level, const char* path) {
+  CHECK_ARG(level >= 0 && level <= 2, "log level must be 0, 1 or 2");
+  std::lock_guard<std::mutex> g(g_log_mu);
+  if (g_log_file) {
+    std::fclose(g_log_file);
+    g_log_file = nullptr;
+  }
+  if (path && *path) {
+    g_log_file = std::fopen(path, "a");
+    if (!g_log_file) {
+      g_log_level = 0;
+      g_err = std::string("tenet_set_log: cannot open ") + path;
+      return TENET_ERR_ARG;
+    }
+  }
+  g_log_level = level;
   return TENET_OK;
 }
 
@@ -769,7 +861,7 @@ static int valid_dtype_prec(int dtype, int prec) {
     set_error("unknown dtype");
     return TENET_ERR_ARG;
   }
-  if (prec < TENET_PREC_NATIVE || prec > TENET_PREC_SPLIT_FP16) {
+  if (prec < TENET_PREC_NATIVE || prec > TENET_PREC_SPLIT_I8_FAST) {
     set_error("unknown precision");
     return TENET_ERR_ARG;
   }
@@ -1083,6 +1175,11 @@ static size_t svd_ws_bytes(const SvdShape& sh) {
   return t + 256;
 }
 
+// Sweeps a captured SVD runs (it cannot stop at convergence; rounds of a
+// converged matrix return at once): Vidal-gauge bond matrices at chi = 256
+// have needed up to ~20 of the eager limit of 40.
+constexpr int CAPTURED_SWEEPS = 30;
+
 static double svd_tol(int64_t nr) { return 1e-14 * std::max(1.0, std::sqrt((double)nr / 16.0)); }
 
 static int svd_run(tenet_ctx ctx, hipStream_t s, int dtype, int64_t m, int64_t n, const void* A, int64_t lda, void* U,
@@ -1126,7 +1223,7 @@ static int svd_run(tenet_ctx ctx, hipStream_t s, int dtype, int64_t m, int64_t n
   B.one = P;
   TRY(launch_svd_init(B, 1, sh.nr * sh.ncp + sh.ncp * sh.ncp, (int)sh.ncp, dtype, s));
   const bool sync_ok = !capturing(s);
-  const int max_sweeps = sync_ok ? 40 : 14;
+  const int max_sweeps = sync_ok ? 40 : CAPTURED_SWEEPS;
   bool converged = false;
   // one sweep = a full-inner round on the fixed pairs (2i, 2i+1), then the
   // circle rounds with cross-block inner steps (every column pair once)
@@ -1155,6 +1252,9 @@ static int svd_run(tenet_ctx ctx, hipStream_t s, int dtype, int64_t m, int64_t n
     }
   }
   TRY(launch_svd_finalize(B, 1, (int)sh.ncp, (int64_t)(m + n) * sh.ncols, dtype, s));
+  // under capture the sweep count is fixed (rounds of converged problems
+  // return at once); whether it sufficed is known only on the device
+  if (!sync_ok) TRY(launch_svd_status(B, 1, ctx->dev_status, s));
   if (sync_ok && !converged) {
     set_error("Jacobi SVD did not converge within the sweep limit");
     return TENET_ERR_NOCONV;
@@ -1308,6 +1408,41 @@ extern "C" int tenet_svd_thin_noalloc(tenet_ctx ctx, tenet_stream stream, int dt
                  discarded_dev, g_scratch_ptr[ctx->device], true);
 }
 
+// workspace of one tenet_simple_update with bond matrix m x n
+static size_t simple_update_ws_bytes(tenet_ctx ctx, int dtype, int64_t m, int64_t n) {
+  const size_t es = ELEM[dtype];
+  const int64_t kmax = std::min(m, n);
+  return 2 * align_up((size_t)m * n * es, 256) + align_up((size_t)m * kmax * es, 256) +
+         align_up((size_t)kmax * n * es, 256) + align_up((size_t)kmax * 8, 256) + 512 +
+         align_up(svd_any_ws_bytes(ctx, dtype, m, n), 256);
+}
+
+extern "C" int tenet_workspace_size(tenet_ctx ctx, int op, int dtype, int64_t m, int64_t n, size_t* bytes) {
+  TRY(use_device(ctx));
+  TRY(valid_dtype_prec(dtype, 0));
+  CHECK_ARG(bytes && m > 0 && n > 0, "bad argument");
+  switch (op) {
+    case TENET_OP_QR:
+      *bytes = qr_ws_bytes(m, n);
+      return TENET_OK;
+    case TENET_OP_SVD:
+      *bytes = svd_any_ws_bytes(ctx, dtype, m, n);
+      return TENET_OK;
+    case TENET_OP_SIMPLE_UPDATE:
+      *bytes = simple_update_ws_bytes(ctx, dtype, m, n);
+      return TENET_OK;
+    default:
+      set_error("tenet_workspace_size: unknown op");
+      return TENET_ERR_ARG;
+  }
+}
+
+extern "C" int tenet_workspace_reserve(tenet_ctx ctx, tenet_stream stream, size_t bytes) {
+  TRY(use_device(ctx));
+  void* w = nullptr;
+  return ws_get(ctx, pick(ctx, stream), bytes, &w);
+}
+
 // ---------------------------------------------------------------------------
 // fused gate update
 // ---------------------------------------------------------------------------
@@ -1331,8 +1466,7 @@ extern "C" int tenet_simple_update(tenet_ctx ctx, tenet_stream stream, int dtype
   if (S_out) TRY(check_span(ctx, S_out, (size_t)kmax * 8, "S_out"));
   const size_t svd_b = align_up(svd_any_ws_bytes(ctx, dtype, m, n), 256);
   const size_t theta_b = align_up((size_t)m * n * es, 256);
-  const size_t need = 2 * theta_b + align_up((size_t)m * kmax * es, 256) + align_up((size_t)kmax * n * es, 256) +
-                      align_up((size_t)kmax * 8, 256) + 512 + svd_b;
+  const size_t need = simple_update_ws_bytes(ctx, dtype, m, n);
   void* w = nullptr;
   TRY(ws_get(ctx, s, need, &w));
   Bump bm(w);
@@ -1443,6 +1577,38 @@ extern "C" int tenet_amplitudes(tenet_ctx ctx, tenet_stream stream, int dtype, i
   hipStream_t s = pick(ctx, stream);
   StatScope st(ctx, s, TENET_KSTAT_AMPLITUDE, 0.0, 0.0);
   return launch_amplitudes(nreq, n_dev, site_ptrs_dev, chis_dev, bits_dev, d, out, dtype, 2048, s);
+}
+
+extern "C" int tenet_amplitudes_sites(tenet_ctx ctx, tenet_stream stream, int dtype, int64_t nreq, int64_t n_max,
+                                      int64_t chi_max, const int64_t* n_dev, const uint64_t* site_ptrs_dev,
+                                      const uint64_t* chis_dev, const uint64_t* bits_dev, int64_t d, double* out) {
+  TRY(use_device(ctx));
+  TRY(valid_dtype_prec(dtype, 0));
+  CHECK_ARG(nreq >= 0 && d >= 1 && n_max >= 1 && chi_max >= 1, "bad argument");
+  if (nreq == 0) return TENET_OK;
+  CHECK_ARG(n_dev && site_ptrs_dev && chis_dev && bits_dev && out, "null pointer");
+  CHECK_ARG(nreq <= 65535, "more than 65535 requests in one call");
+  hipStream_t s = pick(ctx, stream);
+  void* w = nullptr;
+  const size_t vb = align_up((size_t)nreq * 4 * chi_max * sizeof(double), 256);
+  TRY(ws_get(ctx, s, vb + align_up((size_t)nreq * sizeof(double), 256), &w));
+  AmpSiteArgs a{};
+  a.n = n_dev;
+  a.sites = site_ptrs_dev;
+  a.chis = chis_dev;
+  a.bits = bits_dev;
+  a.d = d;
+  a.chi_max = chi_max;
+  a.n_max = n_max;
+  a.vbuf = static_cast<double*>(w);
+  a.logs = reinterpret_cast<double*>(static_cast<char*>(w) + vb);
+  a.out = out;
+  for (int64_t i = 0; i < n_max; ++i) {
+    StatScope st(ctx, s, TENET_KSTAT_AMPLITUDE, 0.0, 0.0, true);
+    TRY(launch_amp_site(a, nreq, i, dtype, s, st.start_event(), st.stop_event()));
+    st.launched();
+  }
+  return TENET_OK;
 }
 
 // Transfer-matrix sweep over sites [start, start + w): env <- conj(A) (env .
@@ -1576,7 +1742,7 @@ struct tenet_gplan_s {
   uint64_t* ptrs_dev = nullptr;
   std::vector<int64_t> spanA, spanB, spanC;
   double flops = 0.0, bytes = 0.0;
-  int64_t shape_count[3] = {0, 0, 0};
+  int64_t shape_count[4] = {0, 0, 0, 0};  // 64, 32, 16 tiles; GROUPED_SMALLK
   int64_t split_tiles = 0;
 };
 
@@ -1631,6 +1797,26 @@ static int grouped_plan_build(tenet_ctx ctx, int dtype, int prec, int64_t count,
     g->spanC[(size_t)i] = hp.spanC;
     g->flops += 8.0 * hp.M * hp.N * hp.K * hp.batch;
     g->bytes += (double)es * ((double)hp.M * hp.K + (double)hp.K * hp.N + (double)hp.M * hp.N) * hp.batch;
+    if (force_shape < 0 && hp.K <= 8 && std::min(hp.M, hp.N) < 16) {
+      // a tiny contracted range and a thin output: one thread per output
+      // element, the element order following C's unit-stride class
+      q.shape = GROUPED_SMALLK;
+      g->shape_count[GROUPED_SMALLK] += 1;
+      const int64_t total = hp.M * hp.N * hp.batch;
+      q.tiles_m = (total + SMALLK_ELEMS - 1) / SMALLK_ELEMS;
+      q.tiles_n = 1;
+      auto stride_of = [](const std::vector<int64_t>& v) -> int64_t {
+        return v.size() > 1 ? std::llabs(v[1] - v[0]) : INT64_MAX;
+      };
+      const int64_t sm = hp.M > 1 ? stride_of(hp.v[6]) : INT64_MAX;
+      const int64_t sn = hp.N > 1 ? stride_of(hp.v[7]) : INT64_MAX;
+      const int64_t sz = hp.batch > 1 ? stride_of(hp.v[8]) : INT64_MAX;
+      q.order = (sm <= sn && sm <= sz) ? 0 : (sn <= sz ? 1 : 2);
+      tile_step_cost[(size_t)i] = c[0];
+      if (total > 0) total_cost += c[0] * (double)q.tiles_m;
+      continue;
+    }
+    q.order = 0;
     // tile shape with the least modelled time for this problem
     const int64_t ksteps = std::max<int64_t>(1, (hp.K + 15) / 16);
     int best = 0;
@@ -1662,6 +1848,14 @@ static int grouped_plan_build(tenet_ctx ctx, int dtype, int prec, int64_t count,
   int64_t part_elems = 0, ctr_elems = 0, nunits = 0;
   for (int64_t i = 0; i < count; ++i) {
     GroupedProblem& q = probs[(size_t)i];
+    if (q.shape == GROUPED_SMALLK) {
+      q.kchunks = 1;
+      q.kc = q.K;
+      q.part_base = -1;
+      q.ctr_base = -1;
+      nunits += q.tiles_m;
+      continue;
+    }
     const int64_t ksteps = std::max<int64_t>(1, (q.K + 15) / 16);
     const double per_tile = tile_step_cost[(size_t)i] * ksteps;
     int64_t ch = 1;
@@ -1699,7 +1893,7 @@ static int grouped_plan_build(tenet_ctx ctx, int dtype, int prec, int64_t count,
                    [](const std::pair<double, int64_t>& x, const std::pair<double, int64_t>& y) { return x.first > y.first; });
   for (auto& o : order) {
     const GroupedProblem& q = probs[(size_t)o.second];
-    const int64_t tiles = q.tiles_m * q.tiles_n * q.batch;
+    const int64_t tiles = q.shape == GROUPED_SMALLK ? q.tiles_m : q.tiles_m * q.tiles_n * q.batch;
     for (int64_t t = 0; t < tiles; ++t)
       for (int32_t ch = 0; ch < q.kchunks; ++ch) units.push_back({(int32_t)o.second, ch, t});
   }
@@ -1765,13 +1959,13 @@ extern "C" int tenet_grouped_plan_conj(tenet_ctx ctx, int dtype, int prec, int64
   return TENET_OK;
 }
 
-extern "C" int tenet_grouped_plan_info(tenet_gplan plan, int64_t* units, int64_t* split_tiles, int64_t shapes[3],
+extern "C" int tenet_grouped_plan_info(tenet_gplan plan, int64_t* units, int64_t* split_tiles, int64_t shapes[4],
                                        double* flops, double* bytes) {
   CHECK_ARG(plan, "null plan");
   if (units) *units = plan->nunits;
   if (split_tiles) *split_tiles = plan->split_tiles;
   if (shapes)
-    for (int i = 0; i < 3; ++i) shapes[i] = plan->shape_count[i];
+    for (int i = 0; i < 4; ++i) shapes[i] = plan->shape_count[i];
   if (flops) *flops = plan->flops;
   if (bytes) *bytes = plan->bytes;
   return TENET_OK;
@@ -1900,21 +2094,44 @@ extern "C" int tenet_autotune_tiles(tenet_ctx ctx, int dtype, double costs_out[3
   return TENET_OK;
 }
 
-extern "C" int tenet_grouped_svd(tenet_ctx ctx, tenet_stream stream, int dtype, int64_t count, const int64_t* ms,
-                                 const int64_t* ns, const void* const* As, void* const* Us, double* const* Ss,
-                                 void* const* Vhs, const int64_t* maxdims, double threshold, int64_t* k_dev,
-                                 double* disc_dev) {
-  TRY(use_device(ctx));
-  TRY(valid_dtype_prec(dtype, 0));
-  CHECK_ARG(count >= 0, "bad count");
-  if (count == 0) return TENET_OK;
-  CHECK_ARG(ms && ns && As && Us && Vhs, "null argument");
-  hipStream_t s = pick(ctx, stream);
-  std::vector<SvdShape> shapes((size_t)count);
+// A grouped SVD bound to its operands: shapes, the problem table and every
+// problem's Jacobi workspace live in one pool block uploaded at build time,
+// so a run is kernel launches only (graph-capturable).
+struct tenet_gsvd_s {
+  tenet_ctx ctx = nullptr;
+  int dtype = 0;
+  int64_t count = 0;
+  std::vector<SvdShape> shapes;
+  std::vector<SvdProblem> probs;  // host copy of the device table
+  void* dev = nullptr;
+  SvdProblem* dprobs = nullptr;
+  int32_t* flags_base = nullptr;
+  int64_t max_rounds = 0, max_pairs = 0, max_ncp = 0, max_elems = 0, max_out = 0;
+  // operands, for the single-matrix fallback of problems the sweeps leave unconverged
+  std::vector<int64_t> ms, ns, maxdims;
+  std::vector<const void*> As;
+  std::vector<void*> Us, Vhs;
+  std::vector<double*> Ss;
+  double threshold = 0.0;
+  int64_t* k_dev = nullptr;
+  double* disc_dev = nullptr;
+};
+
+static int gsvd_build(tenet_ctx ctx, int dtype, int64_t count, const int64_t* ms, const int64_t* ns,
+                      const void* const* As, void* const* Us, double* const* Ss, void* const* Vhs,
+                      const int64_t* maxdims, double threshold, int64_t* k_dev, double* disc_dev, int s_mode,
+                      tenet_gsvd g, hipStream_t s) {
+  g->ctx = ctx;
+  g->dtype = dtype;
+  g->count = count;
+  g->shapes.resize((size_t)count);
+  g->threshold = threshold;
+  g->k_dev = k_dev;
+  g->disc_dev = disc_dev;
   size_t total = align_up((size_t)count * sizeof(SvdProblem), 256) + align_up((size_t)count * 32, 256) +
                  align_up((size_t)count * 8, 256);
-  int64_t max_rounds = 0, max_pairs = 0, max_ncp = 0, max_elems = 0, max_out = 0;
   const size_t es = ELEM[dtype];
+  const size_t s_es = s_mode == 0 ? 8 : (s_mode == 1 ? 16 : 8);
   for (int64_t i = 0; i < count; ++i) {
     CHECK_ARG(ms[i] > 0 && ns[i] > 0 && std::min(ms[i], ns[i]) <= 4096, "bad matrix shape");
     const int64_t m = ms[i], n = ns[i], k = std::min(m, n);
@@ -1922,27 +2139,32 @@ extern "C" int tenet_grouped_svd(tenet_ctx ctx, tenet_stream stream, int dtype, 
     TRY(check_span(ctx, As[i], (size_t)m * n * es, "A"));
     if (Us[i]) TRY(check_span(ctx, Us[i], (size_t)m * k * es, "U"));
     TRY(check_span(ctx, Vhs[i], (size_t)k * n * es, "Vh"));
-    if (Ss && Ss[i]) TRY(check_span(ctx, Ss[i], (size_t)k * 8, "S"));
+    if (Ss && Ss[i]) TRY(check_span(ctx, Ss[i], (size_t)k * s_es, "S"));
     // Without U and with m <= n, the Jacobi runs on X = A^H: V^H comes from X
     // itself and the rotations need not be accumulated
-    shapes[(size_t)i] = svd_shape(ms[i], ns[i], false, !Us[i] && m <= n);
-    const SvdShape& sh = shapes[(size_t)i];
+    g->shapes[(size_t)i] = svd_shape(ms[i], ns[i], false, !Us[i] && m <= n);
+    const SvdShape& sh = g->shapes[(size_t)i];
     total += svd_ws_bytes(sh);
-    max_rounds = std::max(max_rounds, sh.nblk - 1);
-    max_pairs = std::max(max_pairs, sh.npairs);
-    max_ncp = std::max(max_ncp, sh.ncp);
-    max_elems = std::max(max_elems, sh.nr * sh.ncp + sh.ncp * sh.ncp);
-    max_out = std::max(max_out, (ms[i] + ns[i]) * sh.ncols);
+    g->max_rounds = std::max(g->max_rounds, sh.nblk - 1);
+    g->max_pairs = std::max(g->max_pairs, sh.npairs);
+    g->max_ncp = std::max(g->max_ncp, sh.ncp);
+    g->max_elems = std::max(g->max_elems, sh.nr * sh.ncp + sh.ncp * sh.ncp);
+    g->max_out = std::max(g->max_out, (ms[i] + ns[i]) * sh.ncols);
+    g->ms.push_back(m);
+    g->ns.push_back(n);
+    g->maxdims.push_back(maxdims ? maxdims[i] : 0);
+    g->As.push_back(As[i]);
+    g->Us.push_back(Us[i]);
+    g->Vhs.push_back(Vhs[i]);
+    g->Ss.push_back(Ss ? Ss[i] : nullptr);
   }
-  void* w = nullptr;
-  TRY(ws_get(ctx, s, total, &w));
-  Bump bm(w);
-  SvdProblem* dprobs = (SvdProblem*)bm.take((size_t)count * sizeof(SvdProblem));
-  int32_t* flags_base = nullptr;
-  std::vector<SvdProblem> probs((size_t)count);
+  TRY(tenet_malloc(ctx, total, &g->dev));
+  Bump bm(g->dev);
+  g->dprobs = (SvdProblem*)bm.take((size_t)count * sizeof(SvdProblem));
+  g->probs.resize((size_t)count);
   for (int64_t i = 0; i < count; ++i) {
-    const SvdShape& sh = shapes[(size_t)i];
-    SvdProblem& P = probs[(size_t)i];
+    const SvdShape& sh = g->shapes[(size_t)i];
+    SvdProblem& P = g->probs[(size_t)i];
     P.X = (double*)bm.take((size_t)sh.nr * sh.ncp * 16);
     P.V = (double*)bm.take((size_t)sh.ncp * sh.ncp * 16);
     P.Gpart = (double*)bm.take((size_t)sh.npairs * sh.q * (2 * sh.b) * (2 * sh.b) * 16);
@@ -1958,6 +2180,7 @@ extern "C" int tenet_grouped_svd(tenet_ctx ctx, tenet_stream stream, int dtype, 
     P.n = (int32_t)ns[i];
     P.transposed = sh.transposed ? 1 : 0;
     P.accum = (Us[i] || !sh.transposed) ? 1 : 0;
+    P.s_mode = s_mode;
     P.A = As[i];
     P.lda = ms[i];
     P.U = Us[i];
@@ -1973,58 +2196,86 @@ extern "C" int tenet_grouped_svd(tenet_ctx ctx, tenet_stream stream, int dtype, 
   }
   // per problem 8 int32: rotated, converged, sweeps, spare, then the sweep's
   // largest squared relative off-diagonal (8 bytes at flag + 4)
-  flags_base = (int32_t*)bm.take((size_t)count * 32);
+  g->flags_base = (int32_t*)bm.take((size_t)count * 32);
   double* fro_base = (double*)bm.take((size_t)count * 8);
   for (int64_t i = 0; i < count; ++i) {
-    probs[(size_t)i].flag = flags_base + 8 * i;
-    probs[(size_t)i].fro2 = fro_base + i;
+    g->probs[(size_t)i].flag = g->flags_base + 8 * i;
+    g->probs[(size_t)i].fro2 = fro_base + i;
   }
-  TENET_HIP_CHECK(hipMemcpyAsync(dprobs, probs.data(), probs.size() * sizeof(SvdProblem), hipMemcpyHostToDevice, s));
+  // ordered on the stream after earlier work (the pool block may have been
+  // freed by a caller while that work still used it); g->probs outlives it
+  TENET_HIP_CHECK(hipMemcpyAsync(g->dprobs, g->probs.data(), g->probs.size() * sizeof(SvdProblem),
+                                 hipMemcpyHostToDevice, s));
+  return TENET_OK;
+}
+
+// One grouped run.  Eagerly: sweeps until every problem converges (flags read
+// back per sweep), then problems left unconverged go through the
+// single-matrix path.  Under capture: a fixed CAPTURED_SWEEPS sweeps (rounds
+// of converged problems return at once) and the context's device status bit
+// for problems they did not finish.
+static int gsvd_run(tenet_gsvd g, hipStream_t s) {
+  tenet_ctx ctx = g->ctx;
+  const int64_t count = g->count;
+  const bool sync_ok = !capturing(s);
   SvdBatch B{};
-  B.arr = dprobs;
-  TRY(launch_svd_init(B, count, max_elems, (int)max_ncp, dtype, s));
+  B.arr = g->dprobs;
+  TRY(launch_svd_init(B, count, g->max_elems, (int)g->max_ncp, g->dtype, s));
   std::vector<int32_t> hflags((size_t)count * 8, 0);
   bool all = false;
-  for (int sw = 0; sw < 40 && !all; ++sw) {
+  const int max_sweeps = sync_ok ? 40 : CAPTURED_SWEEPS;
+  for (int sw = 0; sw < max_sweeps && !all; ++sw) {
     // nominal work of one round over the problems still sweeping: the Gram of
-    // every pair over its rows and the rotation of its X and V rows
+    // every pair over its rows and the rotation of its X and V rows (a cross
+    // round r exists only for problems of more than one block pair with
+    // r < nblk - 1)
     double fl = 0.0, by = 0.0;
+    std::vector<double> flr((size_t)g->max_rounds, 0.0), byr((size_t)g->max_rounds, 0.0);
     for (int64_t i = 0; i < count; ++i) {
       if (hflags[(size_t)(8 * i + 1)]) continue;
-      const SvdShape& sh = shapes[(size_t)i];
-      const double n2 = 2.0 * sh.b, vr = probs[(size_t)i].accum ? (double)sh.ncp : 0.0;
-      fl += 8.0 * sh.npairs * n2 * n2 * (2.0 * sh.nr + vr);
-      by += 16.0 * sh.npairs * n2 * (3.0 * sh.nr + 2.0 * vr);
+      const SvdShape& sh = g->shapes[(size_t)i];
+      const double n2 = 2.0 * sh.b, vr = g->probs[(size_t)i].accum ? (double)sh.ncp : 0.0;
+      const double f = 8.0 * sh.npairs * n2 * n2 * (2.0 * sh.nr + vr);
+      const double b = 16.0 * sh.npairs * n2 * (3.0 * sh.nr + 2.0 * vr);
+      fl += f;
+      by += b;
+      if (sh.nblk > 2)
+        for (int64_t r = 0; r < sh.nblk - 1; ++r) {
+          flr[(size_t)r] += f;
+          byr[(size_t)r] += b;
+        }
     }
-    TRY(svd_round_timed(ctx, s, fl, by, B, count, (int)max_pairs, 0, 2, 0));
-    for (int r = 0; r < max_rounds; ++r) TRY(svd_round_timed(ctx, s, fl, by, B, count, (int)max_pairs, r, 2, 1));
+    TRY(svd_round_timed(ctx, s, fl, by, B, count, (int)g->max_pairs, 0, 2, 0));
+    for (int r = 0; r < g->max_rounds; ++r)
+      TRY(svd_round_timed(ctx, s, flr[(size_t)r], byr[(size_t)r], B, count, (int)g->max_pairs, r, 2, 1));
     TRY(launch_svd_sweep_end(B, count, s));
-    {
-      TENET_HIP_CHECK(hipMemcpyAsync(hflags.data(), flags_base, (size_t)count * 32, hipMemcpyDeviceToHost, s));
+    if (sync_ok) {
+      TENET_HIP_CHECK(hipMemcpyAsync(hflags.data(), g->flags_base, (size_t)count * 32, hipMemcpyDeviceToHost, s));
       TENET_HIP_CHECK(hipStreamSynchronize(s));
       all = true;
       for (int64_t i = 0; i < count; ++i) all = all && hflags[(size_t)(8 * i + 1)] != 0;
     }
   }
-  TRY(launch_svd_finalize(B, count, (int)max_ncp, max_out, dtype, s));
+  TRY(launch_svd_finalize(B, count, (int)g->max_ncp, g->max_out, g->dtype, s));
+  if (!sync_ok) return launch_svd_status(B, count, ctx->dev_status, s);
   TENET_HIP_CHECK(hipStreamSynchronize(s));
   if (!all) {
     // Problems the plain sweeps left unconverged (strongly graded spectra) go
     // one at a time through the single-matrix path, which preconditions
-    // complex128 by a QR, from their untouched inputs (the stream is idle, so
-    // its workspace may be reused).
+    // complex128 by a QR, from their untouched inputs.
     for (int64_t i = 0; i < count; ++i) {
       if (hflags[(size_t)(8 * i + 1)]) continue;
-      const int64_t m = ms[i], n = ns[i];
+      CHECK_ARG(g->probs[(size_t)i].s_mode == 0, "grouped SVD: unconverged problem with a complex S output");
+      const int64_t m = g->ms[(size_t)i], n = g->ns[(size_t)i];
       void* w = nullptr;
       // the single-matrix path always forms U: scratch for a problem without one
-      const size_t ub = Us[i] ? 0 : align_up((size_t)m * std::min(m, n) * ELEM[dtype], 256);
-      TRY(ws_get(ctx, s, svd_any_ws_bytes(ctx, dtype, m, n) + ub, &w));
-      void* u = Us[i] ? Us[i] : (void*)((char*)w + svd_any_ws_bytes(ctx, dtype, m, n));
-      // k_dev, disc_dev and Ss are optional, as on the grouped path above
-      const int st = svd_any(ctx, s, dtype, m, n, As[i], m, u, m, Ss ? Ss[i] : nullptr, Vhs[i], std::min(m, n),
-                             maxdims ? maxdims[i] : 0, threshold, k_dev ? k_dev + i : nullptr,
-                             disc_dev ? disc_dev + i : nullptr, w);
+      const size_t ub = g->Us[(size_t)i] ? 0 : align_up((size_t)m * std::min(m, n) * ELEM[g->dtype], 256);
+      TRY(ws_get(ctx, s, svd_any_ws_bytes(ctx, g->dtype, m, n) + ub, &w));
+      void* u = g->Us[(size_t)i] ? g->Us[(size_t)i] : (void*)((char*)w + svd_any_ws_bytes(ctx, g->dtype, m, n));
+      // k_dev, disc_dev and S are optional, as on the grouped path above
+      const int st = svd_any(ctx, s, g->dtype, m, n, g->As[(size_t)i], m, u, m, g->Ss[(size_t)i], g->Vhs[(size_t)i],
+                             std::min(m, n), g->maxdims[(size_t)i], g->threshold, g->k_dev ? g->k_dev + i : nullptr,
+                             g->disc_dev ? g->disc_dev + i : nullptr, w);
       if (st != TENET_OK) {
         if (st == TENET_ERR_NOCONV) set_error("grouped Jacobi SVD: some problems did not converge");
         return st;
@@ -2032,6 +2283,71 @@ extern "C" int tenet_grouped_svd(tenet_ctx ctx, tenet_stream stream, int dtype, 
     }
   }
   return TENET_OK;
+}
+
+extern "C" int tenet_grouped_svd_plan(tenet_ctx ctx, int dtype, int64_t count, const int64_t* ms, const int64_t* ns,
+                                      const void* const* As, void* const* Us, void* const* Ss, void* const* Vhs,
+                                      const int64_t* maxdims, double threshold, int64_t* k_dev, double* disc_dev,
+                                      int s_mode, tenet_gsvd* plan) {
+  TRY(use_device(ctx));
+  TRY(valid_dtype_prec(dtype, 0));
+  CHECK_ARG(plan && count > 0 && ms && ns && As && Us && Vhs, "bad argument");
+  CHECK_ARG(s_mode == TENET_S_REAL || s_mode == TENET_S_DTYPE, "bad S mode");
+  *plan = nullptr;
+  auto* g = new tenet_gsvd_s();
+  const int sm = s_mode == TENET_S_REAL ? 0 : (dtype == TENET_C128 ? 1 : 2);
+  const int st = gsvd_build(ctx, dtype, count, ms, ns, As, Us, reinterpret_cast<double* const*>(Ss), Vhs, maxdims,
+                            threshold, k_dev, disc_dev, sm, g, ctx->stream);
+  if (st == TENET_OK) TENET_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  if (st != TENET_OK) {
+    if (g->dev) (void)tenet_free(ctx, g->dev);
+    delete g;
+    return st;
+  }
+  *plan = g;
+  return TENET_OK;
+}
+
+extern "C" int tenet_grouped_svd_exec(tenet_gsvd plan, tenet_stream stream) {
+  CHECK_ARG(plan, "null plan");
+  TRY(use_device(plan->ctx));
+  return gsvd_run(plan, pick(plan->ctx, stream));
+}
+
+extern "C" int tenet_grouped_svd_destroy(tenet_gsvd plan) {
+  if (!plan) return TENET_OK;
+  if (plan->dev) {
+    (void)hipSetDevice(plan->ctx->device);
+    (void)hipDeviceSynchronize();
+    (void)tenet_free(plan->ctx, plan->dev);
+  }
+  delete plan;
+  return TENET_OK;
+}
+
+extern "C" int tenet_grouped_svd(tenet_ctx ctx, tenet_stream stream, int dtype, int64_t count, const int64_t* ms,
+                                 const int64_t* ns, const void* const* As, void* const* Us, double* const* Ss,
+                                 void* const* Vhs, const int64_t* maxdims, double threshold, int64_t* k_dev,
+                                 double* disc_dev) {
+  TRY(use_device(ctx));
+  TRY(valid_dtype_prec(dtype, 0));
+  CHECK_ARG(count >= 0, "bad count");
+  if (count == 0) return TENET_OK;
+  CHECK_ARG(ms && ns && As && Us && Vhs, "null argument");
+  hipStream_t s = pick(ctx, stream);
+  if (capturing(s)) {
+    set_error("tenet_grouped_svd cannot be captured (it allocates its plan): use tenet_grouped_svd_plan / exec");
+    return TENET_ERR_UNSUPPORTED;
+  }
+  tenet_gsvd_s g;
+  int st = gsvd_build(ctx, dtype, count, ms, ns, As, Us, Ss, Vhs, maxdims, threshold, k_dev, disc_dev, 0, &g, s);
+  if (st == TENET_OK) st = gsvd_run(&g, s);
+  if (g.dev) {
+    // the run ends with a stream synchronisation, except on an early error
+    (void)hipStreamSynchronize(s);
+    (void)tenet_free(ctx, g.dev);
+  }
+  return st;
 }
 
 // ---------------------------------------------------------------------------
@@ -2099,16 +2415,16 @@ extern "C" int tenet_nrm2sq(tenet_ctx ctx, tenet_stream stream, int dtype, int64
 // ---------------------------------------------------------------------------
 extern "C" int tenet_graph_begin(tenet_ctx ctx, tenet_stream stream) {
   TRY(use_device(ctx));
-  CHECK_ARG(stream, "capture needs an explicit stream");
-  TENET_HIP_CHECK(hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeRelaxed));
+  TENET_HIP_CHECK(hipStreamBeginCapture(pick(ctx, stream), hipStreamCaptureModeRelaxed));
   return TENET_OK;
 }
 extern "C" int tenet_graph_end(tenet_ctx ctx, tenet_stream stream, tenet_graph* graph) {
   TRY(use_device(ctx));
-  CHECK_ARG(stream && graph, "bad argument");
+  CHECK_ARG(graph, "bad argument");
   auto* g = new tenet_graph_s();
   g->device = ctx->device;
-  hipError_t e = hipStreamEndCapture((hipStream_t)stream, &g->graph);
+  g->stream = pick(ctx, stream);
+  hipError_t e = hipStreamEndCapture(g->stream, &g->graph);
   if (e != hipSuccess) {
     delete g;
     set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
@@ -2125,9 +2441,22 @@ extern "C" int tenet_graph_end(tenet_ctx ctx, tenet_stream stream, tenet_graph* 
   return TENET_OK;
 }
 extern "C" int tenet_graph_launch(tenet_graph graph, tenet_stream stream) {
-  CHECK_ARG(graph && stream, "bad argument");
+  CHECK_ARG(graph, "bad argument");
   TENET_HIP_CHECK(hipSetDevice(graph->device));
-  TENET_HIP_CHECK(hipGraphLaunch(graph->exec, (hipStream_t)stream));
+  TENET_HIP_CHECK(hipGraphLaunch(graph->exec, stream ? (hipStream_t)stream : graph->stream));
+  return TENET_OK;
+}
+extern "C" int tenet_graph_status(tenet_ctx ctx, int* status) {
+  TRY(use_device(ctx));
+  CHECK_ARG(status, "null status");
+  int32_t v = 0;
+  TENET_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  TENET_HIP_CHECK(hipMemcpy(&v, ctx->dev_status, sizeof v, hipMemcpyDeviceToHost));
+  if (v) {
+    const int32_t zero = 0;
+    TENET_HIP_CHECK(hipMemcpy(ctx->dev_status, &zero, sizeof zero, hipMemcpyHostToDevice));
+  }
+  *status = v;
   return TENET_OK;
 }
 extern "C" int tenet_graph_destroy(tenet_graph graph) {

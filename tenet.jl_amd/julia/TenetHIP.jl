@@ -47,14 +47,17 @@ dtype_code(::Type{ComplexF32}) = Cint(1)
 mutable struct HIPArray{T<:Union{ComplexF64,ComplexF32},N} <: DenseArray{T,N}
     ptr::Ptr{Cvoid}
     dims::NTuple{N,Int}
+    owner::Any  # the array whose allocation a view shares (nothing: owns its block)
     function HIPArray{T,N}(::UndefInitializer, dims::NTuple{N,Int}) where {T,N}
         p = Ref{Ptr{Cvoid}}(C_NULL)
         check(ccall((:tenet_malloc, libtenet), Cint, (Ptr{Cvoid}, Csize_t, Ref{Ptr{Cvoid}}),
                     context().h, max(prod(dims; init=1) * sizeof(T), 1), p))
-        a = new{T,N}(p[], dims)
+        a = new{T,N}(p[], dims, nothing)
         finalizer(x -> ccall((:tenet_free, libtenet), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), context().h, x.ptr), a)
         return a
     end
+    # a view of `owner`'s memory: no allocation, no finalizer; keeps owner alive
+    HIPArray{T,N}(ptr::Ptr{Cvoid}, dims::NTuple{N,Int}, owner) where {T,N} = new{T,N}(ptr, dims, owner)
 end
 Base.size(a::HIPArray) = a.dims
 Base.pointer(a::HIPArray) = a.ptr
@@ -83,8 +86,7 @@ struct TDesc
     label::NTuple{MAX_RANK,Int64}
 end
 pad(v) = ntuple(i -> i <= length(v) ? Int64(v[i]) : Int64(0), MAX_RANK)
-function tdesc(dims, labels)
-    strides = cumprod([1; collect(dims)])[1:end-1]
+function tdesc(dims, labels, strides=cumprod([1; collect(dims)])[1:end-1])
     TDesc(Int32(length(dims)), Int32(0), pad(dims), pad(strides), pad(labels))
 end
 
@@ -149,10 +151,29 @@ function Muscle.tensor_svd_thin(A::HIPTensor{T}; inds_u, inds_v=setdiff(inds(A),
                 context().h, C_NULL, dtype_code(T), m, n, M.ptr, m, U.ptr, m, S.ptr, V.ptr, kmax, maxdim, threshold,
                 kd.ptr, kd.ptr + 8))
     s = reinterpret(Float64, Array(S))[1:kmax]
+    # the kept count of compress.jl:46-59 (maxdim, then the absolute
+    # threshold), computed on the device
+    k = (maxdim > 0 || threshold > 0) ? Int(reinterpret(Int64, Array(kd))[1]) : kmax
     du = Tuple(size(A, i) for i in inds_u)
     dv = Tuple(size(A, i) for i in inds_v)
-    return Tensor(reshape_view(U, (du..., kmax)), [inds_u..., ind_s]), Tensor(s, [ind_s]),
-           Tensor(reshape_view(V, (kmax, dv...)), [ind_s, inds_v...])
+    # U's first k columns are a prefix of its column-major block: a view;
+    # V's first k rows are strided (leading dimension kmax): one gather
+    Vk = k == kmax ? V : leading_rows(V, k)
+    return Tensor(reshape_view(U, (du..., k)), [inds_u..., ind_s]), Tensor(s[1:k], [ind_s]),
+           Tensor(reshape_view(Vk, (k, dv...)), [ind_s, inds_v...])
+end
+
+# contiguous copy of the first k rows of a column-major m x n device matrix
+function leading_rows(M::HIPArray{T,2}, k::Int) where {T}
+    m, n = size(M)
+    out = HIPArray{T,2}(undef, (k, n))
+    one = HIPArray(ones(T))
+    check(ccall((:tenet_einsum, libtenet), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Cint, Ref{TDesc}, Ptr{Cvoid}, Ref{TDesc}, Ptr{Cvoid}, Ref{TDesc},
+                 Ptr{Cvoid}, Cint, Cint),
+                context().h, C_NULL, dtype_code(T), Cint(0), tdesc((k, n), (1, 2), (1, m)), M.ptr,
+                tdesc((), ()), one.ptr, tdesc((k, n), (1, 2)), out.ptr, 0, 0))
+    return out
 end
 
 # ---- one canonize move in one launch (generic_canonize_site!, canonize.jl:58-70) ----
@@ -213,12 +234,11 @@ function Muscle.simple_update(A::HIPTensor{T}, ind_a, B::HIPTensor{T}, ind_b, in
            Tensor(reshape_view(Bo, (kk, d2, dlb...)), [ind_bond, ind_b, lb...])
 end
 
-# a column-major view of the leading prod(dims) elements that keeps the parent alive
+# a column-major view of the leading prod(dims) elements of `a` (no copy;
+# the view keeps `a`, or the array `a` itself views, alive)
 function reshape_view(a::HIPArray{T}, dims::NTuple{N,Int}) where {T,N}
-    v = HIPArray{T,N}(undef, dims)
-    check(ccall((:tenet_memcpy_d2d, libtenet), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Csize_t, Ptr{Cvoid}),
-                context().h, v.ptr, a.ptr, prod(dims; init=1) * sizeof(T), C_NULL))
-    return v
+    prod(dims; init=1) <= prod(size(a); init=1) || throw(DimensionMismatch("view larger than its parent"))
+    return HIPArray{T,N}(a.ptr, dims, a.owner === nothing ? a : a.owner)
 end
 
 end # module

@@ -28,11 +28,13 @@ STATUS_NAMES = {
 }
 
 C128, C64 = 0, 1
-PREC_NATIVE, PREC_FP32, PREC_SPLIT_BF16, PREC_SPLIT_FP16 = 0, 1, 2, 3
+PREC_NATIVE, PREC_FP32, PREC_SPLIT_BF16, PREC_SPLIT_FP16, PREC_SPLIT_I8, PREC_SPLIT_I8_FAST = 0, 1, 2, 3, 4, 5
 ABSORB_NONE, ABSORB_EQUAL, ABSORB_LEFT, ABSORB_RIGHT = 0, 1, 2, 3
 QR_IN_ADJ, QR_Q_ADJ, QR_R_ADJ = 1, 2, 4
 ABSORB_R_B, ABSORB_B_RH = 0, 1
 KSTAT_OFF, KSTAT_SVD_ROUND, KSTAT_GEMM, KSTAT_QR, KSTAT_AMPLITUDE = 0, 1, 2, 3, 4
+OP_QR, OP_SVD, OP_SIMPLE_UPDATE = 1, 2, 3
+S_REAL, S_DTYPE = 0, 1
 MAX_RANK = 8
 
 
@@ -86,6 +88,10 @@ SIGNATURES = {
     "tenet_memcpy_d2d": [_p, _p, _p, _sz, _p],
     "tenet_memset_zero": [_p, _p, _sz, _p],
     "tenet_mem_info": [_p, C.POINTER(_sz), C.POINTER(_sz), C.POINTER(_sz)],
+    "tenet_free_async": [_p, _p, _p],
+    "tenet_workspace_size": [_p, _i, _i, _i64, _i64, C.POINTER(_sz)],
+    "tenet_workspace_reserve": [_p, _p, _sz],
+    "tenet_set_log": [_i, C.c_char_p],
     "tenet_host_alloc": [_p, _sz, C.POINTER(_p)],
     "tenet_host_free": [_p, _p],
     "tenet_memcpy_async": [_p, _p, _p, _sz, _i, _p],
@@ -108,6 +114,7 @@ SIGNATURES = {
                             _p, _p, _pi64],
     "tenet_overlap": [_p, _p, _i, _i, _i64, _i64, _pi64, C.POINTER(_p), _pi64, C.POINTER(_p), _p],
     "tenet_amplitudes": [_p, _p, _i, _i64, _p, _p, _p, _p, _i64, _p],
+    "tenet_amplitudes_sites": [_p, _p, _i, _i64, _i64, _i64, _p, _p, _p, _p, _i64, _p],
     "tenet_expect_pauli": [_p, _p, _i, _i, _i64, _i64, _i64, _pi64, C.POINTER(_p), _p, C.c_char_p, _p],
     "tenet_left_env": [_p, _p, _i, _i, _i64, _i64, _pi64, C.POINTER(_p), _p, _p],
     "tenet_left_envs": [_p, _p, _i, _i, _i64, _i64, _pi64, C.POINTER(_p), C.POINTER(_p), _p],
@@ -121,6 +128,10 @@ SIGNATURES = {
     "tenet_autotune_tiles": [_p, _i, _pd],
     "tenet_grouped_svd": [_p, _p, _i, _i64, _pi64, _pi64, C.POINTER(_p), C.POINTER(_p), C.POINTER(_p),
                           C.POINTER(_p), _pi64, _d, _p, _p],
+    "tenet_grouped_svd_plan": [_p, _i, _i64, _pi64, _pi64, C.POINTER(_p), C.POINTER(_p), C.POINTER(_p),
+                               C.POINTER(_p), _pi64, _d, _p, _p, _i, C.POINTER(_p)],
+    "tenet_grouped_svd_exec": [_p, _p],
+    "tenet_grouped_svd_destroy": [_p],
     "tenet_fill_gaussian": [_p, _p, _i, _i64, _p, C.c_uint64, C.c_uint64],
     "tenet_sample_select": [_p, _p, _i, _p, _i, _i, _i64, _i, _i, C.c_uint64, _p, _p],
     "tenet_scale": [_p, _p, _i, _i64, _p, _pd],
@@ -129,6 +140,7 @@ SIGNATURES = {
     "tenet_graph_begin": [_p, _p],
     "tenet_graph_end": [_p, _p, C.POINTER(_p)],
     "tenet_graph_launch": [_p, _p],
+    "tenet_graph_status": [_p, C.POINTER(C.c_int)],
     "tenet_graph_destroy": [_p],
     "tenet_stats_enable": [_p, _i],
     "tenet_stats_read": [_p, _pi64, _pd, _pd, _pd],

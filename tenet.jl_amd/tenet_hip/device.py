@@ -26,6 +26,9 @@ class Context:
         # plans of the Vidal-gauge layers, ...), keyed by the caller; closed
         # before the context
         self.plan_cache = {}
+        # while a graph is being captured: device blocks freed meanwhile
+        # (the graph's replays still write them), released with the graph
+        self._defer = None
 
     def close(self):
         if not self._closed:
@@ -53,7 +56,26 @@ class Context:
 
     def free(self, ptr: int):
         if ptr and not self._closed:
+            if self._defer is not None:
+                self._defer.append(ptr)
+                return
             L.check(L.lib.tenet_free(self.h, C.c_void_p(ptr)), "tenet_free")
+
+    def graph_status(self) -> int:
+        """Device status word of captured work (bit 0: a captured Jacobi SVD
+        ended unconverged), read and cleared (tenet_graph_status)."""
+        v = C.c_int()
+        L.check(L.lib.tenet_graph_status(self.h, C.byref(v)), "graph_status")
+        return v.value
+
+    def capture(self):
+        """``with ctx.capture() as g:`` records every library call issued on
+        the context's stream into a HIP graph (tenet_graph_begin / end);
+        ``g.launch()`` replays it.  Run the sequence once eagerly first, so
+        that workspaces and contraction plans exist (neither may grow during
+        a capture).  No host synchronisation or download may happen inside
+        the block."""
+        return Graph(self)
 
     def scalar_buffer(self) -> int:
         """A small device buffer (64 doubles) owned by the context, for
@@ -94,7 +116,63 @@ class Context:
 def _free(ctx_ref, ptr):
     ctx = ctx_ref()
     if ctx is not None and not ctx._closed:
+        if ctx._defer is not None:
+            ctx._defer.append(ptr)
+            return
         L.lib.tenet_free(ctx.h, C.c_void_p(ptr))
+
+
+class Graph:
+    """A HIP graph of the library calls made on the context's stream inside
+    ``with ctx.capture() as g:``.  Every device buffer the captured calls
+    touch must outlive the graph: blocks freed during the capture are held
+    until ``close()``, and objects passed to ``keep()`` (the inputs and
+    outputs the caller reads after a replay) are referenced by the graph."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        self.h = None
+        self._held = []
+        self._keep = []
+
+    def __enter__(self):
+        if self.ctx._defer is not None:
+            raise RuntimeError("a capture is already active on this context")
+        self.ctx._defer = self._held
+        try:
+            L.check(L.lib.tenet_graph_begin(self.ctx.h, None), "graph_begin")
+        except Exception:
+            self.ctx._defer = None
+            raise
+        return self
+
+    def __exit__(self, exc_type, exc, tb):
+        h = C.c_void_p()
+        st = L.lib.tenet_graph_end(self.ctx.h, None, C.byref(h))
+        self.ctx._defer = None
+        if exc_type is None:
+            L.check(st, "graph_end")
+            self.h = h
+        elif st == L.TENET_OK:
+            L.lib.tenet_graph_destroy(h)
+        return False
+
+    def keep(self, *objs):
+        self._keep.extend(objs)
+        return self
+
+    def launch(self):
+        """Replay on the capture stream (ordered after earlier calls on it)."""
+        L.check(L.lib.tenet_graph_launch(self.h, None), "graph_launch")
+
+    def close(self):
+        if self.h is not None:
+            L.lib.tenet_graph_destroy(self.h)
+            self.h = None
+        held, self._held = self._held, []
+        for p in held:
+            self.ctx.free(p)
+        self._keep = []
 
 
 class DeviceArray:

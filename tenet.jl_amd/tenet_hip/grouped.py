@@ -62,11 +62,11 @@ class GroupedPlan:
 
     def info(self):
         u, st = C.c_int64(), C.c_int64()
-        shapes = (C.c_int64 * 3)()
+        shapes = (C.c_int64 * 4)()
         fl, by = C.c_double(), C.c_double()
         L.check(L.lib.tenet_grouped_plan_info(self.h, C.byref(u), C.byref(st), shapes, C.byref(fl), C.byref(by)))
-        return {"units": u.value, "split_tiles": st.value, "tiles_64_32_16": list(shapes), "flops": fl.value,
-                "bytes": by.value}
+        return {"units": u.value, "split_tiles": st.value, "tiles_64_32_16": list(shapes)[:3], "smallk": shapes[3],
+                "flops": fl.value, "bytes": by.value}
 
     def exec(self, ptrs, stream=None):
         """ptrs: 3 * count device addresses (A, B, C per problem).  Returns

@@ -432,15 +432,9 @@ class MPS:
     def overlap(self, other: "MPS", prec=L.PREC_NATIVE, log=False):
         """overlap(a, b) = <b|a> (overlap.jl:36-50).  With log=True returns
         (mantissa, log_scale)."""
-        if self.n != other.n:
-            raise ValueError("site count mismatch")
-        d = self.sites[0].shape[1]
-        ca = (C.c_int64 * (self.n + 1))(*self.chis())
-        cb = (C.c_int64 * (other.n + 1))(*other.chis())
         out = self.ctx.malloc(64)
         try:
-            L.check(L.lib.tenet_overlap(self.ctx.h, None, self.code, prec, self.n, d, ca, self._site_ptrs(), cb,
-                                        other._site_ptrs(), C.c_void_p(out)), "overlap")
+            self._overlap_into(other, out, prec)
             h = np.zeros(3)
             L.check(L.lib.tenet_memcpy_d2h(self.ctx.h, h.ctypes.data_as(C.c_void_p), C.c_void_p(out), 24, None))
         finally:
@@ -449,6 +443,24 @@ class MPS:
         if log:
             return val, float(h[2])
         return val * math.exp(h[2])
+
+    def _overlap_into(self, other: "MPS", out: int, prec=L.PREC_NATIVE):
+        """<other|self> into device doubles out[0:3] = (re, im, log-scale),
+        with no download (graph-capturable)."""
+        if self.n != other.n:
+            raise ValueError("site count mismatch")
+        d = self.sites[0].shape[1]
+        ca = (C.c_int64 * (self.n + 1))(*self.chis())
+        cb = (C.c_int64 * (other.n + 1))(*other.chis())
+        L.check(L.lib.tenet_overlap(self.ctx.h, None, self.code, prec, self.n, d, ca, self._site_ptrs(), cb,
+                                    other._site_ptrs(), C.c_void_p(out)), "overlap")
+
+    def _norm2_dev(self):
+        """|A_c|^2 of the (MixedCanonical, single-site) centre tensor as a 0-d
+        device array, with no download (graph-capturable)."""
+        lo = self.form[1]
+        A = self.sites[lo]
+        return self._einsum(A, [1, 2, 3], A, [1, 2, 3], (), [], conj_b=1)
 
     def norm(self):
         """norm(::MPS) (norm.jl:17-37): canonize to the minimum centre, then
@@ -459,9 +471,7 @@ class MPS:
             self.form = ("mixed", 0, self.n - 1)
         lo = self.form[1]
         self.canonize_(lo, lo)
-        A = self.sites[lo]
-        v = self._einsum(A, [1, 2, 3], A, [1, 2, 3], (), [], conj_b=1)
-        return math.sqrt(abs(v.to_host().reshape(()).item()))
+        return math.sqrt(abs(self._norm2_dev().to_host().reshape(()).item()))
 
     def normalize_(self):
         """normalize!(::MPS) (normalize.jl:15-37)."""
@@ -475,9 +485,12 @@ class MPS:
         self.sites[lo] = self._einsum(A, [1, 2, 3], t, [], A.shape, [1, 2, 3])
         return self
 
-    def amplitudes(self, bitstrings):
-        """<x|psi> for each 0/1 bitstring (overlap.jl:22-32), one workgroup
-        per request (K4).  Returns complex128 values."""
+    def amplitudes(self, bitstrings, site_parallel=None):
+        """<x|psi> for each 0/1 bitstring (overlap.jl:22-32).  Returns
+        complex128 values.  ``site_parallel`` (default: when a bond reaches
+        64) takes tenet_amplitudes_sites (one launch per site, each slice
+        product spread over many workgroups) instead of one workgroup per
+        request."""
         bits = np.asarray(bitstrings, dtype=np.uint8).reshape(-1, self.n)
         nreq = bits.shape[0]
         chis = np.asarray(self.chis(), dtype=np.int64)
@@ -507,10 +520,19 @@ class MPS:
                                                None))
             L.check(L.lib.tenet_memcpy_h2d(ctx.h, C.c_void_p(p_tab), hostblob.ctypes.data_as(C.c_void_p),
                                            hostblob.nbytes, None))
-            L.check(L.lib.tenet_amplitudes(ctx.h, None, self.code, nreq, C.c_void_p(p_tab),
-                                           C.c_void_p(p_tab + 8 * nreq), C.c_void_p(p_tab + 16 * nreq),
-                                           C.c_void_p(p_tab + 24 * nreq), self.sites[0].shape[1], C.c_void_p(p_out)),
-                    "amplitudes")
+            chi_max = int(chis.max())
+            if site_parallel is None:
+                site_parallel = chi_max >= 64
+            if site_parallel:
+                L.check(L.lib.tenet_amplitudes_sites(ctx.h, None, self.code, nreq, self.n, chi_max, C.c_void_p(p_tab),
+                                                     C.c_void_p(p_tab + 8 * nreq), C.c_void_p(p_tab + 16 * nreq),
+                                                     C.c_void_p(p_tab + 24 * nreq), self.sites[0].shape[1],
+                                                     C.c_void_p(p_out)), "amplitudes_sites")
+            else:
+                L.check(L.lib.tenet_amplitudes(ctx.h, None, self.code, nreq, C.c_void_p(p_tab),
+                                               C.c_void_p(p_tab + 8 * nreq), C.c_void_p(p_tab + 16 * nreq),
+                                               C.c_void_p(p_tab + 24 * nreq), self.sites[0].shape[1],
+                                               C.c_void_p(p_out)), "amplitudes")
             h = np.zeros((nreq, 3))
             L.check(L.lib.tenet_memcpy_d2h(ctx.h, h.ctypes.data_as(C.c_void_p), C.c_void_p(p_out), h.nbytes, None))
         finally:

@@ -90,13 +90,15 @@ class ServedState:
 
 
 AMP_REQ_BYTES = 64  # device table entries + result per amplitude request
+SITE_PARALLEL_CHI = 64  # batches with a bond this large take tenet_amplitudes_sites
 
 
 def request_footprint(st: ServedState, r) -> int:
     """Device bytes a request holds while in flight (its tables and result,
     or the sweep's two environments and two site buffers)."""
     if r.kind == "amp":
-        return AMP_REQ_BYTES + st.n
+        # tables, bits, result, and the site-parallel path's two vector rows
+        return AMP_REQ_BYTES + st.n + 32 * int(np.max(st.chis))
     if r.kind == "sample":
         mx = int(np.max(st.chis))
         d = max(s.shape[1] for s in st.mps.sites)
@@ -273,9 +275,19 @@ class Server:
         s = self._stream()
         L.check(L.lib.tenet_memcpy_async(ctx.h, C.c_void_p(dev), C.c_void_p(pin.ptr), tab_bytes + nb, 0, s))
         code = self.states[reqs[0].state].code
-        L.check(L.lib.tenet_amplitudes(ctx.h, s, code, nreq, C.c_void_p(dev), C.c_void_p(dev + 8 * nreq),
-                                       C.c_void_p(dev + 16 * nreq), C.c_void_p(dev + 24 * nreq), 2,
-                                       C.c_void_p(p_out)), "amplitudes")
+        n_max = max(self.states[r.state].n for r in reqs)
+        chi_max = max(int(np.max(self.states[r.state].chis)) for r in reqs)
+        if chi_max >= SITE_PARALLEL_CHI:
+            # one launch per site position, every request's slice product
+            # spread over ceil(chi / 16) workgroups
+            L.check(L.lib.tenet_amplitudes_sites(ctx.h, s, code, nreq, n_max, chi_max, C.c_void_p(dev),
+                                                 C.c_void_p(dev + 8 * nreq), C.c_void_p(dev + 16 * nreq),
+                                                 C.c_void_p(dev + 24 * nreq), 2, C.c_void_p(p_out)),
+                    "amplitudes_sites")
+        else:
+            L.check(L.lib.tenet_amplitudes(ctx.h, s, code, nreq, C.c_void_p(dev), C.c_void_p(dev + 8 * nreq),
+                                           C.c_void_p(dev + 16 * nreq), C.c_void_p(dev + 24 * nreq), 2,
+                                           C.c_void_p(p_out)), "amplitudes")
         # results land after the request tables in the same pinned buffer
         L.check(L.lib.tenet_memcpy_async(ctx.h, C.c_void_p(pin.ptr + tab_bytes + nb), C.c_void_p(p_out),
                                          24 * nreq, 1, s))

@@ -56,6 +56,9 @@ class VidalMPS:
         self._plans = {}
         self._one = DeviceArray.from_host(ctx, np.ones((), dtype=self.dtype), self.dtype)
         self.flops = {"svd": 0.0, "contract": 0.0}
+        self._graphs = {}  # (parity, layer signature) -> _LayerGraph
+        self._lam_bufs = None  # fixed per-bond Lambda buffers once layers are captured
+        self._lam_stale = False  # lam_host behind the device Lambdas (captured layers ran)
 
     # ---- construction -------------------------------------------------
     def _upload_lams(self, lams):
@@ -95,6 +98,9 @@ class VidalMPS:
         v._plans = {}
         v._one = DeviceArray.from_host(ctx, np.ones((), dtype=dt), dt)
         v.flops = {"svd": 0.0, "contract": 0.0}
+        v._graphs = {}
+        v._lam_bufs = None
+        v._lam_stale = False
         B, lam = [], []
         bt = p.sites[0]
         prev = None
@@ -148,6 +154,9 @@ class VidalMPS:
             return []
         if len(set(bonds)) != P or any(abs(a - b) < 2 for a in bonds for b in bonds if a != b):
             raise ValueError("a layer's bonds must be disjoint")
+        # an eager layer replaces sites and Lambdas: captured layers bound to
+        # the old buffers are dropped
+        self._drop_graphs()
         md = 0 if maxdim is None else int(maxdim)
         thr = 0.0 if threshold is None else float(threshold)
         Gs = []
@@ -189,27 +198,25 @@ class VidalMPS:
         # U is never used (B_a is formed from theta~ and V): no U pointers, so
         # the SVD of an m <= n problem runs on X^H and accumulates no rotations
         Vs = [DeviceArray(ctx, (int(k), int(n)), dt) for k, n in zip(kmax, ns)]
-        Ss = [ctx.malloc(int(k) * 8) for k in kmax]
-        kd = ctx.malloc(P * 8)
+        # the kept counts and every S in one buffer: one download each
+        s_off = np.concatenate([[0], np.cumsum(kmax)]).astype(np.int64)
+        kd = ctx.malloc(P * 8 + int(s_off[-1]) * 8)
+        sbase = kd + P * 8
         mds = np.full(P, md, dtype=np.int64)
         Pp = C.c_void_p
         try:
             L.check(L.lib.tenet_grouped_svd(
                 ctx.h, None, self.code, P, ms.ctypes.data_as(C.POINTER(C.c_int64)),
                 ns.ctypes.data_as(C.POINTER(C.c_int64)), (Pp * P)(*[x.ptr for x in X]),
-                (Pp * P)(*([None] * P)), (Pp * P)(*Ss), (Pp * P)(*[v.ptr for v in Vs]),
-                mds.ctypes.data_as(C.POINTER(C.c_int64)), thr, Pp(kd), None), "grouped_svd")
-            ks = np.zeros(P, dtype=np.int64)
-            L.check(L.lib.tenet_memcpy_d2h(ctx.h, ks.ctypes.data_as(Pp), Pp(kd), ks.nbytes, None))
-            svals = []
-            for j in range(P):
-                s = np.zeros(int(ks[j]))
-                if len(s):
-                    L.check(L.lib.tenet_memcpy_d2h(ctx.h, s.ctypes.data_as(Pp), Pp(Ss[j]), s.nbytes, None))
-                svals.append(s)
+                (Pp * P)(*([None] * P)), (Pp * P)(*[sbase + 8 * int(o) for o in s_off[:-1]]),
+                (Pp * P)(*[v.ptr for v in Vs]), mds.ctypes.data_as(C.POINTER(C.c_int64)), thr, Pp(kd), None),
+                "grouped_svd")
+            host = np.zeros(P + int(s_off[-1]), dtype=np.float64)
+            L.check(L.lib.tenet_memcpy_d2h(ctx.h, host.ctypes.data_as(Pp), Pp(kd), host.nbytes, None))
+            ks = host[:P].view(np.int64).copy()
+            sv = host[P:]
+            svals = [sv[int(s_off[j]):int(s_off[j]) + int(ks[j])].copy() for j in range(P)]
         finally:
-            for p in Ss:
-                ctx.free(p)
             ctx.free(kd)
         for (l, d1, b, d2, r), m, n in zip(shp, ms, ns):
             self.flops["svd"] += svd_flops(int(m), int(n))
@@ -240,8 +247,82 @@ class VidalMPS:
         del Vs, th, tt, X
         return [int(k) for k in ks]
 
+    # ---- graph-captured layers -------------------------------------------------
+    def layer_signature(self, bonds):
+        """Shapes a layer on ``bonds`` reads: (l, d1, b, d2, r) per gate."""
+        return tuple((self.B[a].shape[0], self.B[a].shape[1], self.B[a].shape[2], self.B[a + 1].shape[1],
+                      self.B[a + 1].shape[2]) for a in bonds)
+
+    def _stabilise_lams(self):
+        """Give every Lambda a fixed device buffer (capacity d * max bond),
+        the current values at its start; captured layers write S there."""
+        if self._lam_bufs is not None:
+            return
+        ctx, dt = self.ctx, self.dtype
+        cap = max(2, max(b.shape[1] for b in self.B) * max(self.bondsizes() + [1]))
+        bufs = []
+        for a, lam in enumerate(self.lam):
+            k = lam.shape[0]
+            buf = DeviceArray(ctx, (max(cap, k),), dt)
+            L.check(L.lib.tenet_memcpy_d2d(ctx.h, C.c_void_p(buf.ptr), C.c_void_p(lam.ptr), k * dt.itemsize, None),
+                    "d2d")
+            self.lam[a] = buf.view((k,))
+            bufs.append(buf)
+        self._lam_bufs = bufs
+
+    def _drop_graphs(self):
+        for g in self._graphs.values():
+            g.close()
+        self._graphs = {}
+        if self._lam_stale:
+            self.refresh_lams()
+        self._lam_bufs = None
+
+    def refresh_lams(self):
+        """Download the device Lambdas into lam_host (after captured layers)."""
+        for a, lam in enumerate(self.lam):
+            self.lam_host[a] = lam.to_host().real.astype(np.float64).copy()
+        self._lam_stale = False
+
+    def apply_layer_graph_(self, gates, bonds, maxdim=None, threshold=None):
+        """The same layer as apply_layer_, replayed from a HIP graph captured
+        for the layer's shapes (once per parity and shape signature).  Valid
+        in the steady state, where every gate keeps the bond dimension it
+        found (the caller establishes that with an eager layer of the same
+        signature); the graph writes sites, Lambda and the SVD's S in place.
+        After the replay the kept counts are read back: a gate that kept
+        fewer (its threshold cut deeper) has its sites and Lambda cut to k,
+        as the eager layer would, and the graphs are dropped.  Returns the
+        kept counts."""
+        sig = self.layer_signature(bonds)
+        key = (tuple(bonds), sig, maxdim, threshold)
+        g = self._graphs.get(key)
+        if g is None:
+            self._stabilise_lams()
+            g = _LayerGraph(self, list(bonds), sig, maxdim, threshold)
+            self._graphs[key] = g
+        ks = g.replay(gates)
+        self._lam_stale = True
+        for (l, d1, b, d2, r), m, n in zip(sig, g.ms, g.ns):
+            self.flops["svd"] += svd_flops(int(m), int(n))
+            self.flops["contract"] += 8.0 * l * d1 * b * d2 * r + 8.0 * l * d1 * d2 * r * d1 * d2 + \
+                8.0 * l * d1 * d2 * r * b
+        short = [j for j, (k, sh) in enumerate(zip(ks, sig)) if k != sh[2]]
+        if short:
+            for j in short:
+                a, k = bonds[j], int(ks[j])
+                l, d1, b, d2, r = sig[j]
+                if k > b:
+                    raise RuntimeError("captured Vidal layer: a gate kept more singular values than its bond holds")
+                self.B[a] = self.B[a].view((l, d1, k)).copy()
+                self.B[a + 1] = _leading_rows(self, self.B[a + 1].view((b, d2 * r)), k).view((k, d2, r))
+                self.lam[a] = self.lam[a].view((k,))
+            self._drop_graphs()
+        return [int(k) for k in ks]
+
     def close(self):
         """Plans live on the context (Context.close releases them)."""
+        self._drop_graphs()
         self._plans = {}
 
 
@@ -255,18 +336,159 @@ def _leading_rows(v: VidalMPS, M: DeviceArray, k: int) -> DeviceArray:
     return out
 
 
+class _LayerGraph:
+    """One brickwork layer at fixed shapes as a HIP graph: the contraction
+    launches of apply_layer_ (same grouped plans, on device pointer tables),
+    the grouped SVD bound by tenet_grouped_svd_plan writing S as complex
+    values straight into each bond's Lambda buffer, and the site updates
+    written in place (the steady state keeps every shape).  The layer's
+    gates are copied into a staging buffer before each replay."""
+
+    def __init__(self, v: VidalMPS, bonds, sig, maxdim, threshold):
+        ctx, dt = v.ctx, v.dtype
+        self.ctx, self.v, self.bonds = ctx, v, bonds
+        P = len(bonds)
+        self.P = P
+        md = 0 if maxdim is None else int(maxdim)
+        thr = 0.0 if threshold is None else float(threshold)
+        es = dt.itemsize
+        d0 = sig[0][1]
+        self.gsize = d0 ** 4
+        self.stage = DeviceArray(ctx, (P * self.gsize,), dt)
+        shp = list(sig)
+        th = [DeviceArray(ctx, (l, d1, d2, r), dt) for (l, d1, b, d2, r) in shp]
+        tt = [DeviceArray(ctx, (l, d1, d2, r), dt) for (l, d1, b, d2, r) in shp]
+        self.ms = np.array([l * d1 for (l, d1, b, d2, r) in shp], dtype=np.int64)
+        self.ns = np.array([d2 * r for (l, d1, b, d2, r) in shp], dtype=np.int64)
+        kmax = np.minimum(self.ms, self.ns)
+        Vs = [DeviceArray(ctx, (int(k), int(n)), dt) for k, n in zip(kmax, self.ns)]
+        for j, a in enumerate(bonds):
+            if v._lam_bufs[a].size < kmax[j]:
+                raise RuntimeError("Lambda buffer below the layer's kmax")
+        self._keep = [th, tt, Vs]
+        # 1. theta = B_a B_{a+1}; 2. theta~ = theta . G; 3. X = Lambda_{a-1} (.) theta~ (theta's memory)
+        p1 = v._plan(("theta", tuple(shp)), [((l, d1, b), "lxb", (b, d2, r), "byr", (l, d1, d2, r), "lxyr")
+                                             for (l, d1, b, d2, r) in shp])
+        t1 = [x for j, a in enumerate(bonds) for x in (v.B[a].ptr, v.B[a + 1].ptr, th[j].ptr)]
+        p2 = v._plan(("gate", tuple(shp)), [((l, d1, d2, r), "lxyr", (d1, d2, d1, d2), "pqxy", (l, d1, d2, r),
+                                             "lpqr") for (l, d1, b, d2, r) in shp])
+        t2 = [x for j in range(P) for x in (th[j].ptr, self.stage.ptr + j * self.gsize * es, tt[j].ptr)]
+        idx = [j for j, a in enumerate(bonds) if a > 0]
+        X = list(tt)
+        p3 = t3 = None
+        if idx:
+            for j in idx:
+                X[j] = th[j]
+            p3 = v._plan(("lam", tuple(shp[j] for j in idx)),
+                         [((shp[j][0],), "l", shp[j][:2] + shp[j][3:], "lpqr", shp[j][:2] + shp[j][3:], "lpqr")
+                          for j in idx])
+            t3 = [x for j in idx for x in (v.lam[bonds[j] - 1].ptr, tt[j].ptr, X[j].ptr)]
+        # 4. the grouped SVD, bound once: S as complex into the Lambda buffers
+        self.kd = ctx.malloc(P * 8)
+        Pp = C.c_void_p
+        mds = np.full(P, md, dtype=np.int64)
+        h = Pp()
+        L.check(L.lib.tenet_grouped_svd_plan(
+            ctx.h, v.code, P, self.ms.ctypes.data_as(C.POINTER(C.c_int64)),
+            self.ns.ctypes.data_as(C.POINTER(C.c_int64)), (Pp * P)(*[x.ptr for x in X]), (Pp * P)(*([None] * P)),
+            (Pp * P)(*[v._lam_bufs[a].ptr for a in bonds]), (Pp * P)(*[x.ptr for x in Vs]),
+            mds.ctypes.data_as(C.POINTER(C.c_int64)), thr, Pp(self.kd), None, L.S_DTYPE, C.byref(h)),
+            "grouped_svd_plan")
+        self.svd = h
+        # 5. B_a <- theta~ conj(V^H[:k])^T and B_{a+1} <- V^H[:k], in place (k = b)
+        probs, conj, t5 = [], [], []
+        for j, ((l, d1, b, d2, r), km) in enumerate(zip(shp, kmax)):
+            k, km = int(b), int(km)
+            vst = (1, km, km * d2)
+            probs.append(((l, d1, d2, r), "lpqr", (k, d2, r), "kqr", (l, d1, k), "lpk", None, vst))
+            conj.append((0, 1))
+            t5 += [tt[j].ptr, Vs[j].ptr, v.B[bonds[j]].ptr]
+            probs.append(((k, d2, r), "kqr", (), "", (k, d2, r), "kqr", vst, None))
+            conj.append((0, 0))
+            t5 += [Vs[j].ptr, v._one.ptr, v.B[bonds[j] + 1].ptr]
+        p5 = v._plan(("out", tuple(shp), tuple(int(x[2]) for x in shp)), probs, conj)
+        self.tables = []
+        for t in (t1, t2, t3, t5):
+            if t is None:
+                self.tables.append(None)
+                continue
+            arr = np.asarray(t, dtype=np.uint64)
+            dev = ctx.malloc(arr.nbytes)
+            L.check(L.lib.tenet_memcpy_h2d(ctx.h, C.c_void_p(dev), arr.ctypes.data_as(C.c_void_p), arr.nbytes,
+                                           None))
+            self.tables.append(dev)
+        with ctx.capture() as g:
+            p1.exec_dev(self.tables[0])
+            p2.exec_dev(self.tables[1])
+            if p3 is not None:
+                p3.exec_dev(self.tables[2])
+            L.check(L.lib.tenet_grouped_svd_exec(self.svd, None), "grouped_svd_exec")
+            p5.exec_dev(self.tables[3])
+        self.graph = g
+
+    def replay(self, gates):
+        """Stage the layer's gates (one copy when they are consecutive in one
+        buffer, as upload_gates lays them out), replay, and return the kept
+        counts (synchronises)."""
+        ctx, es = self.ctx, self.v.dtype.itemsize
+        nb = self.gsize * es
+        if all(g.ptr == gates[0].ptr + j * nb for j, g in enumerate(gates)):
+            L.check(L.lib.tenet_memcpy_d2d(ctx.h, C.c_void_p(self.stage.ptr), C.c_void_p(gates[0].ptr),
+                                           self.P * nb, None), "d2d")
+        else:
+            for j, g in enumerate(gates):
+                L.check(L.lib.tenet_memcpy_d2d(ctx.h, C.c_void_p(self.stage.ptr + j * nb), C.c_void_p(g.ptr), nb,
+                                               None), "d2d")
+        self.graph.launch()
+        ks = np.zeros(self.P, dtype=np.int64)
+        L.check(L.lib.tenet_memcpy_d2h(ctx.h, ks.ctypes.data_as(C.c_void_p), C.c_void_p(self.kd), ks.nbytes, None))
+        if ctx.graph_status() & 1:
+            raise L.TenetError(L.TENET_ERR_NOCONV, "captured Vidal layer: a Jacobi SVD did not converge")
+        return ks
+
+    def close(self):
+        if self.graph is not None:
+            self.graph.close()
+            self.graph = None
+        if self.svd is not None:
+            L.lib.tenet_grouped_svd_destroy(self.svd)
+            self.svd = None
+        for t in self.tables:
+            if t:
+                self.ctx.free(t)
+        self.tables = []
+        if self.kd:
+            self.ctx.free(self.kd)
+            self.kd = 0
+        self._keep = []
+
+
 def brickwork_layers(n, depth):
     """The brickwork schedule as layers of disjoint bonds (0-based left sites):
     odd layers (1,2),(3,4),...; even layers (2,3),(4,5),..."""
     return [list(range(0, n - 1, 2)) if layer % 2 == 0 else list(range(1, n - 1, 2)) for layer in range(depth)]
 
 
-def tebd_vidal_sweep(ctx: Context, n, layers, gate_views, maxdim, threshold, psi=None, dtype=np.complex128):
+def tebd_vidal_sweep(ctx: Context, n, layers, gate_views, maxdim, threshold, psi=None, dtype=np.complex128,
+                     graphs=False):
     """Vidal-gauge TEBD from |0...0> (or ``psi``): gate_views consumed in
-    order, one per bond entry of ``layers``."""
+    order, one per bond entry of ``layers``.  With ``graphs``, a layer whose
+    shape signature an earlier eager layer left unchanged (the steady state,
+    once the bonds have saturated) is replayed from a captured graph
+    (VidalMPS.apply_layer_graph_); the results equal the eager sweep's."""
     v = psi if psi is not None else VidalMPS.product_state(ctx, [[1.0, 0.0]] * n, dtype)
+    steady = set()
     g = 0
     for bonds in layers:
-        v.apply_layer_(gate_views[g:g + len(bonds)], bonds, maxdim, threshold)
+        gates = gate_views[g:g + len(bonds)]
+        sig = (tuple(bonds), v.layer_signature(bonds))
+        if graphs and sig in steady:
+            v.apply_layer_graph_(gates, bonds, maxdim, threshold)
+        else:
+            ks = v.apply_layer_(gates, bonds, maxdim, threshold)
+            if all(k == shp[2] for k, shp in zip(ks, sig[1])):
+                steady.add(sig)
         g += len(bonds)
+    if v._lam_stale:
+        v.refresh_lams()
     return v

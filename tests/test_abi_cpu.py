@@ -75,3 +75,20 @@ def test_brickwork_schedule_counts():
     assert len(s) == 630
     assert s == list(O.brickwork_schedule(64, 20))
     assert s[:3] == [0, 2, 4] and s[32:35] == [61, 59, 57]
+
+
+def test_set_log_writes_errors_to_the_file(tmp_path):
+    """tenet_set_log(1, path): every error message set afterwards is appended
+    to the file; level 0 stops it; a bad level is refused."""
+    from tenet_hip import _lib
+
+    path = tmp_path / "tenet.log"
+    assert _lib.lib.tenet_set_log(3, None) == _lib.TENET_ERR_ARG
+    assert _lib.lib.tenet_set_log(1, str(path).encode()) == 0
+    try:
+        assert _lib.lib.tenet_malloc(None, 16, None) == _lib.TENET_ERR_ARG
+    finally:
+        assert _lib.lib.tenet_set_log(0, None) == 0
+    assert _lib.lib.tenet_malloc(None, 16, None) == _lib.TENET_ERR_ARG
+    lines = path.read_text().splitlines()
+    assert len(lines) == 1 and "error: null context" in lines[0]

@@ -188,7 +188,8 @@ def test_d3_grouped_batch_full_size(ctx):
 
 # ---------------------------------------------------------------- D4
 @pytest.mark.parametrize("form,prec,tol", [("fp64", 0, 1e-12), ("fp32", 1, 1e-5), ("split_bf16", 2, 1e-10),
-                                           ("split_fp16", 3, 1e-10)])
+                                           ("split_fp16", 3, 1e-10), ("split_i8", 4, 1e-12),
+                                           ("split_i8_fast", 5, 5e-9)])
 def test_d4_heff_chi2048_vs_oracle_rows(ctx, heff2048, form, prec, tol):
     from tenet_hip.dmrg import heff_apply
 

@@ -55,7 +55,7 @@ def test_grouped_random_problems(ctx, dtype, tol):
     specs = grouped_contractions(11, 300, cap=1 << 12)
     plan, hosts, outs, ptrs = _run_grouped(ctx, specs, dtype, 5)
     info = plan.info()
-    assert sum(info["tiles_64_32_16"]) == len(specs)
+    assert sum(info["tiles_64_32_16"]) + info["smallk"] == len(specs)
     plan.exec(ptrs)
     _check(specs, hosts, outs, tol)
     plan.close()
@@ -178,3 +178,59 @@ def test_grouped_svd_graded_problems_converge(ctx, want):
         if want_s:
             assert np.allclose(S[:kref], s_ref[:kref], rtol=1e-12, atol=1e-13 * s_ref[0])
             assert _rel((U.to_host()[:, :kref] * S[:kref]) @ Vk, best) < 1e-11
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.complex64, 2e-6), (np.complex128, 1e-14)])
+def test_grouped_small_k_elementwise_units(ctx, dtype, tol):
+    """Contractions with K <= 8 and a thin output run as element-wise units
+    (no MFMA tile): the Vidal layer's Lambda scaling (dims = [] with the
+    label kept, a batch of 1 x N products), its gate application (K = 4
+    over the physical pair), a strided copy through a scalar, and a
+    conjugated K = 2 product; each against the oracle's binary_einsum."""
+    import tenet_hip as T
+    from tenet_hip.grouped import GroupedPlan
+
+    rng = np.random.default_rng(21)
+    l, r = 40, 36
+    cases = [
+        # (shape_a, labels_a, shape_b, labels_b, shape_c, labels_c, strides_a, conj)
+        ((l,), "l", (l, 2, 2, r), "lpqr", (l, 2, 2, r), "lpqr", None, (0, 0)),
+        ((l, 2, 2, r), "lxyr", (2, 2, 2, 2), "pqxy", (l, 2, 2, r), "lpqr", None, (0, 0)),
+        ((5, 2, r), "kqr", (), "", (5, 2, r), "kqr", (1, 9, 18), (0, 0)),
+        ((3, 2, 7), "akb", (2, 5), "kc", (3, 7, 5), "abc", None, (1, 1)),
+    ]
+    probs, hosts, outs, ptrs, conj = [], [], [], [], []
+    for sa, la, sb, lb, sc, lc, st_a, cj in cases:
+        la, lb, lc = list(la), list(lb), list(lc)
+        if st_a is not None:  # a view of the first 5 rows of a 9-row buffer
+            full = O.complex_gaussian(rng, (9, 2, r), dtype)
+            a = full[:5]
+            da = T.DeviceArray.from_host(ctx, full, dtype)
+        else:
+            a = O.complex_gaussian(rng, sa, dtype)
+            da = T.DeviceArray.from_host(ctx, a, dtype)
+        b = O.complex_gaussian(rng, sb, dtype)
+        db = T.DeviceArray.from_host(ctx, b, dtype)
+        dc = T.DeviceArray(ctx, sc, dtype)
+        pr = (sa, la, sb, lb, sc, lc)
+        if st_a is not None:
+            pr = pr + (st_a, None)
+        probs.append(pr)
+        hosts.append((a, b, la, lb, lc, cj))
+        outs.append((da, db, dc))
+        ptrs += [da.ptr, db.ptr, dc.ptr]
+        conj.append(cj)
+    plan = GroupedPlan(ctx, probs, dtype, conj=conj)
+    try:
+        assert plan.info()["smallk"] == len(cases)
+        for rep in range(2):
+            plan.exec(ptrs)
+            for (a, b, la, lb, lc, cj), (_, _, dc) in zip(hosts, outs):
+                a2 = np.conj(a) if cj[0] else a
+                b2 = np.conj(b) if cj[1] else b
+                dims = [x for x in la if x in lb and x not in lc]
+                ref = O.binary_einsum(O.OTensor(np.asarray(a2, np.complex128), la),
+                                      O.OTensor(np.asarray(b2, np.complex128), lb), dims).permute(lc).data
+                assert _rel(dc.to_host(), ref) < tol, (la, lb, lc)
+    finally:
+        plan.close()

@@ -108,3 +108,45 @@ def test_vidal_layer_rejects_overlapping_bonds(ctx):
     g = O.haar_unitary(np.random.default_rng(0), 4)
     with pytest.raises(ValueError):
         v.apply_layer_([g, g], [1, 2])
+
+
+def test_vidal_sweep_with_captured_layers_equals_eager(ctx):
+    """n = 14, depth 18, maxdim 8: the bonds saturate after a few layers and
+    the later layers of each parity replay a captured graph
+    (apply_layer_graph_); sites, Lambdas and bond sizes equal the eager
+    sweep's bit for bit, and the state equals the oracle's Vidal sweep."""
+    import tenet_hip as T
+    from tenet_hip.tebd import upload_gates
+    from tenet_hip.vidal import VidalMPS
+
+    n, depth, chi, thr = 14, 18, 8, 1e-10
+    rng = np.random.default_rng(77)
+    layers = [list(b) if i % 2 == 0 else list(reversed(b)) for i, b in enumerate(O.brickwork_layers(n, depth))]
+    gates = _gates(rng, layers)
+    views = upload_gates(ctx, gates)
+    eager = T.tebd_vidal_sweep(ctx, n, layers, views, chi, thr)
+    used = []
+    orig = VidalMPS.apply_layer_graph_
+
+    def counting(self, *a, **k):
+        used.append(1)
+        return orig(self, *a, **k)
+
+    VidalMPS.apply_layer_graph_ = counting
+    try:
+        graph = T.tebd_vidal_sweep(ctx, n, layers, views, chi, thr, graphs=True)
+    finally:
+        VidalMPS.apply_layer_graph_ = orig
+    print(f"captured layers replayed: {len(used)} of {depth}", flush=True)
+    assert len(used) >= 6
+    assert graph.bondsizes() == eager.bondsizes()
+    for a, b in zip(graph.B, eager.B):
+        assert np.array_equal(a.to_host(), b.to_host())
+    for a, b in zip(graph.lam_host, eager.lam_host):
+        assert np.array_equal(a, b)
+    ref = O.vidal_tebd_run(n, layers, gates, chi, thr)
+    hv = _host_vidal(graph)
+    assert hv.bondsizes() == ref.bondsizes()
+    assert _rel(hv.to_mps().to_dense(), ref.to_mps().to_dense()) < 1e-10
+    graph.close()
+    eager.close()
