@@ -312,6 +312,9 @@ def bench_heff(args):
             ex = ach * pairs
             roof.update({"executed_slice_products_tops": round(ex, 2), "slice_mfma_peak": mpeak,
                          "executed_frac": round(ex / mpeak, 4),
+                         "peak_basis": "FP64 matrix peak, the format the split result is accurate to; frac above 1 "
+                                       "is FP64-accurate work faster than native FP64 can run; executed_frac "
+                                       "prices the slice products against their own matrix-core peak",
                          "executed_basis": f"{pairs} slice-pair products, each the apply's nominal work"})
     line = {
         "metric": "DMRG two-site H_eff apply throughput (nominal complex flops)",

@@ -148,9 +148,12 @@ struct SvdProblem {
   double tol;
 };
 // Problems are passed either as a device array (grouped) or inline (one).
+// units (grouped rounds): when set, a one-dimensional grid whose workgroup i
+// takes (problem, pair) = units[i] instead of (blockIdx.y, blockIdx.x)
 struct SvdBatch {
   const SvdProblem* arr;
   SvdProblem one;
+  const int2* units;
 };
 int launch_svd_init(const SvdBatch& b, int64_t nprob, int64_t max_elems, int max_ncp, int dtype, hipStream_t s);
 // ev0 / ev1 (optional): stamped at the kernel's start and end by the dispatch

@@ -813,7 +813,7 @@ __device__ void gram_mfma(SvSmem& sm, const double* X, int64_t ld, int r0, int r
 
 // Rows [r0, r1) of the pair's columns of M <- M W by MFMA: per 64-row chunk
 // wave w computes the 16 x 16 output block (rows 16 (w % 4), columns
-// 16 (w / 4)) over K = 32 (Yr = Xr Wr - Xi Wi, Yi = Xr Wi + Xi Wr), the W
+// 16 (w / 4)) over K = 32 in three real products (see below), the W
 // operands held in registers for the whole call; the block goes back
 // through LDS so the stores are column-contiguous.  The next chunk's loads
 // are in flight during the products.
@@ -822,6 +822,13 @@ __device__ void apply_mfma(SvSmem& sm, double* M, int64_t ld, int r0, int r1, in
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int rblk = wave & 3, cblk = wave >> 2;
   const int kr = lane >> 4, kc = lane & 15;
+  double wre[B2MAX / 4], wim[B2MAX / 4], wsum[B2MAX / 4];
+#pragma unroll
+  for (int s = 0; s < B2MAX / 4; ++s) {
+    wre[s] = sm.wr[4 * s + kr][16 * cblk + kc];
+    wim[s] = sm.wi[4 * s + kr][16 * cblk + kc];
+    wsum[s] = wre[s] + wim[s];
+  }
   double2 v[STAGE_PER];
   load_rows(v, M, ld, r0, min(RS, r1 - r0), I, J, b);
   for (int rb = r0; rb < r1; rb += RS) {
@@ -830,21 +837,23 @@ __device__ void apply_mfma(SvSmem& sm, double* M, int64_t ld, int r0, int r1, in
     store_rows_all(sm, v);
     __syncthreads();
     if (rb + RS < r1) load_rows(v, M, ld, rb + RS, min(RS, r1 - rb - RS), I, J, b);
-    svd_d4v yr = {0, 0, 0, 0}, yi = {0, 0, 0, 0};
+    // three real products per complex one (Gauss):
+    //   T1 = Xr Wr, T2 = Xi Wi, T3 = (Xr + Xi)(Wr + Wi);
+    //   Re Y = T1 - T2, Im Y = T3 - T1 - T2
+    // (W is unitary, so the cancellation in Im Y costs O(eps |X|))
+    svd_d4v t1 = {0, 0, 0, 0}, t2 = {0, 0, 0, 0}, t3 = {0, 0, 0, 0};
 #pragma unroll
     for (int s = 0; s < B2MAX / 4; ++s) {
       const double xr = sm.xr[16 * rblk + kc][4 * s + kr], xi = sm.xi[16 * rblk + kc][4 * s + kr];
-      const double wre = sm.wr[4 * s + kr][16 * cblk + kc], wim = sm.wi[4 * s + kr][16 * cblk + kc];
-      yr = mfma_f64(xr, wre, yr);
-      yr = mfma_f64(-xi, wim, yr);
-      yi = mfma_f64(xr, wim, yi);
-      yi = mfma_f64(xi, wre, yi);
+      t1 = mfma_f64(xr, wre[s], t1);
+      t2 = mfma_f64(xi, wim[s], t2);
+      t3 = mfma_f64(xr + xi, wsum[s], t3);
     }
     __syncthreads();  // every wave has read its X operands
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      sm.xr[16 * rblk + 4 * r + kr][16 * cblk + kc] = yr[r];
-      sm.xi[16 * rblk + 4 * r + kr][16 * cblk + kc] = yi[r];
+      sm.xr[16 * rblk + 4 * r + kr][16 * cblk + kc] = t1[r] - t2[r];
+      sm.xi[16 * rblk + 4 * r + kr][16 * cblk + kc] = t3[r] - t1[r] - t2[r];
     }
     __syncthreads();
     double2* M2 = reinterpret_cast<double2*>(M);
@@ -885,11 +894,16 @@ __device__ __forceinline__ void round_pair(int p, int round, int mode, int nb, i
 // mode 0: full inner sweep on the fixed pairs; mode 1: cross-block inner steps.
 __global__ __launch_bounds__(SR_THREADS, 4) void svd_round_mfma_kernel(SvdBatch Bt, int round, int mode) {
   __shared__ SvSmem sm;
-  const SvdProblem& P = Bt.arr ? Bt.arr[blockIdx.y] : Bt.one;
+  int prob = blockIdx.y, p = blockIdx.x;
+  if (Bt.units) {
+    const int2 u = Bt.units[blockIdx.x];
+    prob = u.x;
+    p = u.y;
+  }
+  const SvdProblem& P = Bt.arr ? Bt.arr[prob] : Bt.one;
   if (P.flag[1]) return;  // converged
   const int nb = P.nblk;
   if (mode == 1 && (nb <= 2 || round >= nb - 1)) return;
-  const int p = blockIdx.x;
   if (p >= nb / 2) return;
   int I, J;
   round_pair(p, round, mode, nb, I, J);
@@ -1415,7 +1429,12 @@ int launch_svd_init(const SvdBatch& b, int64_t nprob, int64_t max_elems, int max
 
 int launch_svd_round(const SvdBatch& b, int64_t nprob, int max_items, int round, int phase, int mode,
                      hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
-  const dim3 grid((unsigned)max_items, (unsigned)nprob);
+  if (b.units && phase != 2) {
+    set_error("SVD round: unit lists need the fused round kernel");
+    return TENET_ERR_UNSUPPORTED;
+  }
+  // with a unit list, max_items is its length
+  const dim3 grid = b.units ? dim3((unsigned)max_items) : dim3((unsigned)max_items, (unsigned)nprob);
   if (phase == 2) {
     if (ev0)
       hipExtLaunchKernelGGL(svd_round_mfma_kernel, grid, dim3(SR_THREADS), 0, s, ev0, ev1, 0, b, round, mode);

@@ -2107,6 +2107,11 @@ struct tenet_gsvd_s {
   SvdProblem* dprobs = nullptr;
   int32_t* flags_base = nullptr;
   int64_t max_rounds = 0, max_pairs = 0, max_ncp = 0, max_elems = 0, max_out = 0;
+  // per round (0: the fixed-pair round, 1 + r: cross round r) the
+  // (problem, pair) units that have work in it, largest problems first
+  std::vector<int32_t> units;  // pairs of int32
+  std::vector<int64_t> uoff, ucnt;
+  int2* units_dev = nullptr;
   // operands, for the single-matrix fallback of problems the sweeps leave unconverged
   std::vector<int64_t> ms, ns, maxdims;
   std::vector<const void*> As;
@@ -2158,9 +2163,35 @@ static int gsvd_build(tenet_ctx ctx, int dtype, int64_t count, const int64_t* ms
     g->Vhs.push_back(Vhs[i]);
     g->Ss.push_back(Ss ? Ss[i] : nullptr);
   }
+  // round unit lists: a problem of nblk blocks has npairs units in the
+  // fixed-pair round and, when nblk > 2, in cross rounds 0 .. nblk - 2 (the
+  // rounds launch only these units, not a pairs x problems rectangle)
+  {
+    std::vector<int64_t> order((size_t)count);
+    for (int64_t i = 0; i < count; ++i) order[(size_t)i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int64_t x, int64_t y) {
+      return g->shapes[(size_t)x].nr * g->shapes[(size_t)x].ncp > g->shapes[(size_t)y].nr * g->shapes[(size_t)y].ncp;
+    });
+    g->uoff.assign((size_t)g->max_rounds + 1, 0);
+    g->ucnt.assign((size_t)g->max_rounds + 1, 0);
+    for (int64_t r = -1; r < g->max_rounds; ++r) {
+      g->uoff[(size_t)(r + 1)] = (int64_t)g->units.size() / 2;
+      for (int64_t i : order) {
+        const SvdShape& sh = g->shapes[(size_t)i];
+        if (r >= 0 && (sh.nblk <= 2 || r >= sh.nblk - 1)) continue;
+        for (int64_t p = 0; p < sh.npairs; ++p) {
+          g->units.push_back((int32_t)i);
+          g->units.push_back((int32_t)p);
+        }
+      }
+      g->ucnt[(size_t)(r + 1)] = (int64_t)g->units.size() / 2 - g->uoff[(size_t)(r + 1)];
+    }
+    total += align_up(std::max<size_t>(g->units.size(), 2) * 4, 256);
+  }
   TRY(tenet_malloc(ctx, total, &g->dev));
   Bump bm(g->dev);
   g->dprobs = (SvdProblem*)bm.take((size_t)count * sizeof(SvdProblem));
+  g->units_dev = (int2*)bm.take(std::max<size_t>(g->units.size(), 2) * 4);
   g->probs.resize((size_t)count);
   for (int64_t i = 0; i < count; ++i) {
     const SvdShape& sh = g->shapes[(size_t)i];
@@ -2206,6 +2237,8 @@ static int gsvd_build(tenet_ctx ctx, int dtype, int64_t count, const int64_t* ms
   // freed by a caller while that work still used it); g->probs outlives it
   TENET_HIP_CHECK(hipMemcpyAsync(g->dprobs, g->probs.data(), g->probs.size() * sizeof(SvdProblem),
                                  hipMemcpyHostToDevice, s));
+  if (!g->units.empty())
+    TENET_HIP_CHECK(hipMemcpyAsync(g->units_dev, g->units.data(), g->units.size() * 4, hipMemcpyHostToDevice, s));
   return TENET_OK;
 }
 
@@ -2245,9 +2278,14 @@ static int gsvd_run(tenet_gsvd g, hipStream_t s) {
           byr[(size_t)r] += b;
         }
     }
-    TRY(svd_round_timed(ctx, s, fl, by, B, count, (int)g->max_pairs, 0, 2, 0));
-    for (int r = 0; r < g->max_rounds; ++r)
-      TRY(svd_round_timed(ctx, s, flr[(size_t)r], byr[(size_t)r], B, count, (int)g->max_pairs, r, 2, 1));
+    SvdBatch Bu = B;
+    Bu.units = g->units_dev + g->uoff[0];
+    TRY(svd_round_timed(ctx, s, fl, by, Bu, count, (int)g->ucnt[0], 0, 2, 0));
+    for (int r = 0; r < g->max_rounds; ++r) {
+      if (g->ucnt[(size_t)r + 1] == 0) continue;
+      Bu.units = g->units_dev + g->uoff[(size_t)r + 1];
+      TRY(svd_round_timed(ctx, s, flr[(size_t)r], byr[(size_t)r], Bu, count, (int)g->ucnt[(size_t)r + 1], r, 2, 1));
+    }
     TRY(launch_svd_sweep_end(B, count, s));
     if (sync_ok) {
       TENET_HIP_CHECK(hipMemcpyAsync(hflags.data(), g->flags_base, (size_t)count * 32, hipMemcpyDeviceToHost, s));
