@@ -116,6 +116,21 @@ bool qr_small_fits(int64_t m, int64_t n);
 int launch_qr_small(const void* A, int64_t lda, int64_t m, int64_t n, void* Qout, int64_t ldq, void* Rout,
                     int64_t ldr, int dtype, int flags, const QrAbsorbArgs* abs, hipStream_t s,
                     hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+// Batched blocked QR (k_qr.hip): problem i factors its m x n complex128 W in
+// place (R on and above the diagonal, reflectors below), keeping tau (min(m,
+// n) complex) and the panels' T factors (qr_batched_t_bytes); Yp is the
+// trailing update's partial products (qr_batched_yp_bytes with the batch's
+// largest n).  Rows at most 512.
+struct QrBatchProb {
+  double* W;
+  double* tau;
+  double* T;
+  double* Yp;
+  int32_t m, n;
+};
+size_t qr_batched_yp_bytes(int64_t m, int64_t max_n);
+size_t qr_batched_t_bytes(int64_t m, int64_t n);
+int launch_qr_batched(const QrBatchProb* probs, int64_t nprob, int max_m, int max_n, int max_k, hipStream_t s);
 bool qr_blocked_fits(int64_t m);
 size_t qr_blocked_ws_bytes(int64_t m, int64_t n);
 int launch_qr_blocked(double* W, int64_t m, int64_t n, void* Qout, int64_t ldq, void* Rout, int64_t ldr, int dtype,
@@ -146,6 +161,13 @@ struct SvdProblem {
   int64_t* k_dev;
   double* disc_dev;
   double tol;
+  // QR preconditioning (grouped problems with m >= n and no U): W = A[:, cperm]
+  // (m x n complex128, columns of A by descending norm) = Q R in place, the
+  // Jacobi runs on X = R^H and V^H's columns are written through cperm
+  double* W;
+  int32_t* cperm;
+  int32_t pre;
+  int32_t pad1;
 };
 // Problems are passed either as a device array (grouped) or inline (one).
 // units (grouped rounds): when set, a one-dimensional grid whose workgroup i
@@ -164,6 +186,9 @@ int launch_svd_finalize(const SvdBatch& b, int64_t nprob, int max_ncp, int64_t m
                         hipStream_t s);
 // bit 0 of *status <- 1 for every problem not flagged converged
 int launch_svd_status(const SvdBatch& b, int64_t nprob, int32_t* status, hipStream_t s);
+// column norms of A, their descending order into cperm and W = A[:, cperm]
+// for the problems with pre set (the others return at once)
+int launch_svd_pre_pack(const SvdBatch& b, int64_t nprob, int max_n, int64_t max_elems, int dtype, hipStream_t s);
 
 // ---- misc elementwise / K4 ----------------------------------------------------
 int launch_fill_zero(void* dst, size_t bytes, hipStream_t s);

@@ -625,8 +625,8 @@ __device__ __forceinline__ void qp4_tcol(int jj, int lane, const double2* taus, 
 }
 
 template <int RL>
-__global__ __launch_bounds__(64 * QB) void qr_panel_v4_kernel(double* __restrict__ W, double* __restrict__ tau,
-                                                             double* __restrict__ Tout, int m, int j0, int jb) {
+__device__ __forceinline__ void qp4_panel(double* __restrict__ W, double* __restrict__ tau, double* __restrict__ Tout,
+                                          int m, int j0, int jb) {
   __shared__ double2 Vs[2][64 * RL];  // reflector j in Vs[j & 1], panel rows
   __shared__ double2 taus[QB];
   __shared__ double2 Ws[2][QB];    // w_c = v_c^H v_j by the parity of j
@@ -718,6 +718,22 @@ __global__ __launch_bounds__(64 * QB) void qr_panel_v4_kernel(double* __restrict
   }
 }
 
+template <int RL>
+__global__ __launch_bounds__(64 * QB) void qr_panel_v4_kernel(double* __restrict__ W, double* __restrict__ tau,
+                                                             double* __restrict__ Tout, int m, int j0, int jb) {
+  qp4_panel<RL>(W, tau, Tout, m, j0, jb);
+}
+
+// Panel p of every problem of a batch (one workgroup per problem; problems
+// with fewer than p + 1 panels return at once)
+template <int RL>
+__global__ __launch_bounds__(64 * QB) void qr_panel_v4_batched_kernel(const QrBatchProb* __restrict__ probs, int p) {
+  const QrBatchProb& q = probs[blockIdx.x];
+  const int k = min(q.m, q.n), j0 = p * QB;
+  if (j0 >= k) return;
+  qp4_panel<RL>(q.W, q.tau, q.T + (size_t)p * QB * QB * 2, q.m, j0, min(QB, k - j0));
+}
+
 // Q (m x k) = [I; 0] in FP64
 __global__ void qr_eye_kernel(double* __restrict__ Q, int m, int k) {
   const size_t total = (size_t)m * k;
@@ -799,15 +815,13 @@ __device__ __forceinline__ void qa_load_v(const double2* __restrict__ Wc, int m,
   }
 }
 
-__global__ __launch_bounds__(QR_THREADS) void qr_apply_y_kernel(const double* __restrict__ W,
-                                                                 const double* __restrict__ M, double* __restrict__ Yp,
-                                                                 int m, int ldm, int j0, int jb, int col_begin,
-                                                                 int col_end) {
+__device__ __forceinline__ void qa_y_body(const double* __restrict__ W, const double* __restrict__ M,
+                                          double* __restrict__ Yp, int m, int ldm, int j0, int jb, int col_begin,
+                                          int col_end, int blk, int chunk, int nblk) {
   __shared__ double vr[QA_RC][QB + 1], vi[QA_RC][QB + 1];
   __shared__ double xr[QA_RC][QB + 1], xi[QA_RC][QB + 1];
   __shared__ double yred[4][QB * QB][2];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int blk = blockIdx.x, chunk = blockIdx.y;
   const int cb0 = col_begin + blk * QB;
   const int nc = min(QB, col_end - cb0);
   const int rb = j0 + chunk * QA_RC;
@@ -876,7 +890,6 @@ __global__ __launch_bounds__(QR_THREADS) void qr_apply_y_kernel(const double* __
   }
   __syncthreads();
   // Ypart[chunk][blk][a][c], fixed summation order over the four waves
-  const int nblk = gridDim.x;
   double* out = Yp + ((size_t)chunk * nblk + blk) * QB * QB * 2;
   {
     const int e = tid;  // QB * QB = 256 = QR_THREADS
@@ -885,16 +898,21 @@ __global__ __launch_bounds__(QR_THREADS) void qr_apply_y_kernel(const double* __
   }
 }
 
-__global__ __launch_bounds__(QR_THREADS) void qr_apply_x_kernel(const double* __restrict__ W,
-                                                                 const double* __restrict__ T,
-                                                                 const double* __restrict__ Yp, double* __restrict__ M,
+__global__ __launch_bounds__(QR_THREADS) void qr_apply_y_kernel(const double* __restrict__ W,
+                                                                 const double* __restrict__ M, double* __restrict__ Yp,
                                                                  int m, int ldm, int j0, int jb, int col_begin,
-                                                                 int col_end, int nchunks, int adj) {
+                                                                 int col_end) {
+  qa_y_body(W, M, Yp, m, ldm, j0, jb, col_begin, col_end, blockIdx.x, blockIdx.y, gridDim.x);
+}
+
+__device__ __forceinline__ void qa_x_body(const double* __restrict__ W, const double* __restrict__ T,
+                                          const double* __restrict__ Yp, double* __restrict__ M, int m, int ldm,
+                                          int j0, int jb, int col_begin, int col_end, int nchunks, int adj, int blk,
+                                          int chunk, int nblk) {
   __shared__ double vr[QA_RC][QB + 1], vi[QA_RC][QB + 1];
   __shared__ double yr[QB][QB + 1], yi[QB][QB + 1];
   __shared__ double zr[QB][QB + 1], zi[QB][QB + 1];
   const int tid = threadIdx.x;
-  const int blk = blockIdx.x, chunk = blockIdx.y;
   const int cb0 = col_begin + blk * QB;
   const int nc = min(QB, col_end - cb0);
   const int rb = j0 + chunk * QA_RC;
@@ -905,7 +923,6 @@ __global__ __launch_bounds__(QR_THREADS) void qr_apply_x_kernel(const double* __
   qa_load_v(Wc, m, j0, jb, rb, rows, vr, vi);
   {
     // Y = sum over chunks (fixed order), one entry per thread
-    const int nblk = gridDim.x;
     const int a = tid / QB, c = tid % QB;
     double sr = 0.0, si = 0.0;
     for (int ch = 0; ch < nchunks; ++ch) {
@@ -962,6 +979,35 @@ __global__ __launch_bounds__(QR_THREADS) void qr_apply_x_kernel(const double* __
   }
 }
 
+__global__ __launch_bounds__(QR_THREADS) void qr_apply_x_kernel(const double* __restrict__ W,
+                                                                 const double* __restrict__ T,
+                                                                 const double* __restrict__ Yp, double* __restrict__ M,
+                                                                 int m, int ldm, int j0, int jb, int col_begin,
+                                                                 int col_end, int nchunks, int adj) {
+  qa_x_body(W, T, Yp, M, m, ldm, j0, jb, col_begin, col_end, nchunks, adj, blockIdx.x, blockIdx.y, gridDim.x);
+}
+
+// Batched trailing update after panel p: blockIdx.z is the problem; the
+// problem's trailing columns [j0 + jb, n) of W itself, rows [j0, m).  Ypart
+// strides by the grid's (largest) column-block count.
+__global__ __launch_bounds__(QR_THREADS) void qr_apply_y_batched_kernel(const QrBatchProb* __restrict__ probs, int p) {
+  const QrBatchProb& q = probs[blockIdx.z];
+  const int k = min(q.m, q.n), j0 = p * QB;
+  if (j0 >= k) return;
+  const int jb = min(QB, k - j0);
+  qa_y_body(q.W, q.W, q.Yp, q.m, q.m, j0, jb, j0 + jb, q.n, blockIdx.x, blockIdx.y, gridDim.x);
+}
+
+__global__ __launch_bounds__(QR_THREADS) void qr_apply_x_batched_kernel(const QrBatchProb* __restrict__ probs, int p) {
+  const QrBatchProb& q = probs[blockIdx.z];
+  const int k = min(q.m, q.n), j0 = p * QB;
+  if (j0 >= k) return;
+  const int jb = min(QB, k - j0);
+  const int nch = (q.m - j0 + QA_RC - 1) / QA_RC;
+  qa_x_body(q.W, q.T + (size_t)p * QB * QB * 2, q.Yp, q.W, q.m, q.m, j0, jb, j0 + jb, q.n, nch, 1, blockIdx.x,
+            blockIdx.y, gridDim.x);
+}
+
 // M <- (I - V T' V^H) M (adj = 1: T' = T^H, forward application during the
 // factorisation; adj = 0: T' = T, backward accumulation of Q)
 static int qr_apply_chunked(const double* W, const double* T, double* M, int m, int ldm, int j0, int jb, int c_begin,
@@ -975,6 +1021,47 @@ static int qr_apply_chunked(const double* W, const double* T, double* M, int m, 
   hipLaunchKernelGGL(qr_apply_x_kernel, dim3(nblk, nch), dim3(QR_THREADS), 0, s, W, T, Yp, M, m, ldm, j0, jb, c_begin,
                      c_end, (int)nch, adj);
   TENET_HIP_CHECK(hipGetLastError());
+  return TENET_OK;
+}
+
+// Householder QR of every problem of a batch in place (R on and above the
+// diagonal of W, the reflectors below; tau and the panels' T factors kept):
+// per panel one launch of the panel kernel over the problems and one pair of
+// trailing-update launches.  Every m <= 512 (the register panel).
+size_t qr_batched_yp_bytes(int64_t m, int64_t max_n) {
+  return (size_t)((m + QA_RC - 1) / QA_RC) * (size_t)((max_n + QB - 1) / QB) * QB * QB * 16;
+}
+size_t qr_batched_t_bytes(int64_t m, int64_t n) { return (size_t)((std::min(m, n) + QB - 1) / QB) * QB * QB * 16; }
+
+int launch_qr_batched(const QrBatchProb* probs, int64_t nprob, int max_m, int max_n, int max_k, hipStream_t s) {
+  if (nprob <= 0 || max_k <= 0) return TENET_OK;
+  if (max_m > 512 || nprob > 65535) {
+    set_error("batched QR: rows above 512 or more than 65535 problems");
+    return TENET_ERR_UNSUPPORTED;
+  }
+  const int np = (max_k + QB - 1) / QB;
+  for (int p = 0; p < np; ++p) {
+    const int j0 = p * QB;
+    const int mr = max_m - j0;
+    const dim3 g1((unsigned)nprob);
+    if (mr <= 64)
+      hipLaunchKernelGGL(qr_panel_v4_batched_kernel<1>, g1, dim3(64 * QB), 0, s, probs, p);
+    else if (mr <= 128)
+      hipLaunchKernelGGL(qr_panel_v4_batched_kernel<2>, g1, dim3(64 * QB), 0, s, probs, p);
+    else if (mr <= 256)
+      hipLaunchKernelGGL(qr_panel_v4_batched_kernel<4>, g1, dim3(64 * QB), 0, s, probs, p);
+    else
+      hipLaunchKernelGGL(qr_panel_v4_batched_kernel<8>, g1, dim3(64 * QB), 0, s, probs, p);
+    TENET_HIP_CHECK(hipGetLastError());
+    // a problem's trailing columns start at j0 + jb >= j0 + 1: blocks over
+    // [j0 + 1, max_n) cover every problem's (the kernels return past its n)
+    if (max_n - j0 <= 1) continue;
+    const dim3 g2((unsigned)((max_n - j0 - 1 + QB - 1) / QB), (unsigned)((mr + QA_RC - 1) / QA_RC), (unsigned)nprob);
+    hipLaunchKernelGGL(qr_apply_y_batched_kernel, g2, dim3(QR_THREADS), 0, s, probs, p);
+    TENET_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(qr_apply_x_batched_kernel, g2, dim3(QR_THREADS), 0, s, probs, p);
+    TENET_HIP_CHECK(hipGetLastError());
+  }
   return TENET_OK;
 }
 

@@ -2112,6 +2112,10 @@ struct tenet_gsvd_s {
   std::vector<int32_t> units;  // pairs of int32
   std::vector<int64_t> uoff, ucnt;
   int2* units_dev = nullptr;
+  // QR-preconditioned problems (pre set): their batched-QR table, sizes
+  std::vector<QrBatchProb> qr_probs;
+  QrBatchProb* qr_dev = nullptr;
+  int64_t pre_max_m = 0, pre_max_n = 0, pre_max_elems = 0;
   // operands, for the single-matrix fallback of problems the sweeps leave unconverged
   std::vector<int64_t> ms, ns, maxdims;
   std::vector<const void*> As;
@@ -2137,6 +2141,19 @@ static int gsvd_build(tenet_ctx ctx, int dtype, int64_t count, const int64_t* ms
                  align_up((size_t)count * 8, 256);
   const size_t es = ELEM[dtype];
   const size_t s_es = s_mode == 0 ? 8 : (s_mode == 1 ? 16 : 8);
+  // Without U, an m >= n problem with 64 <= n and m <= 512 is QR-preconditioned:
+  // A[:, cperm] = Q R with the columns by descending norm, then the Jacobi on
+  // X = R^H (n x n): V^H comes from X, Q is never formed and no rotation is
+  // accumulated (Drmac and Veselic, SIAM J. Matrix Anal. Appl. 29 (2008))
+  std::vector<char> pre((size_t)count, 0);
+  for (int64_t i = 0; i < count; ++i) {
+    pre[(size_t)i] = ctx->svd_precond && !Us[i] && ms[i] >= ns[i] && ns[i] >= 64 && ms[i] <= 512;
+    if (pre[(size_t)i]) {
+      g->pre_max_m = std::max(g->pre_max_m, ms[i]);
+      g->pre_max_n = std::max(g->pre_max_n, ns[i]);
+      g->pre_max_elems = std::max(g->pre_max_elems, ms[i] * ns[i]);
+    }
+  }
   for (int64_t i = 0; i < count; ++i) {
     CHECK_ARG(ms[i] > 0 && ns[i] > 0 && std::min(ms[i], ns[i]) <= 4096, "bad matrix shape");
     const int64_t m = ms[i], n = ns[i], k = std::min(m, n);
@@ -2147,9 +2164,13 @@ static int gsvd_build(tenet_ctx ctx, int dtype, int64_t count, const int64_t* ms
     if (Ss && Ss[i]) TRY(check_span(ctx, Ss[i], (size_t)k * s_es, "S"));
     // Without U and with m <= n, the Jacobi runs on X = A^H: V^H comes from X
     // itself and the rotations need not be accumulated
-    g->shapes[(size_t)i] = svd_shape(ms[i], ns[i], false, !Us[i] && m <= n);
+    g->shapes[(size_t)i] = pre[(size_t)i] ? svd_shape(n, n, false, true) : svd_shape(m, n, false, !Us[i] && m <= n);
     const SvdShape& sh = g->shapes[(size_t)i];
     total += svd_ws_bytes(sh);
+    if (pre[(size_t)i])
+      total += align_up((size_t)m * n * 16, 256) + align_up((size_t)n * 16, 256) +
+               align_up(qr_batched_t_bytes(m, n), 256) + align_up(qr_batched_yp_bytes(m, g->pre_max_n), 256) +
+               align_up((size_t)n * 4, 256) + align_up(sizeof(QrBatchProb), 256);
     g->max_rounds = std::max(g->max_rounds, sh.nblk - 1);
     g->max_pairs = std::max(g->max_pairs, sh.npairs);
     g->max_ncp = std::max(g->max_ncp, sh.ncp);
@@ -2224,7 +2245,24 @@ static int gsvd_build(tenet_ctx ctx, int dtype, int64_t count, const int64_t* ms
     P.k_dev = k_dev ? k_dev + i : nullptr;
     P.disc_dev = disc_dev ? disc_dev + i : nullptr;
     P.tol = svd_tol(sh.nr);
+    if (pre[(size_t)i]) {
+      const int64_t m = ms[i], n = ns[i];
+      P.pre = 1;
+      P.accum = 0;
+      P.ldvh = n;
+      P.W = (double*)bm.take((size_t)m * n * 16);
+      P.cperm = (int32_t*)bm.take((size_t)n * 4);
+      QrBatchProb q{};
+      q.W = P.W;
+      q.tau = (double*)bm.take((size_t)n * 16);
+      q.T = (double*)bm.take(qr_batched_t_bytes(m, n));
+      q.Yp = (double*)bm.take(qr_batched_yp_bytes(m, g->pre_max_n));
+      q.m = (int32_t)m;
+      q.n = (int32_t)n;
+      g->qr_probs.push_back(q);
+    }
   }
+  if (!g->qr_probs.empty()) g->qr_dev = (QrBatchProb*)bm.take(g->qr_probs.size() * sizeof(QrBatchProb));
   // per problem 8 int32: rotated, converged, sweeps, spare, then the sweep's
   // largest squared relative off-diagonal (8 bytes at flag + 4)
   g->flags_base = (int32_t*)bm.take((size_t)count * 32);
@@ -2239,6 +2277,9 @@ static int gsvd_build(tenet_ctx ctx, int dtype, int64_t count, const int64_t* ms
                                  hipMemcpyHostToDevice, s));
   if (!g->units.empty())
     TENET_HIP_CHECK(hipMemcpyAsync(g->units_dev, g->units.data(), g->units.size() * 4, hipMemcpyHostToDevice, s));
+  if (!g->qr_probs.empty())
+    TENET_HIP_CHECK(hipMemcpyAsync(g->qr_dev, g->qr_probs.data(), g->qr_probs.size() * sizeof(QrBatchProb),
+                                   hipMemcpyHostToDevice, s));
   return TENET_OK;
 }
 
@@ -2253,6 +2294,11 @@ static int gsvd_run(tenet_gsvd g, hipStream_t s) {
   const bool sync_ok = !capturing(s);
   SvdBatch B{};
   B.arr = g->dprobs;
+  if (!g->qr_probs.empty()) {
+    TRY(launch_svd_pre_pack(B, count, (int)g->pre_max_n, g->pre_max_elems, g->dtype, s));
+    TRY(launch_qr_batched(g->qr_dev, (int64_t)g->qr_probs.size(), (int)g->pre_max_m, (int)g->pre_max_n,
+                          (int)g->pre_max_n, s));
+  }
   TRY(launch_svd_init(B, count, g->max_elems, (int)g->max_ncp, g->dtype, s));
   std::vector<int32_t> hflags((size_t)count * 8, 0);
   bool all = false;

@@ -98,13 +98,15 @@ def autotune_tiles(ctx: Context, dtype=np.complex64):
     return list(out)
 
 
-def grouped_svd(ctx: Context, mats, maxdims=None, threshold=0.0, want_s=True, want_k=True, want_disc=True):
+def grouped_svd(ctx: Context, mats, maxdims=None, threshold=0.0, want_s=True, want_k=True, want_disc=True,
+                want_u=True):
     """Thin SVDs of device matrices (DeviceArray, column-major m x n) in one
     grouped launch with per-problem truncation (compress.jl:46-59 rule).
     Returns per problem (U (m x kmax), S (kmax,), Vh (kmax x n), k, discarded
     weight) with only the first k columns / entries / rows meaningful.
-    want_s / want_k / want_disc = False pass NULL for that output (the
-    returned entry is then None)."""
+    want_s / want_k / want_disc / want_u = False pass NULL for that output
+    (the returned entry is then None); without U, problems with
+    64 <= n <= m <= 512 take the QR-preconditioned Jacobi."""
     cnt = len(mats)
     if cnt == 0:
         return []
@@ -120,7 +122,7 @@ def grouped_svd(ctx: Context, mats, maxdims=None, threshold=0.0, want_s=True, wa
     u_off = np.concatenate([[0], np.cumsum(ms * kmax)])
     v_off = np.concatenate([[0], np.cumsum(kmax * ns)])
     s_off = np.concatenate([[0], np.cumsum(kmax)])
-    u_arena = DeviceArray(ctx, (int(u_off[-1]),), dt)
+    u_arena = DeviceArray(ctx, (int(u_off[-1]) if want_u else 1,), dt)
     v_arena = DeviceArray(ctx, (int(v_off[-1]),), dt)
     Us, Vs = [], []
     for i in range(cnt):
@@ -138,7 +140,7 @@ def grouped_svd(ctx: Context, mats, maxdims=None, threshold=0.0, want_s=True, wa
         s_ptrs = (P * cnt)(*[s_all + 8 * int(o) for o in s_off[:-1]]) if want_s else None
         L.check(L.lib.tenet_grouped_svd(
             ctx.h, None, code, cnt, ms.ctypes.data_as(C.POINTER(C.c_int64)), ns.ctypes.data_as(C.POINTER(C.c_int64)),
-            (P * cnt)(*[a.ptr for a in mats]), (P * cnt)(*[u.ptr for u in Us]), s_ptrs,
+            (P * cnt)(*[a.ptr for a in mats]), (P * cnt)(*[u.ptr if want_u else None for u in Us]), s_ptrs,
             (P * cnt)(*[v.ptr for v in Vs]), md.ctypes.data_as(C.POINTER(C.c_int64)), float(threshold),
             P(kd) if want_k else None, P(dd) if want_disc else None),
             "grouped_svd")
@@ -148,7 +150,7 @@ def grouped_svd(ctx: Context, mats, maxdims=None, threshold=0.0, want_s=True, wa
         L.check(L.lib.tenet_memcpy_d2h(ctx.h, kh.ctypes.data_as(P), P(kd), kh.nbytes, None))
         L.check(L.lib.tenet_memcpy_d2h(ctx.h, dh.ctypes.data_as(P), P(dd), dh.nbytes, None))
         L.check(L.lib.tenet_memcpy_d2h(ctx.h, sh.ctypes.data_as(P), P(s_all), sh.nbytes, None))
-        return [(Us[i], sh[s_off[i]:s_off[i + 1]].copy() if want_s else None, Vs[i],
+        return [(Us[i] if want_u else None, sh[s_off[i]:s_off[i + 1]].copy() if want_s else None, Vs[i],
                  int(kh[i]) if want_k else None, float(dh[i]) if want_disc else None) for i in range(cnt)]
     finally:
         ctx.free(s_all)

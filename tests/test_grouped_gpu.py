@@ -234,3 +234,56 @@ def test_grouped_small_k_elementwise_units(ctx, dtype, tol):
                 assert _rel(dc.to_host(), ref) < tol, (la, lb, lc)
     finally:
         plan.close()
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.complex128, 1e-11), (np.complex64, 2e-5)])
+def test_grouped_svd_without_u_preconditioned(ctx, dtype, tol):
+    """Grouped SVDs with no U output: the problems with 64 <= n <= m <= 512
+    take the QR-preconditioned Jacobi (columns sorted by norm, Householder
+    QR, Jacobi on R^H, V^H written through the column order); the others
+    (wide, m > 512, n < 64) the plain one.  Gaussian, graded (sigma over 12
+    decades), exactly rank-deficient and Vidal-like (rows and columns
+    scaled by decaying Schmidt values) matrices, with a maxdim cut on two.
+    Against LAPACK on the same (rounded) input: the kept count, S, V^H's
+    kept rows orthonormal, and A projected on them equal to the best rank-k
+    product."""
+    import tenet_hip as T
+    from tenet_hip.grouped import grouped_svd
+
+    rng = np.random.default_rng(31)
+
+    def graded(m, n, decades=12):
+        k = min(m, n)
+        qa, _ = np.linalg.qr(O.complex_gaussian(rng, (m, k)))
+        qb, _ = np.linalg.qr(O.complex_gaussian(rng, (n, k)))
+        return (qa * np.logspace(0, -decades, k)) @ qb.conj().T
+
+    def lowrank(m, n, r):
+        return O.complex_gaussian(rng, (m, r)) @ O.complex_gaussian(rng, (r, n))
+
+    def vidal_like(chi):
+        lam_l = np.sort(rng.exponential(size=chi))[::-1] ** 3
+        lam_r = np.sort(rng.exponential(size=chi))[::-1] ** 3
+        th = O.complex_gaussian(rng, (chi, 2, 2, chi)) * lam_r[None, None, None, :]
+        return (lam_l[:, None, None, None] * th).reshape(2 * chi, 2 * chi)
+
+    hosts = [O.complex_gaussian(rng, (512, 512)), graded(512, 256), O.complex_gaussian(rng, (300, 64)),
+             O.complex_gaussian(rng, (64, 100)), O.complex_gaussian(rng, (600, 100)), O.complex_gaussian(rng, (40, 17)),
+             lowrank(130, 130, 40), vidal_like(256), vidal_like(96)]
+    hosts = [h / np.linalg.norm(h) for h in hosts]
+    hosts = [h.astype(dtype).astype(np.complex128) for h in hosts]
+    maxdims = [min(h.shape) for h in hosts]
+    maxdims[0], maxdims[7] = 256, 200
+    mats = [T.DeviceArray.from_host(ctx, h.astype(dtype)) for h in hosts]
+    res = grouped_svd(ctx, mats, maxdims, 1e-10, want_u=False)
+    for i, (h, md, (U, S, Vh, k, disc)) in enumerate(zip(hosts, maxdims, res)):
+        assert U is None
+        u, s_ref, vh = np.linalg.svd(h, full_matrices=False)
+        kref = min(md, int(np.sum(s_ref > 1e-10)))
+        assert k == kref, i
+        assert np.allclose(S[:k], s_ref[:k], rtol=1e-12, atol=1e-13 * s_ref[0]), i
+        assert disc == pytest.approx(np.sum(s_ref[k:] ** 2), rel=1e-8, abs=1e-20), i
+        Vk = Vh.to_host()[:k, :].astype(np.complex128)
+        assert np.abs(Vk @ Vk.conj().T - np.eye(k)).max() < tol, i
+        best = (u[:, :k] * s_ref[:k]) @ vh[:k]
+        assert _rel(h @ Vk.conj().T @ Vk, best) < tol, i
