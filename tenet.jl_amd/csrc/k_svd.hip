@@ -1432,23 +1432,40 @@ __global__ __launch_bounds__(CP_THREADS) void svd_complete_kernel(SvdBatch Bt) {
         __syncthreads();
       }
       for (int pass = 0; pass < 2; ++pass) {
-        // c = F[:, :j]^H v (one wave per column, lanes over rows)
-        for (int i = wave; i < j; i += CP_THREADS / 64) {
-          double sr = 0.0, si = 0.0;
-          for (int64_t r = lane; r < L; r += 64) {
-            const double2 a = F.get(r, i), v = F.get(r, j);
-            sr += a.x * v.x + a.y * v.y;
-            si += a.x * v.y - a.y * v.x;
+        // c = F[:, :j]^H v
+        if (!F.adj) {
+          // columns contiguous: one wave per column, lanes over rows
+          for (int i = wave; i < j; i += CP_THREADS / 64) {
+            double sr = 0.0, si = 0.0;
+            for (int64_t r = lane; r < L; r += 64) {
+              const double2 a = F.get(r, i), v = F.get(r, j);
+              sr += a.x * v.x + a.y * v.y;
+              si += a.x * v.y - a.y * v.x;
+            }
+            sr = cp_wave_sum(sr);
+            si = cp_wave_sum(si);
+            if (lane == 0) {
+              cr[i] = sr;
+              ci[i] = si;
+            }
           }
-          sr = cp_wave_sum(sr);
-          si = cp_wave_sum(si);
-          if (lane == 0) {
+        } else {
+          // rows of V^H: F's column index is the contiguous one, so one
+          // thread per column i walks the rows (v[r] is the same for all)
+          for (int i = tid; i < j; i += CP_THREADS) {
+            double sr = 0.0, si = 0.0;
+            for (int64_t r = 0; r < L; ++r) {
+              const double2 a = F.get(r, i), v = F.get(r, j);
+              sr += a.x * v.x + a.y * v.y;
+              si += a.x * v.y - a.y * v.x;
+            }
             cr[i] = sr;
             ci[i] = si;
           }
         }
         __syncthreads();
-        // v -= F[:, :j] c (one thread per row)
+        // v -= F[:, :j] c (one thread per row; with V^H's rows each thread
+        // walks contiguous memory)
         for (int64_t r = tid; r < L; r += CP_THREADS) {
           double2 v = F.get(r, j);
           for (int i = 0; i < j; ++i) {

@@ -117,6 +117,10 @@ def test_captured_svd_sets_status_when_sweeps_do_not_suffice(ctx):
     S, kd = ctx.malloc(8 * n), ctx.malloc(64)
     ctx.svd_precondition(False)
     try:
+        # the capture cannot grow the workspace: reserve what this SVD needs
+        need = C.c_size_t()
+        L.check(L.lib.tenet_workspace_size(ctx.h, L.OP_SVD, L.C128, m, n, C.byref(need)), "workspace_size")
+        L.check(L.lib.tenet_workspace_reserve(ctx.h, None, need.value), "workspace_reserve")
         assert ctx.graph_status() == 0
         with ctx.capture() as g:
             L.check(_svd_call(ctx, A, m, n, U, S, Vh, kd), "svd (captured)")
