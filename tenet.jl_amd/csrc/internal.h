@@ -186,6 +186,14 @@ int launch_svd_finalize(const SvdBatch& b, int64_t nprob, int max_ncp, int64_t m
                         hipStream_t s);
 // bit 0 of *status <- 1 for every problem not flagged converged
 int launch_svd_status(const SvdBatch& b, int64_t nprob, int32_t* status, hipStream_t s);
+// single matrix: T = A (adj 0) or A^H (adj 1), rows x cols; squared column
+// norms into nrm, their descending order into perm, Tp = T[:, perm] (c128)
+int launch_pre1_order_pack(const void* A, int64_t lda, int rows, int cols, int adj, int dtype, double* nrm,
+                           int32_t* perm, double* Tp, hipStream_t s);
+// complex128 src (rows x cols, ld lds) through the column order perm: mode 0
+// dst[:, perm[c]] = src[:, c]; mode 1 dst[perm[c], r] = conj(src[r, c])
+int launch_perm_scatter(const double* src, int rows, int cols, int64_t lds, double* dst, int64_t ldd,
+                        const int32_t* perm, int mode, hipStream_t s);
 // column norms of A, their descending order into cperm and W = A[:, cperm]
 // for the problems with pre set (the others return at once)
 int launch_svd_pre_pack(const SvdBatch& b, int64_t nprob, int max_n, int64_t max_elems, int dtype, hipStream_t s);

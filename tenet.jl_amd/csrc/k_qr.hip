@@ -987,6 +987,82 @@ __global__ __launch_bounds__(QR_THREADS) void qr_apply_x_kernel(const double* __
   qa_x_body(W, T, Yp, M, m, ldm, j0, jb, col_begin, col_end, nchunks, adj, blockIdx.x, blockIdx.y, gridDim.x);
 }
 
+// Batched form: Z = T^H sum_chunks Ypart formed once per (problem, column
+// block) and stored over that block's chunk-0 partial (its only reader is
+// the workgroup that overwrites it, after the sum), so each row-chunk
+// workgroup of the update reads one 16 x 16 block instead of every partial.
+__device__ __forceinline__ void qa_z_body(const double* __restrict__ T, double* __restrict__ Yp, int jb,
+                                          int nchunks, int blk, int nblk) {
+  __shared__ double yr[QB][QB + 1], yi[QB][QB + 1];
+  const int tid = threadIdx.x;
+  const int a = tid / QB, c = tid % QB;
+  {
+    double sr = 0.0, si = 0.0;
+    for (int ch = 0; ch < nchunks; ++ch) {
+      const double* in = Yp + ((size_t)ch * nblk + blk) * QB * QB * 2;
+      sr += in[2 * (a * QB + c)];
+      si += in[2 * (a * QB + c) + 1];
+    }
+    yr[a][c] = sr;
+    yi[a][c] = si;
+  }
+  __syncthreads();
+  double sr = 0.0, si = 0.0;
+  for (int b = 0; b < jb; ++b) {
+    const double tr = T[2 * (a * QB + b)], ti = -T[2 * (a * QB + b) + 1];
+    sr += tr * yr[b][c] - ti * yi[b][c];
+    si += tr * yi[b][c] + ti * yr[b][c];
+  }
+  double* out = Yp + (size_t)blk * QB * QB * 2;
+  out[2 * (a * QB + c)] = (a < jb) ? sr : 0.0;
+  out[2 * (a * QB + c) + 1] = (a < jb) ? si : 0.0;
+}
+
+__device__ __forceinline__ void qa_xz_body(const double* __restrict__ W, const double* __restrict__ Z,
+                                           double* __restrict__ M, int m, int ldm, int j0, int jb, int col_begin,
+                                           int col_end, int blk, int chunk) {
+  __shared__ double vr[QA_RC][QB + 1], vi[QA_RC][QB + 1];
+  __shared__ double zr[QB][QB + 1], zi[QB][QB + 1];
+  const int tid = threadIdx.x;
+  const int cb0 = col_begin + blk * QB;
+  const int nc = min(QB, col_end - cb0);
+  const int rb = j0 + chunk * QA_RC;
+  const int rows = min(QA_RC, m - rb);
+  if (nc <= 0 || rows <= 0) return;
+  const double2* Wc = reinterpret_cast<const double2*>(W);
+  double2* Mc = reinterpret_cast<double2*>(M);
+  qa_load_v(Wc, m, j0, jb, rb, rows, vr, vi);
+  {
+    const int a = tid / QB, c = tid % QB;
+    zr[a][c] = Z[2 * (a * QB + c)];
+    zi[a][c] = Z[2 * (a * QB + c) + 1];
+  }
+  __syncthreads();
+  // M[chunk rows] -= V Z: thread owns row tid % 64 and 4 columns
+  const int rsub = tid % QA_RC, cq = 4 * (tid / QA_RC);
+  if (rsub < rows) {
+    double sr[4] = {0, 0, 0, 0}, si[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int b = 0; b < QB; ++b) {
+      const double pr = vr[rsub][b], pi = vi[rsub][b];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        sr[j] += pr * zr[b][cq + j] - pi * zi[b][cq + j];
+        si[j] += pr * zi[b][cq + j] + pi * zr[b][cq + j];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if (cq + j < nc) {
+        double2 x = Mc[(size_t)(cb0 + cq + j) * ldm + rb + rsub];
+        x.x -= sr[j];
+        x.y -= si[j];
+        Mc[(size_t)(cb0 + cq + j) * ldm + rb + rsub] = x;
+      }
+    }
+  }
+}
+
 // Batched trailing update after panel p: blockIdx.z is the problem; the
 // problem's trailing columns [j0 + jb, n) of W itself, rows [j0, m).  Ypart
 // strides by the grid's (largest) column-block count.
@@ -998,14 +1074,23 @@ __global__ __launch_bounds__(QR_THREADS) void qr_apply_y_batched_kernel(const Qr
   qa_y_body(q.W, q.W, q.Yp, q.m, q.m, j0, jb, j0 + jb, q.n, blockIdx.x, blockIdx.y, gridDim.x);
 }
 
+__global__ __launch_bounds__(QR_THREADS) void qr_apply_z_batched_kernel(const QrBatchProb* __restrict__ probs, int p) {
+  const QrBatchProb& q = probs[blockIdx.z];
+  const int k = min(q.m, q.n), j0 = p * QB;
+  if (j0 >= k) return;
+  const int jb = min(QB, k - j0);
+  if (j0 + jb + (int)blockIdx.x * QB >= q.n) return;  // no such column block
+  const int nch = (q.m - j0 + QA_RC - 1) / QA_RC;
+  qa_z_body(q.T + (size_t)p * QB * QB * 2, q.Yp, jb, nch, blockIdx.x, gridDim.x);
+}
+
 __global__ __launch_bounds__(QR_THREADS) void qr_apply_x_batched_kernel(const QrBatchProb* __restrict__ probs, int p) {
   const QrBatchProb& q = probs[blockIdx.z];
   const int k = min(q.m, q.n), j0 = p * QB;
   if (j0 >= k) return;
   const int jb = min(QB, k - j0);
-  const int nch = (q.m - j0 + QA_RC - 1) / QA_RC;
-  qa_x_body(q.W, q.T + (size_t)p * QB * QB * 2, q.Yp, q.W, q.m, q.m, j0, jb, j0 + jb, q.n, nch, 1, blockIdx.x,
-            blockIdx.y, gridDim.x);
+  qa_xz_body(q.W, q.Yp + (size_t)blockIdx.x * QB * QB * 2, q.W, q.m, q.m, j0, jb, j0 + jb, q.n, blockIdx.x,
+             blockIdx.y);
 }
 
 // M <- (I - V T' V^H) M (adj = 1: T' = T^H, forward application during the
@@ -1058,6 +1143,8 @@ int launch_qr_batched(const QrBatchProb* probs, int64_t nprob, int max_m, int ma
     if (max_n - j0 <= 1) continue;
     const dim3 g2((unsigned)((max_n - j0 - 1 + QB - 1) / QB), (unsigned)((mr + QA_RC - 1) / QA_RC), (unsigned)nprob);
     hipLaunchKernelGGL(qr_apply_y_batched_kernel, g2, dim3(QR_THREADS), 0, s, probs, p);
+    TENET_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(qr_apply_z_batched_kernel, dim3(g2.x, 1, g2.z), dim3(QR_THREADS), 0, s, probs, p);
     TENET_HIP_CHECK(hipGetLastError());
     hipLaunchKernelGGL(qr_apply_x_batched_kernel, g2, dim3(QR_THREADS), 0, s, probs, p);
     TENET_HIP_CHECK(hipGetLastError());

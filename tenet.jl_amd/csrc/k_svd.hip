@@ -1598,6 +1598,128 @@ int launch_svd_pre_pack(const SvdBatch& b, int64_t nprob, int max_n, int64_t max
   return TENET_OK;
 }
 
+// Single-matrix form of the column ordering (tenet_svd_thin's preconditioned
+// path): T = A (adj = 0, rows x cols = m x n) or A^H (adj = 1, n x m);
+// norms of T's columns, their descending order (ties by index) into perm, and
+// Tp = T[:, perm] in complex128 (rows x cols, ld rows).
+template <typename S2>
+__global__ __launch_bounds__(SV_THREADS) void pre1_norm_kernel(const S2* __restrict__ A, int64_t lda, int rows,
+                                                                int cols, int adj, double* __restrict__ nrm) {
+  const int lane = threadIdx.x & 63;
+  const int j = blockIdx.x * (SV_THREADS / 64) + (threadIdx.x >> 6);
+  if (j >= cols) return;
+  double s = 0.0;
+  for (int r = lane; r < rows; r += 64) {
+    const S2 v = adj ? A[(size_t)r * lda + j] : A[(size_t)j * lda + r];
+    s += (double)v.x * v.x + (double)v.y * v.y;
+  }
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if (lane == 0) nrm[j] = s;
+}
+
+__global__ __launch_bounds__(SV_THREADS) void pre1_sort_kernel(const double* __restrict__ nrm, int cols,
+                                                                int32_t* __restrict__ perm) {
+  __shared__ double key[SORT_MAX];
+  __shared__ int idx[SORT_MAX];
+  const int tid = threadIdx.x;
+  int np2 = 1;
+  while (np2 < cols) np2 <<= 1;
+  for (int j = tid; j < np2; j += SV_THREADS) {
+    key[j] = (j < cols) ? nrm[j] : -1.0;
+    idx[j] = j;
+  }
+  __syncthreads();
+  for (int size = 2; size <= np2; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int i = tid; i < np2; i += SV_THREADS) {
+        const int j = i ^ stride;
+        if (j > i) {
+          const bool desc = ((i & size) == 0);
+          const double ki = key[i], kj = key[j];
+          const int ii = idx[i], ij = idx[j];
+          const bool i_first = (ki > kj) || (ki == kj && ii < ij);
+          if (desc != i_first) {
+            key[i] = kj;
+            key[j] = ki;
+            idx[i] = ij;
+            idx[j] = ii;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int j = tid; j < cols; j += SV_THREADS) perm[j] = idx[j];
+}
+
+template <typename S2>
+__global__ void pre1_pack_kernel(const S2* __restrict__ A, int64_t lda, int rows, int cols, int adj,
+                                 const int32_t* __restrict__ perm, double2* __restrict__ Tp) {
+  const size_t total = (size_t)rows * cols;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    const int r = (int)(e % rows), c = (int)(e / rows);
+    const int pc = perm[c];
+    const S2 v = adj ? A[(size_t)r * lda + pc] : A[(size_t)pc * lda + r];
+    Tp[e] = make_double2((double)v.x, adj ? -(double)v.y : (double)v.y);
+  }
+}
+
+// dst from src (complex128, rows x cols, ld lds) through the column order
+// perm: mode 0 dst[:, perm[c]] = src[:, c] (ld ldd); mode 1 (conjugate
+// transpose) dst[perm[c], r] = conj(src[r, c]).
+__global__ void perm_scatter_kernel(const double2* __restrict__ src, int rows, int cols, int64_t lds,
+                                    double2* __restrict__ dst, int64_t ldd, const int32_t* __restrict__ perm,
+                                    int mode) {
+  const size_t total = (size_t)rows * cols;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    const int r = (int)(e % rows), c = (int)(e / rows);
+    const double2 v = src[(size_t)c * lds + r];
+    if (mode == 0)
+      dst[(size_t)perm[c] * ldd + r] = v;
+    else
+      dst[(size_t)r * ldd + perm[c]] = make_double2(v.x, -v.y);
+  }
+}
+
+int launch_pre1_order_pack(const void* A, int64_t lda, int rows, int cols, int adj, int dtype, double* nrm,
+                           int32_t* perm, double* Tp, hipStream_t s) {
+  if (cols > SORT_MAX) {
+    set_error("SVD preconditioning: more than 4096 columns to order");
+    return TENET_ERR_UNSUPPORTED;
+  }
+  const dim3 gn((unsigned)((cols + SV_THREADS / 64 - 1) / (SV_THREADS / 64)));
+  if (dtype == TENET_C128)
+    hipLaunchKernelGGL(pre1_norm_kernel<double2>, gn, dim3(SV_THREADS), 0, s, (const double2*)A, lda, rows, cols,
+                       adj, nrm);
+  else
+    hipLaunchKernelGGL(pre1_norm_kernel<float2>, gn, dim3(SV_THREADS), 0, s, (const float2*)A, lda, rows, cols, adj,
+                       nrm);
+  TENET_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(pre1_sort_kernel, dim3(1), dim3(SV_THREADS), 0, s, nrm, cols, perm);
+  TENET_HIP_CHECK(hipGetLastError());
+  const size_t total = (size_t)rows * cols;
+  const unsigned g = (unsigned)std::max<size_t>(1, std::min<size_t>((total + 255) / 256, 2048));
+  if (dtype == TENET_C128)
+    hipLaunchKernelGGL(pre1_pack_kernel<double2>, dim3(g), dim3(256), 0, s, (const double2*)A, lda, rows, cols, adj,
+                       perm, (double2*)Tp);
+  else
+    hipLaunchKernelGGL(pre1_pack_kernel<float2>, dim3(g), dim3(256), 0, s, (const float2*)A, lda, rows, cols, adj,
+                       perm, (double2*)Tp);
+  TENET_HIP_CHECK(hipGetLastError());
+  return TENET_OK;
+}
+
+int launch_perm_scatter(const double* src, int rows, int cols, int64_t lds, double* dst, int64_t ldd,
+                        const int32_t* perm, int mode, hipStream_t s) {
+  const size_t total = (size_t)rows * cols;
+  if (total == 0) return TENET_OK;
+  const unsigned g = (unsigned)std::max<size_t>(1, std::min<size_t>((total + 255) / 256, 2048));
+  hipLaunchKernelGGL(perm_scatter_kernel, dim3(g), dim3(256), 0, s, (const double2*)src, rows, cols, lds,
+                     (double2*)dst, ldd, perm, mode);
+  TENET_HIP_CHECK(hipGetLastError());
+  return TENET_OK;
+}
+
 int launch_svd_status(const SvdBatch& b, int64_t nprob, int32_t* status, hipStream_t s) {
   hipLaunchKernelGGL(svd_status_kernel, dim3((unsigned)nprob), dim3(64), 0, s, b, status);
   TENET_HIP_CHECK(hipGetLastError());
