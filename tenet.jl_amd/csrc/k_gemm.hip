@@ -514,23 +514,18 @@ __device__ __forceinline__ void grouped_unit(const GroupedProblem& q, const uint
 // A GROUPED_SMALLK unit: output elements [tile * SMALLK_ELEMS, +SMALLK_ELEMS)
 // of the flattened (m, n, z) space, the class q.order varying fastest (the
 // one with C's smallest stride, so consecutive threads write consecutive
-// addresses); each thread sums its K <= 8 products directly.
-template <typename S2, typename R>
-__device__ __forceinline__ void grouped_smallk_unit(const GroupedProblem& q, const uint64_t* __restrict__ ptrs,
-                                                    const GroupedUnit& u) {
-  const S2* __restrict__ A = reinterpret_cast<const S2*>(ptrs[3 * u.prob]);
-  const S2* __restrict__ B = reinterpret_cast<const S2*>(ptrs[3 * u.prob + 1]);
-  S2* __restrict__ C = reinterpret_cast<S2*>(ptrs[3 * u.prob + 2]);
-  const int64_t total = q.M * q.N * q.batch;
-  const int64_t e0 = u.tile * SMALLK_ELEMS;
-  const int64_t e1 = std::min<int64_t>(total, e0 + SMALLK_ELEMS);
-  // extents in decode order: fastest, middle, slowest
-  const int64_t ext0 = q.order == 0 ? q.M : (q.order == 1 ? q.N : q.batch);
-  const int64_t ext1 = q.order == 0 ? q.N : q.M;
+// addresses); each thread sums its K <= 8 products directly.  The element
+// index is decoded in 32-bit arithmetic when the problem's element count
+// fits (a 64-bit division is a long software sequence on the GPU; the
+// Vidal layer's gate contraction spent most of its time there).
+template <typename S2, typename R, typename IT>
+__device__ __forceinline__ void smallk_elems(const GroupedProblem& q, const S2* __restrict__ A,
+                                             const S2* __restrict__ B, S2* __restrict__ C, IT e0, IT e1, IT ext0,
+                                             IT ext1) {
   const R sa = q.conjA ? (R)-1 : (R)1, sb = q.conjB ? (R)-1 : (R)1;
-  for (int64_t e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
-    const int64_t i0 = e % ext0, rest = e / ext0;
-    const int64_t i1 = rest % ext1, i2 = rest / ext1;
+  for (IT e = e0 + (IT)threadIdx.x; e < e1; e += (IT)blockDim.x) {
+    const IT i0 = e % ext0, rest = e / ext0;
+    const IT i1 = rest % ext1, i2 = rest / ext1;
     int64_t m, n, z;
     if (q.order == 0) {
       m = i0;
@@ -564,6 +559,24 @@ __device__ __forceinline__ void grouped_smallk_unit(const GroupedProblem& q, con
     o.y = ci;
     C[q.t.rowC[m] + q.t.colC[n] + q.t.bC[z]] = o;
   }
+}
+
+template <typename S2, typename R>
+__device__ __forceinline__ void grouped_smallk_unit(const GroupedProblem& q, const uint64_t* __restrict__ ptrs,
+                                                    const GroupedUnit& u) {
+  const S2* __restrict__ A = reinterpret_cast<const S2*>(ptrs[3 * u.prob]);
+  const S2* __restrict__ B = reinterpret_cast<const S2*>(ptrs[3 * u.prob + 1]);
+  S2* __restrict__ C = reinterpret_cast<S2*>(ptrs[3 * u.prob + 2]);
+  const int64_t total = q.M * q.N * q.batch;
+  const int64_t e0 = u.tile * SMALLK_ELEMS;
+  const int64_t e1 = std::min<int64_t>(total, e0 + SMALLK_ELEMS);
+  // extents in decode order: fastest, middle, slowest
+  const int64_t ext0 = q.order == 0 ? q.M : (q.order == 1 ? q.N : q.batch);
+  const int64_t ext1 = q.order == 0 ? q.N : q.M;
+  if (total <= (int64_t)INT32_MAX)
+    smallk_elems<S2, R, uint32_t>(q, A, B, C, (uint32_t)e0, (uint32_t)e1, (uint32_t)ext0, (uint32_t)ext1);
+  else
+    smallk_elems<S2, R, int64_t>(q, A, B, C, e0, e1, ext0, ext1);
 }
 
 template <typename S2, typename R>
