@@ -1341,10 +1341,14 @@ __global__ void svd_write_kernel(SvdBatch Bt) {
 // 1e-3 of its length before), the unit vector e_t (t = j, j+1, ... mod L)
 // is used instead.  The result is an orthonormal completion as LAPACK
 // returns it; the product U diag(S) V^H changes by at most
-// 1e-8 sigma_0 * O(eps) per column.  One workgroup per problem; a problem
-// without such columns returns at once.
+// 1e-8 sigma_0 * O(eps) per column.  Such columns form the tail [j0, k) (sigma
+// descends); their projections off the head [0, j0) run in parallel
+// (svd_complete_head_kernel), the serial part is the tail alone
+// (svd_complete_kernel, one workgroup per problem).  A problem without such
+// columns returns at once.
 constexpr int CP_THREADS = 256;
 constexpr int CP_MAX_K = 4096;
+constexpr int CP_HEAD_BLOCKS = 32;  // workgroups per problem projecting tail columns off the head
 
 template <typename S2>
 struct Fac {
@@ -1386,15 +1390,14 @@ __device__ __forceinline__ double cp_block_sum(double v, double* red) {
   return t;
 }
 
+// The factor to complete and its extent: U (m rows) or, when the Jacobi ran
+// on A^H, the rows of V^H (n entries each, read as columns of F).
 template <typename S2>
-__global__ __launch_bounds__(CP_THREADS) void svd_complete_kernel(SvdBatch Bt) {
-  const SvdProblem& P = Bt.arr ? Bt.arr[blockIdx.y] : Bt.one;
+__device__ __forceinline__ bool cp_setup(const SvdProblem& P, Fac<S2>& F, int64_t& L, int& klim, int& j0) {
   const int kmax = P.ncols;
-  if (kmax > CP_MAX_K) return;
-  Fac<S2> F;
-  int64_t L;
+  if (kmax > CP_MAX_K) return false;
   if (!P.transposed) {
-    if (!P.U) return;
+    if (!P.U) return false;
     F.p = static_cast<S2*>(P.U);
     F.ld = P.ldu;
     F.adj = false;
@@ -1405,77 +1408,131 @@ __global__ __launch_bounds__(CP_THREADS) void svd_complete_kernel(SvdBatch Bt) {
     F.adj = true;
     L = P.n;
   }
-  const double tau = P.sigma[0] * 1e-8;
   // only the kept columns (the truncation's k, written by the sort kernel)
   // are completed: the caller drops the others (a TEBD bond matrix of rank
   // 256 in 512 columns would otherwise orthogonalise 256 discarded ones)
-  const int klim = P.k_dev ? (int)min((int64_t)kmax, *P.k_dev) : kmax;
+  klim = P.k_dev ? (int)min((int64_t)kmax, *P.k_dev) : kmax;
+  // sigma descends: the columns to complete are the tail [j0, klim)
+  const double tau = P.sigma[0] * 1e-8;
+  int lo = 0, hi = klim;
+  while (lo < hi) {
+    const int mid = (lo + hi) / 2;
+    if (P.sigma[mid] > tau)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  j0 = lo;
+  return j0 < klim;
+}
+
+// Two passes (classical Gram-Schmidt with reorthogonalisation) of column j
+// against columns [lo, hi) of F.
+template <typename S2>
+__device__ __forceinline__ void cp_project(const Fac<S2>& F, int64_t L, int lo, int hi, int j, double* cr, double* ci) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int pass = 0; pass < 2; ++pass) {
+    // c = F[:, lo:hi]^H v
+    if (!F.adj) {
+      // columns contiguous: one wave per column, lanes over rows
+      for (int i = lo + wave; i < hi; i += CP_THREADS / 64) {
+        double sr = 0.0, si = 0.0;
+        for (int64_t r = lane; r < L; r += 64) {
+          const double2 a = F.get(r, i), v = F.get(r, j);
+          sr += a.x * v.x + a.y * v.y;
+          si += a.x * v.y - a.y * v.x;
+        }
+        sr = cp_wave_sum(sr);
+        si = cp_wave_sum(si);
+        if (lane == 0) {
+          cr[i] = sr;
+          ci[i] = si;
+        }
+      }
+    } else {
+      // rows of V^H: F's column index is the contiguous one, so one
+      // thread per column i walks the rows (v[r] is the same for all)
+      for (int i = lo + tid; i < hi; i += CP_THREADS) {
+        double sr = 0.0, si = 0.0;
+        for (int64_t r = 0; r < L; ++r) {
+          const double2 a = F.get(r, i), v = F.get(r, j);
+          sr += a.x * v.x + a.y * v.y;
+          si += a.x * v.y - a.y * v.x;
+        }
+        cr[i] = sr;
+        ci[i] = si;
+      }
+    }
+    __syncthreads();
+    // v -= F[:, lo:hi] c (one thread per row; with V^H's rows each thread
+    // walks contiguous memory)
+    for (int64_t r = tid; r < L; r += CP_THREADS) {
+      double2 v = F.get(r, j);
+      for (int i = lo; i < hi; ++i) {
+        const double2 a = F.get(r, i);
+        v.x -= a.x * cr[i] - a.y * ci[i];
+        v.y -= a.x * ci[i] + a.y * cr[i];
+      }
+      F.put(r, j, v.x, v.y);
+    }
+    __syncthreads();
+  }
+}
+
+// Step 1, in parallel over the tail columns (grid.x workgroups per problem):
+// each column's length before any projection goes to n0 (the Jacobi work
+// matrix X, free once V^H and U are written), then the column is projected
+// off the head columns [0, j0), which are orthonormal to working accuracy.
+template <typename S2>
+__global__ __launch_bounds__(CP_THREADS) void svd_complete_head_kernel(SvdBatch Bt) {
+  const SvdProblem& P = Bt.arr ? Bt.arr[blockIdx.y] : Bt.one;
+  Fac<S2> F;
+  int64_t L;
+  int klim, j0;
+  if (!cp_setup(P, F, L, klim, j0)) return;
   __shared__ double cr[CP_MAX_K], ci[CP_MAX_K];
   __shared__ double red[CP_THREADS / 64];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  int64_t next_unit = 0;
-  for (int j = 0; j < klim; ++j) {
-    if (P.sigma[j] > tau) continue;  // uniform: sigma is read by every thread
+  double* n0s = P.X;
+  for (int j = j0 + blockIdx.x; j < klim; j += gridDim.x) {
     double n0 = 0.0;
-    for (int64_t r = tid; r < L; r += CP_THREADS) {
+    for (int64_t r = threadIdx.x; r < L; r += CP_THREADS) {
       const double2 v = F.get(r, j);
       n0 += v.x * v.x + v.y * v.y;
     }
     n0 = sqrt(cp_block_sum(n0, red));
+    if (threadIdx.x == 0) n0s[j] = n0;
+    if (n0 > 0.0 && j0 > 0) cp_project(F, L, 0, j0, j, cr, ci);
+    __syncthreads();
+  }
+}
+
+// Step 2, one workgroup per problem: each tail column in order is projected
+// off the tail columns before it and normalised; if too little of it
+// survives (below 1e-3 of its length before step 1), the unit vector e_t
+// (t = j, j+1, ... mod L) is projected off every column before it instead.
+template <typename S2>
+__global__ __launch_bounds__(CP_THREADS) void svd_complete_kernel(SvdBatch Bt) {
+  const SvdProblem& P = Bt.arr ? Bt.arr[blockIdx.y] : Bt.one;
+  Fac<S2> F;
+  int64_t L;
+  int klim, j0;
+  if (!cp_setup(P, F, L, klim, j0)) return;
+  __shared__ double cr[CP_MAX_K], ci[CP_MAX_K];
+  __shared__ double red[CP_THREADS / 64];
+  const int tid = threadIdx.x;
+  const double* n0s = P.X;
+  int64_t next_unit = 0;
+  for (int j = j0; j < klim; ++j) {
+    const double n0 = n0s[j];
     bool unit = !(n0 > 0.0);
+    if (!unit) cp_project(F, L, j0, j, j, cr, ci);
     for (int attempt = 0; attempt < 16; ++attempt) {
       if (unit) {
         // e_t, t taken in turn from the unit vectors not yet tried
         const int64_t t = (j + next_unit++) % L;
         for (int64_t r = tid; r < L; r += CP_THREADS) F.put(r, j, r == t ? 1.0 : 0.0, 0.0);
-        n0 = 1.0;
         __syncthreads();
-      }
-      for (int pass = 0; pass < 2; ++pass) {
-        // c = F[:, :j]^H v
-        if (!F.adj) {
-          // columns contiguous: one wave per column, lanes over rows
-          for (int i = wave; i < j; i += CP_THREADS / 64) {
-            double sr = 0.0, si = 0.0;
-            for (int64_t r = lane; r < L; r += 64) {
-              const double2 a = F.get(r, i), v = F.get(r, j);
-              sr += a.x * v.x + a.y * v.y;
-              si += a.x * v.y - a.y * v.x;
-            }
-            sr = cp_wave_sum(sr);
-            si = cp_wave_sum(si);
-            if (lane == 0) {
-              cr[i] = sr;
-              ci[i] = si;
-            }
-          }
-        } else {
-          // rows of V^H: F's column index is the contiguous one, so one
-          // thread per column i walks the rows (v[r] is the same for all)
-          for (int i = tid; i < j; i += CP_THREADS) {
-            double sr = 0.0, si = 0.0;
-            for (int64_t r = 0; r < L; ++r) {
-              const double2 a = F.get(r, i), v = F.get(r, j);
-              sr += a.x * v.x + a.y * v.y;
-              si += a.x * v.y - a.y * v.x;
-            }
-            cr[i] = sr;
-            ci[i] = si;
-          }
-        }
-        __syncthreads();
-        // v -= F[:, :j] c (one thread per row; with V^H's rows each thread
-        // walks contiguous memory)
-        for (int64_t r = tid; r < L; r += CP_THREADS) {
-          double2 v = F.get(r, j);
-          for (int i = 0; i < j; ++i) {
-            const double2 a = F.get(r, i);
-            v.x -= a.x * cr[i] - a.y * ci[i];
-            v.y -= a.x * ci[i] + a.y * cr[i];
-          }
-          F.put(r, j, v.x, v.y);
-        }
-        __syncthreads();
+        cp_project(F, L, 0, j, j, cr, ci);
       }
       double nn = 0.0;
       for (int64_t r = tid; r < L; r += CP_THREADS) {
@@ -1483,7 +1540,7 @@ __global__ __launch_bounds__(CP_THREADS) void svd_complete_kernel(SvdBatch Bt) {
         nn += v.x * v.x + v.y * v.y;
       }
       nn = sqrt(cp_block_sum(nn, red));
-      if (nn > 1e-3 * n0) {
+      if (nn > 1e-3 * (unit ? 1.0 : n0)) {
         const double inv = 1.0 / nn;
         for (int64_t r = tid; r < L; r += CP_THREADS) {
           const double2 v = F.get(r, j);
@@ -1568,10 +1625,17 @@ int launch_svd_finalize(const SvdBatch& b, int64_t nprob, int max_ncp, int64_t m
   else
     hipLaunchKernelGGL(svd_write_kernel<float2>, dim3(gx, (unsigned)nprob), dim3(256), 0, s, b);
   TENET_HIP_CHECK(hipGetLastError());
-  if (dtype == TENET_C128)
+  if (dtype == TENET_C128) {
+    hipLaunchKernelGGL(svd_complete_head_kernel<double2>, dim3(CP_HEAD_BLOCKS, (unsigned)nprob), dim3(CP_THREADS), 0,
+                       s, b);
+    TENET_HIP_CHECK(hipGetLastError());
     hipLaunchKernelGGL(svd_complete_kernel<double2>, dim3(1, (unsigned)nprob), dim3(CP_THREADS), 0, s, b);
-  else
+  } else {
+    hipLaunchKernelGGL(svd_complete_head_kernel<float2>, dim3(CP_HEAD_BLOCKS, (unsigned)nprob), dim3(CP_THREADS), 0,
+                       s, b);
+    TENET_HIP_CHECK(hipGetLastError());
     hipLaunchKernelGGL(svd_complete_kernel<float2>, dim3(1, (unsigned)nprob), dim3(CP_THREADS), 0, s, b);
+  }
   TENET_HIP_CHECK(hipGetLastError());
   return TENET_OK;
 }

@@ -1845,8 +1845,10 @@ static int grouped_plan_build(tenet_ctx ctx, int dtype, int prec, int64_t count,
   TENET_HIP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
   const int occ = grouped_occupancy(dtype);
   g->grid = (int64_t)ncu * std::max(1, occ);
-  // K chunking: no unit above ~1/8 of one workgroup's fair share
-  const double unit_cap = std::max(total_cost / ((double)g->grid * 8.0), 16.0 * c[0]);
+  // K chunking: no unit above ~1/4 of one workgroup's fair share (with four
+  // or more tiles per workgroup the LPT queue balances them whole; chunks
+  // cost partial sums through memory)
+  const double unit_cap = std::max(total_cost / ((double)g->grid * 4.0), 16.0 * c[0]);
   int64_t part_elems = 0, ctr_elems = 0, nunits = 0;
   for (int64_t i = 0; i < count; ++i) {
     GroupedProblem& q = probs[(size_t)i];
