@@ -126,9 +126,11 @@ struct QrBatchProb {
   double* tau;
   double* T;
   double* Yp;
+  double* TB;  // T of the current 64-column block reflector (qr_batched_tb_bytes)
   int32_t m, n;
 };
 size_t qr_batched_yp_bytes(int64_t m, int64_t max_n);
+size_t qr_batched_tb_bytes();
 size_t qr_batched_t_bytes(int64_t m, int64_t n);
 int launch_qr_batched(const QrBatchProb* probs, int64_t nprob, int max_m, int max_n, int max_k, hipStream_t s);
 bool qr_blocked_fits(int64_t m);

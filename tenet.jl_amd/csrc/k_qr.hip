@@ -1066,31 +1066,362 @@ __device__ __forceinline__ void qa_xz_body(const double* __restrict__ W, const d
 // Batched trailing update after panel p: blockIdx.z is the problem; the
 // problem's trailing columns [j0 + jb, n) of W itself, rows [j0, m).  Ypart
 // strides by the grid's (largest) column-block count.
-__global__ __launch_bounds__(QR_THREADS) void qr_apply_y_batched_kernel(const QrBatchProb* __restrict__ probs, int p) {
+// (columns end at min(n, cap): the panels of a 64-column block update only
+// the rest of their block; the block reflector then updates the columns
+// right of the block at once)
+__global__ __launch_bounds__(QR_THREADS) void qr_apply_y_batched_kernel(const QrBatchProb* __restrict__ probs, int p,
+                                                                         int cap) {
   const QrBatchProb& q = probs[blockIdx.z];
   const int k = min(q.m, q.n), j0 = p * QB;
   if (j0 >= k) return;
   const int jb = min(QB, k - j0);
-  qa_y_body(q.W, q.W, q.Yp, q.m, q.m, j0, jb, j0 + jb, q.n, blockIdx.x, blockIdx.y, gridDim.x);
+  qa_y_body(q.W, q.W, q.Yp, q.m, q.m, j0, jb, j0 + jb, min(q.n, cap), blockIdx.x, blockIdx.y, gridDim.x);
 }
 
-__global__ __launch_bounds__(QR_THREADS) void qr_apply_z_batched_kernel(const QrBatchProb* __restrict__ probs, int p) {
+__global__ __launch_bounds__(QR_THREADS) void qr_apply_z_batched_kernel(const QrBatchProb* __restrict__ probs, int p,
+                                                                         int cap) {
   const QrBatchProb& q = probs[blockIdx.z];
   const int k = min(q.m, q.n), j0 = p * QB;
   if (j0 >= k) return;
   const int jb = min(QB, k - j0);
-  if (j0 + jb + (int)blockIdx.x * QB >= q.n) return;  // no such column block
+  if (j0 + jb + (int)blockIdx.x * QB >= min(q.n, cap)) return;  // no such column block
   const int nch = (q.m - j0 + QA_RC - 1) / QA_RC;
   qa_z_body(q.T + (size_t)p * QB * QB * 2, q.Yp, jb, nch, blockIdx.x, gridDim.x);
 }
 
-__global__ __launch_bounds__(QR_THREADS) void qr_apply_x_batched_kernel(const QrBatchProb* __restrict__ probs, int p) {
+__global__ __launch_bounds__(QR_THREADS) void qr_apply_x_batched_kernel(const QrBatchProb* __restrict__ probs, int p,
+                                                                         int cap) {
   const QrBatchProb& q = probs[blockIdx.z];
   const int k = min(q.m, q.n), j0 = p * QB;
   if (j0 >= k) return;
   const int jb = min(QB, k - j0);
-  qa_xz_body(q.W, q.Yp + (size_t)blockIdx.x * QB * QB * 2, q.W, q.m, q.m, j0, jb, j0 + jb, q.n, blockIdx.x,
-             blockIdx.y);
+  qa_xz_body(q.W, q.Yp + (size_t)blockIdx.x * QB * QB * 2, q.W, q.m, q.m, j0, jb, j0 + jb, min(q.n, cap),
+             blockIdx.x, blockIdx.y);
+}
+
+// ---- block reflectors of 64 columns (four panels) ----------------------------
+// After the four panels of block [b0, b0 + kb) are factored (each updating
+// only the rest of the block), the columns right of the block get the whole
+// block reflector H_b0 ... H_{b0+kb-1} = I - V T V^H at once: one pass over
+// the trailing matrix per 64 columns instead of per 16.
+constexpr int QBW = 64;  // block width
+constexpr int WRC = 64;  // rows per chunk of the block update
+constexpr int WRH = 32;  // rows per LDS pass
+
+// V(r, l) of the block starting at column b0: 0 above row b0 + l, 1 on it,
+// the stored reflector below
+__device__ __forceinline__ double2 blk_v(const double2* __restrict__ W, int m, int b0, int r, int l) {
+  const int c = b0 + l;
+  if (r < c) return make_double2(0.0, 0.0);
+  if (r == c) return make_double2(1.0, 0.0);
+  return W[(size_t)c * m + r];
+}
+
+// Cross products of the block's reflectors, S = V^H V for panel(l) < panel(j),
+// summed per 64-row chunk (blockIdx.x) into partials Sp[chunk] (64 x 64,
+// the trailing update's partial buffer, free at this point); each thread
+// sums a 4 x 4 block.
+__global__ __launch_bounds__(QR_THREADS) void qr_sblock_batched_kernel(const QrBatchProb* __restrict__ probs, int b0) {
+  __shared__ double2 vs[WRH][QBW + 1];
+  const QrBatchProb& q = probs[blockIdx.y];
+  const int k = min(q.m, q.n);
+  if (b0 >= k) return;
+  const int kb = min(QBW, k - b0);
+  if (b0 + kb >= q.n) return;  // no columns right of the block
+  const int rb = b0 + (int)blockIdx.x * WRC;
+  if (rb >= q.m) return;
+  const double2* W = reinterpret_cast<const double2*>(q.W);
+  const int tid = threadIdx.x;
+  const int lb = tid / 16, jb = tid % 16;
+  const bool mine = (lb / 4) < (jb / 4) && 4 * jb < kb;
+  double sr[4][4], si[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      sr[a][c] = 0.0;
+      si[a][c] = 0.0;
+    }
+  for (int h = 0; h < WRC; h += WRH) {
+    const int r0 = rb + h;
+    __syncthreads();
+    for (int e = tid; e < WRH * QBW; e += QR_THREADS) {
+      const int rr = e % WRH, l = e / WRH, r = r0 + rr;
+      vs[rr][l] = (r < q.m && l < kb) ? blk_v(W, q.m, b0, r, l) : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    if (mine) {
+      for (int rr = 0; rr < WRH; ++rr) {
+        double2 va[4], vb[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+          va[a] = vs[rr][4 * lb + a];
+          vb[a] = vs[rr][4 * jb + a];
+        }
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+          for (int c = 0; c < 4; ++c) {  // conj(v_l) v_j
+            sr[a][c] += va[a].x * vb[c].x + va[a].y * vb[c].y;
+            si[a][c] += va[a].x * vb[c].y - va[a].y * vb[c].x;
+          }
+      }
+    }
+  }
+  if (mine) {
+    double* out = q.Yp + (size_t)blockIdx.x * QBW * QBW * 2;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int f = (4 * lb + a) * QBW + 4 * jb + c;
+        out[2 * f] = sr[a][c];
+        out[2 * f + 1] = si[a][c];
+      }
+  }
+}
+
+// T of the block, assembled from the panels' own 16 x 16 factors: for
+// H_A H_B with V = [V_A V_B], T = [[T_A, -T_A (V_A^H V_B) T_B], [0, T_B]]
+// (the rule zlarft's recursion satisfies), applied for B = panel 1, 2, 3 of
+// the block in turn, with V_A^H V_B the chunk partials of
+// qr_sblock_batched_kernel summed in chunk order; one workgroup per problem.
+// TB (column-major, 64 x 64) <- T.
+__global__ __launch_bounds__(QR_THREADS) void qr_tblock_batched_kernel(const QrBatchProb* __restrict__ probs, int b0) {
+  __shared__ double2 S[QBW][QBW + 1];  // S[l][j] = v_l^H v_j, panel(l) < panel(j)
+  __shared__ double2 T[QBW][QBW + 1];  // T[i][j]
+  __shared__ double2 Ys[QBW - QB][QB + 1];
+  const QrBatchProb& q = probs[blockIdx.x];
+  const int k = min(q.m, q.n);
+  if (b0 >= k) return;
+  const int kb = min(QBW, k - b0);
+  if (b0 + kb >= q.n) return;  // no columns right of the block
+  const int tid = threadIdx.x;
+  const int nch = (q.m - b0 + WRC - 1) / WRC;
+  const int p0 = b0 / QB;
+  for (int e = tid; e < QBW * QBW; e += QR_THREADS) {
+    const int l = e / QBW, j = e % QBW;
+    double2 sv = make_double2(0.0, 0.0);
+    if (l / QB < j / QB && j < kb)
+      for (int ch = 0; ch < nch; ++ch) {
+        const double* in = q.Yp + (size_t)ch * QBW * QBW * 2;
+        sv.x += in[2 * e];
+        sv.y += in[2 * e + 1];
+      }
+    S[l][j] = sv;
+    // the diagonal blocks: each panel's T (column-major 16 x 16), zero elsewhere
+    const int i2 = e % QBW, j2 = e / QBW;
+    double2 v = make_double2(0.0, 0.0);
+    if (i2 / QB == j2 / QB && j2 < kb) {
+      const double* Tp = q.T + (size_t)(p0 + j2 / QB) * QB * QB * 2;
+      const int f = (j2 % QB) * QB + (i2 % QB);
+      v = make_double2(Tp[2 * f], Tp[2 * f + 1]);
+    }
+    T[i2][j2] = v;
+  }
+  __syncthreads();
+  for (int bq = 1; bq * QB < kb; ++bq) {
+    const int na = bq * QB, nbw = min(QB, kb - na);  // A = [0, na), B = [na, na + nbw)
+    // Y = T_A S(A, B)
+    for (int e = tid; e < na * QB; e += QR_THREADS) {
+      const int i = e % na, c = e / na;
+      double yr = 0.0, yi = 0.0;
+      if (c < nbw)
+        for (int l = i; l < na; ++l) {
+          const double2 t = T[i][l], x = S[l][na + c];
+          yr += t.x * x.x - t.y * x.y;
+          yi += t.x * x.y + t.y * x.x;
+        }
+      Ys[i][c] = make_double2(yr, yi);
+    }
+    __syncthreads();
+    // T(A, B) = -Y T_B
+    for (int e = tid; e < na * QB; e += QR_THREADS) {
+      const int i = e % na, c = e / na;
+      if (c < nbw) {
+        double tr = 0.0, ti = 0.0;
+        for (int l = 0; l <= c; ++l) {  // T_B upper triangular
+          const double2 y = Ys[i][l], t = T[na + l][na + c];
+          tr += y.x * t.x - y.y * t.y;
+          ti += y.x * t.y + y.y * t.x;
+        }
+        T[i][na + c] = make_double2(-tr, -ti);
+      }
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < QBW * QBW; e += QR_THREADS) {
+    const int i = e % QBW, j = e / QBW;
+    const double2 v = (i < kb && j < kb) ? T[i][j] : make_double2(0.0, 0.0);
+    q.TB[2 * e] = v.x;
+    q.TB[2 * e + 1] = v.y;
+  }
+}
+
+// Ypart[chunk][blk] (64 x WCB) = V^H C over the chunk's rows [b0 + 64 chunk,
+// +64), C = the WCB columns of block blk right of the reflector block
+constexpr int WCB = 16;  // columns per workgroup of the block update
+
+__global__ __launch_bounds__(QR_THREADS) void qr_wy_batched_kernel(const QrBatchProb* __restrict__ probs, int b0) {
+  __shared__ double2 vs[WRH][QBW + 1];
+  __shared__ double2 cs[WRH][WCB + 1];
+  const QrBatchProb& q = probs[blockIdx.z];
+  const int k = min(q.m, q.n);
+  if (b0 >= k) return;
+  const int kb = min(QBW, k - b0);
+  const int cb0 = b0 + kb + (int)blockIdx.x * WCB;
+  if (cb0 >= q.n) return;
+  const int nc = min(WCB, q.n - cb0);
+  const int rb = b0 + (int)blockIdx.y * WRC;
+  if (rb >= q.m) return;
+  const double2* W = reinterpret_cast<const double2*>(q.W);
+  const int tid = threadIdx.x;
+  constexpr int CPT = WCB * QBW / QR_THREADS;  // 8 outputs per thread
+  const int l = tid % QBW, cc0 = CPT * (tid / QBW);
+  double yr[CPT], yi[CPT];
+#pragma unroll
+  for (int j = 0; j < CPT; ++j) {
+    yr[j] = 0.0;
+    yi[j] = 0.0;
+  }
+  for (int h = 0; h < WRC; h += WRH) {
+    const int r0 = rb + h;
+    __syncthreads();
+    for (int e = tid; e < WRH * QBW; e += QR_THREADS) {
+      const int rr = e % WRH, ll = e / WRH, r = r0 + rr;
+      vs[rr][ll] = (r < q.m && ll < kb) ? blk_v(W, q.m, b0, r, ll) : make_double2(0.0, 0.0);
+    }
+    for (int e = tid; e < WRH * WCB; e += QR_THREADS) {
+      const int rr = e % WRH, cc = e / WRH, r = r0 + rr;
+      cs[rr][cc] = (r < q.m && cc < nc) ? W[(size_t)(cb0 + cc) * q.m + r] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    for (int rr = 0; rr < WRH; ++rr) {
+      const double2 v = vs[rr][l];
+#pragma unroll
+      for (int j = 0; j < CPT; ++j) {
+        const double2 c = cs[rr][cc0 + j];
+        yr[j] += v.x * c.x + v.y * c.y;  // conj(v) c
+        yi[j] += v.x * c.y - v.y * c.x;
+      }
+    }
+  }
+  double* out = q.Yp + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * QBW * WCB * 2;
+#pragma unroll
+  for (int j = 0; j < CPT; ++j) {
+    out[2 * (l * WCB + cc0 + j)] = yr[j];
+    out[2 * (l * WCB + cc0 + j) + 1] = yi[j];
+  }
+}
+
+// Z = T^H sum_chunks Ypart (64 x WCB), over the block's chunk-0 partial
+__global__ __launch_bounds__(QR_THREADS) void qr_wz_batched_kernel(const QrBatchProb* __restrict__ probs, int b0) {
+  __shared__ double2 ys[QBW][WCB + 1];
+  __shared__ double2 ts[QBW][QBW + 1];  // ts[i][j] = T(i, j)
+  const QrBatchProb& q = probs[blockIdx.z];
+  const int k = min(q.m, q.n);
+  if (b0 >= k) return;
+  const int kb = min(QBW, k - b0);
+  const int cb0 = b0 + kb + (int)blockIdx.x * WCB;
+  if (cb0 >= q.n) return;
+  const int nch = (q.m - b0 + WRC - 1) / WRC;
+  const int tid = threadIdx.x;
+  constexpr int CPT = WCB * QBW / QR_THREADS;
+  const int a = tid % QBW, cc0 = CPT * (tid / QBW);
+  const int nblk = gridDim.x;
+  for (int j = 0; j < CPT; ++j) {
+    double sr = 0.0, si = 0.0;
+    for (int ch = 0; ch < nch; ++ch) {
+      const double* in = q.Yp + ((size_t)ch * nblk + blockIdx.x) * QBW * WCB * 2;
+      sr += in[2 * (a * WCB + cc0 + j)];
+      si += in[2 * (a * WCB + cc0 + j) + 1];
+    }
+    ys[a][cc0 + j] = make_double2(sr, si);
+  }
+  for (int e = tid; e < QBW * QBW; e += QR_THREADS) ts[e % QBW][e / QBW] = make_double2(q.TB[2 * e], q.TB[2 * e + 1]);
+  __syncthreads();
+  double zr[CPT], zi[CPT];
+#pragma unroll
+  for (int j = 0; j < CPT; ++j) {
+    zr[j] = 0.0;
+    zi[j] = 0.0;
+  }
+  if (a < kb) {
+    for (int b = 0; b <= a; ++b) {  // conj(T(b, a)), T upper triangular
+      const double2 t = ts[b][a];
+#pragma unroll
+      for (int j = 0; j < CPT; ++j) {
+        const double2 y = ys[b][cc0 + j];
+        zr[j] += t.x * y.x + t.y * y.y;
+        zi[j] += t.x * y.y - t.y * y.x;
+      }
+    }
+  }
+  double* out = q.Yp + (size_t)blockIdx.x * QBW * WCB * 2;
+#pragma unroll
+  for (int j = 0; j < CPT; ++j) {
+    out[2 * (a * WCB + cc0 + j)] = zr[j];
+    out[2 * (a * WCB + cc0 + j) + 1] = zi[j];
+  }
+}
+
+// C[chunk rows, block blk] -= V Z
+__global__ __launch_bounds__(QR_THREADS) void qr_wx_batched_kernel(const QrBatchProb* __restrict__ probs, int b0) {
+  __shared__ double2 vs[WRH][QBW + 1];
+  __shared__ double2 zs[QBW][WCB + 1];
+  const QrBatchProb& q = probs[blockIdx.z];
+  const int k = min(q.m, q.n);
+  if (b0 >= k) return;
+  const int kb = min(QBW, k - b0);
+  const int cb0 = b0 + kb + (int)blockIdx.x * WCB;
+  if (cb0 >= q.n) return;
+  const int nc = min(WCB, q.n - cb0);
+  const int rb = b0 + (int)blockIdx.y * WRC;
+  if (rb >= q.m) return;
+  double2* W = reinterpret_cast<double2*>(q.W);
+  const int tid = threadIdx.x;
+  {
+    const double* z = q.Yp + (size_t)blockIdx.x * QBW * WCB * 2;
+    for (int e = tid; e < QBW * WCB; e += QR_THREADS) zs[e / WCB][e % WCB] = make_double2(z[2 * e], z[2 * e + 1]);
+  }
+  constexpr int CPT = WCB * WRH / QR_THREADS;  // 4 columns per thread
+  const int rr = tid % WRH, cc0 = CPT * (tid / WRH);
+  for (int h = 0; h < WRC; h += WRH) {
+    const int r0 = rb + h;
+    __syncthreads();
+    for (int e = tid; e < WRH * QBW; e += QR_THREADS) {
+      const int r2 = e % WRH, ll = e / WRH, r = r0 + r2;
+      vs[r2][ll] = (r < q.m && ll < kb) ? blk_v(W, q.m, b0, r, ll) : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    const int r = r0 + rr;
+    if (r < q.m) {
+      double sr[CPT], si[CPT];
+#pragma unroll
+      for (int j = 0; j < CPT; ++j) {
+        sr[j] = 0.0;
+        si[j] = 0.0;
+      }
+      for (int l = 0; l < kb; ++l) {
+        const double2 v = vs[rr][l];
+#pragma unroll
+        for (int j = 0; j < CPT; ++j) {
+          const double2 z = zs[l][cc0 + j];
+          sr[j] += v.x * z.x - v.y * z.y;
+          si[j] += v.x * z.y + v.y * z.x;
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < CPT; ++j) {
+        if (cc0 + j < nc) {
+          double2 x = W[(size_t)(cb0 + cc0 + j) * q.m + r];
+          x.x -= sr[j];
+          x.y -= si[j];
+          W[(size_t)(cb0 + cc0 + j) * q.m + r] = x;
+        }
+      }
+    }
+  }
 }
 
 // M <- (I - V T' V^H) M (adj = 1: T' = T^H, forward application during the
@@ -1114,8 +1445,12 @@ static int qr_apply_chunked(const double* W, const double* T, double* M, int m, 
 // per panel one launch of the panel kernel over the problems and one pair of
 // trailing-update launches.  Every m <= 512 (the register panel).
 size_t qr_batched_yp_bytes(int64_t m, int64_t max_n) {
-  return (size_t)((m + QA_RC - 1) / QA_RC) * (size_t)((max_n + QB - 1) / QB) * QB * QB * 16;
+  // the block update's partials (64 x WCB per chunk and column block; the
+  // 64 x 64 cross products per chunk) and the panels' (16 x 16)
+  return (size_t)((m + WRC - 1) / WRC) * (size_t)((max_n + WCB - 1) / WCB + 2) * QBW * WCB * 16 +
+         (size_t)((m + QA_RC - 1) / QA_RC) * (size_t)((max_n + QB - 1) / QB) * QB * QB * 16;
 }
+size_t qr_batched_tb_bytes() { return (size_t)QBW * QBW * 16; }
 size_t qr_batched_t_bytes(int64_t m, int64_t n) { return (size_t)((std::min(m, n) + QB - 1) / QB) * QB * QB * 16; }
 
 int launch_qr_batched(const QrBatchProb* probs, int64_t nprob, int max_m, int max_n, int max_k, hipStream_t s) {
@@ -1124,29 +1459,46 @@ int launch_qr_batched(const QrBatchProb* probs, int64_t nprob, int max_m, int ma
     set_error("batched QR: rows above 512 or more than 65535 problems");
     return TENET_ERR_UNSUPPORTED;
   }
-  const int np = (max_k + QB - 1) / QB;
-  for (int p = 0; p < np; ++p) {
-    const int j0 = p * QB;
-    const int mr = max_m - j0;
-    const dim3 g1((unsigned)nprob);
-    if (mr <= 64)
-      hipLaunchKernelGGL(qr_panel_v4_batched_kernel<1>, g1, dim3(64 * QB), 0, s, probs, p);
-    else if (mr <= 128)
-      hipLaunchKernelGGL(qr_panel_v4_batched_kernel<2>, g1, dim3(64 * QB), 0, s, probs, p);
-    else if (mr <= 256)
-      hipLaunchKernelGGL(qr_panel_v4_batched_kernel<4>, g1, dim3(64 * QB), 0, s, probs, p);
-    else
-      hipLaunchKernelGGL(qr_panel_v4_batched_kernel<8>, g1, dim3(64 * QB), 0, s, probs, p);
+  for (int b0 = 0; b0 < max_k; b0 += QBW) {
+    const int cap = b0 + QBW;
+    for (int j0 = b0; j0 < std::min(cap, max_k); j0 += QB) {
+      const int p = j0 / QB;
+      const int mr = max_m - j0;
+      const dim3 g1((unsigned)nprob);
+      if (mr <= 64)
+        hipLaunchKernelGGL(qr_panel_v4_batched_kernel<1>, g1, dim3(64 * QB), 0, s, probs, p);
+      else if (mr <= 128)
+        hipLaunchKernelGGL(qr_panel_v4_batched_kernel<2>, g1, dim3(64 * QB), 0, s, probs, p);
+      else if (mr <= 256)
+        hipLaunchKernelGGL(qr_panel_v4_batched_kernel<4>, g1, dim3(64 * QB), 0, s, probs, p);
+      else
+        hipLaunchKernelGGL(qr_panel_v4_batched_kernel<8>, g1, dim3(64 * QB), 0, s, probs, p);
+      TENET_HIP_CHECK(hipGetLastError());
+      // the rest of the block: a problem's columns start at j0 + jb >= j0 + 1,
+      // so blocks over [j0 + 1, min(max_n, cap)) cover every problem's
+      const int ce = std::min(max_n, cap);
+      if (ce - j0 <= 1) continue;
+      const dim3 g2((unsigned)((ce - j0 - 1 + QB - 1) / QB), (unsigned)((mr + QA_RC - 1) / QA_RC), (unsigned)nprob);
+      hipLaunchKernelGGL(qr_apply_y_batched_kernel, g2, dim3(QR_THREADS), 0, s, probs, p, cap);
+      TENET_HIP_CHECK(hipGetLastError());
+      hipLaunchKernelGGL(qr_apply_z_batched_kernel, dim3(g2.x, 1, g2.z), dim3(QR_THREADS), 0, s, probs, p, cap);
+      TENET_HIP_CHECK(hipGetLastError());
+      hipLaunchKernelGGL(qr_apply_x_batched_kernel, g2, dim3(QR_THREADS), 0, s, probs, p, cap);
+      TENET_HIP_CHECK(hipGetLastError());
+    }
+    // the block reflector on the columns right of the block (a problem's
+    // start at b0 + kb >= b0 + 1)
+    if (max_n - b0 <= 1) continue;
+    const dim3 g3((unsigned)((max_n - b0 - 1 + WCB - 1) / WCB), (unsigned)((max_m - b0 + WRC - 1) / WRC), (unsigned)nprob);
+    hipLaunchKernelGGL(qr_sblock_batched_kernel, dim3(g3.y, g3.z), dim3(QR_THREADS), 0, s, probs, b0);
     TENET_HIP_CHECK(hipGetLastError());
-    // a problem's trailing columns start at j0 + jb >= j0 + 1: blocks over
-    // [j0 + 1, max_n) cover every problem's (the kernels return past its n)
-    if (max_n - j0 <= 1) continue;
-    const dim3 g2((unsigned)((max_n - j0 - 1 + QB - 1) / QB), (unsigned)((mr + QA_RC - 1) / QA_RC), (unsigned)nprob);
-    hipLaunchKernelGGL(qr_apply_y_batched_kernel, g2, dim3(QR_THREADS), 0, s, probs, p);
+    hipLaunchKernelGGL(qr_tblock_batched_kernel, dim3((unsigned)nprob), dim3(QR_THREADS), 0, s, probs, b0);
     TENET_HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(qr_apply_z_batched_kernel, dim3(g2.x, 1, g2.z), dim3(QR_THREADS), 0, s, probs, p);
+    hipLaunchKernelGGL(qr_wy_batched_kernel, g3, dim3(QR_THREADS), 0, s, probs, b0);
     TENET_HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(qr_apply_x_batched_kernel, g2, dim3(QR_THREADS), 0, s, probs, p);
+    hipLaunchKernelGGL(qr_wz_batched_kernel, dim3(g3.x, 1, g3.z), dim3(QR_THREADS), 0, s, probs, b0);
+    TENET_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(qr_wx_batched_kernel, g3, dim3(QR_THREADS), 0, s, probs, b0);
     TENET_HIP_CHECK(hipGetLastError());
   }
   return TENET_OK;

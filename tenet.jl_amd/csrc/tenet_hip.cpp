@@ -2174,6 +2174,7 @@ static int gsvd_build(tenet_ctx ctx, int dtype, int64_t count, const int64_t* ms
     if (pre[(size_t)i])
       total += align_up((size_t)m * n * 16, 256) + align_up((size_t)n * 16, 256) +
                align_up(qr_batched_t_bytes(m, n), 256) + align_up(qr_batched_yp_bytes(m, g->pre_max_n), 256) +
+               align_up(qr_batched_tb_bytes(), 256) +
                align_up((size_t)n * 4, 256) + align_up(sizeof(QrBatchProb), 256);
     g->max_rounds = std::max(g->max_rounds, sh.nblk - 1);
     g->max_pairs = std::max(g->max_pairs, sh.npairs);
@@ -2261,6 +2262,7 @@ static int gsvd_build(tenet_ctx ctx, int dtype, int64_t count, const int64_t* ms
       q.tau = (double*)bm.take((size_t)n * 16);
       q.T = (double*)bm.take(qr_batched_t_bytes(m, n));
       q.Yp = (double*)bm.take(qr_batched_yp_bytes(m, g->pre_max_n));
+      q.TB = (double*)bm.take(qr_batched_tb_bytes());
       q.m = (int32_t)m;
       q.n = (int32_t)n;
       g->qr_probs.push_back(q);
