@@ -293,8 +293,11 @@ int tenet_grouped_plan(tenet_ctx ctx, int dtype, int prec, int64_t count, const 
  * transfer-matrix and Vidal-gauge updates need; NULL = none). */
 int tenet_grouped_plan_conj(tenet_ctx ctx, int dtype, int prec, int64_t count, const tenet_tdesc* descs,
                             const int32_t* conj, tenet_gplan* plan);
-/* ptrs: HOST array of 3*count device pointers (A, B, C per problem); returns
- * after the launch completes */
+/* ptrs: HOST array of 3*count device pointers (A, B, C per problem), copied
+ * before the call returns (the host array may be reused at once).  The
+ * launch is asynchronous on the stream: later work on the stream (and a
+ * synchronising copy to the host) sees the results; one plan's launches must
+ * stay on one stream. */
 int tenet_grouped_exec(tenet_gplan plan, tenet_stream stream, void* const* ptrs);
 /* ptrs_dev: DEVICE array of 3*count pointers; asynchronous, graph-capturable */
 int tenet_grouped_exec_dev(tenet_gplan plan, tenet_stream stream, const uint64_t* ptrs_dev);

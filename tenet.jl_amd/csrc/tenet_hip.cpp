@@ -1741,7 +1741,15 @@ struct tenet_gplan_s {
   GroupedUnit* units = nullptr;
   uint32_t* counters = nullptr;
   void* partials = nullptr;
-  uint64_t* ptrs_dev = nullptr;
+  // operand pointers of tenet_grouped_exec: a ring of device tables filled
+  // from pinned host slots, each slot reused once the event recorded after
+  // its last launch has completed, so an exec returns without waiting
+  static constexpr int RING = 8;
+  uint64_t* ptrs_dev = nullptr;  // RING tables of 3 * count
+  uint64_t* ptrs_pinned = nullptr;
+  hipEvent_t ring_ev[RING] = {};
+  bool ring_used[RING] = {};
+  int ring_next = 0;
   std::vector<int64_t> spanA, spanB, spanC;
   double flops = 0.0, bytes = 0.0;
   int64_t shape_count[4] = {0, 0, 0, 0};  // 64, 32, 16 tiles; GROUPED_SMALLK
@@ -1908,7 +1916,7 @@ static int grouped_plan_build(tenet_ctx ctx, int dtype, int prec, int64_t count,
   const size_t b_unit = align_up(std::max<size_t>(units.size(), 1) * sizeof(GroupedUnit), 256);
   const size_t b_ctr = align_up((size_t)(64 + ctr_elems) * 4, 256);
   const size_t b_part = align_up((size_t)std::max<int64_t>(part_elems, 1) * 2 * rs, 256);
-  const size_t b_ptr = align_up((size_t)count * 3 * 8, 256);
+  const size_t b_ptr = align_up((size_t)count * 3 * 8, 256) * tenet_gplan_s::RING;
   // from the context's caching pool: plans are built per batch shape, and a
   // device allocation per plan would cost more than the plan's analysis
   TRY(tenet_malloc(ctx, b_tab + b_prob + b_unit + b_ctr + b_part + b_ptr, &g->dev));
@@ -1975,12 +1983,12 @@ extern "C" int tenet_grouped_plan_info(tenet_gplan plan, int64_t* units, int64_t
   return TENET_OK;
 }
 
-static int grouped_launch(tenet_gplan g, hipStream_t s) {
+static int grouped_launch(tenet_gplan g, hipStream_t s, const uint64_t* ptrs) {
   GroupedLaunch L{};
   L.probs = g->probs;
   L.units = g->units;
   L.nunits = g->nunits;
-  L.ptrs = g->ptrs_dev;
+  L.ptrs = ptrs;
   L.partials = g->partials;
   L.counters = g->counters;
   L.grid = g->grid;
@@ -2005,10 +2013,24 @@ extern "C" int tenet_grouped_exec(tenet_gplan plan, tenet_stream stream, void* c
     if (plan->spanA[(size_t)i]) TRY(check_span(ctx, ptrs[3 * i], (size_t)plan->spanA[(size_t)i] * es, "A"));
     if (plan->spanB[(size_t)i]) TRY(check_span(ctx, ptrs[3 * i + 1], (size_t)plan->spanB[(size_t)i] * es, "B"));
   }
-  // pointers travel with the launch; the host array may be reused on return
-  TENET_HIP_CHECK(hipMemcpyAsync(plan->ptrs_dev, ptrs, (size_t)plan->count * 3 * 8, hipMemcpyHostToDevice, s));
-  TRY(grouped_launch(plan, s));
-  TENET_HIP_CHECK(hipStreamSynchronize(s));
+  // pointers travel through the next ring slot (pinned host, then device:
+  // the copy is asynchronous); the host array may be reused on return
+  const size_t nb = (size_t)plan->count * 3 * 8;
+  if (!plan->ptrs_pinned) {
+    TENET_HIP_CHECK(hipHostMalloc((void**)&plan->ptrs_pinned, nb * tenet_gplan_s::RING, hipHostMallocDefault));
+    for (int i = 0; i < tenet_gplan_s::RING; ++i)
+      TENET_HIP_CHECK(hipEventCreateWithFlags(&plan->ring_ev[i], hipEventDisableTiming));
+  }
+  const int slot = plan->ring_next;
+  plan->ring_next = (slot + 1) % tenet_gplan_s::RING;
+  if (plan->ring_used[slot]) TENET_HIP_CHECK(hipEventSynchronize(plan->ring_ev[slot]));
+  uint64_t* host = plan->ptrs_pinned + (size_t)slot * plan->count * 3;
+  uint64_t* dev = (uint64_t*)((char*)plan->ptrs_dev + align_up(nb, 256) * slot);
+  std::memcpy(host, ptrs, nb);
+  TENET_HIP_CHECK(hipMemcpyAsync(dev, host, nb, hipMemcpyHostToDevice, s));
+  TRY(grouped_launch(plan, s, dev));
+  TENET_HIP_CHECK(hipEventRecord(plan->ring_ev[slot], s));
+  plan->ring_used[slot] = true;
   return TENET_OK;
 }
 
@@ -2020,9 +2042,8 @@ extern "C" int tenet_grouped_exec_dev(tenet_gplan plan, tenet_stream stream, con
   CHECK_ARG(ptrs_dev, "null pointer array");
   hipStream_t s = pick(ctx, stream);
   TRY(check_span(ctx, ptrs_dev, (size_t)plan->count * 3 * 8, "pointer array"));
-  if ((const void*)ptrs_dev != (const void*)plan->ptrs_dev)
-    TENET_HIP_CHECK(hipMemcpyAsync(plan->ptrs_dev, ptrs_dev, (size_t)plan->count * 3 * 8, hipMemcpyDeviceToDevice, s));
-  return grouped_launch(plan, s);
+  // the launch reads the caller's table (it stays valid until the launch has run)
+  return grouped_launch(plan, s, ptrs_dev);
 }
 
 extern "C" int tenet_grouped_plan_destroy(tenet_gplan plan) {
@@ -2030,6 +2051,10 @@ extern "C" int tenet_grouped_plan_destroy(tenet_gplan plan) {
   (void)hipSetDevice(plan->ctx->device);
   (void)hipDeviceSynchronize();  // no launch still reads the arena when the pool hands it out again
   if (plan->dev) (void)tenet_free(plan->ctx, plan->dev);
+  if (plan->ptrs_pinned) {
+    (void)hipHostFree(plan->ptrs_pinned);
+    for (int i = 0; i < tenet_gplan_s::RING; ++i) (void)hipEventDestroy(plan->ring_ev[i]);
+  }
   delete plan;
   return TENET_OK;
 }
@@ -2076,7 +2101,7 @@ extern "C" int tenet_autotune_tiles(tenet_ctx ctx, int dtype, double costs_out[3
       float best = 1e30f;
       for (int rep = 0; rep < 4 && st == TENET_OK; ++rep) {
         (void)hipEventRecord(ctx->ev0, s);
-        st = grouped_launch(&g, s);
+        st = grouped_launch(&g, s, g.ptrs_dev);
         (void)hipEventRecord(ctx->ev1, s);
         (void)hipEventSynchronize(ctx->ev1);
         float ms = 0.f;

@@ -69,8 +69,9 @@ class GroupedPlan:
                 "flops": fl.value, "bytes": by.value}
 
     def exec(self, ptrs, stream=None):
-        """ptrs: 3 * count device addresses (A, B, C per problem).  Returns
-        when the launch has completed."""
+        """ptrs: 3 * count device addresses (A, B, C per problem).  The
+        launch is asynchronous on the stream (the addresses are copied on
+        the call); later work on the same stream sees the results."""
         arr = (C.c_void_p * (3 * self.count))(*[int(p) for p in ptrs])
         L.check(L.lib.tenet_grouped_exec(self.h, stream, arr), "grouped_exec")
 
