@@ -319,7 +319,10 @@ int tenet_autotune_tiles(tenet_ctx ctx, int dtype, double costs_out[3]);
  * ms, ns, maxdims are host arrays; k_dev / disc_dev device arrays (count).
  * Optional: Us[i] = NULL (U not formed; a problem with m <= n then runs the
  * Jacobi on A^H and accumulates no rotations), Ss = NULL, maxdims = NULL
- * (no cap), k_dev = NULL, disc_dev = NULL.  As[i] and Vhs[i] are required. */
+ * (no cap), k_dev = NULL, disc_dev = NULL.  As[i] and Vhs[i] are required.
+ * Returns once the last kernels are queued on the stream (the host reads the
+ * convergence flags once per Jacobi sweep, one sweep behind the launches):
+ * U, S, Vh, k_dev and disc_dev are read after a later synchronisation. */
 int tenet_grouped_svd(tenet_ctx ctx, tenet_stream stream, int dtype, int64_t count, const int64_t* ms,
                       const int64_t* ns, const void* const* As, void* const* Us, double* const* Ss,
                       void* const* Vhs, const int64_t* maxdims, double threshold, int64_t* k_dev, double* disc_dev);

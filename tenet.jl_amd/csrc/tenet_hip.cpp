@@ -139,6 +139,22 @@ struct tenet_ctx_s {
   // grouped-launch tile costs per (tile, K step) relative to 64 x 64, per dtype (0 = unset)
   double tile_cost[2][3] = {{0, 0, 0}, {0, 0, 0}};
   double stat_ms = 0.0, stat_flops = 0.0, stat_bytes = 0.0;
+  // pinned landing slots for the per-sweep convergence flags of grouped SVD
+  // runs (free list; a run takes one for its duration)
+  struct FlagSlot {
+    int32_t* host = nullptr;
+    size_t bytes = 0;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+  };
+  std::vector<FlagSlot*> flag_slots;
+  // pinned staging ring for small per-call table uploads (a copy from
+  // pageable memory would wait for the stream to drain first)
+  static constexpr int UP_RING = 8;
+  char* up_host[UP_RING] = {};
+  size_t up_cap[UP_RING] = {};
+  hipEvent_t up_ev[UP_RING] = {};
+  bool up_used[UP_RING] = {};
+  int up_next = 0;
 };
 // Every stat_every-th launch of the tracked family is bracketed by events
 // (default 17: an odd stride, so the two alternating phases of a Jacobi round
@@ -209,6 +225,43 @@ struct StatScope {
     if (hipEventRecord(ctx->stat_events[ctx->stat_used + 1], s) == hipSuccess) ctx->stat_used += 2;
   }
 };
+
+// Take work back out of a family's running totals: launches counted with
+// problems that, as it turned out, had already finished (their kernels
+// returned at once).  Applies exactly when StatScope counted them.
+static void stats_retract(tenet_ctx ctx, int family, hipStream_t s, double flops, double bytes) {
+  if (!ctx || family == 0 || ctx->stat_family != family || (flops == 0.0 && bytes == 0.0)) return;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &cs) == hipSuccess && cs == hipStreamCaptureStatusActive) return;
+  ctx->stat_flops -= flops;
+  ctx->stat_bytes -= bytes;
+}
+
+// Queue an upload of host bytes through the context's pinned ring: the bytes
+// are copied into a slot at once (the caller's buffer may go right after), the
+// slot is reused once its copy has completed.  Not for captured streams (a
+// graph's copy node would read the slot at every replay).
+static int upload_async(tenet_ctx ctx, hipStream_t s, void* dst, const void* src, size_t bytes) {
+  if (bytes == 0) return TENET_OK;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  const int slot = ctx->up_next;
+  ctx->up_next = (slot + 1) % tenet_ctx_s::UP_RING;
+  if (!ctx->up_ev[slot]) TENET_HIP_CHECK(hipEventCreateWithFlags(&ctx->up_ev[slot], hipEventDisableTiming));
+  if (ctx->up_used[slot]) TENET_HIP_CHECK(hipEventSynchronize(ctx->up_ev[slot]));
+  if (ctx->up_cap[slot] < bytes) {
+    if (ctx->up_host[slot]) TENET_HIP_CHECK(hipHostFree(ctx->up_host[slot]));
+    ctx->up_host[slot] = nullptr;
+    ctx->up_cap[slot] = 0;
+    const size_t cap = std::max<size_t>((bytes + 65535) / 65536 * 65536, 65536);
+    TENET_HIP_CHECK(hipHostMalloc((void**)&ctx->up_host[slot], cap, hipHostMallocDefault));
+    ctx->up_cap[slot] = cap;
+  }
+  std::memcpy(ctx->up_host[slot], src, bytes);
+  TENET_HIP_CHECK(hipMemcpyAsync(dst, ctx->up_host[slot], bytes, hipMemcpyHostToDevice, s));
+  TENET_HIP_CHECK(hipEventRecord(ctx->up_ev[slot], s));
+  ctx->up_used[slot] = true;
+  return TENET_OK;
+}
 
 // launch_svd_round with its timing sample stamped by the dispatch
 static int svd_round_timed(tenet_ctx ctx, hipStream_t s, double flops, double bytes, const SvdBatch& B, int64_t nprob,
@@ -385,6 +438,15 @@ extern "C" int tenet_destroy(tenet_ctx ctx) {
     if (kv.second.ptr) (void)hipFree(kv.second.ptr);
   (void)hipHostFree(ctx->host_flag);
   (void)hipHostFree(ctx->host_k);
+  for (int i = 0; i < tenet_ctx_s::UP_RING; ++i) {
+    if (ctx->up_host[i]) (void)hipHostFree(ctx->up_host[i]);
+    if (ctx->up_ev[i]) (void)hipEventDestroy(ctx->up_ev[i]);
+  }
+  for (auto* f : ctx->flag_slots) {
+    (void)hipHostFree(f->host);
+    for (hipEvent_t e : f->ev) (void)hipEventDestroy(e);
+    delete f;
+  }
   if (ctx->dev_status) (void)hipFree(ctx->dev_status);
   (void)hipEventDestroy(ctx->ev0);
   (void)hipEventDestroy(ctx->ev1);
@@ -2303,22 +2365,68 @@ static int gsvd_build(tenet_ctx ctx, int dtype, int64_t count, const int64_t* ms
     g->probs[(size_t)i].fro2 = fro_base + i;
   }
   // ordered on the stream after earlier work (the pool block may have been
-  // freed by a caller while that work still used it); g->probs outlives it
-  TENET_HIP_CHECK(hipMemcpyAsync(g->dprobs, g->probs.data(), g->probs.size() * sizeof(SvdProblem),
-                                 hipMemcpyHostToDevice, s));
-  if (!g->units.empty())
-    TENET_HIP_CHECK(hipMemcpyAsync(g->units_dev, g->units.data(), g->units.size() * 4, hipMemcpyHostToDevice, s));
-  if (!g->qr_probs.empty())
-    TENET_HIP_CHECK(hipMemcpyAsync(g->qr_dev, g->qr_probs.data(), g->qr_probs.size() * sizeof(QrBatchProb),
-                                   hipMemcpyHostToDevice, s));
+  // freed by a caller while that work still used it)
+  TRY(upload_async(ctx, s, g->dprobs, g->probs.data(), g->probs.size() * sizeof(SvdProblem)));
+  TRY(upload_async(ctx, s, g->units_dev, g->units.data(), g->units.size() * 4));
+  TRY(upload_async(ctx, s, g->qr_dev, g->qr_probs.data(), g->qr_probs.size() * sizeof(QrBatchProb)));
   return TENET_OK;
 }
 
-// One grouped run.  Eagerly: sweeps until every problem converges (flags read
-// back per sweep), then problems left unconverged go through the
-// single-matrix path.  Under capture: a fixed CAPTURED_SWEEPS sweeps (rounds
-// of converged problems return at once) and the context's device status bit
-// for problems they did not finish.
+// Pinned landing slot for a grouped run's convergence flags: two sweeps'
+// copies in flight, each followed by an event.  Taken from the context's free
+// list and handed back (after both copies have landed) when the run ends.
+struct FlagSlotGuard {
+  tenet_ctx ctx;
+  tenet_ctx_s::FlagSlot* f = nullptr;
+  explicit FlagSlotGuard(tenet_ctx c) : ctx(c) {}
+  int take(size_t bytes) {
+    {
+      std::lock_guard<std::mutex> lk(ctx->mu);
+      for (size_t i = 0; i < ctx->flag_slots.size(); ++i)
+        if (ctx->flag_slots[i]->bytes >= bytes) {
+          f = ctx->flag_slots[i];
+          ctx->flag_slots.erase(ctx->flag_slots.begin() + (std::ptrdiff_t)i);
+          return TENET_OK;
+        }
+    }
+    auto* n = new tenet_ctx_s::FlagSlot();
+    n->bytes = align_up(bytes, 4096);
+    if (hipHostMalloc((void**)&n->host, n->bytes, hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      delete n;
+      set_error("pinned host allocation failed");
+      return TENET_ERR_OOM;
+    }
+    for (hipEvent_t& e : n->ev)
+      if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipHostFree(n->host);
+        for (hipEvent_t x : n->ev)
+          if (x) (void)hipEventDestroy(x);
+        delete n;
+        set_error("event creation failed");
+        return TENET_ERR_HIP;
+      }
+    f = n;
+    return TENET_OK;
+  }
+  ~FlagSlotGuard() {
+    if (!f) return;
+    // no copy may still be landing in the slot when another run takes it
+    for (hipEvent_t e : f->ev) (void)hipEventSynchronize(e);
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->flag_slots.push_back(f);
+  }
+};
+
+// One grouped run.  Eagerly: sweeps until every problem converges, then
+// problems left unconverged go through the single-matrix path.  The flags of
+// sweep s are copied to pinned memory behind it and read while sweep s + 1 is
+// already queued, so the stream never idles on the host's convergence test;
+// the sweep queued after the last one needed costs only its launches (rounds
+// of converged problems return at once).  Returns without waiting for the
+// final kernels.  Under capture: a fixed CAPTURED_SWEEPS sweeps and the
+// context's device status bit for problems they did not finish.
 static int gsvd_run(tenet_gsvd g, hipStream_t s) {
   tenet_ctx ctx = g->ctx;
   const int64_t count = g->count;
@@ -2331,22 +2439,31 @@ static int gsvd_run(tenet_gsvd g, hipStream_t s) {
                           (int)g->pre_max_n, s));
   }
   TRY(launch_svd_init(B, count, g->max_elems, (int)g->max_ncp, g->dtype, s));
+  const size_t fbytes = (size_t)count * 32;
+  FlagSlotGuard slot(ctx);
+  if (sync_ok) TRY(slot.take(2 * fbytes));
+  // converged flags as last read back (one sweep behind the launches)
   std::vector<int32_t> hflags((size_t)count * 8, 0);
+  // nominal work of one sweep of problem i: the Gram of every pair over its
+  // rows and the rotation of its X and V rows, in the fixed-pair round and,
+  // when nblk > 2, in each cross round r < nblk - 1
+  auto unit_cost = [&](int64_t i, double* f, double* b) {
+    const SvdShape& sh = g->shapes[(size_t)i];
+    const double n2 = 2.0 * sh.b, vr = g->probs[(size_t)i].accum ? (double)sh.ncp : 0.0;
+    *f = 8.0 * sh.npairs * n2 * n2 * (2.0 * sh.nr + vr);
+    *b = 16.0 * sh.npairs * n2 * (3.0 * sh.nr + 2.0 * vr);
+  };
   bool all = false;
+  int inflight = -1;  // sweep whose flags copy is queued and not yet read
   const int max_sweeps = sync_ok ? 40 : CAPTURED_SWEEPS;
-  for (int sw = 0; sw < max_sweeps && !all; ++sw) {
-    // nominal work of one round over the problems still sweeping: the Gram of
-    // every pair over its rows and the rotation of its X and V rows (a cross
-    // round r exists only for problems of more than one block pair with
-    // r < nblk - 1)
+  for (int sw = 0; sw < max_sweeps; ++sw) {
     double fl = 0.0, by = 0.0;
     std::vector<double> flr((size_t)g->max_rounds, 0.0), byr((size_t)g->max_rounds, 0.0);
     for (int64_t i = 0; i < count; ++i) {
       if (hflags[(size_t)(8 * i + 1)]) continue;
       const SvdShape& sh = g->shapes[(size_t)i];
-      const double n2 = 2.0 * sh.b, vr = g->probs[(size_t)i].accum ? (double)sh.ncp : 0.0;
-      const double f = 8.0 * sh.npairs * n2 * n2 * (2.0 * sh.nr + vr);
-      const double b = 16.0 * sh.npairs * n2 * (3.0 * sh.nr + 2.0 * vr);
+      double f, b;
+      unit_cost(i, &f, &b);
       fl += f;
       by += b;
       if (sh.nblk > 2)
@@ -2364,20 +2481,50 @@ static int gsvd_run(tenet_gsvd g, hipStream_t s) {
       TRY(svd_round_timed(ctx, s, flr[(size_t)r], byr[(size_t)r], Bu, count, (int)g->ucnt[(size_t)r + 1], r, 2, 1));
     }
     TRY(launch_svd_sweep_end(B, count, s));
-    if (sync_ok) {
-      TENET_HIP_CHECK(hipMemcpyAsync(hflags.data(), g->flags_base, (size_t)count * 32, hipMemcpyDeviceToHost, s));
-      TENET_HIP_CHECK(hipStreamSynchronize(s));
-      all = true;
-      for (int64_t i = 0; i < count; ++i) all = all && hflags[(size_t)(8 * i + 1)] != 0;
+    if (!sync_ok) continue;
+    int32_t* land = slot.f->host + (size_t)(sw & 1) * count * 8;
+    TENET_HIP_CHECK(hipMemcpyAsync(land, g->flags_base, fbytes, hipMemcpyDeviceToHost, s));
+    TENET_HIP_CHECK(hipEventRecord(slot.f->ev[sw & 1], s));
+    const int prev = inflight;
+    inflight = sw;
+    if (prev < 0) continue;
+    TENET_HIP_CHECK(hipEventSynchronize(slot.f->ev[prev & 1]));
+    const int32_t* nf = slot.f->host + (size_t)(prev & 1) * count * 8;
+    // problems that converged in sweep prev were counted in this sweep's
+    // launches, whose rounds returned at once for them: take their share
+    // back out of the round statistics
+    double dfl = 0.0, dby = 0.0;
+    all = true;
+    for (int64_t i = 0; i < count; ++i) {
+      const bool was = hflags[(size_t)(8 * i + 1)] != 0, now = nf[8 * i + 1] != 0;
+      if (!was && now) {
+        const SvdShape& sh = g->shapes[(size_t)i];
+        double f, b;
+        unit_cost(i, &f, &b);
+        const double rounds = 1.0 + (sh.nblk > 2 ? (double)(sh.nblk - 1) : 0.0);
+        dfl += f * rounds;
+        dby += b * rounds;
+      }
+      all = all && now;
     }
+    std::memcpy(hflags.data(), nf, fbytes);
+    stats_retract(ctx, TENET_KSTAT_SVD_ROUND, s, dfl, dby);
+    if (all) break;
+  }
+  if (sync_ok && !all && inflight >= 0) {
+    // the sweep limit ran out: the last sweep's flags decide
+    TENET_HIP_CHECK(hipEventSynchronize(slot.f->ev[inflight & 1]));
+    std::memcpy(hflags.data(), slot.f->host + (size_t)(inflight & 1) * count * 8, fbytes);
+    all = true;
+    for (int64_t i = 0; i < count; ++i) all = all && hflags[(size_t)(8 * i + 1)] != 0;
   }
   TRY(launch_svd_finalize(B, count, (int)g->max_ncp, g->max_out, g->dtype, s));
   if (!sync_ok) return launch_svd_status(B, count, ctx->dev_status, s);
-  TENET_HIP_CHECK(hipStreamSynchronize(s));
   if (!all) {
     // Problems the plain sweeps left unconverged (strongly graded spectra) go
     // one at a time through the single-matrix path, which preconditions
-    // complex128 by a QR, from their untouched inputs.
+    // complex128 by a QR, from their untouched inputs (stream-ordered after
+    // the finalisation above).
     for (int64_t i = 0; i < count; ++i) {
       if (hflags[(size_t)(8 * i + 1)]) continue;
       CHECK_ARG(g->probs[(size_t)i].s_mode == 0, "grouped SVD: unconverged problem with a complex S output");
@@ -2409,6 +2556,10 @@ extern "C" int tenet_grouped_svd_plan(tenet_ctx ctx, int dtype, int64_t count, c
   CHECK_ARG(plan && count > 0 && ms && ns && As && Us && Vhs, "bad argument");
   CHECK_ARG(s_mode == TENET_S_REAL || s_mode == TENET_S_DTYPE, "bad S mode");
   *plan = nullptr;
+  if (capturing(ctx->stream)) {
+    set_error("tenet_grouped_svd_plan: build the plan before the capture (it uploads its tables)");
+    return TENET_ERR_UNSUPPORTED;
+  }
   auto* g = new tenet_gsvd_s();
   const int sm = s_mode == TENET_S_REAL ? 0 : (dtype == TENET_C128 ? 1 : 2);
   const int st = gsvd_build(ctx, dtype, count, ms, ns, As, Us, reinterpret_cast<double* const*>(Ss), Vhs, maxdims,
@@ -2457,11 +2608,9 @@ extern "C" int tenet_grouped_svd(tenet_ctx ctx, tenet_stream stream, int dtype, 
   tenet_gsvd_s g;
   int st = gsvd_build(ctx, dtype, count, ms, ns, As, Us, Ss, Vhs, maxdims, threshold, k_dev, disc_dev, 0, &g, s);
   if (st == TENET_OK) st = gsvd_run(&g, s);
-  if (g.dev) {
-    // the run ends with a stream synchronisation, except on an early error
-    (void)hipStreamSynchronize(s);
-    (void)tenet_free(ctx, g.dev);
-  }
+  // the run's last kernels may still be in flight: the block returns to the
+  // pool once they have finished
+  if (g.dev) (void)tenet_free_async(ctx, g.dev, (tenet_stream)s);
   return st;
 }
 

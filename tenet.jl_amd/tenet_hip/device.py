@@ -3,6 +3,7 @@ the C-ABI expects).  Memory comes from the context's caching pool."""
 from __future__ import annotations
 
 import ctypes as C
+import math
 import weakref
 
 import numpy as np
@@ -29,6 +30,7 @@ class Context:
         # while a graph is being captured: device blocks freed meanwhile
         # (the graph's replays still write them), released with the graph
         self._defer = None
+        self._staging = None
 
     def close(self):
         if not self._closed:
@@ -38,6 +40,10 @@ class Context:
                 except Exception:
                     pass
             self.plan_cache = {}
+            if self._staging is not None:
+                self.sync()
+                self._staging.free()
+                self._staging = None
             L.lib.tenet_destroy(self.h)
             self._closed = True
 
@@ -77,6 +83,18 @@ class Context:
         the block."""
         return Graph(self)
 
+    def staging(self, nbytes: int) -> "PinnedBuffer":
+        """The context's pinned host staging buffer, at least ``nbytes``.
+        Copies through it are queued on the context's stream; a caller
+        rewrites a region only after a wait that follows the previous copy
+        out of that region on the stream.  Growing it waits for the stream."""
+        if self._staging is None or self._staging.nbytes < nbytes:
+            if self._staging is not None:
+                self.sync()
+                self._staging.free()
+            self._staging = PinnedBuffer(self, max(int(nbytes), 1 << 16) * 2)
+        return self._staging
+
     def scalar_buffer(self) -> int:
         """A small device buffer (64 doubles) owned by the context, for
         scalar results read back right away."""
@@ -111,6 +129,44 @@ class Context:
         ms = C.c_float()
         L.check(L.lib.tenet_timer_stop(self.h, stream, C.byref(ms)))
         return ms.value
+
+
+class PinnedBuffer:
+    """A pinned host buffer (tenet_host_alloc) viewed as numpy arrays."""
+
+    def __init__(self, ctx: Context, nbytes: int):
+        self.ctx = ctx
+        p = C.c_void_p()
+        L.check(L.lib.tenet_host_alloc(ctx.h, C.c_size_t(max(nbytes, 8)), C.byref(p)), "host_alloc")
+        self.ptr = p.value
+        self.nbytes = max(nbytes, 8)
+
+    def view(self, dtype, count, offset=0):
+        buf = (C.c_char * self.nbytes).from_address(self.ptr)
+        return np.frombuffer(buf, dtype=dtype, count=count, offset=offset)
+
+    def free(self):
+        if self.ptr:
+            L.lib.tenet_host_free(self.ctx.h, C.c_void_p(self.ptr))
+            self.ptr = None
+
+
+def carve(ctx: Context, shapes, dtype):
+    """Device arrays of the given shapes cut from one pool block (each at a
+    256-byte boundary): one allocation for a layer's worth of operands.  The
+    block is released when the last of them is dropped."""
+    dt = np.dtype(dtype)
+    offs, tot = [], 0
+    for s in shapes:
+        offs.append(tot)
+        tot += (math.prod(s) * dt.itemsize + 255) // 256 * 256
+    owner = DeviceArray(ctx, (max(tot // dt.itemsize, 1),), dt)
+    out = []
+    for s, o in zip(shapes, offs):
+        v = DeviceArray(ctx, s, dt, ptr=owner.ptr + o, owner=False)
+        v._base = owner
+        out.append(v)
+    return out
 
 
 def _free(ctx_ref, ptr):
@@ -185,7 +241,7 @@ class DeviceArray:
         if self.dtype not in DTYPES:
             raise TypeError("only complex128 / complex64 device arrays")
         self.code = DTYPES[self.dtype]
-        self.size = int(np.prod(self.shape, dtype=np.int64)) if self.shape else 1
+        self.size = math.prod(self.shape)
         self.nbytes = self.size * self.dtype.itemsize
         self.ptr = ptr if ptr is not None else ctx.malloc(self.nbytes)
         self._fin = weakref.finalize(self, _free, weakref.ref(ctx), self.ptr) if owner else None

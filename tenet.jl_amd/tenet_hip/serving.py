@@ -32,7 +32,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _lib as L
-from .device import Context, DeviceArray, tdesc
+from .device import Context, DeviceArray, PinnedBuffer, tdesc
 
 
 @dataclass
@@ -110,26 +110,6 @@ def request_footprint(st: ServedState, r) -> int:
     return (2 * mx_env + 2 * mx_site) * st.dtype.itemsize + 64
 
 
-class _Pinned:
-    """A pinned host buffer viewed as a numpy array."""
-
-    def __init__(self, ctx: Context, nbytes: int):
-        self.ctx = ctx
-        p = C.c_void_p()
-        L.check(L.lib.tenet_host_alloc(ctx.h, C.c_size_t(max(nbytes, 8)), C.byref(p)), "host_alloc")
-        self.ptr = p.value
-        self.nbytes = max(nbytes, 8)
-
-    def view(self, dtype, count, offset=0):
-        buf = (C.c_char * self.nbytes).from_address(self.ptr)
-        return np.frombuffer(buf, dtype=dtype, count=count, offset=offset)
-
-    def free(self):
-        if self.ptr:
-            L.lib.tenet_host_free(self.ctx.h, C.c_void_p(self.ptr))
-            self.ptr = None
-
-
 class _PinnedPool:
     """Pinned buffers by power-of-two size class, reused across requests
     (pinning pages is far slower than a request)."""
@@ -138,14 +118,14 @@ class _PinnedPool:
         self.ctx = ctx
         self.free = {}
 
-    def get(self, nbytes: int) -> _Pinned:
+    def get(self, nbytes: int) -> PinnedBuffer:
         cls = 1 << max(6, (int(nbytes) - 1).bit_length())
         lst = self.free.get(cls)
         if lst:
             return lst.pop()
-        return _Pinned(self.ctx, cls)
+        return PinnedBuffer(self.ctx, cls)
 
-    def put(self, p: _Pinned):
+    def put(self, p: PinnedBuffer):
         self.free.setdefault(p.nbytes, []).append(p)
 
     def close(self):
@@ -179,7 +159,7 @@ class _Batch:
     reqs: list
     kind: str
     event: _Event
-    pinned: _Pinned
+    pinned: PinnedBuffer
     dev: list = field(default_factory=list)
     footprint: int = 0
 

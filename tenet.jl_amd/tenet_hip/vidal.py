@@ -40,7 +40,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib as L
-from .device import DTYPES, Context, DeviceArray, tdesc
+from .device import DTYPES, Context, DeviceArray, carve, tdesc
 from .grouped import GroupedPlan
 from .mps import MPS, _gate_tensor, svd_flops
 
@@ -59,6 +59,7 @@ class VidalMPS:
         self._graphs = {}  # (parity, layer signature) -> _LayerGraph
         self._lam_bufs = None  # fixed per-bond Lambda buffers once layers are captured
         self._lam_stale = False  # lam_host behind the device Lambdas (captured layers ran)
+        self._kept = {}  # (bonds, shapes, maxdim, threshold) -> the kept counts of the last such layer
 
     # ---- construction -------------------------------------------------
     def _upload_lams(self, lams):
@@ -101,6 +102,7 @@ class VidalMPS:
         v._graphs = {}
         v._lam_bufs = None
         v._lam_stale = False
+        v._kept = {}
         B, lam = [], []
         bt = p.sites[0]
         prev = None
@@ -135,24 +137,59 @@ class VidalMPS:
 
     # ---- one brickwork layer ------------------------------------------------
     def _plan(self, key, problems, conj=None):
+        """The cached grouped plan for ``key``; ``problems`` (a list, or a
+        callable returning (problems, conj)) is only built on a miss."""
         # cached on the context: every sweep (and every VidalMPS) with the
         # same layer shapes reuses the plans
         key = ("vidal", self.dtype.str) + key
         p = self.ctx.plan_cache.get(key)
         if p is None:
+            if callable(problems):
+                problems, conj = problems()
             p = GroupedPlan(self.ctx, problems, self.dtype, L.PREC_NATIVE, conj)
             self.ctx.plan_cache[key] = p
         return p
 
+    def _out_step(self, shp, kmax, ks, tt, Vs):
+        """Step 5 of a layer for kept counts ``ks``: the new sites (one pool
+        block) and the grouped plan B_a <- theta~ conj(V^H[:k])^T,
+        B_{a+1} <- V^H[:k] (a strided view of V^H), with its pointer list."""
+        ctx, dt, P = self.ctx, self.dtype, len(shp)
+        sites = carve(ctx, [(l, d1, k) for (l, d1, b, d2, r), k in zip(shp, ks)] +
+                      [(k, d2, r) for (l, d1, b, d2, r), k in zip(shp, ks)], dt)
+        newA, newB = sites[:P], sites[P:]
+
+        def problems():
+            probs, conj = [], []
+            for (l, d1, b, d2, r), k, km in zip(shp, ks, kmax):
+                km = int(km)
+                vst = (1, km, km * d2)
+                probs.append(((l, d1, d2, r), "lpqr", (k, d2, r), "kqr", (l, d1, k), "lpk", None, vst))
+                conj.append((0, 1))
+                probs.append(((k, d2, r), "kqr", (), "", (k, d2, r), "kqr", vst, None))
+                conj.append((0, 0))
+            return probs, conj
+
+        plan = self._plan(("out", tuple(shp), ks), problems)
+        one = self._one.ptr
+        ptrs = [x for j in range(P) for x in (tt[j].ptr, Vs[j].ptr, newA[j].ptr, Vs[j].ptr, one, newB[j].ptr)]
+        return newA, newB, plan, ptrs
+
     def apply_layer_(self, gates, bonds, maxdim=None, threshold=None):
         """Apply the two-site gates ``gates[j]`` (4x4, site a as the most
         significant digit, U4[d*oa+ob, d*ia+ib] as in simple_update!) on the
-        disjoint bonds ``bonds[j]`` = (a, a+1).  Returns the kept counts."""
+        disjoint bonds ``bonds[j]`` = (a, a+1).  Returns the kept counts.
+
+        The host waits on the device once per layer, for the kept counts;
+        everything else is queued behind.  While the counts come back the
+        output step is prepared for the counts the same layer kept last time
+        (or min(kmax, maxdim)), so the stream idles only for the check."""
         ctx, dt = self.ctx, self.dtype
         P = len(bonds)
         if P == 0:
             return []
-        if len(set(bonds)) != P or any(abs(a - b) < 2 for a in bonds for b in bonds if a != b):
+        sb = sorted(bonds)
+        if any(y - x < 2 for x, y in zip(sb, sb[1:])):
             raise ValueError("a layer's bonds must be disjoint")
         # an eager layer replaces sites and Lambdas: captured layers bound to
         # the old buffers are dropped
@@ -171,15 +208,16 @@ class VidalMPS:
             l, d1, b = self.B[a].shape
             _, d2, r = self.B[a + 1].shape
             shp.append((l, d1, b, d2, r))
+        shp = tuple(shp)
         # 1. theta = B_a B_{a+1}
-        th = [DeviceArray(ctx, (l, d1, d2, r), dt) for (l, d1, b, d2, r) in shp]
-        p1 = self._plan(("theta", tuple(shp)), [((l, d1, b), "lxb", (b, d2, r), "byr", (l, d1, d2, r), "lxyr")
-                                                 for (l, d1, b, d2, r) in shp])
+        th = carve(ctx, [(l, d1, d2, r) for (l, d1, b, d2, r) in shp], dt)
+        p1 = self._plan(("theta", shp), lambda: ([((l, d1, b), "lxb", (b, d2, r), "byr", (l, d1, d2, r), "lxyr")
+                                                  for (l, d1, b, d2, r) in shp], None))
         p1.exec([x for j, a in enumerate(bonds) for x in (self.B[a].ptr, self.B[a + 1].ptr, th[j].ptr)])
         # 2. theta~ = theta . G   (G[oa, ob, ia, ib])
-        tt = [DeviceArray(ctx, (l, d1, d2, r), dt) for (l, d1, b, d2, r) in shp]
-        p2 = self._plan(("gate", tuple(shp)), [((l, d1, d2, r), "lxyr", (d1, d2, d1, d2), "pqxy", (l, d1, d2, r),
-                                                "lpqr") for (l, d1, b, d2, r) in shp])
+        tt = carve(ctx, [(l, d1, d2, r) for (l, d1, b, d2, r) in shp], dt)
+        p2 = self._plan(("gate", shp), lambda: ([((l, d1, d2, r), "lxyr", (d1, d2, d1, d2), "pqxy",
+                                                  (l, d1, d2, r), "lpqr") for (l, d1, b, d2, r) in shp], None))
         p2.exec([x for j in range(P) for x in (th[j].ptr, Gs[j].ptr, tt[j].ptr)])
         # 3. X = Lambda_{a-1} (.) theta~
         idx = [j for j, a in enumerate(bonds) if a > 0]
@@ -187,9 +225,9 @@ class VidalMPS:
         if idx:
             for j in idx:
                 X[j] = th[j]  # theta is no longer needed: reuse its memory
-            p3 = self._plan(("lam", tuple(shp[j] for j in idx)),
-                            [((shp[j][0],), "l", shp[j][:2] + shp[j][3:], "lpqr", shp[j][:2] + shp[j][3:], "lpqr")
-                             for j in idx])
+            sub = tuple(shp[j] for j in idx)
+            p3 = self._plan(("lam", sub), lambda: ([((s[0],), "l", s[:2] + s[3:], "lpqr", s[:2] + s[3:], "lpqr")
+                                                   for s in sub], None))
             p3.exec([x for j in idx for x in (self.lam[bonds[j] - 1].ptr, tt[j].ptr, X[j].ptr)])
         # 4. grouped SVD of X as (l d) x (d r)
         ms = np.array([l * d1 for (l, d1, b, d2, r) in shp], dtype=np.int64)
@@ -197,13 +235,16 @@ class VidalMPS:
         kmax = np.minimum(ms, ns)
         # U is never used (B_a is formed from theta~ and V): no U pointers, so
         # the SVD of an m <= n problem runs on X^H and accumulates no rotations
-        Vs = [DeviceArray(ctx, (int(k), int(n)), dt) for k, n in zip(kmax, ns)]
-        # the kept counts and every S in one buffer: one download each
+        Vs = carve(ctx, [(int(k), int(n)) for k, n in zip(kmax, ns)], dt)
+        # the kept counts and every S in one buffer: one download
         s_off = np.concatenate([[0], np.cumsum(kmax)]).astype(np.int64)
-        kd = ctx.malloc(P * 8 + int(s_off[-1]) * 8)
+        nk = P + int(s_off[-1])
+        kd = ctx.malloc(nk * 8)
         sbase = kd + P * 8
         mds = np.full(P, md, dtype=np.int64)
         Pp = C.c_void_p
+        lam_off = (nk * 8 + 255) // 256 * 256
+        stg = ctx.staging(lam_off + int(s_off[-1]) * dt.itemsize)
         try:
             L.check(L.lib.tenet_grouped_svd(
                 ctx.h, None, self.code, P, ms.ctypes.data_as(C.POINTER(C.c_int64)),
@@ -211,41 +252,51 @@ class VidalMPS:
                 (Pp * P)(*([None] * P)), (Pp * P)(*[sbase + 8 * int(o) for o in s_off[:-1]]),
                 (Pp * P)(*[v.ptr for v in Vs]), mds.ctypes.data_as(C.POINTER(C.c_int64)), thr, Pp(kd), None),
                 "grouped_svd")
-            host = np.zeros(P + int(s_off[-1]), dtype=np.float64)
-            L.check(L.lib.tenet_memcpy_d2h(ctx.h, host.ctypes.data_as(Pp), Pp(kd), host.nbytes, None))
-            ks = host[:P].view(np.int64).copy()
-            sv = host[P:]
-            svals = [sv[int(s_off[j]):int(s_off[j]) + int(ks[j])].copy() for j in range(P)]
+            # the download is queued behind the SVD's last kernels
+            L.check(L.lib.tenet_memcpy_async(ctx.h, Pp(stg.ptr), Pp(kd), C.c_size_t(nk * 8), 1, None), "d2h")
+            guess = self._kept.get((tuple(bonds), shp, md, thr))
+            if guess is None:
+                guess = tuple(int(min(k, md)) if md > 0 else int(k) for k in kmax)
+            out = self._out_step(shp, kmax, guess, tt, Vs)
+            ctx.sync()
+            host = stg.view(np.float64, nk)
+            ks = tuple(int(k) for k in host[:P].view(np.int64))
+            svals = [host[P + int(s_off[j]):P + int(s_off[j]) + ks[j]].copy() for j in range(P)]
         finally:
             ctx.free(kd)
-        for (l, d1, b, d2, r), m, n in zip(shp, ms, ns):
+        if ks != guess:
+            out = self._out_step(shp, kmax, ks, tt, Vs)
+        self._kept[(tuple(bonds), shp, md, thr)] = ks
+        newA, newB, plan, ptrs = out
+        plan.exec(ptrs)
+        for (l, d1, b, d2, r), m, n, k in zip(shp, ms, ns, ks):
             self.flops["svd"] += svd_flops(int(m), int(n))
-            self.flops["contract"] += 8.0 * l * d1 * b * d2 * r + 8.0 * l * d1 * d2 * r * d1 * d2
-        # 5. B_a <- theta~ conj(V^H[:k])^T ; B_{a+1} <- V^H[:k] (strided view of V^H)
-        newA = [DeviceArray(ctx, (l, d1, int(k)), dt) for (l, d1, b, d2, r), k in zip(shp, ks)]
-        newB = [DeviceArray(ctx, (int(k), d2, r), dt) for (l, d1, b, d2, r), k in zip(shp, ks)]
-        probs, conj, ptrs = [], [], []
-        for j, ((l, d1, b, d2, r), k, km) in enumerate(zip(shp, ks, kmax)):
-            k, km = int(k), int(km)
-            vst = (1, km, km * d2)
-            probs.append(((l, d1, d2, r), "lpqr", (k, d2, r), "kqr", (l, d1, k), "lpk", None, vst))
-            conj.append((0, 1))
-            ptrs += [tt[j].ptr, Vs[j].ptr, newA[j].ptr]
-            probs.append(((k, d2, r), "kqr", (), "", (k, d2, r), "kqr", vst, None))
-            conj.append((0, 0))
-            ptrs += [Vs[j].ptr, self._one.ptr, newB[j].ptr]
-        key = ("out", tuple(shp), tuple(int(k) for k in ks))
-        self._plan(key, probs, conj).exec(ptrs)
-        self.flops["contract"] += sum(8.0 * l * d1 * d2 * r * int(k) for (l, d1, b, d2, r), k in zip(shp, ks))
+            self.flops["contract"] += 8.0 * l * d1 * b * d2 * r + 8.0 * l * d1 * d2 * r * d1 * d2 + \
+                8.0 * l * d1 * d2 * r * k
         for j, a in enumerate(bonds):
             self.B[a] = newA[j]
             self.B[a + 1] = newB[j]
             self.lam_host[a] = svals[j]
-        fresh = self._upload_lams([svals[j] for j in range(P)])
-        for j, a in enumerate(bonds):
-            self.lam[a] = fresh[j]
+        # the new Lambdas: complex copies of the kept S, uploaded from the
+        # staging buffer behind the output step (no wait)
+        tot = sum(ks)
+        if tot:
+            lam = stg.view(dt, tot, offset=lam_off)
+            lam[:] = np.concatenate(svals)
+            owner = DeviceArray(ctx, (tot,), dt)
+            L.check(L.lib.tenet_memcpy_async(ctx.h, Pp(owner.ptr), Pp(stg.ptr + lam_off),
+                                             C.c_size_t(tot * dt.itemsize), 0, None), "h2d")
+            off = 0
+            for j, a in enumerate(bonds):
+                v = DeviceArray(ctx, (ks[j],), dt, ptr=owner.ptr + off * dt.itemsize, owner=False)
+                v._owner = owner
+                self.lam[a] = v
+                off += ks[j]
+        else:
+            for j, a in enumerate(bonds):
+                self.lam[a] = DeviceArray(ctx, (0,), dt)
         del Vs, th, tt, X
-        return [int(k) for k in ks]
+        return list(ks)
 
     # ---- graph-captured layers -------------------------------------------------
     def layer_signature(self, bonds):
