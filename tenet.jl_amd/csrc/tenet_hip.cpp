@@ -2421,12 +2421,13 @@ struct FlagSlotGuard {
 
 // One grouped run.  Eagerly: sweeps until every problem converges, then
 // problems left unconverged go through the single-matrix path.  The flags of
-// sweep s are copied to pinned memory behind it and read while sweep s + 1 is
-// already queued, so the stream never idles on the host's convergence test;
-// the sweep queued after the last one needed costs only its launches (rounds
-// of converged problems return at once).  Returns without waiting for the
-// final kernels.  Under capture: a fixed CAPTURED_SWEEPS sweeps and the
-// context's device status bit for problems they did not finish.
+// sweep s are copied to pinned memory behind it and read while the fixed-pair
+// round of sweep s + 1 is already queued, so the stream does not idle on the
+// host's convergence test; after the last sweep needed that one round is
+// launched in vain (it returns at once for converged problems).  Returns
+// without waiting for the final kernels.  Under capture: a fixed
+// CAPTURED_SWEEPS sweeps and the context's device status bit for problems
+// they did not finish.
 static int gsvd_run(tenet_gsvd g, hipStream_t s) {
   tenet_ctx ctx = g->ctx;
   const int64_t count = g->count;
@@ -2453,70 +2454,101 @@ static int gsvd_run(tenet_gsvd g, hipStream_t s) {
     *f = 8.0 * sh.npairs * n2 * n2 * (2.0 * sh.nr + vr);
     *b = 16.0 * sh.npairs * n2 * (3.0 * sh.nr + 2.0 * vr);
   };
-  bool all = false;
-  int inflight = -1;  // sweep whose flags copy is queued and not yet read
-  const int max_sweeps = sync_ok ? 40 : CAPTURED_SWEEPS;
-  for (int sw = 0; sw < max_sweeps; ++sw) {
-    double fl = 0.0, by = 0.0;
-    std::vector<double> flr((size_t)g->max_rounds, 0.0), byr((size_t)g->max_rounds, 0.0);
+  // The work of the fixed-pair round (f0, b0) and of each cross round over
+  // the problems not known to have converged
+  double f0 = 0.0, b0 = 0.0;
+  std::vector<double> flr((size_t)g->max_rounds, 0.0), byr((size_t)g->max_rounds, 0.0);
+  auto tally = [&]() {
+    f0 = b0 = 0.0;
+    std::fill(flr.begin(), flr.end(), 0.0);
+    std::fill(byr.begin(), byr.end(), 0.0);
     for (int64_t i = 0; i < count; ++i) {
       if (hflags[(size_t)(8 * i + 1)]) continue;
       const SvdShape& sh = g->shapes[(size_t)i];
       double f, b;
       unit_cost(i, &f, &b);
-      fl += f;
-      by += b;
+      f0 += f;
+      b0 += b;
       if (sh.nblk > 2)
         for (int64_t r = 0; r < sh.nblk - 1; ++r) {
           flr[(size_t)r] += f;
           byr[(size_t)r] += b;
         }
     }
+  };
+  auto round0 = [&]() {
     SvdBatch Bu = B;
     Bu.units = g->units_dev + g->uoff[0];
-    TRY(svd_round_timed(ctx, s, fl, by, Bu, count, (int)g->ucnt[0], 0, 2, 0));
+    return svd_round_timed(ctx, s, f0, b0, Bu, count, (int)g->ucnt[0], 0, 2, 0);
+  };
+  // the cross rounds, the sweep's end and (eagerly) its flags copy
+  auto rest = [&](int sw) -> int {
+    SvdBatch Bu = B;
     for (int r = 0; r < g->max_rounds; ++r) {
       if (g->ucnt[(size_t)r + 1] == 0) continue;
       Bu.units = g->units_dev + g->uoff[(size_t)r + 1];
       TRY(svd_round_timed(ctx, s, flr[(size_t)r], byr[(size_t)r], Bu, count, (int)g->ucnt[(size_t)r + 1], r, 2, 1));
     }
     TRY(launch_svd_sweep_end(B, count, s));
-    if (!sync_ok) continue;
-    int32_t* land = slot.f->host + (size_t)(sw & 1) * count * 8;
-    TENET_HIP_CHECK(hipMemcpyAsync(land, g->flags_base, fbytes, hipMemcpyDeviceToHost, s));
-    TENET_HIP_CHECK(hipEventRecord(slot.f->ev[sw & 1], s));
-    const int prev = inflight;
-    inflight = sw;
-    if (prev < 0) continue;
-    TENET_HIP_CHECK(hipEventSynchronize(slot.f->ev[prev & 1]));
-    const int32_t* nf = slot.f->host + (size_t)(prev & 1) * count * 8;
-    // problems that converged in sweep prev were counted in this sweep's
-    // launches, whose rounds returned at once for them: take their share
-    // back out of the round statistics
-    double dfl = 0.0, dby = 0.0;
-    all = true;
+    if (sync_ok) {
+      TENET_HIP_CHECK(hipMemcpyAsync(slot.f->host + (size_t)(sw & 1) * count * 8, g->flags_base, fbytes,
+                                     hipMemcpyDeviceToHost, s));
+      TENET_HIP_CHECK(hipEventRecord(slot.f->ev[sw & 1], s));
+    }
+    return TENET_OK;
+  };
+  // Read the flags of sweep sw into hflags: 1 when every problem has
+  // converged, else 0 (or an error).  *dfl / *dby: the fixed-pair round's
+  // share of the problems that converged in sweep sw.
+  auto read_flags = [&](int sw, double* dfl, double* dby) -> int {
+    TENET_HIP_CHECK(hipEventSynchronize(slot.f->ev[sw & 1]));
+    const int32_t* nf = slot.f->host + (size_t)(sw & 1) * count * 8;
+    *dfl = *dby = 0.0;
+    bool a = true;
     for (int64_t i = 0; i < count; ++i) {
       const bool was = hflags[(size_t)(8 * i + 1)] != 0, now = nf[8 * i + 1] != 0;
       if (!was && now) {
-        const SvdShape& sh = g->shapes[(size_t)i];
         double f, b;
         unit_cost(i, &f, &b);
-        const double rounds = 1.0 + (sh.nblk > 2 ? (double)(sh.nblk - 1) : 0.0);
-        dfl += f * rounds;
-        dby += b * rounds;
+        *dfl += f;
+        *dby += b;
       }
-      all = all && now;
+      a = a && now;
     }
     std::memcpy(hflags.data(), nf, fbytes);
-    stats_retract(ctx, TENET_KSTAT_SVD_ROUND, s, dfl, dby);
-    if (all) break;
+    return a ? 1 : 0;
+  };
+  bool all = false;
+  const int max_sweeps = sync_ok ? 40 : CAPTURED_SWEEPS;
+  tally();
+  TRY(round0());
+  TRY(rest(0));
+  int sw = 1;
+  for (; sw < max_sweeps; ++sw) {
+    // the next sweep's fixed-pair round keeps the stream busy while the host
+    // reads the previous sweep's flags
+    TRY(round0());
+    if (sync_ok) {
+      double dfl, dby;
+      const int a = read_flags(sw - 1, &dfl, &dby);
+      if (a < 0) return a;
+      // that round returned at once for problems that converged in sweep
+      // sw - 1, though it counted them
+      stats_retract(ctx, TENET_KSTAT_SVD_ROUND, s, dfl, dby);
+      if (a == 1) {
+        all = true;
+        break;
+      }
+      tally();
+    }
+    TRY(rest(sw));
   }
-  if (sync_ok && !all && inflight >= 0) {
+  if (sync_ok && !all) {
     // the sweep limit ran out: the last sweep's flags decide
-    TENET_HIP_CHECK(hipEventSynchronize(slot.f->ev[inflight & 1]));
-    std::memcpy(hflags.data(), slot.f->host + (size_t)(inflight & 1) * count * 8, fbytes);
-    all = true;
-    for (int64_t i = 0; i < count; ++i) all = all && hflags[(size_t)(8 * i + 1)] != 0;
+    double dfl, dby;
+    const int a = read_flags(sw - 1, &dfl, &dby);
+    if (a < 0) return a;
+    all = a == 1;
   }
   TRY(launch_svd_finalize(B, count, (int)g->max_ncp, g->max_out, g->dtype, s));
   if (!sync_ok) return launch_svd_status(B, count, ctx->dev_status, s);
