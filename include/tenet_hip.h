@@ -372,6 +372,22 @@ int tenet_sample_select(tenet_ctx ctx, tenet_stream stream, int dtype, const voi
  * of U sqrt(S) is U times sqrt(S), up to diagonal phases). */
 int tenet_sigma_scales(tenet_ctx ctx, tenet_stream stream, int dtype, int64_t k, const double* S, void* sqrt_out,
                        void* inv_sqrt_out);
+/* Two-site gates on `count` bonds at once: the gate contraction inside
+ * Muscle.simple_update (simple_update.jl:53-64, the contraction of the gate's
+ * inputs with theta) batched over the disjoint bonds of a brickwork layer,
+ * fused with the Vidal-gauge scaling by the left bond's Lambda.  Per
+ * problem i (column-major, d = physical dimension, 2..4):
+ *   theta_out[l, p, q, r] = sum_{x, y} G[p, q, x, y] theta[l, x, y, r]
+ * with theta (chil x d x d x chir) and G (d x d x d x d, G[oa, ob, ia, ib]:
+ * the 4x4 U4[d oa + ob, d ia + ib] of simple_update!), and, when lams[i] is
+ * set, x_out[l, p, q, r] = lams[i][l] theta_out[l, p, q, r] (lams[i] a
+ * vector of chil values of the dtype).  lams and x_out may be NULL, or hold
+ * NULL entries (no X for that problem; both or neither).  x_out[i] may equal
+ * theta[i] (computed in place); theta_out[i] must not overlap theta[i].
+ * Asynchronous; the problem table travels in the launch (capturable). */
+int tenet_grouped_gate(tenet_ctx ctx, tenet_stream stream, int dtype, int64_t count, int d, const int64_t* chil,
+                       const int64_t* chir, const void* const* theta, const void* const* gates,
+                       const void* const* lams, void* const* theta_out, void* const* x_out);
 /* out (device double) = sum |x_i|^2 in FP64 (norm(tensor), norm.jl:28) */
 int tenet_nrm2sq(tenet_ctx ctx, tenet_stream stream, int dtype, int64_t count, const void* x, double* out);
 

@@ -250,5 +250,24 @@ int launch_sample_select(const void* T, int dtype, int d, int chir, int64_t nsam
                          int32_t* bits, void* Vn, hipStream_t s);
 int launch_nrm2(const void* x, int dtype, int64_t count, double* part, double* out, hipStream_t s);
 int launch_sigma_scales(const double* S, int64_t k, void* sq, void* isq, int dtype, hipStream_t s);
+// Two-site gates on a batch of bonds (passed by value: no table upload, so
+// the launch can be captured): per problem theta~[l, p, q, r] =
+// sum_{x, y} G[p, q, x, y] theta[l, x, y, r] and, with lam set,
+// X[l, p, q, r] = lam[l] theta~[l, p, q, r] (column-major, d x d gate).
+struct GateProb {
+  const void* theta;
+  const void* gate;
+  const void* lam;  // may be null (no X)
+  void* theta_out;  // must not overlap theta
+  void* x_out;      // may equal theta (each element is read before it is written, by the same thread)
+  int64_t chil, chir;
+};
+constexpr int GATE_BATCH = 32;
+struct GateBatch {
+  GateProb p[GATE_BATCH];
+  int32_t count;
+  int32_t d;
+};
+int launch_gate_apply(const GateBatch& b, int64_t max_elems, int dtype, hipStream_t s);
 
 }  // namespace tenet

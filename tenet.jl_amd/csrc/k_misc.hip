@@ -952,4 +952,86 @@ int launch_sigma_scales(const double* S, int64_t k, void* sq, void* isq, int dty
   return TENET_OK;
 }
 
+// Two-site gate application (the gate contraction of a simple update,
+// batched over the disjoint bonds of a brickwork layer).  One thread per
+// (l, r) of a problem reads the d^2 entries theta[l, :, :, r] once and
+// writes all d^2 outputs, so theta leaves memory once; for fixed (x, y)
+// consecutive threads touch consecutive l (coalesced).  The gate is held in
+// registers.  R: the accumulation type (double for complex128, float for
+// complex64, as the library's other complex64 contractions).
+template <typename S2, typename R, int D>
+__global__ void gate_apply_kernel(GateBatch B) {
+  const GateProb& P = B.p[blockIdx.y];
+  const int64_t n = P.chil * P.chir;
+  const S2* __restrict__ th = static_cast<const S2*>(P.theta);
+  const S2* __restrict__ G = static_cast<const S2*>(P.gate);
+  const S2* lam = static_cast<const S2*>(P.lam);
+  S2* __restrict__ to = static_cast<S2*>(P.theta_out);
+  S2* xo = static_cast<S2*>(P.x_out);
+  R gr[D * D * D * D], gi[D * D * D * D];
+#pragma unroll
+  for (int e = 0; e < D * D * D * D; ++e) {
+    const S2 g = G[e];
+    gr[e] = (R)g.x;
+    gi[e] = (R)g.y;
+  }
+  const int64_t ld = P.chil;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t l = e % ld, r = e / ld;
+    const int64_t base = l + ld * D * D * r;
+    R tr[D * D], ti[D * D];
+#pragma unroll
+    for (int xy = 0; xy < D * D; ++xy) {  // xy = x + D y
+      const S2 t = th[base + ld * xy];
+      tr[xy] = (R)t.x;
+      ti[xy] = (R)t.y;
+    }
+    R lr = (R)1, li = (R)0;
+    if (xo) {
+      const S2 a = lam[l];
+      lr = (R)a.x;
+      li = (R)a.y;
+    }
+#pragma unroll
+    for (int pq = 0; pq < D * D; ++pq) {  // pq = p + D q; G[p, q, x, y] at pq + D^2 xy
+      R ar = (R)0, ai = (R)0;
+#pragma unroll
+      for (int xy = 0; xy < D * D; ++xy) {
+        const R a = gr[pq + D * D * xy], b = gi[pq + D * D * xy];
+        ar = fma(a, tr[xy], fma(-b, ti[xy], ar));
+        ai = fma(a, ti[xy], fma(b, tr[xy], ai));
+      }
+      S2 o;
+      o.x = ar;
+      o.y = ai;
+      to[base + ld * pq] = o;
+      if (xo) {
+        S2 x;
+        x.x = lr * ar - li * ai;
+        x.y = lr * ai + li * ar;
+        xo[base + ld * pq] = x;
+      }
+    }
+  }
+}
+
+template <typename S2, typename R>
+static int launch_gate_t(const GateBatch& b, int64_t max_elems, hipStream_t s) {
+  const unsigned gx = (unsigned)std::max<int64_t>(1, std::min<int64_t>((max_elems + 255) / 256, 4096));
+  const dim3 grid(gx, (unsigned)b.count);
+  switch (b.d) {
+    case 2: hipLaunchKernelGGL((gate_apply_kernel<S2, R, 2>), grid, dim3(256), 0, s, b); break;
+    case 3: hipLaunchKernelGGL((gate_apply_kernel<S2, R, 3>), grid, dim3(256), 0, s, b); break;
+    case 4: hipLaunchKernelGGL((gate_apply_kernel<S2, R, 4>), grid, dim3(256), 0, s, b); break;
+    default: set_error("gate: physical dimension outside 2..4"); return TENET_ERR_UNSUPPORTED;
+  }
+  TENET_HIP_CHECK(hipGetLastError());
+  return TENET_OK;
+}
+
+int launch_gate_apply(const GateBatch& b, int64_t max_elems, int dtype, hipStream_t s) {
+  if (b.count <= 0 || max_elems <= 0) return TENET_OK;
+  return dtype == TENET_C128 ? launch_gate_t<double2, double>(b, max_elems, s) : launch_gate_t<float2, float>(b, max_elems, s);
+}
+
 }  // namespace tenet

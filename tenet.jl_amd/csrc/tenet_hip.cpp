@@ -2693,6 +2693,51 @@ extern "C" int tenet_sigma_scales(tenet_ctx ctx, tenet_stream stream, int dtype,
   return launch_sigma_scales(S, k, sqrt_out, inv_sqrt_out, dtype, pick(ctx, stream));
 }
 
+extern "C" int tenet_grouped_gate(tenet_ctx ctx, tenet_stream stream, int dtype, int64_t count, int d,
+                                  const int64_t* chil, const int64_t* chir, const void* const* theta,
+                                  const void* const* gates, const void* const* lams, void* const* theta_out,
+                                  void* const* x_out) {
+  TRY(use_device(ctx));
+  TRY(valid_dtype_prec(dtype, 0));
+  CHECK_ARG(count >= 0, "bad count");
+  if (count == 0) return TENET_OK;
+  CHECK_ARG(chil && chir && theta && gates && theta_out, "null argument");
+  CHECK_ARG(d >= 2 && d <= 4, "gate: physical dimension outside 2..4");
+  const size_t es = ELEM[dtype];
+  hipStream_t s = pick(ctx, stream);
+  GateBatch b{};
+  b.d = d;
+  int64_t max_elems = 0;
+  for (int64_t i0 = 0; i0 < count; i0 += GATE_BATCH) {
+    b.count = (int32_t)std::min<int64_t>(GATE_BATCH, count - i0);
+    max_elems = 0;
+    for (int j = 0; j < b.count; ++j) {
+      const int64_t i = i0 + j;
+      CHECK_ARG(chil[i] > 0 && chir[i] > 0, "gate: bad bond dimension");
+      const void* lam = lams ? lams[i] : nullptr;
+      void* xo = x_out ? x_out[i] : nullptr;
+      CHECK_ARG((lam == nullptr) == (xo == nullptr), "gate: Lambda and X come together");
+      const size_t nb = (size_t)chil[i] * d * d * chir[i] * es;
+      TRY(check_span(ctx, theta[i], nb, "theta"));
+      TRY(check_span(ctx, gates[i], (size_t)d * d * d * d * es, "gate"));
+      TRY(check_span(ctx, theta_out[i], nb, "theta_out"));
+      const uintptr_t t0 = (uintptr_t)theta[i], o0 = (uintptr_t)theta_out[i];
+      CHECK_ARG(o0 + nb <= t0 || t0 + nb <= o0, "gate: theta_out overlaps theta");
+      if (xo) {
+        TRY(check_span(ctx, xo, nb, "X"));
+        TRY(check_span(ctx, lam, (size_t)chil[i] * es, "Lambda"));
+        const uintptr_t x0 = (uintptr_t)xo;
+        CHECK_ARG(x0 == t0 || x0 + nb <= t0 || t0 + nb <= x0, "gate: X partially overlaps theta");
+        CHECK_ARG(x0 + nb <= o0 || o0 + nb <= x0, "gate: X overlaps theta_out");
+      }
+      b.p[j] = GateProb{theta[i], gates[i], lam, theta_out[i], xo, chil[i], chir[i]};
+      max_elems = std::max(max_elems, chil[i] * chir[i]);
+    }
+    TRY(launch_gate_apply(b, max_elems, dtype, s));
+  }
+  return TENET_OK;
+}
+
 extern "C" int tenet_nrm2sq(tenet_ctx ctx, tenet_stream stream, int dtype, int64_t count, const void* x,
                             double* out) {
   TRY(use_device(ctx));

@@ -137,6 +137,8 @@ SIGNATURES = {
     "tenet_scale": [_p, _p, _i, _i64, _p, _pd],
     "tenet_nrm2sq": [_p, _p, _i, _i64, _p, _p],
     "tenet_sigma_scales": [_p, _p, _i, _i64, _p, _p, _p],
+    "tenet_grouped_gate": [_p, _p, _i, _i64, _i, _pi64, _pi64, C.POINTER(_p), C.POINTER(_p), C.POINTER(_p),
+                           C.POINTER(_p), C.POINTER(_p)],
     "tenet_graph_begin": [_p, _p],
     "tenet_graph_end": [_p, _p, C.POINTER(_p)],
     "tenet_graph_launch": [_p, _p],

@@ -20,11 +20,13 @@ gate on (a, a+1):
 
 (B_a = Lambda_{a-1}^{-1} U S without dividing by Lambda).  The gates of a
 brickwork layer act on disjoint bonds and commute, so ``apply_layer_`` runs
-a layer as four grouped contraction launches and one grouped SVD:
+a layer as three grouped launches (two contractions and the fused gate) and
+one grouped SVD:
 
   1. theta_k = B_a B_{a+1}                 (contract the bond)
   2. theta~_k = theta_k . G_k              (contract the gate inputs)
-  3. X_k = Lambda_{a-1} (.) theta~_k       (Hadamard; at a = 0 X = theta~)
+  3. X_k = Lambda_{a-1} (.) theta~_k       (Hadamard; at a = 0 X = theta~;
+                                            2 and 3 in one tenet_grouped_gate launch)
   4. grouped one-sided Jacobi SVD of X_k, truncation on the device
   5. B_a <- theta~_k conj(V^H[:k])^T and B_{a+1} <- V^H[:k] (a conjugated
      contraction and a strided copy, one launch)
@@ -214,21 +216,11 @@ class VidalMPS:
         p1 = self._plan(("theta", shp), lambda: ([((l, d1, b), "lxb", (b, d2, r), "byr", (l, d1, d2, r), "lxyr")
                                                   for (l, d1, b, d2, r) in shp], None))
         p1.exec([x for j, a in enumerate(bonds) for x in (self.B[a].ptr, self.B[a + 1].ptr, th[j].ptr)])
-        # 2. theta~ = theta . G   (G[oa, ob, ia, ib])
+        # 2. theta~ = theta . G (G[oa, ob, ia, ib]) and 3. X = Lambda_{a-1} (.) theta~,
+        # one launch; X in theta's memory (theta is no longer needed)
         tt = carve(ctx, [(l, d1, d2, r) for (l, d1, b, d2, r) in shp], dt)
-        p2 = self._plan(("gate", shp), lambda: ([((l, d1, d2, r), "lxyr", (d1, d2, d1, d2), "pqxy",
-                                                  (l, d1, d2, r), "lpqr") for (l, d1, b, d2, r) in shp], None))
-        p2.exec([x for j in range(P) for x in (th[j].ptr, Gs[j].ptr, tt[j].ptr)])
-        # 3. X = Lambda_{a-1} (.) theta~
-        idx = [j for j, a in enumerate(bonds) if a > 0]
-        X = list(tt)
-        if idx:
-            for j in idx:
-                X[j] = th[j]  # theta is no longer needed: reuse its memory
-            sub = tuple(shp[j] for j in idx)
-            p3 = self._plan(("lam", sub), lambda: ([((s[0],), "l", s[:2] + s[3:], "lpqr", s[:2] + s[3:], "lpqr")
-                                                   for s in sub], None))
-            p3.exec([x for j in idx for x in (self.lam[bonds[j] - 1].ptr, tt[j].ptr, X[j].ptr)])
+        X = [th[j] if a > 0 else tt[j] for j, a in enumerate(bonds)]
+        _gate_step(self, shp, bonds, th, Gs, tt)
         # 4. grouped SVD of X as (l d) x (d r)
         ms = np.array([l * d1 for (l, d1, b, d2, r) in shp], dtype=np.int64)
         ns = np.array([d2 * r for (l, d1, b, d2, r) in shp], dtype=np.int64)
@@ -377,6 +369,48 @@ class VidalMPS:
         self._plans = {}
 
 
+def _gate_dim(shp):
+    """The common physical dimension of a layer's bonds when the fused gate
+    launch applies (every d1 = d2 = d with 2 <= d <= 4), else None."""
+    ds = {s[1] for s in shp} | {s[3] for s in shp}
+    d = next(iter(ds))
+    return d if len(ds) == 1 and 2 <= d <= 4 else None
+
+
+def _gate_fused(v: VidalMPS, shp, bonds, th_ptrs, gate_ptrs, tt_ptrs):
+    """theta~_j = theta_j . G_j into tt_ptrs[j] and, for a > 0, X_j =
+    Lambda_{a-1} (.) theta~_j into th_ptrs[j] (in place), one
+    tenet_grouped_gate launch (capturable: no table upload)."""
+    ctx, P = v.ctx, len(bonds)
+    Pp = C.c_void_p
+    chil = np.array([s[0] for s in shp], dtype=np.int64)
+    chir = np.array([s[4] for s in shp], dtype=np.int64)
+    lam = [v.lam[a - 1].ptr if a > 0 else None for a in bonds]
+    xo = [th_ptrs[j] if a > 0 else None for j, a in enumerate(bonds)]
+    L.check(L.lib.tenet_grouped_gate(
+        ctx.h, None, v.code, P, _gate_dim(shp), chil.ctypes.data_as(C.POINTER(C.c_int64)),
+        chir.ctypes.data_as(C.POINTER(C.c_int64)), (Pp * P)(*th_ptrs), (Pp * P)(*gate_ptrs), (Pp * P)(*lam),
+        (Pp * P)(*tt_ptrs), (Pp * P)(*xo)), "grouped_gate")
+
+
+def _gate_step(v: VidalMPS, shp, bonds, th, gates, tt):
+    """Steps 2 and 3 of a layer: the fused launch, or (bonds of mixed or
+    larger physical dimensions) the two grouped contractions."""
+    P = len(bonds)
+    if _gate_dim(shp) is not None:
+        _gate_fused(v, shp, bonds, [x.ptr for x in th], [g.ptr for g in gates], [x.ptr for x in tt])
+        return
+    p2 = v._plan(("gate", shp), lambda: ([((l, d1, d2, r), "lxyr", (d1, d2, d1, d2), "pqxy", (l, d1, d2, r), "lpqr")
+                                          for (l, d1, b, d2, r) in shp], None))
+    p2.exec([x for j in range(P) for x in (th[j].ptr, gates[j].ptr, tt[j].ptr)])
+    idx = [j for j, a in enumerate(bonds) if a > 0]
+    if idx:
+        sub = tuple(shp[j] for j in idx)
+        p3 = v._plan(("lam", sub), lambda: ([((s[0],), "l", s[:2] + s[3:], "lpqr", s[:2] + s[3:], "lpqr")
+                                             for s in sub], None))
+        p3.exec([x for j in idx for x in (v.lam[bonds[j] - 1].ptr, tt[j].ptr, th[j].ptr)])
+
+
 def _leading_rows(v: VidalMPS, M: DeviceArray, k: int) -> DeviceArray:
     """Contiguous copy of the first k rows of a column-major matrix."""
     m, n = M.shape
@@ -421,19 +455,22 @@ class _LayerGraph:
         p1 = v._plan(("theta", tuple(shp)), [((l, d1, b), "lxb", (b, d2, r), "byr", (l, d1, d2, r), "lxyr")
                                              for (l, d1, b, d2, r) in shp])
         t1 = [x for j, a in enumerate(bonds) for x in (v.B[a].ptr, v.B[a + 1].ptr, th[j].ptr)]
-        p2 = v._plan(("gate", tuple(shp)), [((l, d1, d2, r), "lxyr", (d1, d2, d1, d2), "pqxy", (l, d1, d2, r),
-                                             "lpqr") for (l, d1, b, d2, r) in shp])
-        t2 = [x for j in range(P) for x in (th[j].ptr, self.stage.ptr + j * self.gsize * es, tt[j].ptr)]
+        gate_ptrs = [self.stage.ptr + j * self.gsize * es for j in range(P)]
         idx = [j for j, a in enumerate(bonds) if a > 0]
         X = list(tt)
-        p3 = t3 = None
-        if idx:
-            for j in idx:
-                X[j] = th[j]
-            p3 = v._plan(("lam", tuple(shp[j] for j in idx)),
-                         [((shp[j][0],), "l", shp[j][:2] + shp[j][3:], "lpqr", shp[j][:2] + shp[j][3:], "lpqr")
-                          for j in idx])
-            t3 = [x for j in idx for x in (v.lam[bonds[j] - 1].ptr, tt[j].ptr, X[j].ptr)]
+        for j in idx:
+            X[j] = th[j]
+        fused = _gate_dim(tuple(shp)) is not None
+        p2 = p3 = t2 = t3 = None
+        if not fused:
+            p2 = v._plan(("gate", tuple(shp)), [((l, d1, d2, r), "lxyr", (d1, d2, d1, d2), "pqxy", (l, d1, d2, r),
+                                                 "lpqr") for (l, d1, b, d2, r) in shp])
+            t2 = [x for j in range(P) for x in (th[j].ptr, gate_ptrs[j], tt[j].ptr)]
+            if idx:
+                p3 = v._plan(("lam", tuple(shp[j] for j in idx)),
+                             [((shp[j][0],), "l", shp[j][:2] + shp[j][3:], "lpqr", shp[j][:2] + shp[j][3:], "lpqr")
+                              for j in idx])
+                t3 = [x for j in idx for x in (v.lam[bonds[j] - 1].ptr, tt[j].ptr, X[j].ptr)]
         # 4. the grouped SVD, bound once: S as complex into the Lambda buffers
         self.kd = ctx.malloc(P * 8)
         Pp = C.c_void_p
@@ -470,9 +507,12 @@ class _LayerGraph:
             self.tables.append(dev)
         with ctx.capture() as g:
             p1.exec_dev(self.tables[0])
-            p2.exec_dev(self.tables[1])
-            if p3 is not None:
-                p3.exec_dev(self.tables[2])
+            if fused:
+                _gate_fused(v, tuple(shp), bonds, [x.ptr for x in th], gate_ptrs, [x.ptr for x in tt])
+            else:
+                p2.exec_dev(self.tables[1])
+                if p3 is not None:
+                    p3.exec_dev(self.tables[2])
             L.check(L.lib.tenet_grouped_svd_exec(self.svd, None), "grouped_svd_exec")
             p5.exec_dev(self.tables[3])
         self.graph = g

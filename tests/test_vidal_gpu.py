@@ -150,3 +150,78 @@ def test_vidal_sweep_with_captured_layers_equals_eager(ctx):
     assert _rel(hv.to_mps().to_dense(), ref.to_mps().to_dense()) < 1e-10
     graph.close()
     eager.close()
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.complex128, 1e-13), (np.complex64, 3e-6)])
+@pytest.mark.parametrize("d", [2, 3, 4])
+def test_grouped_gate_against_oracle(ctx, dtype, tol, d):
+    """tenet_grouped_gate (the gate contraction of simple_update, batched and
+    fused with the Lambda scaling) against the oracle's contraction and
+    Hadamard product: 37 problems (two launch batches), ragged bonds
+    including 1, every third without Lambda, every other X computed in place
+    over theta."""
+    import ctypes as C
+
+    import tenet_hip as T
+    from tenet_hip import _lib as L
+
+    rng = np.random.default_rng(40 + d)
+    count = 37
+    shapes = [(int(rng.integers(1, 30)), int(rng.integers(1, 30))) for _ in range(count)]
+    shapes[0] = (1, 1)
+    ths = [O.complex_gaussian(rng, (l, d, d, r)).astype(dtype) for l, r in shapes]
+    gs = [O.complex_gaussian(rng, (d, d, d, d)).astype(dtype) for _ in range(count)]
+    lams = [O.complex_gaussian(rng, (l,)).astype(dtype) if i % 3 else None for i, (l, r) in enumerate(shapes)]
+    th_dev = [T.DeviceArray.from_host(ctx, x, dtype) for x in ths]
+    g_dev = [T.DeviceArray.from_host(ctx, x, dtype) for x in gs]
+    lam_dev = [T.DeviceArray.from_host(ctx, x, dtype) if x is not None else None for x in lams]
+    out_dev = [T.DeviceArray(ctx, x.shape, dtype) for x in ths]
+    x_dev = []
+    for i, x in enumerate(ths):
+        if lams[i] is None:
+            x_dev.append(None)
+        elif i % 2:
+            x_dev.append(th_dev[i])  # in place
+        else:
+            x_dev.append(T.DeviceArray(ctx, x.shape, dtype))
+    Pp = C.c_void_p
+    chil = np.array([s[0] for s in shapes], dtype=np.int64)
+    chir = np.array([s[1] for s in shapes], dtype=np.int64)
+    L.check(L.lib.tenet_grouped_gate(
+        ctx.h, None, L.C128 if dtype == np.complex128 else L.C64, count, d,
+        chil.ctypes.data_as(C.POINTER(C.c_int64)), chir.ctypes.data_as(C.POINTER(C.c_int64)),
+        (Pp * count)(*[x.ptr for x in th_dev]), (Pp * count)(*[x.ptr for x in g_dev]),
+        (Pp * count)(*[x.ptr if x is not None else None for x in lam_dev]),
+        (Pp * count)(*[x.ptr for x in out_dev]), (Pp * count)(*[x.ptr if x is not None else None for x in x_dev])),
+        "grouped_gate")
+    for i in range(count):
+        ref = O.binary_einsum(O.OTensor(gs[i].astype(np.complex128), ["p", "q", "x", "y"]),
+                              O.OTensor(ths[i].astype(np.complex128), ["l", "x", "y", "r"]), dims=["x", "y"])
+        ref = np.transpose(ref.data, (2, 0, 1, 3))  # (p, q, l, r) -> (l, p, q, r)
+        assert _rel(out_dev[i].to_host(), ref) < tol, i
+        if lams[i] is not None:
+            xref = O.binary_einsum(O.OTensor(lams[i].astype(np.complex128), ["l"]),
+                                   O.OTensor(ref, ["l", "p", "q", "r"]), dims=[]).data
+            assert _rel(x_dev[i].to_host(), xref) < tol, i
+
+
+def test_grouped_gate_rejects_overlap(ctx):
+    """theta_out may not overlap theta; X and Lambda come together."""
+    import ctypes as C
+
+    import tenet_hip as T
+    from tenet_hip import _lib as L
+
+    th = T.DeviceArray(ctx, (4, 2, 2, 4), np.complex128).zero_()
+    g = T.DeviceArray(ctx, (2, 2, 2, 2), np.complex128).zero_()
+    lam = T.DeviceArray(ctx, (4,), np.complex128).zero_()
+    Pp = C.c_void_p
+    one = np.array([4], dtype=np.int64)
+    pi = one.ctypes.data_as(C.POINTER(C.c_int64))
+    st = L.lib.tenet_grouped_gate(ctx.h, None, L.C128, 1, 2, pi, pi, (Pp * 1)(th.ptr), (Pp * 1)(g.ptr),
+                                  (Pp * 1)(lam.ptr), (Pp * 1)(th.ptr), (Pp * 1)(th.ptr))
+    assert st == L.TENET_ERR_ARG
+    out = T.DeviceArray(ctx, (4, 2, 2, 4), np.complex128)
+    st = L.lib.tenet_grouped_gate(ctx.h, None, L.C128, 1, 2, pi, pi, (Pp * 1)(th.ptr), (Pp * 1)(g.ptr),
+                                  (Pp * 1)(lam.ptr), (Pp * 1)(out.ptr), (Pp * 1)(None))
+    assert st == L.TENET_ERR_ARG
