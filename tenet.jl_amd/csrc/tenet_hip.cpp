@@ -5,6 +5,7 @@
 #include "../../include/tenet_hip.h"
 
 #include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
 
 #include <algorithm>
 #include <cmath>
@@ -12,6 +13,7 @@
 #include <cstring>
 #include <list>
 #include <map>
+#include <tuple>
 #include <mutex>
 #include <string>
 #include <unordered_map>
@@ -147,6 +149,9 @@ struct tenet_ctx_s {
     hipEvent_t ev[2] = {nullptr, nullptr};
   };
   std::vector<FlagSlot*> flag_slots;
+  // library GEMM handle (created on first use; calls on it hold blas_mu)
+  rocblas_handle blas = nullptr;
+  std::mutex blas_mu;
   // pinned staging ring for small per-call table uploads (a copy from
   // pageable memory would wait for the stream to drain first)
   static constexpr int UP_RING = 8;
@@ -438,6 +443,7 @@ extern "C" int tenet_destroy(tenet_ctx ctx) {
     if (kv.second.ptr) (void)hipFree(kv.second.ptr);
   (void)hipHostFree(ctx->host_flag);
   (void)hipHostFree(ctx->host_k);
+  if (ctx->blas) (void)rocblas_destroy_handle(ctx->blas);
   for (int i = 0; i < tenet_ctx_s::UP_RING; ++i) {
     if (ctx->up_host[i]) (void)hipHostFree(ctx->up_host[i]);
     if (ctx->up_ev[i]) (void)hipEventDestroy(ctx->up_ev[i]);
@@ -1816,7 +1822,72 @@ struct tenet_gplan_s {
   double flops = 0.0, bytes = 0.0;
   int64_t shape_count[4] = {0, 0, 0, 0};  // 64, 32, 16 tiles; GROUPED_SMALLK
   int64_t split_tiles = 0;
+  // Problems that are plain column-major GEMMs (every operand a matrix with
+  // unit stride along rows or columns, no batch label), grouped by shape:
+  // tenet_grouped_exec runs each group as one batched library GEMM (rocBLAS)
+  // and the persistent kernel takes the other problems (units_rest).  The
+  // device-table exec (capturable) keeps every problem in the kernel.
+  struct LibGroup {
+    rocblas_operation opA, opB;
+    int64_t M, N, K, lda, ldb, ldc;
+    std::vector<int64_t> idx;
+    int64_t arr_off;  // first pointer of its A array after the 3 * count table (B, C follow)
+  };
+  std::vector<LibGroup> lib;
+  int64_t nlib = 0;
+  GroupedUnit* units_rest = nullptr;
+  int64_t nunits_rest = 0;
+  double flops_rest = 0.0, bytes_rest = 0.0;
+  int64_t slot_elems = 0;  // pointers per ring slot: 3 * count + 3 * nlib
 };
+
+// A problem's matricised operands as plain column-major GEMM operands, if
+// they are: op(A) (M x K), op(B) (K x N), C (M x N, unit stride along M).
+static bool plain_gemm(const HostPlan& hp, bool conjA, bool conjB, rocblas_operation* opA, int64_t* lda,
+                       rocblas_operation* opB, int64_t* ldb, int64_t* ldc) {
+  if (hp.batch != 1 || hp.M < 1 || hp.N < 1 || hp.K < 1) return false;
+  auto affine = [](const std::vector<int64_t>& v, int64_t* st) {
+    if (v.empty() || v[0] != 0) return false;
+    *st = v.size() > 1 ? v[1] : 0;
+    for (size_t i = 1; i < v.size(); ++i)
+      if (v[i] != (int64_t)i * *st) return false;
+    return true;
+  };
+  int64_t sM, sKa, sKb, sN, sMc, sNc;
+  if (!affine(hp.v[0], &sM) || !affine(hp.v[1], &sKa) || !affine(hp.v[3], &sKb) || !affine(hp.v[4], &sN) ||
+      !affine(hp.v[6], &sMc) || !affine(hp.v[7], &sNc))
+    return false;
+  if (hp.M == 1) sM = 1;
+  if (hp.K == 1) sKa = sKb = 1;
+  if (hp.N == 1) sN = 1;
+  if (hp.M == 1) sMc = 1;
+  // C: unit stride along M, columns ldc >= M apart
+  if (sMc != 1) return false;
+  *ldc = hp.N == 1 ? hp.M : sNc;
+  if (*ldc < hp.M) return false;
+  // A: M x K column-major (op N) or K x M column-major (op T / C)
+  if (sM == 1 && !conjA && (hp.K == 1 || sKa >= hp.M)) {
+    *opA = rocblas_operation_none;
+    *lda = hp.K == 1 ? hp.M : sKa;
+  } else if (sKa == 1 && (hp.M == 1 || sM >= hp.K)) {
+    *opA = conjA ? rocblas_operation_conjugate_transpose : rocblas_operation_transpose;
+    *lda = hp.M == 1 ? hp.K : sM;
+  } else {
+    return false;
+  }
+  // B: K x N column-major (op N) or N x K column-major (op T / C)
+  if (sKb == 1 && !conjB && (hp.N == 1 || sN >= hp.K)) {
+    *opB = rocblas_operation_none;
+    *ldb = hp.N == 1 ? hp.K : sN;
+  } else if (sN == 1 && (hp.K == 1 || sKb >= hp.N)) {
+    *opB = conjB ? rocblas_operation_conjugate_transpose : rocblas_operation_transpose;
+    *ldb = hp.K == 1 ? hp.N : sKb;
+  } else {
+    return false;
+  }
+  return *lda <= INT32_MAX && *ldb <= INT32_MAX && *ldc <= INT32_MAX && hp.M <= INT32_MAX && hp.N <= INT32_MAX &&
+         hp.K <= INT32_MAX;
+}
 
 static const int TILE_EDGE[3] = {64, 32, 16};
 
@@ -1909,6 +1980,47 @@ static int grouped_plan_build(tenet_ctx ctx, int dtype, int prec, int64_t count,
     tile_step_cost[(size_t)i] = c[best];
     if (hp.M * hp.N * hp.batch > 0) total_cost += best_t;
   }
+  // plain GEMMs grouped by shape; a group with at least LIB_MIN_FLOPS of work
+  // goes to the library
+  std::vector<char> in_lib((size_t)count, 0);
+  if (force_shape < 0 && prec == TENET_PREC_NATIVE) {
+    constexpr double LIB_MIN_FLOPS = 1e8;
+    std::map<std::tuple<int, int, int64_t, int64_t, int64_t, int64_t, int64_t, int64_t>, std::vector<int64_t>> groups;
+    for (int64_t i = 0; i < count; ++i) {
+      const GroupedProblem& q = probs[(size_t)i];
+      if (q.shape == GROUPED_SMALLK) continue;
+      rocblas_operation oa, ob;
+      int64_t lda, ldb, ldc;
+      if (!plain_gemm(hps[(size_t)i], q.conjA != 0, q.conjB != 0, &oa, &lda, &ob, &ldb, &ldc)) continue;
+      groups[std::make_tuple((int)oa, (int)ob, q.M, q.N, q.K, lda, ldb, ldc)].push_back(i);
+    }
+    for (auto& kv : groups) {
+      const auto& key = kv.first;
+      const double fl = 8.0 * std::get<2>(key) * std::get<3>(key) * std::get<4>(key) * (double)kv.second.size();
+      if (fl < LIB_MIN_FLOPS || std::get<2>(key) < 16 || std::get<3>(key) < 16 || std::get<4>(key) < 16) continue;
+      tenet_gplan_s::LibGroup lg;
+      lg.opA = (rocblas_operation)std::get<0>(key);
+      lg.opB = (rocblas_operation)std::get<1>(key);
+      lg.M = std::get<2>(key);
+      lg.N = std::get<3>(key);
+      lg.K = std::get<4>(key);
+      lg.lda = std::get<5>(key);
+      lg.ldb = std::get<6>(key);
+      lg.ldc = std::get<7>(key);
+      lg.idx = kv.second;
+      lg.arr_off = 3 * count + 3 * g->nlib;
+      g->nlib += (int64_t)lg.idx.size();
+      for (int64_t i : lg.idx) in_lib[(size_t)i] = 1;
+      g->lib.push_back(std::move(lg));
+    }
+  }
+  g->slot_elems = 3 * count + 3 * g->nlib;
+  for (int64_t i = 0; i < count; ++i) {
+    if (in_lib[(size_t)i]) continue;
+    const HostPlan& hp = hps[(size_t)i];
+    g->flops_rest += 8.0 * hp.M * hp.N * hp.K * hp.batch;
+    g->bytes_rest += (double)es * ((double)hp.M * hp.K + (double)hp.K * hp.N + (double)hp.M * hp.N) * hp.batch;
+  }
   // persistent grid: resident workgroups on every CU
   int dev = 0, ncu = 256;
   TENET_HIP_CHECK(hipGetDevice(&dev));
@@ -1965,23 +2077,29 @@ static int grouped_plan_build(tenet_ctx ctx, int dtype, int prec, int64_t count,
   }
   std::stable_sort(order.begin(), order.end(),
                    [](const std::pair<double, int64_t>& x, const std::pair<double, int64_t>& y) { return x.first > y.first; });
+  std::vector<GroupedUnit> units_rest;
   for (auto& o : order) {
     const GroupedProblem& q = probs[(size_t)o.second];
     const int64_t tiles = q.shape == GROUPED_SMALLK ? q.tiles_m : q.tiles_m * q.tiles_n * q.batch;
     for (int64_t t = 0; t < tiles; ++t)
-      for (int32_t ch = 0; ch < q.kchunks; ++ch) units.push_back({(int32_t)o.second, ch, t});
+      for (int32_t ch = 0; ch < q.kchunks; ++ch) {
+        units.push_back({(int32_t)o.second, ch, t});
+        if (!in_lib[(size_t)o.second]) units_rest.push_back({(int32_t)o.second, ch, t});
+      }
   }
+  g->nunits_rest = (int64_t)units_rest.size();
   // one device allocation
   const size_t rs = dtype == TENET_C128 ? 8 : 4;  // partial accumulators in the MFMA precision
   const size_t b_tab = align_up(std::max<size_t>(table_elems, 1) * 8, 256);
   const size_t b_prob = align_up((size_t)count * sizeof(GroupedProblem), 256);
   const size_t b_unit = align_up(std::max<size_t>(units.size(), 1) * sizeof(GroupedUnit), 256);
+  const size_t b_unit_rest = g->nlib ? align_up(std::max<size_t>(units_rest.size(), 1) * sizeof(GroupedUnit), 256) : 0;
   const size_t b_ctr = align_up((size_t)(64 + ctr_elems) * 4, 256);
   const size_t b_part = align_up((size_t)std::max<int64_t>(part_elems, 1) * 2 * rs, 256);
-  const size_t b_ptr = align_up((size_t)count * 3 * 8, 256) * tenet_gplan_s::RING;
+  const size_t b_ptr = align_up((size_t)g->slot_elems * 8, 256) * tenet_gplan_s::RING;
   // from the context's caching pool: plans are built per batch shape, and a
   // device allocation per plan would cost more than the plan's analysis
-  TRY(tenet_malloc(ctx, b_tab + b_prob + b_unit + b_ctr + b_part + b_ptr, &g->dev));
+  TRY(tenet_malloc(ctx, b_tab + b_prob + b_unit + b_ctr + b_part + b_ptr + b_unit_rest, &g->dev));
   char* base = (char*)g->dev;
   int64_t* tables = (int64_t*)base;
   g->probs = (GroupedProblem*)(base + b_tab);
@@ -1989,6 +2107,12 @@ static int grouped_plan_build(tenet_ctx ctx, int dtype, int prec, int64_t count,
   g->counters = (uint32_t*)(base + b_tab + b_prob + b_unit);
   g->partials = (void*)(base + b_tab + b_prob + b_unit + b_ctr);
   g->ptrs_dev = (uint64_t*)(base + b_tab + b_prob + b_unit + b_ctr + b_part);
+  g->units_rest = g->nlib ? (GroupedUnit*)(base + b_tab + b_prob + b_unit + b_ctr + b_part + b_ptr) : g->units;
+  if (!g->nlib) {
+    g->nunits_rest = g->nunits;
+    g->flops_rest = g->flops;
+    g->bytes_rest = g->bytes;
+  }
   std::vector<int64_t> host;
   host.reserve(table_elems);
   for (int64_t i = 0; i < count; ++i) {
@@ -2006,6 +2130,9 @@ static int grouped_plan_build(tenet_ctx ctx, int dtype, int prec, int64_t count,
   TENET_HIP_CHECK(hipMemcpy(g->probs, probs.data(), probs.size() * sizeof(GroupedProblem), hipMemcpyHostToDevice));
   if (!units.empty())
     TENET_HIP_CHECK(hipMemcpy(g->units, units.data(), units.size() * sizeof(GroupedUnit), hipMemcpyHostToDevice));
+  if (g->nlib && !units_rest.empty())
+    TENET_HIP_CHECK(hipMemcpy(g->units_rest, units_rest.data(), units_rest.size() * sizeof(GroupedUnit),
+                              hipMemcpyHostToDevice));
   TENET_HIP_CHECK(hipMemset(g->counters, 0, b_ctr));
   return TENET_OK;
 }
@@ -2045,19 +2172,90 @@ extern "C" int tenet_grouped_plan_info(tenet_gplan plan, int64_t* units, int64_t
   return TENET_OK;
 }
 
-static int grouped_launch(tenet_gplan g, hipStream_t s, const uint64_t* ptrs) {
+static int grouped_launch(tenet_gplan g, hipStream_t s, const uint64_t* ptrs, bool rest = false) {
   GroupedLaunch L{};
   L.probs = g->probs;
-  L.units = g->units;
-  L.nunits = g->nunits;
+  L.units = rest ? g->units_rest : g->units;
+  L.nunits = rest ? g->nunits_rest : g->nunits;
+  if (L.nunits == 0) return TENET_OK;
   L.ptrs = ptrs;
   L.partials = g->partials;
   L.counters = g->counters;
   L.grid = g->grid;
   L.dtype = g->dtype;
   L.prec = g->prec;
-  StatScope st(g->ctx, s, TENET_KSTAT_GEMM, g->flops, g->bytes);
+  StatScope st(g->ctx, s, TENET_KSTAT_GEMM, rest ? g->flops_rest : g->flops, rest ? g->bytes_rest : g->bytes);
   return launch_grouped_gemm(L, s);
+}
+
+// The plan's library groups.  A group whose A, B and C pointers (host array
+// ptrs) are each evenly spaced runs as one strided-batched GEMM (the tuned
+// library path); otherwise from its pointer arrays at dev (the ring slot).
+static int lib_launch(tenet_gplan g, hipStream_t s, void* const* ptrs, const uint64_t* dev) {
+  tenet_ctx ctx = g->ctx;
+  std::lock_guard<std::mutex> lk(ctx->blas_mu);
+  if (!ctx->blas && rocblas_create_handle(&ctx->blas) != rocblas_status_success) {
+    ctx->blas = nullptr;
+    set_error("rocBLAS handle creation failed");
+    return TENET_ERR_HIP;
+  }
+  if (rocblas_set_stream(ctx->blas, s) != rocblas_status_success) {
+    set_error("rocBLAS: cannot bind the stream");
+    return TENET_ERR_HIP;
+  }
+  const int64_t es = (int64_t)ELEM[g->dtype];
+  for (const auto& lg : g->lib) {
+    const int n = (int)lg.idx.size();
+    const uint64_t* a = dev + lg.arr_off;
+    // element strides between consecutive problems of the group, if even
+    int64_t st3[3] = {0, 0, 0};
+    bool even = true;
+    for (int w = 0; w < 3 && even; ++w) {
+      const intptr_t p0 = (intptr_t)ptrs[3 * lg.idx[0] + w];
+      const intptr_t d = n > 1 ? (intptr_t)ptrs[3 * lg.idx[1] + w] - p0 : 0;
+      even = d >= 0 && d % es == 0;
+      for (int j = 2; j < n && even; ++j) even = (intptr_t)ptrs[3 * lg.idx[j] + w] - p0 == (intptr_t)j * d;
+      st3[w] = d / es;
+    }
+    rocblas_status st;
+    if (even) {
+      void* A0 = ptrs[3 * lg.idx[0]];
+      void* B0 = ptrs[3 * lg.idx[0] + 1];
+      void* C0 = ptrs[3 * lg.idx[0] + 2];
+      if (g->dtype == TENET_C128) {
+        const rocblas_double_complex one = {1.0, 0.0}, zero = {0.0, 0.0};
+        st = rocblas_zgemm_strided_batched(ctx->blas, lg.opA, lg.opB, (rocblas_int)lg.M, (rocblas_int)lg.N,
+                                           (rocblas_int)lg.K, &one, (const rocblas_double_complex*)A0,
+                                           (rocblas_int)lg.lda, st3[0], (const rocblas_double_complex*)B0,
+                                           (rocblas_int)lg.ldb, st3[1], &zero, (rocblas_double_complex*)C0,
+                                           (rocblas_int)lg.ldc, st3[2], n);
+      } else {
+        const rocblas_float_complex one = {1.0f, 0.0f}, zero = {0.0f, 0.0f};
+        st = rocblas_cgemm_strided_batched(ctx->blas, lg.opA, lg.opB, (rocblas_int)lg.M, (rocblas_int)lg.N,
+                                           (rocblas_int)lg.K, &one, (const rocblas_float_complex*)A0,
+                                           (rocblas_int)lg.lda, st3[0], (const rocblas_float_complex*)B0,
+                                           (rocblas_int)lg.ldb, st3[1], &zero, (rocblas_float_complex*)C0,
+                                           (rocblas_int)lg.ldc, st3[2], n);
+      }
+    } else if (g->dtype == TENET_C128) {
+      const rocblas_double_complex one = {1.0, 0.0}, zero = {0.0, 0.0};
+      st = rocblas_zgemm_batched(ctx->blas, lg.opA, lg.opB, (rocblas_int)lg.M, (rocblas_int)lg.N, (rocblas_int)lg.K,
+                                 &one, (const rocblas_double_complex* const*)a, (rocblas_int)lg.lda,
+                                 (const rocblas_double_complex* const*)(a + n), (rocblas_int)lg.ldb, &zero,
+                                 (rocblas_double_complex* const*)(a + 2 * n), (rocblas_int)lg.ldc, n);
+    } else {
+      const rocblas_float_complex one = {1.0f, 0.0f}, zero = {0.0f, 0.0f};
+      st = rocblas_cgemm_batched(ctx->blas, lg.opA, lg.opB, (rocblas_int)lg.M, (rocblas_int)lg.N, (rocblas_int)lg.K,
+                                 &one, (const rocblas_float_complex* const*)a, (rocblas_int)lg.lda,
+                                 (const rocblas_float_complex* const*)(a + n), (rocblas_int)lg.ldb, &zero,
+                                 (rocblas_float_complex* const*)(a + 2 * n), (rocblas_int)lg.ldc, n);
+    }
+    if (st != rocblas_status_success) {
+      set_error(std::string("rocBLAS batched GEMM failed: ") + rocblas_status_to_string(st));
+      return TENET_ERR_HIP;
+    }
+  }
+  return TENET_OK;
 }
 
 extern "C" int tenet_grouped_exec(tenet_gplan plan, tenet_stream stream, void* const* ptrs) {
@@ -2077,7 +2275,7 @@ extern "C" int tenet_grouped_exec(tenet_gplan plan, tenet_stream stream, void* c
   }
   // pointers travel through the next ring slot (pinned host, then device:
   // the copy is asynchronous); the host array may be reused on return
-  const size_t nb = (size_t)plan->count * 3 * 8;
+  const size_t nb = (size_t)plan->slot_elems * 8;
   if (!plan->ptrs_pinned) {
     TENET_HIP_CHECK(hipHostMalloc((void**)&plan->ptrs_pinned, nb * tenet_gplan_s::RING, hipHostMallocDefault));
     for (int i = 0; i < tenet_gplan_s::RING; ++i)
@@ -2086,11 +2284,22 @@ extern "C" int tenet_grouped_exec(tenet_gplan plan, tenet_stream stream, void* c
   const int slot = plan->ring_next;
   plan->ring_next = (slot + 1) % tenet_gplan_s::RING;
   if (plan->ring_used[slot]) TENET_HIP_CHECK(hipEventSynchronize(plan->ring_ev[slot]));
-  uint64_t* host = plan->ptrs_pinned + (size_t)slot * plan->count * 3;
+  uint64_t* host = plan->ptrs_pinned + (size_t)slot * plan->slot_elems;
   uint64_t* dev = (uint64_t*)((char*)plan->ptrs_dev + align_up(nb, 256) * slot);
-  std::memcpy(host, ptrs, nb);
+  std::memcpy(host, ptrs, (size_t)plan->count * 3 * 8);
+  // the library groups' A, B and C pointer arrays follow the table
+  for (const auto& lg : plan->lib) {
+    const size_t n = lg.idx.size();
+    for (size_t j = 0; j < n; ++j) {
+      const int64_t i = lg.idx[j];
+      host[lg.arr_off + j] = (uint64_t)(uintptr_t)ptrs[3 * i];
+      host[lg.arr_off + n + j] = (uint64_t)(uintptr_t)ptrs[3 * i + 1];
+      host[lg.arr_off + 2 * n + j] = (uint64_t)(uintptr_t)ptrs[3 * i + 2];
+    }
+  }
   TENET_HIP_CHECK(hipMemcpyAsync(dev, host, nb, hipMemcpyHostToDevice, s));
-  TRY(grouped_launch(plan, s, dev));
+  if (!plan->lib.empty()) TRY(lib_launch(plan, s, ptrs, dev));
+  TRY(grouped_launch(plan, s, dev, true));
   TENET_HIP_CHECK(hipEventRecord(plan->ring_ev[slot], s));
   plan->ring_used[slot] = true;
   return TENET_OK;

@@ -193,6 +193,16 @@ class VidalMPS:
         sb = sorted(bonds)
         if any(y - x < 2 for x, y in zip(sb, sb[1:])):
             raise ValueError("a layer's bonds must be disjoint")
+        if list(bonds) != sb:
+            # the gates commute: run them in site order, so that every layer
+            # lays its operands out in the same order and the bond matrices
+            # of equal shape sit evenly spaced (one strided-batched GEMM)
+            order = sorted(range(P), key=lambda j: bonds[j])
+            ks = self.apply_layer_([gates[j] for j in order], sb, maxdim, threshold)
+            out = [0] * P
+            for pos, j in enumerate(order):
+                out[j] = ks[pos]
+            return out
         # an eager layer replaces sites and Lambdas: captured layers bound to
         # the old buffers are dropped
         self._drop_graphs()

@@ -287,3 +287,122 @@ def test_grouped_svd_without_u_preconditioned(ctx, dtype, tol):
         assert np.abs(Vk @ Vk.conj().T - np.eye(k)).max() < tol, i
         best = (u[:, :k] * s_ref[:k]) @ vh[:k]
         assert _rel(h @ Vk.conj().T @ Vk, best) < tol, i
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.complex128, 1e-12), (np.complex64, 2e-5)])
+def test_grouped_plain_gemm_groups(ctx, dtype, tol):
+    """Problems whose operands are plain column-major matrices run as batched
+    library GEMMs from tenet_grouped_exec (grouped by shape and operand op):
+    A.B, A^T.B, A.B^T, conj(A)^T.B, A.conj(B)^T (a strided view with ld
+    above the rows, as the Vidal output step reads V^H[:k]), next to a
+    permuted (gathered) problem and a small one that stay in the persistent
+    kernel.  Each against the oracle's binary_einsum, and the device-table
+    launch (every problem in the kernel) against the same reference."""
+    import ctypes as C
+
+    import tenet_hip as T
+    from tenet_hip import _lib as L
+    from tenet_hip.grouped import GroupedPlan
+
+    rng = np.random.default_rng(33)
+    m, n, k = 256, 192, 160  # 63 MFLOP per problem: two make a library group
+    # (shape_a, labels_a, shape_b, labels_b, shape_c, labels_c, strides_b, conj, copies)
+    cases = [
+        ((m, k), "ik", (k, n), "kj", (m, n), "ij", None, (0, 0), 3),
+        ((k, m), "ki", (k, n), "kj", (m, n), "ij", None, (0, 0), 2),
+        ((m, k), "ik", (n, k), "jk", (m, n), "ij", None, (0, 0), 2),
+        ((k, m), "ki", (k, n), "kj", (m, n), "ij", None, (1, 0), 2),
+        ((m, 2, k // 2), "iqr", (n, 2, k // 2), "jqr", (m, n), "ij", (1, n + 7, 2 * (n + 7)), (0, 1), 2),
+        ((k, m), "ki", (n, k), "jk", (n, m), "ji", None, (0, 0), 2),  # C transposed: kernel
+        ((6, 5), "ik", (5, 7), "kj", (6, 7), "ij", None, (0, 0), 3),  # small: kernel
+    ]
+    probs, hosts, outs, ptrs, conj = [], [], [], [], []
+    for sa, la, sb, lb, sc, lc, st_b, cj, copies in cases:
+        for _ in range(copies):
+            la_, lb_, lc_ = list(la), list(lb), list(lc)
+            a = O.complex_gaussian(rng, sa, dtype)
+            da = T.DeviceArray.from_host(ctx, a, dtype)
+            if st_b is not None:  # a view of the first n rows of an (n + 7)-row buffer
+                full = O.complex_gaussian(rng, (sb[0] + 7,) + tuple(sb[1:]), dtype)
+                b = full[:sb[0]]
+                db = T.DeviceArray.from_host(ctx, full, dtype)
+            else:
+                b = O.complex_gaussian(rng, sb, dtype)
+                db = T.DeviceArray.from_host(ctx, b, dtype)
+            dc = T.DeviceArray(ctx, sc, dtype)
+            pr = (sa, la_, sb, lb_, sc, lc_)
+            if st_b is not None:
+                pr = pr + (None, st_b)
+            probs.append(pr)
+            hosts.append((a, b, la_, lb_, lc_, cj))
+            outs.append((da, db, dc))
+            ptrs += [da.ptr, db.ptr, dc.ptr]
+            conj.append(cj)
+    plan = GroupedPlan(ctx, probs, dtype, conj=conj)
+
+    def check():
+        for (a, b, la_, lb_, lc_, cj), (_, _, dc) in zip(hosts, outs):
+            a2 = np.conj(a) if cj[0] else a
+            b2 = np.conj(b) if cj[1] else b
+            dims = [x for x in la_ if x in lb_ and x not in lc_]
+            ref = O.binary_einsum(O.OTensor(np.asarray(a2, np.complex128), la_),
+                                  O.OTensor(np.asarray(b2, np.complex128), lb_), dims).permute(lc_).data
+            assert _rel(dc.to_host(), ref) < tol, (la_, lb_, lc_, cj)
+
+    try:
+        for rep in range(2):
+            for _, _, dc in outs:
+                dc.zero_()
+            plan.exec(ptrs)
+            check()
+        for _, _, dc in outs:
+            dc.zero_()
+        words = np.zeros(len(ptrs) + len(ptrs) % 2, dtype=np.uint64)
+        words[:len(ptrs)] = ptrs
+        tab = T.DeviceArray.from_host(ctx, words.view(np.complex128), np.complex128)
+        plan.exec_dev(tab.ptr)
+        check()
+    finally:
+        plan.close()
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.complex128, 1e-12), (np.complex64, 2e-5)])
+def test_grouped_plain_gemm_strided(ctx, dtype, tol):
+    """A library group whose operands sit evenly spaced in one pool block each
+    (as a Vidal layer carves them) runs as one strided-batched GEMM: A.B and
+    A.conj(B)^T through a strided view, against the oracle."""
+    import ctypes as C
+
+    import tenet_hip as T
+    from tenet_hip import _lib as L
+    from tenet_hip.device import carve
+    from tenet_hip.grouped import GroupedPlan
+
+    rng = np.random.default_rng(34)
+    m, n, k, P = 256, 192, 160, 4
+    for cj, sb, lb, st_b in [((0, 0), (k, n), "kj", None), ((0, 1), (n, k), "jk", (1, n + 5))]:
+        a = [O.complex_gaussian(rng, (m, k), dtype) for _ in range(P)]
+        bfull = [O.complex_gaussian(rng, (sb[0] + (5 if st_b else 0), sb[1]), dtype) for _ in range(P)]
+        b = [x[:sb[0]] for x in bfull]
+        da = carve(ctx, [(m, k)] * P, dtype)
+        db = carve(ctx, [x.shape for x in bfull], dtype)
+        dc = carve(ctx, [(m, n)] * P, dtype)
+        for x, h in zip(da, a):
+            hh = np.asfortranarray(h)
+            L.check(L.lib.tenet_memcpy_h2d(ctx.h, C.c_void_p(x.ptr), hh.ctypes.data_as(C.c_void_p), hh.nbytes, None))
+        for x, h in zip(db, bfull):
+            hh = np.asfortranarray(h)
+            L.check(L.lib.tenet_memcpy_h2d(ctx.h, C.c_void_p(x.ptr), hh.ctypes.data_as(C.c_void_p), hh.nbytes, None))
+        pr = ((m, k), list("ik"), sb, list(lb), (m, n), list("ij"))
+        if st_b is not None:
+            pr = pr + (None, st_b)
+        plan = GroupedPlan(ctx, [pr] * P, dtype, conj=[cj] * P)
+        try:
+            plan.exec([p for j in range(P) for p in (da[j].ptr, db[j].ptr, dc[j].ptr)])
+            for j in range(P):
+                b2 = np.conj(b[j]) if cj[1] else b[j]
+                ref = O.binary_einsum(O.OTensor(np.asarray(a[j], np.complex128), list("ik")),
+                                      O.OTensor(np.asarray(b2, np.complex128), list(lb)), ["k"]).permute(list("ij")).data
+                assert _rel(dc[j].to_host(), ref) < tol, (cj, j)
+        finally:
+            plan.close()
