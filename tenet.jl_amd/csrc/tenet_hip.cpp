@@ -60,6 +60,18 @@ struct Workspace {
   size_t bytes = 0;
 };
 
+// A contraction whose matricised operands are ordinary column-major
+// matrices: op(A) (M x K), op(B) (K x N), C (M x N, unit stride along M),
+// and, with batch labels, each operand's batch entries evenly spaced.  The
+// base ops are N or T (the caller's conjugation turns T into C; a
+// conjugated operand with op N is not a plain GEMM).
+struct LibLayout {
+  bool ok = false;
+  rocblas_operation opA = rocblas_operation_none, opB = rocblas_operation_none;
+  int64_t lda = 0, ldb = 0, ldc = 0;
+  int64_t sA = 0, sB = 0, sC = 0;  // batch strides (elements)
+};
+
 struct tenet_plan_s {
   tenet_ctx ctx = nullptr;
   int dtype = 0, prec = 0;
@@ -71,7 +83,11 @@ struct tenet_plan_s {
   bool cached = false;
   // spans (elements) each operand must cover, for pool bounds checks
   int64_t spanA = 0, spanB = 0, spanC = 0;
+  // a plain (strided-batched) GEMM of at least LIB_MIN_FLOPS in the native
+  // precision runs in the vendor library (not under graph capture)
+  LibLayout lib;
 };
+constexpr double LIB_MIN_FLOPS = 1e8;
 
 struct tenet_graph_s {
   hipGraph_t graph = nullptr;
@@ -767,6 +783,65 @@ struct HostPlan {
   std::vector<int64_t> v[9];
 };
 
+static LibLayout plain_layout(const HostPlan& hp) {
+  LibLayout L;
+  if (hp.M < 1 || hp.N < 1 || hp.K < 1 || hp.batch < 1) return L;
+  auto affine = [](const std::vector<int64_t>& v, int64_t* st) {
+    if (v.empty() || v[0] != 0) return false;
+    *st = v.size() > 1 ? v[1] : 0;
+    for (size_t i = 1; i < v.size(); ++i)
+      if (v[i] != (int64_t)i * *st) return false;
+    return true;
+  };
+  int64_t sM, sKa, sKb, sN, sMc, sNc, bA, bB, bC;
+  if (!affine(hp.v[0], &sM) || !affine(hp.v[1], &sKa) || !affine(hp.v[3], &sKb) || !affine(hp.v[4], &sN) ||
+      !affine(hp.v[6], &sMc) || !affine(hp.v[7], &sNc) || !affine(hp.v[2], &bA) || !affine(hp.v[5], &bB) ||
+      !affine(hp.v[8], &bC))
+    return L;
+  if (bA < 0 || bB < 0 || bC < 0) return L;
+  if (hp.M == 1) sM = sMc = 1;
+  if (hp.K == 1) sKa = sKb = 1;
+  if (hp.N == 1) sN = 1;
+  if (sMc != 1) return L;
+  L.ldc = hp.N == 1 ? hp.M : sNc;
+  if (L.ldc < hp.M) return L;
+  if (sM == 1 && (hp.K == 1 || sKa >= hp.M)) {
+    L.opA = rocblas_operation_none;
+    L.lda = hp.K == 1 ? hp.M : sKa;
+  } else if (sKa == 1 && (hp.M == 1 || sM >= hp.K)) {
+    L.opA = rocblas_operation_transpose;
+    L.lda = hp.M == 1 ? hp.K : sM;
+  } else {
+    return L;
+  }
+  if (sKb == 1 && (hp.N == 1 || sN >= hp.K)) {
+    L.opB = rocblas_operation_none;
+    L.ldb = hp.N == 1 ? hp.K : sN;
+  } else if (sN == 1 && (hp.K == 1 || sKb >= hp.N)) {
+    L.opB = rocblas_operation_transpose;
+    L.ldb = hp.K == 1 ? hp.N : sKb;
+  } else {
+    return L;
+  }
+  L.sA = bA;
+  L.sB = bB;
+  L.sC = bC;
+  L.ok = L.lda <= INT32_MAX && L.ldb <= INT32_MAX && L.ldc <= INT32_MAX && hp.M <= INT32_MAX &&
+         hp.N <= INT32_MAX && hp.K <= INT32_MAX && hp.batch <= INT32_MAX;
+  return L;
+}
+
+// the op of an operand after the caller's conjugation (false: not plain)
+static bool lib_op(rocblas_operation base, bool conj, rocblas_operation* op) {
+  if (!conj) {
+    *op = base;
+    return true;
+  }
+  if (base == rocblas_operation_none) return false;
+  *op = rocblas_operation_conjugate_transpose;
+  return true;
+}
+
 static int analyse_plan(const tenet_tdesc* a, const tenet_tdesc* b, const tenet_tdesc* c, HostPlan& hp) {
   for (const tenet_tdesc* d : {a, b, c}) {
     if (d->rank < 0 || d->rank > TENET_MAX_RANK) {
@@ -905,6 +980,8 @@ static int build_plan(tenet_ctx ctx, int dtype, int prec, const tenet_tdesc* a, 
   p->spanA = hp.spanA;
   p->spanB = hp.spanB;
   p->spanC = hp.spanC;
+  if (prec == TENET_PREC_NATIVE && 8.0 * hp.M * hp.N * hp.K * hp.batch >= LIB_MIN_FLOPS)
+    p->lib = plain_layout(hp);
   std::vector<int64_t> host;
   size_t off[9];
   for (int i = 0; i < 9; ++i) {
@@ -952,6 +1029,48 @@ extern "C" int tenet_einsum_plan(tenet_ctx ctx, int dtype, int prec, const tenet
   return TENET_OK;
 }
 
+// The context's rocBLAS handle bound to stream s (the caller holds blas_mu).
+static int blas_bind(tenet_ctx ctx, hipStream_t s) {
+  if (!ctx->blas && rocblas_create_handle(&ctx->blas) != rocblas_status_success) {
+    ctx->blas = nullptr;
+    set_error("rocBLAS handle creation failed");
+    return TENET_ERR_HIP;
+  }
+  if (rocblas_set_stream(ctx->blas, s) != rocblas_status_success) {
+    set_error("rocBLAS: cannot bind the stream");
+    return TENET_ERR_HIP;
+  }
+  return TENET_OK;
+}
+
+// One plain (strided-batched) GEMM through rocBLAS: C = alpha op(A) op(B) + beta C.
+static int lib_gemm(tenet_ctx ctx, hipStream_t s, int dtype, rocblas_operation opA, rocblas_operation opB,
+                    int64_t M, int64_t N, int64_t K, const double* alpha, const void* A, int64_t lda, int64_t sA,
+                    const void* B, int64_t ldb, int64_t sB, const double* beta, void* C, int64_t ldc, int64_t sC,
+                    int64_t batch) {
+  std::lock_guard<std::mutex> lk(ctx->blas_mu);
+  TRY(blas_bind(ctx, s));
+  rocblas_status st;
+  if (dtype == TENET_C128) {
+    const rocblas_double_complex al = {alpha[0], alpha[1]}, be = {beta[0], beta[1]};
+    st = rocblas_zgemm_strided_batched(ctx->blas, opA, opB, (rocblas_int)M, (rocblas_int)N, (rocblas_int)K, &al,
+                                       (const rocblas_double_complex*)A, (rocblas_int)lda, sA,
+                                       (const rocblas_double_complex*)B, (rocblas_int)ldb, sB, &be,
+                                       (rocblas_double_complex*)C, (rocblas_int)ldc, sC, (rocblas_int)batch);
+  } else {
+    const rocblas_float_complex al = {(float)alpha[0], (float)alpha[1]}, be = {(float)beta[0], (float)beta[1]};
+    st = rocblas_cgemm_strided_batched(ctx->blas, opA, opB, (rocblas_int)M, (rocblas_int)N, (rocblas_int)K, &al,
+                                       (const rocblas_float_complex*)A, (rocblas_int)lda, sA,
+                                       (const rocblas_float_complex*)B, (rocblas_int)ldb, sB, &be,
+                                       (rocblas_float_complex*)C, (rocblas_int)ldc, sC, (rocblas_int)batch);
+  }
+  if (st != rocblas_status_success) {
+    set_error(std::string("rocBLAS GEMM failed: ") + rocblas_status_to_string(st));
+    return TENET_ERR_HIP;
+  }
+  return TENET_OK;
+}
+
 static int exec_plan(tenet_plan p, hipStream_t s, const void* A, const void* B, void* C, int conj_a, int conj_b,
                      const double* alpha, const double* beta) {
   tenet_ctx ctx = p->ctx;
@@ -979,6 +1098,13 @@ static int exec_plan(tenet_plan p, hipStream_t s, const void* A, const void* B, 
   g.alpha_im = alpha ? alpha[1] : 0.0;
   g.beta_re = beta ? beta[0] : 0.0;
   g.beta_im = beta ? beta[1] : 0.0;
+  rocblas_operation oa, ob;
+  if (p->lib.ok && p->K > 0 && lib_op(p->lib.opA, conj_a != 0, &oa) && lib_op(p->lib.opB, conj_b != 0, &ob) &&
+      !capturing(s)) {
+    const double one[2] = {1.0, 0.0}, zero[2] = {0.0, 0.0};
+    return lib_gemm(ctx, s, p->dtype, oa, ob, p->M, p->N, p->K, alpha ? alpha : one, A, p->lib.lda, p->lib.sA, B,
+                    p->lib.ldb, p->lib.sB, beta ? beta : zero, C, p->lib.ldc, p->lib.sC, p->batch);
+  }
   void* ws = nullptr;
   if (p->ws_bytes) TRY(ws_get(ctx, s, p->ws_bytes, &ws, 1));
   StatScope st(ctx, s, TENET_KSTAT_GEMM, 8.0 * p->M * p->N * p->K * p->batch,
@@ -1841,52 +1967,16 @@ struct tenet_gplan_s {
   int64_t slot_elems = 0;  // pointers per ring slot: 3 * count + 3 * nlib
 };
 
-// A problem's matricised operands as plain column-major GEMM operands, if
-// they are: op(A) (M x K), op(B) (K x N), C (M x N, unit stride along M).
+// A grouped problem as a plain GEMM (no batch label), with its conjugation.
 static bool plain_gemm(const HostPlan& hp, bool conjA, bool conjB, rocblas_operation* opA, int64_t* lda,
                        rocblas_operation* opB, int64_t* ldb, int64_t* ldc) {
-  if (hp.batch != 1 || hp.M < 1 || hp.N < 1 || hp.K < 1) return false;
-  auto affine = [](const std::vector<int64_t>& v, int64_t* st) {
-    if (v.empty() || v[0] != 0) return false;
-    *st = v.size() > 1 ? v[1] : 0;
-    for (size_t i = 1; i < v.size(); ++i)
-      if (v[i] != (int64_t)i * *st) return false;
-    return true;
-  };
-  int64_t sM, sKa, sKb, sN, sMc, sNc;
-  if (!affine(hp.v[0], &sM) || !affine(hp.v[1], &sKa) || !affine(hp.v[3], &sKb) || !affine(hp.v[4], &sN) ||
-      !affine(hp.v[6], &sMc) || !affine(hp.v[7], &sNc))
-    return false;
-  if (hp.M == 1) sM = 1;
-  if (hp.K == 1) sKa = sKb = 1;
-  if (hp.N == 1) sN = 1;
-  if (hp.M == 1) sMc = 1;
-  // C: unit stride along M, columns ldc >= M apart
-  if (sMc != 1) return false;
-  *ldc = hp.N == 1 ? hp.M : sNc;
-  if (*ldc < hp.M) return false;
-  // A: M x K column-major (op N) or K x M column-major (op T / C)
-  if (sM == 1 && !conjA && (hp.K == 1 || sKa >= hp.M)) {
-    *opA = rocblas_operation_none;
-    *lda = hp.K == 1 ? hp.M : sKa;
-  } else if (sKa == 1 && (hp.M == 1 || sM >= hp.K)) {
-    *opA = conjA ? rocblas_operation_conjugate_transpose : rocblas_operation_transpose;
-    *lda = hp.M == 1 ? hp.K : sM;
-  } else {
-    return false;
-  }
-  // B: K x N column-major (op N) or N x K column-major (op T / C)
-  if (sKb == 1 && !conjB && (hp.N == 1 || sN >= hp.K)) {
-    *opB = rocblas_operation_none;
-    *ldb = hp.N == 1 ? hp.K : sN;
-  } else if (sN == 1 && (hp.K == 1 || sKb >= hp.N)) {
-    *opB = conjB ? rocblas_operation_conjugate_transpose : rocblas_operation_transpose;
-    *ldb = hp.K == 1 ? hp.N : sKb;
-  } else {
-    return false;
-  }
-  return *lda <= INT32_MAX && *ldb <= INT32_MAX && *ldc <= INT32_MAX && hp.M <= INT32_MAX && hp.N <= INT32_MAX &&
-         hp.K <= INT32_MAX;
+  if (hp.batch != 1) return false;
+  const LibLayout L = plain_layout(hp);
+  if (!L.ok || !lib_op(L.opA, conjA, opA) || !lib_op(L.opB, conjB, opB)) return false;
+  *lda = L.lda;
+  *ldb = L.ldb;
+  *ldc = L.ldc;
+  return true;
 }
 
 static const int TILE_EDGE[3] = {64, 32, 16};
@@ -1984,7 +2074,6 @@ static int grouped_plan_build(tenet_ctx ctx, int dtype, int prec, int64_t count,
   // goes to the library
   std::vector<char> in_lib((size_t)count, 0);
   if (force_shape < 0 && prec == TENET_PREC_NATIVE) {
-    constexpr double LIB_MIN_FLOPS = 1e8;
     std::map<std::tuple<int, int, int64_t, int64_t, int64_t, int64_t, int64_t, int64_t>, std::vector<int64_t>> groups;
     for (int64_t i = 0; i < count; ++i) {
       const GroupedProblem& q = probs[(size_t)i];
@@ -2194,15 +2283,7 @@ static int grouped_launch(tenet_gplan g, hipStream_t s, const uint64_t* ptrs, bo
 static int lib_launch(tenet_gplan g, hipStream_t s, void* const* ptrs, const uint64_t* dev) {
   tenet_ctx ctx = g->ctx;
   std::lock_guard<std::mutex> lk(ctx->blas_mu);
-  if (!ctx->blas && rocblas_create_handle(&ctx->blas) != rocblas_status_success) {
-    ctx->blas = nullptr;
-    set_error("rocBLAS handle creation failed");
-    return TENET_ERR_HIP;
-  }
-  if (rocblas_set_stream(ctx->blas, s) != rocblas_status_success) {
-    set_error("rocBLAS: cannot bind the stream");
-    return TENET_ERR_HIP;
-  }
+  TRY(blas_bind(ctx, s));
   const int64_t es = (int64_t)ELEM[g->dtype];
   for (const auto& lg : g->lib) {
     const int n = (int)lg.idx.size();

@@ -599,6 +599,50 @@ def test_einsum_split_i8_batched_conj_alpha_beta(ctx, conj):
         assert _rel(Cd.to_host(), ref) < tol
 
 
+@pytest.mark.parametrize("conj", [(0, 0), (1, 0), (0, 1), (1, 1)])
+@pytest.mark.parametrize("layout", ["nn", "tt"])
+def test_einsum_plain_gemm_library_path(ctx, conj, layout):
+    """Native-precision contractions that are plain column-major GEMMs of at
+    least 1e8 flops run through the vendor library (strided-batched, with a
+    batch label, C = alpha op(A) op(B) + beta C): A.B with both operands
+    untransposed (a conjugated one then stays in the tile kernel) and with
+    both transposed (conjugation folds into op C), against numpy."""
+    import ctypes as C
+
+    import tenet_hip as T
+    from tenet_hip import _lib as L
+    from tenet_hip.device import tdesc
+
+    rng = np.random.default_rng(51 + 2 * conj[0] + conj[1] + (10 if layout == "tt" else 0))
+    m, n, k, z = 160, 144, 200, 3  # 8 m n k z = 1.1e8 flops
+    for dt, tol in ((np.complex128, 1e-13), (np.complex64, 2e-6)):
+        if layout == "nn":
+            a = O.complex_gaussian(rng, (m, k, z)).astype(dt)
+            b = O.complex_gaussian(rng, (k, n, z)).astype(dt)
+            la, lb, spec = [1, 2, 3], [2, 4, 3], "ikz,kjz->ijz"
+        else:
+            a = O.complex_gaussian(rng, (k, m, z)).astype(dt)
+            b = O.complex_gaussian(rng, (n, k, z)).astype(dt)
+            la, lb, spec = [2, 1, 3], [4, 2, 3], "kiz,jkz->ijz"
+        c0 = O.complex_gaussian(rng, (m, n, z)).astype(dt)
+        A, B = T.DeviceArray.from_host(ctx, a, dt), T.DeviceArray.from_host(ctx, b, dt)
+        Cd = T.DeviceArray.from_host(ctx, c0, dt)
+        h = C.c_void_p()
+        L.check(L.lib.tenet_einsum_plan(ctx.h, A.code, L.PREC_NATIVE, C.byref(tdesc(a.shape, la)),
+                                        C.byref(tdesc(b.shape, lb)), C.byref(tdesc(c0.shape, [1, 4, 3])),
+                                        C.byref(h)), "plan")
+        al = (C.c_double * 2)(0.5, -1.25)
+        be = (C.c_double * 2)(-0.75, 0.5)
+        L.check(L.lib.tenet_einsum_exec(h, None, C.c_void_p(A.ptr), C.c_void_p(B.ptr), C.c_void_p(Cd.ptr), conj[0],
+                                        conj[1], al, be), "exec")
+        L.lib.tenet_plan_destroy(h)
+        aa = np.conj(a) if conj[0] else a
+        bb = np.conj(b) if conj[1] else b
+        ref = (0.5 - 1.25j) * np.einsum(spec, aa.astype(np.complex128), bb.astype(np.complex128)) + \
+            (-0.75 + 0.5j) * c0.astype(np.complex128)
+        assert _rel(Cd.to_host(), ref) < tol
+
+
 # ---------------------------------------------------------------- DMRG H_eff
 @pytest.mark.parametrize("prec,tol", [(0, 1e-13), (1, 1e-5), (2, 1e-12), (3, 1e-12), (4, 1e-12),
                                       (5, 5e-9)])
