@@ -47,7 +47,11 @@ def heff_apply(L_env: Tensor, W1: Tensor, W2: Tensor, R_env: Tensor, theta: Tens
     with labels ``labels`` = (l, o1, o2, r)."""
     op = binary_einsum(W1, W2)  # (w, o1, i1, wr, o2, i2)
     t1 = binary_einsum(L_env, theta, prec=prec)  # (l, w, i1, i2, rk)
-    t2 = binary_einsum(t1, op)  # (l, rk, o1, wr, o2)
+    # the labels R contracts last, in R's order (l, o1, o2, wr, rk): the last
+    # contraction's operands are then plain matrices, a library GEMM
+    nat = [x for x in t1.labels if x not in op.labels] + [x for x in op.labels if x not in t1.labels]
+    kl = [x for x in R_env.labels if x in nat]
+    t2 = binary_einsum(t1, op, out_labels=[x for x in nat if x not in kl] + kl)
     y = binary_einsum(t2, R_env, prec=prec)  # (l, o1, o2, r)
     return permute(y, list(labels))
 
