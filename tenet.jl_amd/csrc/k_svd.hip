@@ -728,16 +728,18 @@ __device__ __forceinline__ void store_rows_all(SvSmem& sm, const double2 (&v)[ST
 // Gram G = X^H X of the pair's columns over rows [r0, r1) into sm.gr / sm.gi.
 // With P = Xr^T Xi:  Re G = Xr^T Xr + Xi^T Xi (symmetric), Im G = P - P^T.
 // For a 4-row K step the A operand of X^T and the B operand of X are the same
-// register (X[k = l / 16][c = l % 16]).  Waves 0-3 take the staged rows
-// [0, 32) of every chunk, waves 4-7 the rows [32, 64); within a group the
-// four waves own: Re G(0,0) + P(0,0); Re G(1,1) + P(1,1); Re G(0,1);
-// P(0,1) + P(1,0) (3, 3, 2 and 2 products per K step).  The two row groups
-// are added in LDS at the end (fixed order: deterministic).
+// register (X[k = l / 16][c = l % 16]).  Each 64-row chunk is cut into four
+// 16-row groups; group g belongs to waves g and g + 4 (the two waves of one
+// SIMD): wave g sums Re G(0,0), Re G(1,1) and P(0,0), wave g + 4 sums P(1,1),
+// Re G(0,1), P(0,1) and P(1,0) -- five products per K step each, so every
+// SIMD issues the same number of MFMAs.  The groups' partials are added in
+// LDS in a fixed order (1 into 0 and 3 into 2, then 2 into 0: deterministic).
 __device__ void gram_mfma(SvSmem& sm, const double* X, int64_t ld, int r0, int r1, int I, int J, int b) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int role = wave & 3, grp = wave >> 2;
+  const int grp = wave & 3;
+  const bool hiw = wave >= 4;
   const int kr = lane >> 4, kc = lane & 15;
-  svd_d4v a0 = {0, 0, 0, 0}, a1 = {0, 0, 0, 0};
+  svd_d4v a0 = {0, 0, 0, 0}, a1 = {0, 0, 0, 0}, a2 = {0, 0, 0, 0}, a3 = {0, 0, 0, 0};
   double2 v[STAGE_PER];
   load_rows(v, X, ld, r0, min(RS, r1 - r0), I, J, b);
   for (int rb = r0; rb < r1; rb += RS) {
@@ -746,60 +748,83 @@ __device__ void gram_mfma(SvSmem& sm, const double* X, int64_t ld, int r0, int r
     __syncthreads();
     if (rb + RS < r1) load_rows(v, X, ld, rb + RS, min(RS, r1 - rb - RS), I, J, b);
 #pragma unroll
-    for (int s = 0; s < RS / 8; ++s) {
-      const int row = grp * (RS / 2) + 4 * s + kr;
+    for (int s = 0; s < RS / 16; ++s) {
+      const int row = grp * (RS / 4) + 4 * s + kr;
       const double xr0 = sm.xr[row][kc], xi0 = sm.xi[row][kc];
       const double xr1 = sm.xr[row][16 + kc], xi1 = sm.xi[row][16 + kc];
-      if (role == 0) {
+      if (!hiw) {
         a0 = mfma_f64(xr0, xr0, a0);
         a0 = mfma_f64(xi0, xi0, a0);
-        a1 = mfma_f64(xr0, xi0, a1);
-      } else if (role == 1) {
-        a0 = mfma_f64(xr1, xr1, a0);
-        a0 = mfma_f64(xi1, xi1, a0);
-        a1 = mfma_f64(xr1, xi1, a1);
-      } else if (role == 2) {
-        a0 = mfma_f64(xr0, xr1, a0);
-        a0 = mfma_f64(xi0, xi1, a0);
+        a1 = mfma_f64(xr1, xr1, a1);
+        a1 = mfma_f64(xi1, xi1, a1);
+        a2 = mfma_f64(xr0, xi0, a2);
       } else {
-        a0 = mfma_f64(xr0, xi1, a0);
-        a1 = mfma_f64(xr1, xi0, a1);
+        a0 = mfma_f64(xr1, xi1, a0);
+        a1 = mfma_f64(xr0, xr1, a1);
+        a1 = mfma_f64(xi0, xi1, a1);
+        a2 = mfma_f64(xr0, xi1, a2);
+        a3 = mfma_f64(xr1, xi0, a3);
       }
     }
   }
-  // row group 1 -> LDS (staging area), group 0 adds
+  // partials through the staging area: slot (pair, wave half) of 4 x 4 x 64
   double* red = &sm.xr[0][0];
+  const int slot = (grp >> 1) * 2 + (hiw ? 1 : 0);
   __syncthreads();
-  if (grp == 1) {
+  if (grp & 1) {
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      red[((role * 2 + 0) * 4 + r) * 64 + lane] = a0[r];
-      red[((role * 2 + 1) * 4 + r) * 64 + lane] = a1[r];
+      red[((slot * 4 + 0) * 4 + r) * 64 + lane] = a0[r];
+      red[((slot * 4 + 1) * 4 + r) * 64 + lane] = a1[r];
+      red[((slot * 4 + 2) * 4 + r) * 64 + lane] = a2[r];
+      red[((slot * 4 + 3) * 4 + r) * 64 + lane] = a3[r];
+    }
+  }
+  __syncthreads();
+  if (!(grp & 1)) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      a0[r] += red[((slot * 4 + 0) * 4 + r) * 64 + lane];
+      a1[r] += red[((slot * 4 + 1) * 4 + r) * 64 + lane];
+      a2[r] += red[((slot * 4 + 2) * 4 + r) * 64 + lane];
+      a3[r] += red[((slot * 4 + 3) * 4 + r) * 64 + lane];
+    }
+  }
+  __syncthreads();
+  if (grp == 2) {
+    const int s2 = hiw ? 1 : 0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      red[((s2 * 4 + 0) * 4 + r) * 64 + lane] = a0[r];
+      red[((s2 * 4 + 1) * 4 + r) * 64 + lane] = a1[r];
+      red[((s2 * 4 + 2) * 4 + r) * 64 + lane] = a2[r];
+      red[((s2 * 4 + 3) * 4 + r) * 64 + lane] = a3[r];
     }
   }
   __syncthreads();
   if (grp == 0) {
+    const int s2 = hiw ? 1 : 0;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      a0[r] += red[((role * 2 + 0) * 4 + r) * 64 + lane];
-      a1[r] += red[((role * 2 + 1) * 4 + r) * 64 + lane];
+      a0[r] += red[((s2 * 4 + 0) * 4 + r) * 64 + lane];
+      a1[r] += red[((s2 * 4 + 1) * 4 + r) * 64 + lane];
+      a2[r] += red[((s2 * 4 + 2) * 4 + r) * 64 + lane];
+      a3[r] += red[((s2 * 4 + 3) * 4 + r) * 64 + lane];
     }
     // Re G into gr (mirrored), P into wr (scratch until the inner solve sets W)
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int i = 4 * r + kr, j = kc;
-      if (role == 0) {
+      if (!hiw) {
         sm.gr[i][j] = a0[r];
-        sm.wr[i][j] = a1[r];
-      } else if (role == 1) {
-        sm.gr[16 + i][16 + j] = a0[r];
-        sm.wr[16 + i][16 + j] = a1[r];
-      } else if (role == 2) {
-        sm.gr[i][16 + j] = a0[r];
-        sm.gr[16 + j][i] = a0[r];
+        sm.gr[16 + i][16 + j] = a1[r];
+        sm.wr[i][j] = a2[r];
       } else {
-        sm.wr[i][16 + j] = a0[r];
-        sm.wr[16 + i][j] = a1[r];
+        sm.wr[16 + i][16 + j] = a0[r];
+        sm.gr[i][16 + j] = a1[r];
+        sm.gr[16 + j][i] = a1[r];
+        sm.wr[i][16 + j] = a2[r];
+        sm.wr[16 + i][j] = a3[r];
       }
     }
   }
