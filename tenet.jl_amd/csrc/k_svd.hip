@@ -728,12 +728,15 @@ __device__ __forceinline__ void store_rows_all(SvSmem& sm, const double2 (&v)[ST
 // Gram G = X^H X of the pair's columns over rows [r0, r1) into sm.gr / sm.gi.
 // With P = Xr^T Xi:  Re G = Xr^T Xr + Xi^T Xi (symmetric), Im G = P - P^T.
 // For a 4-row K step the A operand of X^T and the B operand of X are the same
-// register (X[k = l / 16][c = l % 16]).  Each 64-row chunk is cut into four
-// 16-row groups; group g belongs to waves g and g + 4 (the two waves of one
-// SIMD): wave g sums Re G(0,0), Re G(1,1) and P(0,0), wave g + 4 sums P(1,1),
-// Re G(0,1), P(0,1) and P(1,0) -- five products per K step each, so every
-// SIMD issues the same number of MFMAs.  The groups' partials are added in
-// LDS in a fixed order (1 into 0 and 3 into 2, then 2 into 0: deterministic).
+// register (X[k = l / 16][c = l % 16]).  The off-diagonal block takes three
+// real products (Gauss): T1 = Xr0^T Xr1, T2 = Xi0^T Xi1 and
+// T3 = (Xr0 + Xi0)^T (Xr1 - Xi1) give Re G(0,1) = T1 + T2 and
+// Im G(0,1) = T1 - T2 - T3 (errors O(eps |x_p| |x_q|), as for the other
+// entries).  Each 64-row chunk is cut into four 16-row groups; group g
+// belongs to waves g and g + 4 (the two waves of one SIMD): wave g sums
+// Re G(0,0), Re G(1,1) and P(0,0) (five products per K step), wave g + 4
+// P(1,1), T1, T2 and T3 (four).  The groups' partials are added in LDS in a
+// fixed order (1 into 0 and 3 into 2, then 2 into 0: deterministic).
 __device__ void gram_mfma(SvSmem& sm, const double* X, int64_t ld, int r0, int r1, int I, int J, int b) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int grp = wave & 3;
@@ -761,9 +764,8 @@ __device__ void gram_mfma(SvSmem& sm, const double* X, int64_t ld, int r0, int r
       } else {
         a0 = mfma_f64(xr1, xi1, a0);
         a1 = mfma_f64(xr0, xr1, a1);
-        a1 = mfma_f64(xi0, xi1, a1);
-        a2 = mfma_f64(xr0, xi1, a2);
-        a3 = mfma_f64(xr1, xi0, a3);
+        a2 = mfma_f64(xi0, xi1, a2);
+        a3 = mfma_f64(xr0 + xi0, xr1 - xi1, a3);
       }
     }
   }
@@ -820,11 +822,13 @@ __device__ void gram_mfma(SvSmem& sm, const double* X, int64_t ld, int r0, int r
         sm.gr[16 + i][16 + j] = a1[r];
         sm.wr[i][j] = a2[r];
       } else {
+        // the off-diagonal block of Im G goes into wr above the diagonal
+        // (zeros below), so that gi = wr - wr^T holds it and its mirror
         sm.wr[16 + i][16 + j] = a0[r];
-        sm.gr[i][16 + j] = a1[r];
-        sm.gr[16 + j][i] = a1[r];
-        sm.wr[i][16 + j] = a2[r];
-        sm.wr[16 + i][j] = a3[r];
+        sm.gr[i][16 + j] = a1[r] + a2[r];
+        sm.gr[16 + j][i] = a1[r] + a2[r];
+        sm.wr[i][16 + j] = a1[r] - a2[r] - a3[r];
+        sm.wr[16 + j][i] = 0.0;
       }
     }
   }
