@@ -843,9 +843,8 @@ __device__ void gram_mfma(SvSmem& sm, const double* X, int64_t ld, int r0, int r
 // Rows [r0, r1) of the pair's columns of M <- M W by MFMA: per 64-row chunk
 // wave w computes the 16 x 16 output block (rows 16 (w % 4), columns
 // 16 (w / 4)) over K = 32 in three real products (see below), the W
-// operands held in registers for the whole call; the block goes back
-// through LDS so the stores are column-contiguous.  The next chunk's loads
-// are in flight during the products.
+// operands held in registers for the whole call, and stores it from its
+// accumulators.  The next chunk's loads are in flight during the products.
 __device__ void apply_mfma(SvSmem& sm, double* M, int64_t ld, int r0, int r1, int I, int J, int b) {
   const int n2 = 2 * b;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -878,23 +877,22 @@ __device__ void apply_mfma(SvSmem& sm, double* M, int64_t ld, int r0, int r1, in
       t2 = mfma_f64(xi, wim[s], t2);
       t3 = mfma_f64(xr + xi, wsum[s], t3);
     }
-    __syncthreads();  // every wave has read its X operands
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      sm.xr[16 * rblk + 4 * r + kr][16 * cblk + kc] = t1[r] - t2[r];
-      sm.xi[16 * rblk + 4 * r + kr][16 * cblk + kc] = t3[r] - t1[r] - t2[r];
-    }
-    __syncthreads();
+    // straight from the accumulators: lanes l and l + 16, l + 32, l + 48
+    // write four consecutive rows of one column (64 contiguous bytes); the
+    // staged chunk in LDS is not touched, so no barrier before the next one
     double2* M2 = reinterpret_cast<double2*>(M);
+    const int l = 16 * cblk + kc;
+    if (l < n2) {
+      double2* col = M2 + (size_t)pair_col(l, I, J, b) * ld + rb;
 #pragma unroll
-    for (int u = 0; u < STAGE_PER; ++u) {
-      const int e = threadIdx.x + u * SR_THREADS;
-      const int r = e % RS, l = e / RS;
-      if (l < n2 && r < rows) {
-        double2 o;
-        o.x = sm.xr[r][l];
-        o.y = sm.xi[r][l];
-        M2[(size_t)pair_col(l, I, J, b) * ld + rb + r] = o;
+      for (int r = 0; r < 4; ++r) {
+        const int row = 16 * rblk + 4 * r + kr;
+        if (row < rows) {
+          double2 o;
+          o.x = t1[r] - t2[r];
+          o.y = t3[r] - t1[r] - t2[r];
+          col[row] = o;
+        }
       }
     }
   }
