@@ -1258,166 +1258,185 @@ __global__ __launch_bounds__(QR_THREADS) void qr_tblock_batched_kernel(const QrB
   }
 }
 
-// Ypart[chunk][blk] (64 x WCB) = V^H C over the chunk's rows [b0 + 64 chunk,
-// +64), C = the WCB columns of block blk right of the reflector block
-constexpr int WCB = 16;  // columns per workgroup of the block update
-
-__global__ __launch_bounds__(QR_THREADS) void qr_wy_batched_kernel(const QrBatchProb* __restrict__ probs, int b0) {
-  __shared__ double2 vs[WRH][QBW + 1];
-  __shared__ double2 cs[WRH][WCB + 1];
-  const QrBatchProb& q = probs[blockIdx.z];
-  const int k = min(q.m, q.n);
-  if (b0 >= k) return;
-  const int kb = min(QBW, k - b0);
-  const int cb0 = b0 + kb + (int)blockIdx.x * WCB;
-  if (cb0 >= q.n) return;
-  const int nc = min(WCB, q.n - cb0);
-  const int rb = b0 + (int)blockIdx.y * WRC;
-  if (rb >= q.m) return;
-  const double2* W = reinterpret_cast<const double2*>(q.W);
-  const int tid = threadIdx.x;
-  constexpr int CPT = WCB * QBW / QR_THREADS;  // 8 outputs per thread
-  const int l = tid % QBW, cc0 = CPT * (tid / QBW);
-  double yr[CPT], yi[CPT];
-#pragma unroll
-  for (int j = 0; j < CPT; ++j) {
-    yr[j] = 0.0;
-    yi[j] = 0.0;
-  }
-  for (int h = 0; h < WRC; h += WRH) {
-    const int r0 = rb + h;
-    __syncthreads();
-    for (int e = tid; e < WRH * QBW; e += QR_THREADS) {
-      const int rr = e % WRH, ll = e / WRH, r = r0 + rr;
-      vs[rr][ll] = (r < q.m && ll < kb) ? blk_v(W, q.m, b0, r, ll) : make_double2(0.0, 0.0);
-    }
-    for (int e = tid; e < WRH * WCB; e += QR_THREADS) {
-      const int rr = e % WRH, cc = e / WRH, r = r0 + rr;
-      cs[rr][cc] = (r < q.m && cc < nc) ? W[(size_t)(cb0 + cc) * q.m + r] : make_double2(0.0, 0.0);
-    }
-    __syncthreads();
-    for (int rr = 0; rr < WRH; ++rr) {
-      const double2 v = vs[rr][l];
-#pragma unroll
-      for (int j = 0; j < CPT; ++j) {
-        const double2 c = cs[rr][cc0 + j];
-        yr[j] += v.x * c.x + v.y * c.y;  // conj(v) c
-        yi[j] += v.x * c.y - v.y * c.x;
-      }
-    }
-  }
-  double* out = q.Yp + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * QBW * WCB * 2;
-#pragma unroll
-  for (int j = 0; j < CPT; ++j) {
-    out[2 * (l * WCB + cc0 + j)] = yr[j];
-    out[2 * (l * WCB + cc0 + j) + 1] = yi[j];
-  }
+// The block reflector on the columns right of the block in one launch, on
+// FP64 MFMA (v_mfma_f64_16x16x4): per workgroup 32 columns of C (the trailing
+// columns of W, rows b0 .. m) and
+//   Y = V^H C (64 x 32),  Z = T^H Y,  C <- C - V Z,
+// i.e. C <- (I - V T^H V^H) C.  Y: the rows in 32-row passes staged through
+// LDS (V transposed for the A operand); Z: T^H read from TB (L2) as the A
+// operand, Y from LDS; C - V Z: V read as the A operand straight from W,
+// Z from LDS, C read and written from the accumulators.  Every complex
+// product is three real ones (Gauss); wave w owns output rows 16 w .. 16 w + 15
+// of each 64-row tile and both 16-column halves.
+typedef double qr_d4v __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ qr_d4v qr_mfma(double a, double b, qr_d4v c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
+constexpr int WF_COLS = 32;  // trailing columns per workgroup
+constexpr int WF_RP = 32;    // rows per staged pass of Y
+struct WfSmem {
+  union {
+    struct {
+      double vr[WF_RP][QBW + 1], vi[WF_RP][QBW + 1];
+      double cr[WF_RP][WF_COLS + 1], ci[WF_RP][WF_COLS + 1];
+    } y;
+    struct {
+      double zr[QBW][WF_COLS + 1], zi[QBW][WF_COLS + 1];
+    } z;
+  };
+};
 
-// Z = T^H sum_chunks Ypart (64 x WCB), over the block's chunk-0 partial
-__global__ __launch_bounds__(QR_THREADS) void qr_wz_batched_kernel(const QrBatchProb* __restrict__ probs, int b0) {
-  __shared__ double2 ys[QBW][WCB + 1];
-  __shared__ double2 ts[QBW][QBW + 1];  // ts[i][j] = T(i, j)
-  const QrBatchProb& q = probs[blockIdx.z];
+__global__ __launch_bounds__(QR_THREADS, 2) void qr_wfused_batched_kernel(const QrBatchProb* __restrict__ probs, int b0) {
+  __shared__ WfSmem sm;
+  const QrBatchProb& q = probs[blockIdx.y];
   const int k = min(q.m, q.n);
   if (b0 >= k) return;
   const int kb = min(QBW, k - b0);
-  const int cb0 = b0 + kb + (int)blockIdx.x * WCB;
+  const int cb0 = b0 + kb + (int)blockIdx.x * WF_COLS;
   if (cb0 >= q.n) return;
-  const int nch = (q.m - b0 + WRC - 1) / WRC;
-  const int tid = threadIdx.x;
-  constexpr int CPT = WCB * QBW / QR_THREADS;
-  const int a = tid % QBW, cc0 = CPT * (tid / QBW);
-  const int nblk = gridDim.x;
-  for (int j = 0; j < CPT; ++j) {
-    double sr = 0.0, si = 0.0;
-    for (int ch = 0; ch < nch; ++ch) {
-      const double* in = q.Yp + ((size_t)ch * nblk + blockIdx.x) * QBW * WCB * 2;
-      sr += in[2 * (a * WCB + cc0 + j)];
-      si += in[2 * (a * WCB + cc0 + j) + 1];
-    }
-    ys[a][cc0 + j] = make_double2(sr, si);
-  }
-  for (int e = tid; e < QBW * QBW; e += QR_THREADS) ts[e % QBW][e / QBW] = make_double2(q.TB[2 * e], q.TB[2 * e + 1]);
-  __syncthreads();
-  double zr[CPT], zi[CPT];
-#pragma unroll
-  for (int j = 0; j < CPT; ++j) {
-    zr[j] = 0.0;
-    zi[j] = 0.0;
-  }
-  if (a < kb) {
-    for (int b = 0; b <= a; ++b) {  // conj(T(b, a)), T upper triangular
-      const double2 t = ts[b][a];
-#pragma unroll
-      for (int j = 0; j < CPT; ++j) {
-        const double2 y = ys[b][cc0 + j];
-        zr[j] += t.x * y.x + t.y * y.y;
-        zi[j] += t.x * y.y - t.y * y.x;
-      }
-    }
-  }
-  double* out = q.Yp + (size_t)blockIdx.x * QBW * WCB * 2;
-#pragma unroll
-  for (int j = 0; j < CPT; ++j) {
-    out[2 * (a * WCB + cc0 + j)] = zr[j];
-    out[2 * (a * WCB + cc0 + j) + 1] = zi[j];
-  }
-}
-
-// C[chunk rows, block blk] -= V Z
-__global__ __launch_bounds__(QR_THREADS) void qr_wx_batched_kernel(const QrBatchProb* __restrict__ probs, int b0) {
-  __shared__ double2 vs[WRH][QBW + 1];
-  __shared__ double2 zs[QBW][WCB + 1];
-  const QrBatchProb& q = probs[blockIdx.z];
-  const int k = min(q.m, q.n);
-  if (b0 >= k) return;
-  const int kb = min(QBW, k - b0);
-  const int cb0 = b0 + kb + (int)blockIdx.x * WCB;
-  if (cb0 >= q.n) return;
-  const int nc = min(WCB, q.n - cb0);
-  const int rb = b0 + (int)blockIdx.y * WRC;
-  if (rb >= q.m) return;
+  const int nc = min(WF_COLS, q.n - cb0);
+  const int m = q.m;
   double2* W = reinterpret_cast<double2*>(q.W);
-  const int tid = threadIdx.x;
-  {
-    const double* z = q.Yp + (size_t)blockIdx.x * QBW * WCB * 2;
-    for (int e = tid; e < QBW * WCB; e += QR_THREADS) zs[e / WCB][e % WCB] = make_double2(z[2 * e], z[2 * e + 1]);
-  }
-  constexpr int CPT = WCB * WRH / QR_THREADS;  // 4 columns per thread
-  const int rr = tid % WRH, cc0 = CPT * (tid / WRH);
-  for (int h = 0; h < WRC; h += WRH) {
-    const int r0 = rb + h;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  // ---- Y = V^H C: T1 = Vr^T Cr, T2 = Vi^T Ci, T3 = (Vr + Vi)^T (Cr - Ci);
+  //      Re Y = T1 + T2, Im Y = T1 - T2 - T3
+  qr_d4v t1[2], t2[2], t3[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) t1[h] = t2[h] = t3[h] = qr_d4v{0, 0, 0, 0};
+  // the next pass's V and C entries are loaded into registers while the
+  // current pass multiplies (one memory latency per pass, not one per load)
+  constexpr int NV = WF_RP * QBW / QR_THREADS, NCC = WF_RP * WF_COLS / QR_THREADS;
+  double2 pv[NV], pc[NCC];
+  auto load_pass = [&](int r0) {
+#pragma unroll
+    for (int u = 0; u < NV; ++u) {
+      const int e = tid + u * QR_THREADS;
+      const int rr = e % WF_RP, l = e / WF_RP, r = r0 + rr;
+      pv[u] = (r < m && l < kb) ? blk_v(W, m, b0, r, l) : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int u = 0; u < NCC; ++u) {
+      const int e = tid + u * QR_THREADS;
+      const int rr = e % WF_RP, c = e / WF_RP, r = r0 + rr;
+      pc[u] = (r < m && c < nc) ? W[(size_t)(cb0 + c) * m + r] : make_double2(0.0, 0.0);
+    }
+  };
+  load_pass(b0);
+  for (int r0 = b0; r0 < m; r0 += WF_RP) {
     __syncthreads();
-    for (int e = tid; e < WRH * QBW; e += QR_THREADS) {
-      const int r2 = e % WRH, ll = e / WRH, r = r0 + r2;
-      vs[r2][ll] = (r < q.m && ll < kb) ? blk_v(W, q.m, b0, r, ll) : make_double2(0.0, 0.0);
+#pragma unroll
+    for (int u = 0; u < NV; ++u) {
+      const int e = tid + u * QR_THREADS;
+      sm.y.vr[e % WF_RP][e / WF_RP] = pv[u].x;
+      sm.y.vi[e % WF_RP][e / WF_RP] = pv[u].y;
+    }
+#pragma unroll
+    for (int u = 0; u < NCC; ++u) {
+      const int e = tid + u * QR_THREADS;
+      sm.y.cr[e % WF_RP][e / WF_RP] = pc[u].x;
+      sm.y.ci[e % WF_RP][e / WF_RP] = pc[u].y;
     }
     __syncthreads();
-    const int r = r0 + rr;
-    if (r < q.m) {
-      double sr[CPT], si[CPT];
+    if (r0 + WF_RP < m) load_pass(r0 + WF_RP);
+#pragma unroll 2
+    for (int st = 0; st < WF_RP / 4; ++st) {
+      const int row = 4 * st + lk;
+      const double ar = sm.y.vr[row][16 * w + li], ai = sm.y.vi[row][16 * w + li];
 #pragma unroll
-      for (int j = 0; j < CPT; ++j) {
-        sr[j] = 0.0;
-        si[j] = 0.0;
+      for (int h = 0; h < 2; ++h) {
+        const double br = sm.y.cr[row][16 * h + li], bi = sm.y.ci[row][16 * h + li];
+        t1[h] = qr_mfma(ar, br, t1[h]);
+        t2[h] = qr_mfma(ai, bi, t2[h]);
+        t3[h] = qr_mfma(ar + ai, br - bi, t3[h]);
       }
-      for (int l = 0; l < kb; ++l) {
-        const double2 v = vs[rr][l];
+    }
+  }
+  // Y into LDS: row 16 w + 4 j + lk, column 16 h + li
+  __syncthreads();
 #pragma unroll
-        for (int j = 0; j < CPT; ++j) {
-          const double2 z = zs[l][cc0 + j];
-          sr[j] += v.x * z.x - v.y * z.y;
-          si[j] += v.x * z.y + v.y * z.x;
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      sm.z.zr[16 * w + 4 * j + lk][16 * h + li] = t1[h][j] + t2[h][j];
+      sm.z.zi[16 * w + 4 * j + lk][16 * h + li] = t1[h][j] - t2[h][j] - t3[h][j];
+    }
+  __syncthreads();
+  // ---- Z = T^H Y: A = conj(T)^T (T upper triangular, column-major in TB),
+  //      Re Z = T1 + T2, Im Z = T1 - T2 - T3 as above
+#pragma unroll
+  for (int h = 0; h < 2; ++h) t1[h] = t2[h] = t3[h] = qr_d4v{0, 0, 0, 0};
+  const int a = 16 * w + li;  // Z row of this lane's A operand
+  const double2* TB2 = reinterpret_cast<const double2*>(q.TB);
+  for (int s0 = 0; s0 < QBW / 4; s0 += 8) {  // eight K steps' T entries in flight
+    double2 tv[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int bb = 4 * (s0 + u) + lk;
+      tv[u] = (bb <= a && a < kb) ? TB2[bb + QBW * a] : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int bb = 4 * (s0 + u) + lk;
+      const double tr = tv[u].x, ti = tv[u].y;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const double yr = sm.z.zr[bb][16 * h + li], yi = sm.z.zi[bb][16 * h + li];
+        t1[h] = qr_mfma(tr, yr, t1[h]);
+        t2[h] = qr_mfma(ti, yi, t2[h]);
+        t3[h] = qr_mfma(tr + ti, yr - yi, t3[h]);
+      }
+    }
+  }
+  __syncthreads();  // every wave has read Y
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      sm.z.zr[16 * w + 4 * j + lk][16 * h + li] = t1[h][j] + t2[h][j];
+      sm.z.zi[16 * w + 4 * j + lk][16 * h + li] = t1[h][j] - t2[h][j] - t3[h][j];
+    }
+  __syncthreads();
+  // ---- C <- C - V Z over 64-row tiles: T1 = Vr Zr, T2 = Vi Zi,
+  //      T3 = (Vr + Vi)(Zr + Zi); Re = T1 - T2, Im = T3 - T1 - T2
+  for (int r0 = b0; r0 < m; r0 += QBW) {
+    const int r = r0 + 16 * w + li;  // this lane's V row (A operand)
+#pragma unroll
+    for (int h = 0; h < 2; ++h) t1[h] = t2[h] = t3[h] = qr_d4v{0, 0, 0, 0};
+    for (int s0 = 0; s0 < QBW / 4; s0 += 8) {  // eight K steps' V entries in flight
+      double2 vv[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int l = 4 * (s0 + u) + lk;
+        vv[u] = (r < m && l < kb) ? blk_v(W, m, b0, r, l) : make_double2(0.0, 0.0);
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int l = 4 * (s0 + u) + lk;
+        const double2 v = vv[u];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const double zr = sm.z.zr[l][16 * h + li], zi = sm.z.zi[l][16 * h + li];
+          t1[h] = qr_mfma(v.x, zr, t1[h]);
+          t2[h] = qr_mfma(v.y, zi, t2[h]);
+          t3[h] = qr_mfma(v.x + v.y, zr + zi, t3[h]);
         }
       }
+    }
+    // accumulator j of this lane: row r0 + 16 w + 4 j + lk, column 16 h + li
 #pragma unroll
-      for (int j = 0; j < CPT; ++j) {
-        if (cc0 + j < nc) {
-          double2 x = W[(size_t)(cb0 + cc0 + j) * q.m + r];
-          x.x -= sr[j];
-          x.y -= si[j];
-          W[(size_t)(cb0 + cc0 + j) * q.m + r] = x;
+    for (int h = 0; h < 2; ++h) {
+      const int c = 16 * h + li;
+      if (c >= nc) continue;
+      double2* col = W + (size_t)(cb0 + c) * m;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int rr = r0 + 16 * w + 4 * j + lk;
+        if (rr < m) {
+          double2 x = col[rr];
+          x.x -= t1[h][j] - t2[h][j];
+          x.y -= t3[h][j] - t1[h][j] - t2[h][j];
+          col[rr] = x;
         }
       }
     }
@@ -1445,9 +1464,9 @@ static int qr_apply_chunked(const double* W, const double* T, double* M, int m, 
 // per panel one launch of the panel kernel over the problems and one pair of
 // trailing-update launches.  Every m <= 512 (the register panel).
 size_t qr_batched_yp_bytes(int64_t m, int64_t max_n) {
-  // the block update's partials (64 x WCB per chunk and column block; the
-  // 64 x 64 cross products per chunk) and the panels' (16 x 16)
-  return (size_t)((m + WRC - 1) / WRC) * (size_t)((max_n + WCB - 1) / WCB + 2) * QBW * WCB * 16 +
+  // the reflector block's cross products (64 x 64 per 64-row chunk) and the
+  // panels' trailing-update partials (16 x 16 per chunk and column block)
+  return (size_t)((m + WRC - 1) / WRC) * QBW * QBW * 16 +
          (size_t)((m + QA_RC - 1) / QA_RC) * (size_t)((max_n + QB - 1) / QB) * QB * QB * 16;
 }
 size_t qr_batched_tb_bytes() { return (size_t)QBW * QBW * 16; }
@@ -1489,16 +1508,13 @@ int launch_qr_batched(const QrBatchProb* probs, int64_t nprob, int max_m, int ma
     // the block reflector on the columns right of the block (a problem's
     // start at b0 + kb >= b0 + 1)
     if (max_n - b0 <= 1) continue;
-    const dim3 g3((unsigned)((max_n - b0 - 1 + WCB - 1) / WCB), (unsigned)((max_m - b0 + WRC - 1) / WRC), (unsigned)nprob);
-    hipLaunchKernelGGL(qr_sblock_batched_kernel, dim3(g3.y, g3.z), dim3(QR_THREADS), 0, s, probs, b0);
+    const unsigned nchunk = (unsigned)((max_m - b0 + WRC - 1) / WRC);
+    hipLaunchKernelGGL(qr_sblock_batched_kernel, dim3(nchunk, (unsigned)nprob), dim3(QR_THREADS), 0, s, probs, b0);
     TENET_HIP_CHECK(hipGetLastError());
     hipLaunchKernelGGL(qr_tblock_batched_kernel, dim3((unsigned)nprob), dim3(QR_THREADS), 0, s, probs, b0);
     TENET_HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(qr_wy_batched_kernel, g3, dim3(QR_THREADS), 0, s, probs, b0);
-    TENET_HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(qr_wz_batched_kernel, dim3(g3.x, 1, g3.z), dim3(QR_THREADS), 0, s, probs, b0);
-    TENET_HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(qr_wx_batched_kernel, g3, dim3(QR_THREADS), 0, s, probs, b0);
+    const dim3 gf((unsigned)((max_n - b0 - 1 + WF_COLS - 1) / WF_COLS), (unsigned)nprob);
+    hipLaunchKernelGGL(qr_wfused_batched_kernel, gf, dim3(QR_THREADS), 0, s, probs, b0);
     TENET_HIP_CHECK(hipGetLastError());
   }
   return TENET_OK;
