@@ -498,15 +498,9 @@ __device__ __forceinline__ void jacobi_inner(SvSmem& sm, int n2, double tol, dou
 // blocks (closed form).
 constexpr int XB = 16;         // b
 constexpr int XROT_WAVE = 7;   // rotation wave
-constexpr int XW_THREADS = 192;  // W threads: waves 4-6, 512 elements, 3 each
 
 __device__ __forceinline__ void jacobi_cross32(SvSmem& sm, double tol, double abs_tol) {
   const int tid = threadIdx.x, wave = tid >> 6, ln = tid & 63;
-  for (int e = tid; e < 2 * XB * 2 * XB; e += SR_THREADS) {
-    const int i = e % (2 * XB), c = e / (2 * XB);
-    sm.wr[i][c] = (i == c) ? 1.0 : 0.0;
-    sm.wi[i][c] = 0.0;
-  }
   if (tid == 0) sm.any_outer = 0;
   const double tol2 = tol * tol, abs2 = abs_tol * abs_tol;
   // roles (wave-uniform)
@@ -525,19 +519,24 @@ __device__ __forceinline__ void jacobi_cross32(SvSmem& sm, double tol, double ab
     bk = k;
     bl = k + 1 + rem;
   }
-  // W: element e = (tid - 256) + 192 u is (row e / 16, slot e % 16)
-  int wrow[3], wslot[3];
-  bool wown[3];
-#pragma unroll
-  for (int u = 0; u < 3; ++u) {
-    const int e = (tid - 4 * 64) + XW_THREADS * u;
-    wown[u] = wwave && e < 2 * XB * XB;
-    const int ec = wown[u] ? e : 0;
-    wrow[u] = ec / XB;
-    wslot[u] = ec % XB;
-  }
   // rotation wave: lane k < 16 owns slot k
   const int k = ln & (XB - 1), kn = (k + 1) & (XB - 1);
+  // W lives in the registers of waves 4-6: lane k of 16-lane row group wg
+  // (0..11) holds, for rows wg, wg + 12 and wg + 24 (< 32), P = W[row][k]
+  // (slot k's p column is k at every step) and Q = W[row][q_k(step)] with
+  // q_k(s) = 16 + (k + s) mod 16; since q_k(s + 1) = q_{k+1}(s), Q moves
+  // one lane down between steps (a DPP row rotation), and W goes to LDS once
+  // at the end.
+  const int wg = wwave ? ((tid - 4 * 64) >> 4) : 0;
+  double Pr[3], Pi[3], Qr[3], Qi[3];
+#pragma unroll
+  for (int u = 0; u < 3; ++u) {
+    const int row = wg + 12 * u;
+    Pr[u] = row == k ? 1.0 : 0.0;
+    Pi[u] = 0.0;
+    Qr[u] = row == XB + k ? 1.0 : 0.0;
+    Qi[u] = 0.0;
+  }
   const bool rown = rwave && ln < XB;
   Rot rt;
   double gr = 0.0, gi = 0.0;  // G[k][q_k] the current rotation was formed from
@@ -594,17 +593,23 @@ __device__ __forceinline__ void jacobi_cross32(SvSmem& sm, double tol, double ab
         ni[q1][ph] = -n1i;
       }
     } else if (wwave) {
+      if (step > 0) {
 #pragma unroll
-      for (int u = 0; u < 3; ++u) {
-        const int sl = wslot[u], row = wrow[u];
-        const int wp = sl, wq = XB + ((sl + step) & (XB - 1));
-        const double pr = sm.wr[row][wp], pi = sm.wi[row][wp], qr = sm.wr[row][wq], qi = sm.wi[row][wq];
-        if (!wown[u] || !sm.rpq[par][sl]) continue;
-        const double c = sm.rcs[par][sl][0], er = sm.rcs[par][sl][1], ei = sm.rcs[par][sl][2];
-        sm.wr[row][wp] = c * pr - (er * qr + ei * qi);
-        sm.wi[row][wp] = c * pi - (er * qi - ei * qr);
-        sm.wr[row][wq] = (er * pr - ei * pi) + c * qr;
-        sm.wi[row][wq] = (er * pi + ei * pr) + c * qi;
+        for (int u = 0; u < 3; ++u) {
+          Qr[u] = from_next_lane(Qr[u]);
+          Qi[u] = from_next_lane(Qi[u]);
+        }
+      }
+      if (sm.rpq[par][k]) {
+        const double c = sm.rcs[par][k][0], er = sm.rcs[par][k][1], ei = sm.rcs[par][k][2];
+#pragma unroll
+        for (int u = 0; u < 3; ++u) {
+          const double pr = Pr[u], pi = Pi[u], qr = Qr[u], qi = Qi[u];
+          Pr[u] = c * pr - (er * qr + ei * qi);
+          Pi[u] = c * pi - (er * qi - ei * qr);
+          Qr[u] = (er * pr - ei * pi) + c * qr;
+          Qi[u] = (er * pi + ei * pr) + c * qi;
+        }
       }
     } else {
       // rotation wave: this step's diagonal blocks, then the next step's rotations
@@ -650,6 +655,19 @@ __device__ __forceinline__ void jacobi_cross32(SvSmem& sm, double tol, double ab
       }
     }
     __syncthreads();
+  }
+  if (wwave) {
+    const int q = XB + ((k + XB - 1) & (XB - 1));  // q_k of the last step
+#pragma unroll
+    for (int u = 0; u < 3; ++u) {
+      const int row = wg + 12 * u;
+      if (row < 2 * XB) {
+        sm.wr[row][k] = Pr[u];
+        sm.wi[row][k] = Pi[u];
+        sm.wr[row][q] = Qr[u];
+        sm.wi[row][q] = Qi[u];
+      }
+    }
   }
   if (rown) sm.rmax[k] = rmax2;
   if (rown && any) sm.any_outer = 1;
