@@ -491,11 +491,13 @@ __device__ __forceinline__ void jacobi_inner(SvSmem& sm, int n2, double tol, dou
 //   G'[k][q_{k+1}] = row p_k, column q_{k+1} of J_k^H B J_{k+1}, B the 2x2
 //                    block {k, q_k} x {k+1, q_{k+1}} of G before step s,
 // all known from G_s and step s's rotations.  Wave 7 (lanes 0-15: one slot
-// each) forms them and step s + 1's rotations while waves 0-3 update the G
-// blocks above the diagonal and waves 4-6 rotate W with step s's rotations;
-// the rotations pass through LDS slots double-buffered by step parity, so a
-// step still takes one workgroup barrier.  Wave 7 also writes the diagonal
-// blocks (closed form).
+// each) forms them and step s + 1's rotations while 120 threads of waves
+// 0-1 update the G blocks above the diagonal (one 2 x 2 block each, read
+// once, both rows written with their mirrors) and waves 4-6 rotate W, which
+// they keep in registers, with step s's rotations; the rotations pass
+// through LDS slots double-buffered by step parity, so a step still takes
+// one workgroup barrier.  Wave 7 also writes the diagonal blocks (closed
+// form).
 constexpr int XB = 16;         // b
 constexpr int XROT_WAVE = 7;   // rotation wave
 
@@ -505,11 +507,12 @@ __device__ __forceinline__ void jacobi_cross32(SvSmem& sm, double tol, double ab
   const double tol2 = tol * tol, abs2 = abs_tol * abs_tol;
   // roles (wave-uniform)
   const bool gwave = wave < 4, wwave = wave >= 4 && wave < XROT_WAVE, rwave = wave == XROT_WAVE;
-  // G: blocks bk < bl in row-major order, two threads (output rows) each
-  int bk = 0, bl = 1, half = tid & 1;
+  // G: blocks bk < bl in row-major order, one thread each (both output rows:
+  // the 2 x 2 block is read once)
+  int bk = 0, bl = 1;
   bool gown = false;
   if (gwave) {
-    const int u = tid >> 1;
+    const int u = tid;
     gown = u < XB * (XB - 1) / 2;
     int rem = gown ? u : 0, k = 0;
     while (k < XB - 1 && rem >= XB - 1 - k) {
@@ -579,18 +582,20 @@ __device__ __forceinline__ void jacobi_cross32(SvSmem& sm, double tol, double ab
       const double ck = sm.rcs[par][bk][0], ekr = sm.rcs[par][bk][1], eki = sm.rcs[par][bk][2];
       const double cl = sm.rcs[par][bl][0], elr = sm.rcs[par][bl][1], eli = sm.rcs[par][bl][2];
       if (gown) {
-        const bool h1 = half != 0;
-        double n0r, n0i, n1r, n1i;
-        block_row(h1, ck, ekr, eki, cl, elr, eli, b, n0r, n0i, n1r, n1i);
-        const int ph = h1 ? q0 : p0;
-        nr[ph][p1] = n0r;
-        ni[ph][p1] = n0i;
-        nr[ph][q1] = n1r;
-        ni[ph][q1] = n1i;
-        nr[p1][ph] = n0r;
-        ni[p1][ph] = -n0i;
-        nr[q1][ph] = n1r;
-        ni[q1][ph] = -n1i;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          double n0r, n0i, n1r, n1i;
+          block_row(h != 0, ck, ekr, eki, cl, elr, eli, b, n0r, n0i, n1r, n1i);
+          const int ph = h ? q0 : p0;
+          nr[ph][p1] = n0r;
+          ni[ph][p1] = n0i;
+          nr[ph][q1] = n1r;
+          ni[ph][q1] = n1i;
+          nr[p1][ph] = n0r;
+          ni[p1][ph] = -n0i;
+          nr[q1][ph] = n1r;
+          ni[q1][ph] = -n1i;
+        }
       }
     } else if (wwave) {
       if (step > 0) {
