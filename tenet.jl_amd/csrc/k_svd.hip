@@ -493,7 +493,8 @@ __device__ __forceinline__ void jacobi_inner(SvSmem& sm, int n2, double tol, dou
 // all known from G_s and step s's rotations.  Wave 7 (lanes 0-15: one slot
 // each) forms them and step s + 1's rotations while 120 threads of waves
 // 0-1 update the G blocks above the diagonal (one 2 x 2 block each, read
-// once, both rows written with their mirrors) and waves 4-6 rotate W, which
+// once; only G's upper triangle is kept, entries below it read as
+// conjugated mirrors) and waves 4-6 rotate W, which
 // they keep in registers, with step s's rotations; the rotations pass
 // through LDS slots double-buffered by step parity, so a step still takes
 // one workgroup barrier.  Wave 7 also writes the diagonal blocks (closed
@@ -568,17 +569,21 @@ __device__ __forceinline__ void jacobi_cross32(SvSmem& sm, double tol, double ab
     GramRow* nr = par ? sm.gr : sm.hr;
     GramRow* ni = par ? sm.gi : sm.hi;
     if (gwave) {
+      // only the upper triangle of G is kept up to date: an entry below the
+      // diagonal is read as the conjugate of its mirror
       const int p0 = bk, q0 = XB + ((bk + step) & (XB - 1));
       const int p1 = bl, q1 = XB + ((bl + step) & (XB - 1));
+      const int qa = min(q0, q1), qb = max(q0, q1);
+      const double qs = q0 <= q1 ? 1.0 : -1.0;
       double b[8];
       b[0] = cr[p0][p1];
       b[1] = ci[p0][p1];
       b[2] = cr[p0][q1];
       b[3] = ci[p0][q1];
-      b[4] = cr[q0][p1];
-      b[5] = ci[q0][p1];
-      b[6] = cr[q0][q1];
-      b[7] = ci[q0][q1];
+      b[4] = cr[p1][q0];
+      b[5] = -ci[p1][q0];
+      b[6] = cr[qa][qb];
+      b[7] = qs * ci[qa][qb];
       const double ck = sm.rcs[par][bk][0], ekr = sm.rcs[par][bk][1], eki = sm.rcs[par][bk][2];
       const double cl = sm.rcs[par][bl][0], elr = sm.rcs[par][bl][1], eli = sm.rcs[par][bl][2];
       if (gown) {
@@ -586,15 +591,17 @@ __device__ __forceinline__ void jacobi_cross32(SvSmem& sm, double tol, double ab
         for (int h = 0; h < 2; ++h) {
           double n0r, n0i, n1r, n1i;
           block_row(h != 0, ck, ekr, eki, cl, elr, eli, b, n0r, n0i, n1r, n1i);
-          const int ph = h ? q0 : p0;
-          nr[ph][p1] = n0r;
-          ni[ph][p1] = n0i;
-          nr[ph][q1] = n1r;
-          ni[ph][q1] = n1i;
-          nr[p1][ph] = n0r;
-          ni[p1][ph] = -n0i;
-          nr[q1][ph] = n1r;
-          ni[q1][ph] = -n1i;
+          if (h == 0) {
+            nr[p0][p1] = n0r;
+            ni[p0][p1] = n0i;
+            nr[p0][q1] = n1r;
+            ni[p0][q1] = n1i;
+          } else {
+            nr[p1][q0] = n0r;
+            ni[p1][q0] = -n0i;
+            nr[qa][qb] = n1r;
+            ni[qa][qb] = qs * n1i;
+          }
         }
       }
     } else if (wwave) {
@@ -620,15 +627,17 @@ __device__ __forceinline__ void jacobi_cross32(SvSmem& sm, double tol, double ab
       // rotation wave: this step's diagonal blocks, then the next step's rotations
       const int q = XB + ((k + step) & (XB - 1));
       const int qn = XB + ((kn + step) & (XB - 1));
+      const int ka = min(k, kn), kb2 = max(k, kn), qa = min(q, qn), qb = max(q, qn);
+      const double ks = k <= kn ? 1.0 : -1.0, qs = q <= qn ? 1.0 : -1.0;
       double b[8];
-      b[0] = cr[k][kn];
-      b[1] = ci[k][kn];
+      b[0] = cr[ka][kb2];
+      b[1] = ks * ci[ka][kb2];
       b[2] = cr[k][qn];
       b[3] = ci[k][qn];
-      b[4] = cr[q][kn];
-      b[5] = ci[q][kn];
-      b[6] = cr[q][qn];
-      b[7] = ci[q][qn];
+      b[4] = cr[kn][q];
+      b[5] = -ci[kn][q];
+      b[6] = cr[qa][qb];
+      b[7] = qs * ci[qa][qb];
       if (rown) {
         const double ogr = rt.act ? 0.0 : gr, ogi = rt.act ? 0.0 : gi;
         nr[k][k] = rt.an;
@@ -637,8 +646,6 @@ __device__ __forceinline__ void jacobi_cross32(SvSmem& sm, double tol, double ab
         ni[q][q] = 0.0;
         nr[k][q] = ogr;
         ni[k][q] = ogi;
-        nr[q][k] = ogr;
-        ni[q][k] = -ogi;
       }
       if (step + 1 < XB) {
         // slot k + 1's rotation and new diagonal d' (lanes 0-15 hold slots 0-15)
