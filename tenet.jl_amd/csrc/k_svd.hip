@@ -192,15 +192,21 @@ __device__ void gram_over_rows(SvSmem& sm, const double* X, int64_t ld, int r0, 
 // p_k < q_k), G <- J^H G J and W <- W J.  (The cross rounds, which pair only
 // columns of different blocks, run jacobi_cross32 below.)
 //
-// One workgroup barrier per step.  G is double-buffered (the second buffer
-// lies over the row-staging area, idle during the solve).  Every wave
-// computes the step's rotations in its lanes 0..n2/2-1 from the current
-// buffer and passes them to its other lanes through its own LDS slots
-// (the same code on the same operands in every wave, so all waves hold the
-// same rotations; LDS operations of one wave complete in order, so no
-// barrier is needed for that exchange).  G stays exactly Hermitian: only
-// the blocks above the diagonal are computed (their mirrors are written as
-// conjugates), and the diagonal 2x2 blocks take the closed-form update.
+// One workgroup barrier per step, with the rotations formed one step ahead.
+// G is double-buffered (the second buffer lies over the row-staging area,
+// idle during the solve) and only its upper triangle is kept current
+// (entries below it read as conjugated mirrors).  In the circle order the
+// columns of slot k at step s + 1 come from two different slots k1, k2 of
+// step s, so the entries step s + 1's rotation k needs are
+//   G'[p'][p'], G'[q'][q'] = new diagonals of rotations k1, k2 of step s,
+//   G'[p'][q']             = one entry of J_k1^H B J_k2, B the 2x2 block
+//                            {p_k1, q_k1} x {p_k2, q_k2} of G before step s,
+// all known from G_s and step s's rotations.  Wave 7 (one lane per slot)
+// writes step s's diagonal blocks (closed form) and forms step s + 1's
+// rotations while waves 0-1 update the off-diagonal 2 x 2 blocks (one thread
+// per block: both output rows) and waves 2-6 rotate W with step s's
+// rotations; the rotations (c, s and the new diagonal) pass through LDS
+// slots double-buffered by step parity.
 // columns of slot k at step s of the circle order
 __device__ __forceinline__ void circle_pair(int k, int step, int n2, int& p, int& q) {
   // circle_pos without the modulo: i - 1 + r < 2 (n2 - 1)
@@ -290,24 +296,6 @@ __device__ __forceinline__ Rot form_rotation(double ra, double rd, double rxr, d
   return o;
 }
 
-// Per-thread ownership in the inner solve (fixed over the steps).  G is kept
-// Hermitian in full: waves 0-3 update the off-diagonal 2x2 blocks above the
-// diagonal (bk < bl, one output row per thread) and write their conjugate
-// mirrors; the rotation lanes of wave 0 write the diagonal blocks; waves 4-7
-// rotate W.  Lanes < npair of every wave form the step's rotations, so no
-// wave waits on another before its update.
-constexpr int GW_THREADS = SR_THREADS / 2;
-struct InnerOwn {
-  int rslot;  // rotation slot of this lane (lanes >= npair repeat slot 0)
-  bool rown;  // lane < npair
-  bool gwave;
-  int bk, bl;  // G block {p_bk, q_bk} x {p_bl, q_bl}, bk < bl
-  int half;    // output row of the block: 0 -> p_bk, 1 -> q_bk
-  bool gown;
-  int wrow[2], wslot[2];  // W elements (row, slot)
-  bool wown[2];
-};
-
 // row h of B' = J_k^H B J_l for the 2x2 block B = [B0; B1] and the rotations
 // J = [[c, e], [-conj(e), c]]: row h of T = J_k^H B is a B0 + b B1 with
 // (a, b) = (c_k, -e_k) for h = 0 and (conj(e_k), c_k) for h = 1, and row h of
@@ -328,108 +316,43 @@ __device__ __forceinline__ void block_row(bool h1, double ck, double ekr, double
   n1i = (elr * t0i + eli * t0r) + cl * t1i;
 }
 
-// One step; wave w = threadIdx.x / 64 publishes its rotations in rcs[w], rpq[w].
-// The G and W entries a thread touches depend only on the step's pairing,
-// so they are read before the rotations are formed.  Every thread reads
-// valid entries (unowned units repeat unit 0); only owners store.
-__device__ __forceinline__ void inner_step(SvSmem& sm, const GramRow* cr, const GramRow* ci, GramRow* nr, GramRow* ni,
-                                           int step, int n2, const InnerOwn& o, double tol2, double abs2,
-                                           double& rmax2, int& any) {
-  const int w = threadIdx.x >> 6;
-  int rp, rq;
-  circle_pair(o.rslot, step, n2, rp, rq);
-  const double ra = cr[rp][rp], rd = cr[rq][rq], rxr = cr[rp][rq], rxi = ci[rp][rq];
-  int p0 = 0, q0 = 0, p1 = 0, q1 = 0;
-  double b[8];
-  int wp[2], wq[2];
-  double wv[2][4];
-  if (o.gwave) {
-    circle_pair(o.bk, step, n2, p0, q0);
-    circle_pair(o.bl, step, n2, p1, q1);
-    b[0] = cr[p0][p1];
-    b[1] = ci[p0][p1];
-    b[2] = cr[p0][q1];
-    b[3] = ci[p0][q1];
-    b[4] = cr[q0][p1];
-    b[5] = ci[q0][p1];
-    b[6] = cr[q0][q1];
-    b[7] = ci[q0][q1];
-  } else {
-#pragma unroll
-    for (int u = 0; u < 2; ++u) {
-      circle_pair(o.wslot[u], step, n2, wp[u], wq[u]);
-      wv[u][0] = sm.wr[o.wrow[u]][wp[u]];
-      wv[u][1] = sm.wi[o.wrow[u]][wp[u]];
-      wv[u][2] = sm.wr[o.wrow[u]][wq[u]];
-      wv[u][3] = sm.wi[o.wrow[u]][wq[u]];
-    }
-  }
-  const Rot rt = form_rotation(ra, rd, rxr, rxi, tol2, abs2);
-  const int act = rt.act;
-  const double c = rt.c, sr = rt.sr, si = rt.si, an = rt.an, dn = rt.dn;
-  if (o.rown) {
-    sm.rcs[w][o.rslot][0] = c;
-    sm.rcs[w][o.rslot][1] = sr;
-    sm.rcs[w][o.rslot][2] = si;
-    sm.rpq[w][o.rslot] = act;
-    if (w == 0) {
-      // squared relative off-diagonal, for the convergence test
-      rmax2 = fmax(rmax2, rt.rel2);
-      any |= act;
-      // the pair's diagonal block: real diagonal, off-diagonal zeroed when rotated
-      const double gr = act ? 0.0 : rxr, gi = act ? 0.0 : rxi;
-      nr[rp][rp] = an;
-      ni[rp][rp] = 0.0;
-      nr[rq][rq] = dn;
-      ni[rq][rq] = 0.0;
-      nr[rp][rq] = gr;
-      ni[rp][rq] = gi;
-      nr[rq][rp] = gr;
-      ni[rq][rp] = -gi;
-    }
-  }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  if (o.gwave) {
-    if (o.gown) {
-      const int bk = o.bk, bl = o.bl;
-      const double ck = sm.rcs[w][bk][0], ekr = sm.rcs[w][bk][1], eki = sm.rcs[w][bk][2];
-      const double cl = sm.rcs[w][bl][0], elr = sm.rcs[w][bl][1], eli = sm.rcs[w][bl][2];
-      const bool h1 = o.half != 0;
-      double n0r, n0i, n1r, n1i;
-      block_row(h1, ck, ekr, eki, cl, elr, eli, b, n0r, n0i, n1r, n1i);
-      const int ph = h1 ? q0 : p0;
-      nr[ph][p1] = n0r;
-      ni[ph][p1] = n0i;
-      nr[ph][q1] = n1r;
-      ni[ph][q1] = n1i;
-      nr[p1][ph] = n0r;
-      ni[p1][ph] = -n0i;
-      nr[q1][ph] = n1r;
-      ni[q1][ph] = -n1i;
-    }
-  } else {
-    // W <- W J on this thread's elements
-#pragma unroll
-    for (int u = 0; u < 2; ++u) {
-      const int k = o.wslot[u];
-      if (!o.wown[u] || !sm.rpq[w][k]) continue;
-      const double cw = sm.rcs[w][k][0], er = sm.rcs[w][k][1], ei = sm.rcs[w][k][2];
-      const double pr = wv[u][0], pi = wv[u][1], qr = wv[u][2], qi = wv[u][3];
-      const int row = o.wrow[u];
-      sm.wr[row][wp[u]] = cw * pr - (er * qr + ei * qi);
-      sm.wi[row][wp[u]] = cw * pi - (er * qi - ei * qr);
-      sm.wr[row][wq[u]] = (er * pr - ei * pi) + cw * qr;
-      sm.wi[row][wq[u]] = (er * pi + ei * pr) + cw * qi;
-    }
-  }
-  __syncthreads();
+// G entry (r, c), r != c, from the upper triangle (an entry below it is the
+// conjugate of its mirror), and the matching store
+__device__ __forceinline__ void g_get(const GramRow* gr, const GramRow* gi, int r, int c, double& re, double& im) {
+  const int lo = min(r, c), hi = max(r, c);
+  re = gr[lo][hi];
+  im = r < c ? gi[lo][hi] : -gi[lo][hi];
+}
+__device__ __forceinline__ void g_put(GramRow* gr, GramRow* gi, int r, int c, double re, double im) {
+  const int lo = min(r, c), hi = max(r, c);
+  gr[lo][hi] = re;
+  gi[lo][hi] = r < c ? im : -im;
 }
 
+// Slot and side (0: p, 1: q) of column c at step s of the circle order over
+// n2 columns (the inverse of circle_pair: the column at position i >= 1 is
+// 1 + (i - 1 + s) mod (n2 - 1), position i belongs to slot min(i, n2 - 1 - i))
+__device__ __forceinline__ void circle_slot(int c, int step, int n2, int& slot, int& side) {
+  const int m = n2 - 1;
+  int i = 0;
+  if (c != 0) {
+    int t = c - 1 - step;
+    t = t < 0 ? t + m : t;
+    i = 1 + t;
+  }
+  slot = min(i, n2 - 1 - i);
+  int p, q;
+  circle_pair(slot, step, n2, p, q);
+  side = c == q ? 1 : 0;
+}
+
+constexpr int IROT_WAVE = 7;      // rotation wave of the full inner sweep
+constexpr int IW_FIRST = 2 * 64;  // first W thread (waves 2-6)
+constexpr int IW_THREADS = 5 * 64;
+
 __device__ __forceinline__ void jacobi_inner(SvSmem& sm, int n2, double tol, double abs_tol) {
-  const int tid = threadIdx.x;
-  const int npair = n2 / 2;
+  const int tid = threadIdx.x, wave = tid >> 6, ln = tid & 63;
+  const int npair = n2 / 2, nsteps = n2 - 1;
   // W = I on the pair's n2 columns and 0 elsewhere in the 32 x 32 block (the
   // MFMA rotation runs over all 32 columns)
   for (int e = tid; e < B2MAX * B2MAX; e += SR_THREADS) {
@@ -438,47 +361,149 @@ __device__ __forceinline__ void jacobi_inner(SvSmem& sm, int n2, double tol, dou
     sm.wi[i][c] = 0.0;
   }
   if (tid == 0) sm.any_outer = 0;
-  InnerOwn o;
-  const int ln = tid & 63;
-  o.rown = ln < npair;
-  o.rslot = o.rown ? ln : 0;
-  o.gwave = tid < GW_THREADS;
-  // G: the npair (npair - 1) / 2 blocks bk < bl in row-major order, two
-  // threads (output rows) each: at most 240 threads
-  {
-    const int nblk = npair * (npair - 1) / 2;
-    const int u = tid >> 1;
-    o.half = tid & 1;
-    o.gown = o.gwave && u < nblk;
-    int rem = o.gown ? u : 0, k = 0;
+  const double tol2 = tol * tol, abs2 = abs_tol * abs_tol;
+  const bool gwave = wave < 2, wwave = tid >= IW_FIRST && tid < IW_FIRST + IW_THREADS, rwave = wave == IROT_WAVE;
+  // G: the npair (npair - 1) / 2 blocks bk < bl in row-major order, one thread each
+  int bk = 0, bl = 1;
+  bool gown = false;
+  if (gwave) {
+    gown = tid < npair * (npair - 1) / 2;
+    int rem = gown ? tid : 0, k = 0;
     while (k < npair - 1 && rem >= npair - 1 - k) {
       rem -= npair - 1 - k;
       ++k;
     }
-    o.bk = k;
-    o.bl = min(k + 1 + rem, npair - 1);
+    bk = k;
+    bl = k + 1 + rem;
   }
-  // W: element e = (tid - 256) + 256 u is (row e / npair, slot e % npair); npair n2 <= 512
+  // W: element e = (tid - IW_FIRST) + IW_THREADS u is (row e / npair, slot e % npair); npair n2 <= 512
+  int wrow[2], wslot[2];
+  bool wown[2];
 #pragma unroll
   for (int u = 0; u < 2; ++u) {
-    const int e = (tid - GW_THREADS) + GW_THREADS * u;
-    o.wown[u] = !o.gwave && e < npair * n2;
-    const int ec = o.wown[u] ? e : 0;
-    o.wrow[u] = ec / npair;
-    o.wslot[u] = ec % npair;
+    const int e = (tid - IW_FIRST) + IW_THREADS * u;
+    wown[u] = wwave && e < npair * n2;
+    const int ec = wown[u] ? e : 0;
+    wrow[u] = ec / npair;
+    wslot[u] = ec % npair;
   }
-  const double tol2 = tol * tol, abs2 = abs_tol * abs_tol;
+  // rotation wave: lane k < npair owns slot k
+  const bool rown = rwave && ln < npair;
+  const int k = rwave ? min(ln, npair - 1) : 0;
+  Rot rt;
+  double rgr = 0.0, rgi = 0.0;  // G[p_k][q_k] the current rotation was formed from
   double rmax2 = 0.0;
   int any = 0;
-  const int nsteps = n2 - 1;
   __syncthreads();
-  // even steps read (gr, gi) and write (hr, hi); odd steps the reverse
-  for (int step = 0; step < nsteps; step += 2) {
-    inner_step(sm, sm.gr, sm.gi, sm.hr, sm.hi, step, n2, o, tol2, abs2, rmax2, any);
-    if (step + 1 < nsteps) inner_step(sm, sm.hr, sm.hi, sm.gr, sm.gi, step + 1, n2, o, tol2, abs2, rmax2, any);
+  if (rwave) {
+    int p, q;
+    circle_pair(k, 0, n2, p, q);
+    rgr = sm.gr[p][q];
+    rgi = sm.gi[p][q];
+    rt = form_rotation(sm.gr[p][p], sm.gr[q][q], rgr, rgi, tol2, abs2);
+    if (rown) {
+      sm.rcs[0][k][0] = rt.c;
+      sm.rcs[0][k][1] = rt.sr;
+      sm.rcs[0][k][2] = rt.si;
+      sm.rcs[2][k][0] = rt.an;
+      sm.rcs[2][k][1] = rt.dn;
+      sm.rpq[0][k] = rt.act;
+      rmax2 = fmax(rmax2, rt.rel2);
+      any |= rt.act;
+    }
   }
-  if (tid < npair) sm.rmax[tid] = rmax2;
-  if (any) sm.any_outer = 1;
+  __syncthreads();
+  for (int step = 0; step < nsteps; ++step) {
+    const int par = step & 1;
+    const GramRow* cr = par ? sm.hr : sm.gr;
+    const GramRow* ci = par ? sm.hi : sm.gi;
+    GramRow* nr = par ? sm.gr : sm.hr;
+    GramRow* ni = par ? sm.gi : sm.hi;
+    if (gwave) {
+      if (gown) {
+        int p0, q0, p1, q1;
+        circle_pair(bk, step, n2, p0, q0);
+        circle_pair(bl, step, n2, p1, q1);
+        double b[8];
+        g_get(cr, ci, p0, p1, b[0], b[1]);
+        g_get(cr, ci, p0, q1, b[2], b[3]);
+        g_get(cr, ci, q0, p1, b[4], b[5]);
+        g_get(cr, ci, q0, q1, b[6], b[7]);
+        const double ck = sm.rcs[par][bk][0], ekr = sm.rcs[par][bk][1], eki = sm.rcs[par][bk][2];
+        const double cl = sm.rcs[par][bl][0], elr = sm.rcs[par][bl][1], eli = sm.rcs[par][bl][2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          double n0r, n0i, n1r, n1i;
+          block_row(h != 0, ck, ekr, eki, cl, elr, eli, b, n0r, n0i, n1r, n1i);
+          const int ph = h ? q0 : p0;
+          g_put(nr, ni, ph, p1, n0r, n0i);
+          g_put(nr, ni, ph, q1, n1r, n1i);
+        }
+      }
+    } else if (wwave) {
+      // W <- W J on this thread's elements
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int ks = wslot[u];
+        if (!wown[u] || !sm.rpq[par][ks]) continue;
+        int wp, wq;
+        circle_pair(ks, step, n2, wp, wq);
+        const int row = wrow[u];
+        const double pr = sm.wr[row][wp], pi = sm.wi[row][wp], qr = sm.wr[row][wq], qi = sm.wi[row][wq];
+        const double cw = sm.rcs[par][ks][0], er = sm.rcs[par][ks][1], ei = sm.rcs[par][ks][2];
+        sm.wr[row][wp] = cw * pr - (er * qr + ei * qi);
+        sm.wi[row][wp] = cw * pi - (er * qi - ei * qr);
+        sm.wr[row][wq] = (er * pr - ei * pi) + cw * qr;
+        sm.wi[row][wq] = (er * pi + ei * pr) + cw * qi;
+      }
+    } else if (rwave) {
+      // this step's diagonal blocks, then the next step's rotations
+      if (rown) {
+        int p, q;
+        circle_pair(k, step, n2, p, q);
+        nr[p][p] = rt.an;
+        ni[p][p] = 0.0;
+        nr[q][q] = rt.dn;
+        ni[q][q] = 0.0;
+        nr[p][q] = rt.act ? 0.0 : rgr;
+        ni[p][q] = rt.act ? 0.0 : rgi;
+      }
+      if (step + 1 < nsteps) {
+        int pn, qn, k1, h1, k2, h2, pk1, qk1, pk2, qk2;
+        circle_pair(k, step + 1, n2, pn, qn);
+        circle_slot(pn, step, n2, k1, h1);
+        circle_slot(qn, step, n2, k2, h2);
+        circle_pair(k1, step, n2, pk1, qk1);
+        circle_pair(k2, step, n2, pk2, qk2);
+        double b[8];
+        g_get(cr, ci, pk1, pk2, b[0], b[1]);
+        g_get(cr, ci, pk1, qk2, b[2], b[3]);
+        g_get(cr, ci, qk1, pk2, b[4], b[5]);
+        g_get(cr, ci, qk1, qk2, b[6], b[7]);
+        const double c1 = sm.rcs[par][k1][0], s1r = sm.rcs[par][k1][1], s1i = sm.rcs[par][k1][2];
+        const double c2 = sm.rcs[par][k2][0], s2r = sm.rcs[par][k2][1], s2i = sm.rcs[par][k2][2];
+        const double an = sm.rcs[2 + par][k1][h1], dn = sm.rcs[2 + par][k2][h2];
+        double n0r, n0i, n1r, n1i;
+        block_row(h1 != 0, c1, s1r, s1i, c2, s2r, s2i, b, n0r, n0i, n1r, n1i);
+        rgr = h2 ? n1r : n0r;
+        rgi = h2 ? n1i : n0i;
+        rt = form_rotation(an, dn, rgr, rgi, tol2, abs2);
+        if (rown) {
+          sm.rcs[par ^ 1][k][0] = rt.c;
+          sm.rcs[par ^ 1][k][1] = rt.sr;
+          sm.rcs[par ^ 1][k][2] = rt.si;
+          sm.rcs[2 + (par ^ 1)][k][0] = rt.an;
+          sm.rcs[2 + (par ^ 1)][k][1] = rt.dn;
+          sm.rpq[par ^ 1][k] = rt.act;
+          rmax2 = fmax(rmax2, rt.rel2);
+          any |= rt.act;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (rown) sm.rmax[k] = rmax2;
+  if (rown && any) sm.any_outer = 1;
   __syncthreads();
 }
 
