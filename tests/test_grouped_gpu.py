@@ -136,6 +136,32 @@ def test_grouped_svd_vs_lapack(ctx):
         assert _rel((Uh * S[:k]) @ Vhh, best) < 1e-11
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+def test_grouped_svd_every_small_width(ctx, dtype):
+    """Every width 1..33 in one grouped batch: widths up to 32 are one column
+    pair of 2b = 2, 4, ..., 32 columns solved by the full inner sweep alone
+    (33 takes the block rounds).  Singular values against LAPACK and
+    U diag(S) Vh against the input, no truncation."""
+    import tenet_hip as T
+    from tenet_hip.grouped import grouped_svd
+
+    rng = np.random.default_rng(21)
+    shapes = [(n + 3, n) for n in range(1, 34)] + [(n, n + 5) for n in (2, 7, 16, 31)]
+    hosts = [O.complex_gaussian(rng, s).astype(dtype) for s in shapes]
+    mats = [T.DeviceArray.from_host(ctx, h) for h in hosts]
+    res = grouped_svd(ctx, mats, None, 0.0)
+    tol = 1e-12 if dtype == np.complex128 else 2e-5
+    for h, (U, S, Vh, k, disc) in zip(hosts, res):
+        h64 = h.astype(np.complex128)
+        s_ref = np.linalg.svd(h64, compute_uv=False)
+        assert k == min(h.shape)
+        assert np.allclose(S[:k], s_ref, rtol=tol, atol=tol * s_ref[0])
+        Uh, Vhh = U.to_host()[:, :k].astype(np.complex128), Vh.to_host()[:k, :].astype(np.complex128)
+        assert _rel((Uh * S[:k]) @ Vhh, h64) < 10 * tol
+        assert _rel(Uh.conj().T @ Uh, np.eye(k)) < 10 * tol
+
+
 @pytest.mark.parametrize("want", [(True, True, True), (True, True, False), (False, False, False)],
                          ids=["all_outputs", "no_disc", "no_s_k_disc"])
 def test_grouped_svd_graded_problems_converge(ctx, want):
