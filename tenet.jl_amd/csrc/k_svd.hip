@@ -783,15 +783,18 @@ __device__ __forceinline__ void store_rows_all(SvSmem& sm, const double2 (&v)[ST
 // Gram G = X^H X of the pair's columns over rows [r0, r1) into sm.gr / sm.gi.
 // With P = Xr^T Xi:  Re G = Xr^T Xr + Xi^T Xi (symmetric), Im G = P - P^T.
 // For a 4-row K step the A operand of X^T and the B operand of X are the same
-// register (X[k = l / 16][c = l % 16]).  The off-diagonal block takes three
-// real products (Gauss): T1 = Xr0^T Xr1, T2 = Xi0^T Xi1 and
+// register (X[k = l / 16][c = l % 16]).  Every block takes fewer real
+// products than the four of a complex one (errors O(eps |x_p| |x_q|), the
+// order of a plain dot product's): a diagonal block two, S^T S with
+// S = Xr + Xi and P, since Re G = S^T S - P - P^T; the off-diagonal block
+// three (Gauss): T1 = Xr0^T Xr1, T2 = Xi0^T Xi1 and
 // T3 = (Xr0 + Xi0)^T (Xr1 - Xi1) give Re G(0,1) = T1 + T2 and
-// Im G(0,1) = T1 - T2 - T3 (errors O(eps |x_p| |x_q|), as for the other
-// entries).  Each 64-row chunk is cut into four 16-row groups; group g
-// belongs to waves g and g + 4 (the two waves of one SIMD): wave g sums
-// Re G(0,0), Re G(1,1) and P(0,0) (five products per K step), wave g + 4
-// P(1,1), T1, T2 and T3 (four).  The groups' partials are added in LDS in a
-// fixed order (1 into 0 and 3 into 2, then 2 into 0: deterministic).
+// Im G(0,1) = T1 - T2 - T3.  Each 64-row chunk is cut into four 16-row
+// groups; group g belongs to waves g and g + 4 (the two waves of one SIMD):
+// wave g sums S0^T S0, S1^T S1 and P(0,0), wave g + 4 P(1,1), T1, T2 and T3
+// (seven products per K step on the SIMD).  The groups' partials are added
+// in LDS in a fixed order (1 into 0 and 3 into 2, then 2 into 0:
+// deterministic).
 __device__ void gram_mfma(SvSmem& sm, const double* X, int64_t ld, int r0, int r1, int I, int J, int b) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int grp = wave & 3;
@@ -811,10 +814,9 @@ __device__ void gram_mfma(SvSmem& sm, const double* X, int64_t ld, int r0, int r
       const double xr0 = sm.xr[row][kc], xi0 = sm.xi[row][kc];
       const double xr1 = sm.xr[row][16 + kc], xi1 = sm.xi[row][16 + kc];
       if (!hiw) {
-        a0 = mfma_f64(xr0, xr0, a0);
-        a0 = mfma_f64(xi0, xi0, a0);
-        a1 = mfma_f64(xr1, xr1, a1);
-        a1 = mfma_f64(xi1, xi1, a1);
+        const double s0 = xr0 + xi0, s1 = xr1 + xi1;
+        a0 = mfma_f64(s0, s0, a0);
+        a1 = mfma_f64(s1, s1, a1);
         a2 = mfma_f64(xr0, xi0, a2);
       } else {
         a0 = mfma_f64(xr1, xi1, a0);
@@ -868,7 +870,8 @@ __device__ void gram_mfma(SvSmem& sm, const double* X, int64_t ld, int r0, int r
       a2[r] += red[((s2 * 4 + 2) * 4 + r) * 64 + lane];
       a3[r] += red[((s2 * 4 + 3) * 4 + r) * 64 + lane];
     }
-    // Re G into gr (mirrored), P into wr (scratch until the inner solve sets W)
+    // S^T S of the diagonal blocks and Re G(0,1) (mirrored) into gr, P into
+    // wr (scratch until the inner solve sets W)
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int i = 4 * r + kr, j = kc;
@@ -890,7 +893,9 @@ __device__ void gram_mfma(SvSmem& sm, const double* X, int64_t ld, int r0, int r
   __syncthreads();
   for (int e = threadIdx.x; e < B2MAX * B2MAX; e += SR_THREADS) {
     const int i = e & (B2MAX - 1), j = e / B2MAX;
-    sm.gi[i][j] = sm.wr[i][j] - sm.wr[j][i];
+    const double pij = sm.wr[i][j], pji = sm.wr[j][i];
+    sm.gi[i][j] = pij - pji;
+    if ((i < 16) == (j < 16)) sm.gr[i][j] -= pij + pji;  // Re G = S^T S - P - P^T
   }
   __syncthreads();
 }
