@@ -900,58 +900,66 @@ __device__ void gram_mfma(SvSmem& sm, const double* X, int64_t ld, int r0, int r
   __syncthreads();
 }
 
-// Rows [r0, r1) of the pair's columns of M <- M W by MFMA: per 64-row chunk
-// wave w computes the 16 x 16 output block (rows 16 (w % 4), columns
-// 16 (w / 4)) over K = 32 in three real products (see below), the W
-// operands held in registers for the whole call, and stores it from its
-// accumulators.  The next chunk's loads are in flight during the products.
+// Rows [r0, r1) of the pair's columns of M <- M W by MFMA, with no LDS
+// staging: wave w takes the 16-row strips w, w + 8, ... and computes all 32
+// output columns of its strip (two 16-column blocks over K = 32, three real
+// products each, see below).  Its A operands come straight from memory (lane
+// l: row l % 16 of the strip, columns 4s + l / 16: 256 contiguous bytes per
+// column and load), W's from LDS, and the outputs go out from the
+// accumulators (lanes l, l + 16, l + 32, l + 48: four consecutive rows of one
+// column).  A wave reads and writes only its own rows, so the waves run
+// without a barrier.
 __device__ void apply_mfma(SvSmem& sm, double* M, int64_t ld, int r0, int r1, int I, int J, int b) {
   const int n2 = 2 * b;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int rblk = wave & 3, cblk = wave >> 2;
   const int kr = lane >> 4, kc = lane & 15;
-  double wre[B2MAX / 4], wim[B2MAX / 4], wsum[B2MAX / 4];
+  double2* M2 = reinterpret_cast<double2*>(M);
+  const size_t safe = (size_t)pair_col(0, I, J, b) * ld + r0;  // a valid element (masked loads read it)
+  for (int rs = r0 + 16 * wave; rs < r1; rs += 16 * (SR_THREADS / 64)) {
+    const int row = rs + kc;
+    double2 x[B2MAX / 4];
 #pragma unroll
-  for (int s = 0; s < B2MAX / 4; ++s) {
-    wre[s] = sm.wr[4 * s + kr][16 * cblk + kc];
-    wim[s] = sm.wi[4 * s + kr][16 * cblk + kc];
-    wsum[s] = wre[s] + wim[s];
-  }
-  double2 v[STAGE_PER];
-  load_rows(v, M, ld, r0, min(RS, r1 - r0), I, J, b);
-  for (int rb = r0; rb < r1; rb += RS) {
-    const int rows = min(RS, r1 - rb);
-    __syncthreads();
-    store_rows_all(sm, v);
-    __syncthreads();
-    if (rb + RS < r1) load_rows(v, M, ld, rb + RS, min(RS, r1 - rb - RS), I, J, b);
+    for (int s = 0; s < B2MAX / 4; ++s) {
+      const int l = 4 * s + kr;
+      const bool ok = row < r1 && l < n2;
+      const double2 v = M2[ok ? (size_t)pair_col(l, I, J, b) * ld + row : safe];
+      x[s].x = ok ? v.x : 0.0;
+      x[s].y = ok ? v.y : 0.0;
+    }
     // three real products per complex one (Gauss):
     //   T1 = Xr Wr, T2 = Xi Wi, T3 = (Xr + Xi)(Wr + Wi);
     //   Re Y = T1 - T2, Im Y = T3 - T1 - T2
     // (W is unitary, so the cancellation in Im Y costs O(eps |X|))
-    svd_d4v t1 = {0, 0, 0, 0}, t2 = {0, 0, 0, 0}, t3 = {0, 0, 0, 0};
+    svd_d4v t[2][3];
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+      for (int u = 0; u < 3; ++u) t[cb][u] = svd_d4v{0, 0, 0, 0};
 #pragma unroll
     for (int s = 0; s < B2MAX / 4; ++s) {
-      const double xr = sm.xr[16 * rblk + kc][4 * s + kr], xi = sm.xi[16 * rblk + kc][4 * s + kr];
-      t1 = mfma_f64(xr, wre[s], t1);
-      t2 = mfma_f64(xi, wim[s], t2);
-      t3 = mfma_f64(xr + xi, wsum[s], t3);
-    }
-    // straight from the accumulators: lanes l and l + 16, l + 32, l + 48
-    // write four consecutive rows of one column (64 contiguous bytes); the
-    // staged chunk in LDS is not touched, so no barrier before the next one
-    double2* M2 = reinterpret_cast<double2*>(M);
-    const int l = 16 * cblk + kc;
-    if (l < n2) {
-      double2* col = M2 + (size_t)pair_col(l, I, J, b) * ld + rb;
+      const double xr = x[s].x, xi = x[s].y, xs = xr + xi;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = 16 * rblk + 4 * r + kr;
-        if (row < rows) {
-          double2 o;
-          o.x = t1[r] - t2[r];
-          o.y = t3[r] - t1[r] - t2[r];
-          col[row] = o;
+      for (int cb = 0; cb < 2; ++cb) {
+        const double wre = sm.wr[4 * s + kr][16 * cb + kc], wim = sm.wi[4 * s + kr][16 * cb + kc];
+        t[cb][0] = mfma_f64(xr, wre, t[cb][0]);
+        t[cb][1] = mfma_f64(xi, wim, t[cb][1]);
+        t[cb][2] = mfma_f64(xs, wre + wim, t[cb][2]);
+      }
+    }
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb) {
+      const int l = 16 * cb + kc;
+      if (l < n2) {
+        double2* col = M2 + (size_t)pair_col(l, I, J, b) * ld + rs;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int rr = 4 * r + kr;
+          if (rs + rr < r1) {
+            double2 o;
+            o.x = t[cb][0][r] - t[cb][1][r];
+            o.y = t[cb][2][r] - t[cb][0][r] - t[cb][1][r];
+            col[rr] = o;
+          }
         }
       }
     }
