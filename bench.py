@@ -53,7 +53,7 @@ PEAK_FP64_TFLOPS = 78.6
 # HBM bytes per round-kernel dispatch (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 KiB,
 # factors from the scale kernel over 256 MiB) for each TEBD form, from
 # rocprofv3 --pmc passes over one sweep; None where not measured.
-ROUND_TRAFFIC_BYTES = {"fp64": 17_005_568, "vidal": 216_809_661, "vidal_graph": 216_809_661}
+ROUND_TRAFFIC_BYTES = {"fp64": 17_005_568, "vidal": 216_834_819, "vidal_graph": 216_834_819}
 
 FORMS = {"fp64": 0, "fp32": 1, "split_bf16": 2, "split_fp16": 3, "vidal": 0, "vidal_graph": 0}
 DEFAULT_FORMS = "vidal,vidal_graph,fp64,split_bf16,split_fp16,fp32"
